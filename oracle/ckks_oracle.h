@@ -1,0 +1,109 @@
+/*
+ * CPU oracle for the CKKS evaluation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithms (TrustworthyComputing/aes-walsh-ckks); every function cites the
+ * reference file:line it follows.  It exists to check the HIP library's
+ * results.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it, and only as the checker -- the product path never links,
+ * loads or falls back to it.
+ *
+ * Conventions (the reference's): an RNS polynomial is a flat uint64 array of
+ * limbs*N words, limb-major (limb*N + coeff); evaluation-domain index i holds
+ * A(psi^(2*brev(i)+1)); stored values are canonical residues in [0, q).
+ * Floating-point HPS estimates are evaluated without FMA contraction
+ * (the build uses -ffp-contract=off).
+ */
+#ifndef CKKS_ORACLE_H
+#define CKKS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- modular arithmetic (src/math/modarith) ---- */
+uint64_t oc_powmod(uint64_t a, uint64_t e, uint64_t m);
+uint64_t oc_mod_inverse(uint64_t a, uint64_t m);
+uint64_t oc_compute_shoup(uint64_t w, uint64_t q);
+void oc_barrett_const(uint64_t q, uint64_t out_lo_hi[2]);
+uint64_t oc_barrett_mul(uint64_t a, uint64_t b, uint64_t q);
+uint64_t oc_barrett_reduce_u64(uint64_t a, uint64_t q);
+uint64_t oc_shoup_mul(uint64_t a, uint64_t w, uint64_t q);
+uint64_t oc_montgomery_neg_inverse(uint64_t q);
+uint64_t oc_montgomery_radix_mod_q(uint64_t q);
+uint64_t oc_montgomery_mul(uint64_t normal, uint64_t mont, uint64_t q);
+
+/* ---- prime search (src/util/random.cpp) ---- */
+int oc_is_prime(uint64_t n);
+int oc_find_p_and_q_primes(const uint64_t* p_bits, int np, const uint64_t* q_bits,
+                           int nq, uint64_t N, uint64_t* p_out, uint64_t* q_out);
+
+/* ---- twiddles and NTT (src/math/ntt/ntt.cpp) ---- */
+uint64_t oc_find_root_of_unity(uint64_t N, uint64_t q);
+/* fwd, fwd_shoup, inv: N words; inv_shoup: N+1 words (last = Shoup(N^-1)). */
+int oc_ntt_tables(uint64_t N, uint64_t q, uint64_t* fwd, uint64_t* fwd_shoup,
+                  uint64_t* inv, uint64_t* inv_shoup, uint64_t* n_inv);
+void oc_ntt_forward(uint64_t* a, uint64_t N, uint64_t q, const uint64_t* fwd,
+                    const uint64_t* fwd_shoup, int lazy);
+void oc_ntt_inverse(uint64_t* a, uint64_t N, uint64_t q, const uint64_t* inv,
+                    const uint64_t* inv_shoup, uint64_t n_inv, int lazy);
+
+/* ---- context: per-modulus tables and key-switch precomputes ---- */
+typedef struct oc_ctx oc_ctx;
+/* q: nq chain moduli (limb 0 .. Lmax), p: np special moduli, dnum digits. */
+oc_ctx* oc_ctx_create(uint64_t N, const uint64_t* q, int nq, const uint64_t* p, int np, int dnum);
+void oc_ctx_destroy(oc_ctx* ctx);
+int oc_ctx_alpha(const oc_ctx* ctx);
+
+/* NTT over the first `limbs` chain moduli (flat RNS poly). */
+void oc_ntt_fwd_q(const oc_ctx* ctx, uint64_t* data, int limbs, int lazy);
+void oc_ntt_inv_q(const oc_ctx* ctx, uint64_t* data, int limbs, int lazy);
+/* NTT over an arbitrary list of basis indices: idx < nq is q[idx], idx >= nq is p[idx-nq]. */
+void oc_ntt_fwd_basis(const oc_ctx* ctx, uint64_t* data, const int* basis, int limbs, int lazy);
+void oc_ntt_inv_basis(const oc_ctx* ctx, uint64_t* data, const int* basis, int limbs, int lazy);
+
+/* HPS ModUp of one digit `part` at `level`: src = digit limbs (coefficient
+ * domain, alpha_d*N words), dst = (L - alpha_d + K)*N words in the
+ * reference's destination order (Q limbs below the digit, Q limbs above it, P). */
+int oc_modup(const oc_ctx* ctx, int level, int part, const uint64_t* src, uint64_t* dst);
+
+/* Hybrid key switch (ckks_keyswitch.cpp:1330-1641), fold_hybrid_keyswitch=true.
+ * key_term: (level+1)*N eval-domain words (overwritten, like the reference).
+ * keys: dnum_active digits laid out [digit][poly b=0,a=1][Lmax+1+K limbs][N].
+ * mode 0 = Add, 1 = Assign. montgomery selects Montgomery-form keys. */
+int oc_keyswitch(const oc_ctx* ctx, int level, uint64_t* key_term, const uint64_t* keys,
+                 uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery);
+
+/* Galois automorphisms (rotation_key_generation.cpp:73-90, conjugation :59-61). */
+uint64_t oc_rotation_galois_element(uint64_t N, int64_t rotation);
+void oc_ntt_automorphism_map(uint64_t N, uint64_t galois, uint64_t* dst_to_src);
+void oc_apply_automorphism(uint64_t* data, uint64_t N, int limbs, const uint64_t* dst_to_src);
+
+/* Evaluator ops on ciphertexts stored as polys [b, a, (c2)] each (level+1)*N. */
+int oc_rotate(const oc_ctx* ctx, int level, uint64_t* b, uint64_t* a, int64_t rotation,
+              const uint64_t* keys, int montgomery);
+int oc_conjugate(const oc_ctx* ctx, int level, uint64_t* b, uint64_t* a,
+                 const uint64_t* keys, int montgomery);
+int oc_relin(const oc_ctx* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2,
+             const uint64_t* keys, int montgomery);
+/* polys: npolys consecutive polys of stride (level+1)*N; after the call the
+ * first level*N words of each poly hold the rescaled result. */
+int oc_rescale(const oc_ctx* ctx, int level, uint64_t* polys, int npolys);
+int oc_tensor(const oc_ctx* ctx, int level, uint64_t* b1, uint64_t* a1, const uint64_t* b2,
+              const uint64_t* a2, uint64_t* c2);
+int oc_square(const oc_ctx* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2, int montgomery);
+int oc_mul_plain(const oc_ctx* ctx, int level, uint64_t* polys, int npolys, const uint64_t* pt,
+                 int pt_montgomery);
+int oc_mul_int(const oc_ctx* ctx, int level, uint64_t* polys, int npolys, int64_t scalar,
+               int montgomery);
+int oc_add(const oc_ctx* ctx, int level, uint64_t* x, const uint64_t* y, int npolys);
+int oc_sub(const oc_ctx* ctx, int level, uint64_t* x, const uint64_t* y, int npolys);
+int oc_add_plain(const oc_ctx* ctx, int level, uint64_t* b, const uint64_t* pt, int pt_montgomery);
+int oc_add_scalar_residues(const oc_ctx* ctx, int level, uint64_t* b, const uint64_t* residues);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,238 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The HIP product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u64 = ctypes.c_uint64
+i64 = ctypes.c_int64
+cint = ctypes.c_int
+vp = ctypes.c_void_p
+
+
+def build():
+    """Compile liboracle.so (gcc) in place."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "oc_powmod": (u64, [u64, u64, u64]),
+            "oc_mod_inverse": (u64, [u64, u64]),
+            "oc_compute_shoup": (u64, [u64, u64]),
+            "oc_barrett_mul": (u64, [u64, u64, u64]),
+            "oc_barrett_reduce_u64": (u64, [u64, u64]),
+            "oc_shoup_mul": (u64, [u64, u64, u64]),
+            "oc_montgomery_neg_inverse": (u64, [u64]),
+            "oc_montgomery_radix_mod_q": (u64, [u64]),
+            "oc_montgomery_mul": (u64, [u64, u64, u64]),
+            "oc_is_prime": (cint, [u64]),
+            "oc_find_p_and_q_primes": (cint, [vp, cint, vp, cint, u64, vp, vp]),
+            "oc_find_root_of_unity": (u64, [u64, u64]),
+            "oc_ntt_tables": (cint, [u64, u64, vp, vp, vp, vp, vp]),
+            "oc_ntt_forward": (None, [vp, u64, u64, vp, vp, cint]),
+            "oc_ntt_inverse": (None, [vp, u64, u64, vp, vp, u64, cint]),
+            "oc_ctx_create": (vp, [u64, vp, cint, vp, cint, cint]),
+            "oc_ctx_destroy": (None, [vp]),
+            "oc_ctx_alpha": (cint, [vp]),
+            "oc_ntt_fwd_q": (None, [vp, vp, cint, cint]),
+            "oc_ntt_inv_q": (None, [vp, vp, cint, cint]),
+            "oc_ntt_fwd_basis": (None, [vp, vp, vp, cint, cint]),
+            "oc_ntt_inv_basis": (None, [vp, vp, vp, cint, cint]),
+            "oc_modup": (cint, [vp, cint, cint, vp, vp]),
+            "oc_keyswitch": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, cint]),
+            "oc_rotation_galois_element": (u64, [u64, i64]),
+            "oc_ntt_automorphism_map": (None, [u64, u64, vp]),
+            "oc_apply_automorphism": (None, [vp, u64, cint, vp]),
+            "oc_rotate": (cint, [vp, cint, vp, vp, i64, vp, cint]),
+            "oc_conjugate": (cint, [vp, cint, vp, vp, vp, cint]),
+            "oc_relin": (cint, [vp, cint, vp, vp, vp, vp, cint]),
+            "oc_rescale": (cint, [vp, cint, vp, cint]),
+            "oc_tensor": (cint, [vp, cint, vp, vp, vp, vp, vp]),
+            "oc_square": (cint, [vp, cint, vp, vp, vp, cint]),
+            "oc_mul_plain": (cint, [vp, cint, vp, cint, vp, cint]),
+            "oc_mul_int": (cint, [vp, cint, vp, cint, i64, cint]),
+            "oc_add": (cint, [vp, cint, vp, vp, cint]),
+            "oc_sub": (cint, [vp, cint, vp, vp, cint]),
+            "oc_add_plain": (cint, [vp, cint, vp, vp, cint]),
+            "oc_add_scalar_residues": (cint, [vp, cint, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data
+
+
+def u64a(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+
+
+def find_primes(p_bits, q_bits, N):
+    pb, qb = u64a(p_bits), u64a(q_bits)
+    po = np.zeros(max(len(pb), 1), np.uint64)
+    qo = np.zeros(max(len(qb), 1), np.uint64)
+    rc = lib().oc_find_p_and_q_primes(_p(pb) if len(pb) else None, len(pb),
+                                      _p(qb) if len(qb) else None, len(qb), N, _p(po), _p(qo))
+    if rc:
+        raise RuntimeError("prime search failed (%d)" % rc)
+    return po[: len(pb)], qo[: len(qb)]
+
+
+def ntt_tables(N, q):
+    fwd = np.zeros(N, np.uint64); fs = np.zeros(N, np.uint64)
+    inv = np.zeros(N, np.uint64); is_ = np.zeros(N + 1, np.uint64)
+    ni = np.zeros(1, np.uint64)
+    if lib().oc_ntt_tables(N, int(q), _p(fwd), _p(fs), _p(inv), _p(is_), _p(ni)):
+        raise RuntimeError("no 2N-th root of unity")
+    return fwd, fs, inv, is_, int(ni[0])
+
+
+def ntt_forward(x, q, tables, lazy=False):
+    a = u64a(x).copy()
+    fwd, fs = tables[0], tables[1]
+    lib().oc_ntt_forward(_p(a), len(a), int(q), _p(fwd), _p(fs), int(lazy))
+    return a
+
+
+def ntt_inverse(x, q, tables, lazy=False):
+    a = u64a(x).copy()
+    inv, is_, ni = tables[2], tables[3], tables[4]
+    lib().oc_ntt_inverse(_p(a), len(a), int(q), _p(inv), _p(is_), ni, int(lazy))
+    return a
+
+
+class Ctx:
+    """Oracle context: chain moduli q (limb 0..Lmax), special moduli p, dnum."""
+
+    def __init__(self, N, q, p, dnum):
+        self.N, self.q, self.p, self.dnum = int(N), u64a(q), u64a(p), int(dnum)
+        self.h = lib().oc_ctx_create(self.N, _p(self.q), len(self.q), _p(self.p), len(self.p), self.dnum)
+        if not self.h:
+            raise RuntimeError("oc_ctx_create failed")
+        self.alpha = lib().oc_ctx_alpha(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oc_ctx_destroy(self.h)
+            self.h = None
+
+    def parts(self, level):
+        L = level + 1
+        return min(self.dnum, (L + self.alpha - 1) // self.alpha)
+
+    def ntt_fwd(self, data, limbs, lazy=False):
+        a = u64a(data).copy(); lib().oc_ntt_fwd_q(self.h, _p(a), limbs, int(lazy)); return a
+
+    def ntt_inv(self, data, limbs, lazy=False):
+        a = u64a(data).copy(); lib().oc_ntt_inv_q(self.h, _p(a), limbs, int(lazy)); return a
+
+    def modup(self, level, part, src):
+        L = level + 1
+        s0 = part * self.alpha
+        s1 = min(s0 + self.alpha, L)
+        nd = L - (s1 - s0) + len(self.p)
+        out = np.zeros(nd * self.N, np.uint64)
+        src = u64a(src)
+        if lib().oc_modup(self.h, level, part, _p(src), _p(out)):
+            raise RuntimeError("oc_modup")
+        return out
+
+    def keyswitch(self, level, key_term, keys, b, a, b_mode, a_mode, montgomery=True):
+        kt, b, a, keys = u64a(key_term).copy(), u64a(b).copy(), u64a(a).copy(), u64a(keys)
+        if lib().oc_keyswitch(self.h, level, _p(kt), _p(keys), _p(b), _p(a), b_mode, a_mode, int(montgomery)):
+            raise RuntimeError("oc_keyswitch")
+        return b, a
+
+    def rotate(self, level, b, a, r, keys, montgomery=True):
+        b, a = u64a(b).copy(), u64a(a).copy()
+        if lib().oc_rotate(self.h, level, _p(b), _p(a), int(r), _p(u64a(keys)), int(montgomery)):
+            raise RuntimeError("oc_rotate")
+        return b, a
+
+    def conjugate(self, level, b, a, keys, montgomery=True):
+        b, a = u64a(b).copy(), u64a(a).copy()
+        if lib().oc_conjugate(self.h, level, _p(b), _p(a), _p(u64a(keys)), int(montgomery)):
+            raise RuntimeError("oc_conjugate")
+        return b, a
+
+    def relin(self, level, b, a, c2, keys, montgomery=True):
+        b, a, c2 = u64a(b).copy(), u64a(a).copy(), u64a(c2).copy()
+        if lib().oc_relin(self.h, level, _p(b), _p(a), _p(c2), _p(u64a(keys)), int(montgomery)):
+            raise RuntimeError("oc_relin")
+        return b, a
+
+    def rescale(self, level, polys):
+        """polys: (npolys, (level+1)*N) -> (npolys, level*N)."""
+        x = u64a(polys).copy()
+        n = x.shape[0]
+        if lib().oc_rescale(self.h, level, _p(x), n):
+            raise RuntimeError("oc_rescale")
+        return x[:, : level * self.N].copy()
+
+    def tensor(self, level, b1, a1, b2, a2):
+        b1, a1 = u64a(b1).copy(), u64a(a1).copy()
+        c2 = np.zeros_like(b1)
+        lib().oc_tensor(self.h, level, _p(b1), _p(a1), _p(u64a(b2)), _p(u64a(a2)), _p(c2))
+        return b1, a1, c2
+
+    def square(self, level, b, a, montgomery=True):
+        b, a = u64a(b).copy(), u64a(a).copy()
+        c2 = np.zeros_like(b)
+        lib().oc_square(self.h, level, _p(b), _p(a), _p(c2), int(montgomery))
+        return b, a, c2
+
+    def mul_plain(self, level, polys, pt, pt_mont=False):
+        x = u64a(polys).copy()
+        lib().oc_mul_plain(self.h, level, _p(x), x.shape[0], _p(u64a(pt)), int(pt_mont))
+        return x
+
+    def mul_int(self, level, polys, s, montgomery=True):
+        x = u64a(polys).copy()
+        lib().oc_mul_int(self.h, level, _p(x), x.shape[0], int(s), int(montgomery))
+        return x
+
+    def add(self, level, x, y):
+        x = u64a(x).copy()
+        lib().oc_add(self.h, level, _p(x), _p(u64a(y)), x.shape[0])
+        return x
+
+    def sub(self, level, x, y):
+        x = u64a(x).copy()
+        lib().oc_sub(self.h, level, _p(x), _p(u64a(y)), x.shape[0])
+        return x
+
+    def add_scalar_residues(self, level, b, res):
+        b = u64a(b).copy()
+        lib().oc_add_scalar_residues(self.h, level, _p(b), _p(u64a(res)))
+        return b
+
+
+def galois_rotation(N, r):
+    return lib().oc_rotation_galois_element(N, int(r))
+
+
+def automorphism_map(N, g):
+    m = np.zeros(N, np.uint64)
+    lib().oc_ntt_automorphism_map(N, int(g), _p(m))
+    return m

@@ -1,0 +1,254 @@
+// Golden-vector harness over the reference's own C++ sources.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile (target `ref`) from the
+// reference sources where they lie under /root/reference, with output only in
+// oracle/_ref/.  It drives the reference's public API (CKKSParams,
+// CKKSContext, ckks_* evaluator functions) and writes raw little-endian
+// arrays that tests/golden/make_golden.py packs into committed fixtures.
+// Inputs the harness chooses are drawn from splitmix64 so the tests can
+// regenerate them; the reference's own randomness (keys, encryption masks) is
+// captured as data.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <sys/stat.h>
+
+#include "ckks_addition.hpp"
+#include "ckks_conjugate.hpp"
+#include "ckks_context.hpp"
+#include "ckks_encoding.hpp"
+#include "ckks_keyswitch.hpp"
+#include "ckks_mul.hpp"
+#include "ckks_params.hpp"
+#include "ckks_aes_params.hpp"
+#include "ckks_relin.hpp"
+#include "ckks_rescale.hpp"
+#include "ckks_rotation.hpp"
+#include "ntt.hpp"
+#include "random.hpp"
+
+static std::string g_out;
+
+static uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static void dump(const std::string& name, const void* data, size_t bytes) {
+    std::string path = g_out + "/" + name + ".bin";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); std::exit(2); }
+    std::fwrite(data, 1, bytes, f);
+    std::fclose(f);
+}
+template <class V> static void dumpv(const std::string& name, const V& v) {
+    dump(name, v.data(), v.size() * sizeof(v[0]));
+}
+static void dump_ct(const std::string& name, const CKKSCiphertext& ct) {
+    std::vector<uint64_t> flat;
+    for (size_t k = 0; k < ct.getNumPolys(); ++k) {
+        const auto& p = ct.getPoly(k);
+        flat.insert(flat.end(), p.begin(), p.end());
+    }
+    dumpv(name, flat);
+    uint64_t meta[3] = {ct.getNumPolys(), ct.getLevel(), 0};
+    long double sc = ct.getScale();
+    double scd = (double)sc;
+    std::memcpy(&meta[2], &scd, 8);
+    dump(name + "_meta", meta, sizeof(meta));
+}
+static void dump_key(const std::string& name, std::span<const CKKSCiphertext> key) {
+    std::vector<uint64_t> flat;
+    for (const auto& d : key) {
+        flat.insert(flat.end(), d.getB().begin(), d.getB().end());
+        flat.insert(flat.end(), d.getA().begin(), d.getA().end());
+    }
+    dumpv(name, flat);
+}
+
+// Prime lists of the parameter presets and of the benchmark configurations.
+static void run_primes() {
+    auto dump_params = [](const std::string& name, const CKKSParams& p) {
+        dumpv(name + "_q", p.getModuli());
+        dumpv(name + "_p", p.getPModuli());
+    };
+    dump_params("walsh15", CKKSParams::AESWalsh15());
+    dump_params("xboot13", CKKSParams::XBootParamAes13Reference());
+    dump_params("xboot14", CKKSParams::AESXBoot14Secure());
+    dump_params("xboot15", CKKSParams::AESXBoot15());
+    {
+        CKKSParamsConfig c;
+        c.N = 1 << 14; c.mult_depth = 7; c.q0_bit = 50; c.q_bit = 50;
+        c.p_bits = {50, 50}; c.dnum = 4; c.secret_key_hamming_weight = 192;
+        c.sigma = 3.2; c.secure = false; c.rotate_values = {1};
+        dump_params("chain14", CKKSParams(c));
+    }
+    {
+        std::vector<uint64_t> qb(40, 60);
+        auto [p, q] = find_p_and_q_primes({}, qb, 1 << 16);
+        dumpv("ntt16_q", q);
+        std::vector<uint64_t> qb3(30, 50), pb3(10, 50);
+        auto [p3, q3] = find_p_and_q_primes(pb3, qb3, 1 << 16);
+        dumpv("ks16_q", q3);
+        dumpv("ks16_p", p3);
+    }
+}
+
+// Twiddle tables and NTT outputs for splitmix64 inputs.
+static void run_ntt(size_t N, const std::vector<uint64_t>& bits, const std::string& tag, bool dump_tables) {
+    auto [p, q] = find_p_and_q_primes({}, bits, N);
+    dumpv(tag + "_q", q);
+    for (size_t i = 0; i < q.size(); ++i) {
+        auto tw = generate_negacyclic_ntt_twiddles(N, q[i]);
+        if (dump_tables) {
+            dumpv(tag + "_fwd" + std::to_string(i), tw.forward_ntt);
+            dumpv(tag + "_fwds" + std::to_string(i), tw.forward_shoup);
+            dumpv(tag + "_inv" + std::to_string(i), tw.inverse_ntt);
+            dumpv(tag + "_invs" + std::to_string(i), tw.inverse_shoup);
+        }
+        uint64_t seed = 1000 + i;
+        std::vector<uint64_t> x(N);
+        for (auto& v : x) v = splitmix64(seed) % q[i];
+        std::vector<uint64_t> y = x;
+        ntt_forward_dit2_dispatch(y.data(), N, q[i], tw, false);
+        dumpv(tag + "_fwdout" + std::to_string(i), y);
+        std::vector<uint64_t> yl = x;
+        ntt_forward_dit2_dispatch(yl.data(), N, q[i], tw, true);
+        dumpv(tag + "_fwdlazy" + std::to_string(i), yl);
+        std::vector<uint64_t> z = x;
+        ntt_inverse_dif2_dispatch(z.data(), N, q[i], tw, false);
+        dumpv(tag + "_invout" + std::to_string(i), z);
+    }
+}
+
+// Evaluator operations on a small context with captured keys.
+static void run_eval(size_t logN, size_t depth, size_t dnum, std::vector<uint64_t> p_bits,
+                     const std::string& tag) {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << logN;
+    c.mult_depth = depth;
+    c.q0_bit = 50;
+    c.q_bit = 40;
+    c.p_bits = p_bits;
+    c.dnum = dnum;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    c.rotate_values = {1, 5};
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN(), L = P.getMaxLevel();
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    std::vector<uint64_t> sk;
+    for (const auto& limb : ctx.getSecretKey()) sk.insert(sk.end(), limb.begin(), limb.end());
+    dumpv(tag + "_sk", sk);
+    dump_key(tag + "_relin", ctx.getRelinHybrid(L));
+    dump_key(tag + "_rot1", ctx.getRotationKey(1));
+    dump_key(tag + "_rot5", ctx.getRotationKey(5));
+    dump_key(tag + "_conj", ctx.getConjugationKey());
+
+    uint64_t seed = 77;
+    auto rnd_slots = [&](double amp) {
+        MyVector<Complex> v(N / 2);
+        for (auto& z : v) {
+            double re = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1;
+            double im = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1;
+            z = Complex(re * amp, im * amp);
+        }
+        return v;
+    };
+    auto m1 = rnd_slots(1.0), m2 = rnd_slots(1.0);
+    dumpv(tag + "_m1", m1);
+    dumpv(tag + "_m2", m2);
+    auto ct1 = ckks_encrypt(ctx, m1, L);
+    auto ct2 = ckks_encrypt(ctx, m2, L);
+    dump_ct(tag + "_ct1", ct1);
+    dump_ct(tag + "_ct2", ct2);
+
+    {   // encode (eval domain), no randomness
+        auto pt = ckks_encode(ctx, m1, L, true);
+        dumpv(tag + "_encode_m1", pt.getPlaintext());
+    }
+    {   // decrypt of ct1 (FP output)
+        auto dec = ckks_decrypt(ctx, ct1);
+        dumpv(tag + "_dec_ct1", dec);
+    }
+    {   // tensor
+        auto x = ct1.clone();
+        ckks_mul_inplace(ctx, x, ct2);
+        dump_ct(tag + "_mul", x);
+        ckks_relin_hybrid_inplace(ctx, x);
+        dump_ct(tag + "_mul_relin", x);
+        ckks_rescale(ctx, x);
+        dump_ct(tag + "_mul_relin_rescale", x);
+        ckks_rotate_inplace(ctx, x, 1);
+        dump_ct(tag + "_chain_rot1", x);
+    }
+    {
+        auto x = ct1.clone();
+        ckks_square_inplace(ctx, x);
+        dump_ct(tag + "_square", x);
+    }
+    {
+        auto x = ct1.clone();
+        ckks_rotate_inplace(ctx, x, 1);
+        dump_ct(tag + "_rot1_out", x);
+        auto y = ct1.clone();
+        ckks_rotate_inplace(ctx, y, 5);
+        dump_ct(tag + "_rot5_out", y);
+        auto z = ct1.clone();
+        ckks_conjugate_inplace(ctx, z);
+        dump_ct(tag + "_conj_out", z);
+    }
+    {
+        auto x = ct1.clone();
+        ckks_rescale(ctx, x);
+        dump_ct(tag + "_rescale_ct1", x);
+        ckks_rescale(ctx, x);
+        dump_ct(tag + "_rescale2_ct1", x);
+    }
+    {
+        auto x = ct1.clone();
+        ckks_add_inplace(ctx, x, ct2);
+        dump_ct(tag + "_add", x);
+        auto y = ct1.clone();
+        ckks_sub_inplace(ctx, y, ct2);
+        dump_ct(tag + "_sub", y);
+        auto z = ct1.clone();
+        ckks_mul_inplace(ctx, z, int64_t{-7});
+        dump_ct(tag + "_mulint", z);
+        auto w = ct1.clone();
+        ckks_add_integer_constant_inplace(ctx, w, 3);
+        dump_ct(tag + "_addconst", w);
+        auto pt = ckks_encode(ctx, m2, L, true);
+        dumpv(tag + "_pt_m2", pt.getPlaintext());
+        auto v = ct1.clone();
+        ckks_mul_inplace(ctx, v, pt);
+        dump_ct(tag + "_mulplain", v);
+    }
+    {   // key switch of a lower-level ciphertext (fewer digits)
+        auto x = ct1.clone();
+        ckks_drop_to_level_inplace(x, 1);
+        ckks_rotate_inplace(ctx, x, 1);
+        dump_ct(tag + "_rot1_level1", x);
+    }
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR\n"); return 2; }
+    g_out = argv[1];
+    mkdir(g_out.c_str(), 0755);
+    run_primes();
+    run_ntt(1 << 10, {60, 59, 50, 36}, "ntt10", true);
+    run_ntt(1 << 12, {61, 55, 45, 30}, "ntt12", false);
+    run_ntt(1 << 16, {60}, "ntt16one", false);
+    run_eval(10, 3, 2, {60, 60}, "ev10");
+    run_eval(10, 4, 3, {61, 61}, "ev10b");
+    std::printf("ref_harness: wrote %s\n", g_out.c_str());
+    return 0;
+}

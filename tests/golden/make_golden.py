@@ -1,0 +1,84 @@
+"""Pack the reference harness output into the committed golden fixtures.
+
+Run in a container where /root/reference exists:
+    make -C oracle ref && oracle/_ref/ref_harness oracle/_ref/raw
+    python tests/golden/make_golden.py
+
+The harness (oracle/ref_harness.cpp) runs the reference's own C++ code; this
+script only repacks its raw arrays.  Inputs are stored in full; large outputs
+are stored as SHA-256 digests of their little-endian uint64 bytes (tests hash
+their own results and compare), small ones in full.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+RAW = os.path.join(HERE, "..", "..", "oracle", "_ref", "raw")
+
+
+def rd(name, dtype=np.uint64):
+    return np.fromfile(os.path.join(RAW, name + ".bin"), dtype=dtype)
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    primes = {}
+    for tag in ["walsh15", "xboot13", "xboot14", "xboot15", "chain14", "ks16"]:
+        primes[tag] = {"q": [int(v) for v in rd(tag + "_q")], "p": [int(v) for v in rd(tag + "_p")]}
+    primes["ntt16"] = {"q": [int(v) for v in rd("ntt16_q")], "p": []}
+    with open(os.path.join(HERE, "primes.json"), "w") as f:
+        json.dump(primes, f, indent=1)
+
+    ntt = {}
+    for tag, N in [("ntt10", 1 << 10), ("ntt12", 1 << 12), ("ntt16one", 1 << 16)]:
+        q = rd(tag + "_q")
+        ent = {"N": N, "q": [int(v) for v in q], "seed_base": 1000, "limbs": []}
+        for i in range(len(q)):
+            e = {
+                "fwd_out": digest(rd(f"{tag}_fwdout{i}")),
+                "fwd_lazy": digest(rd(f"{tag}_fwdlazy{i}")),
+                "inv_out": digest(rd(f"{tag}_invout{i}")),
+            }
+            if tag == "ntt10":
+                for t in ["fwd", "fwds", "inv", "invs"]:
+                    e["tab_" + t] = digest(rd(f"{tag}_{t}{i}"))
+            ent["limbs"].append(e)
+        ntt[tag] = ent
+    with open(os.path.join(HERE, "ntt.json"), "w") as f:
+        json.dump(ntt, f, indent=1)
+    # one full set of tables/outputs for diagnostics
+    np.savez_compressed(
+        os.path.join(HERE, "ntt10_limb0.npz"),
+        fwd=rd("ntt10_fwd0"), fwds=rd("ntt10_fwds0"), inv=rd("ntt10_inv0"), invs=rd("ntt10_invs0"),
+        fwd_out=rd("ntt10_fwdout0"), inv_out=rd("ntt10_invout0"))
+
+    outs = ["mul", "mul_relin", "mul_relin_rescale", "chain_rot1", "square", "rot1_out", "rot5_out",
+            "conj_out", "rescale_ct1", "rescale2_ct1", "add", "sub", "mulint", "addconst", "mulplain",
+            "rot1_level1"]
+    for tag, keys in [("ev10", ["relin", "rot1", "rot5", "conj"]), ("ev10b", ["relin", "rot1"])]:
+        arr = {"q": rd(tag + "_q"), "p": rd(tag + "_p"), "sk": rd(tag + "_sk"),
+               "ct1": rd(tag + "_ct1"), "ct2": rd(tag + "_ct2"),
+               "ct1_meta": rd(tag + "_ct1_meta"), "m1": rd(tag + "_m1", np.complex128),
+               "m2": rd(tag + "_m2", np.complex128), "encode_m1": rd(tag + "_encode_m1"), "pt_m2": rd(tag + "_pt_m2"),
+               "dec_ct1": rd(tag + "_dec_ct1", np.complex128)}
+        for k in keys:
+            arr["key_" + k] = rd(f"{tag}_{k}")
+        meta = {}
+        for o in outs:
+            m = rd(f"{tag}_{o}_meta")
+            meta[o] = {"digest": digest(rd(f"{tag}_{o}")), "npolys": int(m[0]), "level": int(m[1]),
+                       "scale": float(m[2:3].view(np.float64)[0])}
+        np.savez_compressed(os.path.join(HERE, tag + ".npz"), **arr)
+        with open(os.path.join(HERE, tag + "_outputs.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

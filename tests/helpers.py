@@ -1,0 +1,60 @@
+"""Shared test helpers: deterministic inputs, digests, fixtures, big-int forms."""
+import hashlib
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+M64 = (1 << 64) - 1
+
+
+def splitmix64_stream(seed, n):
+    """splitmix64 outputs for state `seed` (matches oracle/ref_harness.cpp)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.uint64).tobytes()).hexdigest()
+
+
+def load_json(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def load_npz(name):
+    return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+
+
+def rand_residues(rng, moduli, N):
+    """Uniform residues, limb-major, one limb per modulus."""
+    return np.concatenate([rng.integers(0, int(q), N, dtype=np.uint64) for q in moduli])
+
+
+def brev(x, bits):
+    r = 0
+    for i in range(bits):
+        if x >> i & 1:
+            r |= 1 << (bits - 1 - i)
+    return r
+
+
+def crt(residues, moduli):
+    """Big-int CRT of one coefficient: value in [0, prod)."""
+    Q = 1
+    for m in moduli:
+        Q *= int(m)
+    x = 0
+    for r, m in zip(residues, moduli):
+        m = int(m)
+        qh = Q // m
+        x += int(r) * qh * pow(qh, -1, m)
+    return x % Q, Q
