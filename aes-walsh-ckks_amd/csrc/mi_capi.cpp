@@ -1,0 +1,313 @@
+// extern "C" entry points of libmickks.so (declared in include/mickks.h).
+#include <cstring>
+#include <string>
+
+#include "../../include/mickks.h"
+#include "mi_internal.hpp"
+
+struct mi_context {
+    mi::Context* c;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const char* fn, int code, const std::string& msg) {
+    g_err = std::string(fn) + ": " + msg;
+    return code;
+}
+
+template <class F>
+int guard(const char* fn, F&& f) {
+    try {
+        f();
+        return MI_OK;
+    } catch (const mi::Error& e) {
+        return fail(fn, e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail(fn, MI_ERUNTIME, e.what());
+    }
+}
+
+inline hipStream_t S(void* s) { return (hipStream_t)s; }
+
+void check_ctx(const mi_context* ctx) {
+    if (!ctx || !ctx->c) throw mi::Error(MI_EINVAL, "null context");
+}
+void check_level(const mi::Context& c, int level) {
+    if (level < 0 || level >= c.nq) throw mi::Error(MI_EINVAL, "invalid level");
+}
+void check_ptr(const void* p) {
+    if (!p) throw mi::Error(MI_EINVAL, "null buffer");
+}
+}  // namespace
+
+extern "C" {
+
+const char* mi_last_error(void) { return g_err.c_str(); }
+
+int mi_context_create(mi_context** out, uint64_t N, const uint64_t* q, int nq, const uint64_t* p,
+                      int np, int dnum) {
+    return guard("mi_context_create", [&] {
+        if (!out || !q || !p) throw mi::Error(MI_EINVAL, "null argument");
+        *out = nullptr;
+        mi::Context* c = mi::context_create(N, q, nq, p, np, dnum);
+        *out = new mi_context{c};
+    });
+}
+
+int mi_context_destroy(mi_context* ctx) {
+    return guard("mi_context_destroy", [&] {
+        if (!ctx) return;
+        mi::context_destroy(ctx->c);
+        delete ctx;
+    });
+}
+
+int mi_context_info(const mi_context* ctx, int64_t* info) {
+    return guard("mi_context_info", [&] {
+        check_ctx(ctx);
+        const mi::Context& c = *ctx->c;
+        info[0] = (int64_t)c.N;
+        info[1] = c.nq;
+        info[2] = c.np;
+        info[3] = c.dnum;
+        info[4] = c.alpha;
+        info[5] = c.device;
+    });
+}
+
+int64_t mi_context_device_bytes(const mi_context* ctx) {
+    if (!ctx || !ctx->c) return -1;
+    int64_t b = (int64_t)ctx->c->table_bytes;
+    for (auto& kv : ctx->c->ws) b += (int64_t)kv.second.bytes;
+    return b;
+}
+
+int mi_ntt_forward(mi_context* ctx, uint64_t* data, int npolys, int limbs, int64_t poly_stride,
+                   int basis0, void* stream) {
+    return guard("mi_ntt_forward", [&] {
+        check_ctx(ctx);
+        check_ptr(data);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 0 || limbs < 1 || basis0 < 0 || basis0 + limbs > c.nb() ||
+            poly_stride < (int64_t)limbs * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid batch shape");
+        mi::ntt_forward(c, mi::LimbBatch{data, poly_stride, limbs, basis0, npolys * limbs}, S(stream));
+    });
+}
+
+int mi_ntt_inverse(mi_context* ctx, uint64_t* data, int npolys, int limbs, int64_t poly_stride,
+                   int basis0, void* stream) {
+    return guard("mi_ntt_inverse", [&] {
+        check_ctx(ctx);
+        check_ptr(data);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 0 || limbs < 1 || basis0 < 0 || basis0 + limbs > c.nb() ||
+            poly_stride < (int64_t)limbs * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid batch shape");
+        mi::ntt_inverse(c, mi::LimbBatch{data, poly_stride, limbs, basis0, npolys * limbs}, S(stream));
+    });
+}
+
+int mi_tensor(mi_context* ctx, int level, uint64_t* b1, uint64_t* a1, const uint64_t* b2,
+              const uint64_t* a2, uint64_t* c2, void* stream) {
+    return guard("ckks_mul_inplace(ct,ct)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b1); check_ptr(a1); check_ptr(b2); check_ptr(a2); check_ptr(c2);
+        mi::tensor(*ctx->c, level + 1, b1, a1, b2, a2, c2, S(stream));
+    });
+}
+
+int mi_square(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2, void* stream) {
+    return guard("ckks_square_inplace", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(a); check_ptr(c2);
+        mi::square(*ctx->c, level + 1, b, a, c2, S(stream));
+    });
+}
+
+int mi_mul_plain(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+                 const uint64_t* pt, int pt_montgomery, void* stream) {
+    return guard("ckks_mul_inplace(ct,pt)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(polys); check_ptr(pt);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
+        mi::ew_binary(c, pt_montgomery ? mi::EW_MUL_MONT : mi::EW_MUL, polys, pt, level + 1, npolys,
+                      poly_stride, 0, S(stream));
+    });
+}
+
+int mi_mul_scalar(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+                  const uint64_t* residues, void* stream) {
+    return guard("ckks_mul_inplace(ct,int)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(polys); check_ptr(residues);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
+        mi::ew_scalar(c, mi::EW_MUL, polys, residues, level + 1, npolys, poly_stride, S(stream));
+    });
+}
+
+int mi_add(mi_context* ctx, int level, uint64_t* x, const uint64_t* y, int npolys,
+           int64_t poly_stride, void* stream) {
+    return guard("ckks_add_inplace(ct,ct)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(x); check_ptr(y);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
+        mi::ew_binary(c, mi::EW_ADD, x, y, level + 1, npolys, poly_stride, poly_stride, S(stream));
+    });
+}
+
+int mi_sub(mi_context* ctx, int level, uint64_t* x, const uint64_t* y, int npolys,
+           int64_t poly_stride, void* stream) {
+    return guard("ckks_sub_inplace(ct,ct)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(x); check_ptr(y);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
+        mi::ew_binary(c, mi::EW_SUB, x, y, level + 1, npolys, poly_stride, poly_stride, S(stream));
+    });
+}
+
+int mi_add_plain(mi_context* ctx, int level, uint64_t* b, const uint64_t* pt, int pt_montgomery,
+                 void* stream) {
+    return guard("ckks_add_inplace(ct,pt)", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(pt);
+        const mi::Context& c = *ctx->c;
+        if (pt_montgomery)
+            mi::add_plain_mont(c, level + 1, b, pt, S(stream));
+        else
+            mi::ew_binary(c, mi::EW_ADD, b, pt, level + 1, 1, 0, 0, S(stream));
+    });
+}
+
+int mi_add_scalar(mi_context* ctx, int level, uint64_t* b, const uint64_t* residues, void* stream) {
+    return guard("ckks_add_integer_constant_inplace", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(residues);
+        mi::ew_scalar(*ctx->c, mi::EW_ADD, b, residues, level + 1, 1, 0, S(stream));
+    });
+}
+
+int mi_automorphism(mi_context* ctx, int limbs, uint64_t galois, const uint64_t* src, uint64_t* dst,
+                    int npolys, int64_t poly_stride, void* stream) {
+    return guard("ckks_apply_ntt_automorphism", [&] {
+        check_ctx(ctx);
+        check_ptr(src); check_ptr(dst);
+        const mi::Context& c = *ctx->c;
+        if (limbs < 1 || limbs > c.nb() || npolys < 1 || poly_stride < (int64_t)limbs * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid batch shape");
+        mi::automorphism(c, limbs, galois, src, dst, npolys, poly_stride, poly_stride, S(stream));
+    });
+}
+
+int mi_rescale(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+               void* stream) {
+    return guard("ckks_rescale", [&] {
+        check_ctx(ctx);
+        check_ptr(polys);
+        mi::Context& c = *ctx->c;
+        if (level < 1 || level >= c.nq) throw mi::Error(MI_EINVAL, "invalid level");
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext buffer size");
+        mi::rescale(c, level, polys, npolys, poly_stride, S(stream));
+    });
+}
+
+int mi_keyswitch(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                 uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
+                 void* stream) {
+    return guard("ckks_hybrid_keyswitch", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(key_term); check_ptr(key); check_ptr(b_out); check_ptr(a_out);
+        if ((b_mode != 0 && b_mode != 1) || (a_mode != 0 && a_mode != 1))
+            throw mi::Error(MI_EINVAL, "invalid combine mode");
+        mi::keyswitch(*ctx->c, mi::KsItem{key_term, key, b_out, a_out, level, b_mode, a_mode},
+                      montgomery != 0, S(stream));
+    });
+}
+
+int mi_keyswitch_batch(mi_context* ctx, const mi_keyswitch_item* items, int count, int montgomery,
+                       void* stream) {
+    return guard("ckks_hybrid_keyswitch", [&] {
+        check_ctx(ctx);
+        if (count < 0 || (count > 0 && !items)) throw mi::Error(MI_EINVAL, "invalid batch");
+        for (int i = 0; i < count; ++i) {
+            const mi_keyswitch_item& it = items[i];
+            check_level(*ctx->c, it.level);
+            check_ptr(it.key_term); check_ptr(it.key); check_ptr(it.b_out); check_ptr(it.a_out);
+        }
+        for (int i = 0; i < count; ++i) {
+            const mi_keyswitch_item& it = items[i];
+            mi::keyswitch(*ctx->c,
+                          mi::KsItem{it.key_term, it.key, it.b_out, it.a_out, it.level, it.b_mode, it.a_mode},
+                          montgomery != 0, S(stream));
+        }
+    });
+}
+
+int mi_relin(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uint64_t* c2,
+             const uint64_t* key, int montgomery, void* stream) {
+    return guard("ckks_relin", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(a); check_ptr(c2); check_ptr(key);
+        mi::keyswitch(*ctx->c, mi::KsItem{c2, key, b, a, level, 0, 0}, montgomery != 0, S(stream));
+    });
+}
+
+int mi_rotate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t galois,
+              const uint64_t* key, int montgomery, void* stream) {
+    return guard("ckks_rotate", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(a); check_ptr(key);
+        mi::Context& c = *ctx->c;
+        const int L = level + 1;
+        const long long words = (long long)L * (long long)c.N;
+        mi::keyswitch(c, mi::KsItem{a, key, b, a, level, 0, 1}, montgomery != 0, S(stream));
+        uint64_t* tmp = (uint64_t*)c.workspace(S(stream), sizeof(uint64_t) * 2 * words);
+        mi::automorphism(c, L, galois, b, tmp, 1, words, words, S(stream));
+        mi::automorphism(c, L, galois, a, tmp + words, 1, words, words, S(stream));
+        MI_HIP(hipMemcpyAsync(b, tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+        MI_HIP(hipMemcpyAsync(a, tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+    });
+}
+
+int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uint64_t* key,
+                 int montgomery, void* stream) {
+    return guard("ckks_conjugate", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(b); check_ptr(a); check_ptr(key);
+        mi::Context& c = *ctx->c;
+        const int L = level + 1;
+        const long long words = (long long)L * (long long)c.N;
+        const uint64_t g = 2 * c.N - 1;
+        uint64_t* tmp = (uint64_t*)c.workspace(S(stream), sizeof(uint64_t) * 2 * words);
+        mi::automorphism(c, L, g, b, tmp, 1, words, words, S(stream));
+        mi::automorphism(c, L, g, a, tmp + words, 1, words, words, S(stream));
+        MI_HIP(hipMemcpyAsync(b, tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+        MI_HIP(hipMemcpyAsync(a, tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+        mi::keyswitch(c, mi::KsItem{a, key, b, a, level, 0, 1}, montgomery != 0, S(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Register-resident sub-transforms used by every NTT-shaped kernel.
+//
+// An N = 2^n negacyclic NTT (the reference's Harvey DIT forward / DIF inverse
+// with merged psi twiddles, ntt.cpp:235-386) is split into two passes over
+// HBM: n1 = n/2 "column" stages and n2 = n - n1 "row" stages.  Inside a pass
+// each group of R = 2^LOGR elements (a column or a contiguous row block) is
+// held by TG = R/16 threads, 16 words per thread in VGPRs.  Butterflies whose
+// partner is in the same thread run in registers; one LDS exchange switches
+// between the strided layout (position t + TG*j, coalesced in HBM) and the
+// contiguous layout (position t*16 + j) that makes the remaining stages
+// thread-local.  Twiddle indices are the reference's global table indices.
+#pragma once
+#include "mi_arith.hpp"
+
+namespace mi {
+
+constexpr int E = 16;
+constexpr int LOGE = 4;
+constexpr int NTT_THREADS = 256;
+// LDS words per workgroup: NTT_THREADS*E data words plus one pad per 16.
+constexpr int NTT_LDS_WORDS = NTT_THREADS * E + NTT_THREADS;
+
+__device__ __forceinline__ int lpad(int p) { return p + (p >> LOGE); }
+
+// Forward (DIT, Harvey lazy): x0,x1 in [0,4q) -> [0,4q).
+__device__ __forceinline__ void bf_fwd(u64& x0, u64& x1, u64 w, u64 ws, u64 q, u64 q2) {
+    u64 t = x0;
+    if (t >= q2) t -= q2;
+    u64 u = shoup_lazy(x1, w, ws, q);
+    x0 = t + u;
+    x1 = t + q2 - u;
+}
+
+// Inverse (DIF): x0,x1 in [0,2q) -> [0,2q).
+__device__ __forceinline__ void bf_inv(u64& x0, u64& x1, u64 w, u64 ws, u64 q, u64 q2) {
+    u64 t = x0, u = x1;
+    u64 s = t + u;
+    if (s >= q2) s -= q2;
+    x1 = shoup_lazy(t - u + q2, w, ws, q);
+    x0 = s;
+}
+
+template <int LOGR>
+struct Grp {
+    static constexpr int R = 1 << LOGR;
+    static constexpr int TG = R / E;
+    static constexpr int LOGTG = LOGR - LOGE;
+    static constexpr int LDS = R + R / E;  // padded words per group
+    static_assert(LOGR >= LOGE + 1 && LOGR <= 2 * LOGE, "group size 32..256");
+};
+
+// Exchange: strided (t + TG*j) <-> contiguous (t*E + j).  Caller guarantees a
+// barrier separates it from any earlier reads of `lds`.
+template <int LOGR>
+__device__ __forceinline__ void to_contiguous(u64 (&x)[E], int t, u64* lds) {
+    using G = Grp<LOGR>;
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t + G::TG * j)] = x[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) x[j] = lds[lpad(t * E + j)];
+}
+template <int LOGR>
+__device__ __forceinline__ void to_strided(u64 (&x)[E], int t, u64* lds) {
+    using G = Grp<LOGR>;
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t * E + j)] = x[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) x[j] = lds[lpad(t + G::TG * j)];
+}
+
+// Forward group transform.  In: strided layout; out: strided layout.
+// idx(s, i) gives the global twiddle index of group sub-stage s, block i.
+template <int LOGR, class IDX>
+__device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, u64 q,
+                                          const u64* __restrict__ W, const u64* __restrict__ WS,
+                                          IDX idx) {
+    using G = Grp<LOGR>;
+    const u64 q2 = q << 1;
+#pragma unroll
+    for (int s = 0; s < LOGE; ++s) {
+        const int dj = E >> (s + 1);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & dj) continue;
+            const int k = idx(s, j0 >> (LOGE - s));
+            bf_fwd(x[j0], x[j0 + dj], W[k], WS[k], q, q2);
+        }
+    }
+    __syncthreads();
+    to_contiguous<LOGR>(x, t, lds);
+#pragma unroll
+    for (int s = LOGE; s < LOGR; ++s) {
+        const int d = G::R >> (s + 1);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & d) continue;
+            const int k = idx(s, (t * E + j0) >> (LOGR - s));
+            bf_fwd(x[j0], x[j0 + d], W[k], WS[k], q, q2);
+        }
+    }
+    __syncthreads();
+    to_strided<LOGR>(x, t, lds);
+}
+
+// Inverse group transform.  In: strided layout; out: strided layout.
+// idx(u, i): global twiddle index for group sub-stage u (distance 2^u), block i.
+template <int LOGR, class IDX>
+__device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, u64 q,
+                                          const u64* __restrict__ W, const u64* __restrict__ WS,
+                                          IDX idx) {
+    using G = Grp<LOGR>;
+    const u64 q2 = q << 1;
+    __syncthreads();
+    to_contiguous<LOGR>(x, t, lds);
+#pragma unroll
+    for (int u = 0; u < LOGE; ++u) {
+        const int d = 1 << u;
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & d) continue;
+            const int k = idx(u, (t * E + j0) >> (u + 1));
+            bf_inv(x[j0], x[j0 + d], W[k], WS[k], q, q2);
+        }
+    }
+    __syncthreads();
+    to_strided<LOGR>(x, t, lds);
+#pragma unroll
+    for (int u = LOGE; u < LOGR; ++u) {
+        const int dj = 1 << (u - G::LOGTG);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & dj) continue;
+            const int k = idx(u, j0 >> (u + 1 - G::LOGTG));
+            bf_inv(x[j0], x[j0 + dj], W[k], WS[k], q, q2);
+        }
+    }
+}
+
+// Geometry of one pass.  Column passes: group = column (stride R2 words
+// between elements), thread (g = tid % G, t = tid / G).  Row passes: group =
+// contiguous block of R2 words, thread (g = tid / TG, t = tid % TG).
+struct PassGeom {
+    int n, n1, n2;
+    int threads;     // workgroup size
+    int groups;      // groups per workgroup
+    int wg_per_limb; // workgroups per limb
+};
+
+}  // namespace mi

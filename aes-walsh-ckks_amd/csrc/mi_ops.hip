@@ -1,0 +1,252 @@
+// Element-wise evaluator kernels, automorphisms and the fused rescale.
+//
+//   tensor / square / x plaintext / x int   ckks_mul.cpp:37-272
+//   add / sub / + plaintext / + constant    ckks_addition.cpp:35-139
+//   NTT-domain automorphism gather          ckks_keyswitch.cpp:37-60 with the
+//                                           map of rotation_key_generation.cpp:80-90
+//   rescale by the top modulus              ckks_rescale.cpp:68-148
+// All products are exact (Barrett or Montgomery REDC), so canonical outputs
+// equal the reference's.
+#include "mi_ntt_kernels.hpp"
+
+namespace mi {
+
+constexpr int EW_THREADS = 256;
+
+static unsigned ew_blocks(long long work) {
+    long long b = (work + EW_THREADS - 1) / EW_THREADS;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+__global__ void __launch_bounds__(EW_THREADS)
+k_ew_binary(const BasisConst* __restrict__ bc, int op, u64* x, const u64* __restrict__ y,
+            int logN, int limbs, long long xstride, long long ystride, long long total) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const long long per = (long long)limbs << logN;
+    const long long k = i / per, r = i - k * per;
+    const int l = (int)(r >> logN);
+    const BasisConst b = bc[l];
+    u64* px = x + k * xstride + r;
+    const u64 vy = y[k * ystride + r];
+    const u64 vx = *px;
+    u64 o;
+    switch (op) {
+        case EW_ADD: o = add_mod(vx, vy, b.q); break;
+        case EW_SUB: o = sub_mod(vx, vy, b.q); break;
+        case EW_MUL: o = mul_mod(vx, vy, b.q, b.mu_lo, b.mu_hi); break;
+        default: o = mont_mul(vx, vy, b.q, b.qninv); break;
+    }
+    *px = o;
+}
+
+void ew_binary(const Context& c, int op, u64* x, const u64* y, int limbs, int npolys,
+               long long xstride, long long ystride, hipStream_t s) {
+    long long total = (long long)npolys * limbs * (long long)c.N;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_ew_binary, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, op, x,
+                       y, c.logN, limbs, xstride, ystride, total);
+    MI_HIP(hipGetLastError());
+}
+
+struct Scalars {
+    u64 v[MAX_LIMBS];
+};
+
+__global__ void __launch_bounds__(EW_THREADS)
+k_ew_scalar(const BasisConst* __restrict__ bc, int op, u64* x, Scalars sc, int logN, int limbs,
+            long long stride, long long total) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const long long per = (long long)limbs << logN;
+    const long long k = i / per, r = i - k * per;
+    const int l = (int)(r >> logN);
+    const BasisConst b = bc[l];
+    u64* px = x + k * stride + r;
+    const u64 s = sc.v[l];
+    *px = op == EW_ADD ? add_mod(*px, s, b.q) : mul_mod(*px, s, b.q, b.mu_lo, b.mu_hi);
+}
+
+void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int limbs, int npolys,
+               long long stride, hipStream_t s) {
+    if (limbs > MAX_LIMBS) throw Error(-1, "too many limbs for a scalar operation");
+    Scalars sc{};
+    for (int l = 0; l < limbs; ++l) {
+        if (scalars_host[l] >= c.h_bc[l].q) throw Error(-1, "scalar residue is not reduced");
+        sc.v[l] = scalars_host[l];
+    }
+    long long total = (long long)npolys * limbs * (long long)c.N;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_ew_scalar, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, op, x,
+                       sc, c.logN, limbs, stride, total);
+    MI_HIP(hipGetLastError());
+}
+
+// ckks_mul.cpp:127-154: c2 = a1 a2, b1 <- b1 b2, a1 <- a1 b2 + b1 a2
+__global__ void __launch_bounds__(EW_THREADS)
+k_tensor(const BasisConst* __restrict__ bc, int logN, long long total, u64* b1, u64* a1,
+         const u64* __restrict__ b2, const u64* __restrict__ a2, u64* c2) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const BasisConst b = bc[i >> logN];
+    const u64 vb1 = b1[i], va1 = a1[i], vb2 = b2[i], va2 = a2[i];
+    c2[i] = mul_mod(va1, va2, b.q, b.mu_lo, b.mu_hi);
+    b1[i] = mul_mod(vb1, vb2, b.q, b.mu_lo, b.mu_hi);
+    a1[i] = add_mod(mul_mod(va1, vb2, b.q, b.mu_lo, b.mu_hi), mul_mod(vb1, va2, b.q, b.mu_lo, b.mu_hi), b.q);
+}
+
+void tensor(const Context& c, int limbs, u64* b1, u64* a1, const u64* b2, const u64* a2, u64* c2,
+            hipStream_t s) {
+    long long total = (long long)limbs * (long long)c.N;
+    hipLaunchKernelGGL(k_tensor, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, c.logN,
+                       total, b1, a1, b2, a2, c2);
+    MI_HIP(hipGetLastError());
+}
+
+// ckks_mul.cpp:69-83: c2 = a^2, b <- b^2, a <- 2ab
+__global__ void __launch_bounds__(EW_THREADS)
+k_square(const BasisConst* __restrict__ bc, int logN, long long total, u64* b, u64* a, u64* c2) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const BasisConst k = bc[i >> logN];
+    const u64 vb = b[i], va = a[i];
+    c2[i] = mul_mod(va, va, k.q, k.mu_lo, k.mu_hi);
+    b[i] = mul_mod(vb, vb, k.q, k.mu_lo, k.mu_hi);
+    const u64 ab = mul_mod(va, vb, k.q, k.mu_lo, k.mu_hi);
+    a[i] = add_mod(ab, ab, k.q);
+}
+
+void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s) {
+    long long total = (long long)limbs * (long long)c.N;
+    hipLaunchKernelGGL(k_square, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, c.logN,
+                       total, b, a, c2);
+    MI_HIP(hipGetLastError());
+}
+
+// b += REDC(1 * pt): Montgomery plaintext back to normal form (ckks_encoding.cpp:67-78)
+__global__ void __launch_bounds__(EW_THREADS)
+k_add_plain_mont(const BasisConst* __restrict__ bc, int logN, long long total, u64* b,
+                 const u64* __restrict__ pt) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const BasisConst k = bc[i >> logN];
+    b[i] = add_mod(b[i], redc128(pt[i], 0, k.q, k.qninv), k.q);
+}
+
+void add_plain_mont(const Context& c, int limbs, u64* b, const u64* pt, hipStream_t s) {
+    long long total = (long long)limbs * (long long)c.N;
+    hipLaunchKernelGGL(k_add_plain_mont, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc,
+                       c.logN, total, b, pt);
+    MI_HIP(hipGetLastError());
+}
+
+// dst[brev(d)] = src[brev(((2d+1) g mod 2N - 1) / 2)]
+__device__ __forceinline__ u32 brev_n(u32 x, int logN) { return __brev(x) >> (32 - logN); }
+
+__global__ void __launch_bounds__(EW_THREADS)
+k_automorphism(int logN, u64 galois, int limbs, const u64* __restrict__ src, u64* __restrict__ dst,
+               long long sstride, long long dstride, long long total) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const long long per = (long long)limbs << logN;
+    const long long k = i / per, r = i - k * per;
+    const long long limb_off = r & ~(((long long)1 << logN) - 1);
+    const u32 idx = (u32)(r & (((long long)1 << logN) - 1));
+    const u32 mask2n = (2u << logN) - 1;
+    const u32 d = brev_n(idx, logN);
+    const u32 e = (u32)(((u64)(2 * d + 1) * galois) & mask2n);
+    const u32 sidx = brev_n((e - 1) >> 1, logN);
+    dst[k * dstride + limb_off + idx] = src[k * sstride + limb_off + sidx];
+}
+
+void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
+                  long long sstride, long long dstride, hipStream_t s) {
+    if (src == dst) throw Error(-1, "source and destination must differ");
+    if ((galois & 1) == 0) throw Error(-1, "Galois element must be odd");
+    long long total = (long long)npolys * limbs * (long long)c.N;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_automorphism, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.logN,
+                       galois % (2 * c.N), limbs, src, dst, sstride, dstride, total);
+    MI_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- rescale
+// Column pass input: rem_i = ((x_l + floor(p/2)) mod p) mod q_i from the
+// coefficient-domain top limb; output to scratch.
+struct RescaleInJob {
+    const u64* polys;
+    long long stride;
+    u64* scratch;     // [npolys][level][N]
+    int level;
+    u64 N;
+    u64 p, half;
+    const BasisConst* bc;
+    struct View {
+        int basis;
+        const u64* top;
+        u64* out;
+        u64 p, half, q, mu_hi;
+        __device__ u64 load(int i) const {
+            u64 s = top[i] + half;
+            if (s >= p) s -= p;
+            return reduce64(s, q, mu_hi);
+        }
+        __device__ void store(int i, u64 v) const { out[i] = v; }
+    };
+    __device__ View view(int k) const {
+        const int poly = k / level, i = k - poly * level;
+        const BasisConst b = bc[i];
+        return View{i, polys + poly * stride + (long long)level * N,
+                    scratch + ((long long)poly * level + i) * N, p, half, b.q, b.mu_hi};
+    }
+    __device__ bool skip(int) const { return false; }
+};
+
+// Row pass output: x_i <- (x_i + NTT(half) - NTT(rem)) * p^-1 mod q_i
+struct RescaleOutJob {
+    u64* polys;
+    long long stride;
+    const u64* scratch;
+    int level;
+    u64 N;
+    const u64* half_ntt;
+    const u64* inv_p;
+    const u64* inv_p_shoup;
+    const BasisConst* bc;
+    struct View {
+        int basis;
+        const u64* src;
+        u64* x;
+        const u64* half;
+        u64 q, ip, ips;
+        __device__ u64 load(int i) const { return src[i]; }
+        __device__ void store(int i, u64 v) const {
+            const u64 num = add_mod(x[i], half[i], q);
+            x[i] = shoup(sub_mod(num, v, q), ip, ips, q);
+        }
+    };
+    __device__ View view(int k) const {
+        const int poly = k / level, i = k - poly * level;
+        return View{i, scratch + ((long long)poly * level + i) * N,
+                    polys + poly * stride + (long long)i * N, half_ntt + (long long)i * N,
+                    bc[i].q, inv_p[i], inv_p_shoup[i]};
+    }
+    __device__ bool skip(int) const { return false; }
+};
+
+void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s) {
+    if (level < 1 || level >= c.nq) throw Error(-1, "invalid level");
+    if (npolys <= 0) return;
+    const u64 N = c.N;
+    // 1. top limb of every poly to the coefficient domain (in place)
+    ntt_inverse(c, LimbBatch{polys + (long long)level * N, stride, 1, level, npolys}, s);
+    // 2. fused remainder -> NTT -> finish
+    u64* scratch = (u64*)c.workspace(s, sizeof(u64) * (size_t)npolys * level * N);
+    const u64 p = c.q[level];
+    const RescaleTables& rt = c.rescale[level];
+    RescaleInJob in{polys, stride, scratch, level, N, p, p >> 1, c.d_bc};
+    RescaleOutJob out{polys, stride, scratch, level, N, rt.half_ntt, rt.inv_p, rt.inv_p_shoup, c.d_bc};
+    run_ntt_fwd(c, in, out, npolys * level, s);
+}
+
+}  // namespace mi

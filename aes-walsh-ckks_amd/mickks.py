@@ -1,0 +1,219 @@
+"""Python binding of libmickks.so (the HIP C-ABI in include/mickks.h).
+
+Device memory and streams come from PyTorch (ROCm): buffers are torch int64
+tensors on the GPU holding uint64 residues bit for bit.  There is no CPU
+fallback: if the library is missing or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmickks.so")
+
+_lib = None
+
+vp = ctypes.c_void_p
+cint = ctypes.c_int
+u64 = ctypes.c_uint64
+i64 = ctypes.c_int64
+
+_SIGS = {
+    "mi_last_error": (ctypes.c_char_p, []),
+    "mi_context_create": (cint, [ctypes.POINTER(vp), u64, vp, cint, vp, cint, cint]),
+    "mi_context_destroy": (cint, [vp]),
+    "mi_context_info": (cint, [vp, vp]),
+    "mi_context_device_bytes": (i64, [vp]),
+    "mi_ntt_forward": (cint, [vp, vp, cint, cint, i64, cint, vp]),
+    "mi_ntt_inverse": (cint, [vp, vp, cint, cint, i64, cint, vp]),
+    "mi_tensor": (cint, [vp, cint, vp, vp, vp, vp, vp, vp]),
+    "mi_square": (cint, [vp, cint, vp, vp, vp, vp]),
+    "mi_mul_plain": (cint, [vp, cint, vp, cint, i64, vp, cint, vp]),
+    "mi_mul_scalar": (cint, [vp, cint, vp, cint, i64, vp, vp]),
+    "mi_add": (cint, [vp, cint, vp, vp, cint, i64, vp]),
+    "mi_sub": (cint, [vp, cint, vp, vp, cint, i64, vp]),
+    "mi_add_plain": (cint, [vp, cint, vp, vp, cint, vp]),
+    "mi_add_scalar": (cint, [vp, cint, vp, vp, vp]),
+    "mi_automorphism": (cint, [vp, cint, u64, vp, vp, cint, i64, vp]),
+    "mi_rescale": (cint, [vp, cint, vp, cint, i64, vp]),
+    "mi_keyswitch": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, cint, vp]),
+    "mi_keyswitch_batch": (cint, [vp, vp, cint, cint, vp]),
+    "mi_relin": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
+    "mi_rotate": (cint, [vp, cint, vp, vp, u64, vp, cint, vp]),
+    "mi_conjugate": (cint, [vp, cint, vp, vp, vp, cint, vp]),
+}
+
+EXPORTED = sorted(_SIGS)
+
+
+class KeySwitchItem(ctypes.Structure):
+    _fields_ = [("key_term", vp), ("key", vp), ("b_out", vp), ("a_out", vp),
+                ("level", ctypes.c_int32), ("b_mode", ctypes.c_int32),
+                ("a_mode", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "-j8"])
+
+
+def lib():
+    """Load libmickks.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        # Load PyTorch's HIP runtime first so that libmickks.so binds to the
+        # same libamdhip64.so.7 (one runtime per process; a second copy would
+        # hide the GPU from the first).
+        import torch  # noqa: F401
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmickks.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class MiError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise MiError(lib().mi_last_error().decode())
+
+
+def _stream(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return vp(stream.cuda_stream)
+
+
+def ptr(t):
+    return vp(t.data_ptr())
+
+
+class Context:
+    """Device-resident CKKS tables for chain moduli q, special moduli p, dnum digits."""
+
+    def __init__(self, N, q, p, dnum):
+        import numpy as np
+        self.N, self.dnum = int(N), int(dnum)
+        self.q = np.ascontiguousarray(np.asarray(q, dtype=np.uint64))
+        self.p = np.ascontiguousarray(np.asarray(p, dtype=np.uint64))
+        h = vp()
+        check(lib().mi_context_create(ctypes.byref(h), self.N, self.q.ctypes.data, len(self.q),
+                                      self.p.ctypes.data, len(self.p), self.dnum))
+        self.h = h
+        info = (ctypes.c_int64 * 6)()
+        check(lib().mi_context_info(self.h, ctypes.addressof(info)))
+        self.alpha = int(info[4])
+        self.nq, self.np = len(self.q), len(self.p)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mi_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def parts(self, level):
+        L = level + 1
+        return min(self.dnum, (L + self.alpha - 1) // self.alpha)
+
+    def device_bytes(self):
+        return int(lib().mi_context_device_bytes(self.h))
+
+    # ---- NTT
+    def ntt_forward(self, data, npolys, limbs, stride=None, basis0=0, stream=None):
+        stride = limbs * self.N if stride is None else stride
+        check(lib().mi_ntt_forward(self.h, ptr(data), npolys, limbs, stride, basis0, _stream(stream)))
+
+    def ntt_inverse(self, data, npolys, limbs, stride=None, basis0=0, stream=None):
+        stride = limbs * self.N if stride is None else stride
+        check(lib().mi_ntt_inverse(self.h, ptr(data), npolys, limbs, stride, basis0, _stream(stream)))
+
+    # ---- element-wise
+    def tensor(self, level, b1, a1, b2, a2, c2, stream=None):
+        check(lib().mi_tensor(self.h, level, ptr(b1), ptr(a1), ptr(b2), ptr(a2), ptr(c2), _stream(stream)))
+
+    def square(self, level, b, a, c2, stream=None):
+        check(lib().mi_square(self.h, level, ptr(b), ptr(a), ptr(c2), _stream(stream)))
+
+    def mul_plain(self, level, polys, npolys, pt, pt_montgomery=False, stride=None, stream=None):
+        stride = (level + 1) * self.N if stride is None else stride
+        check(lib().mi_mul_plain(self.h, level, ptr(polys), npolys, stride, ptr(pt), int(pt_montgomery),
+                                 _stream(stream)))
+
+    def mul_scalar(self, level, polys, npolys, residues, stride=None, stream=None):
+        import numpy as np
+        r = np.ascontiguousarray(np.asarray(residues, dtype=np.uint64))
+        stride = (level + 1) * self.N if stride is None else stride
+        check(lib().mi_mul_scalar(self.h, level, ptr(polys), npolys, stride, r.ctypes.data, _stream(stream)))
+
+    def add(self, level, x, y, npolys, stride=None, stream=None):
+        stride = (level + 1) * self.N if stride is None else stride
+        check(lib().mi_add(self.h, level, ptr(x), ptr(y), npolys, stride, _stream(stream)))
+
+    def sub(self, level, x, y, npolys, stride=None, stream=None):
+        stride = (level + 1) * self.N if stride is None else stride
+        check(lib().mi_sub(self.h, level, ptr(x), ptr(y), npolys, stride, _stream(stream)))
+
+    def add_plain(self, level, b, pt, pt_montgomery=False, stream=None):
+        check(lib().mi_add_plain(self.h, level, ptr(b), ptr(pt), int(pt_montgomery), _stream(stream)))
+
+    def add_scalar(self, level, b, residues, stream=None):
+        import numpy as np
+        r = np.ascontiguousarray(np.asarray(residues, dtype=np.uint64))
+        check(lib().mi_add_scalar(self.h, level, ptr(b), r.ctypes.data, _stream(stream)))
+
+    def automorphism(self, limbs, galois, src, dst, npolys=1, stride=None, stream=None):
+        stride = limbs * self.N if stride is None else stride
+        check(lib().mi_automorphism(self.h, limbs, int(galois), ptr(src), ptr(dst), npolys, stride,
+                                    _stream(stream)))
+
+    def rescale(self, level, polys, npolys, stride=None, stream=None):
+        stride = (level + 1) * self.N if stride is None else stride
+        check(lib().mi_rescale(self.h, level, ptr(polys), npolys, stride, _stream(stream)))
+
+    # ---- key switching
+    def keyswitch(self, level, key_term, key, b_out, a_out, b_mode, a_mode, montgomery=True, stream=None):
+        check(lib().mi_keyswitch(self.h, level, ptr(key_term), ptr(key), ptr(b_out), ptr(a_out), b_mode,
+                                 a_mode, int(montgomery), _stream(stream)))
+
+    def keyswitch_batch(self, items, montgomery=True, stream=None):
+        arr = (KeySwitchItem * len(items))()
+        for i, (lv, kt, key, b, a, bm, am) in enumerate(items):
+            arr[i] = KeySwitchItem(kt.data_ptr(), key.data_ptr(), b.data_ptr(), a.data_ptr(), lv, bm, am, 0)
+        check(lib().mi_keyswitch_batch(self.h, ctypes.addressof(arr), len(items), int(montgomery),
+                                       _stream(stream)))
+
+    def relin(self, level, b, a, c2, key, montgomery=True, stream=None):
+        check(lib().mi_relin(self.h, level, ptr(b), ptr(a), ptr(c2), ptr(key), int(montgomery), _stream(stream)))
+
+    def rotate(self, level, b, a, galois, key, montgomery=True, stream=None):
+        check(lib().mi_rotate(self.h, level, ptr(b), ptr(a), int(galois), ptr(key), int(montgomery),
+                              _stream(stream)))
+
+    def conjugate(self, level, b, a, key, montgomery=True, stream=None):
+        check(lib().mi_conjugate(self.h, level, ptr(b), ptr(a), ptr(key), int(montgomery), _stream(stream)))
+
+
+def to_dev(arr):
+    """numpy uint64 array -> torch int64 tensor on the GPU (same bits)."""
+    import numpy as np
+    import torch
+    a = np.ascontiguousarray(np.asarray(arr, dtype=np.uint64))
+    return torch.from_numpy(a.view(np.int64).copy()).cuda()
+
+
+def from_dev(t):
+    """torch int64 GPU tensor -> numpy uint64 array."""
+    import numpy as np
+    return t.detach().cpu().numpy().view(np.uint64).copy()

@@ -1,0 +1,138 @@
+/*
+ * mickks.h -- C-ABI of the MI355X (gfx950) CKKS evaluation library.
+ *
+ * Plain extern "C" entry points over device pointers and sizes.  Every call
+ * returns 0 on success or a negative MI_E* code; mi_last_error() gives the
+ * thread's last message, prefixed "<function>: " like the reference's
+ * exceptions.  Streams are passed as void* (a hipStream_t; NULL = default).
+ *
+ * Data conventions are the reference's (aes-walsh-ckks src/): an RNS
+ * polynomial is limb-major uint64[limbs][N]; evaluation-domain index i holds
+ * A(psi^(2*brev(i)+1)); stored residues are canonical [0, q).  A ciphertext is
+ * the polys [b, a, (c2)].  A switching key is uint64[parts][2][nq+np][N]
+ * (poly 0 = b, poly 1 = a, Q limbs 0..nq-1 then P limbs), i.e. the
+ * reference's key digits (CKKSCiphertext at the maximum level over Q u P,
+ * ckks_keyswitch.cpp:20-35) stored back to back, already folded and (when
+ * montgomery=1) in Montgomery form as CKKSContext generates them.
+ *
+ * Basis indices: 0..nq-1 name the chain moduli q_i, nq..nq+np-1 the special
+ * moduli p_j.
+ */
+#ifndef MICKKS_H
+#define MICKKS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MI_OK = 0,
+    MI_EINVAL = -1,   /* bad argument (reference: std::invalid_argument) */
+    MI_ERUNTIME = -2, /* missing precompute / internal (std::runtime_error) */
+    MI_EHIP = -3      /* HIP runtime failure */
+};
+
+typedef struct mi_context mi_context;
+
+/* Thread-local text of the last error. */
+const char* mi_last_error(void);
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the device-relevant part of CKKSContext's constructor
+ * (ckks_context.cpp:603-705): NTT twiddle tables (ntt.cpp:54-148), HPS ModUp
+ * tables per (level, digit) (ckks_context.cpp:468-535,
+ * relinearization_key_generation.cpp:45-129), folded P^-1 tables
+ * (ckks_context.cpp:537-560) and rescale constants (ckks_context.cpp:565-599),
+ * all uploaded once to HBM.  q: chain moduli (limb 0..Lmax), p: special moduli.
+ */
+int mi_context_create(mi_context** out, uint64_t N, const uint64_t* q, int nq,
+                      const uint64_t* p, int np, int dnum);
+int mi_context_destroy(mi_context* ctx);
+/* info[0..5] = N, nq, np, dnum, alpha, device id */
+int mi_context_info(const mi_context* ctx, int64_t* info);
+/* Device memory the context holds (tables + workspaces), in bytes. */
+int64_t mi_context_device_bytes(const mi_context* ctx);
+
+/* ---- negacyclic NTT (ntt.cpp:235-447) -----------------------------------
+ * Batched over npolys polys of `limbs` limbs each; limb l of every poly uses
+ * basis index basis0+l; polys are poly_stride words apart (>= limbs*N).
+ * Forward: natural -> bit-reversed evaluation order.  Inverse includes N^-1.
+ * Inputs and outputs canonical. */
+int mi_ntt_forward(mi_context* ctx, uint64_t* data, int npolys, int limbs, int64_t poly_stride,
+                   int basis0, void* stream);
+int mi_ntt_inverse(mi_context* ctx, uint64_t* data, int npolys, int limbs, int64_t poly_stride,
+                   int basis0, void* stream);
+
+/* ---- element-wise ciphertext ops (level = limbs-1 over q_0..q_level) ----- */
+/* ckks_mul_inplace(ct, ct) tensor, ckks_mul.cpp:90-157: (b1,a1) <- (b1 b2, a1 b2 + b1 a2), c2 <- a1 a2 */
+int mi_tensor(mi_context* ctx, int level, uint64_t* b1, uint64_t* a1, const uint64_t* b2,
+              const uint64_t* a2, uint64_t* c2, void* stream);
+/* ckks_square_inplace, ckks_mul.cpp:37-88 */
+int mi_square(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2, void* stream);
+/* ckks_mul_inplace(ct, pt), ckks_mul.cpp:160-212 (pt_montgomery: plaintext stored x 2^64) */
+int mi_mul_plain(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+                 const uint64_t* pt, int pt_montgomery, void* stream);
+/* ckks_mul_inplace(ct, int64), ckks_mul.cpp:214-272: residues[i] = scalar mod q_i (host array) */
+int mi_mul_scalar(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+                  const uint64_t* residues, void* stream);
+/* ckks_add_inplace / ckks_sub_inplace (ct, ct), ckks_addition.cpp:62-139 */
+int mi_add(mi_context* ctx, int level, uint64_t* x, const uint64_t* y, int npolys,
+           int64_t poly_stride, void* stream);
+int mi_sub(mi_context* ctx, int level, uint64_t* x, const uint64_t* y, int npolys,
+           int64_t poly_stride, void* stream);
+/* ckks_add_inplace(ct, pt), ckks_addition.cpp:35-60 */
+int mi_add_plain(mi_context* ctx, int level, uint64_t* b, const uint64_t* pt, int pt_montgomery,
+                 void* stream);
+/* ckks_add_integer_constant_inplace, ckks_addition.cpp:89-112 (host residues per limb) */
+int mi_add_scalar(mi_context* ctx, int level, uint64_t* b, const uint64_t* residues, void* stream);
+
+/* ---- automorphism (ckks_keyswitch.cpp:37-60, rotation_key_generation.cpp:80-90)
+ * dst[i] = src[map_g(i)] in the evaluation domain for Galois element g
+ * (5^r mod 2N for a rotation by r, 2N-1 for conjugation). src != dst. */
+int mi_automorphism(mi_context* ctx, int limbs, uint64_t galois, const uint64_t* src,
+                    uint64_t* dst, int npolys, int64_t poly_stride, void* stream);
+
+/* ---- rescale (ckks_rescale.cpp:68-148) ------------------------------------
+ * Divides by q_level with rounding; the first level*N words of each poly
+ * hold the result (stride unchanged). */
+int mi_rescale(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
+               void* stream);
+
+/* ---- hybrid key switching (ckks_keyswitch.cpp:1330-1641) ------------------
+ * key_term: (level+1)*N eval-domain words (read only).  b_out/a_out are
+ * combined with mode 0 = Add, 1 = Assign (CKKSHybridKeySwitchCombine).
+ * key_term may alias a_out (rotation).  fold_hybrid_keyswitch=true keys. */
+int mi_keyswitch(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                 uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
+                 void* stream);
+
+/* One key switch of a ragged batch (mixed levels and keys in one launch). */
+typedef struct {
+    const uint64_t* key_term;
+    const uint64_t* key;
+    uint64_t* b_out;
+    uint64_t* a_out;
+    int32_t level;
+    int32_t b_mode;
+    int32_t a_mode;
+    int32_t reserved;
+} mi_keyswitch_item;
+int mi_keyswitch_batch(mi_context* ctx, const mi_keyswitch_item* items, int count, int montgomery,
+                       void* stream);
+
+/* ckks_relin_hybrid_inplace (ckks_relin.cpp:8-44): (b, a) += KS_rlk(c2) */
+int mi_relin(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uint64_t* c2,
+             const uint64_t* key, int montgomery, void* stream);
+/* ckks_rotate_inplace (ckks_rotation.cpp:41-96): switch a, then permute b and a */
+int mi_rotate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t galois,
+              const uint64_t* key, int montgomery, void* stream);
+/* ckks_conjugate_inplace (ckks_conjugate.cpp:16-61): permute, then switch */
+int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uint64_t* key,
+                 int montgomery, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -165,20 +165,20 @@ class Ctx:
         return b, a
 
     def rotate(self, level, b, a, r, keys, montgomery=True):
-        b, a = u64a(b).copy(), u64a(a).copy()
-        if lib().oc_rotate(self.h, level, _p(b), _p(a), int(r), _p(u64a(keys)), int(montgomery)):
+        b, a, keys = u64a(b).copy(), u64a(a).copy(), u64a(keys)
+        if lib().oc_rotate(self.h, level, _p(b), _p(a), int(r), _p(keys), int(montgomery)):
             raise RuntimeError("oc_rotate")
         return b, a
 
     def conjugate(self, level, b, a, keys, montgomery=True):
-        b, a = u64a(b).copy(), u64a(a).copy()
-        if lib().oc_conjugate(self.h, level, _p(b), _p(a), _p(u64a(keys)), int(montgomery)):
+        b, a, keys = u64a(b).copy(), u64a(a).copy(), u64a(keys)
+        if lib().oc_conjugate(self.h, level, _p(b), _p(a), _p(keys), int(montgomery)):
             raise RuntimeError("oc_conjugate")
         return b, a
 
     def relin(self, level, b, a, c2, keys, montgomery=True):
-        b, a, c2 = u64a(b).copy(), u64a(a).copy(), u64a(c2).copy()
-        if lib().oc_relin(self.h, level, _p(b), _p(a), _p(c2), _p(u64a(keys)), int(montgomery)):
+        b, a, c2, keys = u64a(b).copy(), u64a(a).copy(), u64a(c2).copy(), u64a(keys)
+        if lib().oc_relin(self.h, level, _p(b), _p(a), _p(c2), _p(keys), int(montgomery)):
             raise RuntimeError("oc_relin")
         return b, a
 
@@ -191,9 +191,9 @@ class Ctx:
         return x[:, : level * self.N].copy()
 
     def tensor(self, level, b1, a1, b2, a2):
-        b1, a1 = u64a(b1).copy(), u64a(a1).copy()
+        b1, a1, b2, a2 = u64a(b1).copy(), u64a(a1).copy(), u64a(b2), u64a(a2)
         c2 = np.zeros_like(b1)
-        lib().oc_tensor(self.h, level, _p(b1), _p(a1), _p(u64a(b2)), _p(u64a(a2)), _p(c2))
+        lib().oc_tensor(self.h, level, _p(b1), _p(a1), _p(b2), _p(a2), _p(c2))
         return b1, a1, c2
 
     def square(self, level, b, a, montgomery=True):
@@ -203,8 +203,8 @@ class Ctx:
         return b, a, c2
 
     def mul_plain(self, level, polys, pt, pt_mont=False):
-        x = u64a(polys).copy()
-        lib().oc_mul_plain(self.h, level, _p(x), x.shape[0], _p(u64a(pt)), int(pt_mont))
+        x, pt = u64a(polys).copy(), u64a(pt)
+        lib().oc_mul_plain(self.h, level, _p(x), x.shape[0], _p(pt), int(pt_mont))
         return x
 
     def mul_int(self, level, polys, s, montgomery=True):
@@ -213,18 +213,18 @@ class Ctx:
         return x
 
     def add(self, level, x, y):
-        x = u64a(x).copy()
-        lib().oc_add(self.h, level, _p(x), _p(u64a(y)), x.shape[0])
+        x, y = u64a(x).copy(), u64a(y)
+        lib().oc_add(self.h, level, _p(x), _p(y), x.shape[0])
         return x
 
     def sub(self, level, x, y):
-        x = u64a(x).copy()
-        lib().oc_sub(self.h, level, _p(x), _p(u64a(y)), x.shape[0])
+        x, y = u64a(x).copy(), u64a(y)
+        lib().oc_sub(self.h, level, _p(x), _p(y), x.shape[0])
         return x
 
     def add_scalar_residues(self, level, b, res):
-        b = u64a(b).copy()
-        lib().oc_add_scalar_residues(self.h, level, _p(b), _p(u64a(res)))
+        b, res = u64a(b).copy(), u64a(res)
+        lib().oc_add_scalar_residues(self.h, level, _p(b), _p(res))
         return b
 
 
@@ -236,3 +236,11 @@ def automorphism_map(N, g):
     m = np.zeros(N, np.uint64)
     lib().oc_ntt_automorphism_map(N, int(g), _p(m))
     return m
+
+
+def apply_automorphism(data, N, limbs, galois):
+    """NTT-domain automorphism of `limbs` limbs (returns a new array)."""
+    out = u64a(data).copy()
+    m = automorphism_map(N, galois)
+    lib().oc_apply_automorphism(_p(out), N, limbs, _p(m))
+    return out

@@ -58,3 +58,20 @@ def crt(residues, moduli):
         qh = Q // m
         x += int(r) * qh * pow(qh, -1, m)
     return x % Q, Q
+
+
+ROOT = os.path.dirname(HERE)
+PKG_DIR = os.path.join(ROOT, "aes-walsh-ckks_amd")
+_product = None
+
+
+def product():
+    """The HIP product's Python binding (aes-walsh-ckks_amd/mickks.py)."""
+    global _product
+    if _product is None:
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("mickks", os.path.join(PKG_DIR, "mickks.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _product = mod
+    return _product
