@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
 // ---- step 3: forward NTT of the ModUp'd slots ----
 struct DigitJob {
     u64* digits;
-    int L, K, nq, alpha;
+    int L, K, nq, alpha, parts;
     u64 N;
     struct View {
         int basis;
@@ -94,14 +94,14 @@ struct DigitJob {
         __device__ u64 load(int i) const { return p[i]; }
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
+    // k walks slot-major (every digit's slot 0, then slot 1, ...) so that
+    // co-resident workgroups share one modulus's twiddles.
     __device__ View view(int k) const {
-        const int slots = L + K;
-        const int l = k % slots;
-        return View{l < L ? l : nq + (l - L), digits + (long long)k * N};
+        const int l = k / parts, d = k - l * parts;
+        return View{l < L ? l : nq + (l - L), digits + ((long long)d * (L + K) + l) * N};
     }
     __device__ bool skip(int k) const {
-        const int slots = L + K;
-        const int d = k / slots, l = k % slots;
+        const int l = k / parts, d = k - l * parts;
         const int s0 = d * alpha;
         return l >= s0 && l < s0 + alpha && l < L;
     }
@@ -190,11 +190,12 @@ struct MdInJob {
         }
         __device__ void store(int i, u64 v) const { out[i] = v; }
     };
+    // k = 2*i + poly: limb-major, a and b of one modulus side by side
     __device__ View view(int k) const {
-        const int poly = k / L, i = k - poly * L;
+        const int i = k >> 1, poly = k & 1;
         const BasisConst b = bc[i];
-        return View{i, accp + (long long)poly * K * N, tmp + (long long)k * N, pinv + (long long)i * K,
-                    bc, K, nq, N, b.q, b.mu_lo, b.mu_hi};
+        return View{i, accp + (long long)poly * K * N, tmp + ((long long)poly * L + i) * N,
+                    pinv + (long long)i * K, bc, K, nq, N, b.q, b.mu_lo, b.mu_hi};
     }
     __device__ bool skip(int) const { return false; }
 };
@@ -215,9 +216,9 @@ struct MdOutJob {
         __device__ void store(int i, u64 v) const { out[i] = sub_mod(out[i], v, q); }
     };
     __device__ View view(int k) const {
-        const int poly = k / L, i = k - poly * L;
+        const int i = k >> 1, poly = k & 1;
         u64* o = (poly == 0 ? a_out : b_out) + (long long)i * N;
-        return View{i, tmp + (long long)k * N, o, bc[i].q};
+        return View{i, tmp + ((long long)poly * L + i) * N, o, bc[i].q};
     }
     __device__ bool skip(int) const { return false; }
 };
@@ -260,7 +261,7 @@ void keyswitch(Context& c, const KsItem& it, bool montgomery, hipStream_t s) {
     }
     // 3. NTT of the ModUp'd slots
     {
-        DigitJob j{D, L, K, c.nq, c.alpha, N};
+        DigitJob j{D, L, K, c.nq, c.alpha, parts, N};
         run_ntt_fwd(c, j, j, parts * (L + K), s);
     }
     // 4. inner product with the key, combine Q slots

@@ -30,8 +30,9 @@ inline NttTabs tabs_of(const Context& c) {
     return NttTabs{c.d_fwd, c.d_fwd_shoup, c.d_inv, c.d_inv_shoup, c.d_bc, c.N};
 }
 
-// In-place NTT over a strided batch (optionally mapping limb slots >= qcount
-// to P bases and skipping a digit's source limbs).
+// In-place NTT over a strided batch.  Workgroups walk the batch modulus-major
+// (all polys' limb 0, then limb 1, ...): the workgroups resident on one XCD
+// at a time then share a single modulus's twiddle tables in that XCD's L2.
 struct PlainJob {
     LimbBatch b;
     u64 N;
@@ -42,8 +43,9 @@ struct PlainJob {
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
     __device__ View view(int k) const {
-        const int l = k % b.lpp;
-        return View{b.basis0 + l, b.base + (long long)(k / b.lpp) * b.stride + (long long)l * N};
+        const int npolys = b.count / b.lpp;
+        const int l = k / npolys, poly = k - l * npolys;
+        return View{b.basis0 + l, b.base + (long long)poly * b.stride + (long long)l * N};
     }
     __device__ bool skip(int) const { return false; }
 };

@@ -177,7 +177,7 @@ struct RescaleInJob {
     const u64* polys;
     long long stride;
     u64* scratch;     // [npolys][level][N]
-    int level;
+    int level, npolys;
     u64 N;
     u64 p, half;
     const BasisConst* bc;
@@ -194,7 +194,7 @@ struct RescaleInJob {
         __device__ void store(int i, u64 v) const { out[i] = v; }
     };
     __device__ View view(int k) const {
-        const int poly = k / level, i = k - poly * level;
+        const int i = k / npolys, poly = k - i * npolys;  // modulus-major
         const BasisConst b = bc[i];
         return View{i, polys + poly * stride + (long long)level * N,
                     scratch + ((long long)poly * level + i) * N, p, half, b.q, b.mu_hi};
@@ -207,7 +207,7 @@ struct RescaleOutJob {
     u64* polys;
     long long stride;
     const u64* scratch;
-    int level;
+    int level, npolys;
     u64 N;
     const u64* half_ntt;
     const u64* inv_p;
@@ -226,7 +226,7 @@ struct RescaleOutJob {
         }
     };
     __device__ View view(int k) const {
-        const int poly = k / level, i = k - poly * level;
+        const int i = k / npolys, poly = k - i * npolys;  // modulus-major
         return View{i, scratch + ((long long)poly * level + i) * N,
                     polys + poly * stride + (long long)i * N, half_ntt + (long long)i * N,
                     bc[i].q, inv_p[i], inv_p_shoup[i]};
@@ -244,8 +244,9 @@ void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hi
     u64* scratch = (u64*)c.workspace(s, sizeof(u64) * (size_t)npolys * level * N);
     const u64 p = c.q[level];
     const RescaleTables& rt = c.rescale[level];
-    RescaleInJob in{polys, stride, scratch, level, N, p, p >> 1, c.d_bc};
-    RescaleOutJob out{polys, stride, scratch, level, N, rt.half_ntt, rt.inv_p, rt.inv_p_shoup, c.d_bc};
+    RescaleInJob in{polys, stride, scratch, level, npolys, N, p, p >> 1, c.d_bc};
+    RescaleOutJob out{polys, stride, scratch, level, npolys, N, rt.half_ntt, rt.inv_p, rt.inv_p_shoup,
+                      c.d_bc};
     run_ntt_fwd(c, in, out, npolys * level, s);
 }
 

@@ -1,0 +1,236 @@
+"""Benchmark of the CKKS evaluation hot path on one MI355X.
+
+Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
+60-bit NTT primes (the reference prime search), 64 ciphertexts x 2 polys =
+5120 limbs, residues uniform in [0, q_i) from a seed.  One step = forward NTT
+of the whole batch in one launch pair followed by the inverse NTT.  The
+plain form (one limb per launch) is timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch]
+
+Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
+by torch.cuda.synchronize(); per-launch durations come from HIP events on the
+stream the library launches on.  The CPU baseline times the oracle (a C
+restatement of the reference's NTT, OpenMP over limbs) on a bounded sample.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    return oracle
+
+
+def _primes(p_bits, q_bits, N):
+    """Reference prime search (restated in the product's host layer is not
+    exposed; the oracle's restatement is pinned to the reference's lists)."""
+    return _oracle().find_primes(p_bits, q_bits, N)
+
+
+def _rand_limbs(torch, q, N, count_per_limb, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    L = len(q)
+    out = torch.empty((count_per_limb, L, N), dtype=torch.int64, device="cuda")
+    for i, qi in enumerate(q):
+        out[:, i, :] = torch.randint(0, int(qi), (count_per_limb, N), generator=g, device="cuda",
+                                     dtype=torch.int64)
+    return out.reshape(-1)
+
+
+def _profile_traffic(name):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get(name)
+    return None
+
+
+def bench_ntt(args, M, torch):
+    N = 1 << 16
+    L = 40
+    ncts, npolys = 64, 2
+    _, q = _primes([], [60] * L, N)
+    p, _ = _primes([61], [], N)
+    ctx = M.Context(N, q, p, 1)
+    polys = ncts * npolys
+    limbs = polys * L
+    data = _rand_limbs(torch, q, N, polys, seed=2026)
+    ref = data.clone()
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.ntt_forward(data, polys, L)
+        ctx.ntt_inverse(data, polys, L)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(stream)
+        ctx.ntt_forward(data, polys, L)
+        e1.record(stream)
+        ctx.ntt_inverse(data, polys, L)
+        e2.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fwd_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    inv_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    ok = bool(torch.equal(data, ref))
+    ms_per_step = wall * 1e3 / args.steps
+    value = 2 * limbs / (ms_per_step / 1e3)  # limb-NTTs per second (each direction counts)
+
+    # plain form: one limb per launch pair, on a 2-ciphertext sample
+    plain_rate = None
+    if args.plain:
+        plain_rate = _plain_ntt_rate(ctx, data, N, L, torch)
+
+
+    algo_bytes = 16 * N * limbs  # read + write every word once per direction
+    achieved = algo_bytes / (fwd_ms / 1e3)
+    traffic = _profile_traffic("ntt16_forward")
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "forward NTT (column + row pass) over 5120 limbs",
+                "forward_ms": fwd_ms, "inverse_ms": inv_ms,
+                "inverse_achieved_GBps": algo_bytes / (inv_ms / 1e3) / 1e9}
+
+    cpu = cpu_baseline_ntt(N, q) if args.cpu_baseline else None
+    return {
+        "metric": "batched negacyclic NTT throughput (forward+inverse), N=2^16, 40 x 60-bit limbs",
+        "value": value, "unit": "limb-NTT/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (exact mod-q integer)", "data": "synthetic (uniform residues, seed 2026)",
+        "config": {"workload": "batched NTT: N=2^16, 64 ct x 2 polys x 40 limbs (60-bit primes), "
+                               "forward + inverse, one launch pair per direction",
+                   "limbs": limbs, "round_trip_exact": ok,
+                   "plain_one_limb_per_launch_limb_ntt_per_s": plain_rate,
+                   "batched_over_plain": value / plain_rate if plain_rate else None},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+
+
+def _plain_ntt_rate(ctx, data, N, L, torch):
+    """One limb per launch pair (the plain form), on a 2-ciphertext sample."""
+    sample = 4 * L
+    sub = data[: sample * N]
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for k in range(sample):
+            ctx.ntt_forward(sub[k * N:(k + 1) * N], 1, 1, basis0=k % L)
+        for k in range(sample):
+            ctx.ntt_inverse(sub[k * N:(k + 1) * N], 1, 1, basis0=k % L)
+    torch.cuda.synchronize()
+    return 2 * sample * reps / (time.perf_counter() - t0)
+
+
+def cpu_baseline_ntt(N, q):
+    O = _oracle()
+    octx = O.Ctx(N, q[:40], [O.find_primes([61], [], N)[0][0]], 1)
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.integers(0, int(qi), N, dtype=np.uint64) for qi in q[:40]])
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        y = octx.ntt_fwd(x, 40)
+        octx.ntt_inv(y, 40)
+        done += 80
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "limb-NTT/s", "cores": int(O.lib().oc_omp_threads()),
+            "kind": "port",
+            "sample": "oracle forward+inverse NTT of one 40-limb N=2^16 poly, repeated for 10 s "
+                      "(OpenMP over limbs)"}
+
+
+def bench_keyswitch(args, M, torch):
+    """Batched hybrid key switching at N=2^16: Q = 30 x 50-bit, P = 10 x 50-bit,
+    dnum = 3 (alpha 10), 128 ciphertexts at levels drawn from 8..30 limbs,
+    16 rotation keys (random residues; the arithmetic does not depend on them)."""
+    N = 1 << 16
+    p, q = _primes([50] * 10, [50] * 30, N)
+    ctx = M.Context(N, q, p, 3)
+    rng = np.random.default_rng(3)
+    nkeys, nct = 16, 128
+    KL = 40
+    keys = [_rand_limbs(torch, list(q) + list(p), N, 2 * 3, seed=100 + k) for k in range(nkeys)]
+    levels = rng.integers(7, 30, nct)
+    items = []
+    for i, lv in enumerate(levels):
+        Lc = int(lv) + 1
+        kt = _rand_limbs(torch, q[:Lc], N, 1, seed=1000 + i)
+        b = _rand_limbs(torch, q[:Lc], N, 1, seed=2000 + i)
+        a = _rand_limbs(torch, q[:Lc], N, 1, seed=3000 + i)
+        items.append((int(lv), kt, keys[i % nkeys], b, a, 0, 1))
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ctx.keyswitch_batch(items)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.keyswitch_batch(items)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    Ls = levels + 1
+    algo = float(np.sum(8 * N * (4 * Ls))) + nkeys * 2 * 3 * KL * 8 * N
+    return {
+        "metric": "batched hybrid key switches/s, N=2^16, L in [8,30], K=10, dnum=3",
+        "value": nct / (ms / 1e3), "unit": "keyswitch/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded)",
+        "config": {"workload": "128 key switches, mixed levels, 16 keys", "algorithmic_GBps":
+                   algo / (ms / 1e3) / 1e9},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch"])
+    ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--no-plain", dest="plain", action="store_false",
+                    help="skip the one-limb-per-launch comparison (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(request sharding across GPUs is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    M = _load("mickks", os.path.join(ROOT, "aes-walsh-ckks_amd", "mickks.py"))
+    M.lib()
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py: no GPU visible")
+    out = bench_ntt(args, M, torch) if args.workload == "ntt" else bench_keyswitch(args, M, torch)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

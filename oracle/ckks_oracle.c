@@ -17,6 +17,7 @@
 #include "ckks_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -581,6 +582,8 @@ void oc_ctx_destroy(oc_ctx* c) {
 }
 
 int oc_ctx_alpha(const oc_ctx* c) { return c->alpha; }
+
+int oc_omp_threads(void) { return omp_get_max_threads(); }
 
 static const oc_tables* basis_table(const oc_ctx* c, int idx) {
     return idx < c->nq ? &c->tq[idx] : &c->tp[idx - c->nq];

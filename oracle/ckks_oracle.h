@@ -56,6 +56,7 @@ typedef struct oc_ctx oc_ctx;
 oc_ctx* oc_ctx_create(uint64_t N, const uint64_t* q, int nq, const uint64_t* p, int np, int dnum);
 void oc_ctx_destroy(oc_ctx* ctx);
 int oc_ctx_alpha(const oc_ctx* ctx);
+int oc_omp_threads(void);
 
 /* NTT over the first `limbs` chain moduli (flat RNS poly). */
 void oc_ntt_fwd_q(const oc_ctx* ctx, uint64_t* data, int limbs, int lazy);

@@ -49,6 +49,7 @@ def lib():
             "oc_ctx_create": (vp, [u64, vp, cint, vp, cint, cint]),
             "oc_ctx_destroy": (None, [vp]),
             "oc_ctx_alpha": (cint, [vp]),
+            "oc_omp_threads": (cint, []),
             "oc_ntt_fwd_q": (None, [vp, vp, cint, cint]),
             "oc_ntt_inv_q": (None, [vp, vp, cint, cint]),
             "oc_ntt_fwd_basis": (None, [vp, vp, vp, cint, cint]),
