@@ -1038,3 +1038,372 @@ int oc_add_scalar_residues(const oc_ctx* c, int level, u64* b, const u64* res) {
         }
     return 0;
 }
+
+/* ===================================================================== */
+/* Hoisting and QP-domain key switching: ckks_keyswitch.cpp:95-227,       */
+/* 344-373, 675-1328                                                      */
+/* ===================================================================== */
+
+/* basis index of QP slot l at level L (Q limbs, then P limbs) */
+static int qp_basis(const oc_ctx* c, int L, int l) { return l < L ? l : c->nq + (l - L); }
+
+/* ckks_hoist_ciphertext_a_qp_eval (ckks_keyswitch.cpp:344-373): for every
+ * digit, ModUp of a's digit limbs to the other Q limbs and P (coefficient
+ * domain, then NTT); the digit's own limbs are a's evaluation-domain limbs.
+ * digits: [parts][L+K][N], canonical. */
+int oc_hoist(const oc_ctx* c, int level, const u64* a_eval, u64* digits) {
+    if (level < 0 || level >= c->nq) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K, parts = c->nparts[level], alpha = c->alpha;
+    u64* coeff = (u64*)malloc(8 * N * L);
+    memcpy(coeff, a_eval, 8 * N * L);
+    oc_ntt_inv_q(c, coeff, L, 1);
+    for (int part = 0; part < parts; ++part) {
+        const int s0 = part * alpha, s1 = s0 + alpha < L ? s0 + alpha : L;
+        u64* D = digits + (u64)part * T * N;
+        const u64* sp[64];
+        u64* dp[256];
+        int nd = 0;
+        for (int i = 0; i < s1 - s0; ++i) sp[i] = coeff + (u64)(s0 + i) * N;
+        for (int i = 0; i < s0; ++i) dp[nd++] = D + (u64)i * N;
+        for (int i = s1; i < L; ++i) dp[nd++] = D + (u64)i * N;
+        for (int i = 0; i < K; ++i) dp[nd++] = D + (u64)(L + i) * N;
+        hps_modup(&c->modup[level][part], sp, dp, N);
+        for (int i = s0; i < s1; ++i) memcpy(D + (u64)i * N, a_eval + (u64)i * N, 8 * N);
+#pragma omp parallel for schedule(static)
+        for (int l = 0; l < T; ++l) {
+            if (l >= s0 && l < s1) continue;
+            const oc_tables* t = basis_table(c, qp_basis(c, L, l));
+            oc_ntt_forward(D + (u64)l * N, N, t->q, t->fwd, t->fwd_shoup, 0);
+        }
+    }
+    free(coeff);
+    return 0;
+}
+
+/* sum_d digits[d] * key[d] over the L+K QP slots into acc_b, acc_a (added). */
+static void qp_key_mac(const oc_ctx* c, int level, const u64* digits, const u64* keys, u64* acc_b,
+                       u64* acc_a, int montgomery) {
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K, KL = c->nq + K, parts = c->nparts[level];
+    for (int part = 0; part < parts; ++part) {
+        const u64* kb = keys + (u64)part * 2 * KL * N;
+        const u64* ka = kb + (u64)KL * N;
+        const u64* D = digits + (u64)part * T * N;
+#pragma omp parallel for schedule(static)
+        for (int l = 0; l < T; ++l) {
+            const int basis = qp_basis(c, L, l);
+            const oc_tables* t = basis_table(c, basis);
+            mac_limb(D + (u64)l * N, ka + (u64)basis * N, acc_a + (u64)l * N, N, t, montgomery);
+            mac_limb(D + (u64)l * N, kb + (u64)basis * N, acc_b + (u64)l * N, N, t, montgomery);
+        }
+    }
+}
+
+/* ckks_hybrid_key_switch_hoisted_inplace (ckks_keyswitch.cpp:1072-1328):
+ * MAC over the hoisted digits, INTT of the P limbs, folded ModDown pair,
+ * combine into (b_out, a_out), then the automorphism (galois != 0). */
+int oc_keyswitch_hoisted(const oc_ctx* c, int level, const u64* digits, const u64* keys, u64* b_out,
+                         u64* a_out, int b_mode, int a_mode, u64 galois, int montgomery) {
+    if (level < 0 || level >= c->nq) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K;
+    u64* acc_b = (u64*)calloc(N * T, 8);
+    u64* acc_a = (u64*)calloc(N * T, 8);
+    qp_key_mac(c, level, digits, keys, acc_b, acc_a, montgomery);
+    for (int pi = 0; pi < K; ++pi) {
+        const oc_tables* t = &c->tp[pi];
+        oc_ntt_inverse(acc_a + (u64)(L + pi) * N, N, t->q, t->inv, t->inv_shoup, t->n_inv, 1);
+        oc_ntt_inverse(acc_b + (u64)(L + pi) * N, N, t->q, t->inv, t->inv_shoup, t->n_inv, 1);
+    }
+    folded_moddown_pair(c, L, acc_a + (u64)L * N, acc_a, acc_b + (u64)L * N, acc_b);
+    for (int i = 0; i < L; ++i)
+        for (u64 x = 0; x < N; ++x) {
+            const u64 o = (u64)i * N + x;
+            b_out[o] = b_mode == 0 ? add_q(b_out[o], acc_b[o], c->q[i]) : acc_b[o];
+            a_out[o] = a_mode == 0 ? add_q(a_out[o], acc_a[o], c->q[i]) : acc_a[o];
+        }
+    if (galois) {
+        u64* map = (u64*)malloc(8 * N);
+        oc_ntt_automorphism_map(N, galois, map);
+        oc_apply_automorphism(b_out, N, L, map);
+        oc_apply_automorphism(a_out, N, L, map);
+        free(map);
+    }
+    free(acc_b);
+    free(acc_a);
+    return 0;
+}
+
+/* ckks_hybrid_key_switch_hoisted_qp_rotated (ckks_keyswitch.cpp:675-852):
+ * b_qp = [b_q ; 0] + sum_d digit*key_b, a_qp = sum_d digit*key_a, both in the
+ * QP basis, then the automorphism over all L+K limbs (no ModDown). */
+int oc_keyswitch_hoisted_qp_rotated(const oc_ctx* c, int level, const u64* digits, const u64* keys,
+                                    const u64* b_q_eval, u64* b_qp, u64* a_qp, u64 galois,
+                                    int montgomery) {
+    if (level < 0 || level >= c->nq) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K;
+    memset(b_qp, 0, 8 * N * T);
+    memset(a_qp, 0, 8 * N * T);
+    memcpy(b_qp, b_q_eval, 8 * N * L);
+    qp_key_mac(c, level, digits, keys, b_qp, a_qp, montgomery);
+    u64* map = (u64*)malloc(8 * N);
+    oc_ntt_automorphism_map(N, galois, map);
+    oc_apply_automorphism(b_qp, N, T, map);
+    oc_apply_automorphism(a_qp, N, T, map);
+    free(map);
+    return 0;
+}
+
+/* ckks_hybrid_key_switch_qp_from_key_term (ckks_keyswitch.cpp:854-1070):
+ * the key switch up to the inner product, result kept in the QP basis. */
+int oc_keyswitch_qp_from_key_term(const oc_ctx* c, int level, const u64* key_term_eval,
+                                  const u64* keys, u64* b_qp, u64* a_qp, int montgomery) {
+    if (level < 0 || level >= c->nq) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K, parts = c->nparts[level];
+    u64* digits = (u64*)malloc(8 * N * T * parts);
+    oc_hoist(c, level, key_term_eval, digits);
+    memset(b_qp, 0, 8 * N * T);
+    memset(a_qp, 0, 8 * N * T);
+    qp_key_mac(c, level, digits, keys, b_qp, a_qp, montgomery);
+    free(digits);
+    return 0;
+}
+
+/* ckks_apply_folded_approx_crt_moddown_eval_q (ckks_keyswitch.cpp:609-673)
+ * as used by moddown_qp_poly_to_q_eval (ckks_linear_transform.cpp:581-605):
+ * qp: L+K eval limbs in, the first L limbs hold the Q result. */
+int oc_moddown_qp_to_q(const oc_ctx* c, int level, u64* qp) {
+    if (level < 0 || level >= c->nq) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np;
+    for (int pi = 0; pi < K; ++pi) {
+        const oc_tables* t = &c->tp[pi];
+        oc_ntt_inverse(qp + (u64)(L + pi) * N, N, t->q, t->inv, t->inv_shoup, t->n_inv, 1);
+    }
+    /* the single-poly version is the pair version applied to one poly */
+    u64* dummy_src = (u64*)calloc(N * K, 8);
+    u64* dummy_dst = (u64*)calloc(N * L, 8);
+    folded_moddown_pair(c, L, qp + (u64)L * N, qp, dummy_src, dummy_dst);
+    free(dummy_src);
+    free(dummy_dst);
+    return 0;
+}
+
+/* ===================================================================== */
+/* BSGS linear transform, QP evaluator: ckks_linear_transform.cpp:1174-1407 */
+/* ===================================================================== */
+
+static const u64* find_key(const int64_t* amounts, const u64* const* keys, int nkeys, int64_t r) {
+    for (int i = 0; i < nkeys; ++i)
+        if (amounts[i] == r) return keys[i];
+    return NULL;
+}
+
+/* diags: [nterms][L+K][N] QP plaintexts (Montgomery form if pt_mont);
+ * diag_idx: diagonal index of each term; keys by rotation amount.
+ * Output (a_out, b_out): L limbs each, before the optional rescale. */
+int oc_lt_stage_qp(const oc_ctx* c, int level, const u64* a, const u64* b, int nterms,
+                   const int64_t* diag_idx, const u64* diags, int pt_mont, int baby_steps,
+                   int slots, const int64_t* rot_amounts, const u64* const* rot_keys, int nkeys,
+                   int montgomery, u64* a_out, u64* b_out) {
+    if (level < 0 || level >= c->nq || nterms < 1 || baby_steps < 1) return -1;
+    const u64 N = c->N;
+    const int L = level + 1, K = c->np, T = L + K, parts = c->nparts[level];
+    const u64 W = (u64)T * N;
+    const int giants = (slots + baby_steps - 1) / baby_steps;
+    /* compile_linear_transform_stage_metadata (:818-859) */
+    char* baby_needed = (char*)calloc(baby_steps, 1);
+    for (int t = 0; t < nterms; ++t) baby_needed[diag_idx[t] % baby_steps] = 1;
+    u64* baby_a = (u64*)calloc(W * baby_steps, 8);
+    u64* baby_b = (u64*)calloc(W * baby_steps, 8);
+    if (baby_needed[0]) { /* make_direct_input_qp (:487-532), folded keys */
+        memcpy(baby_a, a, 8 * N * L);
+        memcpy(baby_b, b, 8 * N * L);
+    }
+    u64* digits = (u64*)malloc(8 * W * parts);
+    oc_hoist(c, level, a, digits);
+    for (int r = 1; r < baby_steps; ++r) {
+        if (!baby_needed[r]) continue;
+        const u64* key = find_key(rot_amounts, rot_keys, nkeys, r);
+        if (!key) { free(baby_needed); free(baby_a); free(baby_b); free(digits); return -2; }
+        oc_keyswitch_hoisted_qp_rotated(c, level, digits, key, b, baby_b + W * r, baby_a + W * r,
+                                        oc_rotation_galois_element(N, r), montgomery);
+    }
+    u64* out_a = (u64*)calloc(W, 8);
+    u64* out_b = (u64*)calloc(W, 8);
+    u64* in_a = (u64*)malloc(8 * W);
+    u64* in_b = (u64*)malloc(8 * W);
+    u64* sw_a = (u64*)malloc(8 * W);
+    u64* sw_b = (u64*)malloc(8 * W);
+    int have_out = 0;
+    for (int g = 0; g < giants; ++g) {
+        int have_in = 0;
+        for (int t = 0; t < nterms; ++t) {
+            if (diag_idx[t] / baby_steps != g) continue;
+            const int bb = (int)(diag_idx[t] - (int64_t)g * baby_steps);
+            const u64* pt = diags + (u64)t * W;
+            for (int l = 0; l < T; ++l) {
+                const oc_tables* tb = basis_table(c, qp_basis(c, L, l));
+                const u64 q = tb->q;
+                for (u64 x = 0; x < N; ++x) {
+                    const u64 o = (u64)l * N + x;
+                    u64 ta, tb_;
+                    if (pt_mont) {
+                        ta = mont_mul_c(baby_a[W * bb + o], pt[o], q, tb->mont_ninv);
+                        tb_ = mont_mul_c(baby_b[W * bb + o], pt[o], q, tb->mont_ninv);
+                    } else {
+                        ta = barrett_mul_c(baby_a[W * bb + o], pt[o], q, tb->mu_lo, tb->mu_hi);
+                        tb_ = barrett_mul_c(baby_b[W * bb + o], pt[o], q, tb->mu_lo, tb->mu_hi);
+                    }
+                    in_a[o] = have_in ? add_q(in_a[o], ta, q) : ta;
+                    in_b[o] = have_in ? add_q(in_b[o], tb_, q) : tb_;
+                }
+            }
+            have_in = 1;
+        }
+        if (!have_in) continue;
+        if (g == 0) {
+            for (u64 o = 0; o < W; ++o) {
+                const u64 q = basis_table(c, qp_basis(c, L, (int)(o / N)))->q;
+                out_a[o] = have_out ? add_q(out_a[o], in_a[o], q) : in_a[o];
+                out_b[o] = have_out ? add_q(out_b[o], in_b[o], q) : in_b[o];
+            }
+            have_out = 1;
+            continue;
+        }
+        const int64_t gr = (int64_t)g * baby_steps;
+        const u64* key = find_key(rot_amounts, rot_keys, nkeys, gr);
+        if (!key) { have_out = -1; break; }
+        oc_moddown_qp_to_q(c, level, in_a); /* key term = ModDown(inner_a) */
+        oc_keyswitch_qp_from_key_term(c, level, in_a, key, sw_b, sw_a, montgomery);
+        /* automorphism_qp_pair_accumulate (:373-416) */
+        u64* map = (u64*)malloc(8 * N);
+        oc_ntt_automorphism_map(N, oc_rotation_galois_element(N, gr), map);
+        for (int l = 0; l < T; ++l) {
+            const u64 q = basis_table(c, qp_basis(c, L, l))->q;
+            for (u64 x = 0; x < N; ++x) {
+                const u64 s = (u64)l * N + map[x], o = (u64)l * N + x;
+                const u64 add_a = sw_a[s];
+                const u64 add_b = add_q(sw_b[s], in_b[s], q);
+                out_a[o] = have_out ? add_q(out_a[o], add_a, q) : add_a;
+                out_b[o] = have_out ? add_q(out_b[o], add_b, q) : add_b;
+            }
+        }
+        free(map);
+        have_out = 1;
+    }
+    int rc = 0;
+    if (have_out != 1) {
+        rc = -3;
+    } else {
+        /* moddown_qp_pair_to_q_eval (:641-718), folded */
+        for (int pi = 0; pi < K; ++pi) {
+            const oc_tables* t = &c->tp[pi];
+            oc_ntt_inverse(out_a + (u64)(L + pi) * N, N, t->q, t->inv, t->inv_shoup, t->n_inv, 1);
+            oc_ntt_inverse(out_b + (u64)(L + pi) * N, N, t->q, t->inv, t->inv_shoup, t->n_inv, 1);
+        }
+        folded_moddown_pair(c, L, out_a + (u64)L * N, out_a, out_b + (u64)L * N, out_b);
+        memcpy(a_out, out_a, 8 * N * L);
+        memcpy(b_out, out_b, 8 * N * L);
+    }
+    free(baby_needed); free(baby_a); free(baby_b); free(digits);
+    free(out_a); free(out_b); free(in_a); free(in_b); free(sw_a); free(sw_b);
+    return rc;
+}
+
+/* ===================================================================== */
+/* ModRaise (ckks_modraise.cpp:31-75) and the q0p0 switch (1645-1816)     */
+/* ===================================================================== */
+
+/* poly_q0_eval: one q0 limb (evaluation domain); out: target+1 limbs. */
+int oc_modraise(const oc_ctx* c, int target_level, const u64* poly_q0_eval, u64* out) {
+    if (target_level < 1 || target_level >= c->nq) return -1;
+    const u64 N = c->N;
+    const u64 q0 = c->q[0];
+    u64* coeff = (u64*)malloc(8 * N);
+    memcpy(coeff, poly_q0_eval, 8 * N);
+    oc_ntt_inverse(coeff, N, q0, c->tq[0].inv, c->tq[0].inv_shoup, c->tq[0].n_inv, 0);
+    for (u64 x = 0; x < N; ++x) {
+        const u64 v = coeff[x];
+        const int64_t centred = v > (q0 >> 1) ? -(int64_t)(q0 - v) : (int64_t)v;
+        for (int l = 0; l <= target_level; ++l) {
+            const u64 q = c->q[l];
+            u64 r;
+            if (centred >= 0) r = (u64)centred % q;
+            else {
+                const u64 m = (u64)(-centred) % q;
+                r = m == 0 ? 0 : q - m;
+            }
+            out[(u64)l * N + x] = r;
+        }
+    }
+    oc_ntt_fwd_q(c, out, target_level + 1, 0);
+    free(coeff);
+    return 0;
+}
+
+/* hps_base_change_single_limb (ckks_keyswitch.cpp:1647-1659) */
+static u64 single_limb_change(u64 v, u64 src, u64 dst, u64 dst_mu_hi) {
+    const u64 r = reduce_4q(v, src);
+    const u64 r_dst = barrett_u64_c(r, dst, dst_mu_hi);
+    const u64 src_dst = barrett_u64_c(src, dst, dst_mu_hi);
+    return r >= (src >> 1) + 1 ? sub_q(r_dst, src_dst, dst) : r_dst;
+}
+
+/* ckks_q0p0_key_switch_level0_inplace (ckks_keyswitch.cpp:1661-1816),
+ * folded keys.  key: [b(q0, p0), a(q0, p0)] = 4N words. */
+int oc_q0p0_keyswitch(const oc_ctx* c, u64* b, u64* a, const u64* key, int montgomery) {
+    const u64 N = c->N;
+    const oc_tables* tq0 = &c->tq[0];
+    const oc_tables* tp0 = &c->tp[0];
+    const u64 q0 = tq0->q, p0 = tp0->q;
+    const u64* kb = key;
+    const u64* ka = key + 2 * N;
+    u64* dcoeff = (u64*)malloc(8 * N);
+    u64* dp = (u64*)malloc(8 * N);
+    u64* aq_a = (u64*)calloc(N, 8);
+    u64* aq_b = (u64*)calloc(N, 8);
+    u64* ap_a = (u64*)calloc(N, 8);
+    u64* ap_b = (u64*)calloc(N, 8);
+    memcpy(dcoeff, a, 8 * N);
+    oc_ntt_inverse(dcoeff, N, q0, tq0->inv, tq0->inv_shoup, tq0->n_inv, 1);
+    for (u64 x = 0; x < N; ++x) dp[x] = single_limb_change(dcoeff[x], q0, p0, tp0->mu_hi);
+    oc_ntt_forward(dp, N, p0, tp0->fwd, tp0->fwd_shoup, 1);
+    mac_limb(a, ka, aq_a, N, tq0, montgomery);
+    mac_limb(a, kb, aq_b, N, tq0, montgomery);
+    mac_limb(dp, ka + N, ap_a, N, tp0, montgomery);
+    mac_limb(dp, kb + N, ap_b, N, tp0, montgomery);
+    oc_ntt_inverse(ap_a, N, p0, tp0->inv, tp0->inv_shoup, tp0->n_inv, 1);
+    oc_ntt_inverse(ap_b, N, p0, tp0->inv, tp0->inv_shoup, tp0->n_inv, 1);
+    /* folded ModDown over the {q0} / {p0} bases with p0^-1 mod q0 */
+    {
+        const u64 inv = oc_mod_inverse(p0 % q0, q0);
+        const u64 inv_s = oc_compute_shoup(inv, q0);
+        const double pinv = 1.0 / (double)p0;
+        for (u64 x = 0; x < N; ++x) {
+            const u64 va = reduce_4q(ap_a[x], p0), vb = reduce_4q(ap_b[x], p0);
+            u64 al_a = (u64)((double)va * pinv + 0.5), al_b = (u64)((double)vb * pinv + 0.5);
+            if (al_a > 1) al_a = 1;
+            if (al_b > 1) al_b = 1;
+            const u64 sa = shoup_mul_c(barrett_u64_c(va, q0, tq0->mu_hi), inv, inv_s, q0);
+            const u64 sb = shoup_mul_c(barrett_u64_c(vb, q0, tq0->mu_hi), inv, inv_s, q0);
+            ap_a[x] = sub_q(sa, al_a, q0);
+            ap_b[x] = sub_q(sb, al_b, q0);
+        }
+        oc_ntt_forward(ap_a, N, q0, tq0->fwd, tq0->fwd_shoup, 0);
+        oc_ntt_forward(ap_b, N, q0, tq0->fwd, tq0->fwd_shoup, 0);
+        for (u64 x = 0; x < N; ++x) {
+            aq_a[x] = sub_q(aq_a[x], ap_a[x], q0);
+            aq_b[x] = sub_q(aq_b[x], ap_b[x], q0);
+        }
+    }
+    for (u64 x = 0; x < N; ++x) {
+        b[x] = add_q(b[x], aq_b[x], q0);
+        a[x] = aq_a[x];
+    }
+    free(dcoeff); free(dp); free(aq_a); free(aq_b); free(ap_a); free(ap_b);
+    return 0;
+}

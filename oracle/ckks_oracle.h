@@ -104,6 +104,29 @@ int oc_sub(const oc_ctx* ctx, int level, uint64_t* x, const uint64_t* y, int npo
 int oc_add_plain(const oc_ctx* ctx, int level, uint64_t* b, const uint64_t* pt, int pt_montgomery);
 int oc_add_scalar_residues(const oc_ctx* ctx, int level, uint64_t* b, const uint64_t* residues);
 
+/* Hoisting and QP-basis key switching (ckks_keyswitch.cpp:95-227, 344-373,
+ * 675-1328).  digits: [parts][L+K][N] evaluation domain, canonical. */
+int oc_hoist(const oc_ctx* ctx, int level, const uint64_t* a_eval, uint64_t* digits);
+int oc_keyswitch_hoisted(const oc_ctx* ctx, int level, const uint64_t* digits, const uint64_t* keys,
+                         uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
+                         int montgomery);
+int oc_keyswitch_hoisted_qp_rotated(const oc_ctx* ctx, int level, const uint64_t* digits,
+                                    const uint64_t* keys, const uint64_t* b_q_eval, uint64_t* b_qp,
+                                    uint64_t* a_qp, uint64_t galois, int montgomery);
+int oc_keyswitch_qp_from_key_term(const oc_ctx* ctx, int level, const uint64_t* key_term_eval,
+                                  const uint64_t* keys, uint64_t* b_qp, uint64_t* a_qp, int montgomery);
+int oc_moddown_qp_to_q(const oc_ctx* ctx, int level, uint64_t* qp);
+/* BSGS linear-transform stage on the QP evaluator (ckks_linear_transform.cpp:1174-1407),
+ * without the final rescale. */
+int oc_lt_stage_qp(const oc_ctx* ctx, int level, const uint64_t* a, const uint64_t* b, int nterms,
+                   const int64_t* diag_idx, const uint64_t* diags, int pt_mont, int baby_steps,
+                   int slots, const int64_t* rot_amounts, const uint64_t* const* rot_keys, int nkeys,
+                   int montgomery, uint64_t* a_out, uint64_t* b_out);
+/* ModRaise of one q0 limb to target_level+1 limbs (ckks_modraise.cpp:31-75). */
+int oc_modraise(const oc_ctx* ctx, int target_level, const uint64_t* poly_q0_eval, uint64_t* out);
+/* Dense-to-sparse switch at level 0 over {q0, p0} (ckks_keyswitch.cpp:1661-1816). */
+int oc_q0p0_keyswitch(const oc_ctx* ctx, uint64_t* b, uint64_t* a, const uint64_t* key, int montgomery);
+
 #ifdef __cplusplus
 }
 #endif

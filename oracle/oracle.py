@@ -71,6 +71,15 @@ def lib():
             "oc_sub": (cint, [vp, cint, vp, vp, cint]),
             "oc_add_plain": (cint, [vp, cint, vp, vp, cint]),
             "oc_add_scalar_residues": (cint, [vp, cint, vp, vp]),
+            "oc_hoist": (cint, [vp, cint, vp, vp]),
+            "oc_keyswitch_hoisted": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, u64, cint]),
+            "oc_keyswitch_hoisted_qp_rotated": (cint, [vp, cint, vp, vp, vp, vp, vp, u64, cint]),
+            "oc_keyswitch_qp_from_key_term": (cint, [vp, cint, vp, vp, vp, vp, cint]),
+            "oc_moddown_qp_to_q": (cint, [vp, cint, vp]),
+            "oc_lt_stage_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, vp, vp, cint,
+                                      cint, vp, vp]),
+            "oc_modraise": (cint, [vp, cint, vp, vp]),
+            "oc_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -227,6 +236,89 @@ class Ctx:
         b, res = u64a(b).copy(), u64a(res)
         lib().oc_add_scalar_residues(self.h, level, _p(b), _p(res))
         return b
+
+
+def _ctx_extra():
+    """Methods for hoisting, QP key switching, BSGS, ModRaise and q0p0."""
+
+    def qp_limbs(self, level):
+        return level + 1 + len(self.p)
+
+    def hoist(self, level, a_eval):
+        a = u64a(a_eval)
+        out = np.zeros(self.parts(level) * self.qp_limbs(level) * self.N, np.uint64)
+        if lib().oc_hoist(self.h, level, _p(a), _p(out)):
+            raise RuntimeError("oc_hoist")
+        return out
+
+    def keyswitch_hoisted(self, level, digits, keys, b, a, b_mode, a_mode, galois=0, montgomery=True):
+        digits, keys, b, a = u64a(digits), u64a(keys), u64a(b).copy(), u64a(a).copy()
+        if lib().oc_keyswitch_hoisted(self.h, level, _p(digits), _p(keys), _p(b), _p(a), b_mode, a_mode,
+                                      int(galois), int(montgomery)):
+            raise RuntimeError("oc_keyswitch_hoisted")
+        return b, a
+
+    def keyswitch_hoisted_qp_rotated(self, level, digits, keys, b_q, galois, montgomery=True):
+        digits, keys, b_q = u64a(digits), u64a(keys), u64a(b_q)
+        W = self.qp_limbs(level) * self.N
+        bqp, aqp = np.zeros(W, np.uint64), np.zeros(W, np.uint64)
+        if lib().oc_keyswitch_hoisted_qp_rotated(self.h, level, _p(digits), _p(keys), _p(b_q), _p(bqp),
+                                                 _p(aqp), int(galois), int(montgomery)):
+            raise RuntimeError("oc_keyswitch_hoisted_qp_rotated")
+        return bqp, aqp
+
+    def keyswitch_qp_from_key_term(self, level, key_term, keys, montgomery=True):
+        kt, keys = u64a(key_term), u64a(keys)
+        W = self.qp_limbs(level) * self.N
+        bqp, aqp = np.zeros(W, np.uint64), np.zeros(W, np.uint64)
+        if lib().oc_keyswitch_qp_from_key_term(self.h, level, _p(kt), _p(keys), _p(bqp), _p(aqp),
+                                               int(montgomery)):
+            raise RuntimeError("oc_keyswitch_qp_from_key_term")
+        return bqp, aqp
+
+    def moddown_qp_to_q(self, level, qp):
+        x = u64a(qp).copy()
+        if lib().oc_moddown_qp_to_q(self.h, level, _p(x)):
+            raise RuntimeError("oc_moddown_qp_to_q")
+        return x[: (level + 1) * self.N].copy()
+
+    def lt_stage_qp(self, level, a, b, diag_idx, diags, baby_steps, rot_keys, pt_mont=True,
+                    slots=None, montgomery=True):
+        """rot_keys: {rotation amount: key array}.  Returns (a_out, b_out)."""
+        a, b, diags = u64a(a), u64a(b), u64a(diags)
+        idx = np.ascontiguousarray(np.asarray(diag_idx, np.int64))
+        amounts = np.ascontiguousarray(np.asarray(sorted(rot_keys), np.int64))
+        karr = [u64a(rot_keys[int(r)]) for r in amounts]
+        kptr = (ctypes.c_void_p * max(len(karr), 1))(*[k.ctypes.data for k in karr])
+        L = level + 1
+        ao, bo = np.zeros(L * self.N, np.uint64), np.zeros(L * self.N, np.uint64)
+        slots = self.N // 2 if slots is None else slots
+        rc = lib().oc_lt_stage_qp(self.h, level, _p(a), _p(b), len(idx), idx.ctypes.data, _p(diags),
+                                  int(pt_mont), int(baby_steps), int(slots), amounts.ctypes.data,
+                                  ctypes.addressof(kptr), len(karr), int(montgomery), _p(ao), _p(bo))
+        if rc:
+            raise RuntimeError("oc_lt_stage_qp (%d)" % rc)
+        return ao, bo
+
+    def modraise(self, target_level, poly_q0):
+        x = u64a(poly_q0)
+        out = np.zeros((target_level + 1) * self.N, np.uint64)
+        if lib().oc_modraise(self.h, target_level, _p(x), _p(out)):
+            raise RuntimeError("oc_modraise")
+        return out
+
+    def q0p0_keyswitch(self, b, a, key, montgomery=True):
+        b, a, key = u64a(b).copy(), u64a(a).copy(), u64a(key)
+        if lib().oc_q0p0_keyswitch(self.h, _p(b), _p(a), _p(key), int(montgomery)):
+            raise RuntimeError("oc_q0p0_keyswitch")
+        return b, a
+
+    for f in (qp_limbs, hoist, keyswitch_hoisted, keyswitch_hoisted_qp_rotated, keyswitch_qp_from_key_term,
+              moddown_qp_to_q, lt_stage_qp, modraise, q0p0_keyswitch):
+        setattr(Ctx, f.__name__, f)
+
+
+_ctx_extra()
 
 
 def galois_rotation(N, r):

@@ -26,6 +26,8 @@
 #include "ckks_relin.hpp"
 #include "ckks_rescale.hpp"
 #include "ckks_rotation.hpp"
+#include "ckks_linear_transform.hpp"
+#include "ckks_modraise.hpp"
 #include "ntt.hpp"
 #include "random.hpp"
 
@@ -239,6 +241,156 @@ static void run_eval(size_t logN, size_t depth, size_t dnum, std::vector<uint64_
     }
 }
 
+// splitmix64 residues of one QP plaintext of a linear-transform stage:
+// limb l of term t is drawn from seed 5000 + 64 t + l.
+static MyVector<uint64_t> lt_diag_qp(size_t N, const MyVector<uint64_t>& qp, size_t term) {
+    MyVector<uint64_t> out(N * qp.size());
+    for (size_t l = 0; l < qp.size(); ++l) {
+        uint64_t seed = 5000 + 64 * term + l;
+        for (size_t i = 0; i < N; ++i) out[l * N + i] = splitmix64(seed) % qp[l];
+    }
+    return out;
+}
+
+// Hoisted rotations and a QP-evaluator BSGS stage on a small context.
+static void run_hoist_lt() {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << 10;
+    c.mult_depth = 3;
+    c.q0_bit = 50;
+    c.q_bit = 40;
+    c.p_bits = {60, 60};
+    c.dnum = 2;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    c.rotate_values = {1, 2, 4};
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN(), L = P.getMaxLevel();
+    const std::string tag = "lt10";
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    dump_key(tag + "_rot1", ctx.getRotationKey(1));
+    dump_key(tag + "_rot2", ctx.getRotationKey(2));
+    dump_key(tag + "_rot4", ctx.getRotationKey(4));
+    uint64_t seed = 91;
+    MyVector<Complex> m(N / 2);
+    for (auto& z : m) {
+        double re = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1;
+        z = Complex(re, -re / 2);
+    }
+    auto ct = ckks_encrypt(ctx, m, L);
+    dump_ct(tag + "_ct", ct);
+
+    // hoisted key switches of ct's a (rotation by 1, Q result and QP result)
+    CKKSHoistWorkspace ws;
+    CKKSEvalHoistedCiphertextQP h;
+    ckks_hoist_ciphertext_a_qp_eval(ctx, ct, ws, h);
+    {
+        std::vector<uint64_t> flat;
+        for (const auto& d : h.a_partition_qp_eval) flat.insert(flat.end(), d.begin(), d.end());
+        dumpv(tag + "_hoist_digits_lazy", flat);
+        MyVector<uint64_t> b = ct.getB();
+        MyVector<uint64_t> a(N * (L + 1), 0);
+        ckks_hybrid_key_switch_hoisted_inplace(ctx, h, ctx.getRotationKey(1), b, a,
+                                               CKKSHybridKeySwitchCombine::Add,
+                                               CKKSHybridKeySwitchCombine::Assign, P.montgomery,
+                                               ctx.getRotationNttMap(1));
+        std::vector<uint64_t> o(b.begin(), b.end());
+        o.insert(o.end(), a.begin(), a.end());
+        dumpv(tag + "_hoisted_rot1", o);
+        MyVector<uint64_t> bqp, aqp;
+        ckks_hybrid_key_switch_hoisted_qp_rotated(ctx, h, ctx.getRotationKey(2), ct.getB(), bqp, aqp,
+                                                  P.montgomery, ctx.getRotationNttMap(2));
+        std::vector<uint64_t> o2(bqp.begin(), bqp.end());
+        o2.insert(o2.end(), aqp.begin(), aqp.end());
+        dumpv(tag + "_hoisted_qp_rot2", o2);
+        MyVector<uint64_t> sb, sa;
+        ckks_hybrid_key_switch_qp_from_key_term(ctx, L, ct.getA(), ctx.getRotationKey(4), sb, sa,
+                                                P.montgomery);
+        std::vector<uint64_t> o3(sb.begin(), sb.end());
+        o3.insert(o3.end(), sa.begin(), sa.end());
+        dumpv(tag + "_qp_from_key_term_rot4", o3);
+    }
+
+    // BSGS stage: diagonals 0..5, baby steps 2 (rotations 1; giants 2, 4)
+    MyVector<uint64_t> qp(P.getModuli().begin(), P.getModuli().end());
+    qp.insert(qp.end(), P.getPModuli().begin(), P.getPModuli().end());
+    for (int rescale = 0; rescale < 2; ++rescale) {
+        CKKSLinearTransformPlan plan;
+        plan.use_qp_evaluator = true;
+        plan.stages.emplace_back();
+        auto& st = plan.stages.back();
+        for (size_t t = 0; t < 6; ++t) {
+            MyVector<Complex> d(N / 2, Complex(0.5, 0.25));
+            st.diagonals.emplace_back(ckks_encode(ctx, d, L, true));
+            st.diagonal_qp_plaintexts.emplace_back(lt_diag_qp(N, qp, t));
+            st.diagonal_indices.emplace_back(t);
+        }
+        st.baby_step_count = 2;
+        st.qp_limb_count = qp.size();
+        st.qp_p_limb_count = P.getPModuli().size();
+        st.diagonal_qp_plaintexts_montgomery = true;
+        st.rescale_after = rescale != 0;
+        auto out = ckks_apply_linear_transform_plan(ctx, ct.clone(), plan);
+        dump_ct(tag + (rescale ? "_lt_rescaled" : "_lt"), out);
+    }
+}
+
+// Pieces of bootstrapping on a context with bootstrap keys and EvalMod depth
+// 0 (no EvalMod polynomial, so nothing reaches the MPFR fit).
+static void run_bootstrap_pieces() {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << 10;
+    c.mult_depth = 1;
+    c.q0_bit = 40;
+    c.q_bit = 35;
+    c.p_bits = {60, 60};
+    c.dnum = 2;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    c.bts_keys = true;
+    auto bp = ckks_params_detail::make_evalmod_bootstrap_params(
+        16, {CKKSBootstrapDftScaleSchedule{{25}, {25}, {25}}, CKKSBootstrapDftScaleSchedule{{25}, {25}, {25}}},
+        0, 0, {0, 0});
+    bp.linear_transform_use_qp = {true, true};
+    c.bootstrap_params = bp;
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN();
+    const std::string tag = "bt10";
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    dump_key(tag + "_d2s", ctx.getDenseToSparseSwitchKey());
+    uint64_t seed = 123;
+    MyVector<Complex> m(N / 2);
+    for (auto& z : m) {
+        double re = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1;
+        z = Complex(re * 0.01, 0.0);
+    }
+    auto ct = ckks_encrypt(ctx, m, 0);
+    dump_ct(tag + "_ct0", ct);
+    ckks_key_switch_dense_to_sparse_inplace(ctx, ct);
+    dump_ct(tag + "_q0p0", ct);
+    ckks_modraise_inplace(ctx, ct, P.getMaxLevel());
+    dump_ct(tag + "_modraise", ct);
+    // the S2C DFT plan's QP diagonals (digests pin host plan encoding later)
+    const auto& plan = ctx.getEvalModS2CPlan();
+    std::vector<uint64_t> meta;
+    for (const auto& st : plan.stages) {
+        meta.push_back(st.diagonal_indices.size());
+        meta.push_back(st.baby_step_count);
+        meta.push_back(st.qp_limb_count);
+        meta.insert(meta.end(), st.diagonal_indices.begin(), st.diagonal_indices.end());
+        std::vector<uint64_t> flat;
+        for (const auto& d : st.diagonal_qp_plaintexts) flat.insert(flat.end(), d.begin(), d.end());
+        dumpv(tag + "_s2c_stage" + std::to_string(&st - plan.stages.data()), flat);
+    }
+    dumpv(tag + "_s2c_meta", meta);
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR\n"); return 2; }
     g_out = argv[1];
@@ -249,6 +401,8 @@ int main(int argc, char** argv) {
     run_ntt(1 << 16, {60}, "ntt16one", false);
     run_eval(10, 3, 2, {60, 60}, "ev10");
     run_eval(10, 4, 3, {61, 61}, "ev10b");
+    run_hoist_lt();
+    run_bootstrap_pieces();
     std::printf("ref_harness: wrote %s\n", g_out.c_str());
     return 0;
 }

@@ -77,6 +77,42 @@ def main():
         np.savez_compressed(os.path.join(HERE, tag + ".npz"), **arr)
         with open(os.path.join(HERE, tag + "_outputs.json"), "w") as f:
             json.dump(meta, f, indent=1)
+    # hoisted switches and a QP BSGS stage
+    q, p = rd("lt10_q"), rd("lt10_p")
+    N = 1 << 10
+    L, K = len(q), len(p)
+    qp = np.concatenate([q, p])
+    digits = rd("lt10_hoist_digits_lazy").reshape(-1, L + K, N) % qp[None, :, None]
+    lt_meta = {
+        "hoist_digits": digest(digits),
+        "hoisted_rot1": digest(rd("lt10_hoisted_rot1")),
+        "hoisted_qp_rot2": digest(rd("lt10_hoisted_qp_rot2")),
+        "qp_from_key_term_rot4": digest(rd("lt10_qp_from_key_term_rot4")),
+        "lt": digest(rd("lt10_lt")),
+        "lt_rescaled": digest(rd("lt10_lt_rescaled")),
+        "lt_scale": float(rd("lt10_lt_meta")[2:3].view(np.float64)[0]),
+        "diag_seed_base": 5000, "baby_steps": 2, "diag_indices": list(range(6)),
+    }
+    np.savez_compressed(os.path.join(HERE, "lt10.npz"), q=q, p=p, ct=rd("lt10_ct"),
+                        key_rot1=rd("lt10_rot1"), key_rot2=rd("lt10_rot2"), key_rot4=rd("lt10_rot4"))
+    with open(os.path.join(HERE, "lt10_outputs.json"), "w") as f:
+        json.dump(lt_meta, f, indent=1)
+
+    # bootstrap pieces: dense->sparse q0p0 switch, ModRaise, S2C plan digests
+    meta = rd("bt10_s2c_meta")
+    stages, i = [], 0
+    while i < len(meta):
+        n, bs, qpl = int(meta[i]), int(meta[i + 1]), int(meta[i + 2])
+        idx = [int(v) for v in meta[i + 3:i + 3 + n]]
+        stages.append({"baby_steps": bs, "qp_limbs": qpl, "diag_indices": idx,
+                       "digest": digest(rd("bt10_s2c_stage%d" % len(stages)))})
+        i += 3 + n
+    bt_meta = {"q0p0": digest(rd("bt10_q0p0")), "modraise": digest(rd("bt10_modraise")),
+               "modraise_level": int(rd("bt10_modraise_meta")[1]), "s2c_stages": stages}
+    np.savez_compressed(os.path.join(HERE, "bt10.npz"), q=rd("bt10_q"), p=rd("bt10_p"),
+                        ct0=rd("bt10_ct0"), key_d2s=rd("bt10_d2s"))
+    with open(os.path.join(HERE, "bt10_outputs.json"), "w") as f:
+        json.dump(bt_meta, f, indent=1)
     print("golden fixtures written to", HERE)
 
 

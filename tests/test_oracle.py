@@ -305,3 +305,83 @@ def test_keyswitch_decrypts_under_target_key():
     for i, q in enumerate(int(v) for v in d["q"]):
         centred = np.where(coeff[i] > q // 2, coeff[i].astype(object) - q, coeff[i].astype(object))
         assert max(abs(int(v)) for v in centred) < 2 ** 30
+
+
+# ------------------------------------- hoisting, QP switches, BSGS, bootstrap
+def lt_diags(N, qp, nterms, seed_base=5000):
+    """Diagonal QP plaintexts of the reference harness's BSGS stage."""
+    out = []
+    for t in range(nterms):
+        for l, m in enumerate(qp):
+            out.append(splitmix64_stream(seed_base + 64 * t + l, N) % np.uint64(int(m)))
+    return np.concatenate(out)
+
+
+def test_hoisting_and_qp_switches_match_reference():
+    d = load_npz("lt10.npz")
+    o = load_json("lt10_outputs.json")
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    L = len(q)
+    ctx = O.Ctx(N, q, p, 2)
+    lv = L - 1
+    b, a = d["ct"].reshape(2, L * N)
+    digits = ctx.hoist(lv, a)
+    assert digest(digits) == o["hoist_digits"]
+    hb, ha = ctx.keyswitch_hoisted(lv, digits, d["key_rot1"], b, np.zeros_like(a), 0, 1,
+                                   galois=O.galois_rotation(N, 1))
+    assert digest(np.concatenate([hb, ha])) == o["hoisted_rot1"]
+    qb, qa = ctx.keyswitch_hoisted_qp_rotated(lv, digits, d["key_rot2"], b, O.galois_rotation(N, 2))
+    assert digest(np.concatenate([qb, qa])) == o["hoisted_qp_rot2"]
+    sb, sa = ctx.keyswitch_qp_from_key_term(lv, a, d["key_rot4"])
+    assert digest(np.concatenate([sb, sa])) == o["qp_from_key_term_rot4"]
+    qp = np.concatenate([q, p])
+    diags = lt_diags(N, qp, 6)
+    keys = {1: d["key_rot1"], 2: d["key_rot2"], 4: d["key_rot4"]}
+    ao, bo = ctx.lt_stage_qp(lv, a, b, o["diag_indices"], diags, o["baby_steps"], keys)
+    assert digest(np.concatenate([bo, ao])) == o["lt"]
+    r = ctx.rescale(lv, np.stack([bo, ao]))
+    assert digest(r.ravel()) == o["lt_rescaled"]
+
+
+def test_q0p0_and_modraise_match_reference():
+    d = load_npz("bt10.npz")
+    o = load_json("bt10_outputs.json")
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    ctx = O.Ctx(N, q, p, 2)
+    b, a = d["ct0"].reshape(2, N)
+    sb, sa = ctx.q0p0_keyswitch(b, a, d["key_d2s"])
+    assert digest(np.concatenate([sb, sa])) == o["q0p0"]
+    lv = o["modraise_level"]
+    rb, ra = ctx.modraise(lv, sb), ctx.modraise(lv, sa)
+    assert digest(np.concatenate([rb, ra])) == o["modraise"]
+
+
+def test_modraise_closed_form():
+    """ModRaise = centred lift of the q0 coefficients, re-reduced mod each q_l."""
+    ctx = small_ctx(N=64, nq=4, np_=1, dnum=1)
+    N = ctx.N
+    q0 = int(ctx.q[0])
+    x = RNG.integers(0, q0, N, dtype=np.uint64)
+    out = ctx.modraise(3, x)
+    coeff = ctx.ntt_inv(x, 1)
+    lifted = [int(v) - q0 if int(v) > q0 // 2 else int(v) for v in coeff]
+    expect = np.concatenate([np.array([v % int(ql) for v in lifted], np.uint64) for ql in ctx.q[:4]])
+    assert np.array_equal(out, ctx.ntt_fwd(expect, 4))
+
+
+def test_lt_stage_zero_diagonals():
+    """All-zero diagonals give an exactly zero stage output."""
+    d = load_npz("lt10.npz")
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    L = len(q)
+    ctx = O.Ctx(N, q, p, 2)
+    lv = L - 1
+    b, a = d["ct"].reshape(2, L * N)
+    qp = np.concatenate([q, p])
+    zeros = np.zeros(6 * (L + len(p)) * N, np.uint64)
+    keys = {1: d["key_rot1"], 2: d["key_rot2"], 4: d["key_rot4"]}
+    ao, bo = ctx.lt_stage_qp(lv, a, b, list(range(6)), zeros, 2, keys)
+    assert not ao.any() and not bo.any()
