@@ -40,6 +40,10 @@ void check_level(const mi::Context& c, int level) {
 void check_ptr(const void* p) {
     if (!p) throw mi::Error(MI_EINVAL, "null buffer");
 }
+void check_galois(uint64_t g, bool zero_ok) {
+    if (g == 0 && zero_ok) return;
+    if ((g & 1) == 0) throw mi::Error(MI_EINVAL, "Galois element must be odd");
+}
 }  // namespace
 
 extern "C" {
@@ -307,6 +311,96 @@ int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uin
         MI_HIP(hipMemcpyAsync(b, tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
         MI_HIP(hipMemcpyAsync(a, tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
         mi::keyswitch(c, mi::KsItem{a, key, b, a, level, 0, 1}, montgomery != 0, S(stream));
+    });
+}
+
+int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, void* stream) {
+    return guard("ckks_hoist_ciphertext_a_qp_eval", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(a); check_ptr(digits);
+        mi::hoist(*ctx->c, level, a, digits, S(stream));
+    });
+}
+
+int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
+                         uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
+                         int montgomery, void* stream) {
+    return guard("ckks_hybrid_key_switch_hoisted", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(digits); check_ptr(key); check_ptr(b_out); check_ptr(a_out);
+        if ((b_mode != 0 && b_mode != 1) || (a_mode != 0 && a_mode != 1))
+            throw mi::Error(MI_EINVAL, "invalid combine mode");
+        check_galois(galois, true);
+        mi::keyswitch_hoisted(*ctx->c, level, digits, key, b_out, a_out, b_mode, a_mode, galois,
+                              montgomery != 0, S(stream));
+    });
+}
+
+int mi_keyswitch_hoisted_qp(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
+                            const uint64_t* b, uint64_t* b_qp, uint64_t* a_qp, uint64_t galois,
+                            int montgomery, void* stream) {
+    return guard("ckks_hybrid_key_switch_hoisted_qp_rotated", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(digits); check_ptr(key); check_ptr(b); check_ptr(b_qp); check_ptr(a_qp);
+        check_galois(galois, false);
+        mi::keyswitch_hoisted_qp(*ctx->c, level, digits, key, b, b_qp, a_qp, galois, montgomery != 0,
+                                 S(stream));
+    });
+}
+
+int mi_keyswitch_qp(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                    uint64_t* b_qp, uint64_t* a_qp, int montgomery, void* stream) {
+    return guard("ckks_hybrid_key_switch_qp_from_key_term", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(key_term); check_ptr(key); check_ptr(b_qp); check_ptr(a_qp);
+        mi::keyswitch_qp(*ctx->c, level, key_term, key, b_qp, a_qp, montgomery != 0, S(stream));
+    });
+}
+
+int mi_moddown_qp(mi_context* ctx, int level, uint64_t* qp, void* stream) {
+    return guard("moddown_qp_poly_to_q_eval", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(qp);
+        mi::moddown_qp(*ctx->c, level, qp, S(stream));
+    });
+}
+
+int mi_linear_transform_qp(mi_context* ctx, int level, const uint64_t* a, const uint64_t* b,
+                           int nterms, const int64_t* diag_indices, const uint64_t* const* diags,
+                           int pt_montgomery, int baby_steps, int slots, int nkeys,
+                           const int64_t* key_rotations, const uint64_t* const* keys,
+                           int montgomery, uint64_t* a_out, uint64_t* b_out, void* stream) {
+    return guard("ckks_apply_linear_transform_owned_stage_qp", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (nkeys < 0) throw mi::Error(MI_EINVAL, "invalid key count");
+        static_assert(sizeof(long long) == sizeof(int64_t), "int64 width");
+        mi::LtStage st{level, nterms, (const long long*)diag_indices, diags, pt_montgomery,
+                       baby_steps, slots, nkeys, (const long long*)key_rotations, keys, montgomery};
+        mi::linear_transform_qp(*ctx->c, st, a, b, a_out, b_out, S(stream));
+    });
+}
+
+int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint64_t* out,
+                void* stream) {
+    return guard("modraise_eval_poly_from_q0", [&] {
+        check_ctx(ctx);
+        check_ptr(poly_q0); check_ptr(out);
+        mi::modraise(*ctx->c, target_level, poly_q0, out, S(stream));
+    });
+}
+
+int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
+                      int montgomery, void* stream) {
+    return guard("ckks_q0p0_key_switch_level0_inplace", [&] {
+        check_ctx(ctx);
+        check_ptr(b); check_ptr(a); check_ptr(key);
+        mi::q0p0_keyswitch(*ctx->c, b, a, key, montgomery != 0, S(stream));
     });
 }
 

@@ -291,6 +291,9 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
         MI_HIP(hipDeviceSynchronize());
     }
     c->d_half = d_half;
+    // level-0 dense-to-sparse switch: p_0^-1 mod q_0 (ckks_keyswitch.cpp:1767-1768)
+    c->q0p0_inv = invm(p[0] % q[0], q[0]);
+    c->q0p0_inv_shoup = shoup_of(c->q0p0_inv, q[0]);
     return c;
 }
 

@@ -72,6 +72,7 @@ struct Context {
     std::vector<ModDownTables> moddown;           // [level]
     std::vector<RescaleTables> rescale;           // [level] (level >= 1)
     u64* d_half = nullptr;                        // rescale half_ntt storage
+    u64 q0p0_inv = 0, q0p0_inv_shoup = 0;         // p_0^-1 mod q_0 (level-0 switch)
     size_t table_bytes = 0;
     std::mutex ws_mu;
     std::map<hipStream_t, Workspace> ws;
@@ -139,5 +140,62 @@ struct KsItem {
     int level, b_mode, a_mode;
 };
 void keyswitch(Context& c, const KsItem& it, bool montgomery, hipStream_t s);
+
+// Key inner product over the QP slots of one level.
+struct MacArgs {
+    const u64* digits;    // [parts][L+K][N] evaluation domain
+    const u64* key;       // [parts][2 (b, a)][nq+np][N]
+    u64* b_out;           // split: L limbs; qp_out: L+K limbs
+    u64* a_out;
+    u64* accp_a;          // split only: [K][N] P slots of a
+    u64* accp_b;          //             [K][N] P slots of b
+    const u64* init_b;    // qp_out: Q-limb b term added (read through the automorphism)
+    const u64* add_b;     // qp_out: L+K-limb b term added (read through the automorphism)
+    u64 galois;           // qp_out: 0, or the Galois element applied to the result
+    int L, K, nq, parts, logN;
+    int b_mode, a_mode;   // split: 0 = add into outputs, 1 = assign
+    int montgomery;
+    int qp_out;           // 1: whole result in the QP basis
+    int accumulate;       // qp_out: add into the outputs instead of assigning
+    const BasisConst* bc;
+};
+MacArgs mac_args(const Context& c, int level, const u64* digits, const u64* key, bool montgomery);
+// Device steps (scratch supplied by the caller):
+//   ks_modup: C = L*N words, D = parts*(L+K)*N words (digits, canonical)
+//   ks_moddown: T = npoly*L*N words; the P limbs p0/p1 are destroyed;
+//   out_i = base_i - NTT(contribution_i), base may alias out.
+void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s);
+void ks_mac(const Context& c, const MacArgs& a, int level, hipStream_t s);
+void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const u64* base1, u64* out0,
+                u64* out1, u64* T, hipStream_t s);
+
+// Hoisting and QP-basis switches (ckks_keyswitch.cpp:344-373, 675-1328).
+void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s);
+void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key, u64* b_out,
+                       u64* a_out, int b_mode, int a_mode, u64 galois, bool montgomery, hipStream_t s);
+void keyswitch_hoisted_qp(Context& c, int level, const u64* digits, const u64* key, const u64* b_q,
+                          u64* b_qp, u64* a_qp, u64 galois, bool montgomery, hipStream_t s);
+void keyswitch_qp(Context& c, int level, const u64* key_term, const u64* key, u64* b_qp, u64* a_qp,
+                  bool montgomery, hipStream_t s);
+void moddown_qp(Context& c, int level, u64* qp, hipStream_t s);
+
+// BSGS linear-transform stage in the QP basis (mi_lt.hip).
+u64 rotation_galois(u64 N, long long rotation);
+struct LtStage {
+    int level, nterms;
+    const long long* diag_idx;     // host [nterms]
+    const u64* const* diags;       // host array of device pointers, each (L+K)*N
+    int pt_montgomery, baby_steps, slots;
+    int nkeys;
+    const long long* key_rot;      // host [nkeys]
+    const u64* const* keys;        // host array of device pointers
+    int montgomery;
+};
+void linear_transform_qp(Context& c, const LtStage& st, const u64* a, const u64* b, u64* a_out,
+                         u64* b_out, hipStream_t s);
+
+// Bootstrapping pieces (mi_boot.hip).
+void modraise(Context& c, int target_level, const u64* poly_q0, u64* out, hipStream_t s);
+void q0p0_keyswitch(Context& c, u64* b, u64* a, const u64* key, bool montgomery, hipStream_t s);
 
 }  // namespace mi

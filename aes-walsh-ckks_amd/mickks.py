@@ -41,6 +41,15 @@ _SIGS = {
     "mi_relin": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_rotate": (cint, [vp, cint, vp, vp, u64, vp, cint, vp]),
     "mi_conjugate": (cint, [vp, cint, vp, vp, vp, cint, vp]),
+    "mi_hoist": (cint, [vp, cint, vp, vp, vp]),
+    "mi_keyswitch_hoisted": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, u64, cint, vp]),
+    "mi_keyswitch_hoisted_qp": (cint, [vp, cint, vp, vp, vp, vp, vp, u64, cint, vp]),
+    "mi_keyswitch_qp": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
+    "mi_moddown_qp": (cint, [vp, cint, vp, vp]),
+    "mi_linear_transform_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
+                                      cint, vp, vp, vp]),
+    "mi_modraise": (cint, [vp, cint, vp, vp, vp]),
+    "mi_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint, vp]),
 }
 
 EXPORTED = sorted(_SIGS)
@@ -203,6 +212,53 @@ class Context:
 
     def conjugate(self, level, b, a, key, montgomery=True, stream=None):
         check(lib().mi_conjugate(self.h, level, ptr(b), ptr(a), ptr(key), int(montgomery), _stream(stream)))
+
+    # ---- hoisting and QP-basis switching
+    def qp_limbs(self, level):
+        return level + 1 + self.np
+
+    def hoist(self, level, a, digits, stream=None):
+        """digits: parts(level) * (level+1+np) * N words."""
+        check(lib().mi_hoist(self.h, level, ptr(a), ptr(digits), _stream(stream)))
+
+    def keyswitch_hoisted(self, level, digits, key, b_out, a_out, b_mode, a_mode, galois=0, montgomery=True,
+                          stream=None):
+        check(lib().mi_keyswitch_hoisted(self.h, level, ptr(digits), ptr(key), ptr(b_out), ptr(a_out), b_mode,
+                                         a_mode, int(galois), int(montgomery), _stream(stream)))
+
+    def keyswitch_hoisted_qp(self, level, digits, key, b, b_qp, a_qp, galois, montgomery=True, stream=None):
+        check(lib().mi_keyswitch_hoisted_qp(self.h, level, ptr(digits), ptr(key), ptr(b), ptr(b_qp), ptr(a_qp),
+                                            int(galois), int(montgomery), _stream(stream)))
+
+    def keyswitch_qp(self, level, key_term, key, b_qp, a_qp, montgomery=True, stream=None):
+        check(lib().mi_keyswitch_qp(self.h, level, ptr(key_term), ptr(key), ptr(b_qp), ptr(a_qp),
+                                    int(montgomery), _stream(stream)))
+
+    def moddown_qp(self, level, qp, stream=None):
+        check(lib().mi_moddown_qp(self.h, level, ptr(qp), _stream(stream)))
+
+    def linear_transform_qp(self, level, a, b, diag_indices, diags, baby_steps, rot_keys, a_out, b_out,
+                            pt_montgomery=True, slots=None, montgomery=True, stream=None):
+        """diags: list of device tensors (one QP plaintext per term);
+        rot_keys: {rotation amount: device key tensor}."""
+        n = len(diag_indices)
+        idx = (ctypes.c_int64 * max(n, 1))(*[int(d) for d in diag_indices])
+        dp = (vp * max(n, 1))(*[d.data_ptr() for d in diags])
+        rots = sorted(rot_keys)
+        kr = (ctypes.c_int64 * max(len(rots), 1))(*[int(r) for r in rots])
+        kp = (vp * max(len(rots), 1))(*[rot_keys[r].data_ptr() for r in rots])
+        slots = self.N // 2 if slots is None else slots
+        check(lib().mi_linear_transform_qp(self.h, level, ptr(a), ptr(b), n, ctypes.addressof(idx),
+                                           ctypes.addressof(dp), int(pt_montgomery), int(baby_steps),
+                                           int(slots), len(rots), ctypes.addressof(kr), ctypes.addressof(kp),
+                                           int(montgomery), ptr(a_out), ptr(b_out), _stream(stream)))
+
+    # ---- bootstrapping pieces
+    def modraise(self, target_level, poly_q0, out, stream=None):
+        check(lib().mi_modraise(self.h, target_level, ptr(poly_q0), ptr(out), _stream(stream)))
+
+    def q0p0_keyswitch(self, b, a, key, montgomery=True, stream=None):
+        check(lib().mi_q0p0_keyswitch(self.h, ptr(b), ptr(a), ptr(key), int(montgomery), _stream(stream)))
 
 
 def to_dev(arr):

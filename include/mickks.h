@@ -132,6 +132,61 @@ int mi_rotate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t gal
 int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uint64_t* key,
                  int montgomery, void* stream);
 
+/* ---- hoisting and QP-basis key switching ----------------------------------
+ * Digits: uint64[parts][level+1+np][N] evaluation domain, canonical; QP polys
+ * hold level+1 Q limbs then np P limbs.  parts = min(dnum, ceil((level+1)/alpha)).
+ *
+ * ckks_hoist_ciphertext_a_qp_eval (ckks_keyswitch.cpp:344-373): ModUp of every
+ * digit of a (level+1 eval limbs) to the other Q limbs and P. */
+int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, void* stream);
+/* ckks_hybrid_key_switch_hoisted_inplace (ckks_keyswitch.cpp:1072-1328): inner
+ * product with the key, folded ModDown, combine (mode 0 = Add, 1 = Assign),
+ * then the automorphism g of both outputs (galois 0 = none). */
+int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
+                         uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
+                         int montgomery, void* stream);
+/* ckks_hybrid_key_switch_hoisted_qp_rotated (ckks_keyswitch.cpp:675-852):
+ * b_qp = g([b ; 0] + sum_d digit_d key_b,d), a_qp = g(sum_d digit_d key_a,d),
+ * no ModDown.  b: level+1 eval limbs; b_qp, a_qp: level+1+np limbs. */
+int mi_keyswitch_hoisted_qp(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
+                            const uint64_t* b, uint64_t* b_qp, uint64_t* a_qp, uint64_t galois,
+                            int montgomery, void* stream);
+/* ckks_hybrid_key_switch_qp_from_key_term (ckks_keyswitch.cpp:854-1070):
+ * ModUp of key_term then the inner product, result kept in the QP basis. */
+int mi_keyswitch_qp(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                    uint64_t* b_qp, uint64_t* a_qp, int montgomery, void* stream);
+/* moddown_qp_poly_to_q_eval (ckks_linear_transform.cpp:581-605) with the
+ * folded ModDown (ckks_keyswitch.cpp:609-673): in place; the first level+1
+ * limbs of qp hold the Q result, the P limbs are overwritten. */
+int mi_moddown_qp(mi_context* ctx, int level, uint64_t* qp, void* stream);
+
+/* ---- BSGS linear-transform stage, QP evaluator ---------------------------
+ * ckks_apply_linear_transform_owned_stage_qp (ckks_linear_transform.cpp:
+ * 1174-1407) without its final rescale: a hoisted baby step per needed
+ * rotation r < baby_steps, the diagonal inner products in the QP basis, one
+ * QP key switch and automorphism per giant step, one ModDown pair at the end.
+ * diag_indices[t] >= 0 is term t's diagonal; diags[t] points to its
+ * level+1+np limb plaintext (Montgomery form if pt_montgomery); keys[i] is the
+ * rotation key for key_rotations[i] (host arrays of device pointers).
+ * a_out / b_out: level+1 limbs each, may alias a / b. */
+int mi_linear_transform_qp(mi_context* ctx, int level, const uint64_t* a, const uint64_t* b,
+                           int nterms, const int64_t* diag_indices, const uint64_t* const* diags,
+                           int pt_montgomery, int baby_steps, int slots, int nkeys,
+                           const int64_t* key_rotations, const uint64_t* const* keys,
+                           int montgomery, uint64_t* a_out, uint64_t* b_out, void* stream);
+
+/* ---- bootstrapping pieces --------------------------------------------------
+ * modraise_eval_poly_from_q0 (ckks_modraise.cpp:31-75): one q_0 eval limb,
+ * centred lift, reduced to q_0..q_target and returned to the evaluation
+ * domain (target_level+1 limbs; out may alias poly_q0). */
+int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint64_t* out,
+                void* stream);
+/* ckks_q0p0_key_switch_level0_inplace (ckks_keyswitch.cpp:1661-1816): the
+ * level-0 dense-to-sparse switch over {q_0, p_0}; key = [b(q0), b(p0),
+ * a(q0), a(p0)], N words each.  (b, a) <- (b + KS_b(a), KS_a(a)). */
+int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
+                      int montgomery, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

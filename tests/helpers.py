@@ -21,6 +21,16 @@ def splitmix64_stream(seed, n):
         return z ^ (z >> np.uint64(31))
 
 
+def lt_diags(N, qp, nterms, seed_base=5000):
+    """Diagonal QP plaintexts of the reference harness's BSGS stage
+    (oracle/ref_harness.cpp): term t, limb l = splitmix64(seed_base+64t+l) mod m_l."""
+    out = []
+    for t in range(nterms):
+        for l, m in enumerate(qp):
+            out.append(splitmix64_stream(seed_base + 64 * t + l, N) % np.uint64(int(m)))
+    return np.concatenate(out)
+
+
 def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype=np.uint64).tobytes()).hexdigest()
 

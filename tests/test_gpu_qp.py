@@ -1,0 +1,226 @@
+"""GPU parity of hoisting, the QP-basis key switches, the BSGS linear-transform
+stage, ModRaise and the level-0 q0p0 switch, through the C-ABI.
+
+Bit-exact against the reference's own outputs (lt10 / bt10 golden fixtures,
+captured keys) and against the oracle on seeded random inputs at larger ring
+sizes, with random residues standing in for keys and diagonals."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import digest, load_json, load_npz, lt_diags, product, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+
+def _zeros(M, words):
+    return M.to_dev(np.zeros(words, np.uint64))
+
+
+def test_qp_path_matches_reference_lt10():
+    d = load_npz("lt10.npz")
+    o = load_json("lt10_outputs.json")
+    M = product()
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    L, K = len(q), len(p)
+    T = L + K
+    ctx = M.Context(N, q, p, 2)
+    lv = L - 1
+    b, a = d["ct"].reshape(2, L * N)
+    da, db = M.to_dev(a), M.to_dev(b)
+    keys = {1: M.to_dev(d["key_rot1"]), 2: M.to_dev(d["key_rot2"]), 4: M.to_dev(d["key_rot4"])}
+
+    dig = _zeros(M, ctx.parts(lv) * T * N)
+    ctx.hoist(lv, da, dig)
+    assert digest(M.from_dev(dig)) == o["hoist_digits"]
+
+    hb, ha = M.to_dev(b), _zeros(M, L * N)
+    ctx.keyswitch_hoisted(lv, dig, keys[1], hb, ha, 0, 1, galois=O.galois_rotation(N, 1))
+    assert digest(np.concatenate([M.from_dev(hb), M.from_dev(ha)])) == o["hoisted_rot1"]
+
+    qb, qa = _zeros(M, T * N), _zeros(M, T * N)
+    ctx.keyswitch_hoisted_qp(lv, dig, keys[2], db, qb, qa, O.galois_rotation(N, 2))
+    assert digest(np.concatenate([M.from_dev(qb), M.from_dev(qa)])) == o["hoisted_qp_rot2"]
+
+    sb, sa = _zeros(M, T * N), _zeros(M, T * N)
+    ctx.keyswitch_qp(lv, da, keys[4], sb, sa)
+    assert digest(np.concatenate([M.from_dev(sb), M.from_dev(sa)])) == o["qp_from_key_term_rot4"]
+
+    diags = lt_diags(N, np.concatenate([q, p]), 6).reshape(6, T * N)
+    dd = [M.to_dev(x) for x in diags]
+    ao, bo = _zeros(M, L * N), _zeros(M, L * N)
+    ctx.linear_transform_qp(lv, da, db, o["diag_indices"], dd, o["baby_steps"], keys, ao, bo)
+    out = np.concatenate([M.from_dev(bo), M.from_dev(ao)])
+    assert digest(out) == o["lt"]
+    pair = M.to_dev(out)
+    ctx.rescale(lv, pair, 2)
+    r = M.from_dev(pair).reshape(2, L * N)[:, : (L - 1) * N]
+    assert digest(r.ravel()) == o["lt_rescaled"]
+    # inputs untouched
+    assert np.array_equal(M.from_dev(da), a) and np.array_equal(M.from_dev(db), b)
+
+
+def test_bootstrap_pieces_match_reference_bt10():
+    d = load_npz("bt10.npz")
+    o = load_json("bt10_outputs.json")
+    M = product()
+    N = 1 << 10
+    ctx = M.Context(N, d["q"], d["p"], 2)
+    b, a = d["ct0"].reshape(2, N)
+    db, da = M.to_dev(b), M.to_dev(a)
+    ctx.q0p0_keyswitch(db, da, M.to_dev(d["key_d2s"]))
+    sb, sa = M.from_dev(db), M.from_dev(da)
+    assert digest(np.concatenate([sb, sa])) == o["q0p0"]
+    lv = o["modraise_level"]
+    rb, ra = _zeros(M, (lv + 1) * N), _zeros(M, (lv + 1) * N)
+    ctx.modraise(lv, db, rb)
+    ctx.modraise(lv, da, ra)
+    assert digest(np.concatenate([M.from_dev(rb), M.from_dev(ra)])) == o["modraise"]
+
+
+def _setup(N, nq, np_, dnum, seed, qbits=45, pbits=60):
+    p, q = O.find_primes([pbits] * np_, [qbits] * nq, N)
+    M = product()
+    return M, M.Context(N, q, p, dnum), O.Ctx(N, q, p, dnum), q, p, np.random.default_rng(seed)
+
+
+def _key(rng, q, p, N, parts):
+    basis = list(q) + list(p)
+    return np.concatenate([rand_residues(rng, basis, N) for _ in range(2 * parts)])
+
+
+@pytest.mark.parametrize("N,nq,np_,dnum,mont", [(1 << 13, 9, 3, 3, True), (1 << 14, 12, 4, 4, False),
+                                                 (1 << 16, 10, 3, 2, True)])
+def test_hoisted_and_qp_switches_match_oracle(N, nq, np_, dnum, mont):
+    M, ctx, octx, q, p, rng = _setup(N, nq, np_, dnum, seed=N + nq)
+    key = _key(rng, q, p, N, octx.parts(nq - 1))
+    dkey = M.to_dev(key)
+    for level in sorted({nq - 1, nq // 2, 0}):
+        L, T = level + 1, level + 1 + np_
+        qp = list(q[:L]) + list(p)
+        a = rand_residues(rng, q[:L], N)
+        b = rand_residues(rng, q[:L], N)
+        da, db = M.to_dev(a), M.to_dev(b)
+        dig_o = octx.hoist(level, a)
+        dig = _zeros(M, ctx.parts(level) * T * N)
+        ctx.hoist(level, da, dig)
+        assert np.array_equal(M.from_dev(dig), dig_o), level
+        g3 = O.galois_rotation(N, 3)
+        for bm, am, g in [(0, 1, g3), (0, 0, 0), (1, 1, g3)]:
+            ob, oa = octx.keyswitch_hoisted(level, dig_o, key, b, a, bm, am, galois=g, montgomery=mont)
+            hb, ha = M.to_dev(b), M.to_dev(a)
+            ctx.keyswitch_hoisted(level, dig, dkey, hb, ha, bm, am, galois=g, montgomery=mont)
+            assert np.array_equal(M.from_dev(hb), ob), (level, bm, am, g)
+            assert np.array_equal(M.from_dev(ha), oa), (level, bm, am, g)
+        ob, oa = octx.keyswitch_hoisted_qp_rotated(level, dig_o, key, b, g3, montgomery=mont)
+        qb, qa = _zeros(M, T * N), _zeros(M, T * N)
+        ctx.keyswitch_hoisted_qp(level, dig, dkey, db, qb, qa, g3, montgomery=mont)
+        assert np.array_equal(M.from_dev(qb), ob) and np.array_equal(M.from_dev(qa), oa), level
+        ob, oa = octx.keyswitch_qp_from_key_term(level, a, key, montgomery=mont)
+        ctx.keyswitch_qp(level, da, dkey, qb, qa, montgomery=mont)
+        assert np.array_equal(M.from_dev(qb), ob) and np.array_equal(M.from_dev(qa), oa), level
+        x = rand_residues(rng, qp, N)
+        dx = M.to_dev(x)
+        ctx.moddown_qp(level, dx)
+        assert np.array_equal(M.from_dev(dx)[: L * N], octx.moddown_qp_to_q(level, x)), level
+
+
+def _lt_case(M, ctx, octx, q, p, rng, N, level, diag_idx, bs, mont=True, pt_mont=True, in_place=False):
+    L, T = level + 1, level + 1 + len(p)
+    qp = list(q[:L]) + list(p)
+    parts = octx.parts(len(q) - 1)
+    a = rand_residues(rng, q[:L], N)
+    b = rand_residues(rng, q[:L], N)
+    diags = [rand_residues(rng, qp, N) for _ in diag_idx]
+    rots = set()
+    for dgi in diag_idx:
+        if dgi % bs:
+            rots.add(dgi % bs)
+        if dgi // bs:
+            rots.add((dgi // bs) * bs)
+    keys = {r: _key(rng, q, p, N, parts) for r in sorted(rots)}
+    oa, ob = octx.lt_stage_qp(level, a, b, diag_idx, np.concatenate(diags), bs, keys, pt_mont=pt_mont,
+                              montgomery=mont)
+    da, db = M.to_dev(a), M.to_dev(b)
+    dkeys = {r: M.to_dev(k) for r, k in keys.items()}
+    dd = [M.to_dev(x) for x in diags]
+    if in_place:
+        ao, bo = da, db
+    else:
+        ao, bo = _zeros(M, L * N), _zeros(M, L * N)
+    ctx.linear_transform_qp(level, da, db, diag_idx, dd, bs, dkeys, ao, bo, pt_montgomery=pt_mont,
+                            montgomery=mont)
+    assert np.array_equal(M.from_dev(ao), oa), (level, diag_idx, bs)
+    assert np.array_equal(M.from_dev(bo), ob), (level, diag_idx, bs)
+
+
+@pytest.mark.parametrize("diag_idx,bs", [
+    ([0, 1, 3, 5, 6, 9, 12, 14], 4),   # every giant step, diagonal 0 present
+    ([2, 7, 13], 4),                    # no direct term
+    ([5, 9], 4),                        # giant 0 empty: first output from a key switch
+    ([0], 1),                           # single plaintext product
+    ([1, 2, 3, 4, 5, 6, 7, 8], 8),      # babies only
+])
+def test_linear_transform_matches_oracle(diag_idx, bs):
+    M, ctx, octx, q, p, rng = _setup(1 << 13, 9, 3, 3, seed=len(diag_idx) * 7 + bs)
+    _lt_case(M, ctx, octx, q, p, rng, 1 << 13, 8, diag_idx, bs)
+
+
+def test_linear_transform_modes_levels_and_aliasing():
+    M, ctx, octx, q, p, rng = _setup(1 << 14, 12, 4, 4, seed=11)
+    _lt_case(M, ctx, octx, q, p, rng, 1 << 14, 6, [0, 3, 4, 10, 11], 4, mont=False, pt_mont=False)
+    _lt_case(M, ctx, octx, q, p, rng, 1 << 14, 11, [0, 1, 6, 7], 3, in_place=True)
+    _lt_case(M, ctx, octx, q, p, rng, 1 << 14, 0, [0, 2, 5], 2)
+
+
+def test_linear_transform_many_terms_per_giant():
+    """More terms in one giant step than one inner-product launch takes."""
+    M, ctx, octx, q, p, rng = _setup(1 << 12, 6, 2, 2, seed=5)
+    _lt_case(M, ctx, octx, q, p, rng, 1 << 12, 5, list(range(40)) + [64, 70], 64)
+
+
+@pytest.mark.parametrize("N,nq,mont", [(1 << 14, 10, True), (1 << 16, 24, False)])
+def test_modraise_and_q0p0_match_oracle(N, nq, mont):
+    M, ctx, octx, q, p, rng = _setup(N, nq, 2, 3, seed=N + 1, qbits=50, pbits=58)
+    for target in (1, nq - 1):
+        x = rng.integers(0, int(q[0]), N, dtype=np.uint64)
+        out = _zeros(M, (target + 1) * N)
+        ctx.modraise(target, M.to_dev(x), out)
+        assert np.array_equal(M.from_dev(out), octx.modraise(target, x)), target
+    # in place: out aliases the q0 limb
+    x = rng.integers(0, int(q[0]), N, dtype=np.uint64)
+    buf = M.to_dev(np.concatenate([x, np.zeros((nq - 1) * N, np.uint64)]))
+    ctx.modraise(nq - 1, buf, buf)
+    assert np.array_equal(M.from_dev(buf), octx.modraise(nq - 1, x))
+    b = rng.integers(0, int(q[0]), N, dtype=np.uint64)
+    a = rng.integers(0, int(q[0]), N, dtype=np.uint64)
+    key = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in (q[0], p[0], q[0], p[0])])
+    ob, oa = octx.q0p0_keyswitch(b, a, key, montgomery=mont)
+    db, da = M.to_dev(b), M.to_dev(a)
+    ctx.q0p0_keyswitch(db, da, M.to_dev(key), montgomery=mont)
+    assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa)
+
+
+def test_qp_entry_points_reject_bad_arguments():
+    M, ctx, octx, q, p, rng = _setup(1 << 12, 4, 2, 2, seed=3)
+    N, L, T = 1 << 12, 4, 6
+    a, b = M.to_dev(rand_residues(rng, q, N)), M.to_dev(rand_residues(rng, q, N))
+    key = M.to_dev(_key(rng, q, p, N, 2))
+    dig = _zeros(M, 2 * T * N)
+    qb, qa = _zeros(M, T * N), _zeros(M, T * N)
+    pt = M.to_dev(rand_residues(rng, list(q) + list(p), N))
+    with pytest.raises(M.MiError, match="invalid level"):
+        ctx.hoist(4, a, dig)
+    with pytest.raises(M.MiError, match="odd"):
+        ctx.keyswitch_hoisted_qp(3, dig, key, b, qb, qa, 4)
+    with pytest.raises(M.MiError, match="missing rotation key"):
+        ctx.linear_transform_qp(3, a, b, [0, 1], [pt, pt], 2, {}, qa, qb)
+    with pytest.raises(M.MiError, match="non-negative"):
+        ctx.linear_transform_qp(3, a, b, [-1], [pt], 2, {}, qa, qb)
+    with pytest.raises(M.MiError, match="slot range"):
+        ctx.linear_transform_qp(3, a, b, [8], [pt], 2, {8: key}, qa, qb, slots=4)
+    with pytest.raises(M.MiError, match="target level"):
+        ctx.modraise(0, a, qa)
+    with pytest.raises(M.MiError, match="combine mode"):
+        ctx.keyswitch_hoisted(3, dig, key, qb, qa, 2, 1)

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import GOLDEN, brev, crt, digest, load_json, load_npz, rand_residues, splitmix64_stream
+from helpers import GOLDEN, brev, crt, digest, load_json, load_npz, lt_diags, rand_residues, splitmix64_stream
 
 RNG = np.random.default_rng(1234)
 
@@ -308,15 +308,6 @@ def test_keyswitch_decrypts_under_target_key():
 
 
 # ------------------------------------- hoisting, QP switches, BSGS, bootstrap
-def lt_diags(N, qp, nterms, seed_base=5000):
-    """Diagonal QP plaintexts of the reference harness's BSGS stage."""
-    out = []
-    for t in range(nterms):
-        for l, m in enumerate(qp):
-            out.append(splitmix64_stream(seed_base + 64 * t + l, N) % np.uint64(int(m)))
-    return np.concatenate(out)
-
-
 def test_hoisting_and_qp_switches_match_reference():
     d = load_npz("lt10.npz")
     o = load_json("lt10_outputs.json")
