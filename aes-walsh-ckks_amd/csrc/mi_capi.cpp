@@ -88,6 +88,102 @@ int64_t mi_context_device_bytes(const mi_context* ctx) {
     return b;
 }
 
+int mi_alloc(mi_context* ctx, size_t bytes, void** out) {
+    return guard("mi_alloc", [&] {
+        check_ctx(ctx);
+        check_ptr(out);
+        *out = ctx->c->pool_alloc(bytes);
+    });
+}
+
+int mi_free(mi_context* ctx, void* ptr) {
+    return guard("mi_free", [&] {
+        check_ctx(ctx);
+        ctx->c->pool_release(ptr);
+    });
+}
+
+int mi_memory_trim(mi_context* ctx) {
+    return guard("mi_memory_trim", [&] {
+        check_ctx(ctx);
+        ctx->c->pool_trim();
+    });
+}
+
+int mi_memory_stats(mi_context* ctx, int64_t* stats) {
+    return guard("mi_memory_stats", [&] {
+        check_ctx(ctx);
+        check_ptr(stats);
+        mi::Context& c = *ctx->c;
+        std::lock_guard<std::mutex> g(c.pool_mu);
+        stats[0] = (int64_t)c.pool_live_bytes;
+        stats[1] = (int64_t)c.pool_peak_bytes;
+        stats[2] = (int64_t)c.pool_cached_bytes;
+        stats[3] = (int64_t)c.pool_limit;
+    });
+}
+
+int mi_set_memory_limit(mi_context* ctx, int64_t bytes) {
+    return guard("mi_set_memory_limit", [&] {
+        check_ctx(ctx);
+        if (bytes < 0) throw mi::Error(MI_EINVAL, "negative limit");
+        std::lock_guard<std::mutex> g(ctx->c->pool_mu);
+        ctx->c->pool_limit = (size_t)bytes;
+    });
+}
+
+int mi_memcpy(mi_context* ctx, void* dst, const void* src, size_t bytes, int kind, void* stream) {
+    return guard("mi_memcpy", [&] {
+        check_ctx(ctx);
+        if (bytes == 0) return;
+        check_ptr(dst); check_ptr(src);
+        hipMemcpyKind k;
+        switch (kind) {
+            case MI_COPY_H2D: k = hipMemcpyHostToDevice; break;
+            case MI_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+            case MI_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+            default: throw mi::Error(MI_EINVAL, "invalid copy kind");
+        }
+        MI_HIP(hipMemcpyAsync(dst, src, bytes, k, S(stream)));
+        if (kind != MI_COPY_D2D) MI_HIP(hipStreamSynchronize(S(stream)));
+    });
+}
+
+int mi_stream_synchronize(mi_context* ctx, void* stream) {
+    return guard("mi_stream_synchronize", [&] {
+        check_ctx(ctx);
+        MI_HIP(hipStreamSynchronize(S(stream)));
+    });
+}
+
+int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int npolys,
+               int64_t poly_stride, void* stream) {
+    return guard("mi_poly_op", [&] {
+        check_ctx(ctx);
+        check_ptr(x); check_ptr(y);
+        const mi::Context& c = *ctx->c;
+        if (limbs < 1 || limbs > c.nq + c.np) throw mi::Error(MI_EINVAL, "invalid limb count");
+        if (npolys < 1 || poly_stride < (int64_t)limbs * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid poly count or stride");
+        if (op < MI_OP_ADD || op > MI_OP_MUL_MONT) throw mi::Error(MI_EINVAL, "invalid op");
+        mi::ew_binary(c, op, x, y, limbs, npolys, poly_stride, poly_stride, S(stream));
+    });
+}
+
+int mi_poly_scalar(mi_context* ctx, int op, uint64_t* x, const uint64_t* residues, int limbs,
+                   int npolys, int64_t poly_stride, void* stream) {
+    return guard("mi_poly_scalar", [&] {
+        check_ctx(ctx);
+        check_ptr(x); check_ptr(residues);
+        const mi::Context& c = *ctx->c;
+        if (limbs < 1 || limbs > c.nq + c.np) throw mi::Error(MI_EINVAL, "invalid limb count");
+        if (npolys < 1 || poly_stride < (int64_t)limbs * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid poly count or stride");
+        if (op != MI_OP_ADD && op != MI_OP_MUL) throw mi::Error(MI_EINVAL, "invalid op");
+        mi::ew_scalar(c, op, x, residues, limbs, npolys, poly_stride, S(stream));
+    });
+}
+
 int mi_ntt_forward(mi_context* ctx, uint64_t* data, int npolys, int limbs, int64_t poly_stride,
                    int basis0, void* stream) {
     return guard("mi_ntt_forward", [&] {

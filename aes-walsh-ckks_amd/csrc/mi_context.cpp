@@ -297,8 +297,63 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     return c;
 }
 
+// Pool blocks are rounded to 4 KiB; a cached block is reused for a request of
+// at least half its size.  Frees are not stream-ordered: the caller frees a
+// block only after the work using it is ordered before any later use (one
+// stream, or a synchronisation).
+void* Context::pool_alloc(size_t bytes) {
+    const size_t rounded = (std::max<size_t>(bytes, 1) + 4095) & ~size_t(4095);
+    std::lock_guard<std::mutex> g(pool_mu);
+    if (pool_limit && pool_live_bytes + rounded > pool_limit)
+        throw Error(-2, "device memory limit reached (" + std::to_string(pool_live_bytes) + " of " +
+                            std::to_string(pool_limit) + " bytes live)");
+    void* p = nullptr;
+    size_t got = rounded;
+    auto it = pool_free.lower_bound(rounded);
+    if (it != pool_free.end() && it->first <= 2 * rounded) {
+        p = it->second;
+        got = it->first;
+        pool_cached_bytes -= got;
+        pool_free.erase(it);
+    } else {
+        hipError_t e = hipMalloc(&p, rounded);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            for (auto& kv : pool_free) (void)hipFree(kv.second);
+            pool_free.clear();
+            pool_cached_bytes = 0;
+            MI_HIP(hipMalloc(&p, rounded));
+        }
+    }
+    pool_live[p] = got;
+    pool_live_bytes += got;
+    pool_peak_bytes = std::max(pool_peak_bytes, pool_live_bytes);
+    return p;
+}
+
+void Context::pool_release(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(pool_mu);
+    auto it = pool_live.find(p);
+    if (it == pool_live.end()) throw Error(-1, "pointer was not allocated by this context");
+    pool_live_bytes -= it->second;
+    pool_cached_bytes += it->second;
+    pool_free.emplace(it->second, p);
+    pool_live.erase(it);
+}
+
+void Context::pool_trim() {
+    std::lock_guard<std::mutex> g(pool_mu);
+    for (auto& kv : pool_free) (void)hipFree(kv.second);
+    pool_free.clear();
+    pool_cached_bytes = 0;
+}
+
 void context_destroy(Context* c) {
     if (!c) return;
+    (void)hipDeviceSynchronize();
+    for (auto& kv : c->pool_free) (void)hipFree(kv.second);
+    for (auto& kv : c->pool_live) (void)hipFree(kv.first);
     (void)hipFree(c->d_fwd);
     (void)hipFree(c->d_fwd_shoup);
     (void)hipFree(c->d_inv);

@@ -76,6 +76,13 @@ struct Context {
     size_t table_bytes = 0;
     std::mutex ws_mu;
     std::map<hipStream_t, Workspace> ws;
+    // Device memory pool for ciphertexts, plaintexts and keys: freed blocks
+    // are cached by size and reused; byte counts feed admission decisions.
+    std::mutex pool_mu;
+    std::multimap<size_t, void*> pool_free;
+    std::map<void*, size_t> pool_live;
+    size_t pool_live_bytes = 0, pool_peak_bytes = 0, pool_cached_bytes = 0;
+    size_t pool_limit = 0;  // 0 = no limit
 
     int nb() const { return nq + np; }
     int parts(int level) const {
@@ -84,6 +91,9 @@ struct Context {
         return pp < dnum ? pp : dnum;
     }
     void* workspace(hipStream_t s, size_t bytes);
+    void* pool_alloc(size_t bytes);
+    void pool_release(void* p);
+    void pool_trim();
 };
 
 struct Error : std::runtime_error {

@@ -1,0 +1,355 @@
+// C++ host API of the MI355X CKKS library.
+//
+// The reference's public types and free functions (aes-walsh-ckks
+// src/ckks/{parameters,context,ciphertext,encoding,operations}) with the same
+// names, argument meaning and exceptions (std::invalid_argument for bad shape,
+// level or metadata; std::runtime_error for missing keys or device failures;
+// messages prefixed "<function>: ").  Ciphertext, plaintext and key storage is
+// resident in device memory and every ring operation runs in the HIP kernels
+// behind include/mickks.h; this layer only holds metadata, host precompute
+// (moduli, FFT tables, key sampling) and the floating-point encode/decode.
+//
+// Differences from the reference API, all forced by device residency:
+//   * CKKSCiphertext / CKKSEncoding carry the CKKSContext they live on;
+//     getPoly()/getPlaintext() return host copies, setPoly() uploads, and
+//     polyData() exposes the device pointer.
+//   * randomness is seeded: CKKSParamsConfig::seed drives key generation and
+//     each context's encryption stream (the reference draws from getrandom).
+#pragma once
+
+#include <atomic>
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/mickks.h"
+
+using Complex = std::complex<double>;
+template <class T>
+using MyVector = std::vector<T>;
+
+enum class CKKSSecretOwner { Dense, BootstrapSparse };
+enum class CKKSMessageEncodingState { Slots, Coefficients };
+enum class CKKSDecryptDomain { Slots, Coefficients };
+enum class CKKSHybridKeySwitchCombine { Add, Assign };
+
+// ---------------------------------------------------------------- parameters
+// CKKSParamsConfig (ckks_params.hpp:85-117): the fields of the plain (non
+// bootstrapping) chain.
+struct CKKSParamsConfig {
+    double scale = 0.0;
+    double sigma = 3.19;
+    size_t N = size_t{1} << 15;
+    size_t slots = 0;
+    size_t depth = 0;
+    size_t mult_depth = 0;
+    size_t q0_bit = 33;
+    size_t q_bit = 32;
+    std::vector<uint64_t> q_mult_bits;
+    size_t dnum = 3;
+    size_t p_bit = 61;
+    size_t pmoduli_count = 1;
+    std::vector<uint64_t> p_bits;
+    size_t secret_key_hamming_weight = 256;
+    std::vector<uint64_t> q_bits;
+    MyVector<int64_t> rotate_values;
+    bool secure = true;
+    bool fold_hybrid_keyswitch = true;
+    bool montgomery = true;
+    uint64_t seed = 0x5eed;
+};
+
+// find_p_and_q_primes (random.cpp:66-131): P bit sizes first, then Q; one
+// descending cursor per bit size from 2^bits/(2N); q = 2kN+1; Miller-Rabin.
+std::pair<std::vector<uint64_t>, std::vector<uint64_t>> find_p_and_q_primes(
+    const std::vector<uint64_t>& p_bits, const std::vector<uint64_t>& q_bits, size_t N);
+
+class CKKSParams {
+public:
+    explicit CKKSParams(CKKSParamsConfig config);
+    size_t getN() const { return N; }
+    size_t getSlots() const { return slots; }
+    size_t getDepth() const { return depth; }
+    size_t getMaxLevel() const { return moduli.size() - 1; }
+    size_t getDnum() const { return dnum; }
+    size_t getAlpha() const { return (moduli.size() + dnum - 1) / dnum; }
+    double getScale() const { return scale; }
+    double getSigma() const { return sigma; }
+    size_t getSecretKeyHammingWeight() const { return hamming_weight; }
+    const MyVector<uint64_t>& getModuli() const { return moduli; }
+    const MyVector<uint64_t>& getPModuli() const { return pmoduli; }
+    const std::vector<uint64_t>& getQBits() const { return q_bits; }
+    const std::vector<uint64_t>& getPBits() const { return p_bits; }
+    const MyVector<int64_t>& getRotateValues() const { return rotate_values; }
+    uint64_t getSeed() const { return seed; }
+
+    bool secure = true;
+    bool fold_hybrid_keyswitch = true;
+    bool montgomery = true;
+
+private:
+    size_t N = 0, slots = 0, depth = 0, mult_depth = 0, dnum = 0, hamming_weight = 0;
+    double scale = 0.0, sigma = 0.0;
+    std::vector<uint64_t> q_bits, p_bits;
+    MyVector<uint64_t> moduli, pmoduli;
+    MyVector<int64_t> rotate_values;
+    uint64_t seed = 0;
+};
+
+class CKKSContext;
+
+// ------------------------------------------------------------ device storage
+// One allocation from the context's device pool (mi_alloc / mi_free).
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(mi_context* ctx, size_t words);
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept { swap(o); }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+        if (this != &o) {
+            reset();
+            swap(o);
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { reset(); }
+    uint64_t* data() const { return ptr_; }
+    size_t words() const { return words_; }
+    void reset();
+    void swap(DeviceBuffer& o) noexcept {
+        std::swap(ctx_, o.ctx_);
+        std::swap(ptr_, o.ptr_);
+        std::swap(words_, o.words_);
+    }
+
+private:
+    mi_context* ctx_ = nullptr;
+    uint64_t* ptr_ = nullptr;
+    size_t words_ = 0;
+};
+
+// ---------------------------------------------------------- ciphertext types
+// CKKSCiphertext (ckks_ciphertext.hpp:18-80): polys [b, a, (c2)], each
+// (level+1)*N words in the evaluation domain.  Move-only; clone() copies.
+class CKKSCiphertext {
+public:
+    // Uninitialised device polys.
+    CKKSCiphertext(const CKKSContext& context, size_t num_polys, long double scale, size_t level,
+                   CKKSSecretOwner secret_owner = CKKSSecretOwner::Dense,
+                   CKKSMessageEncodingState message_encoding_state = CKKSMessageEncodingState::Slots);
+    // Uploads host polys (ckks_ciphertext.hpp:20-33 with the context added).
+    CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uint64_t>> polys, long double scale,
+                   size_t level, CKKSSecretOwner secret_owner = CKKSSecretOwner::Dense,
+                   CKKSMessageEncodingState message_encoding_state = CKKSMessageEncodingState::Slots);
+    CKKSCiphertext(const CKKSCiphertext&) = delete;
+    CKKSCiphertext& operator=(const CKKSCiphertext&) = delete;
+    CKKSCiphertext(CKKSCiphertext&&) noexcept = default;
+    CKKSCiphertext& operator=(CKKSCiphertext&&) noexcept = default;
+
+    CKKSCiphertext clone() const;
+
+    long double getScale() const { return scale; }
+    size_t getLevel() const { return level; }
+    size_t getN() const { return N; }
+    size_t getNumPolys() const { return polys.size(); }
+    CKKSSecretOwner getSecretOwner() const { return secret_owner; }
+    CKKSMessageEncodingState getMessageEncodingState() const { return message_encoding_state; }
+    void setScale(long double s) { scale = s; }
+    void setLevel(size_t l);
+    void setSecretOwner(CKKSSecretOwner o) { secret_owner = o; }
+    void setMessageEncodingState(CKKSMessageEncodingState s) { message_encoding_state = s; }
+
+    // Host copies of the active limbs ((level+1)*N words).
+    MyVector<uint64_t> getPoly(size_t idx) const;
+    MyVector<uint64_t> getA() const { return getPoly(1); }
+    MyVector<uint64_t> getB() const { return getPoly(0); }
+    void setPoly(size_t idx, const MyVector<uint64_t>& host);
+    // Device pointers (limb-major, limb l at l*N).
+    uint64_t* polyData(size_t idx);
+    const uint64_t* polyData(size_t idx) const;
+    // Appends an uninitialised poly (the c2 slot of a tensor product).
+    void addPoly();
+    void resizePolys(size_t count);
+    const CKKSContext& getContext() const { return *ctx; }
+
+private:
+    const CKKSContext* ctx;
+    size_t N;
+    long double scale;
+    size_t level;
+    CKKSSecretOwner secret_owner;
+    CKKSMessageEncodingState message_encoding_state;
+    std::vector<DeviceBuffer> polys;
+};
+
+// CKKSEncoding (ckks_ciphertext.hpp:83-108): one (level+1)-limb plaintext.
+class CKKSEncoding {
+public:
+    CKKSEncoding(const CKKSContext& context, MyVector<uint64_t> plaintext, long double scale,
+                 size_t level, bool montgomery_form = false);
+    CKKSEncoding(const CKKSContext& context, DeviceBuffer plaintext, long double scale, size_t level,
+                 bool montgomery_form = false);
+    CKKSEncoding(const CKKSEncoding&) = delete;
+    CKKSEncoding& operator=(const CKKSEncoding&) = delete;
+    CKKSEncoding(CKKSEncoding&&) noexcept = default;
+    CKKSEncoding& operator=(CKKSEncoding&&) noexcept = default;
+
+    MyVector<uint64_t> getPlaintext() const;
+    const uint64_t* data() const { return buf.data(); }
+    bool isMontgomeryForm() const { return montgomery_form; }
+    long double getScale() const { return scale; }
+    size_t getLevel() const { return level; }
+    size_t getN() const { return N; }
+
+private:
+    const CKKSContext* ctx;
+    size_t N;
+    long double scale;
+    size_t level;
+    bool montgomery_form;
+    DeviceBuffer buf;
+};
+
+// ------------------------------------------------------------------- context
+// CKKSContext (ckks_context.hpp:59-90; constructor ckks_context.cpp:603-705):
+// FFT tables and the slot permutation on the host; NTT, ModUp/ModDown and
+// rescale tables on the device (mi_context_create); a seeded secret key and
+// the relinearisation, conjugation and rotation keys generated on the device.
+class CKKSContext {
+public:
+    explicit CKKSContext(CKKSParams params);
+    ~CKKSContext();
+    CKKSContext(const CKKSContext&) = delete;
+    CKKSContext& operator=(const CKKSContext&) = delete;
+
+    const CKKSParams& getParams() const { return params; }
+    mi_context* handle() const { return mi; }
+    void* stream() const { return nullptr; }
+    const MyVector<Complex>& getTwiddleFft() const { return twiddle_fft; }
+    const MyVector<Complex>& getTwiddleIfft() const { return twiddle_ifft; }
+    const MyVector<size_t>& getPermTable() const { return perm_table; }
+
+    const MyVector<int8_t>& getSecretKeyCoeffs() const { return secret_coeffs; }
+    // Evaluation-domain secret over q_0..q_Lmax, p_0..p_{K-1} (device).
+    const uint64_t* secretKeyDevice() const { return secret_qp.data(); }
+    // Host copy of the evaluation-domain secret, one vector per Q limb.
+    MyVector<MyVector<uint64_t>> getSecretKey() const;
+
+    // Switching keys: [digits][2 (b, a)][nq+np][N] on the device.
+    size_t keyDigitCount() const { return key_digits; }
+    size_t switchKeyWords() const;
+    const uint64_t* getRelinKeyDevice() const { return relin_key.data(); }
+    bool hasConjugationKey() const { return conj_key.data() != nullptr; }
+    const uint64_t* getConjugationKeyDevice() const;
+    size_t getRotationKeyCount() const { return rotation_keys.size(); }
+    bool hasRotationKey(int64_t rotation) const;
+    const uint64_t* getRotationKeyDevice(int64_t rotation) const;
+    std::vector<int64_t> getRotationKeyValues() const;
+    // Generates the key for one more rotation (normalised mod N/2).
+    void addRotationKey(int64_t rotation);
+    // Installs keys made elsewhere (a saved key set, or another
+    // implementation's keys for a cross-check): switching keys in the layout
+    // above; the secret as evaluation-domain Q limbs (used by decryption and
+    // encryption; key generation keeps the context's own secret).
+    void setRelinKey(const MyVector<uint64_t>& key);
+    void setConjugationKey(const MyVector<uint64_t>& key);
+    void setRotationKey(int64_t rotation, const MyVector<uint64_t>& key);
+    void setSecretKeyEval(const MyVector<uint64_t>& q_limbs);
+    // Host copy of a switching key (tests and serialisation).
+    MyVector<uint64_t> downloadKey(const uint64_t* key) const;
+
+    // Encryption randomness: a fresh stream per call, derived from the seed.
+    uint64_t nextRandomnessSeed() const;
+
+private:
+    DeviceBuffer upload_key(const MyVector<uint64_t>& key) const;
+    DeviceBuffer generate_switch_key(const uint64_t* src_secret_qp, const uint64_t* tgt_secret_qp,
+                                     uint64_t stream_id) const;
+    DeviceBuffer secret_qp_from_coeffs(const MyVector<int8_t>& coeffs) const;
+
+    CKKSParams params;
+    mi_context* mi = nullptr;
+    MyVector<Complex> twiddle_fft, twiddle_ifft;
+    MyVector<size_t> perm_table;
+    MyVector<int8_t> secret_coeffs;
+    DeviceBuffer secret_qp;
+    size_t key_digits = 0;
+    DeviceBuffer relin_key, conj_key;
+    std::map<int64_t, DeviceBuffer> rotation_keys;
+    mutable std::atomic<uint64_t> randomness_counter{0};
+};
+
+// ------------------------------------------------------------ encode / crypt
+// ckks_encoding.hpp:15-77.
+CKKSEncoding ckks_encode(const CKKSContext& context, const MyVector<Complex>& values, size_t level,
+                         bool eval = false);
+CKKSEncoding ckks_encode(const CKKSContext& context, const MyVector<Complex>& values, double scale,
+                         size_t level, bool eval = false);
+CKKSEncoding ckks_encode_coefficients(const CKKSContext& context, const MyVector<Complex>& values,
+                                      double scale, size_t level, bool eval = false);
+// Integer plaintext of the slot values (before any NTT): limb-major residues.
+MyVector<uint64_t> encode_slots_custom_basis(size_t N, const MyVector<Complex>& values, double scale,
+                                             const MyVector<uint64_t>& moduli,
+                                             const MyVector<Complex>& twiddle_fft,
+                                             const MyVector<size_t>& perm_table);
+CKKSCiphertext ckks_encrypt(const CKKSContext& context, const MyVector<Complex>& values, size_t level,
+                            bool eval = false,
+                            CKKSMessageEncodingState state = CKKSMessageEncodingState::Slots);
+CKKSCiphertext ckks_encrypt(const CKKSContext& context, const CKKSEncoding& encd, bool eval = false,
+                            CKKSMessageEncodingState state = CKKSMessageEncodingState::Slots);
+MyVector<Complex> ckks_decrypt(const CKKSContext& context, const CKKSCiphertext& ct,
+                               CKKSDecryptDomain domain = CKKSDecryptDomain::Slots,
+                               bool use_low_secret = false);
+// Decode of a coefficient-domain integer plaintext (first limbs whose product
+// fits 128 bits, centred) at `scale` (ckks_encoding.cpp:75-170, 240-329).
+MyVector<Complex> ckks_decode_plaintext(const CKKSContext& context, const MyVector<uint64_t>& coeffs,
+                                        size_t limb_count, long double scale,
+                                        CKKSDecryptDomain domain = CKKSDecryptDomain::Slots);
+MyVector<Complex> decode_plaintext_custom_basis(size_t N, const MyVector<uint64_t>& coeffs,
+                                                const MyVector<uint64_t>& moduli, long double scale,
+                                                const MyVector<Complex>& twiddle_ifft,
+                                                const MyVector<size_t>& perm_table,
+                                                CKKSDecryptDomain domain = CKKSDecryptDomain::Slots);
+// Host tables (fft.cpp:37-80, ckks_context.cpp:56-68).
+MyVector<Complex> twiddle_ifft_generation(size_t N);
+MyVector<size_t> perm_table_generation(size_t N);
+
+// ----------------------------------------------------------------- evaluator
+// ckks_mul.hpp, ckks_rescale.hpp, ckks_addition.hpp, ckks_relin.hpp,
+// ckks_rotation.hpp, ckks_conjugate.hpp.
+void ckks_mul_inplace(const CKKSContext& context, CKKSCiphertext& ct1, const CKKSEncoding& pt);
+void ckks_mul_inplace(const CKKSContext& context, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+void ckks_mul_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t scalar);
+void ckks_mul_inplace(const CKKSContext& context, CKKSCiphertext& ct, Complex value,
+                      double plaintext_scale);
+CKKSCiphertext ckks_mul(const CKKSContext& context, const CKKSCiphertext& ct, Complex value,
+                        double plaintext_scale);
+void ckks_square_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+void ckks_rescale(const CKKSContext& context, CKKSCiphertext& ct);
+void ckks_drop_to_level_inplace(CKKSCiphertext& ct, size_t target_level);
+void ckks_add_inplace(const CKKSContext& context, CKKSCiphertext& ct1, const CKKSEncoding& pt);
+void ckks_add_inplace(const CKKSContext& context, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+void ckks_sub_inplace(const CKKSContext& context, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+void ckks_add_integer_constant_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t constant);
+void ckks_relin_hybrid_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+CKKSCiphertext ckks_rotate(const CKKSContext& context, const CKKSCiphertext& ct, int64_t rotation);
+void ckks_rotate_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t rotation);
+CKKSCiphertext ckks_conjugate(const CKKSContext& context, const CKKSCiphertext& ct);
+void ckks_conjugate_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+
+// Galois element of a slot rotation (rotation_key_generation.cpp:58-78).
+int64_t normalize_rotation_value(size_t N, int64_t rotation);
+uint64_t rotation_automorphism_index(size_t N, int64_t rotation);
+uint64_t conjugation_automorphism_index(size_t N);
+
+// Throws std::runtime_error (MI_EHIP / MI_ERUNTIME) or std::invalid_argument
+// (MI_EINVAL) with mi_last_error() for a failed C-ABI call.
+void mi_check(int rc, const char* fn);

@@ -1,0 +1,563 @@
+// Parameters, device storage, ciphertext types, context construction and key
+// generation of the host layer.
+//
+//   find_p_and_q_primes          random.cpp:66-131
+//   CKKSParams                   ckks_params.cpp:13-110 (plain chain:
+//                                q0_bit, then mult_depth x q_bit, ckks_params.cpp:213-269)
+//   auxiliary-basis check        ckks_context.cpp:114-155
+//   FFT twiddles, slot order     fft.cpp:37-80, ckks_context.cpp:56-68
+//   switching keys               relinearization_key_generation.cpp:130-199,
+//                                rotation_key_generation.cpp:90-199,
+//                                conjugation_key_generation.cpp:59-167,
+//                                hybrid_key_folding.hpp:50-77,
+//                                hybrid_key_montgomery.hpp:12-36
+// Key generation runs on the device through the C-ABI: the secret is
+// NTT'd over Q u P, and b = P*s_src (digit limbs) + e - a*s_tgt, folded and
+// put in Montgomery form with per-limb scalar products.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numbers>
+#include <set>
+
+#include "ckks.hpp"
+#include "ckks_rng.hpp"
+
+using ckks_host::GaussianSampler;
+using ckks_host::Rng;
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+uint64_t mulm(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((u128)a * b % m); }
+uint64_t powm(uint64_t a, uint64_t e, uint64_t m) {
+    uint64_t r = 1 % m;
+    a %= m;
+    while (e) {
+        if (e & 1) r = mulm(r, a, m);
+        a = mulm(a, a, m);
+        e >>= 1;
+    }
+    return r;
+}
+uint64_t invm(uint64_t a, uint64_t m) {
+    __int128 t = 0, nt = 1, r = m, nr = a % m;
+    while (nr) {
+        const __int128 q = r / nr;
+        __int128 tmp = t - q * nt;
+        t = nt;
+        nt = tmp;
+        tmp = r - q * nr;
+        r = nr;
+        nr = tmp;
+    }
+    if (r != 1) throw std::invalid_argument("ckks_context: modulus is not invertible");
+    if (t < 0) t += m;
+    return (uint64_t)t;
+}
+size_t bit_count(uint64_t m) { return m ? 64 - (size_t)__builtin_clzll(m) : 0; }
+
+bool is_prime(uint64_t n) {
+    if (n < 2) return false;
+    static const uint64_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    uint64_t d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        ++s;
+    }
+    for (uint64_t a : bases) {
+        if (n == a) return true;
+        if (n % a == 0) return false;
+        uint64_t x = powm(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool witness = true;
+        for (int r = 1; r < s && witness; ++r) {
+            x = mulm(x, x, n);
+            if (x == n - 1) witness = false;
+        }
+        if (witness) return false;
+    }
+    return true;
+}
+
+size_t reverse_bits(size_t v, size_t width) {
+    size_t r = 0;
+    for (size_t i = 0; i < width; ++i) {
+        r = (r << 1) | (v & 1);
+        v >>= 1;
+    }
+    return r;
+}
+
+size_t log2_exact(size_t v) {
+    size_t l = 0;
+    while ((size_t{1} << l) < v) ++l;
+    return l;
+}
+
+// s(X) -> s(X^g) on signed coefficients (rotation_key_generation.cpp:21-55)
+MyVector<int8_t> automorphism_coeffs(const MyVector<int8_t>& s, uint64_t g) {
+    const size_t N = s.size();
+    MyVector<int8_t> out(N, 0);
+    for (size_t i = 0; i < N; ++i) {
+        const size_t mapped = (size_t)((u128)i * g % (2 * N));
+        if (mapped >= N) out[mapped - N] = (int8_t)-s[i];
+        else out[mapped] = s[i];
+    }
+    return out;
+}
+
+}  // namespace
+
+void mi_check(int rc, const char* fn) {
+    if (rc == MI_OK) return;
+    const std::string msg = std::string(fn) + ": " + mi_last_error();
+    if (rc == MI_EINVAL) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+// ---------------------------------------------------------------- primes
+std::pair<std::vector<uint64_t>, std::vector<uint64_t>> find_p_and_q_primes(
+    const std::vector<uint64_t>& p_bits, const std::vector<uint64_t>& q_bits, size_t N) {
+    std::map<uint64_t, uint64_t> cursor;  // bit size -> next k
+    std::set<uint64_t> taken;
+    std::vector<uint64_t> p, q;
+    const size_t total = p_bits.size() + q_bits.size();
+    for (size_t idx = 0; idx < total; ++idx) {
+        const uint64_t bits = idx < p_bits.size() ? p_bits[idx] : q_bits[idx - p_bits.size()];
+        if (bits < 2 || bits > 62)
+            throw std::invalid_argument("find_p_and_q_primes: bit sizes must be in [2, 62]");
+        auto it = cursor.find(bits);
+        if (it == cursor.end()) it = cursor.emplace(bits, (uint64_t{1} << bits) / (2 * N)).first;
+        const uint64_t floor_k = (uint64_t{1} << (bits - 1)) / (2 * N);
+        uint64_t found = 0;
+        while (it->second-- > floor_k) {
+            const uint64_t cand = it->second * 2 * N + 1;
+            if (taken.count(cand)) continue;
+            if (is_prime(cand)) {
+                found = cand;
+                break;
+            }
+        }
+        if (!found) throw std::runtime_error("find_p_and_q_primes: not enough NTT-friendly primes");
+        taken.insert(found);
+        (idx < p_bits.size() ? p : q).push_back(found);
+    }
+    return {p, q};
+}
+
+// ---------------------------------------------------------------- params
+CKKSParams::CKKSParams(CKKSParamsConfig c) {
+    if (c.N == 0 || (c.N & (c.N - 1)))
+        throw std::invalid_argument("CKKSParams: N must be a nonzero power of two");
+    if (c.slots != 0 && c.slots * 2 != c.N)
+        throw std::invalid_argument("CKKSParams: slots must equal N / 2");
+    if (c.q_bits.empty() && c.q_bit == 0)
+        throw std::invalid_argument("CKKSParams: q_bit must be positive when q_bits is empty");
+    if (c.dnum == 0) throw std::invalid_argument("CKKSParams: dnum must be positive");
+    if (!c.q_bits.empty() && !c.q_mult_bits.empty())
+        throw std::invalid_argument("CKKSParams: q_bits and q_mult_bits are mutually exclusive");
+    if (!c.q_mult_bits.empty() && c.q_mult_bits.size() != c.mult_depth)
+        throw std::invalid_argument("CKKSParams: q_mult_bits size must equal mult_depth");
+    N = c.N;
+    slots = N / 2;
+    mult_depth = c.mult_depth;
+    dnum = c.dnum;
+    p_bits = c.p_bits.empty() ? std::vector<uint64_t>(c.pmoduli_count, (uint64_t)c.p_bit) : c.p_bits;
+    hamming_weight = c.secret_key_hamming_weight;
+    sigma = c.sigma;
+    scale = c.scale > 0.0 ? c.scale : std::exp2((double)c.q_bit);
+    rotate_values = c.rotate_values;
+    secure = c.secure;
+    fold_hybrid_keyswitch = c.fold_hybrid_keyswitch;
+    montgomery = c.montgomery;
+    seed = c.seed;
+    if (!c.q_bits.empty()) {
+        q_bits = c.q_bits;
+        depth = q_bits.size() - 1;
+        if (c.depth != 0 && c.depth != depth)
+            throw std::invalid_argument("CKKSParams: depth must match q_bits.size() - 1");
+    } else {
+        depth = c.depth == 0 ? mult_depth : c.depth;
+        if (depth != mult_depth)
+            throw std::runtime_error("CKKSParams: depth must equal mult_depth for a plain chain");
+        q_bits.push_back(c.q0_bit);
+        if (c.q_mult_bits.empty()) q_bits.insert(q_bits.end(), mult_depth, (uint64_t)c.q_bit);
+        else q_bits.insert(q_bits.end(), c.q_mult_bits.begin(), c.q_mult_bits.end());
+    }
+    if (p_bits.empty()) throw std::invalid_argument("CKKSParams: at least one P modulus is required");
+    auto [pp, qq] = find_p_and_q_primes(p_bits, q_bits, N);
+    moduli.assign(qq.begin(), qq.end());
+    pmoduli.assign(pp.begin(), pp.end());
+}
+
+// ---------------------------------------------------------------- storage
+DeviceBuffer::DeviceBuffer(mi_context* ctx, size_t words) : ctx_(ctx), words_(words) {
+    void* p = nullptr;
+    mi_check(mi_alloc(ctx, words * sizeof(uint64_t), &p), "DeviceBuffer");
+    ptr_ = (uint64_t*)p;
+}
+
+void DeviceBuffer::reset() {
+    if (ptr_) (void)mi_free(ctx_, ptr_);
+    ptr_ = nullptr;
+    words_ = 0;
+    ctx_ = nullptr;
+}
+
+static void upload(const CKKSContext& c, uint64_t* dst, const uint64_t* src, size_t words) {
+    mi_check(mi_memcpy(c.handle(), dst, src, words * 8, MI_COPY_H2D, c.stream()), "upload");
+}
+static void download(const CKKSContext& c, uint64_t* dst, const uint64_t* src, size_t words) {
+    mi_check(mi_memcpy(c.handle(), dst, src, words * 8, MI_COPY_D2H, c.stream()), "download");
+}
+static void copy_d2d(const CKKSContext& c, uint64_t* dst, const uint64_t* src, size_t words) {
+    mi_check(mi_memcpy(c.handle(), dst, src, words * 8, MI_COPY_D2D, c.stream()), "copy");
+}
+
+CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, size_t num_polys, long double s, size_t lv,
+                               CKKSSecretOwner o, CKKSMessageEncodingState st)
+    : ctx(&context), N(context.getParams().getN()), scale(s), level(lv), secret_owner(o),
+      message_encoding_state(st) {
+    if (lv > context.getParams().getMaxLevel())
+        throw std::invalid_argument("CKKSCiphertext: invalid level");
+    for (size_t i = 0; i < num_polys; ++i) polys.emplace_back(context.handle(), (lv + 1) * N);
+}
+
+CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uint64_t>> host,
+                               long double s, size_t lv, CKKSSecretOwner o, CKKSMessageEncodingState st)
+    : CKKSCiphertext(context, host.size(), s, lv, o, st) {
+    for (size_t i = 0; i < host.size(); ++i) setPoly(i, host[i]);
+}
+
+CKKSCiphertext CKKSCiphertext::clone() const {
+    CKKSCiphertext out(*ctx, polys.size(), scale, level, secret_owner, message_encoding_state);
+    for (size_t i = 0; i < polys.size(); ++i) copy_d2d(*ctx, out.polys[i].data(), polys[i].data(), (level + 1) * N);
+    return out;
+}
+
+void CKKSCiphertext::setLevel(size_t l) {
+    for (auto& p : polys)
+        if ((l + 1) * N > p.words()) throw std::invalid_argument("CKKSCiphertext: cannot raise level in place");
+    level = l;
+}
+
+MyVector<uint64_t> CKKSCiphertext::getPoly(size_t idx) const {
+    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
+    MyVector<uint64_t> h((level + 1) * N);
+    download(*ctx, h.data(), polys[idx].data(), h.size());
+    return h;
+}
+
+void CKKSCiphertext::setPoly(size_t idx, const MyVector<uint64_t>& host) {
+    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
+    if (host.size() != (level + 1) * N) throw std::invalid_argument("CKKSCiphertext: poly size mismatch");
+    upload(*ctx, polys[idx].data(), host.data(), host.size());
+}
+
+uint64_t* CKKSCiphertext::polyData(size_t idx) {
+    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
+    return polys[idx].data();
+}
+const uint64_t* CKKSCiphertext::polyData(size_t idx) const {
+    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
+    return polys[idx].data();
+}
+
+void CKKSCiphertext::addPoly() { polys.emplace_back(ctx->handle(), (level + 1) * N); }
+
+void CKKSCiphertext::resizePolys(size_t count) {
+    while (polys.size() > count) polys.pop_back();
+    while (polys.size() < count) addPoly();
+}
+
+CKKSEncoding::CKKSEncoding(const CKKSContext& context, MyVector<uint64_t> pt, long double s, size_t lv,
+                           bool mont)
+    : ctx(&context), N(context.getParams().getN()), scale(s), level(lv), montgomery_form(mont),
+      buf(context.handle(), (lv + 1) * context.getParams().getN()) {
+    if (pt.size() != (lv + 1) * N) throw std::invalid_argument("CKKSEncoding: invalid plaintext buffer size");
+    upload(context, buf.data(), pt.data(), pt.size());
+}
+
+CKKSEncoding::CKKSEncoding(const CKKSContext& context, DeviceBuffer pt, long double s, size_t lv, bool mont)
+    : ctx(&context), N(context.getParams().getN()), scale(s), level(lv), montgomery_form(mont),
+      buf(std::move(pt)) {
+    if (buf.words() < (lv + 1) * N) throw std::invalid_argument("CKKSEncoding: invalid plaintext buffer size");
+}
+
+MyVector<uint64_t> CKKSEncoding::getPlaintext() const {
+    MyVector<uint64_t> h((level + 1) * N);
+    download(*ctx, h.data(), buf.data(), h.size());
+    return h;
+}
+
+// ---------------------------------------------------------------- context
+int64_t normalize_rotation_value(size_t N, int64_t rotation) {
+    const int64_t slots = (int64_t)(N >> 1);
+    if (slots == 0) throw std::invalid_argument("ckks_context: invalid slot count");
+    int64_t r = rotation % slots;
+    if (r < 0) r += slots;
+    return r;
+}
+
+uint64_t rotation_automorphism_index(size_t N, int64_t rotation) {
+    return powm(5, (uint64_t)normalize_rotation_value(N, rotation), 2 * N);
+}
+
+uint64_t conjugation_automorphism_index(size_t N) { return 2 * N - 1; }
+
+// Stage twiddles of the special FFT, bit-reversed per stage, with the
+// embedding twist psi^k merged into stage k.
+MyVector<Complex> twiddle_ifft_generation(size_t N) {
+    MyVector<Complex> tw(N, Complex(0.0, 0.0));
+    const double ang = 2.0 * std::numbers::pi / (double)N;
+    const Complex w_inv(std::cos(ang), std::sin(ang));
+    MyVector<Complex> stage(N >> 1);
+    stage[0] = Complex(1.0, 0.0);
+    for (size_t i = 1; i < (N >> 1); ++i) stage[i] = stage[i - 1] * w_inv;
+    for (size_t m = 1; m < N; m <<= 1) {
+        const size_t step = N / (m << 1);
+        const size_t bits = log2_exact(m);
+        for (size_t i = 0; i < m; ++i) tw[m + i] = stage[reverse_bits(i, bits) * step];
+    }
+    const double pang = std::numbers::pi / (double)N;
+    const Complex psi(std::cos(pang), std::sin(pang));
+    MyVector<Complex> psi_pows(N);
+    psi_pows[0] = Complex(1.0, 0.0);
+    for (size_t j = 1; j < N; ++j) psi_pows[j] = psi_pows[j - 1] * psi;
+    for (size_t m = N >> 1, k = 1; m > 0; m >>= 1, k <<= 1)
+        for (size_t i = 0; i < m; ++i) tw[m + i] = tw[m + i] * psi_pows[k];
+    return tw;
+}
+
+// Slot i sits at evaluation index brev((5^i mod 2N - 1) / 2).
+MyVector<size_t> perm_table_generation(size_t N) {
+    const size_t logN = log2_exact(N);
+    MyVector<size_t> perm(N >> 1);
+    uint64_t cur = 1;
+    for (size_t i = 0; i < (N >> 1); ++i) {
+        perm[i] = reverse_bits((size_t)((cur - 1) >> 1), logN);
+        cur = (cur * 5) % (2 * N);
+    }
+    return perm;
+}
+
+CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
+    const size_t N = params.getN();
+    const auto& q = params.getModuli();
+    const auto& pm = params.getPModuli();
+    const size_t nq = q.size(), dnum = params.getDnum();
+    if (dnum == 0 || dnum > nq)
+        throw std::invalid_argument("ckks_context: invalid hybrid key-switch partition count");
+    const size_t alpha = (nq + dnum - 1) / dnum;
+    size_t p_bits = 0, widest = 0;
+    for (uint64_t v : pm) p_bits += bit_count(v);
+    for (size_t s0 = 0; s0 < nq; s0 += alpha) {
+        size_t b = 0;
+        for (size_t i = s0; i < std::min(s0 + alpha, nq); ++i) b += bit_count(q[i]);
+        widest = std::max(widest, b);
+    }
+    if (p_bits < widest)
+        throw std::invalid_argument("ckks_context: hybrid key-switch P basis is smaller than the widest Q digit; dnum = " +
+                                    std::to_string(dnum) + ", P bits = " + std::to_string(p_bits) +
+                                    ", widest Q digit bits = " + std::to_string(widest) +
+                                    "; add P limbs or increase dnum");
+    key_digits = std::min(dnum, (nq + alpha - 1) / alpha);
+
+    twiddle_ifft = twiddle_ifft_generation(N);
+    twiddle_fft.resize(N);
+    for (size_t i = 0; i < N; ++i) twiddle_fft[i] = std::conj(twiddle_ifft[i]);
+    perm_table = perm_table_generation(N);
+
+    mi_check(mi_context_create(&mi, N, q.data(), (int)nq, pm.data(), (int)pm.size(), (int)dnum),
+             "ckks_context");
+    try {
+        Rng rng(params.getSeed());
+        secret_coeffs = ckks_host::hamming_weight_ternary(N, params.getSecretKeyHammingWeight(), rng);
+        secret_qp = secret_qp_from_coeffs(secret_coeffs);
+        const size_t KL = nq + pm.size();
+        // relinearisation: s^2 -> s
+        DeviceBuffer s2(mi, KL * N);
+        copy_d2d(*this, s2.data(), secret_qp.data(), KL * N);
+        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), secret_qp.data(), (int)KL, 1, (int64_t)(KL * N), stream()),
+                 "ckks_context");
+        relin_key = generate_switch_key(s2.data(), secret_qp.data(), 1);
+        // conjugation: conj(s) -> s
+        DeviceBuffer sc = secret_qp_from_coeffs(automorphism_coeffs(secret_coeffs, conjugation_automorphism_index(N)));
+        conj_key = generate_switch_key(sc.data(), secret_qp.data(), 2);
+        for (int64_t r : params.getRotateValues()) {
+            const int64_t n = normalize_rotation_value(N, r);
+            if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
+        }
+    } catch (...) {
+        relin_key.reset();
+        conj_key.reset();
+        rotation_keys.clear();
+        secret_qp.reset();
+        mi_context_destroy(mi);
+        throw;
+    }
+}
+
+CKKSContext::~CKKSContext() {
+    relin_key.reset();
+    conj_key.reset();
+    rotation_keys.clear();
+    secret_qp.reset();
+    if (mi) mi_context_destroy(mi);
+}
+
+DeviceBuffer CKKSContext::secret_qp_from_coeffs(const MyVector<int8_t>& coeffs) const {
+    const size_t N = params.getN();
+    std::vector<uint64_t> basis(params.getModuli().begin(), params.getModuli().end());
+    basis.insert(basis.end(), params.getPModuli().begin(), params.getPModuli().end());
+    const size_t KL = basis.size();
+    std::vector<uint64_t> h(KL * N);
+    for (size_t l = 0; l < KL; ++l)
+        for (size_t i = 0; i < N; ++i) h[l * N + i] = coeffs[i] < 0 ? basis[l] - (uint64_t)(-coeffs[i]) : (uint64_t)coeffs[i];
+    DeviceBuffer d(mi, KL * N);
+    upload(*this, d.data(), h.data(), h.size());
+    mi_check(mi_ntt_forward(mi, d.data(), 1, (int)KL, (int64_t)(KL * N), 0, stream()), "ckks_context");
+    return d;
+}
+
+size_t CKKSContext::switchKeyWords() const {
+    return key_digits * 2 * (params.getModuli().size() + params.getPModuli().size()) * params.getN();
+}
+
+DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_t* tgt, uint64_t stream_id) const {
+    const size_t N = params.getN();
+    const auto& q = params.getModuli();
+    const auto& pm = params.getPModuli();
+    const size_t nq = q.size(), np = pm.size(), KL = nq + np;
+    const size_t alpha = params.getAlpha();
+    std::vector<uint64_t> basis(q.begin(), q.end());
+    basis.insert(basis.end(), pm.begin(), pm.end());
+    std::vector<uint64_t> p_mod(KL), scale(KL);
+    for (size_t l = 0; l < KL; ++l) {
+        uint64_t acc = 1;
+        for (uint64_t pv : pm) acc = mulm(acc, pv % basis[l], basis[l]);
+        p_mod[l] = acc;
+        uint64_t f = 1;
+        if (params.fold_hybrid_keyswitch) {
+            if (l < nq) {
+                f = invm(acc, basis[l]);
+            } else {
+                uint64_t ph = 1;
+                for (size_t j = 0; j < np; ++j)
+                    if (j != l - nq) ph = mulm(ph, pm[j] % basis[l], basis[l]);
+                f = invm(ph, basis[l]);
+            }
+        }
+        if (params.montgomery) f = mulm(f, (uint64_t)(((u128)1 << 64) % basis[l]), basis[l]);
+        scale[l] = f;
+    }
+    uint64_t st = params.getSeed() ^ (0xA5A5A5A5ull * (stream_id + 1));
+    Rng rng(ckks_host::splitmix64(st));
+    GaussianSampler gauss(params.getSigma());
+    DeviceBuffer key(mi, switchKeyWords());
+    DeviceBuffer tmp(mi, KL * N);
+    std::vector<uint64_t> h(KL * N);
+    const int64_t W = (int64_t)(KL * N);
+    for (size_t d = 0; d < key_digits; ++d) {
+        uint64_t* b = key.data() + (2 * d) * KL * N;
+        uint64_t* a = b + KL * N;
+        copy_d2d(*this, b, src, KL * N);
+        std::vector<uint64_t> r(KL, 0);
+        for (size_t l = d * alpha; l < std::min(d * alpha + alpha, nq); ++l) r[l] = p_mod[l];
+        mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, r.data(), (int)KL, 1, W, stream()), "ckks_context");
+        for (size_t l = 0; l < KL; ++l)
+            for (size_t i = 0; i < N; ++i) h[l * N + i] = rng.below(basis[l]);
+        upload(*this, a, h.data(), h.size());
+        for (size_t i = 0; i < N; ++i) {
+            const int64_t e = gauss.sample(rng);
+            for (size_t l = 0; l < KL; ++l) h[l * N + i] = e < 0 ? basis[l] - (uint64_t)(-e) : (uint64_t)e;
+        }
+        upload(*this, tmp.data(), h.data(), h.size());
+        mi_check(mi_ntt_forward(mi, tmp.data(), 1, (int)KL, W, 0, stream()), "ckks_context");
+        mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)KL, 1, W, stream()), "ckks_context");
+        copy_d2d(*this, tmp.data(), a, KL * N);
+        mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), tgt, (int)KL, 1, W, stream()), "ckks_context");
+        mi_check(mi_poly_op(mi, MI_OP_SUB, b, tmp.data(), (int)KL, 1, W, stream()), "ckks_context");
+        if (params.fold_hybrid_keyswitch || params.montgomery)
+            mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, scale.data(), (int)KL, 2, W, stream()), "ckks_context");
+    }
+    mi_check(mi_stream_synchronize(mi, stream()), "ckks_context");
+    return key;
+}
+
+void CKKSContext::addRotationKey(int64_t rotation) {
+    const size_t N = params.getN();
+    const int64_t r = normalize_rotation_value(N, rotation);
+    if (r == 0 || rotation_keys.count(r)) return;
+    // keys switch s to pi^-1(s); the rotation applies pi after the switch
+    DeviceBuffer tgt = secret_qp_from_coeffs(automorphism_coeffs(secret_coeffs, rotation_automorphism_index(N, -r)));
+    rotation_keys.emplace(r, generate_switch_key(secret_qp.data(), tgt.data(), 1000 + (uint64_t)r));
+}
+
+DeviceBuffer CKKSContext::upload_key(const MyVector<uint64_t>& key) const {
+    if (key.size() != switchKeyWords())
+        throw std::invalid_argument("ckks_context: switching key size mismatch (expected " +
+                                    std::to_string(switchKeyWords()) + " words)");
+    DeviceBuffer d(mi, key.size());
+    upload(*this, d.data(), key.data(), key.size());
+    return d;
+}
+
+void CKKSContext::setRelinKey(const MyVector<uint64_t>& key) { relin_key = upload_key(key); }
+void CKKSContext::setConjugationKey(const MyVector<uint64_t>& key) { conj_key = upload_key(key); }
+void CKKSContext::setRotationKey(int64_t rotation, const MyVector<uint64_t>& key) {
+    const int64_t r = normalize_rotation_value(params.getN(), rotation);
+    if (r == 0) throw std::invalid_argument("ckks_context: rotation by 0 needs no key");
+    rotation_keys[r] = upload_key(key);
+}
+void CKKSContext::setSecretKeyEval(const MyVector<uint64_t>& q_limbs) {
+    const size_t N = params.getN(), nq = params.getModuli().size();
+    if (q_limbs.size() != nq * N) throw std::invalid_argument("ckks_context: secret key size mismatch");
+    upload(*this, secret_qp.data(), q_limbs.data(), q_limbs.size());
+}
+
+bool CKKSContext::hasRotationKey(int64_t rotation) const {
+    return rotation_keys.count(normalize_rotation_value(params.getN(), rotation)) != 0;
+}
+
+const uint64_t* CKKSContext::getRotationKeyDevice(int64_t rotation) const {
+    auto it = rotation_keys.find(normalize_rotation_value(params.getN(), rotation));
+    if (it == rotation_keys.end())
+        throw std::runtime_error("ckks_context: rotation key is unavailable for rotation " + std::to_string(rotation));
+    return it->second.data();
+}
+
+std::vector<int64_t> CKKSContext::getRotationKeyValues() const {
+    std::vector<int64_t> v;
+    for (auto& kv : rotation_keys) v.push_back(kv.first);
+    return v;
+}
+
+const uint64_t* CKKSContext::getConjugationKeyDevice() const {
+    if (!conj_key.data()) throw std::runtime_error("ckks_context: conjugation key is unavailable");
+    return conj_key.data();
+}
+
+MyVector<MyVector<uint64_t>> CKKSContext::getSecretKey() const {
+    const size_t N = params.getN(), nq = params.getModuli().size();
+    MyVector<uint64_t> all(nq * N);
+    download(*this, all.data(), secret_qp.data(), all.size());
+    MyVector<MyVector<uint64_t>> out(nq);
+    for (size_t l = 0; l < nq; ++l) out[l].assign(all.begin() + l * N, all.begin() + (l + 1) * N);
+    return out;
+}
+
+MyVector<uint64_t> CKKSContext::downloadKey(const uint64_t* key) const {
+    MyVector<uint64_t> h(switchKeyWords());
+    download(*this, h.data(), key, h.size());
+    return h;
+}
+
+uint64_t CKKSContext::nextRandomnessSeed() const {
+    uint64_t st = params.getSeed() ^ 0x3C6EF372FE94F82Bull;
+    st += 0x9E3779B97F4A7C15ull * (randomness_counter.fetch_add(1) + 1);
+    return ckks_host::splitmix64(st);
+}

@@ -1,0 +1,216 @@
+// Evaluator free functions over device-resident ciphertexts: metadata checks
+// and scale bookkeeping of the reference (ckks_mul.cpp:27-288,
+// ckks_rescale.cpp:42-148, ckks_addition.cpp:17-139, ckks_relin.cpp:8-44,
+// ckks_rotation.cpp:34-96, ckks_conjugate.cpp:10-61) around the C-ABI kernels.
+#include <cmath>
+
+#include "ckks.hpp"
+
+namespace {
+
+void check_ct(const CKKSContext& c, const CKKSCiphertext& ct, const char* fn) {
+    const auto& P = c.getParams();
+    if (ct.getN() != P.getN()) throw std::invalid_argument(std::string(fn) + ": ciphertext/context size mismatch");
+    if (ct.getLevel() > P.getMaxLevel()) throw std::invalid_argument(std::string(fn) + ": invalid level");
+    if (&ct.getContext() != &c) throw std::invalid_argument(std::string(fn) + ": ciphertext belongs to another context");
+}
+
+int64_t words(const CKKSContext& c, size_t level) { return (int64_t)((level + 1) * c.getParams().getN()); }
+
+uint64_t reduce_i64(int64_t v, uint64_t q) {
+    const int64_t qi = (int64_t)q;
+    const int64_t r = v % qi;
+    return (uint64_t)(r < 0 ? r + qi : r);
+}
+
+// rounded_scaled_integer_mod_q (ckks_addition.cpp:17-31): round(v * scale)
+// mod q in long double.
+uint64_t rounded_scaled_integer_mod_q(int64_t value, long double scale, uint64_t q) {
+    const long double scaled = (long double)value * scale;
+    if (!std::isfinite(scaled))
+        throw std::invalid_argument("ckks_add_integer_constant_inplace: non-finite scaled constant");
+    long double r = std::fmod(std::round(scaled), (long double)q);
+    if (r < 0.0L) r += (long double)q;
+    uint64_t out = (uint64_t)r;
+    if (out >= q) out %= q;
+    return out;
+}
+
+}  // namespace
+
+void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncoding& pt) {
+    check_ct(c, ct, "ckks_mul_inplace");
+    if (pt.getN() != ct.getN() || pt.getLevel() != ct.getLevel())
+        throw std::invalid_argument("ckks_mul_inplace: ciphertext/plaintext size mismatch");
+    if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_mul_inplace: empty ciphertext");
+    const size_t lv = ct.getLevel();
+    for (size_t i = 0; i < ct.getNumPolys(); ++i)
+        mi_check(mi_mul_plain(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), pt.data(),
+                              pt.isMontgomeryForm() ? 1 : 0, c.stream()),
+                 "ckks_mul_inplace");
+    ct.setScale(ct.getScale() * pt.getScale());
+}
+
+void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {
+    const size_t lv = ct1.getLevel();
+    if (ct1.getN() != ct2.getN() || lv != ct2.getLevel())
+        throw std::invalid_argument("ckks_mul_inplace(ct,ct): ciphertext size mismatch");
+    check_ct(c, ct1, "ckks_mul_inplace(ct,ct)");
+    if (ct1.getNumPolys() != 2 || ct2.getNumPolys() != 2)
+        throw std::invalid_argument("ckks_mul_inplace(ct,ct): expected 2-component inputs");
+    if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
+        ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
+        throw std::invalid_argument("ckks_mul_inplace(ct,ct): ciphertext metadata mismatch");
+    ct1.addPoly();
+    mi_check(mi_tensor(c.handle(), (int)lv, ct1.polyData(0), ct1.polyData(1), ct2.polyData(0), ct2.polyData(1),
+                       ct1.polyData(2), c.stream()),
+             "ckks_mul_inplace(ct,ct)");
+    ct1.setScale(ct1.getScale() * ct2.getScale());
+}
+
+void ckks_square_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
+    check_ct(c, ct, "ckks_square_inplace");
+    if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_square_inplace: expected 2-component input");
+    ct.addPoly();
+    mi_check(mi_square(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2), c.stream()),
+             "ckks_square_inplace");
+    ct.setScale(ct.getScale() * ct.getScale());
+}
+
+void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t scalar) {
+    check_ct(c, ct, "ckks_mul_inplace(ct,int)");
+    if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_mul_inplace(ct,int): empty ciphertext");
+    if (scalar == 1) return;
+    const size_t lv = ct.getLevel();
+    const auto& q = c.getParams().getModuli();
+    std::vector<uint64_t> r(lv + 1);
+    for (size_t l = 0; l <= lv; ++l) r[l] = reduce_i64(scalar, q[l]);
+    for (size_t i = 0; i < ct.getNumPolys(); ++i)
+        mi_check(mi_mul_scalar(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), r.data(), c.stream()),
+                 "ckks_mul_inplace(ct,int)");
+}
+
+void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, Complex value, double plaintext_scale) {
+    MyVector<Complex> slots(c.getParams().getSlots(), value);
+    CKKSEncoding pt = ckks_encode(c, slots, plaintext_scale, ct.getLevel(), /*eval=*/true);
+    ckks_mul_inplace(c, ct, pt);
+}
+
+CKKSCiphertext ckks_mul(const CKKSContext& c, const CKKSCiphertext& ct, Complex value, double plaintext_scale) {
+    CKKSCiphertext out = ct.clone();
+    ckks_mul_inplace(c, out, value, plaintext_scale);
+    return out;
+}
+
+void ckks_rescale(const CKKSContext& c, CKKSCiphertext& ct) {
+    const size_t lv = ct.getLevel();
+    if (lv == 0 || lv > c.getParams().getMaxLevel()) throw std::invalid_argument("ckks_rescale: invalid level");
+    check_ct(c, ct, "ckks_rescale");
+    if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_rescale: empty ciphertext");
+    for (size_t i = 0; i < ct.getNumPolys(); ++i)
+        mi_check(mi_rescale(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), c.stream()), "ckks_rescale");
+    const uint64_t p = c.getParams().getModuli()[lv];
+    ct.setScale(ct.getScale() / (long double)p);
+    ct.setLevel(lv - 1);
+}
+
+void ckks_drop_to_level_inplace(CKKSCiphertext& ct, size_t target_level) {
+    if (target_level > ct.getLevel())
+        throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
+    if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_drop_to_level_inplace: empty ciphertext");
+    ct.setLevel(target_level);
+}
+
+void ckks_add_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncoding& pt) {
+    check_ct(c, ct, "ckks_add_inplace(ct,pt)");
+    if (pt.getN() != ct.getN() || pt.getLevel() != ct.getLevel())
+        throw std::invalid_argument("ckks_add_inplace(ct,pt): ciphertext/plaintext size mismatch");
+    mi_check(mi_add_plain(c.handle(), (int)ct.getLevel(), ct.polyData(0), pt.data(), pt.isMontgomeryForm() ? 1 : 0,
+                          c.stream()),
+             "ckks_add_inplace(ct,pt)");
+}
+
+static void add_or_sub(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2, bool sub,
+                       const char* fn) {
+    check_ct(c, ct1, fn);
+    if (ct1.getN() != ct2.getN() || ct1.getLevel() != ct2.getLevel() || ct1.getNumPolys() != ct2.getNumPolys())
+        throw std::invalid_argument(std::string(fn) + ": ciphertext size mismatch");
+    if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
+        ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
+        throw std::invalid_argument(std::string(fn) + ": ciphertext metadata mismatch");
+    const size_t lv = ct1.getLevel();
+    for (size_t i = 0; i < ct1.getNumPolys(); ++i) {
+        const int rc = sub ? mi_sub(c.handle(), (int)lv, ct1.polyData(i), ct2.polyData(i), 1, words(c, lv), c.stream())
+                           : mi_add(c.handle(), (int)lv, ct1.polyData(i), ct2.polyData(i), 1, words(c, lv), c.stream());
+        mi_check(rc, fn);
+    }
+}
+
+void ckks_add_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {
+    add_or_sub(c, ct1, ct2, false, "ckks_add_inplace(ct,ct)");
+}
+
+void ckks_sub_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {
+    add_or_sub(c, ct1, ct2, true, "ckks_sub_inplace(ct,ct)");
+}
+
+void ckks_add_integer_constant_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t constant) {
+    if (constant == 0) return;
+    check_ct(c, ct, "ckks_add_integer_constant_inplace");
+    const size_t lv = ct.getLevel();
+    const auto& q = c.getParams().getModuli();
+    std::vector<uint64_t> r(lv + 1);
+    for (size_t l = 0; l <= lv; ++l) r[l] = rounded_scaled_integer_mod_q(constant, ct.getScale(), q[l]);
+    mi_check(mi_add_scalar(c.handle(), (int)lv, ct.polyData(0), r.data(), c.stream()),
+             "ckks_add_integer_constant_inplace");
+}
+
+void ckks_relin_hybrid_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
+    check_ct(c, ct, "ckks_relin");
+    if (ct.getNumPolys() != 3) throw std::invalid_argument("ckks_relin: expected 3-polynomial ciphertext");
+    if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_relin: sparse-secret relinearization keys are not available");
+    mi_check(mi_relin(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2),
+                      c.getRelinKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+             "ckks_relin");
+    ct.resizePolys(2);
+}
+
+void ckks_rotate_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t rotation) {
+    const size_t N = c.getParams().getN();
+    const int64_t r = normalize_rotation_value(N, rotation);
+    if (r == 0) return;
+    if (!c.hasRotationKey(r))
+        throw std::invalid_argument("ckks_rotate: rotation key is unavailable for the requested rotation");
+    if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_rotate: expected a 2-polynomial ciphertext");
+    check_ct(c, ct, "ckks_rotate");
+    if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_rotate: sparse-secret rotation keys are not available");
+    mi_check(mi_rotate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
+                       rotation_automorphism_index(N, r), c.getRotationKeyDevice(r),
+                       c.getParams().montgomery ? 1 : 0, c.stream()),
+             "ckks_rotate");
+}
+
+CKKSCiphertext ckks_rotate(const CKKSContext& c, const CKKSCiphertext& ct, int64_t rotation) {
+    CKKSCiphertext out = ct.clone();
+    ckks_rotate_inplace(c, out, rotation);
+    return out;
+}
+
+void ckks_conjugate_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
+    if (!c.hasConjugationKey()) throw std::invalid_argument("ckks_conjugate: conjugation key is unavailable");
+    if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_conjugate: expected a 2-polynomial ciphertext");
+    check_ct(c, ct, "ckks_conjugate");
+    if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_conjugate: sparse-secret conjugation keys are not available");
+    mi_check(mi_conjugate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
+                          c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+             "ckks_conjugate");
+}
+
+CKKSCiphertext ckks_conjugate(const CKKSContext& c, const CKKSCiphertext& ct) {
+    CKKSCiphertext out = ct.clone();
+    ckks_conjugate_inplace(c, out);
+    return out;
+}

@@ -1,0 +1,231 @@
+// Python binding of the C++ host API (tests and the benchmark drive the
+// library through it).  Device-resident objects stay on the device; polys,
+// plaintexts and keys cross into numpy only through explicit downloads.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "ckks.hpp"
+
+namespace py = pybind11;
+
+namespace {
+
+py::array_t<uint64_t> to_np(const std::vector<uint64_t>& v) {
+    py::array_t<uint64_t> a(v.size());
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+std::vector<uint64_t> from_np(py::array_t<uint64_t, py::array::c_style | py::array::forcecast> a) {
+    return std::vector<uint64_t>(a.data(), a.data() + a.size());
+}
+
+MyVector<Complex> complex_vec(py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> a) {
+    return MyVector<Complex>(a.data(), a.data() + a.size());
+}
+
+py::array_t<std::complex<double>> complex_np(const MyVector<Complex>& v) {
+    py::array_t<std::complex<double>> a(v.size());
+    std::copy(v.begin(), v.end(), a.mutable_data());
+    return a;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(ckks_py, m) {
+    m.doc() = "C++ host API of the MI355X CKKS library";
+
+    py::enum_<CKKSSecretOwner>(m, "CKKSSecretOwner")
+        .value("Dense", CKKSSecretOwner::Dense)
+        .value("BootstrapSparse", CKKSSecretOwner::BootstrapSparse);
+    py::enum_<CKKSMessageEncodingState>(m, "CKKSMessageEncodingState")
+        .value("Slots", CKKSMessageEncodingState::Slots)
+        .value("Coefficients", CKKSMessageEncodingState::Coefficients);
+    py::enum_<CKKSDecryptDomain>(m, "CKKSDecryptDomain")
+        .value("Slots", CKKSDecryptDomain::Slots)
+        .value("Coefficients", CKKSDecryptDomain::Coefficients);
+
+    py::class_<CKKSParamsConfig>(m, "CKKSParamsConfig")
+        .def(py::init<>())
+        .def_readwrite("scale", &CKKSParamsConfig::scale)
+        .def_readwrite("sigma", &CKKSParamsConfig::sigma)
+        .def_readwrite("N", &CKKSParamsConfig::N)
+        .def_readwrite("mult_depth", &CKKSParamsConfig::mult_depth)
+        .def_readwrite("q0_bit", &CKKSParamsConfig::q0_bit)
+        .def_readwrite("q_bit", &CKKSParamsConfig::q_bit)
+        .def_readwrite("q_mult_bits", &CKKSParamsConfig::q_mult_bits)
+        .def_readwrite("dnum", &CKKSParamsConfig::dnum)
+        .def_readwrite("p_bit", &CKKSParamsConfig::p_bit)
+        .def_readwrite("pmoduli_count", &CKKSParamsConfig::pmoduli_count)
+        .def_readwrite("p_bits", &CKKSParamsConfig::p_bits)
+        .def_readwrite("secret_key_hamming_weight", &CKKSParamsConfig::secret_key_hamming_weight)
+        .def_readwrite("q_bits", &CKKSParamsConfig::q_bits)
+        .def_readwrite("rotate_values", &CKKSParamsConfig::rotate_values)
+        .def_readwrite("secure", &CKKSParamsConfig::secure)
+        .def_readwrite("fold_hybrid_keyswitch", &CKKSParamsConfig::fold_hybrid_keyswitch)
+        .def_readwrite("montgomery", &CKKSParamsConfig::montgomery)
+        .def_readwrite("seed", &CKKSParamsConfig::seed);
+
+    py::class_<CKKSParams>(m, "CKKSParams")
+        .def(py::init<CKKSParamsConfig>())
+        .def("getN", &CKKSParams::getN)
+        .def("getSlots", &CKKSParams::getSlots)
+        .def("getMaxLevel", &CKKSParams::getMaxLevel)
+        .def("getDnum", &CKKSParams::getDnum)
+        .def("getAlpha", &CKKSParams::getAlpha)
+        .def("getScale", &CKKSParams::getScale)
+        .def("getSigma", &CKKSParams::getSigma)
+        .def("getModuli", [](const CKKSParams& p) { return to_np(p.getModuli()); })
+        .def("getPModuli", [](const CKKSParams& p) { return to_np(p.getPModuli()); })
+        .def_readonly("montgomery", &CKKSParams::montgomery)
+        .def_readonly("fold_hybrid_keyswitch", &CKKSParams::fold_hybrid_keyswitch);
+
+    m.def("find_p_and_q_primes", &find_p_and_q_primes);
+
+    py::class_<CKKSContext>(m, "CKKSContext")
+        .def(py::init<CKKSParams>())
+        .def("getParams", &CKKSContext::getParams, py::return_value_policy::reference_internal)
+        .def("getSecretKeyCoeffs", [](const CKKSContext& c) {
+            const auto& s = c.getSecretKeyCoeffs();
+            py::array_t<int8_t> a(s.size());
+            std::copy(s.begin(), s.end(), a.mutable_data());
+            return a;
+        })
+        .def("getSecretKey", [](const CKKSContext& c) {
+            std::vector<uint64_t> flat;
+            for (auto& l : c.getSecretKey()) flat.insert(flat.end(), l.begin(), l.end());
+            return to_np(flat);
+        })
+        .def("getRelinKey", [](const CKKSContext& c) { return to_np(c.downloadKey(c.getRelinKeyDevice())); })
+        .def("getConjugationKey", [](const CKKSContext& c) { return to_np(c.downloadKey(c.getConjugationKeyDevice())); })
+        .def("getRotationKey", [](const CKKSContext& c, int64_t r) { return to_np(c.downloadKey(c.getRotationKeyDevice(r))); })
+        .def("getRotationKeyValues", &CKKSContext::getRotationKeyValues)
+        .def("getRotationKeyCount", &CKKSContext::getRotationKeyCount)
+        .def("hasRotationKey", &CKKSContext::hasRotationKey)
+        .def("addRotationKey", &CKKSContext::addRotationKey)
+        .def("keyDigitCount", &CKKSContext::keyDigitCount)
+        .def("getPermTable", [](const CKKSContext& c) { return std::vector<size_t>(c.getPermTable()); })
+        .def("setRelinKey", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setRelinKey(from_np(k)); })
+        .def("setConjugationKey", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setConjugationKey(from_np(k)); })
+        .def("setRotationKey", [](CKKSContext& c, int64_t r, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setRotationKey(r, from_np(k)); })
+        .def("setSecretKeyEval", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setSecretKeyEval(from_np(k)); })
+        .def("memoryStats", [](const CKKSContext& c) {
+            int64_t st[4];
+            mi_check(mi_memory_stats(c.handle(), st), "memoryStats");
+            return std::vector<int64_t>(st, st + 4);
+        });
+
+    py::class_<CKKSCiphertext>(m, "CKKSCiphertext")
+        .def(py::init([](const CKKSContext& c, std::vector<py::array_t<uint64_t, py::array::c_style | py::array::forcecast>> polys,
+                         long double scale, size_t level) {
+                 MyVector<MyVector<uint64_t>> h;
+                 for (auto& p : polys) h.push_back(from_np(p));
+                 return CKKSCiphertext(c, std::move(h), scale, level);
+             }),
+             py::keep_alive<1, 2>())
+        .def("clone", &CKKSCiphertext::clone, py::keep_alive<0, 1>())
+        .def("getScale", [](const CKKSCiphertext& ct) { return (double)ct.getScale(); })
+        .def("getLevel", &CKKSCiphertext::getLevel)
+        .def("getN", &CKKSCiphertext::getN)
+        .def("getNumPolys", &CKKSCiphertext::getNumPolys)
+        .def("getSecretOwner", &CKKSCiphertext::getSecretOwner)
+        .def("getMessageEncodingState", &CKKSCiphertext::getMessageEncodingState)
+        .def("getPoly", [](const CKKSCiphertext& ct, size_t i) { return to_np(ct.getPoly(i)); })
+        .def("setPoly", [](CKKSCiphertext& ct, size_t i, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> a) {
+            ct.setPoly(i, from_np(a));
+        })
+        .def("setScale", [](CKKSCiphertext& ct, double s) { ct.setScale(s); });
+
+    py::class_<CKKSEncoding>(m, "CKKSEncoding")
+        .def(py::init([](const CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> pt,
+                         long double scale, size_t level, bool mont) {
+                 return CKKSEncoding(c, from_np(pt), scale, level, mont);
+             }),
+             py::arg("context"), py::arg("plaintext"), py::arg("scale"), py::arg("level"),
+             py::arg("montgomery_form") = false, py::keep_alive<1, 2>())
+        .def("getPlaintext", [](const CKKSEncoding& e) { return to_np(e.getPlaintext()); })
+        .def("getScale", [](const CKKSEncoding& e) { return (double)e.getScale(); })
+        .def("getLevel", &CKKSEncoding::getLevel)
+        .def("isMontgomeryForm", &CKKSEncoding::isMontgomeryForm);
+
+    m.def("ckks_encode",
+          [](const CKKSContext& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v,
+             size_t level, double scale, bool eval) {
+              return scale > 0 ? ckks_encode(c, complex_vec(v), scale, level, eval)
+                               : ckks_encode(c, complex_vec(v), level, eval);
+          },
+          py::arg("context"), py::arg("values"), py::arg("level"), py::arg("scale") = 0.0, py::arg("eval") = false,
+          py::keep_alive<0, 1>());
+    m.def("ckks_encode_coefficients",
+          [](const CKKSContext& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v,
+             double scale, size_t level, bool eval) { return ckks_encode_coefficients(c, complex_vec(v), scale, level, eval); },
+          py::arg("context"), py::arg("values"), py::arg("scale"), py::arg("level"), py::arg("eval") = false,
+          py::keep_alive<0, 1>());
+    m.def("encode_slots_custom_basis",
+          [](const CKKSContext& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v,
+             double scale, std::vector<uint64_t> moduli) {
+              return to_np(encode_slots_custom_basis(c.getParams().getN(), complex_vec(v), scale,
+                                                     MyVector<uint64_t>(moduli.begin(), moduli.end()),
+                                                     c.getTwiddleFft(), c.getPermTable()));
+          });
+    m.def("ckks_encrypt",
+          [](const CKKSContext& c, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v,
+             size_t level) { return ckks_encrypt(c, complex_vec(v), level); },
+          py::keep_alive<0, 1>());
+    m.def("ckks_encrypt_plaintext", [](const CKKSContext& c, const CKKSEncoding& e, bool eval) {
+              return ckks_encrypt(c, e, eval);
+          },
+          py::arg("context"), py::arg("encoding"), py::arg("eval") = false, py::keep_alive<0, 1>());
+    m.def("ckks_decrypt",
+          [](const CKKSContext& c, const CKKSCiphertext& ct, CKKSDecryptDomain d) {
+              return complex_np(ckks_decrypt(c, ct, d));
+          },
+          py::arg("context"), py::arg("ct"), py::arg("domain") = CKKSDecryptDomain::Slots);
+    m.def("ckks_decode_plaintext",
+          [](const CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> coeffs, size_t limbs,
+             double scale, CKKSDecryptDomain d) {
+              return complex_np(ckks_decode_plaintext(c, from_np(coeffs), limbs, scale, d));
+          },
+          py::arg("context"), py::arg("coeffs"), py::arg("limb_count"), py::arg("scale"),
+          py::arg("domain") = CKKSDecryptDomain::Slots);
+
+    m.def("twiddle_ifft_generation", [](size_t N) { return complex_np(twiddle_ifft_generation(N)); });
+    m.def("perm_table_generation", [](size_t N) { return std::vector<size_t>(perm_table_generation(N)); });
+    m.def("encode_slots_custom_basis_tables",
+          [](size_t N, py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast> v, double scale,
+             std::vector<uint64_t> moduli) {
+              auto tw = twiddle_ifft_generation(N);
+              for (auto& z : tw) z = std::conj(z);
+              return to_np(encode_slots_custom_basis(N, complex_vec(v), scale,
+                                                     MyVector<uint64_t>(moduli.begin(), moduli.end()), tw,
+                                                     perm_table_generation(N)));
+          });
+    m.def("decode_plaintext_custom_basis",
+          [](size_t N, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> coeffs,
+             std::vector<uint64_t> moduli, double scale, CKKSDecryptDomain d) {
+              return complex_np(decode_plaintext_custom_basis(N, from_np(coeffs),
+                                                              MyVector<uint64_t>(moduli.begin(), moduli.end()), scale,
+                                                              twiddle_ifft_generation(N), perm_table_generation(N), d));
+          },
+          py::arg("N"), py::arg("coeffs"), py::arg("moduli"), py::arg("scale"),
+          py::arg("domain") = CKKSDecryptDomain::Slots);
+    m.def("ckks_mul_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, const CKKSCiphertext&>(&ckks_mul_inplace));
+    m.def("ckks_mul_plain_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, const CKKSEncoding&>(&ckks_mul_inplace));
+    m.def("ckks_mul_int_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, int64_t>(&ckks_mul_inplace));
+    m.def("ckks_mul_const_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, Complex, double>(&ckks_mul_inplace));
+    m.def("ckks_square_inplace", &ckks_square_inplace);
+    m.def("ckks_rescale", &ckks_rescale);
+    m.def("ckks_drop_to_level_inplace", &ckks_drop_to_level_inplace);
+    m.def("ckks_add_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, const CKKSCiphertext&>(&ckks_add_inplace));
+    m.def("ckks_add_plain_inplace", py::overload_cast<const CKKSContext&, CKKSCiphertext&, const CKKSEncoding&>(&ckks_add_inplace));
+    m.def("ckks_sub_inplace", &ckks_sub_inplace);
+    m.def("ckks_add_integer_constant_inplace", &ckks_add_integer_constant_inplace);
+    m.def("ckks_relin_hybrid_inplace", &ckks_relin_hybrid_inplace);
+    m.def("ckks_rotate_inplace", &ckks_rotate_inplace);
+    m.def("ckks_rotate", &ckks_rotate, py::keep_alive<0, 1>());
+    m.def("ckks_conjugate_inplace", &ckks_conjugate_inplace);
+    m.def("ckks_conjugate", &ckks_conjugate, py::keep_alive<0, 1>());
+    m.def("rotation_automorphism_index", &rotation_automorphism_index);
+}

@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -198,6 +198,7 @@ def bench_keyswitch(args, M, torch):
     ms = wall * 1e3 / args.steps
     Ls = levels + 1
     algo = float(np.sum(8 * N * (4 * Ls))) + nkeys * 2 * 3 * KL * 8 * N
+    cpu = cpu_baseline_keyswitch(N, q, p) if args.cpu_baseline else None
     return {
         "metric": "batched hybrid key switches/s, N=2^16, L in [8,30], K=10, dnum=3",
         "value": nct / (ms / 1e3), "unit": "keyswitch/s", "n_gpus": 1, "steps": args.steps,
@@ -205,8 +206,96 @@ def bench_keyswitch(args, M, torch):
         "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded)",
         "config": {"workload": "128 key switches, mixed levels, 16 keys", "algorithmic_GBps":
                    algo / (ms / 1e3) / 1e9},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
+
+
+def cpu_baseline_keyswitch(N, q, p):
+    """The oracle's hybrid key switch at the batch's mean level (L = 19), repeated for ~10 s."""
+    O = _oracle()
+    octx = O.Ctx(N, q, p, 3)
+    rng = np.random.default_rng(2)
+    level = 18
+    L = level + 1
+    rnd = lambda mods: np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in mods])
+    key = np.concatenate([rnd(list(q) + list(p)) for _ in range(6)])
+    kt, b, a = rnd(q[:L]), rnd(q[:L]), rnd(q[:L])
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        octx.keyswitch(level, kt, key, b, a, 0, 1)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "keyswitch/s", "cores": int(O.lib().oc_omp_threads()),
+            "kind": "port", "sample": "oracle key switch at L=19 (the batch's mean level), repeated for 10 s"}
+
+
+def bench_lt(args, M, torch):
+    """BSGS linear-transform stage with hoisted rotations at N=2^16, 2^15 slots,
+    level 24 (25 Q limbs), P = 10 x 50-bit, dnum 3: 128 diagonals
+    {16 g + b : g < 8, b < 16} (16 baby x 8 giant steps) over the QP basis, in
+    Montgomery form, applied to 32 input ciphertexts.  Diagonals, keys and
+    ciphertexts are seeded random residues (the integer work does not depend
+    on their values)."""
+    N = 1 << 16
+    p, q = _primes([50] * 10, [50] * 30, N)
+    ctx = M.Context(N, q, p, 3)
+    level, bs, giants, nct = 24, 16, 8, 32
+    L, K = level + 1, len(p)
+    qp = list(q[:L]) + list(p)
+    diag_idx = [bs * g + b for g in range(giants) for b in range(bs)]
+    diags = [_rand_limbs(torch, qp, N, 1, seed=5000 + t) for t in range(len(diag_idx))]
+    parts = ctx.parts(len(q) - 1)
+    rots = list(range(1, bs)) + [bs * g for g in range(1, giants)]
+    keys = {r: _rand_limbs(torch, list(q) + list(p), N, 2 * parts, seed=7000 + r) for r in rots}
+    cts = [_rand_limbs(torch, q[:L], N, 2, seed=9000 + i) for i in range(nct)]
+    outs = [torch.empty(2 * L * N, dtype=torch.int64, device="cuda") for _ in range(nct)]
+    W = L * N
+
+    def step():
+        for c, o in zip(cts, outs):
+            ctx.linear_transform_qp(level, c[W:], c[:W], diag_idx, diags, bs, keys, o[W:], o[:W])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    T = L + K
+    # per ciphertext: ct in/out, diagonals and keys amortised over the batch
+    algo = nct * (2 * L * 8 * N * 2) + len(diag_idx) * T * 8 * N + len(rots) * parts * 2 * (len(q) + K) * 8 * N
+    cpu = None
+    if args.cpu_baseline:
+        cpu = cpu_baseline_lt(N, q, p, level, diag_idx, bs, rots, parts)
+    return {
+        "metric": "BSGS linear-transform stage throughput, N=2^16, level 24, 128 diagonals",
+        "value": nct / (ms / 1e3), "unit": "ciphertext/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
+        "config": {"workload": "32 ct x BSGS stage (16 baby x 8 giant), QP diagonals, 22 rotation keys",
+                   "algorithmic_GBps": algo / (ms / 1e3) / 1e9},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_lt(N, q, p, level, diag_idx, bs, rots, parts):
+    """The oracle's BSGS stage on one ciphertext (bounded sample)."""
+    O = _oracle()
+    octx = O.Ctx(N, q, p, 3)
+    rng = np.random.default_rng(1)
+    L = level + 1
+    qp = list(q[:L]) + list(p)
+    rnd = lambda mods: np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in mods])
+    a, b = rnd(q[:L]), rnd(q[:L])
+    diags = np.concatenate([rnd(qp) for _ in diag_idx])
+    keys = {r: np.concatenate([rnd(list(q) + list(p)) for _ in range(2 * parts)]) for r in rots}
+    t0 = time.perf_counter()
+    octx.lt_stage_qp(level, a, b, diag_idx, diags, bs, keys)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "ciphertext/s", "cores": int(O.lib().oc_omp_threads()),
+            "kind": "port", "sample": "oracle BSGS stage on one ciphertext (same shape)"}
 
 
 def main():
@@ -214,7 +303,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -228,7 +317,7 @@ def main():
     M.lib()
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
-    out = bench_ntt(args, M, torch) if args.workload == "ntt" else bench_keyswitch(args, M, torch)
+    out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 

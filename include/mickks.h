@@ -21,6 +21,7 @@
 #ifndef MICKKS_H
 #define MICKKS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -54,6 +55,36 @@ int mi_context_destroy(mi_context* ctx);
 int mi_context_info(const mi_context* ctx, int64_t* info);
 /* Device memory the context holds (tables + workspaces), in bytes. */
 int64_t mi_context_device_bytes(const mi_context* ctx);
+
+/* ---- device memory ----------------------------------------------------------
+ * Replaces the host MyVector storage of CKKSCiphertext / CKKSEncoding and the
+ * context's key vectors (ckks_ciphertext.hpp:18-108, ckks_context.hpp:94-133):
+ * a per-context pool with byte accounting, used by the C++ host layer and by
+ * admission in the serving loop.  A freed block may be handed out again at
+ * once, so free a block only after the work using it is stream-ordered before
+ * any later use.  mi_memory_stats: live, peak, cached, limit (bytes).  With a
+ * limit set, an allocation that would exceed it fails with MI_ERUNTIME. */
+int mi_alloc(mi_context* ctx, size_t bytes, void** out);
+int mi_free(mi_context* ctx, void* ptr);
+int mi_memory_trim(mi_context* ctx);
+int mi_memory_stats(mi_context* ctx, int64_t* stats);
+int mi_set_memory_limit(mi_context* ctx, int64_t bytes);
+enum { MI_COPY_H2D = 0, MI_COPY_D2H = 1, MI_COPY_D2D = 2 };
+/* Host copies complete before the call returns; device-to-device copies are
+ * stream-ordered. */
+int mi_memcpy(mi_context* ctx, void* dst, const void* src, size_t bytes, int kind, void* stream);
+int mi_stream_synchronize(mi_context* ctx, void* stream);
+
+/* ---- element-wise over basis-ordered limbs (Q limbs 0..nq-1, then P) -------
+ * pointwise_multiply*_dispatch and poly_add/sub_modq (ntt.cpp:450-579,
+ * polynomial.cpp:6-32) over limbs 0..limbs-1 of the full Q u P basis, as key
+ * generation and key folding use them (hybrid_key_folding.hpp:50-77,
+ * hybrid_key_montgomery.hpp:12-36).  op: x (op)= y, or x (op)= residues[l]. */
+enum { MI_OP_ADD = 0, MI_OP_SUB = 1, MI_OP_MUL = 2, MI_OP_MUL_MONT = 3 };
+int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int npolys,
+               int64_t poly_stride, void* stream);
+int mi_poly_scalar(mi_context* ctx, int op, uint64_t* x, const uint64_t* residues, int limbs,
+                   int npolys, int64_t poly_stride, void* stream);
 
 /* ---- negacyclic NTT (ntt.cpp:235-447) -----------------------------------
  * Batched over npolys polys of `limbs` limbs each; limb l of every poly uses

@@ -157,6 +157,19 @@ class Ctx:
     def ntt_inv(self, data, limbs, lazy=False):
         a = u64a(data).copy(); lib().oc_ntt_inv_q(self.h, _p(a), limbs, int(lazy)); return a
 
+    def ntt_fwd_basis(self, data, basis, lazy=False):
+        """NTT over basis indices (idx < nq: q[idx], else p[idx - nq])."""
+        a = u64a(data).copy()
+        b = np.ascontiguousarray(np.asarray(basis, np.int32))
+        lib().oc_ntt_fwd_basis(self.h, _p(a), b.ctypes.data, len(b), int(lazy))
+        return a
+
+    def ntt_inv_basis(self, data, basis, lazy=False):
+        a = u64a(data).copy()
+        b = np.ascontiguousarray(np.asarray(basis, np.int32))
+        lib().oc_ntt_inv_basis(self.h, _p(a), b.ctypes.data, len(b), int(lazy))
+        return a
+
     def modup(self, level, part, src):
         L = level + 1
         s0 = part * self.alpha

@@ -85,3 +85,24 @@ def product():
         spec.loader.exec_module(mod)
         _product = mod
     return _product
+
+
+_host = None
+
+
+def host():
+    """The C++ host API binding (aes-walsh-ckks_amd/ckks_py*.so).  PyTorch is
+    imported first so that the HIP runtime it carries is the one the library
+    binds to (one runtime per process)."""
+    global _host
+    if _host is None:
+        import importlib.util
+        import sysconfig
+        import torch  # noqa: F401
+        path = os.path.join(PKG_DIR, "ckks_py" + sysconfig.get_config_var("EXT_SUFFIX"))
+        spec = importlib.util.spec_from_file_location("ckks_py", path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        _host = mod
+    return _host
+

@@ -1,0 +1,285 @@
+"""GPU tests of the C++ host API (CKKSParams / CKKSContext / CKKSCiphertext and
+the evaluator free functions), which runs every ring operation through the
+HIP C-ABI.
+
+* With the reference's captured keys and ciphertexts installed, the API's
+  operations reproduce the reference's outputs bit for bit (ev10 fixtures).
+* With the API's own seeded keys, the configuration-1 chain (encrypt, HMult,
+  relinearise, rescale, rotate by 1, decrypt) matches the oracle limb for limb
+  and decrypts to the expected slots.
+* Generated switching keys decrypt to P * s_src on their digit limbs.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import digest, host, load_json, load_npz
+
+pytestmark = pytest.mark.gpu
+
+
+def _ev_config(H, tag):
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 3 if tag == "ev10" else 4
+    c.q0_bit = 50
+    c.q_bit = 40
+    c.p_bits = [60, 60] if tag == "ev10" else [61, 61]
+    c.dnum = 2 if tag == "ev10" else 3
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [1, 5]
+    return c
+
+
+@pytest.mark.parametrize("tag", ["ev10", "ev10b"])
+def test_host_api_reproduces_reference_evaluator(tag):
+    H = host()
+    d = load_npz(tag + ".npz")
+    outs = load_json(tag + "_outputs.json")
+    ctx = H.CKKSContext(H.CKKSParams(_ev_config(H, tag)))
+    P = ctx.getParams()
+    assert list(P.getModuli()) == list(d["q"]) and list(P.getPModuli()) == list(d["p"])
+    ctx.setSecretKeyEval(d["sk"])
+    ctx.setRelinKey(d["key_relin"])
+    ctx.setRotationKey(1, d["key_rot1"])
+    if "key_rot5" in d:
+        ctx.setRotationKey(5, d["key_rot5"])
+    if "key_conj" in d:
+        ctx.setConjugationKey(d["key_conj"])
+    N, L = 1 << 10, len(d["q"])
+    lv, W = L - 1, L * N
+    scale = float(d["ct1_meta"][2:3].view(np.float64)[0])
+
+    def ct(arr):
+        return H.CKKSCiphertext(ctx, [arr[:W], arr[W:2 * W]], scale, lv)
+
+    def flat(x):
+        return np.concatenate([x.getPoly(i) for i in range(x.getNumPolys())])
+
+    def check(name, x):
+        assert digest(flat(x)) == outs[name]["digest"], name
+        assert x.getLevel() == outs[name]["level"], name
+        assert x.getScale() == outs[name]["scale"], name
+
+    ct1, ct2 = ct(d["ct1"]), ct(d["ct2"])
+    enc = H.ckks_encode(ctx, d["m1"], lv, eval=True)
+    assert np.array_equal(enc.getPlaintext(), d["encode_m1"])
+    assert np.array_equal(H.ckks_decrypt(ctx, ct1), d["dec_ct1"])
+
+    x = ct1.clone()
+    H.ckks_mul_inplace(ctx, x, ct2)
+    check("mul", x)
+    H.ckks_relin_hybrid_inplace(ctx, x)
+    check("mul_relin", x)
+    H.ckks_rescale(ctx, x)
+    check("mul_relin_rescale", x)
+    H.ckks_rotate_inplace(ctx, x, 1)
+    check("chain_rot1", x)
+    x = ct1.clone()
+    H.ckks_square_inplace(ctx, x)
+    check("square", x)
+    check("rot1_out", H.ckks_rotate(ctx, ct1, 1))
+    if "key_rot5" in d:
+        check("rot5_out", H.ckks_rotate(ctx, ct1, 5))
+    if "key_conj" in d:
+        check("conj_out", H.ckks_conjugate(ctx, ct1))
+    x = ct1.clone()
+    H.ckks_rescale(ctx, x)
+    check("rescale_ct1", x)
+    H.ckks_rescale(ctx, x)
+    check("rescale2_ct1", x)
+    x = ct1.clone()
+    H.ckks_add_inplace(ctx, x, ct2)
+    check("add", x)
+    x = ct1.clone()
+    H.ckks_sub_inplace(ctx, x, ct2)
+    check("sub", x)
+    x = ct1.clone()
+    H.ckks_mul_int_inplace(ctx, x, -7)
+    check("mulint", x)
+    x = ct1.clone()
+    H.ckks_add_integer_constant_inplace(ctx, x, 3)
+    check("addconst", x)
+    pt = H.ckks_encode(ctx, d["m2"], lv, eval=True)
+    assert np.array_equal(pt.getPlaintext(), d["pt_m2"])
+    x = ct1.clone()
+    H.ckks_mul_plain_inplace(ctx, x, pt)
+    check("mulplain", x)
+    x = ct1.clone()
+    H.ckks_drop_to_level_inplace(x, 1)
+    H.ckks_rotate_inplace(ctx, x, 1)
+    check("rot1_level1", x)
+
+
+def _config1(H, seed=11):
+    """Configuration 1: N=2^14, 8 limbs of 50-bit primes, P = 2 x 50 bits,
+    dnum 4, Hamming weight 192, sigma 3.2, rotation key for 1."""
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 14
+    c.mult_depth = 7
+    c.q0_bit = 50
+    c.q_bit = 50
+    c.p_bits = [50, 50]
+    c.dnum = 4
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [1]
+    c.seed = seed
+    return c
+
+
+def test_host_config1_chain_matches_oracle_and_decrypts():
+    H = host()
+    ctx = H.CKKSContext(H.CKKSParams(_config1(H)))
+    P = ctx.getParams()
+    q, p = P.getModuli(), P.getPModuli()
+    N, lv = 1 << 14, 7
+    octx = O.Ctx(N, q, p, 4)
+    rng = np.random.default_rng(2024)
+    m1 = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    m2 = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    ct1 = H.ckks_encrypt(ctx, m1, lv)
+    ct2 = H.ckks_encrypt(ctx, m2, lv)
+    assert np.max(np.abs(H.ckks_decrypt(ctx, ct1) - m1)) < 1e-9
+    b1, a1, b2, a2 = ct1.getPoly(0), ct1.getPoly(1), ct2.getPoly(0), ct2.getPoly(1)
+    relin, rot1 = ctx.getRelinKey(), ctx.getRotationKey(1)
+
+    H.ckks_mul_inplace(ctx, ct1, ct2)
+    ob, oa, oc2 = octx.tensor(lv, b1, a1, b2, a2)
+    assert np.array_equal(ct1.getPoly(0), ob) and np.array_equal(ct1.getPoly(1), oa)
+    assert np.array_equal(ct1.getPoly(2), oc2)
+    H.ckks_relin_hybrid_inplace(ctx, ct1)
+    ob, oa = octx.relin(lv, ob, oa, oc2, relin)
+    assert np.array_equal(ct1.getPoly(0), ob) and np.array_equal(ct1.getPoly(1), oa)
+    H.ckks_rescale(ctx, ct1)
+    r = octx.rescale(lv, np.stack([ob, oa]))
+    assert np.array_equal(ct1.getPoly(0), r[0]) and np.array_equal(ct1.getPoly(1), r[1])
+    H.ckks_rotate_inplace(ctx, ct1, 1)
+    ob, oa = octx.rotate(lv - 1, r[0], r[1], 1, rot1)
+    assert np.array_equal(ct1.getPoly(0), ob) and np.array_equal(ct1.getPoly(1), oa)
+    assert ct1.getLevel() == lv - 1
+    assert ct1.getScale() == 2.0 ** 100 / float(q[lv])
+    dec = H.ckks_decrypt(ctx, ct1)
+    want = np.roll(m1 * m2, -1)
+    assert np.max(np.abs(dec - want)) < 1e-6
+
+
+def _unfold(key_limb, m, K_mods, l, nq, fold, mont):
+    """Undo the folding and Montgomery scalings of one key limb (python ints)."""
+    f = 1
+    if fold:
+        if l < nq:
+            Pm = 1
+            for pv in K_mods:
+                Pm = Pm * pv % m
+            f = pow(Pm, -1, m)
+        else:
+            ph = 1
+            for j, pv in enumerate(K_mods):
+                if j != l - nq:
+                    ph = ph * pv % m
+            f = pow(ph, -1, m)
+    if mont:
+        f = f * (1 << 64) % m
+    inv = pow(f, -1, m)
+    return [int(v) * inv % m for v in key_limb]
+
+
+def test_host_switching_keys_decrypt_to_their_messages():
+    """b + a * s_tgt - P * s_src (on the digit's limbs) is a small error for the
+    relinearisation (s^2 -> s), rotation (s -> pi^-1(s)) and conjugation keys."""
+    H = host()
+    c = _config1(H, seed=5)
+    c.N = 1 << 11
+    c.rotate_values = [3]
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    q, p = [int(v) for v in P.getModuli()], [int(v) for v in P.getPModuli()]
+    N, nq, K = 1 << 11, len(q), len(p)
+    basis = q + p
+    octx = O.Ctx(N, np.array(q, np.uint64), np.array(p, np.uint64), 4)
+    s = [int(v) for v in ctx.getSecretKeyCoeffs()]
+    assert sum(1 for v in s if v) == 192 and set(s) <= {-1, 0, 1}
+
+    def coeff_auto(sv, g):
+        out = [0] * N
+        for i, v in enumerate(sv):
+            j = i * g % (2 * N)
+            if j >= N:
+                out[j - N] = -v
+            else:
+                out[j] = v
+        return out
+
+    def ntt_qp(sv):
+        limbs = np.concatenate([np.array([v % m for v in sv], np.uint64) for m in basis])
+        bidx = np.arange(nq + K, dtype=np.int32)
+        return octx.ntt_fwd_basis(limbs, bidx)
+
+    s_eval = ntt_qp(s)
+    s2_eval = np.concatenate([np.array([int(x) * int(x) % m for x in s_eval[l * N:(l + 1) * N]], np.uint64)
+                              for l, m in enumerate(basis)])
+    g3 = pow(5, 3, 2 * N)
+    g3inv = pow(5, N // 2 - 3, 2 * N)
+    cases = [("relin", ctx.getRelinKey(), s2_eval, s_eval),
+             ("rot3", ctx.getRotationKey(3), s_eval, ntt_qp(coeff_auto(s, g3inv))),
+             ("conj", ctx.getConjugationKey(), ntt_qp(coeff_auto(s, 2 * N - 1)), s_eval)]
+    alpha = (nq + 3) // 4
+    KL = nq + K
+    Pq = 1
+    for pv in p:
+        Pq *= pv
+    for name, key, src, tgt in cases:
+        digits = len(key) // (2 * KL * N)
+        assert digits == ctx.keyDigitCount()
+        for d in range(digits):
+            kb = key[(2 * d) * KL * N:(2 * d + 1) * KL * N]
+            ka = key[(2 * d + 1) * KL * N:(2 * d + 2) * KL * N]
+            err = np.zeros(KL * N, np.uint64)
+            for l, m in enumerate(basis):
+                sl = slice(l * N, (l + 1) * N)
+                bl = _unfold(kb[sl], m, p, l, nq, True, True)
+                al = _unfold(ka[sl], m, p, l, nq, True, True)
+                in_digit = d * alpha <= l < min(d * alpha + alpha, nq)
+                pm = Pq % m if in_digit else 0
+                err[sl] = np.array([(bv + av * int(t) - pm * int(sv)) % m
+                                    for bv, av, t, sv in zip(bl, al, tgt[sl], src[sl])], np.uint64)
+            e = octx.ntt_inv_basis(err, np.arange(KL, dtype=np.int32))
+            for l, m in enumerate(basis):
+                c_ = [int(v) - m if int(v) > m // 2 else int(v) for v in e[l * N:(l + 1) * N]]
+                assert max(abs(v) for v in c_) < 64, (name, d, l)
+                if l:
+                    assert c_ == prev, (name, d, l)  # one integer error reduced per limb
+                prev = c_
+
+
+def test_host_api_errors_and_memory_accounting():
+    H = host()
+    c = _config1(H, seed=3)
+    c.N = 1 << 12
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    live0 = ctx.memoryStats()[0]
+    m = np.zeros(1 << 11, complex)
+    ct = H.ckks_encrypt(ctx, m, 7)
+    assert ctx.memoryStats()[0] > live0
+    with pytest.raises(ValueError, match="ckks_rotate: rotation key is unavailable"):
+        H.ckks_rotate_inplace(ctx, ct, 2)
+    x = ct.clone()
+    H.ckks_square_inplace(ctx, x)
+    with pytest.raises(ValueError, match="expected 2-component"):
+        H.ckks_square_inplace(ctx, x)
+    with pytest.raises(ValueError, match="ckks_relin: expected 3-polynomial"):
+        H.ckks_relin_hybrid_inplace(ctx, ct)
+    y = ct.clone()
+    H.ckks_drop_to_level_inplace(y, 0)
+    with pytest.raises(ValueError, match="ckks_rescale: invalid level"):
+        H.ckks_rescale(ctx, y)
+    with pytest.raises(ValueError, match="ciphertext size mismatch"):
+        H.ckks_add_inplace(ctx, y, ct)
+    del x, y, ct
+    import gc
+    gc.collect()
+    assert ctx.memoryStats()[0] == live0
