@@ -156,31 +156,33 @@ int mi_stream_synchronize(mi_context* ctx, void* stream) {
     });
 }
 
-int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int npolys,
-               int64_t poly_stride, void* stream) {
+int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int basis0,
+               int npolys, int64_t poly_stride, void* stream) {
     return guard("mi_poly_op", [&] {
         check_ctx(ctx);
         check_ptr(x); check_ptr(y);
         const mi::Context& c = *ctx->c;
-        if (limbs < 1 || limbs > c.nq + c.np) throw mi::Error(MI_EINVAL, "invalid limb count");
+        if (limbs < 1 || basis0 < 0 || basis0 + limbs > c.nq + c.np)
+            throw mi::Error(MI_EINVAL, "invalid limb range");
         if (npolys < 1 || poly_stride < (int64_t)limbs * (int64_t)c.N)
             throw mi::Error(MI_EINVAL, "invalid poly count or stride");
         if (op < MI_OP_ADD || op > MI_OP_MUL_MONT) throw mi::Error(MI_EINVAL, "invalid op");
-        mi::ew_binary(c, op, x, y, limbs, npolys, poly_stride, poly_stride, S(stream));
+        mi::ew_binary(c, op, x, y, limbs, npolys, poly_stride, poly_stride, S(stream), basis0);
     });
 }
 
 int mi_poly_scalar(mi_context* ctx, int op, uint64_t* x, const uint64_t* residues, int limbs,
-                   int npolys, int64_t poly_stride, void* stream) {
+                   int basis0, int npolys, int64_t poly_stride, void* stream) {
     return guard("mi_poly_scalar", [&] {
         check_ctx(ctx);
         check_ptr(x); check_ptr(residues);
         const mi::Context& c = *ctx->c;
-        if (limbs < 1 || limbs > c.nq + c.np) throw mi::Error(MI_EINVAL, "invalid limb count");
+        if (limbs < 1 || basis0 < 0 || basis0 + limbs > c.nq + c.np)
+            throw mi::Error(MI_EINVAL, "invalid limb range");
         if (npolys < 1 || poly_stride < (int64_t)limbs * (int64_t)c.N)
             throw mi::Error(MI_EINVAL, "invalid poly count or stride");
         if (op != MI_OP_ADD && op != MI_OP_MUL) throw mi::Error(MI_EINVAL, "invalid op");
-        mi::ew_scalar(c, op, x, residues, limbs, npolys, poly_stride, S(stream));
+        mi::ew_scalar(c, op, x, residues, limbs, npolys, poly_stride, S(stream), basis0);
     });
 }
 
@@ -253,6 +255,28 @@ int mi_mul_scalar(mi_context* ctx, int level, uint64_t* polys, int npolys, int64
         if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
             throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
         mi::ew_scalar(c, mi::EW_MUL, polys, residues, level + 1, npolys, poly_stride, S(stream));
+    });
+}
+
+int mi_mul_scalar_add(mi_context* ctx, int level, uint64_t* dst, const uint64_t* src, int npolys,
+                      int64_t poly_stride, const uint64_t* residues, void* stream) {
+    return guard("fused_multiply_by_real_scalar_then_add", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(dst); check_ptr(src); check_ptr(residues);
+        const mi::Context& c = *ctx->c;
+        if (npolys < 1 || poly_stride < (int64_t)(level + 1) * (int64_t)c.N)
+            throw mi::Error(MI_EINVAL, "invalid ciphertext shape");
+        mi::ew_scalar_mac(c, dst, src, residues, level + 1, npolys, poly_stride, poly_stride, S(stream));
+    });
+}
+
+int mi_memset_zero(mi_context* ctx, void* ptr, size_t bytes, void* stream) {
+    return guard("mi_memset_zero", [&] {
+        check_ctx(ctx);
+        if (bytes == 0) return;
+        check_ptr(ptr);
+        MI_HIP(hipMemsetAsync(ptr, 0, bytes, S(stream)));
     });
 }
 

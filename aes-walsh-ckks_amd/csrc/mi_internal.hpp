@@ -128,11 +128,13 @@ void context_destroy(Context* c);
 enum EwOp { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2, EW_MUL_MONT = 3 };
 // x[k] (op)= y[k] (y stride may be 0 to broadcast one poly)
 void ew_binary(const Context& c, int op, u64* x, const u64* y, int limbs, int npolys,
-               long long xstride, long long ystride, hipStream_t s);
+               long long xstride, long long ystride, hipStream_t s, int basis0 = 0);
 // x[k][l] = x[k][l] (op) scalar[l]   (op: add or mul), scalars on the host
 constexpr int MAX_LIMBS = 64;
 void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int limbs, int npolys,
-               long long stride, hipStream_t s);
+               long long stride, hipStream_t s, int basis0 = 0);
+void ew_scalar_mac(const Context& c, u64* dst, const u64* src, const u64* scalars_host, int limbs,
+                   int npolys, long long dstride, long long sstride, hipStream_t s);
 void tensor(const Context& c, int limbs, u64* b1, u64* a1, const u64* b2, const u64* a2, u64* c2,
             hipStream_t s);
 void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s);

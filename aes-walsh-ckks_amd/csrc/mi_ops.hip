@@ -26,7 +26,7 @@ k_ew_binary(const BasisConst* __restrict__ bc, int op, u64* x, const u64* __rest
     const long long per = (long long)limbs << logN;
     const long long k = i / per, r = i - k * per;
     const int l = (int)(r >> logN);
-    const BasisConst b = bc[l];
+    const BasisConst b = bc[l];  // bc is pre-offset to the first limb's basis
     u64* px = x + k * xstride + r;
     const u64 vy = y[k * ystride + r];
     const u64 vx = *px;
@@ -41,10 +41,10 @@ k_ew_binary(const BasisConst* __restrict__ bc, int op, u64* x, const u64* __rest
 }
 
 void ew_binary(const Context& c, int op, u64* x, const u64* y, int limbs, int npolys,
-               long long xstride, long long ystride, hipStream_t s) {
+               long long xstride, long long ystride, hipStream_t s, int basis0) {
     long long total = (long long)npolys * limbs * (long long)c.N;
     if (total == 0) return;
-    hipLaunchKernelGGL(k_ew_binary, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, op, x,
+    hipLaunchKernelGGL(k_ew_binary, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
                        y, c.logN, limbs, xstride, ystride, total);
     MI_HIP(hipGetLastError());
 }
@@ -67,8 +67,23 @@ k_ew_scalar(const BasisConst* __restrict__ bc, int op, u64* x, Scalars sc, int l
     *px = op == EW_ADD ? add_mod(*px, s, b.q) : mul_mod(*px, s, b.q, b.mu_lo, b.mu_hi);
 }
 
-void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int limbs, int npolys,
-               long long stride, hipStream_t s) {
+// dst[k][l] += src[k][l] * scalar[l]
+// (fused_multiply_by_real_scalar_then_add_inplace, ckks_evalpoly.cpp:352-398)
+__global__ void __launch_bounds__(EW_THREADS)
+k_ew_scalar_mac(const BasisConst* __restrict__ bc, u64* dst, const u64* __restrict__ src, Scalars sc,
+                int logN, int limbs, long long dstride, long long sstride, long long total) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const long long per = (long long)limbs << logN;
+    const long long k = i / per, r = i - k * per;
+    const int l = (int)(r >> logN);
+    const BasisConst b = bc[l];
+    u64* pd = dst + k * dstride + r;
+    *pd = add_mod(*pd, mul_mod(src[k * sstride + r], sc.v[l], b.q, b.mu_lo, b.mu_hi), b.q);
+}
+
+void ew_scalar_mac(const Context& c, u64* dst, const u64* src, const u64* scalars_host, int limbs,
+                   int npolys, long long dstride, long long sstride, hipStream_t s) {
     if (limbs > MAX_LIMBS) throw Error(-1, "too many limbs for a scalar operation");
     Scalars sc{};
     for (int l = 0; l < limbs; ++l) {
@@ -77,7 +92,22 @@ void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int li
     }
     long long total = (long long)npolys * limbs * (long long)c.N;
     if (total == 0) return;
-    hipLaunchKernelGGL(k_ew_scalar, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, op, x,
+    hipLaunchKernelGGL(k_ew_scalar_mac, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, dst, src,
+                       sc, c.logN, limbs, dstride, sstride, total);
+    MI_HIP(hipGetLastError());
+}
+
+void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int limbs, int npolys,
+               long long stride, hipStream_t s, int basis0) {
+    if (limbs > MAX_LIMBS) throw Error(-1, "too many limbs for a scalar operation");
+    Scalars sc{};
+    for (int l = 0; l < limbs; ++l) {
+        if (scalars_host[l] >= c.h_bc[basis0 + l].q) throw Error(-1, "scalar residue is not reduced");
+        sc.v[l] = scalars_host[l];
+    }
+    long long total = (long long)npolys * limbs * (long long)c.N;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_ew_scalar, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
                        sc, c.logN, limbs, stride, total);
     MI_HIP(hipGetLastError());
 }

@@ -345,6 +345,79 @@ void ckks_rotate_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t
 CKKSCiphertext ckks_conjugate(const CKKSContext& context, const CKKSCiphertext& ct);
 void ckks_conjugate_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 
+// Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
+// coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].
+struct CKKSEvalPolyOptions {
+    // Scale assigned to the result; zero keeps the input's scale.
+    double target_scale = 0.0;
+};
+CKKSCiphertext ckks_evalchebyshev_preconditioned(const CKKSContext& context, const CKKSCiphertext& x,
+                                                 const std::vector<double>& chebyshev_coefficients,
+                                                 const CKKSEvalPolyOptions& options = {});
+
+// ---------------------------------------------------- bootstrapping DFT plans
+// ckks_bootstrap_dft.hpp:10-40, ckks_linear_transform_plan.hpp:9-47.  Stage
+// diagonals are kept only in the Q u P basis at the stage level, on the device
+// (the evaluator used for every stage is the QP one, as in all the
+// reference's bootstrapping presets).
+enum class CKKSBootstrapDftType { SlotsToCoeffs, CoeffsToSlots };
+using CKKSBootstrapDftScaleSchedule = std::vector<std::vector<size_t>>;
+
+struct CKKSBootstrapDftMatrixLiteral {
+    CKKSBootstrapDftType type = CKKSBootstrapDftType::SlotsToCoeffs;
+    size_t level = 0;
+    size_t baby_step_count = 0;  // 0: chosen from the diagonal indices
+    size_t stage_count = 1;
+    CKKSBootstrapDftScaleSchedule log_plaintext_scale_groups;
+    std::vector<size_t> merge_depths;
+    int log_bsgs_ratio = 1;
+    size_t active_slots = 0;     // 0: N/2
+    double transform_scale = 1.0;
+    double plaintext_scale = 1.0;
+    size_t input_selector_start = 0;
+    size_t input_selector_size = 0;
+    bool rescale_after_each_stage = true;
+};
+
+struct CKKSLinearTransformOwnedStage {
+    CKKSLinearTransformOwnedStage() = default;
+    CKKSLinearTransformOwnedStage(const CKKSLinearTransformOwnedStage&) = delete;
+    CKKSLinearTransformOwnedStage& operator=(const CKKSLinearTransformOwnedStage&) = delete;
+    CKKSLinearTransformOwnedStage(CKKSLinearTransformOwnedStage&&) noexcept = default;
+    CKKSLinearTransformOwnedStage& operator=(CKKSLinearTransformOwnedStage&&) noexcept = default;
+
+    MyVector<DeviceBuffer> diagonal_qp_plaintexts;  // (level+1+K)*N each
+    MyVector<size_t> diagonal_indices;
+    size_t slot_count = 0;
+    size_t baby_step_count = 0;
+    size_t qp_limb_count = 0;
+    size_t qp_p_limb_count = 0;
+    size_t level = 0;
+    double plaintext_scale = 1.0;
+    bool diagonal_qp_plaintexts_montgomery = false;
+    bool rescale_after = true;
+    CKKSMessageEncodingState output_encoding_state = CKKSMessageEncodingState::Slots;
+    MyVector<uint64_t> downloadDiagonalQp(const CKKSContext& context, size_t term) const;
+};
+
+struct CKKSLinearTransformPlan {
+    CKKSLinearTransformPlan() = default;
+    CKKSLinearTransformPlan(const CKKSLinearTransformPlan&) = delete;
+    CKKSLinearTransformPlan& operator=(const CKKSLinearTransformPlan&) = delete;
+    CKKSLinearTransformPlan(CKKSLinearTransformPlan&&) noexcept = default;
+    CKKSLinearTransformPlan& operator=(CKKSLinearTransformPlan&&) noexcept = default;
+
+    MyVector<CKKSLinearTransformOwnedStage> stages;
+    bool use_qp_evaluator = true;
+};
+
+CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& context,
+                                                         const CKKSBootstrapDftMatrixLiteral& literal);
+CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& context, const CKKSCiphertext& ct,
+                                                          const CKKSLinearTransformOwnedStage& stage);
+CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& context, CKKSCiphertext ct,
+                                                const CKKSLinearTransformPlan& plan);
+
 // Galois element of a slot rotation (rotation_key_generation.cpp:58-78).
 int64_t normalize_rotation_value(size_t N, int64_t rotation);
 uint64_t rotation_automorphism_index(size_t N, int64_t rotation);

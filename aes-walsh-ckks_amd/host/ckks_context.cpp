@@ -381,7 +381,7 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
         // relinearisation: s^2 -> s
         DeviceBuffer s2(mi, KL * N);
         copy_d2d(*this, s2.data(), secret_qp.data(), KL * N);
-        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), secret_qp.data(), (int)KL, 1, (int64_t)(KL * N), stream()),
+        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), secret_qp.data(), (int)KL, 0, 1, (int64_t)(KL * N), stream()),
                  "ckks_context");
         relin_key = generate_switch_key(s2.data(), secret_qp.data(), 1);
         // conjugation: conj(s) -> s
@@ -467,7 +467,7 @@ DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_
         copy_d2d(*this, b, src, KL * N);
         std::vector<uint64_t> r(KL, 0);
         for (size_t l = d * alpha; l < std::min(d * alpha + alpha, nq); ++l) r[l] = p_mod[l];
-        mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, r.data(), (int)KL, 1, W, stream()), "ckks_context");
+        mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, r.data(), (int)KL, 0, 1, W, stream()), "ckks_context");
         for (size_t l = 0; l < KL; ++l)
             for (size_t i = 0; i < N; ++i) h[l * N + i] = rng.below(basis[l]);
         upload(*this, a, h.data(), h.size());
@@ -477,12 +477,12 @@ DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_
         }
         upload(*this, tmp.data(), h.data(), h.size());
         mi_check(mi_ntt_forward(mi, tmp.data(), 1, (int)KL, W, 0, stream()), "ckks_context");
-        mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)KL, 1, W, stream()), "ckks_context");
+        mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)KL, 0, 1, W, stream()), "ckks_context");
         copy_d2d(*this, tmp.data(), a, KL * N);
-        mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), tgt, (int)KL, 1, W, stream()), "ckks_context");
-        mi_check(mi_poly_op(mi, MI_OP_SUB, b, tmp.data(), (int)KL, 1, W, stream()), "ckks_context");
+        mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), tgt, (int)KL, 0, 1, W, stream()), "ckks_context");
+        mi_check(mi_poly_op(mi, MI_OP_SUB, b, tmp.data(), (int)KL, 0, 1, W, stream()), "ckks_context");
         if (params.fold_hybrid_keyswitch || params.montgomery)
-            mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, scale.data(), (int)KL, 2, W, stream()), "ckks_context");
+            mi_check(mi_poly_scalar(mi, MI_OP_MUL, b, scale.data(), (int)KL, 0, 2, W, stream()), "ckks_context");
     }
     mi_check(mi_stream_synchronize(mi, stream()), "ckks_context");
     return key;

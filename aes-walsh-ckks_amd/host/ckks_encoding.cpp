@@ -157,7 +157,7 @@ CKKSCiphertext ckks_encrypt(const CKKSContext& c, const CKKSEncoding& encd, bool
         MyVector<uint64_t> ones(L * N, 1);
         DeviceBuffer one(mi, L * N);
         mi_check(mi_memcpy(mi, one.data(), ones.data(), ones.size() * 8, MI_COPY_H2D, c.stream()), "ckks_encrypt(ptxt)");
-        mi_check(mi_poly_op(mi, MI_OP_MUL_MONT, one.data(), b, (int)L, 1, W, c.stream()), "ckks_encrypt(ptxt)");
+        mi_check(mi_poly_op(mi, MI_OP_MUL_MONT, one.data(), b, (int)L, 0, 1, W, c.stream()), "ckks_encrypt(ptxt)");
         mi_check(mi_memcpy(mi, b, one.data(), L * N * 8, MI_COPY_D2D, c.stream()), "ckks_encrypt(ptxt)");
         mi_check(mi_stream_synchronize(mi, c.stream()), "ckks_encrypt(ptxt)");
     }
@@ -176,10 +176,10 @@ CKKSCiphertext ckks_encrypt(const CKKSContext& c, const CKKSEncoding& encd, bool
     DeviceBuffer tmp(mi, L * N);
     mi_check(mi_memcpy(mi, tmp.data(), h.data(), h.size() * 8, MI_COPY_H2D, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_ntt_forward(mi, tmp.data(), 1, (int)L, W, 0, c.stream()), "ckks_encrypt(ptxt)");
-    mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)L, 1, W, c.stream()), "ckks_encrypt(ptxt)");
+    mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)L, 0, 1, W, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_memcpy(mi, tmp.data(), a, L * N * 8, MI_COPY_D2D, c.stream()), "ckks_encrypt(ptxt)");
-    mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), c.secretKeyDevice(), (int)L, 1, W, c.stream()), "ckks_encrypt(ptxt)");
-    mi_check(mi_poly_op(mi, MI_OP_SUB, b, tmp.data(), (int)L, 1, W, c.stream()), "ckks_encrypt(ptxt)");
+    mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), c.secretKeyDevice(), (int)L, 0, 1, W, c.stream()), "ckks_encrypt(ptxt)");
+    mi_check(mi_poly_op(mi, MI_OP_SUB, b, tmp.data(), (int)L, 0, 1, W, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_stream_synchronize(mi, c.stream()), "ckks_encrypt(ptxt)");
     return ct;
 }
@@ -267,14 +267,14 @@ MyVector<Complex> ckks_decrypt(const CKKSContext& c, const CKKSCiphertext& ct, C
     const int64_t W = (int64_t)(L * N);
     DeviceBuffer t(mi, L * N);
     mi_check(mi_memcpy(mi, t.data(), ct.polyData(1), L * N * 8, MI_COPY_D2D, c.stream()), "ckks_decrypt");
-    mi_check(mi_poly_op(mi, MI_OP_MUL, t.data(), c.secretKeyDevice(), (int)L, 1, W, c.stream()), "ckks_decrypt");
-    mi_check(mi_poly_op(mi, MI_OP_ADD, t.data(), ct.polyData(0), (int)L, 1, W, c.stream()), "ckks_decrypt");
+    mi_check(mi_poly_op(mi, MI_OP_MUL, t.data(), c.secretKeyDevice(), (int)L, 0, 1, W, c.stream()), "ckks_decrypt");
+    mi_check(mi_poly_op(mi, MI_OP_ADD, t.data(), ct.polyData(0), (int)L, 0, 1, W, c.stream()), "ckks_decrypt");
     if (polys == 3) {
         DeviceBuffer s2(mi, L * N);
         mi_check(mi_memcpy(mi, s2.data(), c.secretKeyDevice(), L * N * 8, MI_COPY_D2D, c.stream()), "ckks_decrypt");
-        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), c.secretKeyDevice(), (int)L, 1, W, c.stream()), "ckks_decrypt");
-        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), ct.polyData(2), (int)L, 1, W, c.stream()), "ckks_decrypt");
-        mi_check(mi_poly_op(mi, MI_OP_ADD, t.data(), s2.data(), (int)L, 1, W, c.stream()), "ckks_decrypt");
+        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), c.secretKeyDevice(), (int)L, 0, 1, W, c.stream()), "ckks_decrypt");
+        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), ct.polyData(2), (int)L, 0, 1, W, c.stream()), "ckks_decrypt");
+        mi_check(mi_poly_op(mi, MI_OP_ADD, t.data(), s2.data(), (int)L, 0, 1, W, c.stream()), "ckks_decrypt");
         mi_check(mi_stream_synchronize(mi, c.stream()), "ckks_decrypt");
     }
     mi_check(mi_ntt_inverse(mi, t.data(), 1, (int)L, W, 0, c.stream()), "ckks_decrypt");

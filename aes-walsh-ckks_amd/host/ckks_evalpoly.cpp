@@ -1,0 +1,561 @@
+// Chebyshev-basis polynomial evaluation of an encrypted input
+// (ckks_evalchebyshev_preconditioned, ckks_evalpoly.cpp:1209-1256).
+//
+// The schedule is the reference's: a power basis T_1..T_{2^k} built by
+// T_{a+b} = 2 T_a T_b - T_{|a-b|} with exact scale tracking (seed^e over a
+// product of moduli, ckks_evalpoly.cpp:24-141, 838-966), a recursive
+// Paterson-Stockmeyer split planned on simulated (level, scale) pairs
+// (:549-836), leaves evaluated as scale-aware scalar multiply-adds (:460-530,
+// 998-1058) and the leaves combined by giant steps (:1060-1151).  Every
+// constant is rounded in long double exactly as there, so the ciphertext
+// limbs come out identical; the ring operations themselves run on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <optional>
+#include <sstream>
+
+#include "ckks.hpp"
+
+namespace {
+
+using ld = long double;
+
+size_t bit_length(size_t v) { return v ? (size_t)(std::numeric_limits<size_t>::digits - __builtin_clzll(v)) : 0; }
+
+// seed^seed_pow / prod_i q_i^den[i]
+struct ScaleTerm {
+    uint64_t seed = 1;
+    int seed_pow = 0;
+    std::vector<int> den;
+
+    static ScaleTerm of_integer_scale(double v, size_t nmod) {
+        if (!std::isfinite(v) || v < 0.0) throw std::invalid_argument("ckks_evalchebyshev: invalid exact scale seed");
+        const ld r = std::round((ld)v);
+        const ld tol = std::max<ld>(1.0L, std::abs((ld)v)) * 0x1p-20L;
+        if (std::abs((ld)v - r) > tol)
+            throw std::invalid_argument("ckks_evalchebyshev: non-integral initial scale is unsupported");
+        if (r > (ld)std::numeric_limits<uint64_t>::max())
+            throw std::invalid_argument("ckks_evalchebyshev: initial scale exceeds uint64_t");
+        ScaleTerm s;
+        s.seed = (uint64_t)r;
+        s.seed_pow = 1;
+        s.den.assign(nmod, 0);
+        return s;
+    }
+    ld value(const MyVector<uint64_t>& q) const {
+        ld v = 1.0L;
+        for (int i = 0; i < seed_pow; ++i) v *= (ld)seed;
+        for (size_t i = 0; i < den.size(); ++i)
+            for (int e = 0; e < den[i]; ++e) v /= (ld)q[i];
+        return v;
+    }
+    double value_d(const MyVector<uint64_t>& q) const { return (double)value(q); }
+};
+
+ScaleTerm scale_product(const ScaleTerm& a, const ScaleTerm& b) {
+    ScaleTerm o;
+    o.seed = a.seed;
+    o.seed_pow = a.seed_pow + b.seed_pow;
+    o.den = a.den;
+    if (o.den.size() < b.den.size()) o.den.resize(b.den.size(), 0);
+    for (size_t i = 0; i < b.den.size(); ++i) o.den[i] += b.den[i];
+    return o;
+}
+
+void scale_divide_by_modulus(ScaleTerm& s, size_t idx) {
+    if (s.den.size() < idx + 1) s.den.resize(idx + 1, 0);
+    s.den[idx] += 1;
+}
+
+uint64_t scale_ratio_floor(const ScaleTerm& num, const ScaleTerm& den, const MyVector<uint64_t>& q) {
+    if (num.seed != den.seed) throw std::invalid_argument("ckks_evalchebyshev: mixed exact-scale seeds are unsupported");
+    ld r = 1.0L;
+    const int d = num.seed_pow - den.seed_pow;
+    for (int i = 0; i < d; ++i) r *= (ld)num.seed;
+    for (int i = 0; i < -std::min(d, 0); ++i) r /= (ld)num.seed;
+    for (size_t i = 0; i < q.size(); ++i) {
+        const int ne = i < num.den.size() ? num.den[i] : 0;
+        const int de = i < den.den.size() ? den.den[i] : 0;
+        const int diff = de - ne;
+        for (int e = 0; e < diff; ++e) r *= (ld)q[i];
+        for (int e = 0; e < -diff; ++e) r /= (ld)q[i];
+    }
+    const ld f = std::floor(r);
+    if (!std::isfinite(f) || f < 0.0L || f > (ld)std::numeric_limits<uint64_t>::max())
+        throw std::runtime_error("ckks_evalchebyshev: exact-scale ratio exceeds uint64_t");
+    return (uint64_t)f;
+}
+
+size_t chebyshev_degree(const std::vector<double>& c) {
+    if (c.empty()) throw std::invalid_argument("ckks_evalpoly: polynomial must have at least one coefficient");
+    for (double v : c)
+        if (!std::isfinite(v)) throw std::invalid_argument("ckks_evalpoly: polynomial coefficients must be finite");
+    for (size_t i = c.size(); i > 0; --i)
+        if (c[i - 1] != 0.0) return i - 1;
+    return 0;
+}
+
+size_t ceil_log2(size_t d) { return d <= 1 ? 0 : bit_length(d - 1); }
+size_t eval_depth(size_t d) { return d <= 1 ? 0 : bit_length(d) - 1; }
+
+// power-of-two split of the PS recursion (the cheaper of the two neighbours)
+size_t ps_split_log(size_t log_degree) {
+    size_t ls = log_degree >> 1;
+    const size_t cost_a = (size_t{1} << ls) + (size_t{1} << (log_degree - ls)) + log_degree - ls - 3;
+    const size_t cost_b = (size_t{1} << (ls + 1)) + (size_t{1} << (log_degree - ls - 1)) + log_degree - ls - 4;
+    if (cost_a > cost_b) ++ls;
+    return ls;
+}
+
+// T_n = 2 T_a T_b - T_{|a-b|}
+std::pair<size_t, size_t> chebyshev_split(size_t n) {
+    if (n == 0) throw std::invalid_argument("ckks_evalchebyshev: cannot split degree zero");
+    if ((n & (n - 1)) == 0) return {n >> 1, n >> 1};
+    const size_t k = bit_length(n - 1) - 1;
+    return {(size_t{1} << k) - 1, n + 1 - (size_t{1} << k)};
+}
+
+uint64_t scaled_constant_mod_q(double value, ld scale, uint64_t q) {
+    const ld scaled = (ld)value * scale;
+    if (!std::isfinite(scaled)) throw std::invalid_argument("ckks_evalpoly: non-finite scaled constant");
+    ld res = std::fmod(std::round(scaled), (ld)q);
+    if (res < 0.0L) res += (ld)q;
+    uint64_t out = (uint64_t)res;
+    if (out >= q) out %= q;
+    return out;
+}
+
+bool near_integer(double v, double rel = 0x1p-20) {
+    if (!std::isfinite(v)) return false;
+    return std::abs(v - std::round(v)) <= rel * std::max(1.0, std::abs(v));
+}
+
+bool scales_match(double a, double b, double rel = 0x1p-45) {
+    if (!std::isfinite(a) || !std::isfinite(b)) return false;
+    const ld x = a, y = b;
+    return std::abs(x - y) <= (ld)rel * std::max<ld>({1.0L, std::abs(x), std::abs(y)});
+}
+
+// ---- ciphertext helpers over the C-ABI
+void zero_polys(const CKKSContext& c, CKKSCiphertext& ct, size_t from) {
+    const size_t words = (ct.getLevel() + 1) * ct.getN();
+    for (size_t i = from; i < ct.getNumPolys(); ++i)
+        mi_check(mi_memset_zero(c.handle(), ct.polyData(i), words * 8, c.stream()), "ckks_evalpoly");
+}
+
+CKKSCiphertext zero_ct(const CKKSContext& c, size_t level, double scale, CKKSSecretOwner o,
+                       CKKSMessageEncodingState st) {
+    CKKSCiphertext ct(c, 2, scale, level, o, st);
+    zero_polys(c, ct, 0);
+    return ct;
+}
+
+void add_scaled_constant(const CKKSContext& c, CKKSCiphertext& ct, double constant, ld scale) {
+    if (constant == 0.0) return;
+    const auto& q = c.getParams().getModuli();
+    std::vector<uint64_t> r(ct.getLevel() + 1);
+    for (size_t l = 0; l <= ct.getLevel(); ++l) r[l] = scaled_constant_mod_q(constant, scale, q[l]);
+    mi_check(mi_add_scalar(c.handle(), (int)ct.getLevel(), ct.polyData(0), r.data(), c.stream()), "ckks_evalpoly");
+}
+
+void mul_integer(const CKKSContext& c, CKKSCiphertext& ct, ld scalar, ld out_scale) {
+    if (!std::isfinite(scalar) || scalar < 0.0L) throw std::invalid_argument("ckks_evalchebyshev: invalid scalar scale-up");
+    const ld r = std::round(scalar);
+    if (r < 1.0L || std::abs(scalar - r) > std::max<ld>(1.0L, std::abs(scalar)) * 0x1p-20L)
+        throw std::invalid_argument("ckks_evalchebyshev: non-integral scalar scale-up");
+    if (r > (ld)std::numeric_limits<uint64_t>::max())
+        throw std::invalid_argument("ckks_evalchebyshev: scalar scale-up is too large");
+    const uint64_t s = (uint64_t)r;
+    if (s != 1) {
+        const auto& q = c.getParams().getModuli();
+        const size_t lv = ct.getLevel();
+        std::vector<uint64_t> res(lv + 1);
+        for (size_t l = 0; l <= lv; ++l) res[l] = s % q[l];
+        for (size_t i = 0; i < ct.getNumPolys(); ++i)
+            mi_check(mi_mul_scalar(c.handle(), (int)lv, ct.polyData(i), 1, (int64_t)((lv + 1) * ct.getN()), res.data(),
+                                   c.stream()),
+                     "ckks_evalpoly");
+    }
+    ct.setScale((double)out_scale);
+}
+
+void mul_real_then_add(const CKKSContext& c, const CKKSCiphertext& src, double scalar, ld scalar_scale,
+                       CKKSCiphertext& dst) {
+    if (src.getN() != dst.getN()) throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add size mismatch");
+    if (src.getLevel() != dst.getLevel())
+        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add level mismatch");
+    if (src.getNumPolys() != dst.getNumPolys())
+        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add degree mismatch");
+    const auto& q = c.getParams().getModuli();
+    const size_t lv = src.getLevel();
+    std::vector<uint64_t> r(lv + 1);
+    for (size_t l = 0; l <= lv; ++l) r[l] = scaled_constant_mod_q(scalar, scalar_scale, q[l]);
+    for (size_t i = 0; i < src.getNumPolys(); ++i)
+        mi_check(mi_mul_scalar_add(c.handle(), (int)lv, dst.polyData(i), src.polyData(i), 1,
+                                   (int64_t)((lv + 1) * src.getN()), r.data(), c.stream()),
+                 "ckks_evalpoly");
+}
+
+void raise_scale_to(const CKKSContext& c, CKKSCiphertext& ct, ld target) {
+    const double cur = ct.getScale();
+    const double tgt = (double)target;
+    const double m = std::max({1.0, std::abs(cur), std::abs(tgt)});
+    if (std::isfinite(cur) && std::isfinite(tgt) && std::abs(cur - tgt) <= 0x1p-45 * m) {
+        ct.setScale((double)target);
+        return;
+    }
+    if (ct.getScale() > target)
+        throw std::invalid_argument("ckks_evalchebyshev: cannot scale down ciphertext during scale alignment");
+    mul_integer(c, ct, target / ct.getScale(), target);
+}
+
+size_t levels_per_rescale(const CKKSContext& c) { return std::log2(c.getParams().getScale()) <= 64.0 ? 1 : 2; }
+
+ld rescale_divisor(const CKKSContext& c, size_t level) {
+    const auto& q = c.getParams().getModuli();
+    const size_t n = levels_per_rescale(c);
+    if (level + 1 < n) throw std::invalid_argument("ckks_evalchebyshev: insufficient level for rescale modulus product");
+    ld s = 1.0L;
+    for (size_t i = 0; i < n; ++i) s *= (ld)q[level - i];
+    return s;
+}
+
+CKKSCiphertext aligned(const CKKSContext& c, CKKSCiphertext ct, size_t level, double scale, size_t polys) {
+    ckks_drop_to_level_inplace(ct, level);
+    ct.setScale(scale);
+    if (polys < ct.getNumPolys()) throw std::invalid_argument("ckks_evalchebyshev: cannot shrink ciphertext degree by padding");
+    const size_t had = ct.getNumPolys();
+    ct.resizePolys(polys);
+    zero_polys(c, ct, had);
+    return ct;
+}
+
+// out += scalar * op0 with the scales reconciled (MulThenAdd)
+void mul_then_add(const CKKSContext& c, const CKKSCiphertext& op0, double scalar, CKKSCiphertext& out) {
+    if (scalar == 0.0) return;
+    const size_t level = std::min(op0.getLevel(), out.getLevel());
+    const double out_scale = out.getScale();
+    const size_t polys = std::max(op0.getNumPolys(), out.getNumPolys());
+    out = aligned(c, std::move(out), level, out_scale, polys);
+    CKKSCiphertext a = aligned(c, op0.clone(), level, op0.getScale(), polys);
+    ld s_rlwe = 0.0L;
+    if (scales_match(a.getScale(), out.getScale())) {
+        if (near_integer(scalar)) {
+            s_rlwe = 1.0L;
+        } else {
+            s_rlwe = rescale_divisor(c, level);
+            raise_scale_to(c, out, out.getScale() * s_rlwe);
+        }
+    } else if (a.getScale() < out.getScale()) {
+        s_rlwe = out.getScale() / a.getScale();
+    } else {
+        std::ostringstream os;
+        os << "ckks_evalchebyshev: MulThenAdd requires op0.Scale <= opOut.Scale (op0=" << (double)a.getScale()
+           << ", opOut=" << (double)out.getScale() << ", level=" << level << ")";
+        throw std::invalid_argument(os.str());
+    }
+    const ld term = a.getScale() * s_rlwe;
+    if (!scales_match((double)term, out.getScale())) throw std::runtime_error("ckks_evalchebyshev: MulThenAdd scale mismatch");
+    mul_real_then_add(c, a, scalar, s_rlwe, out);
+}
+
+// ---- simulated (level, scale) planning
+struct SimOp {
+    size_t level = 0;
+    ld scale = 0.0L;
+};
+
+struct Sim {
+    const MyVector<uint64_t>& q;
+    void rescale(SimOp& o) const {
+        o.scale /= (ld)q[o.level];
+        --o.level;
+    }
+    SimOp mul(const SimOp& a, const SimOp& b) const { return SimOp{std::min(a.level, b.level), a.scale * b.scale}; }
+    std::pair<size_t, ld> baby(bool lead, size_t level, ld scale) const {
+        ld s = scale;
+        if (lead) s *= (ld)q[level];
+        return {level, s};
+    }
+    std::pair<size_t, ld> giant(bool lead, size_t level, ld scale, ld xpow_scale) const {
+        const uint64_t qi = lead ? q[level] : q[level + 1];
+        return {level + 1, scale * (ld)qi / xpow_scale};
+    }
+};
+
+void sim_power(size_t n, std::map<size_t, SimOp>& pb, const Sim& sim) {
+    if (n < 2 || pb.count(n)) return;
+    auto [a, b] = chebyshev_split(n);
+    sim_power(a, pb, sim);
+    sim_power(b, pb, sim);
+    pb[n] = sim.mul(pb.at(a), pb.at(b));
+    sim.rescale(pb.at(n));
+}
+
+struct Poly {
+    std::vector<double> c;
+    size_t max_degree = 0;
+    bool lead = true;
+    size_t level = 0;
+    ld scale = 0.0L;
+    bool even = false, odd = false;
+    size_t degree() const { return c.empty() ? 0 : c.size() - 1; }
+};
+
+// p = q * T_split + r in the Chebyshev basis
+std::pair<Poly, Poly> divide_by_chebyshev(const Poly& p, size_t split) {
+    const size_t deg = p.degree();
+    if (split < (deg >> 1)) throw std::invalid_argument("ckks_evalchebyshev: invalid recursive PS split");
+    Poly quo, rem;
+    quo.c.assign(deg - split + 1, 0.0);
+    rem.c.assign(split, 0.0);
+    for (size_t i = 0; i < split && i < p.c.size(); ++i) rem.c[i] = p.c[i];
+    if (split < p.c.size()) quo.c[0] = p.c[split];
+    for (size_t i = split + 1, j = 1; i <= deg; ++i, ++j) {
+        const double v = p.c[i];
+        if (v == 0.0) continue;
+        quo.c[i - split] += 2.0 * v;
+        rem.c[split - j] -= v;
+    }
+    quo.max_degree = p.max_degree;
+    quo.lead = p.lead;
+    quo.even = p.even;
+    quo.odd = p.odd;
+    rem.max_degree = p.max_degree == deg ? (split == 0 ? 0 : split - 1) : p.max_degree - (deg - split + 1);
+    rem.lead = false;
+    rem.even = p.even;
+    rem.odd = p.odd;
+    return {std::move(quo), std::move(rem)};
+}
+
+std::pair<std::vector<Poly>, SimOp> plan_leaves(size_t log_split, size_t target_level, const Poly& p,
+                                                const std::map<size_t, SimOp>& pb, ld out_scale, const Sim& sim) {
+    const size_t deg = p.degree();
+    const size_t base = size_t{1} << log_split;
+    if (deg < base) {
+        if (p.lead && log_split > 1 &&
+            p.max_degree > ((size_t{1} << bit_length(p.max_degree)) - (size_t{1} << (log_split - 1))))
+            return plan_leaves(ps_split_log(bit_length(deg)), target_level, p, pb, out_scale, sim);
+        Poly leaf = p;
+        auto [lv, sc] = sim.baby(leaf.lead, target_level, out_scale);
+        leaf.level = lv;
+        leaf.scale = sc;
+        return {std::vector<Poly>{std::move(leaf)}, SimOp{lv, sc}};
+    }
+    size_t next = base;
+    while (next < (deg >> 1) + 1) next <<= 1;
+    const SimOp& xpow = pb.at(next);
+    auto [quo, rem] = divide_by_chebyshev(p, next);
+    auto [lv_new, sc_new] = sim.giant(p.lead, target_level, out_scale, xpow.scale);
+    auto [qv, res] = plan_leaves(log_split, lv_new, quo, pb, sc_new, sim);
+    sim.rescale(res);
+    res = sim.mul(res, xpow);
+    auto [rv, tmp] = plan_leaves(log_split, target_level, rem, pb, res.scale, sim);
+    if (!scales_match((double)res.scale, (double)tmp.scale))
+        throw std::runtime_error("ckks_evalchebyshev: simulated PS scale mismatch");
+    qv.insert(qv.end(), rv.begin(), rv.end());
+    return {std::move(qv), res};
+}
+
+// ---- power basis on the device
+class Powers {
+public:
+    Powers(const CKKSContext& c, const CKKSCiphertext& x) : c_(c), q_(c.getParams().getModuli()) {
+        ct_.emplace(1, x.clone());
+        sc_.emplace(1, ScaleTerm::of_integer_scale((double)x.getScale(), q_.size()));
+    }
+    void gen(size_t n) {
+        if (n < 2 || ct_.count(n)) return;
+        if (gen_unrescaled(n)) rescale_entry(n);
+    }
+    const CKKSCiphertext& at(size_t n) const { return ct_.at(n); }
+
+private:
+    void rescale_entry(size_t n) {
+        CKKSCiphertext& v = ct_.at(n);
+        const size_t lv = v.getLevel();
+        ckks_rescale(c_, v);
+        scale_divide_by_modulus(sc_.at(n), lv);
+        v.setScale(sc_.at(n).value_d(q_));
+    }
+    bool gen_unrescaled(size_t n) {
+        if (ct_.count(n)) return false;
+        const auto [a, b] = chebyshev_split(n);
+        const bool ra = gen_unrescaled(a);
+        const bool rb = gen_unrescaled(b);
+        if (ra) rescale_entry(a);
+        if (rb) rescale_entry(b);
+        CKKSCiphertext out = ct_.at(a).clone();
+        const size_t level = a == b ? out.getLevel() : std::min(out.getLevel(), ct_.at(b).getLevel());
+        ckks_drop_to_level_inplace(out, level);
+        const ScaleTerm prod = scale_product(sc_.at(a), sc_.at(b));
+        if (a == b) {
+            ckks_square_inplace(c_, out);
+        } else {
+            CKKSCiphertext rhs = ct_.at(b).clone();
+            ckks_drop_to_level_inplace(rhs, level);
+            ckks_mul_inplace(c_, out, rhs);
+        }
+        ckks_relin_hybrid_inplace(c_, out);
+        out.setScale(prod.value_d(q_));
+        mul_integer(c_, out, 2.0L, prod.value(q_));
+        const size_t d = a > b ? a - b : b - a;
+        if (d == 0) {
+            add_scaled_constant(c_, out, -1.0, prod.value(q_));
+        } else {
+            gen(d);
+            CKKSCiphertext corr = ct_.at(d).clone();
+            const uint64_t ratio = scale_ratio_floor(prod, sc_.at(d), q_);
+            corr.setScale(sc_.at(d).value_d(q_));
+            const double cs = corr.getScale();
+            corr = aligned(c_, std::move(corr), out.getLevel(), cs, out.getNumPolys());
+            mul_integer(c_, corr, (ld)ratio, prod.value(q_));
+            ckks_sub_inplace(c_, out, corr);
+        }
+        ct_.insert_or_assign(n, std::move(out));
+        sc_.insert_or_assign(n, prod);
+        return true;
+    }
+
+    const CKKSContext& c_;
+    const MyVector<uint64_t>& q_;
+    std::map<size_t, CKKSCiphertext> ct_;
+    std::map<size_t, ScaleTerm> sc_;
+};
+
+bool coeff_used(const Poly& p, size_t e) {
+    return (!p.even && !p.odd) || ((e & 1) == 0 && p.even) || ((e & 1) == 1 && p.odd);
+}
+
+CKKSCiphertext eval_leaf(const CKKSContext& c, const Poly& p, const Powers& pw) {
+    const size_t deg = p.degree();
+    const CKKSCiphertext& x1 = pw.at(1);
+    CKKSCiphertext res = zero_ct(c, p.level, (double)p.scale, x1.getSecretOwner(), x1.getMessageEncodingState());
+    if (p.even && p.c[0] != 0.0) add_scaled_constant(c, res, p.c[0], p.scale);
+    if (deg == 0) return res;
+    for (size_t e = deg; e > 0; --e) {
+        const double v = p.c[e];
+        if (v == 0.0 || !coeff_used(p, e)) continue;
+        const size_t polys = res.getNumPolys();
+        res = aligned(c, std::move(res), p.level, (double)p.scale, polys);
+        mul_then_add(c, pw.at(e), v, res);
+    }
+    return res;
+}
+
+// odd <- odd * x^(2^k) + even
+void combine_giant(const CKKSContext& c, CKKSCiphertext& even, CKKSCiphertext& odd, const CKKSCiphertext& xpow) {
+    if (odd.getNumPolys() == 3) ckks_relin_hybrid_inplace(c, odd);
+    ckks_rescale(c, odd);
+    CKKSCiphertext xp = xpow.clone();
+    ckks_drop_to_level_inplace(xp, odd.getLevel());
+    ckks_mul_inplace(c, odd, xp);
+    const double es = even.getScale();
+    CKKSCiphertext ea = aligned(c, std::move(even), odd.getLevel(), es, odd.getNumPolys());
+    if (!scales_match(ea.getScale(), odd.getScale())) {
+        std::ostringstream os;
+        os << "ckks_evalchebyshev: EvaluateMonomial scale mismatch (even=" << (double)ea.getScale()
+           << ", odd=" << (double)odd.getScale() << ", level=" << odd.getLevel() << ")";
+        throw std::runtime_error(os.str());
+    }
+    const double common = std::max<double>(ea.getScale(), odd.getScale());
+    ea.setScale(common);
+    odd.setScale(common);
+    ckks_add_inplace(c, odd, ea);
+}
+
+struct Leaf {
+    size_t degree;
+    CKKSCiphertext value;
+};
+
+CKKSCiphertext eval_plan(const CKKSContext& c, const std::vector<Poly>& leaves, const Powers& pw) {
+    const size_t n = leaves.size();
+    std::vector<std::optional<Leaf>> st(n);
+    for (size_t i = 0; i < n; ++i) st[n - i - 1] = Leaf{leaves[i].degree(), eval_leaf(c, leaves[i], pw)};
+    while (st.size() != 1) {
+        std::vector<int> step(st.size(), 0);
+        for (size_t i = 0; i < st.size(); ++i) {
+            if (i == st.size() - 1) step[i] = 2;
+            else if (st[i]->degree == st[i + 1]->degree) {
+                step[i] = 1;
+                ++i;
+            }
+        }
+        for (size_t i = 0; i < st.size(); ++i) {
+            if (step[i] == 2) {
+                st[i]->degree = st[i - 1]->degree;
+            } else if (step[i] == 1) {
+                const size_t d = size_t{1} << bit_length(st[i]->degree);
+                combine_giant(c, st[i]->value, st[i + 1]->value, pw.at(d));
+                st[i + 1]->degree = 2 * d - 1;
+                st[i].reset();
+                ++i;
+            }
+        }
+        size_t w = 0;
+        for (size_t i = 0; i < st.size(); ++i)
+            if (st[i].has_value()) {
+                if (w != i) st[w] = std::move(st[i]);
+                ++w;
+            }
+        st.resize(w);
+    }
+    CKKSCiphertext r = std::move(st.front()->value);
+    if (r.getNumPolys() == 3) ckks_relin_hybrid_inplace(c, r);
+    ckks_rescale(c, r);
+    return r;
+}
+
+}  // namespace
+
+CKKSCiphertext ckks_evalchebyshev_preconditioned(const CKKSContext& c, const CKKSCiphertext& x,
+                                                 const std::vector<double>& coeffs, const CKKSEvalPolyOptions& opt) {
+    const size_t deg = chebyshev_degree(coeffs);
+    const double target = opt.target_scale > 0.0 ? opt.target_scale : (double)x.getScale();
+    if (x.getN() != c.getParams().getN()) throw std::invalid_argument("ckks_evalpoly: ciphertext/context size mismatch");
+    if (x.getNumPolys() != 2) throw std::invalid_argument("ckks_evalpoly: expected a 2-polynomial ciphertext");
+    if (x.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_evalpoly: polynomial evaluation requires a dense-secret ciphertext");
+    if (x.getMessageEncodingState() != CKKSMessageEncodingState::Slots)
+        throw std::invalid_argument("ckks_evalpoly: polynomial evaluation expects slot-encoded input");
+    if (!std::isfinite(target) || target <= 0.0) throw std::invalid_argument("ckks_evalpoly: invalid scale");
+    if (x.getLevel() < ceil_log2(deg))
+        throw std::invalid_argument(
+            "ckks_evalchebyshev_preconditioned: ciphertext level is too small for Chebyshev evaluation depth");
+    if (deg == 0) {
+        CKKSCiphertext out = zero_ct(c, x.getLevel(), target, x.getSecretOwner(), x.getMessageEncodingState());
+        add_scaled_constant(c, out, coeffs[0], target);
+        return out;
+    }
+    if (x.getLevel() < eval_depth(deg))
+        throw std::invalid_argument("ckks_evalchebyshev: insufficient level for polynomial depth");
+    Poly p;
+    p.c.assign(coeffs.begin(), coeffs.begin() + deg + 1);
+    p.max_degree = deg;
+    p.lead = true;
+    p.even = true;
+    p.odd = true;
+    for (size_t i = 1; i < coeffs.size(); i += 2) p.even = p.even && coeffs[i] == 0.0;
+    for (size_t i = 0; i < coeffs.size(); i += 2) p.odd = p.odd && coeffs[i] == 0.0;
+    const size_t log_deg = bit_length(deg);
+    const size_t log_split = ps_split_log(log_deg);
+
+    Powers pw(c, x);
+    pw.gen(size_t{1} << (log_deg - 1));
+    for (size_t i = (size_t{1} << log_split) - 1; i > 2; --i)
+        if (coeff_used(p, i)) pw.gen(i);
+
+    const auto& q = c.getParams().getModuli();
+    Sim sim{q};
+    std::map<size_t, SimOp> pb;
+    pb[1] = SimOp{x.getLevel(), (ld)x.getScale()};
+    sim_power(size_t{1} << log_deg, pb, sim);
+    auto leaves = plan_leaves(log_split, x.getLevel() - eval_depth(deg), p, pb, (ld)target, sim).first;
+
+    CKKSCiphertext out = eval_plan(c, leaves, pw);
+    out.setScale(target);
+    return out;
+}

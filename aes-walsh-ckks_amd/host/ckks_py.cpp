@@ -227,5 +227,60 @@ PYBIND11_MODULE(ckks_py, m) {
     m.def("ckks_rotate", &ckks_rotate, py::keep_alive<0, 1>());
     m.def("ckks_conjugate_inplace", &ckks_conjugate_inplace);
     m.def("ckks_conjugate", &ckks_conjugate, py::keep_alive<0, 1>());
+    py::enum_<CKKSBootstrapDftType>(m, "CKKSBootstrapDftType")
+        .value("SlotsToCoeffs", CKKSBootstrapDftType::SlotsToCoeffs)
+        .value("CoeffsToSlots", CKKSBootstrapDftType::CoeffsToSlots);
+    py::class_<CKKSBootstrapDftMatrixLiteral>(m, "CKKSBootstrapDftMatrixLiteral")
+        .def(py::init<>())
+        .def_readwrite("type", &CKKSBootstrapDftMatrixLiteral::type)
+        .def_readwrite("level", &CKKSBootstrapDftMatrixLiteral::level)
+        .def_readwrite("baby_step_count", &CKKSBootstrapDftMatrixLiteral::baby_step_count)
+        .def_readwrite("stage_count", &CKKSBootstrapDftMatrixLiteral::stage_count)
+        .def_readwrite("log_plaintext_scale_groups", &CKKSBootstrapDftMatrixLiteral::log_plaintext_scale_groups)
+        .def_readwrite("merge_depths", &CKKSBootstrapDftMatrixLiteral::merge_depths)
+        .def_readwrite("log_bsgs_ratio", &CKKSBootstrapDftMatrixLiteral::log_bsgs_ratio)
+        .def_readwrite("active_slots", &CKKSBootstrapDftMatrixLiteral::active_slots)
+        .def_readwrite("transform_scale", &CKKSBootstrapDftMatrixLiteral::transform_scale)
+        .def_readwrite("plaintext_scale", &CKKSBootstrapDftMatrixLiteral::plaintext_scale)
+        .def_readwrite("input_selector_start", &CKKSBootstrapDftMatrixLiteral::input_selector_start)
+        .def_readwrite("input_selector_size", &CKKSBootstrapDftMatrixLiteral::input_selector_size)
+        .def_readwrite("rescale_after_each_stage", &CKKSBootstrapDftMatrixLiteral::rescale_after_each_stage);
+    py::class_<CKKSLinearTransformOwnedStage>(m, "CKKSLinearTransformOwnedStage")
+        .def_readonly("diagonal_indices", &CKKSLinearTransformOwnedStage::diagonal_indices)
+        .def_readonly("slot_count", &CKKSLinearTransformOwnedStage::slot_count)
+        .def_readonly("baby_step_count", &CKKSLinearTransformOwnedStage::baby_step_count)
+        .def_readonly("qp_limb_count", &CKKSLinearTransformOwnedStage::qp_limb_count)
+        .def_readonly("level", &CKKSLinearTransformOwnedStage::level)
+        .def_readonly("plaintext_scale", &CKKSLinearTransformOwnedStage::plaintext_scale)
+        .def_readonly("rescale_after", &CKKSLinearTransformOwnedStage::rescale_after)
+        .def_readonly("output_encoding_state", &CKKSLinearTransformOwnedStage::output_encoding_state)
+        .def("downloadDiagonalQp", [](const CKKSLinearTransformOwnedStage& st, const CKKSContext& c, size_t t) {
+            return to_np(st.downloadDiagonalQp(c, t));
+        });
+    py::class_<CKKSLinearTransformPlan>(m, "CKKSLinearTransformPlan")
+        .def("stageCount", [](const CKKSLinearTransformPlan& p) { return p.stages.size(); })
+        .def("stage", [](const CKKSLinearTransformPlan& p, size_t i) -> const CKKSLinearTransformOwnedStage& {
+                 return p.stages.at(i);
+             },
+             py::return_value_policy::reference_internal);
+    m.def("ckks_generate_bootstrap_dft_plan", &ckks_generate_bootstrap_dft_plan, py::keep_alive<0, 1>());
+    m.def("ckks_apply_linear_transform_plan",
+          [](const CKKSContext& c, const CKKSCiphertext& ct, const CKKSLinearTransformPlan& plan) {
+              return ckks_apply_linear_transform_plan(c, ct.clone(), plan);
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_apply_linear_transform_plan_stage",
+          [](const CKKSContext& c, const CKKSCiphertext& ct, const CKKSLinearTransformPlan& plan, size_t i) {
+              return ckks_apply_linear_transform_owned_stage_qp(c, ct, plan.stages.at(i));
+          },
+          py::keep_alive<0, 1>());
     m.def("rotation_automorphism_index", &rotation_automorphism_index);
+    m.def("ckks_evalchebyshev_preconditioned",
+          [](const CKKSContext& c, const CKKSCiphertext& x, std::vector<double> coeffs, double target_scale) {
+              CKKSEvalPolyOptions o;
+              o.target_scale = target_scale;
+              return ckks_evalchebyshev_preconditioned(c, x, coeffs, o);
+          },
+          py::arg("context"), py::arg("x"), py::arg("coefficients"), py::arg("target_scale") = 0.0,
+          py::keep_alive<0, 1>());
 }

@@ -31,8 +31,10 @@ _SIGS = {
     "mi_set_memory_limit": (cint, [vp, i64]),
     "mi_memcpy": (cint, [vp, vp, vp, ctypes.c_size_t, cint, vp]),
     "mi_stream_synchronize": (cint, [vp, vp]),
-    "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, i64, vp]),
-    "mi_poly_scalar": (cint, [vp, cint, vp, vp, cint, cint, i64, vp]),
+    "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),
+    "mi_mul_scalar_add": (cint, [vp, cint, vp, vp, cint, i64, vp, vp]),
+    "mi_memset_zero": (cint, [vp, vp, ctypes.c_size_t, vp]),
+    "mi_poly_scalar": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),
     "mi_ntt_forward": (cint, [vp, vp, cint, cint, i64, cint, vp]),
     "mi_ntt_inverse": (cint, [vp, vp, cint, cint, i64, cint, vp]),
     "mi_tensor": (cint, [vp, cint, vp, vp, vp, vp, vp, vp]),
@@ -154,16 +156,17 @@ class Context:
         check(lib().mi_memory_stats(self.h, ctypes.addressof(st)))
         return tuple(int(v) for v in st)
 
-    def poly_op(self, op, x, y, limbs, npolys=1, stride=None, stream=None):
-        """op: 0 add, 1 sub, 2 mul, 3 Montgomery mul over basis-ordered limbs (Q then P)."""
+    def poly_op(self, op, x, y, limbs, basis0=0, npolys=1, stride=None, stream=None):
+        """op: 0 add, 1 sub, 2 mul, 3 Montgomery mul; limb l uses basis basis0 + l."""
         stride = limbs * self.N if stride is None else stride
-        check(lib().mi_poly_op(self.h, op, ptr(x), ptr(y), limbs, npolys, stride, _stream(stream)))
+        check(lib().mi_poly_op(self.h, op, ptr(x), ptr(y), limbs, basis0, npolys, stride, _stream(stream)))
 
-    def poly_scalar(self, op, x, residues, limbs, npolys=1, stride=None, stream=None):
+    def poly_scalar(self, op, x, residues, limbs, basis0=0, npolys=1, stride=None, stream=None):
         import numpy as np
         r = np.ascontiguousarray(np.asarray(residues, dtype=np.uint64))
         stride = limbs * self.N if stride is None else stride
-        check(lib().mi_poly_scalar(self.h, op, ptr(x), r.ctypes.data, limbs, npolys, stride, _stream(stream)))
+        check(lib().mi_poly_scalar(self.h, op, ptr(x), r.ctypes.data, limbs, basis0, npolys, stride,
+                                   _stream(stream)))
 
     # ---- NTT
     def ntt_forward(self, data, npolys, limbs, stride=None, basis0=0, stream=None):

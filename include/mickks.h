@@ -75,16 +75,17 @@ enum { MI_COPY_H2D = 0, MI_COPY_D2H = 1, MI_COPY_D2D = 2 };
 int mi_memcpy(mi_context* ctx, void* dst, const void* src, size_t bytes, int kind, void* stream);
 int mi_stream_synchronize(mi_context* ctx, void* stream);
 
-/* ---- element-wise over basis-ordered limbs (Q limbs 0..nq-1, then P) -------
+/* ---- element-wise over a basis range (limb l uses basis index basis0 + l) ---
  * pointwise_multiply*_dispatch and poly_add/sub_modq (ntt.cpp:450-579,
- * polynomial.cpp:6-32) over limbs 0..limbs-1 of the full Q u P basis, as key
- * generation and key folding use them (hybrid_key_folding.hpp:50-77,
- * hybrid_key_montgomery.hpp:12-36).  op: x (op)= y, or x (op)= residues[l]. */
+ * polynomial.cpp:6-32) as key generation, key folding and plaintext
+ * Montgomery conversion use them over Q u P (hybrid_key_folding.hpp:50-77,
+ * hybrid_key_montgomery.hpp:12-36, ckks_encoding.cpp:172-190).
+ * op: x (op)= y, or x (op)= residues[l]. */
 enum { MI_OP_ADD = 0, MI_OP_SUB = 1, MI_OP_MUL = 2, MI_OP_MUL_MONT = 3 };
-int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int npolys,
-               int64_t poly_stride, void* stream);
+int mi_poly_op(mi_context* ctx, int op, uint64_t* x, const uint64_t* y, int limbs, int basis0,
+               int npolys, int64_t poly_stride, void* stream);
 int mi_poly_scalar(mi_context* ctx, int op, uint64_t* x, const uint64_t* residues, int limbs,
-                   int npolys, int64_t poly_stride, void* stream);
+                   int basis0, int npolys, int64_t poly_stride, void* stream);
 
 /* ---- negacyclic NTT (ntt.cpp:235-447) -----------------------------------
  * Batched over npolys polys of `limbs` limbs each; limb l of every poly uses
@@ -108,6 +109,13 @@ int mi_mul_plain(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_
 /* ckks_mul_inplace(ct, int64), ckks_mul.cpp:214-272: residues[i] = scalar mod q_i (host array) */
 int mi_mul_scalar(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
                   const uint64_t* residues, void* stream);
+/* fused_multiply_by_real_scalar_then_add_inplace, ckks_evalpoly.cpp:352-398:
+ * dst += src * residues[i] per limb i (host residues, the rounded scaled
+ * constant mod q_i) */
+int mi_mul_scalar_add(mi_context* ctx, int level, uint64_t* dst, const uint64_t* src, int npolys,
+                      int64_t poly_stride, const uint64_t* residues, void* stream);
+/* zero polynomials (zero_ciphertext_like, ckks_evalpoly.cpp:258-275) */
+int mi_memset_zero(mi_context* ctx, void* ptr, size_t bytes, void* stream);
 /* ckks_add_inplace / ckks_sub_inplace (ct, ct), ckks_addition.cpp:62-139 */
 int mi_add(mi_context* ctx, int level, uint64_t* x, const uint64_t* y, int npolys,
            int64_t poly_stride, void* stream);

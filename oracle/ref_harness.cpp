@@ -28,6 +28,7 @@
 #include "ckks_rotation.hpp"
 #include "ckks_linear_transform.hpp"
 #include "ckks_modraise.hpp"
+#include "ckks_evalpoly.hpp"
 #include "ntt.hpp"
 #include "random.hpp"
 
@@ -391,18 +392,76 @@ static void run_bootstrap_pieces() {
     dumpv(tag + "_s2c_meta", meta);
 }
 
+// Chebyshev evaluation (Paterson-Stockmeyer with exact-scale tracking) of
+// harness-chosen coefficients on an encrypted input in [-1, 1].
+static void run_evalpoly() {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << 10;
+    c.mult_depth = 8;
+    c.q0_bit = 50;
+    c.q_bit = 40;
+    c.p_bits = {60, 60, 60};
+    c.dnum = 3;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN(), L = P.getMaxLevel();
+    const std::string tag = "ep10";
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    std::vector<uint64_t> sk;
+    for (const auto& limb : ctx.getSecretKey()) sk.insert(sk.end(), limb.begin(), limb.end());
+    dumpv(tag + "_sk", sk);
+    dump_key(tag + "_relin", ctx.getRelinHybrid(L));
+    uint64_t seed = 4242;
+    MyVector<Complex> m(N / 2);
+    for (auto& z : m) z = Complex((double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1, 0.0);
+    dumpv(tag + "_m", m);
+    auto ct = ckks_encrypt(ctx, m, L);
+    dump_ct(tag + "_ct", ct);
+    // coefficient sets: generic degree 31, even degree 30, odd degree 15,
+    // generic degree 63, degree 2, degree 1
+    const size_t degrees[6] = {31, 30, 15, 63, 2, 1};
+    const int parity[6] = {0, 2, 1, 0, 0, 0};  // 0 any, 1 odd, 2 even
+    for (int k = 0; k < 6; ++k) {
+        std::vector<double> co(degrees[k] + 1, 0.0);
+        for (size_t i = 0; i <= degrees[k]; ++i) {
+            const double u = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 - 0.5;
+            if ((parity[k] == 1 && i % 2 == 0) || (parity[k] == 2 && i % 2 == 1)) continue;
+            co[i] = u / (double)(i + 1);
+        }
+        dumpv(tag + "_coeffs" + std::to_string(k), co);
+        auto out = ckks_evalchebyshev_preconditioned(ctx, ct, co);
+        dump_ct(tag + "_out" + std::to_string(k), out);
+    }
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR\n"); return 2; }
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR [case ...]\n"); return 2; }
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);
-    run_primes();
-    run_ntt(1 << 10, {60, 59, 50, 36}, "ntt10", true);
-    run_ntt(1 << 12, {61, 55, 45, 30}, "ntt12", false);
-    run_ntt(1 << 16, {60}, "ntt16one", false);
-    run_eval(10, 3, 2, {60, 60}, "ev10");
-    run_eval(10, 4, 3, {61, 61}, "ev10b");
-    run_hoist_lt();
-    run_bootstrap_pieces();
+    // cases: primes ntt eval lt boot evalpoly (all when none is named)
+    auto want = [&](const char* name) {
+        if (argc < 3) return true;
+        for (int i = 2; i < argc; ++i)
+            if (std::strcmp(argv[i], name) == 0) return true;
+        return false;
+    };
+    if (want("primes")) run_primes();
+    if (want("ntt")) {
+        run_ntt(1 << 10, {60, 59, 50, 36}, "ntt10", true);
+        run_ntt(1 << 12, {61, 55, 45, 30}, "ntt12", false);
+        run_ntt(1 << 16, {60}, "ntt16one", false);
+    }
+    if (want("eval")) {
+        run_eval(10, 3, 2, {60, 60}, "ev10");
+        run_eval(10, 4, 3, {61, 61}, "ev10b");
+    }
+    if (want("lt")) run_hoist_lt();
+    if (want("boot")) run_bootstrap_pieces();
+    if (want("evalpoly")) run_evalpoly();
     std::printf("ref_harness: wrote %s\n", g_out.c_str());
     return 0;
 }

@@ -1,14 +1,20 @@
 """Pack the reference harness output into the committed golden fixtures.
 
 Run in a container where /root/reference exists:
-    make -C oracle ref && oracle/_ref/ref_harness oracle/_ref/raw
-    python tests/golden/make_golden.py
+    make -C oracle ref && oracle/_ref/ref_harness oracle/_ref/raw [case ...]
+    python tests/golden/make_golden.py [--raw DIR] [group ...]
+
+Groups (all when none is named): primes ntt eval lt boot evalpoly.  The
+harness draws keys and encryption masks from the reference's own
+unseedable randomness, so repack only the groups whose raw arrays were
+regenerated together.
 
 The harness (oracle/ref_harness.cpp) runs the reference's own C++ code; this
 script only repacks its raw arrays.  Inputs are stored in full; large outputs
 are stored as SHA-256 digests of their little-endian uint64 bytes (tests hash
 their own results and compare), small ones in full.
 """
+import argparse
 import hashlib
 import json
 import os
@@ -27,7 +33,7 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def main():
+def pack_primes():
     primes = {}
     for tag in ["walsh15", "xboot13", "xboot14", "xboot15", "chain14", "ks16"]:
         primes[tag] = {"q": [int(v) for v in rd(tag + "_q")], "p": [int(v) for v in rd(tag + "_p")]}
@@ -35,6 +41,8 @@ def main():
     with open(os.path.join(HERE, "primes.json"), "w") as f:
         json.dump(primes, f, indent=1)
 
+
+def pack_ntt():
     ntt = {}
     for tag, N in [("ntt10", 1 << 10), ("ntt12", 1 << 12), ("ntt16one", 1 << 16)]:
         q = rd(tag + "_q")
@@ -58,6 +66,8 @@ def main():
         fwd=rd("ntt10_fwd0"), fwds=rd("ntt10_fwds0"), inv=rd("ntt10_inv0"), invs=rd("ntt10_invs0"),
         fwd_out=rd("ntt10_fwdout0"), inv_out=rd("ntt10_invout0"))
 
+
+def pack_eval():
     outs = ["mul", "mul_relin", "mul_relin_rescale", "chain_rot1", "square", "rot1_out", "rot5_out",
             "conj_out", "rescale_ct1", "rescale2_ct1", "add", "sub", "mulint", "addconst", "mulplain",
             "rot1_level1"]
@@ -77,6 +87,9 @@ def main():
         np.savez_compressed(os.path.join(HERE, tag + ".npz"), **arr)
         with open(os.path.join(HERE, tag + "_outputs.json"), "w") as f:
             json.dump(meta, f, indent=1)
+
+
+def pack_lt():
     # hoisted switches and a QP BSGS stage
     q, p = rd("lt10_q"), rd("lt10_p")
     N = 1 << 10
@@ -98,6 +111,8 @@ def main():
     with open(os.path.join(HERE, "lt10_outputs.json"), "w") as f:
         json.dump(lt_meta, f, indent=1)
 
+
+def pack_boot():
     # bootstrap pieces: dense->sparse q0p0 switch, ModRaise, S2C plan digests
     meta = rd("bt10_s2c_meta")
     stages, i = [], 0
@@ -113,6 +128,41 @@ def main():
                         ct0=rd("bt10_ct0"), key_d2s=rd("bt10_d2s"))
     with open(os.path.join(HERE, "bt10_outputs.json"), "w") as f:
         json.dump(bt_meta, f, indent=1)
+
+
+def ct_meta(name):
+    m = rd(name + "_meta")
+    return {"digest": digest(rd(name)), "npolys": int(m[0]), "level": int(m[1]),
+            "scale": float(m[2:3].view(np.float64)[0])}
+
+
+def pack_evalpoly():
+    # Chebyshev evaluation of six coefficient sets on one encrypted input
+    arr = {"q": rd("ep10_q"), "p": rd("ep10_p"), "sk": rd("ep10_sk"), "key_relin": rd("ep10_relin"),
+           "ct": rd("ep10_ct"), "m": rd("ep10_m", np.complex128)}
+    meta = {"ct": {"level": int(rd("ep10_ct_meta")[1]),
+                   "scale": float(rd("ep10_ct_meta")[2:3].view(np.float64)[0])}}
+    for k in range(6):
+        arr["coeffs%d" % k] = rd("ep10_coeffs%d" % k, np.float64)
+        meta["out%d" % k] = ct_meta("ep10_out%d" % k)
+    np.savez_compressed(os.path.join(HERE, "ep10.npz"), **arr)
+    with open(os.path.join(HERE, "ep10_outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+GROUPS = {"primes": pack_primes, "ntt": pack_ntt, "eval": pack_eval, "lt": pack_lt, "boot": pack_boot,
+          "evalpoly": pack_evalpoly}
+
+
+def main():
+    global RAW
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--raw", default=RAW)
+    ap.add_argument("groups", nargs="*", choices=sorted(GROUPS) + [[]], default=[])
+    args = ap.parse_args()
+    RAW = args.raw
+    for g in args.groups or sorted(GROUPS):
+        GROUPS[g]()
     print("golden fixtures written to", HERE)
 
 

@@ -283,3 +283,132 @@ def test_host_api_errors_and_memory_accounting():
     import gc
     gc.collect()
     assert ctx.memoryStats()[0] == live0
+
+
+def test_host_evalchebyshev_matches_reference():
+    """Paterson-Stockmeyer Chebyshev evaluation of six coefficient sets
+    (generic, even, odd, degree 63, 2, 1) reproduces the reference's output
+    ciphertexts bit for bit, with the reference's key and input installed."""
+    H = host()
+    d = load_npz("ep10.npz")
+    outs = load_json("ep10_outputs.json")
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 8
+    c.q0_bit = 50
+    c.q_bit = 40
+    c.p_bits = [60, 60, 60]
+    c.dnum = 3
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    assert list(P.getModuli()) == list(d["q"]) and list(P.getPModuli()) == list(d["p"])
+    ctx.setSecretKeyEval(d["sk"])
+    ctx.setRelinKey(d["key_relin"])
+    N, L = 1 << 10, len(d["q"])
+    W = L * N
+    x = H.CKKSCiphertext(ctx, [d["ct"][:W], d["ct"][W:]], outs["ct"]["scale"], L - 1)
+    m = d["m"].real
+    for k in range(6):
+        co = d["coeffs%d" % k]
+        out = H.ckks_evalchebyshev_preconditioned(ctx, x, list(co))
+        flat = np.concatenate([out.getPoly(i) for i in range(out.getNumPolys())])
+        ref = outs["out%d" % k]
+        assert digest(flat) == ref["digest"], k
+        assert out.getLevel() == ref["level"] and out.getScale() == ref["scale"], k
+        # value check: the decrypted slots follow sum c_i T_i(m); the leaf that
+        # holds T_0 adds the constant only for even polynomials, as the
+        # reference's leaf evaluation does
+        dec = H.ckks_decrypt(ctx, out).real
+        want = np.polynomial.chebyshev.chebval(m, co)
+        if any(co[1::2]) and co[0] != 0.0:
+            want = want - co[0]
+        assert np.max(np.abs(dec - want)) < 1e-5, k
+
+
+def _bt10_context(H):
+    """The reference harness's bootstrapping-pieces context: N=2^10, chain
+    [40, 25, 25, 25, 35, 25, 25, 25] (S2C groups, one mult limb, C2S groups),
+    P = 2 x 60 bits, dnum 2, scale 2^35."""
+    d = load_npz("bt10.npz")
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.q_bits = [40, 25, 25, 25, 35, 25, 25, 25]
+    c.q_bit = 35
+    c.p_bits = [60, 60]
+    c.dnum = 2
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    assert list(P.getModuli()) == list(d["q"]) and list(P.getPModuli()) == list(d["p"])
+    return ctx, d
+
+
+def _s2c_literal(H, q):
+    lit = H.CKKSBootstrapDftMatrixLiteral()
+    lit.type = H.CKKSBootstrapDftType.SlotsToCoeffs
+    lit.level = 3
+    lit.stage_count = 3
+    lit.log_plaintext_scale_groups = [[25], [25], [25]]
+    lit.transform_scale = (float(q[0]) * 0.5) / 2.0 ** 35
+    lit.plaintext_scale = float(q[3])
+    return lit
+
+
+def test_host_s2c_dft_plan_matches_reference_and_decodes():
+    """The factorised S2C plan's QP diagonals equal the reference's bit for bit
+    (bt10 digests); applying it maps slots v to coefficients 16 v
+    (transform scale (q0/2)/2^35) and each stage matches the oracle's BSGS
+    stage plus rescale."""
+    H = host()
+    ctx, d = _bt10_context(H)
+    o = load_json("bt10_outputs.json")
+    P = ctx.getParams()
+    q, p = P.getModuli(), P.getPModuli()
+    lit = _s2c_literal(H, q)
+    plan = H.ckks_generate_bootstrap_dft_plan(ctx, lit)
+    assert plan.stageCount() == len(o["s2c_stages"])
+    for i, ref in enumerate(o["s2c_stages"]):
+        st = plan.stage(i)
+        assert st.baby_step_count == ref["baby_steps"] and st.qp_limb_count == ref["qp_limbs"], i
+        assert list(st.diagonal_indices) == ref["diag_indices"], i
+        flat = np.concatenate([st.downloadDiagonalQp(ctx, t) for t in range(len(st.diagonal_indices))])
+        assert digest(flat) == ref["digest"], i
+
+    N = 1 << 10
+    for i in range(plan.stageCount()):
+        st = plan.stage(i)
+        for dgi in st.diagonal_indices:
+            for r in (dgi % st.baby_step_count, (dgi // st.baby_step_count) * st.baby_step_count):
+                if r:
+                    ctx.addRotationKey(int(r))
+    rng = np.random.default_rng(8)
+    v = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    ct = H.ckks_encrypt(ctx, v, 3)
+    out = H.ckks_apply_linear_transform_plan(ctx, ct, plan)
+    assert out.getLevel() == 0
+    assert out.getMessageEncodingState() == H.CKKSMessageEncodingState.Coefficients
+    dec = H.ckks_decrypt(ctx, out, H.CKKSDecryptDomain.Coefficients)
+    assert np.max(np.abs(dec - lit.transform_scale * v)) < 1e-2
+
+    # stage 0 against the oracle's BSGS stage and rescale
+    octx = O.Ctx(N, q, p, 2)
+    st = plan.stage(0)
+    L = st.level + 1
+    diags = np.concatenate([st.downloadDiagonalQp(ctx, t) for t in range(len(st.diagonal_indices))])
+    keys = {}
+    for dgi in st.diagonal_indices:
+        for r in (dgi % st.baby_step_count, (dgi // st.baby_step_count) * st.baby_step_count):
+            if r:
+                keys[int(r)] = ctx.getRotationKey(int(r))
+    ao, bo = octx.lt_stage_qp(st.level, ct.getPoly(1), ct.getPoly(0), list(st.diagonal_indices), diags,
+                               st.baby_step_count, keys)
+    r = octx.rescale(st.level, np.stack([bo, ao]))
+    x = H.CKKSCiphertext(ctx, [ct.getPoly(0), ct.getPoly(1)], ct.getScale(), 3)
+    y = H.ckks_apply_linear_transform_plan_stage(ctx, x, plan, 0)
+    assert np.array_equal(y.getPoly(0), r[0]) and np.array_equal(y.getPoly(1), r[1])
+    assert L == 4
