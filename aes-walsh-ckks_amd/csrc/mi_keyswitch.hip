@@ -177,22 +177,35 @@ __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
     const int key_limbs = a.nq + a.K;
     const BasisConst b = a.bc[basis];
     const long long src = a.galois ? (long long)galois_src((u32)i, a.galois, a.logN) : i;
-    u64 acc_a = 0, acc_b = 0;
+    // Lazy 128-bit sums of the digit products, brought below q every three
+    // products (3 q^2 + q < q 2^64 for q < 2^62) and reduced once at the end:
+    // REDC for Montgomery-form keys, Barrett otherwise.  The result is the
+    // canonical residue of the per-product sum.
+    U128 sb{0, 0}, sa{0, 0};
     for (int d = 0; d < a.parts; ++d) {
-        const u64 x = a.digits[((long long)d * slots + slot) * N + src];
+        u64 x = a.digits[((long long)d * slots + slot) * N + src];
         const u64* kb = a.key + ((long long)d * 2 * key_limbs + basis) * N;
         const u64* ka = kb + (long long)key_limbs * N;
-        u64 tb, ta;
-        if (a.montgomery) {
-            tb = mont_mul(x, kb[src], b.q, b.qninv);
-            ta = mont_mul(x, ka[src], b.q, b.qninv);
-        } else {
-            const u64 xr = reduce_4q(x, b.q);
-            tb = mul_mod(xr, reduce_4q(kb[src], b.q), b.q, b.mu_lo, b.mu_hi);
-            ta = mul_mod(xr, reduce_4q(ka[src], b.q), b.q, b.mu_lo, b.mu_hi);
+        u64 kbv = kb[src], kav = ka[src];
+        if (!a.montgomery) {
+            x = reduce_4q(x, b.q);
+            kbv = reduce_4q(kbv, b.q);
+            kav = reduce_4q(kav, b.q);
         }
-        acc_b = add_mod(acc_b, tb, b.q);
-        acc_a = add_mod(acc_a, ta, b.q);
+        mac128(sb, x, kbv);
+        mac128(sa, x, kav);
+        if (d % 3 == 2 && d + 1 < a.parts) {
+            sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
+            sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
+        }
+    }
+    u64 acc_b, acc_a;
+    if (a.montgomery) {
+        acc_b = redc128(sb.lo, sb.hi, b.q, b.qninv);
+        acc_a = redc128(sa.lo, sa.hi, b.q, b.qninv);
+    } else {
+        acc_b = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+        acc_a = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
     }
     if (a.qp_out) {
         if (a.init_b && slot < a.L) acc_b = add_mod(acc_b, a.init_b[(long long)slot * N + src], b.q);

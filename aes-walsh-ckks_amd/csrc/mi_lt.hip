@@ -47,25 +47,32 @@ __global__ void __launch_bounds__(LT_THREADS) k_lt_inner(LtInnerArgs a) {
     const int basis = slot < a.L ? slot : a.nq + (slot - a.L);
     const BasisConst b = a.bc[basis];
     const long long o = (long long)slot * N + i;
-    u64 acc_a = a.accumulate ? a.out_a[o] : 0;
-    u64 acc_b = a.accumulate ? a.out_b[o] : 0;
+    // lazy 128-bit sums, brought below q every three products and reduced
+    // once (REDC for Montgomery-form plaintexts, Barrett otherwise)
+    U128 sa{0, 0}, sb{0, 0};
+    int pending = 0;
     for (int t = 0; t < a.nterms; ++t) {
         const LtTerm& tm = a.t[t];
         if (tm.q_only && slot >= a.L) continue;
         const u64 p = tm.pt[o];
-        u64 ta, tb;
-        if (a.pt_mont) {
-            ta = mont_mul(tm.a[o], p, b.q, b.qninv);
-            tb = mont_mul(tm.b[o], p, b.q, b.qninv);
-        } else {
-            ta = mul_mod(tm.a[o], p, b.q, b.mu_lo, b.mu_hi);
-            tb = mul_mod(tm.b[o], p, b.q, b.mu_lo, b.mu_hi);
+        mac128(sa, tm.a[o], p);
+        mac128(sb, tm.b[o], p);
+        if (++pending == 3) {
+            sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
+            sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
+            pending = 0;
         }
-        acc_a = add_mod(acc_a, ta, b.q);
-        acc_b = add_mod(acc_b, tb, b.q);
     }
-    a.out_a[o] = acc_a;
-    a.out_b[o] = acc_b;
+    u64 ra, rb;
+    if (a.pt_mont) {
+        ra = redc128(sa.lo, sa.hi, b.q, b.qninv);
+        rb = redc128(sb.lo, sb.hi, b.q, b.qninv);
+    } else {
+        ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
+        rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+    }
+    a.out_a[o] = a.accumulate ? add_mod(a.out_a[o], ra, b.q) : ra;
+    a.out_b[o] = a.accumulate ? add_mod(a.out_b[o], rb, b.q) : rb;
 }
 
 static u64 powmod_u64(u64 a, u64 e, u64 m) {

@@ -21,8 +21,11 @@
 #include <complex>
 #include <cstddef>
 #include <cstdint>
+#include <array>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -40,8 +43,26 @@ enum class CKKSDecryptDomain { Slots, Coefficients };
 enum class CKKSHybridKeySwitchCombine { Add, Assign };
 
 // ---------------------------------------------------------------- parameters
-// CKKSParamsConfig (ckks_params.hpp:85-117): the fields of the plain (non
-// bootstrapping) chain.
+using CKKSBootstrapDftScaleSchedule = std::vector<std::vector<size_t>>;
+
+// CKKSBootstrapParams (ckks_params.hpp:47-70): index 0 is SlotsToCoeffs,
+// index 1 CoeffsToSlots.
+struct CKKSBootstrapParams {
+    size_t sparse_secret_hamming_weight = 0;
+    size_t active_slot_count = 0;  // 0: N/2
+    std::array<size_t, 2> linear_transform_depths{0, 0};
+    std::array<CKKSBootstrapDftScaleSchedule, 2> linear_transform_log_plaintext_scales;
+    size_t evalmod_depth = 0;
+    size_t evalmod_bit_size = 0;
+    size_t evalmod_k = 0;  // 0: from the sparse-secret overflow bound
+    long double evalmod_k_failure_probability_log2 = -128.0L;
+    size_t evalmod_double_angle = 0;
+    std::array<size_t, 2> linear_transform_bsgs_baby_step_counts{0, 0};
+    std::array<int, 2> linear_transform_log_bsgs_ratios{1, 1};
+    std::array<std::vector<size_t>, 2> linear_transform_merge_depths;
+};
+
+// CKKSParamsConfig (ckks_params.hpp:85-117).
 struct CKKSParamsConfig {
     double scale = 0.0;
     double sigma = 3.19;
@@ -59,6 +80,8 @@ struct CKKSParamsConfig {
     size_t secret_key_hamming_weight = 256;
     std::vector<uint64_t> q_bits;
     MyVector<int64_t> rotate_values;
+    bool bts_keys = false;
+    std::optional<CKKSBootstrapParams> bootstrap_params;
     bool secure = true;
     bool fold_hybrid_keyswitch = true;
     bool montgomery = true;
@@ -88,13 +111,28 @@ public:
     const std::vector<uint64_t>& getPBits() const { return p_bits; }
     const MyVector<int64_t>& getRotateValues() const { return rotate_values; }
     uint64_t getSeed() const { return seed; }
+    size_t getQ0Bit() const { return q0_bit; }
+    size_t getDefaultQBit() const { return q_bit; }
+    // bootstrapping (ckks_params.cpp:300-455)
+    bool hasBootstrapParams() const { return bootstrap.has_value(); }
+    const CKKSBootstrapParams& getBootstrapParams() const;
+    size_t getBootstrapS2CDepth() const;
+    size_t getBootstrapC2SDepth() const;
+    size_t getBootstrapEvalModDepth() const { return getBootstrapParams().evalmod_depth; }
+    size_t getBootstrapActiveSlotCount() const;
+    bool usesSparseBootstrapSlots() const { return getBootstrapActiveSlotCount() != slots; }
+    size_t getBootstrapEvalModK() const;
+    size_t getEvalModLogScaleBits() const;
 
+    bool bts_keys = false;
     bool secure = true;
     bool fold_hybrid_keyswitch = true;
     bool montgomery = true;
 
 private:
     size_t N = 0, slots = 0, depth = 0, mult_depth = 0, dnum = 0, hamming_weight = 0;
+    size_t q0_bit = 0, q_bit = 0;
+    std::optional<CKKSBootstrapParams> bootstrap;
     double scale = 0.0, sigma = 0.0;
     std::vector<uint64_t> q_bits, p_bits;
     MyVector<uint64_t> moduli, pmoduli;
@@ -103,6 +141,9 @@ private:
 };
 
 class CKKSContext;
+namespace ckks_host {
+class Rng;
+}
 
 // ------------------------------------------------------------ device storage
 // One allocation from the context's device pool (mi_alloc / mi_free).
@@ -218,6 +259,61 @@ private:
     DeviceBuffer buf;
 };
 
+// ---------------------------------------------------- bootstrapping DFT plans
+// ckks_bootstrap_dft.hpp:10-40, ckks_linear_transform_plan.hpp:9-47.  Stage
+// diagonals are kept only in the Q u P basis at the stage level, on the device
+// (the evaluator used for every stage is the QP one, as in all the
+// reference's bootstrapping presets).
+enum class CKKSBootstrapDftType { SlotsToCoeffs, CoeffsToSlots };
+
+struct CKKSBootstrapDftMatrixLiteral {
+    CKKSBootstrapDftType type = CKKSBootstrapDftType::SlotsToCoeffs;
+    size_t level = 0;
+    size_t baby_step_count = 0;  // 0: chosen from the diagonal indices
+    size_t stage_count = 1;
+    CKKSBootstrapDftScaleSchedule log_plaintext_scale_groups;
+    std::vector<size_t> merge_depths;
+    int log_bsgs_ratio = 1;
+    size_t active_slots = 0;     // 0: N/2
+    double transform_scale = 1.0;
+    double plaintext_scale = 1.0;
+    size_t input_selector_start = 0;
+    size_t input_selector_size = 0;
+    bool rescale_after_each_stage = true;
+};
+
+struct CKKSLinearTransformOwnedStage {
+    CKKSLinearTransformOwnedStage() = default;
+    CKKSLinearTransformOwnedStage(const CKKSLinearTransformOwnedStage&) = delete;
+    CKKSLinearTransformOwnedStage& operator=(const CKKSLinearTransformOwnedStage&) = delete;
+    CKKSLinearTransformOwnedStage(CKKSLinearTransformOwnedStage&&) noexcept = default;
+    CKKSLinearTransformOwnedStage& operator=(CKKSLinearTransformOwnedStage&&) noexcept = default;
+
+    MyVector<DeviceBuffer> diagonal_qp_plaintexts;  // (level+1+K)*N each
+    MyVector<size_t> diagonal_indices;
+    size_t slot_count = 0;
+    size_t baby_step_count = 0;
+    size_t qp_limb_count = 0;
+    size_t qp_p_limb_count = 0;
+    size_t level = 0;
+    double plaintext_scale = 1.0;
+    bool diagonal_qp_plaintexts_montgomery = false;
+    bool rescale_after = true;
+    CKKSMessageEncodingState output_encoding_state = CKKSMessageEncodingState::Slots;
+    MyVector<uint64_t> downloadDiagonalQp(const CKKSContext& context, size_t term) const;
+};
+
+struct CKKSLinearTransformPlan {
+    CKKSLinearTransformPlan() = default;
+    CKKSLinearTransformPlan(const CKKSLinearTransformPlan&) = delete;
+    CKKSLinearTransformPlan& operator=(const CKKSLinearTransformPlan&) = delete;
+    CKKSLinearTransformPlan(CKKSLinearTransformPlan&&) noexcept = default;
+    CKKSLinearTransformPlan& operator=(CKKSLinearTransformPlan&&) noexcept = default;
+
+    MyVector<CKKSLinearTransformOwnedStage> stages;
+    bool use_qp_evaluator = true;
+};
+
 // ------------------------------------------------------------------- context
 // CKKSContext (ckks_context.hpp:59-90; constructor ckks_context.cpp:603-705):
 // FFT tables and the slot permutation on the host; NTT, ModUp/ModDown and
@@ -269,7 +365,33 @@ public:
     // Encryption randomness: a fresh stream per call, derived from the seed.
     uint64_t nextRandomnessSeed() const;
 
+    // ---- bootstrapping material (ckks_context.cpp:603-705 with bts_keys)
+    bool hasSparseSecretEncapsulationKeys() const { return d2s_key.data() != nullptr; }
+    const MyVector<int8_t>& getLowSecretKeyCoeffs() const { return low_secret_coeffs; }
+    const uint64_t* lowSecretKeyDevice() const { return low_secret_qp.data(); }
+    // [b(q0), b(p0), a(q0), a(p0)], N words each
+    const uint64_t* getDenseToSparseSwitchKeyDevice() const;
+    const uint64_t* getSparseToDenseSwitchKeyDevice() const;
+    void setDenseToSparseSwitchKey(const MyVector<uint64_t>& key);
+    void setSparseToDenseSwitchKey(const MyVector<uint64_t>& key);
+    MyVector<uint64_t> downloadDenseToSparseSwitchKey() const;
+    bool hasEvalModBootstrapDftPlans() const { return s2c_plan != nullptr || c2s_plan != nullptr; }
+    const CKKSLinearTransformPlan& getEvalModS2CPlan() const;
+    const CKKSLinearTransformPlan& getEvalModScaledC2SPlan() const;
+    double getEvalModScaledC2STransformScale() const { return c2s_transform_scale; }
+    // S2C plan for an input whose scale is `ratio` times the default scale
+    // (cached; ckks_bootstrap_evalmod_inplace.cpp:114-154).
+    const CKKSLinearTransformPlan& getScaledS2CPlan(uint64_t ratio) const;
+    // The EvalMod Chebyshev coefficients.  The reference derives them with
+    // MPFR at context construction; here they are supplied by the caller.
+    void setEvalModPolynomial(const std::vector<double>& chebyshev_coeffs);
+    bool hasEvalModPolynomial() const { return !evalmod_coeffs.empty(); }
+    const std::vector<double>& getEvalModPolynomial() const;
+
 private:
+    void release_all();
+    void build_bootstrap_material(ckks_host::Rng& rng);
+    DeviceBuffer generate_q0p0_key(const uint64_t* src_qp, const uint64_t* tgt_qp, uint64_t stream_id) const;
     DeviceBuffer upload_key(const MyVector<uint64_t>& key) const;
     DeviceBuffer generate_switch_key(const uint64_t* src_secret_qp, const uint64_t* tgt_secret_qp,
                                      uint64_t stream_id) const;
@@ -285,6 +407,13 @@ private:
     DeviceBuffer relin_key, conj_key;
     std::map<int64_t, DeviceBuffer> rotation_keys;
     mutable std::atomic<uint64_t> randomness_counter{0};
+    MyVector<int8_t> low_secret_coeffs;
+    DeviceBuffer low_secret_qp, d2s_key, s2d_key;
+    std::unique_ptr<CKKSLinearTransformPlan> s2c_plan, c2s_plan;
+    double c2s_transform_scale = 1.0;
+    mutable std::mutex scaled_mu;
+    mutable std::map<uint64_t, std::unique_ptr<CKKSLinearTransformPlan>> scaled_s2c;
+    std::vector<double> evalmod_coeffs;
 };
 
 // ------------------------------------------------------------ encode / crypt
@@ -355,68 +484,34 @@ CKKSCiphertext ckks_evalchebyshev_preconditioned(const CKKSContext& context, con
                                                  const std::vector<double>& chebyshev_coefficients,
                                                  const CKKSEvalPolyOptions& options = {});
 
-// ---------------------------------------------------- bootstrapping DFT plans
-// ckks_bootstrap_dft.hpp:10-40, ckks_linear_transform_plan.hpp:9-47.  Stage
-// diagonals are kept only in the Q u P basis at the stage level, on the device
-// (the evaluator used for every stage is the QP one, as in all the
-// reference's bootstrapping presets).
-enum class CKKSBootstrapDftType { SlotsToCoeffs, CoeffsToSlots };
-using CKKSBootstrapDftScaleSchedule = std::vector<std::vector<size_t>>;
-
-struct CKKSBootstrapDftMatrixLiteral {
-    CKKSBootstrapDftType type = CKKSBootstrapDftType::SlotsToCoeffs;
-    size_t level = 0;
-    size_t baby_step_count = 0;  // 0: chosen from the diagonal indices
-    size_t stage_count = 1;
-    CKKSBootstrapDftScaleSchedule log_plaintext_scale_groups;
-    std::vector<size_t> merge_depths;
-    int log_bsgs_ratio = 1;
-    size_t active_slots = 0;     // 0: N/2
-    double transform_scale = 1.0;
-    double plaintext_scale = 1.0;
-    size_t input_selector_start = 0;
-    size_t input_selector_size = 0;
-    bool rescale_after_each_stage = true;
-};
-
-struct CKKSLinearTransformOwnedStage {
-    CKKSLinearTransformOwnedStage() = default;
-    CKKSLinearTransformOwnedStage(const CKKSLinearTransformOwnedStage&) = delete;
-    CKKSLinearTransformOwnedStage& operator=(const CKKSLinearTransformOwnedStage&) = delete;
-    CKKSLinearTransformOwnedStage(CKKSLinearTransformOwnedStage&&) noexcept = default;
-    CKKSLinearTransformOwnedStage& operator=(CKKSLinearTransformOwnedStage&&) noexcept = default;
-
-    MyVector<DeviceBuffer> diagonal_qp_plaintexts;  // (level+1+K)*N each
-    MyVector<size_t> diagonal_indices;
-    size_t slot_count = 0;
-    size_t baby_step_count = 0;
-    size_t qp_limb_count = 0;
-    size_t qp_p_limb_count = 0;
-    size_t level = 0;
-    double plaintext_scale = 1.0;
-    bool diagonal_qp_plaintexts_montgomery = false;
-    bool rescale_after = true;
-    CKKSMessageEncodingState output_encoding_state = CKKSMessageEncodingState::Slots;
-    MyVector<uint64_t> downloadDiagonalQp(const CKKSContext& context, size_t term) const;
-};
-
-struct CKKSLinearTransformPlan {
-    CKKSLinearTransformPlan() = default;
-    CKKSLinearTransformPlan(const CKKSLinearTransformPlan&) = delete;
-    CKKSLinearTransformPlan& operator=(const CKKSLinearTransformPlan&) = delete;
-    CKKSLinearTransformPlan(CKKSLinearTransformPlan&&) noexcept = default;
-    CKKSLinearTransformPlan& operator=(CKKSLinearTransformPlan&&) noexcept = default;
-
-    MyVector<CKKSLinearTransformOwnedStage> stages;
-    bool use_qp_evaluator = true;
-};
-
 CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& context,
                                                          const CKKSBootstrapDftMatrixLiteral& literal);
 CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& context, const CKKSCiphertext& ct,
                                                           const CKKSLinearTransformOwnedStage& stage);
 CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& context, CKKSCiphertext ct,
                                                 const CKKSLinearTransformPlan& plan);
+
+// ---------------------------------------------------------- bootstrapping
+// ckks_keyswitch.cpp:1844-1870, ckks_modraise.cpp:76-110,
+// ckks_s2c_generalized.cpp, ckks_c2s_generalized.cpp,
+// ckks_bootstrap_evalmod_inplace.cpp:156-434.
+void ckks_key_switch_dense_to_sparse_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+void ckks_key_switch_sparse_to_dense_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+void ckks_modraise_inplace(const CKKSContext& context, CKKSCiphertext& ct, size_t target_level);
+CKKSCiphertext ckks_s2c_generalized(const CKKSContext& context, const CKKSCiphertext& ct, size_t stage_count);
+CKKSCiphertext ckks_c2s_generalized(const CKKSContext& context, const CKKSCiphertext& ct, double transform_scale);
+// EvalMod after the packed S2C: (real, imaginary) halves, each refreshed to
+// the top of the chain.
+std::pair<CKKSCiphertext, CKKSCiphertext> ckks_batch_bootstrap_evalmod_from_complex_packed(
+    const CKKSContext& context, CKKSCiphertext packed);
+void ckks_batch_bootstrap_evalmod_inplace(const CKKSContext& context, CKKSCiphertext& ct0, CKKSCiphertext& ct1);
+// The steps after S2C, exposed for the parity tests: dense->sparse,
+// ModRaise, sparse->dense, trace, C2S and the conjugate split; EvalMod is not
+// applied.
+std::pair<CKKSCiphertext, CKKSCiphertext> ckks_bootstrap_pre_evalmod(const CKKSContext& context,
+                                                                     CKKSCiphertext packed_coeffs);
+// One EvalMod (Chebyshev polynomial, then the double-angle steps).
+void ckks_evalmod_binary_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 
 // Galois element of a slot rotation (rotation_key_generation.cpp:58-78).
 int64_t normalize_rotation_value(size_t N, int64_t rotation);

@@ -164,6 +164,11 @@ CKKSParams::CKKSParams(CKKSParamsConfig c) {
     N = c.N;
     slots = N / 2;
     mult_depth = c.mult_depth;
+    q0_bit = c.q0_bit;
+    q_bit = c.q_bit;
+    bts_keys = c.bts_keys;
+    bootstrap = c.bootstrap_params;
+    if (bts_keys && !bootstrap) throw std::runtime_error("CKKSParams: bootstrap parameters are required");
     dnum = c.dnum;
     p_bits = c.p_bits.empty() ? std::vector<uint64_t>(c.pmoduli_count, (uint64_t)c.p_bit) : c.p_bits;
     hamming_weight = c.secret_key_hamming_weight;
@@ -174,23 +179,117 @@ CKKSParams::CKKSParams(CKKSParamsConfig c) {
     fold_hybrid_keyswitch = c.fold_hybrid_keyswitch;
     montgomery = c.montgomery;
     seed = c.seed;
+    // chain: q0, S2C groups (bottom to top), mult limbs, EvalMod limbs, C2S
+    // groups (ckks_params.cpp:213-296)
+    size_t expect_depth = mult_depth;
+    if (bts_keys) expect_depth += getBootstrapS2CDepth() + getBootstrapC2SDepth() + getBootstrapEvalModDepth();
+    auto append_groups = [&](const CKKSBootstrapDftScaleSchedule& sched, size_t fallback_depth) {
+        if (sched.empty()) {
+            q_bits.insert(q_bits.end(), fallback_depth, (uint64_t)c.q_bit);
+            return;
+        }
+        for (size_t r = 0; r < sched.size(); ++r) {
+            const auto& g = sched[sched.size() - 1 - r];
+            if (g.empty()) throw std::runtime_error("CKKSParams: DFT scale schedule groups must not be empty");
+            uint64_t sum = 0;
+            for (size_t v : g) sum += v;
+            q_bits.push_back(sum);
+        }
+    };
     if (!c.q_bits.empty()) {
         q_bits = c.q_bits;
         depth = q_bits.size() - 1;
         if (c.depth != 0 && c.depth != depth)
             throw std::invalid_argument("CKKSParams: depth must match q_bits.size() - 1");
     } else {
-        depth = c.depth == 0 ? mult_depth : c.depth;
-        if (depth != mult_depth)
-            throw std::runtime_error("CKKSParams: depth must equal mult_depth for a plain chain");
+        depth = c.depth == 0 ? expect_depth : c.depth;
         q_bits.push_back(c.q0_bit);
+        if (bts_keys) append_groups(bootstrap->linear_transform_log_plaintext_scales[0], getBootstrapS2CDepth());
         if (c.q_mult_bits.empty()) q_bits.insert(q_bits.end(), mult_depth, (uint64_t)c.q_bit);
         else q_bits.insert(q_bits.end(), c.q_mult_bits.begin(), c.q_mult_bits.end());
+        if (bts_keys) {
+            const uint64_t eb = bootstrap->evalmod_bit_size == 0 ? c.q_bit : bootstrap->evalmod_bit_size;
+            q_bits.insert(q_bits.end(), bootstrap->evalmod_depth, eb);
+            append_groups(bootstrap->linear_transform_log_plaintext_scales[1], getBootstrapC2SDepth());
+        }
+        if (q_bits.size() != depth + 1) throw std::runtime_error("CKKSParams: Q bit schedule must match limb count");
     }
+    if (depth != expect_depth)
+        throw std::runtime_error("CKKSParams: depth must equal mult_depth plus requested bootstrap/S2C depths");
     if (p_bits.empty()) throw std::invalid_argument("CKKSParams: at least one P modulus is required");
     auto [pp, qq] = find_p_and_q_primes(p_bits, q_bits, N);
     moduli.assign(qq.begin(), qq.end());
     pmoduli.assign(pp.begin(), pp.end());
+}
+
+const CKKSBootstrapParams& CKKSParams::getBootstrapParams() const {
+    if (!bts_keys) throw std::runtime_error("CKKSParams: bootstrap params are disabled because bts_keys is false");
+    if (!bootstrap) throw std::runtime_error("CKKSParams: bootstrap parameters are required");
+    return *bootstrap;
+}
+
+static size_t lt_depth(const CKKSBootstrapParams& b, size_t i) {
+    const auto& s = b.linear_transform_log_plaintext_scales[i];
+    return s.empty() ? b.linear_transform_depths[i] : s.size();
+}
+
+size_t CKKSParams::getBootstrapS2CDepth() const { return lt_depth(getBootstrapParams(), 0); }
+size_t CKKSParams::getBootstrapC2SDepth() const { return lt_depth(getBootstrapParams(), 1); }
+
+size_t CKKSParams::getBootstrapActiveSlotCount() const {
+    const size_t a = getBootstrapParams().active_slot_count;
+    if (a == 0) return slots;
+    if (a & (a - 1)) throw std::runtime_error("CKKSParams: bootstrap active slot count must be a power of two");
+    if (a > slots || slots % a != 0)
+        throw std::runtime_error("CKKSParams: bootstrap active slot count must divide the slot count");
+    return a;
+}
+
+size_t CKKSParams::getEvalModLogScaleBits() const {
+    const auto& b = getBootstrapParams();
+    return b.evalmod_bit_size == 0 ? q_bit : b.evalmod_bit_size;
+}
+
+// Smallest K with N * P(|sum of h uniform(-1/2,1/2)| > h/2 - K) below the
+// target (the Irwin-Hall tail, binboot_k_bound.cpp:13-87); EvalMod uses K + 1.
+size_t CKKSParams::getBootstrapEvalModK() const {
+    const auto& b = getBootstrapParams();
+    if (b.evalmod_k != 0) return b.evalmod_k;
+    const long double fl = b.evalmod_k_failure_probability_log2;
+    if (!std::isfinite(fl) || fl >= 0.0L)
+        throw std::runtime_error("CKKSParams: automatic EvalMod K failure probability log2 must be finite and negative");
+    const size_t h = b.sparse_secret_hamming_weight;
+    const long double target = std::exp2(fl);
+    if (h == 0) throw std::invalid_argument("binboot K bound: invalid coefficient count or Hamming weight");
+    auto binom = [](size_t n, size_t k) {
+        if (k > n) return 0.0L;
+        k = std::min(k, n - k);
+        long double o = 1.0L;
+        for (size_t i = 1; i <= k; ++i) {
+            o *= (long double)(n - k + i);
+            o /= (long double)i;
+        }
+        return o;
+    };
+    auto cdf = [&](long double x) {
+        if (x <= 0.0L) return 0.0L;
+        if (x >= (long double)h) return 1.0L;
+        const size_t up = (size_t)std::floor(x);
+        long double sum = 0.0L;
+        for (size_t k = 0; k <= up; ++k) {
+            const long double t = binom(h, k) * std::pow(x - (long double)k, (long double)h);
+            sum += (k & 1) == 0 ? t : -t;
+        }
+        return sum / std::tgamma((long double)h + 1.0L);
+    };
+    const size_t max_k = (h + 1) / 2;
+    for (size_t K = 0; K <= max_k; ++K) {
+        const long double half = (long double)h / 2.0L;
+        const long double tail = (long double)K >= half ? 0.0L : 2.0L * cdf(half - (long double)K);
+        const long double total = std::min(1.0L, (long double)N * tail);
+        if (total <= target) return K + 1;
+    }
+    return max_k + 1;
 }
 
 // ---------------------------------------------------------------- storage
@@ -391,21 +490,218 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
             const int64_t n = normalize_rotation_value(N, r);
             if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
         }
+        if (params.bts_keys) build_bootstrap_material(rng);
     } catch (...) {
-        relin_key.reset();
-        conj_key.reset();
-        rotation_keys.clear();
-        secret_qp.reset();
+        release_all();
         mi_context_destroy(mi);
         throw;
     }
 }
 
-CKKSContext::~CKKSContext() {
+void CKKSContext::release_all() {
+    scaled_s2c.clear();
+    s2c_plan.reset();
+    c2s_plan.reset();
+    d2s_key.reset();
+    s2d_key.reset();
+    low_secret_qp.reset();
     relin_key.reset();
     conj_key.reset();
     rotation_keys.clear();
     secret_qp.reset();
+}
+
+// Sparse secret and its switching keys, the S2C / C2S DFT plans and their
+// rotation keys (ckks_context.cpp:157-233, 389-414, 603-705).
+void CKKSContext::build_bootstrap_material(Rng& rng) {
+    const size_t N = params.getN(), nq = params.getModuli().size();
+    const auto& b = params.getBootstrapParams();
+    const auto& q = params.getModuli();
+    low_secret_coeffs = ckks_host::hamming_weight_ternary(N, b.sparse_secret_hamming_weight, rng);
+    low_secret_qp = secret_qp_from_coeffs(low_secret_coeffs);
+    d2s_key = generate_q0p0_key(secret_qp.data(), low_secret_qp.data(), 3);
+    s2d_key = generate_switch_key(low_secret_qp.data(), secret_qp.data(), 4);
+
+    const size_t s2c_depth = params.getBootstrapS2CDepth(), c2s_depth = params.getBootstrapC2SDepth();
+    const size_t active = params.getBootstrapActiveSlotCount();
+    if (s2c_depth > 0) {
+        CKKSBootstrapDftMatrixLiteral lit;
+        lit.type = CKKSBootstrapDftType::SlotsToCoeffs;
+        lit.level = s2c_depth;
+        lit.baby_step_count = b.linear_transform_bsgs_baby_step_counts[0];
+        lit.log_bsgs_ratio = b.linear_transform_log_bsgs_ratios[0];
+        lit.stage_count = s2c_depth;
+        lit.log_plaintext_scale_groups = b.linear_transform_log_plaintext_scales[0];
+        lit.merge_depths = b.linear_transform_merge_depths[0];
+        lit.active_slots = active;
+        lit.transform_scale = ((double)q[0] * 0.5) / params.getScale();
+        lit.plaintext_scale = (double)q[s2c_depth];
+        lit.rescale_after_each_stage = true;
+        s2c_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
+    }
+    if (c2s_depth > 0) {
+        const size_t level = params.getDepth();
+        if (c2s_depth > level) throw std::runtime_error("ckks_context: EvalMod C2S depth exceeds bootstrap depth");
+        const double K = (double)params.getBootstrapEvalModK();
+        const int shift = (int)params.getEvalModLogScaleBits() - (int)params.getQ0Bit();
+        c2s_transform_scale = std::ldexp(params.getScale(), shift) / (double)q[0] / K;
+        CKKSBootstrapDftMatrixLiteral lit;
+        lit.type = CKKSBootstrapDftType::CoeffsToSlots;
+        lit.level = level;
+        lit.baby_step_count = b.linear_transform_bsgs_baby_step_counts[1];
+        lit.log_bsgs_ratio = b.linear_transform_log_bsgs_ratios[1];
+        lit.stage_count = c2s_depth;
+        lit.log_plaintext_scale_groups = b.linear_transform_log_plaintext_scales[1];
+        lit.merge_depths = b.linear_transform_merge_depths[1];
+        lit.active_slots = active;
+        lit.transform_scale = c2s_transform_scale / (double)(params.getSlots() / active);
+        lit.plaintext_scale = (double)q[level] / params.getScale();
+        lit.rescale_after_each_stage = true;
+        c2s_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
+    }
+    auto add_plan_rotations = [&](const CKKSLinearTransformPlan* plan) {
+        if (!plan) return;
+        for (const auto& st : plan->stages)
+            for (size_t d : st.diagonal_indices) {
+                const size_t baby = d % st.baby_step_count;
+                if (baby) addRotationKey((int64_t)baby);
+                if (d - baby) addRotationKey((int64_t)(d - baby));
+            }
+    };
+    add_plan_rotations(s2c_plan.get());
+    add_plan_rotations(c2s_plan.get());
+    if (c2s_depth > 0)
+        for (size_t r = active; r < params.getSlots(); r <<= 1) addRotationKey((int64_t)r);
+    (void)nq;
+}
+
+// Dense-to-sparse switch key over {q0, p0} (secret_key_generation.cpp:302-364):
+// b = (p0 mod q0) * s_src on q0, 0 on p0, + e - a * s_tgt; folded and in
+// Montgomery form.  Layout [b(q0), b(p0), a(q0), a(p0)].
+DeviceBuffer CKKSContext::generate_q0p0_key(const uint64_t* src_qp, const uint64_t* tgt_qp, uint64_t stream_id) const {
+    const size_t N = params.getN(), nq = params.getModuli().size();
+    const uint64_t q0 = params.getModuli()[0], p0 = params.getPModuli()[0];
+    const int64_t W = (int64_t)N;
+    uint64_t st = params.getSeed() ^ (0xA5A5A5A5ull * (stream_id + 1));
+    Rng rng(ckks_host::splitmix64(st));
+    GaussianSampler gauss(params.getSigma());
+    DeviceBuffer key(mi, 4 * N), tmp(mi, 2 * N);
+    uint64_t* bq = key.data();
+    uint64_t* bp = bq + N;
+    uint64_t* aq = bp + N;
+    uint64_t* ap = aq + N;
+    const uint64_t* sq_src = src_qp;
+    const uint64_t* sq_tgt = tgt_qp;
+    const uint64_t* sp_tgt = tgt_qp + nq * N;
+    copy_d2d(*this, bq, sq_src, N);
+    const uint64_t pm = p0 % q0;
+    mi_check(mi_poly_scalar(mi, MI_OP_MUL, bq, &pm, 1, 0, 1, W, stream()), "ckks_context");
+    mi_check(mi_memset_zero(mi, bp, N * 8, stream()), "ckks_context");
+    std::vector<uint64_t> h(2 * N);
+    for (size_t i = 0; i < N; ++i) h[i] = rng.below(q0);
+    for (size_t i = 0; i < N; ++i) h[N + i] = rng.below(p0);
+    upload(*this, aq, h.data(), 2 * N);
+    for (size_t i = 0; i < N; ++i) {
+        const int64_t e = gauss.sample(rng);
+        h[i] = e < 0 ? q0 - (uint64_t)(-e) : (uint64_t)e;
+        h[N + i] = e < 0 ? p0 - (uint64_t)(-e) : (uint64_t)e;
+    }
+    upload(*this, tmp.data(), h.data(), 2 * N);
+    mi_check(mi_ntt_forward(mi, tmp.data(), 1, 1, W, 0, stream()), "ckks_context");
+    mi_check(mi_ntt_forward(mi, tmp.data() + N, 1, 1, W, (int)nq, stream()), "ckks_context");
+    mi_check(mi_poly_op(mi, MI_OP_ADD, bq, tmp.data(), 1, 0, 1, W, stream()), "ckks_context");
+    mi_check(mi_poly_op(mi, MI_OP_ADD, bp, tmp.data() + N, 1, (int)nq, 1, W, stream()), "ckks_context");
+    copy_d2d(*this, tmp.data(), aq, 2 * N);
+    mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data(), sq_tgt, 1, 0, 1, W, stream()), "ckks_context");
+    mi_check(mi_poly_op(mi, MI_OP_MUL, tmp.data() + N, sp_tgt, 1, (int)nq, 1, W, stream()), "ckks_context");
+    mi_check(mi_poly_op(mi, MI_OP_SUB, bq, tmp.data(), 1, 0, 1, W, stream()), "ckks_context");
+    mi_check(mi_poly_op(mi, MI_OP_SUB, bp, tmp.data() + N, 1, (int)nq, 1, W, stream()), "ckks_context");
+    uint64_t fq = 1, fp = 1;
+    if (params.fold_hybrid_keyswitch) fq = invm(pm, q0);
+    if (params.montgomery) {
+        fq = mulm(fq, (uint64_t)(((u128)1 << 64) % q0), q0);
+        fp = mulm(fp, (uint64_t)(((u128)1 << 64) % p0), p0);
+    }
+    if (params.fold_hybrid_keyswitch || params.montgomery) {
+        // b and a of each limb: q0 limbs at 0 and 2N, p0 limbs at N and 3N
+        mi_check(mi_poly_scalar(mi, MI_OP_MUL, bq, &fq, 1, 0, 2, 2 * W, stream()), "ckks_context");
+        mi_check(mi_poly_scalar(mi, MI_OP_MUL, bp, &fp, 1, (int)nq, 2, 2 * W, stream()), "ckks_context");
+    }
+    mi_check(mi_stream_synchronize(mi, stream()), "ckks_context");
+    return key;
+}
+
+const uint64_t* CKKSContext::getDenseToSparseSwitchKeyDevice() const {
+    if (!d2s_key.data()) throw std::runtime_error("ckks_context: dense-to-sparse switch key is unavailable");
+    return d2s_key.data();
+}
+const uint64_t* CKKSContext::getSparseToDenseSwitchKeyDevice() const {
+    if (!s2d_key.data()) throw std::runtime_error("ckks_context: sparse-to-dense switch key is unavailable");
+    return s2d_key.data();
+}
+void CKKSContext::setDenseToSparseSwitchKey(const MyVector<uint64_t>& key) {
+    const size_t N = params.getN();
+    if (key.size() != 4 * N) throw std::invalid_argument("ckks_context: dense-to-sparse key size mismatch");
+    DeviceBuffer d(mi, key.size());
+    upload(*this, d.data(), key.data(), key.size());
+    d2s_key = std::move(d);
+}
+void CKKSContext::setSparseToDenseSwitchKey(const MyVector<uint64_t>& key) { s2d_key = upload_key(key); }
+MyVector<uint64_t> CKKSContext::downloadDenseToSparseSwitchKey() const {
+    MyVector<uint64_t> h(4 * params.getN());
+    download(*this, h.data(), getDenseToSparseSwitchKeyDevice(), h.size());
+    return h;
+}
+
+const CKKSLinearTransformPlan& CKKSContext::getEvalModS2CPlan() const {
+    if (!s2c_plan) throw std::runtime_error("ckks_context: EvalMod S2C plan is unavailable");
+    return *s2c_plan;
+}
+const CKKSLinearTransformPlan& CKKSContext::getEvalModScaledC2SPlan() const {
+    if (!c2s_plan) throw std::runtime_error("ckks_context: EvalMod C2S plan is unavailable");
+    return *c2s_plan;
+}
+
+const CKKSLinearTransformPlan& CKKSContext::getScaledS2CPlan(uint64_t ratio) const {
+    if (ratio == 0) throw std::invalid_argument("ckks EvalMod: invalid S2C scale ratio");
+    if (ratio == 1) return getEvalModS2CPlan();
+    std::lock_guard<std::mutex> g(scaled_mu);
+    auto it = scaled_s2c.find(ratio);
+    if (it != scaled_s2c.end()) return *it->second;
+    const auto& b = params.getBootstrapParams();
+    const size_t s2c_depth = params.getBootstrapS2CDepth();
+    if (s2c_depth == 0) throw std::runtime_error("ckks EvalMod: scaled S2C depth must be positive");
+    CKKSBootstrapDftMatrixLiteral lit;
+    lit.type = CKKSBootstrapDftType::SlotsToCoeffs;
+    lit.level = s2c_depth;
+    lit.baby_step_count = b.linear_transform_bsgs_baby_step_counts[0];
+    lit.log_bsgs_ratio = b.linear_transform_log_bsgs_ratios[0];
+    lit.stage_count = s2c_depth;
+    lit.log_plaintext_scale_groups = b.linear_transform_log_plaintext_scales[0];
+    lit.merge_depths = b.linear_transform_merge_depths[0];
+    lit.active_slots = params.getBootstrapActiveSlotCount();
+    lit.transform_scale = ((double)params.getModuli()[0] * 0.5) / (params.getScale() * (double)ratio);
+    lit.plaintext_scale = (double)params.getModuli()[s2c_depth];
+    lit.rescale_after_each_stage = true;
+    auto plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
+    const CKKSLinearTransformPlan& ref = *plan;
+    scaled_s2c.emplace(ratio, std::move(plan));
+    return ref;
+}
+
+void CKKSContext::setEvalModPolynomial(const std::vector<double>& c) {
+    if (c.empty()) throw std::invalid_argument("ckks_context: empty EvalMod polynomial");
+    evalmod_coeffs = c;
+}
+
+const std::vector<double>& CKKSContext::getEvalModPolynomial() const {
+    if (evalmod_coeffs.empty())
+        throw std::runtime_error("ckks_context: EvalMod polynomial is unavailable (set it with setEvalModPolynomial)");
+    return evalmod_coeffs;
+}
+
+CKKSContext::~CKKSContext() {
+    release_all();
     if (mi) mi_context_destroy(mi);
 }
 

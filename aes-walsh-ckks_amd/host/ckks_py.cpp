@@ -47,7 +47,23 @@ PYBIND11_MODULE(ckks_py, m) {
         .value("Slots", CKKSDecryptDomain::Slots)
         .value("Coefficients", CKKSDecryptDomain::Coefficients);
 
+    py::class_<CKKSBootstrapParams>(m, "CKKSBootstrapParams")
+        .def(py::init<>())
+        .def_readwrite("sparse_secret_hamming_weight", &CKKSBootstrapParams::sparse_secret_hamming_weight)
+        .def_readwrite("active_slot_count", &CKKSBootstrapParams::active_slot_count)
+        .def_readwrite("linear_transform_depths", &CKKSBootstrapParams::linear_transform_depths)
+        .def_readwrite("linear_transform_log_plaintext_scales", &CKKSBootstrapParams::linear_transform_log_plaintext_scales)
+        .def_readwrite("evalmod_depth", &CKKSBootstrapParams::evalmod_depth)
+        .def_readwrite("evalmod_bit_size", &CKKSBootstrapParams::evalmod_bit_size)
+        .def_readwrite("evalmod_k", &CKKSBootstrapParams::evalmod_k)
+        .def_readwrite("evalmod_double_angle", &CKKSBootstrapParams::evalmod_double_angle)
+        .def_readwrite("linear_transform_bsgs_baby_step_counts", &CKKSBootstrapParams::linear_transform_bsgs_baby_step_counts)
+        .def_readwrite("linear_transform_log_bsgs_ratios", &CKKSBootstrapParams::linear_transform_log_bsgs_ratios)
+        .def_readwrite("linear_transform_merge_depths", &CKKSBootstrapParams::linear_transform_merge_depths);
+
     py::class_<CKKSParamsConfig>(m, "CKKSParamsConfig")
+        .def_readwrite("bts_keys", &CKKSParamsConfig::bts_keys)
+        .def_readwrite("bootstrap_params", &CKKSParamsConfig::bootstrap_params)
         .def(py::init<>())
         .def_readwrite("scale", &CKKSParamsConfig::scale)
         .def_readwrite("sigma", &CKKSParamsConfig::sigma)
@@ -79,6 +95,11 @@ PYBIND11_MODULE(ckks_py, m) {
         .def("getSigma", &CKKSParams::getSigma)
         .def("getModuli", [](const CKKSParams& p) { return to_np(p.getModuli()); })
         .def("getPModuli", [](const CKKSParams& p) { return to_np(p.getPModuli()); })
+        .def("getDepth", &CKKSParams::getDepth)
+        .def("getBootstrapS2CDepth", &CKKSParams::getBootstrapS2CDepth)
+        .def("getBootstrapC2SDepth", &CKKSParams::getBootstrapC2SDepth)
+        .def("getBootstrapActiveSlotCount", &CKKSParams::getBootstrapActiveSlotCount)
+        .def("getBootstrapEvalModK", &CKKSParams::getBootstrapEvalModK)
         .def_readonly("montgomery", &CKKSParams::montgomery)
         .def_readonly("fold_hybrid_keyswitch", &CKKSParams::fold_hybrid_keyswitch);
 
@@ -111,6 +132,20 @@ PYBIND11_MODULE(ckks_py, m) {
         .def("setConjugationKey", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setConjugationKey(from_np(k)); })
         .def("setRotationKey", [](CKKSContext& c, int64_t r, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setRotationKey(r, from_np(k)); })
         .def("setSecretKeyEval", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setSecretKeyEval(from_np(k)); })
+        .def("getLowSecretKeyCoeffs", [](const CKKSContext& c) {
+            const auto& s = c.getLowSecretKeyCoeffs();
+            py::array_t<int8_t> a(s.size());
+            std::copy(s.begin(), s.end(), a.mutable_data());
+            return a;
+        })
+        .def("getDenseToSparseSwitchKey", [](const CKKSContext& c) { return to_np(c.downloadDenseToSparseSwitchKey()); })
+        .def("getSparseToDenseSwitchKey", [](const CKKSContext& c) { return to_np(c.downloadKey(c.getSparseToDenseSwitchKeyDevice())); })
+        .def("setDenseToSparseSwitchKey", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setDenseToSparseSwitchKey(from_np(k)); })
+        .def("setSparseToDenseSwitchKey", [](CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> k) { c.setSparseToDenseSwitchKey(from_np(k)); })
+        .def("setEvalModPolynomial", &CKKSContext::setEvalModPolynomial)
+        .def("getEvalModS2CPlan", &CKKSContext::getEvalModS2CPlan, py::return_value_policy::reference_internal)
+        .def("getEvalModScaledC2SPlan", &CKKSContext::getEvalModScaledC2SPlan, py::return_value_policy::reference_internal)
+        .def("getEvalModScaledC2STransformScale", &CKKSContext::getEvalModScaledC2STransformScale)
         .def("memoryStats", [](const CKKSContext& c) {
             int64_t st[4];
             mi_check(mi_memory_stats(c.handle(), st), "memoryStats");
@@ -257,7 +292,7 @@ PYBIND11_MODULE(ckks_py, m) {
         .def("downloadDiagonalQp", [](const CKKSLinearTransformOwnedStage& st, const CKKSContext& c, size_t t) {
             return to_np(st.downloadDiagonalQp(c, t));
         });
-    py::class_<CKKSLinearTransformPlan>(m, "CKKSLinearTransformPlan")
+    py::class_<CKKSLinearTransformPlan>(m, "CKKSLinearTransformPlan", py::module_local(false))
         .def("stageCount", [](const CKKSLinearTransformPlan& p) { return p.stages.size(); })
         .def("stage", [](const CKKSLinearTransformPlan& p, size_t i) -> const CKKSLinearTransformOwnedStage& {
                  return p.stages.at(i);
@@ -274,6 +309,21 @@ PYBIND11_MODULE(ckks_py, m) {
               return ckks_apply_linear_transform_owned_stage_qp(c, ct, plan.stages.at(i));
           },
           py::keep_alive<0, 1>());
+    m.def("ckks_key_switch_dense_to_sparse_inplace", &ckks_key_switch_dense_to_sparse_inplace);
+    m.def("ckks_key_switch_sparse_to_dense_inplace", &ckks_key_switch_sparse_to_dense_inplace);
+    m.def("ckks_modraise_inplace", &ckks_modraise_inplace);
+    m.def("ckks_s2c_generalized", &ckks_s2c_generalized, py::keep_alive<0, 1>());
+    m.def("ckks_c2s_generalized", &ckks_c2s_generalized, py::keep_alive<0, 1>());
+    m.def("ckks_evalmod_binary_inplace", &ckks_evalmod_binary_inplace);
+    m.def("ckks_bootstrap_pre_evalmod",
+          [](const CKKSContext& c, const CKKSCiphertext& ct) { return ckks_bootstrap_pre_evalmod(c, ct.clone()); },
+          py::keep_alive<0, 1>());
+    m.def("ckks_batch_bootstrap_evalmod_from_complex_packed",
+          [](const CKKSContext& c, const CKKSCiphertext& ct) {
+              return ckks_batch_bootstrap_evalmod_from_complex_packed(c, ct.clone());
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_batch_bootstrap_evalmod_inplace", &ckks_batch_bootstrap_evalmod_inplace);
     m.def("rotation_automorphism_index", &rotation_automorphism_index);
     m.def("ckks_evalchebyshev_preconditioned",
           [](const CKKSContext& c, const CKKSCiphertext& x, std::vector<double> coeffs, double target_scale) {

@@ -318,14 +318,15 @@ def test_host_evalchebyshev_matches_reference():
         ref = outs["out%d" % k]
         assert digest(flat) == ref["digest"], k
         assert out.getLevel() == ref["level"] and out.getScale() == ref["scale"], k
-        # value check: the decrypted slots follow sum c_i T_i(m); the leaf that
-        # holds T_0 adds the constant only for even polynomials, as the
-        # reference's leaf evaluation does
-        dec = H.ckks_decrypt(ctx, out).real
-        want = np.polynomial.chebyshev.chebval(m, co)
-        if any(co[1::2]) and co[0] != 0.0:
-            want = want - co[0]
-        assert np.max(np.abs(dec - want)) < 1e-5, k
+        # value check for the even and odd polynomials: the decrypted slots
+        # follow sum c_i T_i(m).  (For polynomials with both parities the
+        # reference's leaf evaluation adds each leaf's T_0 term only when the
+        # polynomial is even, ckks_evalpoly.cpp:1011-1017, so those results are
+        # not sum c_i T_i; the digests above pin that behaviour.)
+        if not any(co[1::2]) or not any(co[0::2]):
+            dec = H.ckks_decrypt(ctx, out).real
+            want = np.polynomial.chebyshev.chebval(m, co)
+            assert np.max(np.abs(dec - want)) < 1e-5, k
 
 
 def _bt10_context(H):
@@ -335,13 +336,20 @@ def _bt10_context(H):
     d = load_npz("bt10.npz")
     c = H.CKKSParamsConfig()
     c.N = 1 << 10
-    c.q_bits = [40, 25, 25, 25, 35, 25, 25, 25]
+    c.mult_depth = 1
+    c.q0_bit = 40
     c.q_bit = 35
     c.p_bits = [60, 60]
     c.dnum = 2
     c.secret_key_hamming_weight = 192
     c.sigma = 3.2
     c.secure = False
+    c.bts_keys = True
+    bp = H.CKKSBootstrapParams()
+    bp.sparse_secret_hamming_weight = 16
+    bp.linear_transform_log_plaintext_scales = [[[25], [25], [25]], [[25], [25], [25]]]
+    bp.linear_transform_depths = [3, 3]
+    c.bootstrap_params = bp
     ctx = H.CKKSContext(H.CKKSParams(c))
     P = ctx.getParams()
     assert list(P.getModuli()) == list(d["q"]) and list(P.getPModuli()) == list(d["p"])
@@ -371,13 +379,14 @@ def test_host_s2c_dft_plan_matches_reference_and_decodes():
     q, p = P.getModuli(), P.getPModuli()
     lit = _s2c_literal(H, q)
     plan = H.ckks_generate_bootstrap_dft_plan(ctx, lit)
-    assert plan.stageCount() == len(o["s2c_stages"])
-    for i, ref in enumerate(o["s2c_stages"]):
-        st = plan.stage(i)
-        assert st.baby_step_count == ref["baby_steps"] and st.qp_limb_count == ref["qp_limbs"], i
-        assert list(st.diagonal_indices) == ref["diag_indices"], i
-        flat = np.concatenate([st.downloadDiagonalQp(ctx, t) for t in range(len(st.diagonal_indices))])
-        assert digest(flat) == ref["digest"], i
+    for pl in (plan, ctx.getEvalModS2CPlan()):
+        assert pl.stageCount() == len(o["s2c_stages"])
+        for i, ref in enumerate(o["s2c_stages"]):
+            st = pl.stage(i)
+            assert st.baby_step_count == ref["baby_steps"] and st.qp_limb_count == ref["qp_limbs"], i
+            assert list(st.diagonal_indices) == ref["diag_indices"], i
+            flat = np.concatenate([st.downloadDiagonalQp(ctx, t) for t in range(len(st.diagonal_indices))])
+            assert digest(flat) == ref["digest"], i
 
     N = 1 << 10
     for i in range(plan.stageCount()):
@@ -412,3 +421,77 @@ def test_host_s2c_dft_plan_matches_reference_and_decodes():
     y = H.ckks_apply_linear_transform_plan_stage(ctx, x, plan, 0)
     assert np.array_equal(y.getPoly(0), r[0]) and np.array_equal(y.getPoly(1), r[1])
     assert L == 4
+
+
+def test_host_dense_sparse_switches_and_modraise_match_reference():
+    """With the reference's dense-to-sparse key installed, the host API's
+    switch and ModRaise reproduce the reference's outputs (bt10)."""
+    H = host()
+    ctx, d = _bt10_context(H)
+    o = load_json("bt10_outputs.json")
+    N = 1 << 10
+    ctx.setDenseToSparseSwitchKey(d["key_d2s"])
+    b, a = d["ct0"].reshape(2, N)
+    ct = H.CKKSCiphertext(ctx, [b, a], 2.0 ** 35, 0)
+    H.ckks_key_switch_dense_to_sparse_inplace(ctx, ct)
+    assert ct.getSecretOwner() == H.CKKSSecretOwner.BootstrapSparse
+    assert digest(np.concatenate([ct.getPoly(0), ct.getPoly(1)])) == o["q0p0"]
+    H.ckks_modraise_inplace(ctx, ct, o["modraise_level"])
+    assert ct.getLevel() == o["modraise_level"]
+    assert digest(np.concatenate([ct.getPoly(0), ct.getPoly(1)])) == o["modraise"]
+
+
+def _bits_bootstrap_context(H):
+    """N=2^10 bootstrapping chain: q0 41 bits (one bit above the 2^40 scale, the
+    relation the EvalMod C2S scaling assumes: bit b sits at b * q0 / 2 after
+    S2C, and the C2S output keeps the 2^40 scale), S2C 3 x 30, one 40-bit mult limb,
+    7 EvalMod limbs of 40 bits, C2S 3 x 30; P = 3 x 60 bits, dnum 4 (digits of
+    at most 160 bits); sparse secret of Hamming weight 16 (EvalMod K = 9)."""
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 1
+    c.q0_bit = 41
+    c.q_bit = 40
+    c.p_bits = [60, 60, 60]
+    c.dnum = 4
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.bts_keys = True
+    c.seed = 99
+    bp = H.CKKSBootstrapParams()
+    bp.sparse_secret_hamming_weight = 16
+    bp.linear_transform_log_plaintext_scales = [[[30], [30], [30]], [[30], [30], [30]]]
+    bp.linear_transform_depths = [3, 3]
+    bp.evalmod_depth = 7
+    c.bootstrap_params = bp
+    return H.CKKSContext(H.CKKSParams(c))
+
+
+def test_host_bootstrap_refreshes_encrypted_bits():
+    """End to end: two slot vectors of bits at level 3 go through S2C, the
+    sparse-secret ModRaise, C2S and EvalMod and come back as fresh encryptions
+    of the same bits.  The EvalMod polynomial is the test's own Chebyshev
+    interpolant of 0.5 - 0.5 cos(2 pi K u) on [-1, 1] (the reference derives its
+    coefficients with MPFR; those doubles are not reproduced here)."""
+    H = host()
+    ctx = _bits_bootstrap_context(H)
+    P = ctx.getParams()
+    K = P.getBootstrapEvalModK()
+    assert K == 9
+    deg = 100
+    coeffs = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), deg)
+    coeffs[1::2] = 0.0
+    ctx.setEvalModPolynomial(list(coeffs))
+    N = 1 << 10
+    rng = np.random.default_rng(17)
+    b0 = rng.integers(0, 2, N // 2).astype(float)
+    b1 = rng.integers(0, 2, N // 2).astype(float)
+    s2c = P.getBootstrapS2CDepth()
+    ct0 = H.ckks_encrypt(ctx, b0.astype(complex), s2c)
+    ct1 = H.ckks_encrypt(ctx, b1.astype(complex), s2c)
+    H.ckks_batch_bootstrap_evalmod_inplace(ctx, ct0, ct1)
+    d0 = H.ckks_decrypt(ctx, ct0)
+    d1 = H.ckks_decrypt(ctx, ct1)
+    assert ct0.getLevel() > 0 and ct0.getScale() == P.getScale()
+    assert np.max(np.abs(d0.real - b0)) < 0.05 and np.max(np.abs(d1.real - b1)) < 0.05
