@@ -19,11 +19,20 @@ struct U128 {
     u64 lo, hi;
 };
 
+// Full 64x64 -> 128-bit product from four 32x32 -> 64 multiply-adds (each
+// partial sum fits in 64 bits: (2^32-1)^2 + 2 (2^32-1) = 2^64 - 1).
+__device__ __forceinline__ U128 mul128(u64 a, u64 b) {
+    const u64 a0 = (u32)a, a1 = a >> 32, b0 = (u32)b, b1 = b >> 32;
+    const u64 p00 = a0 * b0;
+    const u64 t = a0 * b1 + (p00 >> 32);
+    const u64 u = a1 * b0 + (u32)t;
+    return U128{(u << 32) | (u32)p00, a1 * b1 + (t >> 32) + (u >> 32)};
+}
+
 __device__ __forceinline__ void mac128(U128& acc, u64 a, u64 b) {
-    u64 lo = a * b;
-    u64 hi = __umul64hi(a, b);
-    u64 s = acc.lo + lo;
-    acc.hi += hi + (s < lo ? 1 : 0);
+    const U128 p = mul128(a, b);
+    const u64 s = acc.lo + p.lo;
+    acc.hi += p.hi + (s < p.lo ? 1 : 0);
     acc.lo = s;
 }
 

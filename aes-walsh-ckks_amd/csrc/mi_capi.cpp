@@ -5,6 +5,8 @@
 #include "../../include/mickks.h"
 #include "mi_internal.hpp"
 
+#include <vector>
+
 struct mi_context {
     mi::Context* c;
 };
@@ -378,12 +380,15 @@ int mi_keyswitch_batch(mi_context* ctx, const mi_keyswitch_item* items, int coun
             check_level(*ctx->c, it.level);
             check_ptr(it.key_term); check_ptr(it.key); check_ptr(it.b_out); check_ptr(it.a_out);
         }
+        std::vector<mi::KsItem> v;
+        v.reserve(count);
         for (int i = 0; i < count; ++i) {
             const mi_keyswitch_item& it = items[i];
-            mi::keyswitch(*ctx->c,
-                          mi::KsItem{it.key_term, it.key, it.b_out, it.a_out, it.level, it.b_mode, it.a_mode},
-                          montgomery != 0, S(stream));
+            if (it.b_mode < 0 || it.b_mode > 1 || it.a_mode < 0 || it.a_mode > 1)
+                throw mi::Error(MI_EINVAL, "combine mode must be 0 (Add) or 1 (Assign)");
+            v.push_back(mi::KsItem{it.key_term, it.key, it.b_out, it.a_out, it.level, it.b_mode, it.a_mode});
         }
+        if (count > 0) mi::keyswitch_batch(*ctx->c, v.data(), count, montgomery != 0, S(stream));
     });
 }
 
@@ -504,6 +509,51 @@ int mi_linear_transform_qp(mi_context* ctx, int level, const uint64_t* a, const 
                        baby_steps, slots, nkeys, (const long long*)key_rotations, keys, montgomery};
         mi::linear_transform_qp(*ctx->c, st, a, b, a_out, b_out, S(stream));
     });
+}
+
+int mi_linear_transform_qp_batch(mi_context* ctx, int level, int count, const uint64_t* const* a,
+                                 const uint64_t* const* b, int nterms, const int64_t* diag_indices,
+                                 const uint64_t* const* diags, int pt_montgomery, int baby_steps, int slots,
+                                 int nkeys, const int64_t* key_rotations, const uint64_t* const* keys,
+                                 int montgomery, uint64_t* const* a_out, uint64_t* const* b_out,
+                                 void* stream) {
+    return guard("ckks_apply_linear_transform_owned_stage_qp", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1) throw mi::Error(MI_EINVAL, "batch needs at least one ciphertext");
+        if (nkeys < 0) throw mi::Error(MI_EINVAL, "invalid key count");
+        if (!a || !b || !a_out || !b_out) throw mi::Error(MI_EINVAL, "null pointer argument");
+        mi::LtStage st{level, nterms, (const long long*)diag_indices, diags, pt_montgomery,
+                       baby_steps, slots, nkeys, (const long long*)key_rotations, keys, montgomery};
+        mi::linear_transform_qp_batch(*ctx->c, st, count, a, b, a_out, b_out, S(stream));
+    });
+}
+
+int mi_linear_transform_q_batch(mi_context* ctx, int level, int count, const uint64_t* const* a,
+                                const uint64_t* const* b, int nterms, const int64_t* diag_indices,
+                                const uint64_t* const* diags, int pt_montgomery, int baby_steps, int slots,
+                                int nkeys, const int64_t* key_rotations, const uint64_t* const* keys,
+                                int montgomery, uint64_t* const* a_out, uint64_t* const* b_out, void* stream) {
+    return guard("ckks_apply_linear_transform_bsgs", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1) throw mi::Error(MI_EINVAL, "batch needs at least one ciphertext");
+        if (nkeys < 0) throw mi::Error(MI_EINVAL, "invalid key count");
+        if (!a || !b || !a_out || !b_out) throw mi::Error(MI_EINVAL, "null pointer argument");
+        mi::LtStage st{level, nterms, (const long long*)diag_indices, diags, pt_montgomery,
+                       baby_steps, slots, nkeys, (const long long*)key_rotations, keys, montgomery};
+        mi::linear_transform_q_batch(*ctx->c, st, count, a, b, a_out, b_out, S(stream));
+    });
+}
+
+int mi_linear_transform_q(mi_context* ctx, int level, const uint64_t* a, const uint64_t* b, int nterms,
+                          const int64_t* diag_indices, const uint64_t* const* diags, int pt_montgomery,
+                          int baby_steps, int slots, int nkeys, const int64_t* key_rotations,
+                          const uint64_t* const* keys, int montgomery, uint64_t* a_out, uint64_t* b_out,
+                          void* stream) {
+    return mi_linear_transform_q_batch(ctx, level, 1, &a, &b, nterms, diag_indices, diags, pt_montgomery,
+                                       baby_steps, slots, nkeys, key_rotations, keys, montgomery, &a_out,
+                                       &b_out, stream);
 }
 
 int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint64_t* out,

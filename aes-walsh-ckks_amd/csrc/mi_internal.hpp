@@ -140,6 +140,9 @@ void tensor(const Context& c, int limbs, u64* b1, u64* a1, const u64* b2, const 
 void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s);
 void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                   long long sstride, long long dstride, hipStream_t s);
+// dst (+)= automorphism(src); src and dst must not overlap
+void automorphism_acc(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
+                      long long sstride, long long dstride, bool accumulate, hipStream_t s);
 void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s);
 void add_plain_mont(const Context& c, int limbs, u64* b, const u64* pt, hipStream_t s);
 
@@ -153,33 +156,59 @@ struct KsItem {
 };
 void keyswitch(Context& c, const KsItem& it, bool montgomery, hipStream_t s);
 
-// Key inner product over the QP slots of one level.
+// Items per batched key-switch launch (pointer lists travel as kernel
+// arguments, so a launch needs no device-side argument buffer and can be
+// captured in a graph).
+constexpr int KS_MAX_ITEMS = 32;
+
+// Key inner product over the QP slots of one level, for `items` items that
+// share the level.  Item i reads digits + i * digits_stride and key[i].
 struct MacArgs {
-    const u64* digits;    // [parts][L+K][N] evaluation domain
-    const u64* key;       // [parts][2 (b, a)][nq+np][N]
-    u64* b_out;           // split: L limbs; qp_out: L+K limbs
-    u64* a_out;
-    u64* accp_a;          // split only: [K][N] P slots of a
-    u64* accp_b;          //             [K][N] P slots of b
-    const u64* init_b;    // qp_out: Q-limb b term added (read through the automorphism)
-    const u64* add_b;     // qp_out: L+K-limb b term added (read through the automorphism)
-    u64 galois;           // qp_out: 0, or the Galois element applied to the result
+    const u64* digits;                 // [parts][L+K][N] evaluation domain, per item
+    long long digits_stride;
+    const u64* key[KS_MAX_ITEMS];      // [parts][2 (b, a)][nq+np][N]
+    u64* b_out[KS_MAX_ITEMS];          // split: L limbs; qp_out: L+K limbs
+    u64* a_out[KS_MAX_ITEMS];
+    u64* accp;                         // split only: item i's P slots of a at
+    long long accp_stride;             //   accp + i * accp_stride, of b K*N later
+    const u64* init_b[KS_MAX_ITEMS];   // Q-limb b term added to the b result before it is combined
+                                       //   (qp_out: read through the automorphism)
+    const u64* add_b[KS_MAX_ITEMS];    // qp_out: L+K-limb b term added (read through the automorphism)
+    u64 galois;                        // qp_out: 0, or the Galois element applied to the result
+    int items;
     int L, K, nq, parts, logN;
-    int b_mode, a_mode;   // split: 0 = add into outputs, 1 = assign
+    int b_mode, a_mode;                // split: 0 = add into outputs, 1 = assign
     int montgomery;
-    int qp_out;           // 1: whole result in the QP basis
-    int accumulate;       // qp_out: add into the outputs instead of assigning
+    int qp_out;                        // 1: whole result in the QP basis
+    int accumulate;                    // qp_out: add into the outputs instead of assigning
     const BasisConst* bc;
 };
 MacArgs mac_args(const Context& c, int level, const u64* digits, const u64* key, bool montgomery);
 // Device steps (scratch supplied by the caller):
-//   ks_modup: C = L*N words, D = parts*(L+K)*N words (digits, canonical)
-//   ks_moddown: T = npoly*L*N words; the P limbs p0/p1 are destroyed;
-//   out_i = base_i - NTT(contribution_i), base may alias out.
+//   ks_modup_n: n <= KS_MAX_ITEMS key terms (L limbs each, evaluation
+//     domain); C = n*L*N words; item i's digits at D + i*dstride
+//     (parts*(L+K)*N words, canonical).
+//   ks_moddown_n: npoly <= 2*KS_MAX_ITEMS polys; poly k's P limbs at p[k]
+//     (K*N words, destroyed); out[k] = base[k] - NTT(contribution_k), base
+//     may alias out; T = npoly*L*N words.
+void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* D, long long dstride,
+                u64* C, hipStream_t s);
 void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s);
 void ks_mac(const Context& c, const MacArgs& a, int level, hipStream_t s);
+void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* const* base, u64* const* out,
+                  u64* T, hipStream_t s);
 void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const u64* base1, u64* out0,
                 u64* out1, u64* T, hipStream_t s);
+// A batch of independent key switches at one level with one pair of combine
+// modes (callers group by these): items[i] outputs must not overlap another
+// item's inputs or outputs.
+void keyswitch_items(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s);
+// The same with caller scratch of n * ks_item_words(c, level) words.
+size_t ks_item_words(const Context& c, int level);
+void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomery, u64* ws, hipStream_t s);
+// Whole batch: items are grouped by (level, modes) into launches of up to
+// KS_MAX_ITEMS, keeping the order of items whose buffers overlap.
+void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s);
 
 // Hoisting and QP-basis switches (ckks_keyswitch.cpp:344-373, 675-1328).
 void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s);
@@ -205,6 +234,16 @@ struct LtStage {
 };
 void linear_transform_qp(Context& c, const LtStage& st, const u64* a, const u64* b, u64* a_out,
                          u64* b_out, hipStream_t s);
+// The same stage applied to n ciphertexts at the stage's level (any n; they
+// are processed KS_MAX_ITEMS at a time).
+void linear_transform_qp_batch(Context& c, const LtStage& st, int n, const u64* const* a,
+                               const u64* const* b, u64* const* a_out, u64* const* b_out, hipStream_t s);
+// BSGS stage on the Q evaluator (ckks_apply_linear_transform_bsgs,
+// ckks_linear_transform.cpp:818-996): baby steps are full hoisted rotations
+// in Q, diagonals are level+1 limb plaintexts, each giant step's inner sum is
+// rotated by a full key switch.
+void linear_transform_q_batch(Context& c, const LtStage& st, int n, const u64* const* a,
+                              const u64* const* b, u64* const* a_out, u64* const* b_out, hipStream_t s);
 
 // Bootstrapping pieces (mi_boot.hip).
 void modraise(Context& c, int target_level, const u64* poly_q0, u64* out, hipStream_t s);

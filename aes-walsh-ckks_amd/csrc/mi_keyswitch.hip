@@ -19,6 +19,8 @@
 // residue unchanged.
 #include "mi_ntt_kernels.hpp"
 
+#include <vector>
+
 namespace mi {
 
 constexpr int KS_THREADS = 256;
@@ -26,10 +28,12 @@ constexpr int MAX_ALPHA = 16;   // digit width held in registers
 constexpr int MODUP_DCH = 8;    // destination limbs per ModUp thread
 constexpr int MODDOWN_TCH = 4;  // target limbs per ModDown-contribution thread
 
-// ---- out-of-place inverse NTT (key term -> C) ----
-struct CopyJob {
-    const u64* src;
+// ---- out-of-place inverse NTT of n key terms -> C (item-minor within a limb) ----
+struct CopyListJob {
+    const u64* src[KS_MAX_ITEMS];
     u64* dst;
+    long long dstride;
+    int n;
     u64 N;
     struct View {
         int basis;
@@ -39,70 +43,16 @@ struct CopyJob {
         __device__ void store(int i, u64 v) const { d[i] = v; }
     };
     __device__ View view(int k) const {
-        return View{k, src + (long long)k * N, dst + (long long)k * N};
+        const int l = k / n, it = k - l * n;
+        return View{l, src[it] + (long long)l * N, dst + it * dstride + (long long)l * N};
     }
     __device__ bool skip(int) const { return false; }
 };
 
-// ---- ModUp of all digits ----
-struct ModUpArgs {
-    const u64* coeff;     // C [L][N]
-    const u64* eval;      // key term [L][N]
-    u64* digits;          // D [parts][L+K][N]
-    int L, K, parts;
-    int logN;
-    ModUpTables t[16];
-    const BasisConst* bc;
-};
-
-__global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
-    const int c = blockIdx.x * KS_THREADS + threadIdx.x;
-    const int d = blockIdx.y;
-    const int chunk = blockIdx.z;
-    const u64 N = (u64)1 << a.logN;
-    if (c >= (int)N) return;
-    const ModUpTables& t = a.t[d];
-    const int j0 = chunk * MODUP_DCH;
-    if (j0 >= t.nd) return;
-    const int j1 = j0 + MODUP_DCH < t.nd ? j0 + MODUP_DCH : t.nd;
-    const int slots = a.L + a.K;
-    u64* D = a.digits + (long long)d * slots * N;
-    u64 y[MAX_ALPHA];
-    double est = 0.0;
-#pragma unroll
-    for (int i = 0; i < MAX_ALPHA; ++i) {
-        y[i] = 0;
-        if (i < t.ns) {
-            const int limb = t.s0 + i;
-            const u64 q = a.bc[limb].q;
-            const u64 x = reduce_4q(a.coeff[(long long)limb * N + c], q);
-            const u64 s = shoup(x, t.qhat_inv[i], t.qhat_inv_shoup[i], q);
-            y[i] = s;
-            est = __dadd_rn(est, __dmul_rn((double)s, t.src_inv[i]));
-            if (chunk == 0) D[(long long)limb * N + c] = a.eval[(long long)limb * N + c];
-        }
-    }
-    u64 al = (u64)__dadd_rn(est, 0.5);
-    if (al > (u64)t.ns) al = (u64)t.ns;
-    for (int j = j0; j < j1; ++j) {
-        const BasisConst b = a.bc[t.dst_basis[j]];
-        const u64* row = t.qhat_mod_dst + (long long)j * t.ns;
-        U128 acc{0, 0};
-#pragma unroll
-        for (int i = 0; i < MAX_ALPHA; ++i)
-            if (i < t.ns) mac128(acc, y[i], row[i]);
-        u64 v = reduce128(acc.lo, acc.hi, b.q, b.mu_lo, b.mu_hi);
-        v = sub_mod(v, t.alpha_corr[(long long)j * (t.ns + 1) + al], b.q);
-        const int slot =
-            j < t.s0 ? j : (j < t.s0 + (a.L - t.s1) ? t.s1 + (j - t.s0) : a.L + (j - (a.L - t.ns)));
-        D[(long long)slot * N + c] = v;
-    }
-}
-
-// ---- forward NTT of the ModUp'd slots ----
-struct DigitJob {
-    u64* digits;
-    int L, K, nq, alpha, parts;
+// ---- in-place NTT over a list of polys (limb-major: shared twiddles) ----
+struct ListJob {
+    u64* p[2 * KS_MAX_ITEMS];
+    int npoly, basis0;
     u64 N;
     struct View {
         int basis;
@@ -110,51 +60,196 @@ struct DigitJob {
         __device__ u64 load(int i) const { return p[i]; }
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
-    // slot-major: every digit's slot 0, then slot 1, ... (shared twiddles)
     __device__ View view(int k) const {
-        const int l = k / parts, d = k - l * parts;
-        return View{l < L ? l : nq + (l - L), digits + ((long long)d * (L + K) + l) * N};
+        const int l = k / npoly, poly = k - l * npoly;
+        return View{basis0 + l, p[poly] + (long long)l * N};
+    }
+    __device__ bool skip(int) const { return false; }
+};
+
+// ---- ModUp of all digits of n items ----
+struct ModUpArgs {
+    const u64* coeff;                   // C: item i's [L][N] at coeff + i*L*N
+    const u64* eval[KS_MAX_ITEMS];      // key terms [L][N]
+    u64* digits;                        // item i's D [parts][L+K][N] at digits + i*dstride
+    long long dstride;
+    int L, K, parts, chunks;
+    int logN;
+    ModUpTables t[16];
+    const BasisConst* bc;
+};
+
+// One block: 256 coefficients of one digit of one item, up to MODUP_DCH
+// destination limbs.  The digit's tables are staged in LDS first: every
+// thread reads the same entries, so they are LDS broadcasts instead of a
+// dependent global load per product.  NS = the digit's limb count.
+template <int NS>
+__global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
+    __shared__ u64 s_row[MODUP_DCH][NS];       // qhat_i mod m_j
+    __shared__ u64 s_corr[MODUP_DCH][NS + 1];  // alpha * Q_digit mod m_j
+    __shared__ u64 s_dq[MODUP_DCH], s_mulo[MODUP_DCH], s_muhi[MODUP_DCH];
+    __shared__ int s_slot[MODUP_DCH];
+    __shared__ u64 s_sq[NS], s_qinv[NS], s_qinvs[NS];
+    __shared__ double s_sinv[NS];
+    const int d = blockIdx.y;
+    const int it = blockIdx.z / a.chunks;
+    const int chunk = blockIdx.z - it * a.chunks;
+    const u64 N = (u64)1 << a.logN;
+    const ModUpTables& t = a.t[d];
+    const int j0 = chunk * MODUP_DCH;
+    if (j0 >= t.nd) return;  // uniform over the block
+    const int nj = j0 + MODUP_DCH < t.nd ? MODUP_DCH : t.nd - j0;
+    for (int e = threadIdx.x; e < nj * NS; e += KS_THREADS)
+        s_row[e / NS][e % NS] = t.qhat_mod_dst[(long long)(j0 + e / NS) * NS + e % NS];
+    for (int e = threadIdx.x; e < nj * (NS + 1); e += KS_THREADS)
+        s_corr[e / (NS + 1)][e % (NS + 1)] = t.alpha_corr[(long long)(j0 + e / (NS + 1)) * (NS + 1) + e % (NS + 1)];
+    if (threadIdx.x < nj) {
+        const int j = j0 + threadIdx.x;
+        const BasisConst b = a.bc[t.dst_basis[j]];
+        s_dq[threadIdx.x] = b.q;
+        s_mulo[threadIdx.x] = b.mu_lo;
+        s_muhi[threadIdx.x] = b.mu_hi;
+        s_slot[threadIdx.x] =
+            j < t.s0 ? j : (j < t.s0 + (a.L - t.s1) ? t.s1 + (j - t.s0) : a.L + (j - (a.L - NS)));
+    }
+    if (threadIdx.x >= 64 && threadIdx.x < 64 + NS) {
+        const int i = threadIdx.x - 64;
+        s_sq[i] = a.bc[t.s0 + i].q;
+        s_qinv[i] = t.qhat_inv[i];
+        s_qinvs[i] = t.qhat_inv_shoup[i];
+        s_sinv[i] = t.src_inv[i];
+    }
+    __syncthreads();
+    // N is a multiple of KS_THREADS: every thread has a coefficient
+    const int c = blockIdx.x * KS_THREADS + threadIdx.x;
+    const int slots = a.L + a.K;
+    const u64* coeff = a.coeff + (long long)it * a.L * N;
+    const u64* eval = a.eval[it];
+    u64* D = a.digits + it * a.dstride + (long long)d * slots * N;
+    u64 y[NS];
+    double est = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int limb = t.s0 + i;
+        const u64 q = s_sq[i];
+        const u64 x = reduce_4q(coeff[(long long)limb * N + c], q);
+        y[i] = shoup(x, s_qinv[i], s_qinvs[i], q);
+        est = __dadd_rn(est, __dmul_rn((double)y[i], s_sinv[i]));
+        if (chunk == 0) D[(long long)limb * N + c] = eval[(long long)limb * N + c];
+    }
+    u64 al = (u64)__dadd_rn(est, 0.5);
+    if (al > (u64)NS) al = (u64)NS;
+    for (int jj = 0; jj < nj; ++jj) {
+        // sum of NS products < 16 * 2^124 fits in 128 bits
+        U128 acc{0, 0};
+#pragma unroll
+        for (int i = 0; i < NS; ++i) mac128(acc, y[i], s_row[jj][i]);
+        const u64 q = s_dq[jj];
+        u64 v = reduce128(acc.lo, acc.hi, q, s_mulo[jj], s_muhi[jj]);
+        v = sub_mod(v, s_corr[jj][al], q);
+        D[(long long)s_slot[jj] * N + c] = v;
+    }
+}
+
+#define MI_NS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+// ---- forward NTT of the ModUp'd slots ----
+struct DigitJob {
+    u64* digits;
+    long long dstride;
+    int L, K, nq, alpha, parts, n;
+    u64 N;
+    struct View {
+        int basis;
+        u64* p;
+        __device__ u64 load(int i) const { return p[i]; }
+        __device__ void store(int i, u64 v) const { p[i] = v; }
+    };
+    // slot-major: every item's and digit's slot 0, then slot 1, ... (shared twiddles)
+    __device__ void split(int k, int& l, int& it, int& d) const {
+        const int per = parts * n;
+        l = k / per;
+        const int r = k - l * per;
+        it = r / parts;
+        d = r - it * parts;
+    }
+    __device__ View view(int k) const {
+        int l, it, d;
+        split(k, l, it, d);
+        return View{l < L ? l : nq + (l - L), digits + it * dstride + ((long long)d * (L + K) + l) * N};
     }
     __device__ bool skip(int k) const {
-        const int l = k / parts, d = k - l * parts;
+        int l, it, d;
+        split(k, l, it, d);
         const int s0 = d * alpha;
         return l >= s0 && l < s0 + alpha && l < L;
     }
 };
 
-void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s) {
+void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* D, long long dstride,
+                u64* C, hipStream_t s) {
     const int L = level + 1, K = c.np, parts = c.parts(level);
     const u64 N = c.N;
+    if (n < 1 || n > KS_MAX_ITEMS) throw Error(-1, "ModUp batch size out of range");
     if (parts > 16) throw Error(-1, "too many digits");
     for (int d = 0; d < parts; ++d)
         if (c.modup[level][d].ns > MAX_ALPHA) throw Error(-1, "digit wider than 16 limbs");
     {
-        CopyJob in{key_term, C, N};
-        PlainJob out{LimbBatch{C, (long long)L * (long long)N, L, 0, L}, N};
-        run_ntt_inv(c, in, out, L, s);
+        CopyListJob in{};
+        for (int i = 0; i < n; ++i) in.src[i] = key_terms[i];
+        in.dst = C;
+        in.dstride = (long long)L * (long long)N;
+        in.n = n;
+        in.N = N;
+        PlainJob out{LimbBatch{C, (long long)L * (long long)N, L, 0, L * n}, N};
+        run_ntt_inv(c, in, out, L * n, s);
     }
+    int maxnd = 0;
     {
         ModUpArgs a{};
         a.coeff = C;
-        a.eval = key_term;
+        for (int i = 0; i < n; ++i) a.eval[i] = key_terms[i];
         a.digits = D;
+        a.dstride = dstride;
         a.L = L;
         a.K = K;
         a.parts = parts;
         a.logN = c.logN;
         a.bc = c.d_bc;
-        int maxnd = 0;
         for (int d = 0; d < parts; ++d) {
             a.t[d] = c.modup[level][d];
             maxnd = a.t[d].nd > maxnd ? a.t[d].nd : maxnd;
         }
-        dim3 grid((unsigned)((N + KS_THREADS - 1) / KS_THREADS), parts,
-                  (maxnd + MODUP_DCH - 1) / MODUP_DCH);
-        hipLaunchKernelGGL(k_modup, grid, dim3(KS_THREADS), 0, s, a);
-        MI_HIP(hipGetLastError());
+        a.chunks = (maxnd + MODUP_DCH - 1) / MODUP_DCH;
+        if (a.chunks < 1) a.chunks = 1;
+        // digits of one width go in one launch (all but the last have alpha limbs)
+        int d0 = 0;
+        while (d0 < parts) {
+            int d1 = d0 + 1;
+            while (d1 < parts && a.t[d1].ns == a.t[d0].ns) ++d1;
+            ModUpArgs b = a;
+            for (int d = d0; d < d1; ++d) b.t[d - d0] = a.t[d];
+            // the launch's digit index starts at 0: shift the digit outputs
+            b.digits = a.digits + (long long)d0 * (L + K) * (long long)N;
+            dim3 grid((unsigned)(N / KS_THREADS), d1 - d0, a.chunks * n);
+            switch (a.t[d0].ns) {
+#define MI_MODUP_CASE(NS) \
+                case NS: hipLaunchKernelGGL(k_modup<NS>, grid, dim3(KS_THREADS), 0, s, b); break;
+                MI_NS_CASES(MI_MODUP_CASE)
+#undef MI_MODUP_CASE
+                default: throw Error(-1, "digit wider than 16 limbs");
+            }
+            MI_HIP(hipGetLastError());
+            d0 = d1;
+        }
     }
-    DigitJob j{D, L, K, c.nq, c.alpha, parts, N};
-    run_ntt_fwd(c, j, j, parts * (L + K), s);
+    DigitJob j{D, dstride, L, K, c.nq, c.alpha, parts, n, N};
+    run_ntt_fwd(c, j, j, parts * (L + K) * n, s);
+}
+
+void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s) {
+    const u64* kt[1] = {key_term};
+    ks_modup_n(c, level, 1, kt, D, 0, C, s);
 }
 
 // ---- key inner product ----
@@ -169,22 +264,26 @@ __device__ __forceinline__ u32 galois_src(u32 i, u64 g, int logN) {
 
 __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
     const u64 N = (u64)1 << a.logN;
-    const long long i = (long long)blockIdx.x * KS_THREADS + threadIdx.x;
-    const int slot = blockIdx.y;
-    if (i >= (long long)N) return;
     const int slots = a.L + a.K;
+    const int tps = (int)(N / KS_THREADS);
+    const TileMap m = map_tile(blockIdx.x, a.items, tps, slots);
+    if (!m.valid) return;
+    const int it = m.item, slot = m.slot;
+    const long long i = (long long)m.tile * KS_THREADS + threadIdx.x;
     const int basis = slot < a.L ? slot : a.nq + (slot - a.L);
     const int key_limbs = a.nq + a.K;
     const BasisConst b = a.bc[basis];
     const long long src = a.galois ? (long long)galois_src((u32)i, a.galois, a.logN) : i;
+    const u64* digits = a.digits + it * a.digits_stride;
+    const u64* key = a.key[it];
     // Lazy 128-bit sums of the digit products, brought below q every three
     // products (3 q^2 + q < q 2^64 for q < 2^62) and reduced once at the end:
     // REDC for Montgomery-form keys, Barrett otherwise.  The result is the
     // canonical residue of the per-product sum.
     U128 sb{0, 0}, sa{0, 0};
     for (int d = 0; d < a.parts; ++d) {
-        u64 x = a.digits[((long long)d * slots + slot) * N + src];
-        const u64* kb = a.key + ((long long)d * 2 * key_limbs + basis) * N;
+        u64 x = digits[((long long)d * slots + slot) * N + src];
+        const u64* kb = key + ((long long)d * 2 * key_limbs + basis) * N;
         const u64* ka = kb + (long long)key_limbs * N;
         u64 kbv = kb[src], kav = ka[src];
         if (!a.montgomery) {
@@ -207,36 +306,41 @@ __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
         acc_b = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
         acc_a = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
     }
+    const u64* ib = a.init_b[it];
+    if (ib && slot < a.L) acc_b = add_mod(acc_b, ib[(long long)slot * N + src], b.q);
     if (a.qp_out) {
-        if (a.init_b && slot < a.L) acc_b = add_mod(acc_b, a.init_b[(long long)slot * N + src], b.q);
-        if (a.add_b) acc_b = add_mod(acc_b, a.add_b[(long long)slot * N + src], b.q);
-        u64* pb = a.b_out + (long long)slot * N + i;
-        u64* pa = a.a_out + (long long)slot * N + i;
+        const u64* ab = a.add_b[it];
+        if (ab) acc_b = add_mod(acc_b, ab[(long long)slot * N + src], b.q);
+        u64* pb = a.b_out[it] + (long long)slot * N + i;
+        u64* pa = a.a_out[it] + (long long)slot * N + i;
         *pb = a.accumulate ? add_mod(*pb, acc_b, b.q) : acc_b;
         *pa = a.accumulate ? add_mod(*pa, acc_a, b.q) : acc_a;
     } else if (slot < a.L) {
-        u64* pb = a.b_out + (long long)slot * N + i;
-        u64* pa = a.a_out + (long long)slot * N + i;
+        u64* pb = a.b_out[it] + (long long)slot * N + i;
+        u64* pa = a.a_out[it] + (long long)slot * N + i;
         *pb = a.b_mode == 0 ? add_mod(*pb, acc_b, b.q) : acc_b;
         *pa = a.a_mode == 0 ? add_mod(*pa, acc_a, b.q) : acc_a;
     } else {
         const int pi = slot - a.L;
-        a.accp_a[(long long)pi * N + i] = acc_a;
-        a.accp_b[(long long)pi * N + i] = acc_b;
+        u64* ap = a.accp + it * a.accp_stride;
+        ap[(long long)pi * N + i] = acc_a;
+        ap[((long long)a.K + pi) * N + i] = acc_b;
     }
 }
 
 void ks_mac(const Context& c, const MacArgs& args, int level, hipStream_t s) {
-    const u64 N = c.N;
-    dim3 grid((unsigned)((N + KS_THREADS - 1) / KS_THREADS), level + 1 + c.np);
-    hipLaunchKernelGGL(k_mac, grid, dim3(KS_THREADS), 0, s, args);
+    if (args.items < 1 || args.items > KS_MAX_ITEMS) throw Error(-1, "key product batch size out of range");
+    const int tps = (int)(c.N / KS_THREADS);
+    const unsigned blocks = tile_grid(args.items, tps, level + 1 + c.np);
+    hipLaunchKernelGGL(k_mac, dim3(blocks), dim3(KS_THREADS), 0, s, args);
     MI_HIP(hipGetLastError());
 }
 
 MacArgs mac_args(const Context& c, int level, const u64* digits, const u64* key, bool montgomery) {
     MacArgs a{};
     a.digits = digits;
-    a.key = key;
+    a.key[0] = key;
+    a.items = 1;
     a.L = level + 1;
     a.K = c.np;
     a.nq = c.nq;
@@ -249,47 +353,63 @@ MacArgs mac_args(const Context& c, int level, const u64* digits, const u64* key,
 
 // ---- folded ModDown ----
 struct MdContribArgs {
-    const u64* src[2];    // P limbs (coefficient domain) of each poly, K limbs apart by N
-    u64* tmp;             // [npoly][L][N]
-    const u64* pinv;      // [L][K]
+    const u64* src[2 * KS_MAX_ITEMS];   // P limbs (coefficient domain) of each poly, K limbs apart by N
+    u64* tmp;                           // [npoly][L][N]
+    const u64* pinv;                    // [L][K]
     int L, K, nq, logN;
     const BasisConst* bc;
 };
 
+// One block: 256 coefficients of one poly, up to MODDOWN_TCH target limbs;
+// the P-side constants and the targets' p_j^-1 rows are staged in LDS.
+template <int K>
 __global__ void __launch_bounds__(KS_THREADS) k_moddown_contrib(MdContribArgs a) {
+    __shared__ u64 s_pinv[MODDOWN_TCH][K];
+    __shared__ u64 s_tq[MODDOWN_TCH], s_mulo[MODDOWN_TCH], s_muhi[MODDOWN_TCH];
+    __shared__ u64 s_pq[K];
+    __shared__ double s_pinvd[K];
     const u64 N = (u64)1 << a.logN;
-    const long long i = (long long)blockIdx.x * KS_THREADS + threadIdx.x;
     const int poly = blockIdx.y;
     const int t0 = blockIdx.z * MODDOWN_TCH;
-    if (i >= (long long)N || t0 >= a.L) return;
-    const int t1 = t0 + MODDOWN_TCH < a.L ? t0 + MODDOWN_TCH : a.L;
+    if (t0 >= a.L) return;  // uniform over the block
+    const int nt = t0 + MODDOWN_TCH < a.L ? MODDOWN_TCH : a.L - t0;
+    for (int e = threadIdx.x; e < nt * K; e += KS_THREADS)
+        s_pinv[e / K][e % K] = a.pinv[(long long)(t0 + e / K) * K + e % K];
+    if (threadIdx.x < nt) {
+        const BasisConst b = a.bc[t0 + threadIdx.x];
+        s_tq[threadIdx.x] = b.q;
+        s_mulo[threadIdx.x] = b.mu_lo;
+        s_muhi[threadIdx.x] = b.mu_hi;
+    }
+    if (threadIdx.x >= 64 && threadIdx.x < 64 + K) {
+        const BasisConst pb = a.bc[a.nq + threadIdx.x - 64];
+        s_pq[threadIdx.x - 64] = pb.q;
+        s_pinvd[threadIdx.x - 64] = pb.inv_d;
+    }
+    __syncthreads();
+    // N is a multiple of KS_THREADS: every thread has a coefficient
+    const long long i = (long long)blockIdx.x * KS_THREADS + threadIdx.x;
     const u64* src = a.src[poly];
-    u64 x[MAX_ALPHA];
+    u64 x[K];
     double est = 0.0;
 #pragma unroll
-    for (int j = 0; j < MAX_ALPHA; ++j) {
-        x[j] = 0;
-        if (j < a.K) {
-            const BasisConst pb = a.bc[a.nq + j];
-            x[j] = reduce_4q(src[(long long)j * N + i], pb.q);
-            est = __dadd_rn(est, __dmul_rn((double)x[j], pb.inv_d));
-        }
+    for (int j = 0; j < K; ++j) {
+        x[j] = reduce_4q(src[(long long)j * N + i], s_pq[j]);
+        est = __dadd_rn(est, __dmul_rn((double)x[j], s_pinvd[j]));
     }
     u64 al = (u64)__dadd_rn(est, 0.5);
-    if (al > (u64)a.K) al = (u64)a.K;
-    for (int t = t0; t < t1; ++t) {
-        const BasisConst b = a.bc[t];
-        const u64* pinv = a.pinv + (long long)t * a.K;
+    if (al > (u64)K) al = (u64)K;
+    for (int tt = 0; tt < nt; ++tt) {
         U128 acc{0, 0};
 #pragma unroll
-        for (int j = 0; j < MAX_ALPHA; ++j)
-            if (j < a.K) mac128(acc, x[j], pinv[j]);
-        const u64 v = reduce128(acc.lo, acc.hi, b.q, b.mu_lo, b.mu_hi);
-        a.tmp[((long long)poly * a.L + t) * N + i] = sub_mod(v, al, b.q);
+        for (int j = 0; j < K; ++j) mac128(acc, x[j], s_pinv[tt][j]);
+        const u64 q = s_tq[tt];
+        const u64 v = reduce128(acc.lo, acc.hi, q, s_mulo[tt], s_muhi[tt]);
+        a.tmp[((long long)poly * a.L + t0 + tt) * N + i] = sub_mod(v, al, q);
     }
 }
 
-// forward NTT of the contributions, limb-major (a and b of one modulus together)
+// forward NTT of the contributions, limb-major (all polys of one modulus together)
 struct MdInJob {
     u64* tmp;
     int L, npoly;
@@ -309,8 +429,8 @@ struct MdInJob {
 
 struct MdOutJob {
     const u64* tmp;
-    const u64* base[2];
-    u64* out[2];
+    const u64* base[2 * KS_MAX_ITEMS];
+    u64* out[2 * KS_MAX_ITEMS];
     int L, npoly;
     u64 N;
     const BasisConst* bc;
@@ -331,18 +451,22 @@ struct MdOutJob {
     __device__ bool skip(int) const { return false; }
 };
 
-void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const u64* base1, u64* out0,
-                u64* out1, u64* T, hipStream_t s) {
-    const int L = level + 1, K = c.np, npoly = p1 ? 2 : 1;
+void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* const* base, u64* const* out,
+                  u64* T, hipStream_t s) {
+    const int L = level + 1, K = c.np;
     const u64 N = c.N;
-    // both polys' P limbs in one batch (the callers' P blocks never overlap)
-    if (p1)
-        ntt_inverse(c, LimbBatch{p0, (long long)(p1 - p0), K, c.nq, 2 * K}, s);
-    else
-        ntt_inverse(c, LimbBatch{p0, (long long)K * (long long)N, K, c.nq, K}, s);
+    if (npoly < 1 || npoly > 2 * KS_MAX_ITEMS) throw Error(-1, "ModDown batch size out of range");
+    if (K > MAX_ALPHA) throw Error(-1, "more than 16 special moduli");
+    {
+        ListJob j{};
+        for (int k = 0; k < npoly; ++k) j.p[k] = p[k];
+        j.npoly = npoly;
+        j.basis0 = c.nq;
+        j.N = N;
+        run_ntt_inv(c, j, j, K * npoly, s);
+    }
     MdContribArgs a{};
-    a.src[0] = p0;
-    a.src[1] = p1 ? p1 : p0;
+    for (int k = 0; k < npoly; ++k) a.src[k] = p[k];
     a.tmp = T;
     a.pinv = c.moddown[level].pinv;
     a.L = L;
@@ -350,45 +474,150 @@ void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const
     a.nq = c.nq;
     a.logN = c.logN;
     a.bc = c.d_bc;
-    if (K > MAX_ALPHA) throw Error(-1, "more than 16 special moduli");
-    dim3 grid((unsigned)((N + KS_THREADS - 1) / KS_THREADS), npoly, (L + MODDOWN_TCH - 1) / MODDOWN_TCH);
-    hipLaunchKernelGGL(k_moddown_contrib, grid, dim3(KS_THREADS), 0, s, a);
+    dim3 grid((unsigned)(N / KS_THREADS), npoly, (L + MODDOWN_TCH - 1) / MODDOWN_TCH);
+    switch (K) {
+#define MI_MODDOWN_CASE(KK) \
+        case KK: hipLaunchKernelGGL(k_moddown_contrib<KK>, grid, dim3(KS_THREADS), 0, s, a); break;
+        MI_NS_CASES(MI_MODDOWN_CASE)
+#undef MI_MODDOWN_CASE
+        default: throw Error(-1, "more than 16 special moduli");
+    }
     MI_HIP(hipGetLastError());
     MdInJob in{T, L, npoly, N};
-    MdOutJob out{T, {base0, base1 ? base1 : base0}, {out0, out1 ? out1 : out0}, L, npoly, N, c.d_bc};
-    run_ntt_fwd(c, in, out, npoly * L, s);
+    MdOutJob o{};
+    o.tmp = T;
+    for (int k = 0; k < npoly; ++k) {
+        o.base[k] = base[k];
+        o.out[k] = out[k];
+    }
+    o.L = L;
+    o.npoly = npoly;
+    o.N = N;
+    o.bc = c.d_bc;
+    run_ntt_fwd(c, in, o, npoly * L, s);
+}
+
+void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const u64* base1, u64* out0,
+                u64* out1, u64* T, hipStream_t s) {
+    u64* p[2] = {p0, p1};
+    const u64* b[2] = {base0, base1};
+    u64* o[2] = {out0, out1};
+    ks_moddown_n(c, level, p1 ? 2 : 1, p, b, o, T, s);
 }
 
 // ---------------------------------------------------------------- entry points
-static size_t ks_scratch_words(const Context& c, int level) {
+size_t ks_item_words(const Context& c, int level) {
     const size_t L = level + 1, K = c.np, parts = c.parts(level), N = c.N;
     return (L + parts * (L + K) + 2 * K + 2 * L) * N;
 }
 
-void keyswitch_with(Context& c, const KsItem& it, bool montgomery, u64* ws, hipStream_t s) {
-    const int level = it.level;
+void keyswitch_items(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s) {
+    if (n < 1 || n > KS_MAX_ITEMS) throw Error(-1, "key switch batch size out of range");
+    u64* ws = (u64*)c.workspace(s, sizeof(u64) * ks_item_words(c, items[0].level) * (size_t)n);
+    keyswitch_items_with(c, items, n, montgomery, ws, s);
+}
+
+void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomery, u64* ws, hipStream_t s) {
+    if (n < 1 || n > KS_MAX_ITEMS) throw Error(-1, "key switch batch size out of range");
+    const int level = items[0].level;
+    for (int i = 1; i < n; ++i)
+        if (items[i].level != level || items[i].b_mode != items[0].b_mode || items[i].a_mode != items[0].a_mode)
+            throw Error(-1, "key switch batch items must share level and combine modes");
     const int L = level + 1, K = c.np, parts = c.parts(level);
     const u64 N = c.N;
+    const long long dw = (long long)parts * (L + K) * (long long)N;
     u64* C = ws;
-    u64* D = C + (size_t)L * N;
-    u64* AP = D + (size_t)parts * (L + K) * N;
-    u64* T = AP + (size_t)2 * K * N;
-    ks_modup(c, level, it.key_term, D, C, s);
-    MacArgs a = mac_args(c, level, D, it.key, montgomery);
-    a.b_out = it.b_out;
-    a.a_out = it.a_out;
-    a.b_mode = it.b_mode;
-    a.a_mode = it.a_mode;
-    a.accp_a = AP;
-    a.accp_b = AP + (size_t)K * N;
+    u64* D = C + (size_t)n * L * N;
+    u64* AP = D + (size_t)n * dw;
+    u64* T = AP + (size_t)n * 2 * K * N;
+    const u64* kt[KS_MAX_ITEMS];
+    for (int i = 0; i < n; ++i) kt[i] = items[i].key_term;
+    ks_modup_n(c, level, n, kt, D, dw, C, s);
+    MacArgs a = mac_args(c, level, D, items[0].key, montgomery);
+    a.items = n;
+    a.digits_stride = dw;
+    a.accp = AP;
+    a.accp_stride = 2LL * K * (long long)N;
+    a.b_mode = items[0].b_mode;
+    a.a_mode = items[0].a_mode;
+    for (int i = 0; i < n; ++i) {
+        a.key[i] = items[i].key;
+        a.b_out[i] = items[i].b_out;
+        a.a_out[i] = items[i].a_out;
+    }
     ks_mac(c, a, level, s);
-    ks_moddown(c, level, AP, AP + (size_t)K * N, it.a_out, it.b_out, it.a_out, it.b_out, T, s);
+    u64* p[2 * KS_MAX_ITEMS];
+    const u64* base[2 * KS_MAX_ITEMS];
+    u64* out[2 * KS_MAX_ITEMS];
+    for (int i = 0; i < n; ++i) {
+        p[2 * i] = AP + (size_t)i * 2 * K * N;
+        p[2 * i + 1] = p[2 * i] + (size_t)K * N;
+        base[2 * i] = out[2 * i] = items[i].a_out;
+        base[2 * i + 1] = out[2 * i + 1] = items[i].b_out;
+    }
+    ks_moddown_n(c, level, 2 * n, p, base, out, T, s);
 }
 
 void keyswitch(Context& c, const KsItem& it, bool montgomery, hipStream_t s) {
     if (it.level < 0 || it.level >= c.nq) throw Error(-1, "invalid ciphertext level");
-    u64* ws = (u64*)c.workspace(s, sizeof(u64) * ks_scratch_words(c, it.level));
-    keyswitch_with(c, it, montgomery, ws, s);
+    keyswitch_items(c, &it, 1, montgomery, s);
+}
+
+namespace {
+struct Span {
+    const u64* lo;
+    const u64* hi;
+};
+bool overlaps(const Span& x, const Span& y) { return x.lo < y.hi && y.lo < x.hi; }
+}  // namespace
+
+// Groups of one (level, modes) whose buffers are independent, up to
+// KS_MAX_ITEMS each.  An item that touches a buffer an earlier item of the
+// open group of its kind writes (or writes one it reads) closes that group
+// and every other open group, so the order of dependent switches is kept.
+void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s) {
+    struct Group {
+        int level, b_mode, a_mode;
+        std::vector<KsItem> v;
+        std::vector<Span> writes, reads;
+    };
+    std::vector<Group> open;
+    auto flush_all = [&]() {
+        for (auto& g : open) keyswitch_items(c, g.v.data(), (int)g.v.size(), montgomery, s);
+        open.clear();
+    };
+    for (int i = 0; i < n; ++i) {
+        const KsItem& it = items[i];
+        const size_t W = (size_t)(it.level + 1) * c.N;
+        const Span rd{it.key_term, it.key_term + W};
+        const Span wb{it.b_out, it.b_out + W}, wa{it.a_out, it.a_out + W};
+        bool conflict = false;
+        for (const auto& g : open) {
+            for (const auto& w : g.writes)
+                if (overlaps(w, rd) || overlaps(w, wb) || overlaps(w, wa)) conflict = true;
+            for (const auto& r : g.reads)
+                if (overlaps(r, wb) || overlaps(r, wa)) conflict = true;
+        }
+        if (conflict) flush_all();
+        Group* g = nullptr;
+        for (auto& x : open)
+            if (x.level == it.level && x.b_mode == it.b_mode && x.a_mode == it.a_mode) g = &x;
+        if (!g) {
+            open.push_back(Group{it.level, it.b_mode, it.a_mode, {}, {}, {}});
+            g = &open.back();
+        }
+        g->v.push_back(it);
+        g->writes.push_back(wb);
+        g->writes.push_back(wa);
+        g->reads.push_back(rd);
+        if ((int)g->v.size() == KS_MAX_ITEMS) {
+            keyswitch_items(c, g->v.data(), (int)g->v.size(), montgomery, s);
+            g->v.clear();
+            g->writes.clear();
+            g->reads.clear();
+        }
+    }
+    flush_all();
 }
 
 void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s) {
@@ -406,12 +635,11 @@ void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key,
     u64* AP = (u64*)c.workspace(s, sizeof(u64) * words);
     u64* T = AP + (size_t)2 * K * N;
     MacArgs a = mac_args(c, level, digits, key, montgomery);
-    a.b_out = b_out;
-    a.a_out = a_out;
+    a.b_out[0] = b_out;
+    a.a_out[0] = a_out;
     a.b_mode = b_mode;
     a.a_mode = a_mode;
-    a.accp_a = AP;
-    a.accp_b = AP + (size_t)K * N;
+    a.accp = AP;
     ks_mac(c, a, level, s);
     ks_moddown(c, level, AP, AP + (size_t)K * N, a_out, b_out, a_out, b_out, T, s);
     if (galois) {
@@ -429,10 +657,10 @@ void keyswitch_hoisted_qp(Context& c, int level, const u64* digits, const u64* k
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
     MacArgs a = mac_args(c, level, digits, key, montgomery);
     a.qp_out = 1;
-    a.b_out = b_qp;
-    a.a_out = a_qp;
+    a.b_out[0] = b_qp;
+    a.a_out[0] = a_qp;
     a.galois = galois % (2 * c.N);
-    a.init_b = b_q;
+    a.init_b[0] = b_q;
     ks_mac(c, a, level, s);
 }
 
@@ -446,8 +674,8 @@ void keyswitch_qp(Context& c, int level, const u64* key_term, const u64* key, u6
     ks_modup(c, level, key_term, D, C, s);
     MacArgs a = mac_args(c, level, D, key, montgomery);
     a.qp_out = 1;
-    a.b_out = b_qp;
-    a.a_out = a_qp;
+    a.b_out[0] = b_qp;
+    a.a_out[0] = a_qp;
     ks_mac(c, a, level, s);
 }
 

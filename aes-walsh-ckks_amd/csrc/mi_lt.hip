@@ -14,65 +14,99 @@
 // rotated baby steps and giant-step accumulations never exist unpermuted.
 #include "mi_ntt_kernels.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace mi {
 
 constexpr int LT_THREADS = 256;
-constexpr int LT_MAX_TERMS = 32;  // terms per inner-product launch
+constexpr int LT_NB = 16;  // baby-step slots held in registers per launch (slot 0 = the input)
+constexpr int LT_NG = 16;  // giant steps per launch
 
-struct LtTerm {
-    const u64* a;   // baby step a (QP, or Q only when q_only)
-    const u64* b;
-    const u64* pt;  // diagonal, L+K limbs
-    int q_only;     // baby step 0: P limbs are zero
-    int pad;
-};
-
-struct LtInnerArgs {
-    LtTerm t[LT_MAX_TERMS];
-    int nterms;
-    u64* out_a;
-    u64* out_b;
-    int accumulate;
-    int L, K, nq, logN, pt_mont;
+// Inner products of every giant step in one pass over the baby steps: each
+// thread holds its coefficient of up to 16 baby steps (a and b) in registers
+// and runs through the diagonals of up to 16 giant steps, so a baby step is
+// read once per ciphertext and a diagonal tile once per XCD for all
+// ciphertexts of the batch (map_tile places a tile's blocks for every item
+// consecutively on one XCD).
+struct LtAllArgs {
+    const u64* diag[LT_NG * LT_NB];     // [giant][baby slot], null: no term
+    const u64* in_a[KS_MAX_ITEMS];      // baby slot 0 (Q limbs; its P limbs are zero)
+    const u64* in_b[KS_MAX_ITEMS];
+    const u64* babies;                  // item i's baby s (s >= 1): a at babies +
+    long long baby_item_stride;         //   i*item_stride + (s-1)*slot_stride, b T*N later
+    long long baby_slot_stride;
+    u64* out[LT_NG];                    // giant g of item i: a at out[g] + i*out_stride[g], b T*N later
+    long long out_stride[LT_NG];
+    int slot0;                          // baby slot of this launch's first register (multiple of 16)
+    int nb, ng, accumulate;
+    int items, L, K, nq, logN, pt_mont;
     const BasisConst* bc;
 };
 
-__global__ void __launch_bounds__(LT_THREADS) k_lt_inner(LtInnerArgs a) {
+__global__ void __launch_bounds__(LT_THREADS) k_lt_inner_all(LtAllArgs a) {
     const u64 N = (u64)1 << a.logN;
-    const long long i = (long long)blockIdx.x * LT_THREADS + threadIdx.x;
-    const int slot = blockIdx.y;
-    if (i >= (long long)N) return;
+    const int T = a.L + a.K;
+    const int tps = (int)(N / LT_THREADS);
+    const TileMap m = map_tile(blockIdx.x, a.items, tps, T);
+    if (!m.valid) return;
+    const int it = m.item, slot = m.slot;
+    const long long i = (long long)m.tile * LT_THREADS + threadIdx.x;
     const int basis = slot < a.L ? slot : a.nq + (slot - a.L);
     const BasisConst b = a.bc[basis];
     const long long o = (long long)slot * N + i;
-    // lazy 128-bit sums, brought below q every three products and reduced
-    // once (REDC for Montgomery-form plaintexts, Barrett otherwise)
-    U128 sa{0, 0}, sb{0, 0};
-    int pending = 0;
-    for (int t = 0; t < a.nterms; ++t) {
-        const LtTerm& tm = a.t[t];
-        if (tm.q_only && slot >= a.L) continue;
-        const u64 p = tm.pt[o];
-        mac128(sa, tm.a[o], p);
-        mac128(sb, tm.b[o], p);
-        if (++pending == 3) {
-            sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
-            sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
-            pending = 0;
+    const long long TN = (long long)T * N;
+    u64 ba[LT_NB], bb[LT_NB];
+#pragma unroll
+    for (int r = 0; r < LT_NB; ++r) {
+        ba[r] = 0;
+        bb[r] = 0;
+        if (r < a.nb) {
+            const int sl = a.slot0 + r;
+            if (sl == 0) {
+                if (slot < a.L) {
+                    ba[r] = a.in_a[it][o];
+                    bb[r] = a.in_b[it][o];
+                }
+            } else {
+                const u64* p = a.babies + it * a.baby_item_stride + (long long)(sl - 1) * a.baby_slot_stride + o;
+                ba[r] = p[0];
+                bb[r] = p[TN];
+            }
         }
     }
-    u64 ra, rb;
-    if (a.pt_mont) {
-        ra = redc128(sa.lo, sa.hi, b.q, b.qninv);
-        rb = redc128(sb.lo, sb.hi, b.q, b.qninv);
-    } else {
-        ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
-        rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+    for (int g = 0; g < a.ng; ++g) {
+        // lazy 128-bit sums, brought below q every three products and reduced
+        // once (REDC for Montgomery-form plaintexts, Barrett otherwise)
+        U128 sa{0, 0}, sb{0, 0};
+        int pending = 0;
+#pragma unroll
+        for (int r = 0; r < LT_NB; ++r) {
+            const u64* dp = a.diag[g * LT_NB + r];
+            if (r < a.nb && dp) {
+                const u64 p = dp[o];
+                mac128(sa, ba[r], p);
+                mac128(sb, bb[r], p);
+                if (++pending == 3) {
+                    sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
+                    sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
+                    pending = 0;
+                }
+            }
+        }
+        u64 ra, rb;
+        if (a.pt_mont) {
+            ra = redc128(sa.lo, sa.hi, b.q, b.qninv);
+            rb = redc128(sb.lo, sb.hi, b.q, b.qninv);
+        } else {
+            ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
+            rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+        }
+        u64* pa = a.out[g] + it * a.out_stride[g] + o;
+        u64* pb = pa + TN;
+        *pa = a.accumulate ? add_mod(*pa, ra, b.q) : ra;
+        *pb = a.accumulate ? add_mod(*pb, rb, b.q) : rb;
     }
-    a.out_a[o] = a.accumulate ? add_mod(a.out_a[o], ra, b.q) : ra;
-    a.out_b[o] = a.accumulate ? add_mod(a.out_b[o], rb, b.q) : rb;
 }
 
 static u64 powmod_u64(u64 a, u64 e, u64 m) {
@@ -94,30 +128,64 @@ u64 rotation_galois(u64 N, long long rotation) {
     return powmod_u64(5, (u64)r, N << 1);
 }
 
-void linear_transform_qp(Context& c, const LtStage& st, const u64* a, const u64* b, u64* a_out,
-                         u64* b_out, hipStream_t s) {
+namespace {
+
+// compile_linear_transform_stage_metadata (ckks_linear_transform.cpp:720-756)
+// plus the key lookups: baby steps in use get slots 1.. in order of first use
+// (slot 0 is the input itself); giant steps in use are listed in order.
+struct Compiled {
+    int bs = 0, nslots = 1, ngp = 0;
+    bool g0 = false;
+    std::vector<int> baby_slot;            // rotation r < bs -> slot, -1 unused
+    std::vector<int> glist;                // giant indices in use, ascending
+    std::vector<const u64*> table;         // [giant index][slot] diagonal
+    std::vector<const u64*> baby_key;      // [slot]
+    std::vector<u64> baby_gal;             // [slot]
+    std::vector<const u64*> giant_key;     // [giant index]
+};
+
+Compiled compile_stage(const Context& c, const LtStage& st, bool q_eval) {
     const int level = st.level;
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
     if (st.nterms < 1 || st.baby_steps < 1 || st.slots < 1)
         throw Error(-1, "need nterms >= 1, baby_steps >= 1, slots >= 1");
-    if (!a || !b || !a_out || !b_out || !st.diag_idx || !st.diags)
-        throw Error(-1, "null pointer argument");
+    if (!st.diag_idx || !st.diags) throw Error(-1, "null pointer argument");
     if (st.nkeys > 0 && (!st.key_rot || !st.keys)) throw Error(-1, "null key table");
-    const int L = level + 1, K = c.np, T = L + K, parts = c.parts(level), bs = st.baby_steps;
-    const u64 N = c.N;
-    const size_t W = (size_t)T * N;
+    Compiled k;
+    const int bs = k.bs = st.baby_steps;
     const int giants = (st.slots + bs - 1) / bs;
-
-    // compile_linear_transform_stage_metadata (:818-859): needed baby steps
-    std::vector<int> baby_slot(bs, -1);
-    int nbaby = 0;
+    k.baby_slot.assign(bs, -1);
+    k.baby_slot[0] = 0;
+    std::vector<int> gidx(giants, -1);
     for (int t = 0; t < st.nterms; ++t) {
         const long long d = st.diag_idx[t];
         if (d < 0) throw Error(-1, "diagonal indices must be non-negative");
         if (!st.diags[t]) throw Error(-1, "null diagonal");
+        if (d / bs >= giants) {
+            // the Q evaluator's validation (:758-814) rejects such a term;
+            // the QP evaluator leaves it out
+            if (q_eval) throw Error(-1, "diagonal index exceeds slot count");
+            continue;
+        }
+        const int r = (int)(d % bs), g = (int)(d / bs);
+        if (k.baby_slot[r] < 0) k.baby_slot[r] = k.nslots++;
+        if (gidx[g] < 0) {
+            gidx[g] = 0;
+            k.glist.push_back(g);
+        }
+    }
+    if (k.glist.empty()) throw Error(-1, "no diagonal falls inside the slot range");
+    std::sort(k.glist.begin(), k.glist.end());
+    for (size_t j = 0; j < k.glist.size(); ++j) gidx[k.glist[j]] = (int)j;
+    k.g0 = k.glist[0] == 0;
+    k.ngp = (int)k.glist.size() - (k.g0 ? 1 : 0);
+    k.table.assign(k.glist.size() * (size_t)k.nslots, nullptr);
+    for (int t = 0; t < st.nterms; ++t) {
+        const long long d = st.diag_idx[t];
         if (d / bs >= giants) continue;
-        const int r = (int)(d % bs);
-        if (r > 0 && baby_slot[r] < 0) baby_slot[r] = nbaby++;
+        const u64*& e = k.table[(size_t)gidx[d / bs] * k.nslots + k.baby_slot[d % bs]];
+        if (e) throw Error(-1, "duplicate diagonal index " + std::to_string(d));
+        e = st.diags[t];
     }
     auto key_for = [&](long long r) -> const u64* {
         for (int i = 0; i < st.nkeys; ++i)
@@ -127,106 +195,275 @@ void linear_transform_qp(Context& c, const LtStage& st, const u64* a, const u64*
             }
         throw Error(-1, "missing rotation key for step " + std::to_string(r));
     };
-    // check every key up front so that no work is queued on failure
+    // every key is looked up before any work is queued
+    k.baby_key.assign(k.nslots, nullptr);
+    k.baby_gal.assign(k.nslots, 0);
     for (int r = 1; r < bs; ++r)
-        if (baby_slot[r] >= 0) key_for(r);
-    for (int g = 1; g < giants; ++g)
-        for (int t = 0; t < st.nterms; ++t)
-            if (st.diag_idx[t] / bs == g) {
-                key_for((long long)g * bs);
-                break;
-            }
-
-    // scratch: digits | C | babies (a, b) | out (a, b) | inner (a, b) | T
-    const size_t words = (size_t)parts * W + (size_t)L * N + 2 * (size_t)nbaby * W + 4 * W +
-                         (size_t)2 * L * N;
-    u64* ws = (u64*)c.workspace(s, sizeof(u64) * words);
-    u64* D = ws;
-    u64* C = D + (size_t)parts * W;
-    u64* BA = C + (size_t)L * N;
-    u64* BB = BA + (size_t)nbaby * W;
-    u64* OA = BB + (size_t)nbaby * W;
-    u64* OB = OA + W;
-    u64* IA = OB + W;
-    u64* IB = IA + W;
-    u64* Tm = IB + W;
-
-    if (nbaby > 0) {
-        ks_modup(c, level, a, D, C, s);
-        for (int r = 1; r < bs; ++r) {
-            if (baby_slot[r] < 0) continue;
-            MacArgs m = mac_args(c, level, D, key_for(r), st.montgomery != 0);
-            m.qp_out = 1;
-            m.b_out = BB + (size_t)baby_slot[r] * W;
-            m.a_out = BA + (size_t)baby_slot[r] * W;
-            m.galois = rotation_galois(N, r);
-            m.init_b = b;
-            ks_mac(c, m, level, s);
+        if (k.baby_slot[r] > 0) {
+            k.baby_key[k.baby_slot[r]] = key_for(r);
+            k.baby_gal[k.baby_slot[r]] = rotation_galois(c.N, r);
         }
-    }
+    k.giant_key.assign(k.glist.size(), nullptr);
+    for (size_t j = 0; j < k.glist.size(); ++j)
+        if (k.glist[j] > 0) k.giant_key[j] = key_for((long long)k.glist[j] * bs);
+    return k;
+}
 
-    bool have_out = false;
-    const dim3 grid((unsigned)((N + LT_THREADS - 1) / LT_THREADS), T);
-    std::vector<int> terms;
-    for (int g = 0; g < giants; ++g) {
-        terms.clear();
-        for (int t = 0; t < st.nterms; ++t)
-            if (st.diag_idx[t] / bs == g) terms.push_back(t);
-        if (terms.empty()) continue;
-        u64* dst_a = g == 0 ? OA : IA;
-        u64* dst_b = g == 0 ? OB : IB;
-        bool acc = g == 0 ? have_out : false;
-        for (size_t t0 = 0; t0 < terms.size(); t0 += LT_MAX_TERMS) {
-            LtInnerArgs ia{};
-            ia.nterms = (int)std::min<size_t>(LT_MAX_TERMS, terms.size() - t0);
-            for (int j = 0; j < ia.nterms; ++j) {
-                const int t = terms[t0 + j];
-                const int r = (int)(st.diag_idx[t] - (long long)g * bs);
-                LtTerm& tm = ia.t[j];
-                tm.pt = st.diags[t];
-                if (r == 0) {
-                    tm.a = a;
-                    tm.b = b;
-                    tm.q_only = 1;
+// Inner products of every giant step for m items.  Giant 0 goes to out0 +
+// i*out0_stride, giant j > 0 (in use order) to outg + (j-1)*outg_giant +
+// i*outg_item; K = 0 for the Q evaluator.
+void launch_inner(Context& c, const Compiled& k, const LtStage& st, int m, int K, const u64* const* a,
+                  const u64* const* b, const u64* babies, long long baby_item, long long baby_slot, u64* out0,
+                  long long out0_stride, u64* outg, long long outg_giant, long long outg_item, hipStream_t s) {
+    const int L = st.level + 1, T = L + K;
+    const int tps = (int)(c.N / LT_THREADS);
+    for (int s0 = 0; s0 < k.nslots; s0 += LT_NB) {
+        for (size_t gk0 = 0; gk0 < k.glist.size(); gk0 += LT_NG) {
+            LtAllArgs ia{};
+            ia.nb = std::min(LT_NB, k.nslots - s0);
+            ia.ng = (int)std::min<size_t>(LT_NG, k.glist.size() - gk0);
+            ia.slot0 = s0;
+            for (int g = 0; g < ia.ng; ++g) {
+                const size_t gk = gk0 + g;
+                for (int r = 0; r < ia.nb; ++r) ia.diag[g * LT_NB + r] = k.table[gk * k.nslots + s0 + r];
+                if (k.glist[gk] == 0) {
+                    ia.out[g] = out0;
+                    ia.out_stride[g] = out0_stride;
                 } else {
-                    tm.a = BA + (size_t)baby_slot[r] * W;
-                    tm.b = BB + (size_t)baby_slot[r] * W;
-                    tm.q_only = 0;
+                    ia.out[g] = outg + (long long)(gk - (k.g0 ? 1 : 0)) * outg_giant;
+                    ia.out_stride[g] = outg_item;
                 }
             }
-            ia.out_a = dst_a;
-            ia.out_b = dst_b;
-            ia.accumulate = acc ? 1 : 0;
+            for (int it = 0; it < m; ++it) {
+                ia.in_a[it] = a[it];
+                ia.in_b[it] = b[it];
+            }
+            ia.babies = babies;
+            ia.baby_item_stride = baby_item;
+            ia.baby_slot_stride = baby_slot;
+            ia.accumulate = s0 > 0 ? 1 : 0;
+            ia.items = m;
             ia.L = L;
             ia.K = K;
             ia.nq = c.nq;
             ia.logN = c.logN;
             ia.pt_mont = st.pt_montgomery ? 1 : 0;
             ia.bc = c.d_bc;
-            hipLaunchKernelGGL(k_lt_inner, grid, dim3(LT_THREADS), 0, s, ia);
+            hipLaunchKernelGGL(k_lt_inner_all, dim3(tile_grid(m, tps, T)), dim3(LT_THREADS), 0, s, ia);
             MI_HIP(hipGetLastError());
-            acc = true;
         }
-        if (g == 0) {
-            have_out = true;
-            continue;
-        }
-        const long long gr = (long long)g * bs;
-        // key term = ModDown(inner.a); QP switch; rotate-accumulate with inner.b
-        ks_moddown(c, level, IA + (size_t)L * N, nullptr, IA, nullptr, IA, nullptr, Tm, s);
-        ks_modup(c, level, IA, D, C, s);
-        MacArgs m = mac_args(c, level, D, key_for(gr), st.montgomery != 0);
-        m.qp_out = 1;
-        m.b_out = OB;
-        m.a_out = OA;
-        m.galois = rotation_galois(N, gr);
-        m.add_b = IB;
-        m.accumulate = have_out ? 1 : 0;
-        ks_mac(c, m, level, s);
-        have_out = true;
     }
-    if (!have_out) throw Error(-1, "no diagonal falls inside the slot range");
-    ks_moddown(c, level, OA + (size_t)L * N, OB + (size_t)L * N, OA, OB, a_out, b_out, Tm, s);
+}
+
+void check_batch(int n, const u64* const* a, const u64* const* b, u64* const* a_out, u64* const* b_out) {
+    if (n < 1 || !a || !b || !a_out || !b_out) throw Error(-1, "null pointer argument");
+    for (int i = 0; i < n; ++i)
+        if (!a[i] || !b[i] || !a_out[i] || !b_out[i]) throw Error(-1, "null pointer argument");
+}
+
+}  // namespace
+
+void linear_transform_qp_batch(Context& c, const LtStage& st, int n, const u64* const* a,
+                               const u64* const* b, u64* const* a_out, u64* const* b_out, hipStream_t s) {
+    const Compiled k = compile_stage(c, st, false);
+    check_batch(n, a, b, a_out, b_out);
+    const int level = st.level;
+    const int L = level + 1, K = c.np, T = L + K, parts = c.parts(level);
+    const u64 N = c.N;
+    const long long W = (long long)T * (long long)N, LN = (long long)L * (long long)N;
+    const int nbaby = k.nslots - 1, ngp = k.ngp;
+
+    // per-item scratch: digits | C | babies (a, b) | out (a, b) | giant inners (a, b) | ModDown temp
+    const long long dw = (long long)parts * W;
+    const long long item_words = dw + LN + 2 * (long long)nbaby * W + 2 * W + 2 * (long long)ngp * W + 2 * LN;
+    const long long budget_words = (32LL << 30) / 8;
+    int chunk = (int)std::min<long long>(KS_MAX_ITEMS, std::max<long long>(1, budget_words / item_words));
+    chunk = std::min(chunk, n);
+    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)(item_words * chunk));
+
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int m = std::min(chunk, n - i0);
+        u64* D = ws;
+        u64* C = D + m * dw;
+        u64* BAB = C + m * LN;
+        u64* OAB = BAB + m * 2 * (long long)nbaby * W;
+        u64* IAB = OAB + m * 2 * W;
+        u64* Tm = IAB + m * 2 * (long long)ngp * W;
+        auto baby = [&](int it, int sl) { return BAB + it * 2LL * nbaby * W + (long long)(sl - 1) * 2 * W; };
+        auto inner = [&](int it, int gp) { return IAB + it * 2LL * ngp * W + (long long)gp * 2 * W; };
+        auto outp = [&](int it) { return OAB + it * 2 * W; };
+
+        // baby steps: one hoisted ModUp, then one key product per baby step
+        // (hoisted QP rotations, :675-852), b added in the Q limbs
+        if (nbaby > 0) {
+            ks_modup_n(c, level, m, a + i0, D, dw, C, s);
+            for (int sl = 1; sl < k.nslots; ++sl) {
+                MacArgs ma = mac_args(c, level, D, k.baby_key[sl], st.montgomery != 0);
+                ma.items = m;
+                ma.digits_stride = dw;
+                ma.qp_out = 1;
+                ma.galois = k.baby_gal[sl];
+                for (int it = 0; it < m; ++it) {
+                    ma.key[it] = k.baby_key[sl];
+                    ma.a_out[it] = baby(it, sl);
+                    ma.b_out[it] = baby(it, sl) + W;
+                    ma.init_b[it] = b[i0 + it];
+                }
+                ks_mac(c, ma, level, s);
+            }
+        }
+
+        launch_inner(c, k, st, m, K, a + i0, b + i0, BAB, 2LL * nbaby * W, 2 * W, OAB, 2 * W, IAB, 2 * W,
+                     2LL * ngp * W, s);
+
+        // giant steps > 0: key term = ModDown(inner.a); QP switch; rotate-accumulate
+        // with inner.b (automorphism_qp_pair_accumulate, :373-416)
+        bool have_out = k.g0;
+        u64* p[2 * KS_MAX_ITEMS];
+        const u64* base[2 * KS_MAX_ITEMS];
+        u64* outs[2 * KS_MAX_ITEMS];
+        const u64* kt[KS_MAX_ITEMS];
+        for (int gp = 0; gp < ngp; ++gp) {
+            const size_t gk = (size_t)gp + (k.g0 ? 1 : 0);
+            for (int it = 0; it < m; ++it) {
+                p[it] = inner(it, gp) + LN;
+                base[it] = outs[it] = inner(it, gp);
+                kt[it] = inner(it, gp);
+            }
+            ks_moddown_n(c, level, m, p, base, outs, Tm, s);
+            ks_modup_n(c, level, m, kt, D, dw, C, s);
+            MacArgs ma = mac_args(c, level, D, k.giant_key[gk], st.montgomery != 0);
+            ma.items = m;
+            ma.digits_stride = dw;
+            ma.qp_out = 1;
+            ma.galois = rotation_galois(N, (long long)k.glist[gk] * k.bs);
+            ma.accumulate = have_out ? 1 : 0;
+            for (int it = 0; it < m; ++it) {
+                ma.key[it] = k.giant_key[gk];
+                ma.a_out[it] = outp(it);
+                ma.b_out[it] = outp(it) + W;
+                ma.add_b[it] = inner(it, gp) + W;
+            }
+            ks_mac(c, ma, level, s);
+            have_out = true;
+        }
+
+        // result = folded ModDown of out (moddown_qp_pair_to_q_eval, :641-718)
+        for (int it = 0; it < m; ++it) {
+            p[2 * it] = outp(it) + LN;
+            p[2 * it + 1] = outp(it) + W + LN;
+            base[2 * it] = outp(it);
+            base[2 * it + 1] = outp(it) + W;
+            outs[2 * it] = a_out[i0 + it];
+            outs[2 * it + 1] = b_out[i0 + it];
+        }
+        ks_moddown_n(c, level, 2 * m, p, base, outs, Tm, s);
+    }
+}
+
+void linear_transform_q_batch(Context& c, const LtStage& st, int n, const u64* const* a,
+                              const u64* const* b, u64* const* a_out, u64* const* b_out, hipStream_t s) {
+    const Compiled k = compile_stage(c, st, true);
+    check_batch(n, a, b, a_out, b_out);
+    const int level = st.level;
+    const int L = level + 1, K = c.np, parts = c.parts(level);
+    const u64 N = c.N;
+    const long long LN = (long long)L * (long long)N, KN = (long long)K * (long long)N;
+    const int nbaby = k.nslots - 1, ngp = k.ngp;
+    const long long dw = (long long)parts * (L + K) * (long long)N;
+
+    // per-item scratch: digits | C | P accumulators | ModDown temp | switched
+    // pair | babies [slot][item] | out | giant inners [giant][item]
+    const long long item_words = dw + LN + 2 * KN + 2 * LN + 2 * LN + 2 * (long long)nbaby * LN + 2 * LN +
+                                 2 * (long long)ngp * LN;
+    const long long budget_words = (32LL << 30) / 8;
+    int chunk = (int)std::min<long long>(KS_MAX_ITEMS, std::max<long long>(1, budget_words / item_words));
+    chunk = std::min(chunk, n);
+    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)(item_words * chunk));
+
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int m = std::min(chunk, n - i0);
+        u64* D = ws;
+        u64* C = D + m * dw;
+        u64* AP = C + m * LN;
+        u64* Tm = AP + m * 2 * KN;
+        u64* SW = Tm + m * 2 * LN;
+        u64* BQ = SW + m * 2 * LN;
+        u64* OQ = BQ + m * 2 * (long long)nbaby * LN;
+        u64* IQ = OQ + m * 2 * LN;
+        u64* p[2 * KS_MAX_ITEMS];
+        const u64* base[2 * KS_MAX_ITEMS];
+        u64* outs[2 * KS_MAX_ITEMS];
+
+        // key switch of m key terms into SW (a at +0, b at +LN per item, b
+        // starting from b_init), modes Assign: the hoisted or full switch
+        // of ckks_hybrid_key_switch_(hoisted_)inplace
+        auto mac_moddown = [&](const u64* key, const u64* const* b_init) {
+            MacArgs ma = mac_args(c, level, D, key, st.montgomery != 0);
+            ma.items = m;
+            ma.digits_stride = dw;
+            ma.accp = AP;
+            ma.accp_stride = 2 * KN;
+            ma.b_mode = 1;
+            ma.a_mode = 1;
+            for (int it = 0; it < m; ++it) {
+                ma.key[it] = key;
+                ma.a_out[it] = SW + it * 2 * LN;
+                ma.b_out[it] = SW + it * 2 * LN + LN;
+                ma.init_b[it] = b_init[it];
+            }
+            ks_mac(c, ma, level, s);
+            for (int it = 0; it < m; ++it) {
+                p[2 * it] = AP + it * 2 * KN;
+                p[2 * it + 1] = AP + it * 2 * KN + KN;
+                base[2 * it] = outs[2 * it] = SW + it * 2 * LN;
+                base[2 * it + 1] = outs[2 * it + 1] = SW + it * 2 * LN + LN;
+            }
+            ks_moddown_n(c, level, 2 * m, p, base, outs, Tm, s);
+        };
+
+        // baby steps: hoisted rotations in Q (ckks_hybrid_key_switch_hoisted_inplace
+        // with b Add and a Assign, then the automorphism), [slot][item] layout
+        if (nbaby > 0) {
+            ks_modup_n(c, level, m, a + i0, D, dw, C, s);
+            for (int sl = 1; sl < k.nslots; ++sl) {
+                mac_moddown(k.baby_key[sl], b + i0);
+                automorphism(c, L, k.baby_gal[sl], SW, BQ + (long long)(sl - 1) * m * 2 * LN, 2 * m, LN, LN, s);
+            }
+        }
+
+        launch_inner(c, k, st, m, 0, a + i0, b + i0, BQ, 2 * LN, (long long)m * 2 * LN, OQ, 2 * LN, IQ,
+                     (long long)m * 2 * LN, 2 * LN, s);
+
+        // giant steps > 0: rotate each inner sum by a full key switch
+        // (ckks_rotate_inplace: switch a, then the automorphism) and add
+        bool have_out = k.g0;
+        for (int gp = 0; gp < ngp; ++gp) {
+            const size_t gk = (size_t)gp + (k.g0 ? 1 : 0);
+            u64* in = IQ + (long long)gp * m * 2 * LN;
+            const u64* kt[KS_MAX_ITEMS];
+            const u64* bi[KS_MAX_ITEMS];
+            for (int it = 0; it < m; ++it) {
+                kt[it] = in + it * 2 * LN;
+                bi[it] = in + it * 2 * LN + LN;
+            }
+            ks_modup_n(c, level, m, kt, D, dw, C, s);
+            mac_moddown(k.giant_key[gk], bi);
+            automorphism_acc(c, L, rotation_galois(N, (long long)k.glist[gk] * k.bs), SW, OQ, 2 * m, LN, LN,
+                             have_out, s);
+            have_out = true;
+        }
+        for (int it = 0; it < m; ++it) {
+            MI_HIP(hipMemcpyAsync(a_out[i0 + it], OQ + it * 2 * LN, sizeof(u64) * LN, hipMemcpyDeviceToDevice, s));
+            MI_HIP(hipMemcpyAsync(b_out[i0 + it], OQ + it * 2 * LN + LN, sizeof(u64) * LN,
+                                  hipMemcpyDeviceToDevice, s));
+        }
+    }
+}
+
+void linear_transform_qp(Context& c, const LtStage& st, const u64* a, const u64* b, u64* a_out,
+                         u64* b_out, hipStream_t s) {
+    linear_transform_qp_batch(c, st, 1, &a, &b, &a_out, &b_out, s);
 }
 
 }  // namespace mi

@@ -170,4 +170,28 @@ void run_ntt_inv(const Context& c, const JobIn& in, const JobOut& out, int count
     launch_cols<false>(c, out, count, s);
 }
 
+// Block -> (item, slot, tile) with the blocks of one (slot, tile) for all
+// items consecutive on one XCD: workgroups are dealt to the 8 XCDs round
+// robin, so block b runs on XCD b % 8.  The key (or diagonal) tile that all
+// items read is then fetched once into that XCD's L2.
+struct TileMap {
+    int item, slot, tile;
+    bool valid;
+};
+__device__ __forceinline__ TileMap map_tile(int b, int items, int tiles_per_slot, int slots) {
+    const int xcd = b & 7, local = b >> 3;
+    const int it = local % items, tl = local / items;
+    const int tile = tl * 8 + xcd;
+    TileMap m;
+    m.item = it;
+    m.slot = tile / tiles_per_slot;
+    m.tile = tile - m.slot * tiles_per_slot;
+    m.valid = m.slot < slots;
+    return m;
+}
+inline unsigned tile_grid(int items, int tiles_per_slot, int slots) {
+    const long long tiles = (long long)tiles_per_slot * slots;
+    return (unsigned)(((tiles + 7) / 8) * 8 * items);
+}
+
 }  // namespace mi

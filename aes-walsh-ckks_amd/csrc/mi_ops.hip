@@ -189,6 +189,37 @@ k_automorphism(int logN, u64 galois, int limbs, const u64* __restrict__ src, u64
     dst[k * dstride + limb_off + idx] = src[k * sstride + limb_off + sidx];
 }
 
+// dst (+)= automorphism(src) over Q limbs 0..limbs-1 (canonical residues)
+__global__ void __launch_bounds__(EW_THREADS)
+k_automorphism_acc(const BasisConst* __restrict__ bc, int logN, u64 galois, int limbs,
+                   const u64* __restrict__ src, u64* __restrict__ dst, long long sstride, long long dstride,
+                   int accumulate, long long total) {
+    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= total) return;
+    const long long per = (long long)limbs << logN;
+    const long long k = i / per, r = i - k * per;
+    const int l = (int)(r >> logN);
+    const long long limb_off = r & ~(((long long)1 << logN) - 1);
+    const u32 idx = (u32)(r & (((long long)1 << logN) - 1));
+    const u32 mask2n = (2u << logN) - 1;
+    const u32 d = brev_n(idx, logN);
+    const u32 e = (u32)(((u64)(2 * d + 1) * galois) & mask2n);
+    const u32 sidx = brev_n((e - 1) >> 1, logN);
+    const u64 v = src[k * sstride + limb_off + sidx];
+    u64* p = dst + k * dstride + limb_off + idx;
+    *p = accumulate ? add_mod(*p, v, bc[l].q) : v;
+}
+
+void automorphism_acc(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
+                      long long sstride, long long dstride, bool accumulate, hipStream_t s) {
+    if ((galois & 1) == 0) throw Error(-1, "Galois element must be odd");
+    long long total = (long long)npolys * limbs * (long long)c.N;
+    if (total == 0) return;
+    hipLaunchKernelGGL(k_automorphism_acc, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, c.logN,
+                       galois % (2 * c.N), limbs, src, dst, sstride, dstride, accumulate ? 1 : 0, total);
+    MI_HIP(hipGetLastError());
+}
+
 void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                   long long sstride, long long dstride, hipStream_t s) {
     if (src == dst) throw Error(-1, "source and destination must differ");

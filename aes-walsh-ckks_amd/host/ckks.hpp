@@ -29,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <span>
 #include <vector>
 
 #include "../../include/mickks.h"
@@ -56,10 +57,19 @@ struct CKKSBootstrapParams {
     size_t evalmod_bit_size = 0;
     size_t evalmod_k = 0;  // 0: from the sparse-secret overflow bound
     long double evalmod_k_failure_probability_log2 = -128.0L;
+    size_t evalmod_polynomial_degree = 120;
     size_t evalmod_double_angle = 0;
+    size_t evalmod_log_message_ratio = 0;
+    // fit controls of the reference's MPFR EvalMod fit (kept for the API; the
+    // coefficients are supplied with CKKSContext::setEvalModPolynomial)
+    double evalmod_coefficient_prune_threshold = 1e-10;
+    double evalmod_discrete_interval_radius = 0.0;
+    size_t evalmod_discrete_validation_points_per_interval = 0;
     std::array<size_t, 2> linear_transform_bsgs_baby_step_counts{0, 0};
     std::array<int, 2> linear_transform_log_bsgs_ratios{1, 1};
     std::array<std::vector<size_t>, 2> linear_transform_merge_depths;
+    // QP evaluator for S2C / C2S stages (the presets turn both on)
+    std::array<bool, 2> linear_transform_use_qp{false, false};
 };
 
 // CKKSParamsConfig (ckks_params.hpp:85-117).
@@ -119,6 +129,10 @@ public:
     size_t getBootstrapS2CDepth() const;
     size_t getBootstrapC2SDepth() const;
     size_t getBootstrapEvalModDepth() const { return getBootstrapParams().evalmod_depth; }
+    size_t getBootstrapEvalModPolynomialDegree() const { return getBootstrapParams().evalmod_polynomial_degree; }
+    size_t getBootstrapEvalModDoubleAngle() const { return getBootstrapParams().evalmod_double_angle; }
+    bool getBootstrapS2CUseQpLinearTransform() const { return getBootstrapParams().linear_transform_use_qp[0]; }
+    bool getBootstrapC2SUseQpLinearTransform() const { return getBootstrapParams().linear_transform_use_qp[1]; }
     size_t getBootstrapActiveSlotCount() const;
     bool usesSparseBootstrapSlots() const { return getBootstrapActiveSlotCount() != slots; }
     size_t getBootstrapEvalModK() const;
@@ -260,10 +274,11 @@ private:
 };
 
 // ---------------------------------------------------- bootstrapping DFT plans
-// ckks_bootstrap_dft.hpp:10-40, ckks_linear_transform_plan.hpp:9-47.  Stage
-// diagonals are kept only in the Q u P basis at the stage level, on the device
-// (the evaluator used for every stage is the QP one, as in all the
-// reference's bootstrapping presets).
+// ckks_bootstrap_dft.hpp:10-40, ckks_linear_transform_plan.hpp:9-47.  The
+// generated DFT stages keep their diagonals once, in the Q u P basis at the
+// stage level on the device; the Q evaluator reads the first level+1 limbs
+// of those (the Q-basis encoding is the same per-modulus encode).  Stages a
+// caller builds for the Q evaluator hold CKKSEncoding diagonals.
 enum class CKKSBootstrapDftType { SlotsToCoeffs, CoeffsToSlots };
 
 struct CKKSBootstrapDftMatrixLiteral {
@@ -282,6 +297,17 @@ struct CKKSBootstrapDftMatrixLiteral {
     bool rescale_after_each_stage = true;
 };
 
+// A stage over caller-owned diagonals (ckks_linear_transform_plan.hpp:9-17):
+// diagonal_indices empty means term t is diagonal t; slot_count 0 means N/2.
+struct CKKSLinearTransformStage {
+    std::span<const CKKSEncoding> diagonals;
+    std::span<const size_t> diagonal_indices;
+    size_t slot_count = 0;
+    size_t baby_step_count = 0;
+    bool rescale_after = true;
+    CKKSMessageEncodingState output_encoding_state = CKKSMessageEncodingState::Slots;
+};
+
 struct CKKSLinearTransformOwnedStage {
     CKKSLinearTransformOwnedStage() = default;
     CKKSLinearTransformOwnedStage(const CKKSLinearTransformOwnedStage&) = delete;
@@ -289,6 +315,7 @@ struct CKKSLinearTransformOwnedStage {
     CKKSLinearTransformOwnedStage(CKKSLinearTransformOwnedStage&&) noexcept = default;
     CKKSLinearTransformOwnedStage& operator=(CKKSLinearTransformOwnedStage&&) noexcept = default;
 
+    MyVector<CKKSEncoding> diagonals;               // Q-evaluator diagonals (may be empty, see above)
     MyVector<DeviceBuffer> diagonal_qp_plaintexts;  // (level+1+K)*N each
     MyVector<size_t> diagonal_indices;
     size_t slot_count = 0;
@@ -311,7 +338,7 @@ struct CKKSLinearTransformPlan {
     CKKSLinearTransformPlan& operator=(CKKSLinearTransformPlan&&) noexcept = default;
 
     MyVector<CKKSLinearTransformOwnedStage> stages;
-    bool use_qp_evaluator = true;
+    bool use_qp_evaluator = false;
 };
 
 // ------------------------------------------------------------------- context
@@ -488,6 +515,10 @@ CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& cont
                                                          const CKKSBootstrapDftMatrixLiteral& literal);
 CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& context, const CKKSCiphertext& ct,
                                                           const CKKSLinearTransformOwnedStage& stage);
+CKKSCiphertext ckks_apply_linear_transform_owned_stage_q(const CKKSContext& context, const CKKSCiphertext& ct,
+                                                         const CKKSLinearTransformOwnedStage& stage);
+CKKSCiphertext ckks_apply_linear_transform_bsgs(const CKKSContext& context, const CKKSCiphertext& ct,
+                                                const CKKSLinearTransformStage& stage);
 CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& context, CKKSCiphertext ct,
                                                 const CKKSLinearTransformPlan& plan);
 

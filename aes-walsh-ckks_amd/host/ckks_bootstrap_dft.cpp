@@ -9,9 +9,11 @@
 // per the plaintext-scale schedule (:240-259) and encoded, pre-rotated by the
 // giant-step offset (:52-96), at the stage's Q u P basis.  The residues are
 // NTT'd and put in Montgomery form on the device and stay there; applying a
-// stage runs the BSGS kernel behind mi_linear_transform_qp.
+// stage runs the BSGS kernels behind mi_linear_transform_qp (QP evaluator) or
+// mi_linear_transform_q (Q evaluator).
 #include <algorithm>
 #include <bit>
+#include <span>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -384,7 +386,7 @@ CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& c, c
     const auto sched = factor_schedule(lit);
     const bool s2c = lit.type == CKKSBootstrapDftType::SlotsToCoeffs;
     CKKSLinearTransformPlan plan;
-    plan.use_qp_evaluator = true;
+    plan.use_qp_evaluator = s2c ? P.getBootstrapS2CUseQpLinearTransform() : P.getBootstrapC2SUseQpLinearTransform();
     const auto factors = fft_factors(log_slots, !s2c);
     const auto merge = merge_depths(log_slots, nf, lit.type, lit.merge_depths);
     const double stage_scale = s2c ? std::pow(lit.transform_scale, 1.0 / (double)nf)
@@ -410,6 +412,16 @@ CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& c, c
     return plan;
 }
 
+namespace {
+// linear_transform_stage_slot_count (ckks_linear_transform.cpp:52-65)
+size_t stage_slot_count(const CKKSParams& P, size_t slot_count) {
+    const size_t full = P.getSlots(), s = slot_count == 0 ? full : slot_count;
+    if (s == 0 || s > full || full % s != 0 || !std::has_single_bit(s))
+        throw std::invalid_argument("ckks_linear_transform: invalid logical stage slot count");
+    return s;
+}
+}  // namespace
+
 CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& c, const CKKSCiphertext& ct,
                                                           const CKKSLinearTransformOwnedStage& st) {
     const auto& P = c.getParams();
@@ -420,10 +432,14 @@ CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& c, 
         throw std::invalid_argument("ckks_linear_transform: invalid QP evaluator ciphertext input");
     const size_t n = st.diagonal_indices.size();
     if (n == 0 || st.diagonal_qp_plaintexts.size() != n) throw std::invalid_argument("ckks_linear_transform: empty stage");
+    if (st.baby_step_count == 0) throw std::invalid_argument("ckks_linear_transform: baby-step count must be positive");
+    const size_t slots = stage_slot_count(P, st.slot_count);
+    for (size_t d : st.diagonal_indices)
+        if (d >= slots) throw std::invalid_argument("ckks_linear_transform: diagonal index exceeds slot count");
     // the rotation keys the stage needs
     std::vector<int64_t> rots;
     std::vector<const uint64_t*> keys;
-    std::vector<char> seen(st.slot_count + 1, 0);
+    std::vector<char> seen(std::max(slots, st.baby_step_count) + 1, 0);
     auto need = [&](size_t r) {
         if (r == 0 || seen[r]) return;
         seen[r] = 1;
@@ -442,7 +458,7 @@ CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& c, 
     CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState());
     mi_check(mi_linear_transform_qp(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, didx.data(),
                                     dptr.data(), st.diagonal_qp_plaintexts_montgomery ? 1 : 0,
-                                    (int)st.baby_step_count, (int)st.slot_count, (int)rots.size(), rots.data(),
+                                    (int)st.baby_step_count, (int)slots, (int)rots.size(), rots.data(),
                                     keys.data(), P.montgomery ? 1 : 0, out.polyData(1), out.polyData(0), c.stream()),
              "ckks_linear_transform");
     out.setScale((double)(ct.getScale() * (long double)st.plaintext_scale));
@@ -451,9 +467,146 @@ CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& c, 
     return out;
 }
 
+namespace {
+
+// The Q evaluator (ckks_apply_linear_transform_bsgs, ckks_linear_transform.cpp:
+// 818-996, and ckks_apply_linear_transform_owned_stage_q, :998-1172) with the
+// checks of validate_stage_plaintexts (:758-814) and
+// ensure_linear_transform_rotation_keys (:76-118).  diags[t] points to a
+// level+1 limb plaintext; diag_scale[t] is its scale.
+CKKSCiphertext apply_q_stage(const CKKSContext& c, const CKKSCiphertext& ct, const std::vector<const uint64_t*>& diags,
+                             const std::vector<long double>& diag_scale, const std::vector<bool>& diag_mont,
+                             std::span<const size_t> indices, size_t slot_count, size_t bs, bool rescale_after,
+                             CKKSMessageEncodingState out_state) {
+    const auto& P = c.getParams();
+    const size_t N = P.getN(), slots = stage_slot_count(P, slot_count), level = ct.getLevel();
+    const size_t n = diags.size();
+    if (n == 0) throw std::invalid_argument("ckks_linear_transform: empty diagonal set");
+    if (bs == 0) throw std::invalid_argument("ckks_linear_transform: baby-step count must be positive");
+    if (!indices.empty() && indices.size() != n)
+        throw std::invalid_argument("ckks_linear_transform: diagonal index count mismatch");
+    if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_linear_transform: expected a dense-secret ciphertext");
+    if (ct.getN() != N || ct.getNumPolys() != 2)
+        throw std::invalid_argument("ckks_linear_transform: invalid ciphertext shape");
+    if (level > P.getMaxLevel()) throw std::invalid_argument("ckks_linear_transform: invalid ciphertext level");
+    std::vector<int64_t> idx(n);
+    std::vector<char> seen(indices.empty() ? 0 : slots, 0);
+    for (size_t t = 0; t < n; ++t) {
+        if (!indices.empty()) {
+            if (indices[t] >= slots)
+                throw std::invalid_argument("ckks_linear_transform: diagonal index exceeds slot count");
+            if (seen[indices[t]]) throw std::invalid_argument("ckks_linear_transform: duplicate diagonal index");
+            seen[indices[t]] = 1;
+        }
+        idx[t] = (int64_t)(indices.empty() ? t : indices[t]);
+    }
+    for (size_t t = 1; t < n; ++t)
+        if (diag_mont[t] != diag_mont[0])
+            throw std::invalid_argument("ckks_linear_transform: diagonals mix Montgomery and normal form");
+    // rotation keys: giant count as compile_linear_transform_stage_metadata (:720-756)
+    const size_t giants = indices.empty() ? (n + bs - 1) / bs : (slots + bs - 1) / bs;
+    std::vector<char> baby(bs, 0), giant(giants, 0);
+    for (size_t t = 0; t < n; ++t) {
+        baby[(size_t)idx[t] % bs] = 1;
+        giant[(size_t)idx[t] / bs] = 1;
+    }
+    std::vector<int64_t> rots;
+    std::vector<const uint64_t*> keys;
+    for (size_t r = 1; r < bs; ++r)
+        if (baby[r]) {
+            if (!c.hasRotationKey((int64_t)r))
+                throw std::invalid_argument("ckks_linear_transform: missing baby-step rotation key");
+            rots.push_back((int64_t)r);
+            keys.push_back(c.getRotationKeyDevice((int64_t)r));
+        }
+    for (size_t g = 1; g < giants; ++g)
+        if (giant[g]) {
+            const int64_t r = (int64_t)(g * bs);
+            if (!c.hasRotationKey(r))
+                throw std::invalid_argument("ckks_linear_transform: missing giant-step rotation key");
+            rots.push_back(r);
+            keys.push_back(c.getRotationKeyDevice(r));
+        }
+    // scale of the first term of the first giant step in use
+    size_t first = 0;
+    for (size_t t = 1; t < n; ++t)
+        if ((size_t)idx[t] / bs < (size_t)idx[first] / bs) first = t;
+    // terms of one giant step keep their order, so the first one listed wins
+    for (size_t t = 0; t < n; ++t)
+        if ((size_t)idx[t] / bs == (size_t)idx[first] / bs) {
+            first = t;
+            break;
+        }
+    const size_t slot_arg = indices.empty() ? giants * bs : slots;
+    CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState());
+    mi_check(mi_linear_transform_q(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, idx.data(),
+                                   diags.data(), diag_mont[0] ? 1 : 0, (int)bs, (int)slot_arg, (int)rots.size(),
+                                   rots.data(), keys.data(), P.montgomery ? 1 : 0, out.polyData(1), out.polyData(0),
+                                   c.stream()),
+             "ckks_linear_transform");
+    out.setScale((double)((long double)ct.getScale() * diag_scale[first]));
+    if (rescale_after) ckks_rescale(c, out);
+    out.setMessageEncodingState(out_state);
+    return out;
+}
+
+}  // namespace
+
+CKKSCiphertext ckks_apply_linear_transform_bsgs(const CKKSContext& c, const CKKSCiphertext& ct,
+                                                const CKKSLinearTransformStage& stage) {
+    const size_t level = ct.getLevel();
+    std::vector<const uint64_t*> d;
+    std::vector<long double> sc;
+    std::vector<bool> mont;
+    for (const auto& e : stage.diagonals) {
+        if (e.getN() != c.getParams().getN() || e.getLevel() != level)
+            throw std::invalid_argument("ckks_linear_transform: invalid diagonal plaintext metadata");
+        d.push_back(e.data());
+        sc.push_back(e.getScale());
+        mont.push_back(e.isMontgomeryForm());
+    }
+    return apply_q_stage(c, ct, d, sc, mont, stage.diagonal_indices, stage.slot_count, stage.baby_step_count,
+                         stage.rescale_after, stage.output_encoding_state);
+}
+
+CKKSCiphertext ckks_apply_linear_transform_owned_stage_q(const CKKSContext& c, const CKKSCiphertext& ct,
+                                                         const CKKSLinearTransformOwnedStage& st) {
+    const size_t level = ct.getLevel();
+    std::vector<const uint64_t*> d;
+    std::vector<long double> sc;
+    std::vector<bool> mont;
+    if (!st.diagonals.empty()) {
+        for (const auto& e : st.diagonals) {
+            if (e.getN() != c.getParams().getN() || e.getLevel() != level)
+                throw std::invalid_argument("ckks_linear_transform: invalid diagonal plaintext metadata");
+            d.push_back(e.data());
+            sc.push_back(e.getScale());
+            mont.push_back(e.isMontgomeryForm());
+        }
+    } else {
+        // the first level+1 limbs of the QP-basis diagonals
+        if (st.level != level || st.qp_limb_count != level + 1 + c.getParams().getPModuli().size())
+            throw std::invalid_argument("ckks_linear_transform: invalid diagonal plaintext metadata");
+        for (const auto& b : st.diagonal_qp_plaintexts) {
+            d.push_back(b.data());
+            sc.push_back((long double)st.plaintext_scale);
+            mont.push_back(st.diagonal_qp_plaintexts_montgomery);
+        }
+    }
+    return apply_q_stage(c, ct, d, sc, mont, std::span<const size_t>(st.diagonal_indices), st.slot_count,
+                         st.baby_step_count, st.rescale_after, st.output_encoding_state);
+}
+
 CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& c, CKKSCiphertext ct,
                                                 const CKKSLinearTransformPlan& plan) {
     if (plan.stages.empty()) throw std::invalid_argument("ckks_linear_transform: empty transform plan");
-    for (const auto& st : plan.stages) ct = ckks_apply_linear_transform_owned_stage_qp(c, ct, st);
+    for (const auto& st : plan.stages) {
+        // should_use_qp_linear_transform (ckks_linear_transform.cpp:67-74)
+        if (!st.diagonal_qp_plaintexts.empty() && plan.use_qp_evaluator)
+            ct = ckks_apply_linear_transform_owned_stage_qp(c, ct, st);
+        else
+            ct = ckks_apply_linear_transform_owned_stage_q(c, ct, st);
+    }
     return ct;
 }

@@ -59,7 +59,10 @@ PYBIND11_MODULE(ckks_py, m) {
         .def_readwrite("evalmod_double_angle", &CKKSBootstrapParams::evalmod_double_angle)
         .def_readwrite("linear_transform_bsgs_baby_step_counts", &CKKSBootstrapParams::linear_transform_bsgs_baby_step_counts)
         .def_readwrite("linear_transform_log_bsgs_ratios", &CKKSBootstrapParams::linear_transform_log_bsgs_ratios)
-        .def_readwrite("linear_transform_merge_depths", &CKKSBootstrapParams::linear_transform_merge_depths);
+        .def_readwrite("linear_transform_merge_depths", &CKKSBootstrapParams::linear_transform_merge_depths)
+        .def_readwrite("linear_transform_use_qp", &CKKSBootstrapParams::linear_transform_use_qp)
+        .def_readwrite("evalmod_polynomial_degree", &CKKSBootstrapParams::evalmod_polynomial_degree)
+        .def_readwrite("evalmod_log_message_ratio", &CKKSBootstrapParams::evalmod_log_message_ratio);
 
     py::class_<CKKSParamsConfig>(m, "CKKSParamsConfig")
         .def_readwrite("bts_keys", &CKKSParamsConfig::bts_keys)
@@ -293,6 +296,7 @@ PYBIND11_MODULE(ckks_py, m) {
             return to_np(st.downloadDiagonalQp(c, t));
         });
     py::class_<CKKSLinearTransformPlan>(m, "CKKSLinearTransformPlan", py::module_local(false))
+        .def_readwrite("use_qp_evaluator", &CKKSLinearTransformPlan::use_qp_evaluator)
         .def("stageCount", [](const CKKSLinearTransformPlan& p) { return p.stages.size(); })
         .def("stage", [](const CKKSLinearTransformPlan& p, size_t i) -> const CKKSLinearTransformOwnedStage& {
                  return p.stages.at(i);
@@ -306,9 +310,34 @@ PYBIND11_MODULE(ckks_py, m) {
           py::keep_alive<0, 1>());
     m.def("ckks_apply_linear_transform_plan_stage",
           [](const CKKSContext& c, const CKKSCiphertext& ct, const CKKSLinearTransformPlan& plan, size_t i) {
-              return ckks_apply_linear_transform_owned_stage_qp(c, ct, plan.stages.at(i));
+              const auto& st = plan.stages.at(i);
+              return plan.use_qp_evaluator ? ckks_apply_linear_transform_owned_stage_qp(c, ct, st)
+                                           : ckks_apply_linear_transform_owned_stage_q(c, ct, st);
           },
           py::keep_alive<0, 1>());
+    // ckks_apply_linear_transform_bsgs over diagonals given as plaintext limbs
+    // (each becomes a CKKSEncoding at the ciphertext's level)
+    m.def("ckks_apply_linear_transform_bsgs",
+          [](const CKKSContext& c, const CKKSCiphertext& ct, const std::vector<py::array_t<uint64_t>>& pts,
+             double scale, bool montgomery, const std::vector<size_t>& indices, size_t baby_step_count,
+             size_t slot_count, bool rescale_after) {
+              MyVector<CKKSEncoding> diags;
+              diags.reserve(pts.size());
+              for (const auto& a : pts) {
+                  auto arr = py::array_t<uint64_t, py::array::c_style | py::array::forcecast>::ensure(a);
+                  diags.emplace_back(c, from_np(arr), (long double)scale, ct.getLevel(), montgomery);
+              }
+              CKKSLinearTransformStage st;
+              st.diagonals = std::span<const CKKSEncoding>(diags.data(), diags.size());
+              st.diagonal_indices = std::span<const size_t>(indices.data(), indices.size());
+              st.slot_count = slot_count;
+              st.baby_step_count = baby_step_count;
+              st.rescale_after = rescale_after;
+              return ckks_apply_linear_transform_bsgs(c, ct, st);
+          },
+          py::arg("context"), py::arg("ct"), py::arg("diagonals"), py::arg("scale"), py::arg("montgomery"),
+          py::arg("diagonal_indices"), py::arg("baby_step_count"), py::arg("slot_count") = 0,
+          py::arg("rescale_after") = true, py::keep_alive<0, 1>());
     m.def("ckks_key_switch_dense_to_sparse_inplace", &ckks_key_switch_dense_to_sparse_inplace);
     m.def("ckks_key_switch_sparse_to_dense_inplace", &ckks_key_switch_sparse_to_dense_inplace);
     m.def("ckks_modraise_inplace", &ckks_modraise_inplace);

@@ -59,6 +59,12 @@ _SIGS = {
     "mi_moddown_qp": (cint, [vp, cint, vp, vp]),
     "mi_linear_transform_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
                                       cint, vp, vp, vp]),
+    "mi_linear_transform_qp_batch": (cint, [vp, cint, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint,
+                                            vp, vp, cint, vp, vp, vp]),
+    "mi_linear_transform_q": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
+                                     cint, vp, vp, vp]),
+    "mi_linear_transform_q_batch": (cint, [vp, cint, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint,
+                                           vp, vp, cint, vp, vp, vp]),
     "mi_modraise": (cint, [vp, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint, vp]),
 }
@@ -281,6 +287,40 @@ class Context:
                                            ctypes.addressof(dp), int(pt_montgomery), int(baby_steps),
                                            int(slots), len(rots), ctypes.addressof(kr), ctypes.addressof(kp),
                                            int(montgomery), ptr(a_out), ptr(b_out), _stream(stream)))
+
+    def linear_transform_q(self, level, a, b, diag_indices, diags, baby_steps, rot_keys, a_out, b_out,
+                           pt_montgomery=True, slots=None, montgomery=True, stream=None):
+        """Q-evaluator stage; diags: level+1 limb plaintexts."""
+        self.linear_transform_qp_batch(level, [(a, b)], diag_indices, diags, baby_steps, rot_keys,
+                                       [(a_out, b_out)], pt_montgomery, slots, montgomery, stream,
+                                       q_evaluator=True)
+
+    def linear_transform_qp_batch(self, level, cts, diag_indices, diags, baby_steps, rot_keys, outs,
+                                  pt_montgomery=True, slots=None, montgomery=True, stream=None,
+                                  q_evaluator=False):
+        """cts / outs: lists of (a, b) device tensor pairs (level+1 limbs each);
+        the stage is applied to every ciphertext of the batch (QP evaluator, or
+        the Q evaluator with level+1 limb diagonals when q_evaluator)."""
+        n = len(diag_indices)
+        m = len(cts)
+        if len(outs) != m:
+            raise ValueError("one output pair per input ciphertext")
+        idx = (ctypes.c_int64 * max(n, 1))(*[int(d) for d in diag_indices])
+        dp = (vp * max(n, 1))(*[d.data_ptr() for d in diags])
+        rots = sorted(rot_keys)
+        kr = (ctypes.c_int64 * max(len(rots), 1))(*[int(r) for r in rots])
+        kp = (vp * max(len(rots), 1))(*[rot_keys[r].data_ptr() for r in rots])
+        pa = (vp * max(m, 1))(*[c[0].data_ptr() for c in cts])
+        pb = (vp * max(m, 1))(*[c[1].data_ptr() for c in cts])
+        oa = (vp * max(m, 1))(*[o[0].data_ptr() for o in outs])
+        ob = (vp * max(m, 1))(*[o[1].data_ptr() for o in outs])
+        slots = self.N // 2 if slots is None else slots
+        fn = lib().mi_linear_transform_q_batch if q_evaluator else lib().mi_linear_transform_qp_batch
+        check(fn(
+            self.h, level, m, ctypes.addressof(pa), ctypes.addressof(pb), n, ctypes.addressof(idx),
+            ctypes.addressof(dp), int(pt_montgomery), int(baby_steps), int(slots), len(rots),
+            ctypes.addressof(kr), ctypes.addressof(kp), int(montgomery), ctypes.addressof(oa),
+            ctypes.addressof(ob), _stream(stream)))
 
     # ---- bootstrapping pieces
     def modraise(self, target_level, poly_q0, out, stream=None):

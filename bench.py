@@ -250,19 +250,52 @@ def bench_lt(args, M, torch):
     cts = [_rand_limbs(torch, q[:L], N, 2, seed=9000 + i) for i in range(nct)]
     outs = [torch.empty(2 * L * N, dtype=torch.int64, device="cuda") for _ in range(nct)]
     W = L * N
+    pairs = [(c[W:], c[:W]) for c in cts]
+    opairs = [(o[W:], o[:W]) for o in outs]
+    stream = torch.cuda.Stream()
 
     def step():
-        for c, o in zip(cts, outs):
-            ctx.linear_transform_qp(level, c[W:], c[:W], diag_idx, diags, bs, keys, o[W:], o[:W])
+        # all 32 ciphertexts through the stage in one call
+        ctx.linear_transform_qp_batch(level, pairs, diag_idx, diags, bs, keys, opairs, stream=stream)
 
-    for _ in range(args.warmup):
-        step()
+    def step_plain():
+        for (a, b), (ao, bo) in zip(pairs, opairs):
+            ctx.linear_transform_qp(level, a, b, diag_idx, diags, bs, keys, ao, bo, stream=stream)
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    with torch.cuda.stream(stream):
+        for _ in range(max(1, args.warmup)):
+            step()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    ref = torch.cat(outs).clone()
+    eager_ms = timed(step, args.steps)
+    plain_ms = timed(step_plain, 1) if args.plain else None
+    # the batch captured once in a HIP graph and replayed
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
         step()
+    for o in outs:
+        o.zero_()
+    replay_ok = True
+    for _ in range(2):
+        graph.replay()
+        torch.cuda.synchronize()
+        replay_ok = replay_ok and bool(torch.equal(torch.cat(outs), ref))
+        for o in outs:
+            o.zero_()
+    if not replay_ok:
+        raise RuntimeError("graph replay differs from the eager batch")
+    ms = timed(graph.replay, args.steps)
     torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    if not torch.equal(torch.cat(outs), ref):
+        raise RuntimeError("graph replay differs from the eager batch")
     T = L + K
     # per ciphertext: ct in/out, diagonals and keys amortised over the batch
     algo = nct * (2 * L * 8 * N * 2) + len(diag_idx) * T * 8 * N + len(rots) * parts * 2 * (len(q) + K) * 8 * N
@@ -274,8 +307,12 @@ def bench_lt(args, M, torch):
         "value": nct / (ms / 1e3), "unit": "ciphertext/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
-        "config": {"workload": "32 ct x BSGS stage (16 baby x 8 giant), QP diagonals, 22 rotation keys",
-                   "algorithmic_GBps": algo / (ms / 1e3) / 1e9},
+        "config": {"workload": "32 ct x BSGS stage (16 baby x 8 giant), QP diagonals, 22 rotation keys, "
+                               "one batched call captured in a HIP graph",
+                   "algorithmic_GBps": algo / (ms / 1e3) / 1e9, "eager_batched_ms": eager_ms,
+                   "plain_one_ct_per_call_ms": plain_ms,
+                   "batched_over_plain": (plain_ms / ms) if plain_ms else None,
+                   "graph_replays_match_eager": replay_ok},
         "roofline": None, "cpu_baseline": cpu,
     }
 

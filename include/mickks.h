@@ -147,7 +147,13 @@ int mi_keyswitch(mi_context* ctx, int level, const uint64_t* key_term, const uin
                  uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
                  void* stream);
 
-/* One key switch of a ragged batch (mixed levels and keys in one launch). */
+/* One key switch of a ragged batch (mixed levels, keys and combine modes):
+ * the reference's OpenMP loops over independent switches
+ * (ckks_aes_ctr_walsh_sbox.cpp:238, :302).  Items of one (level, modes) are
+ * switched together, up to 32 per launch sequence (each phase of the switch
+ * is one launch for all of them); an item whose buffers overlap an earlier
+ * item's outputs (or whose outputs overlap its inputs) runs after it, so the
+ * result equals switching the items one by one in order. */
 typedef struct {
     const uint64_t* key_term;
     const uint64_t* key;
@@ -213,6 +219,39 @@ int mi_linear_transform_qp(mi_context* ctx, int level, const uint64_t* a, const 
                            int pt_montgomery, int baby_steps, int slots, int nkeys,
                            const int64_t* key_rotations, const uint64_t* const* keys,
                            int montgomery, uint64_t* a_out, uint64_t* b_out, void* stream);
+/* The same stage applied to count ciphertexts at one level: the per-batch
+ * OpenMP loops of the reference (ckks_aes_ctr_walsh_sbox.cpp:437-470 over
+ * refresh batches) run independent ciphertexts through the same stages.
+ * One launch sequence serves up to 32 ciphertexts: each diagonal and key is
+ * read once per launch, not once per ciphertext.  a_out[i] / b_out[i] may
+ * alias a[i] / b[i], not another ciphertext's buffers. */
+int mi_linear_transform_qp_batch(mi_context* ctx, int level, int count, const uint64_t* const* a,
+                                 const uint64_t* const* b, int nterms, const int64_t* diag_indices,
+                                 const uint64_t* const* diags, int pt_montgomery, int baby_steps, int slots,
+                                 int nkeys, const int64_t* key_rotations, const uint64_t* const* keys,
+                                 int montgomery, uint64_t* const* a_out, uint64_t* const* b_out,
+                                 void* stream);
+
+/* ---- BSGS linear-transform stage, Q evaluator ----------------------------
+ * ckks_apply_linear_transform_bsgs (ckks_linear_transform.cpp:818-996) and
+ * ckks_apply_linear_transform_owned_stage_q (:998-1172) without the final
+ * rescale: every needed baby rotation r < baby_steps is a full hoisted
+ * rotation in Q, diags[t] are level+1 limb plaintexts (Montgomery form if
+ * pt_montgomery), each giant step's inner sum is rotated by a key switch and
+ * its automorphism, and the results are added.  Indices must be distinct and
+ * below slots.  Arguments as for mi_linear_transform_qp. */
+int mi_linear_transform_q(mi_context* ctx, int level, const uint64_t* a, const uint64_t* b, int nterms,
+                          const int64_t* diag_indices, const uint64_t* const* diags, int pt_montgomery,
+                          int baby_steps, int slots, int nkeys, const int64_t* key_rotations,
+                          const uint64_t* const* keys, int montgomery, uint64_t* a_out, uint64_t* b_out,
+                          void* stream);
+/* The Q-evaluator stage over count ciphertexts at one level (as
+ * mi_linear_transform_qp_batch). */
+int mi_linear_transform_q_batch(mi_context* ctx, int level, int count, const uint64_t* const* a,
+                                const uint64_t* const* b, int nterms, const int64_t* diag_indices,
+                                const uint64_t* const* diags, int pt_montgomery, int baby_steps, int slots,
+                                int nkeys, const int64_t* key_rotations, const uint64_t* const* keys,
+                                int montgomery, uint64_t* const* a_out, uint64_t* const* b_out, void* stream);
 
 /* ---- bootstrapping pieces --------------------------------------------------
  * modraise_eval_poly_from_q0 (ckks_modraise.cpp:31-75): one q_0 eval limb,

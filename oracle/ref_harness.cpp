@@ -339,6 +339,80 @@ static void run_hoist_lt() {
     }
 }
 
+// Seeded residues of one Q-basis diagonal plaintext (level+1 limbs).
+static MyVector<uint64_t> lt_diag_q(size_t N, const MyVector<uint64_t>& q, size_t limbs, size_t term) {
+    MyVector<uint64_t> out(N * limbs);
+    for (size_t l = 0; l < limbs; ++l) {
+        uint64_t seed = 7000 + 64 * term + l;
+        for (size_t i = 0; i < N; ++i) out[l * N + i] = splitmix64(seed) % q[l];
+    }
+    return out;
+}
+
+// Q-evaluator BSGS stages: an owned stage through a plan with the Q
+// evaluator (Montgomery-form diagonals, no rescale) and
+// ckks_apply_linear_transform_bsgs (normal-form diagonals, rescale after).
+static void run_lt_q() {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << 10;
+    c.mult_depth = 3;
+    c.q0_bit = 50;
+    c.q_bit = 40;
+    c.p_bits = {60, 60};
+    c.dnum = 2;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    c.rotate_values = {1, 2, 3, 6};
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN(), L = P.getMaxLevel();
+    const std::string tag = "lq10";
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    for (int r : {1, 2, 3, 6}) dump_key(tag + "_rot" + std::to_string(r), ctx.getRotationKey(r));
+    uint64_t seed = 93;
+    MyVector<Complex> m(N / 2);
+    for (auto& z : m) {
+        double re = (double)(splitmix64(seed) >> 11) * 0x1.0p-53 * 2 - 1;
+        z = Complex(re, re / 3);
+    }
+    auto ct = ckks_encrypt(ctx, m, L);
+    dump_ct(tag + "_ct", ct);
+    const long double pt_scale = 1099511627776.0L;  // 2^40
+    {
+        // diagonals {0, 1, 2, 4, 5, 7}, baby steps 3: giants 0, 3, 6
+        CKKSLinearTransformPlan plan;
+        plan.use_qp_evaluator = false;
+        plan.stages.emplace_back();
+        auto& st = plan.stages.back();
+        const size_t idx[6] = {0, 1, 2, 4, 5, 7};
+        for (size_t t = 0; t < 6; ++t) {
+            st.diagonals.emplace_back(N, lt_diag_q(N, P.getModuli(), L + 1, t), pt_scale, L, true);
+            st.diagonal_indices.emplace_back(idx[t]);
+        }
+        st.baby_step_count = 3;
+        st.rescale_after = false;
+        auto out = ckks_apply_linear_transform_plan(ctx, ct.clone(), plan);
+        dump_ct(tag + "_ltq", out);
+    }
+    {
+        // diagonals {1, 3, 5, 6} in normal form through the stage view API
+        MyVector<CKKSEncoding> diags;
+        const size_t idx[4] = {1, 3, 5, 6};
+        for (size_t t = 0; t < 4; ++t)
+            diags.emplace_back(N, lt_diag_q(N, P.getModuli(), L + 1, 10 + t), pt_scale, L, false);
+        MyVector<size_t> indices(idx, idx + 4);
+        CKKSLinearTransformStage st;
+        st.diagonals = std::span<const CKKSEncoding>(diags.data(), diags.size());
+        st.diagonal_indices = std::span<const size_t>(indices.data(), indices.size());
+        st.baby_step_count = 3;
+        st.rescale_after = true;
+        auto out = ckks_apply_linear_transform_bsgs(ctx, ct, st);
+        dump_ct(tag + "_bsgs", out);
+    }
+}
+
 // Pieces of bootstrapping on a context with bootstrap keys and EvalMod depth
 // 0 (no EvalMod polynomial, so nothing reaches the MPFR fit).
 static void run_bootstrap_pieces() {
@@ -442,7 +516,7 @@ int main(int argc, char** argv) {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR [case ...]\n"); return 2; }
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);
-    // cases: primes ntt eval lt boot evalpoly (all when none is named)
+    // cases: primes ntt eval lt ltq boot evalpoly (all when none is named)
     auto want = [&](const char* name) {
         if (argc < 3) return true;
         for (int i = 2; i < argc; ++i)
@@ -460,6 +534,7 @@ int main(int argc, char** argv) {
         run_eval(10, 4, 3, {61, 61}, "ev10b");
     }
     if (want("lt")) run_hoist_lt();
+    if (want("ltq")) run_lt_q();
     if (want("boot")) run_bootstrap_pieces();
     if (want("evalpoly")) run_evalpoly();
     std::printf("ref_harness: wrote %s\n", g_out.c_str());

@@ -4,7 +4,7 @@ Run in a container where /root/reference exists:
     make -C oracle ref && oracle/_ref/ref_harness oracle/_ref/raw [case ...]
     python tests/golden/make_golden.py [--raw DIR] [group ...]
 
-Groups (all when none is named): primes ntt eval lt boot evalpoly.  The
+Groups (all when none is named): primes ntt eval lt ltq boot evalpoly.  The
 harness draws keys and encryption masks from the reference's own
 unseedable randomness, so repack only the groups whose raw arrays were
 regenerated together.
@@ -112,6 +112,26 @@ def pack_lt():
         json.dump(lt_meta, f, indent=1)
 
 
+def pack_ltq():
+    # Q-evaluator BSGS stages (owned stage through a plan, and the stage-view API)
+    meta = {}
+    for o in ["ltq", "bsgs"]:
+        m = rd(f"lq10_{o}_meta")
+        meta[o] = {"digest": digest(rd(f"lq10_{o}")), "npolys": int(m[0]), "level": int(m[1]),
+                   "scale": float(m[2:3].view(np.float64)[0])}
+    meta["ltq"].update({"diag_seed_base": 7000, "terms": [0, 1, 2, 3, 4, 5], "diag_indices": [0, 1, 2, 4, 5, 7],
+                        "baby_steps": 3, "montgomery": True, "rescale": False})
+    meta["bsgs"].update({"diag_seed_base": 7000, "terms": [10, 11, 12, 13], "diag_indices": [1, 3, 5, 6],
+                         "baby_steps": 3, "montgomery": False, "rescale": True})
+    meta["pt_scale"] = 2.0 ** 40
+    arr = {"q": rd("lq10_q"), "p": rd("lq10_p"), "ct": rd("lq10_ct"), "ct_meta": rd("lq10_ct_meta")}
+    for r in (1, 2, 3, 6):
+        arr[f"key_rot{r}"] = rd(f"lq10_rot{r}")
+    np.savez_compressed(os.path.join(HERE, "lq10.npz"), **arr)
+    with open(os.path.join(HERE, "lq10_outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 def pack_boot():
     # bootstrap pieces: dense->sparse q0p0 switch, ModRaise, S2C plan digests
     meta = rd("bt10_s2c_meta")
@@ -150,7 +170,8 @@ def pack_evalpoly():
         json.dump(meta, f, indent=1)
 
 
-GROUPS = {"primes": pack_primes, "ntt": pack_ntt, "eval": pack_eval, "lt": pack_lt, "boot": pack_boot,
+GROUPS = {"primes": pack_primes, "ntt": pack_ntt, "eval": pack_eval, "lt": pack_lt, "ltq": pack_ltq,
+          "boot": pack_boot,
           "evalpoly": pack_evalpoly}
 
 

@@ -162,6 +162,35 @@ def test_keyswitch_batch_mixed_levels():
         assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa)
 
 
+def test_keyswitch_batch_groups_and_dependencies():
+    """Items at one level are switched together (up to 32 per launch); an
+    item that reads another's output, or adds into the same buffers, still
+    sees the result of switching the items one by one in order."""
+    N = 1 << 12
+    M, ctx, octx, q, p, rng = _random_setup(N, 6, 2, 2, seed=13)
+    keys = [_rand_key(rng, q, p, N, octx.parts(5)) for _ in range(2)]
+    dkeys = [M.to_dev(k) for k in keys]
+    host, specs = {}, []
+    for i in range(37):
+        level = 4 if i % 5 else 2
+        for name in ("kt", "b", "a"):
+            host[(name, i)] = rand_residues(rng, q[: level + 1], N)
+        bm, am = (0, 1) if i % 3 else (1, 0)
+        specs.append([level, ("kt", i), i % 2, ("b", i), ("a", i), bm, am])
+    # item 20 switches item 10's a output (same level); item 21 adds into item
+    # 11's outputs
+    specs[20][0], specs[20][1] = specs[10][0], ("a", 10)
+    specs[21] = [specs[11][0], ("kt", 21), 1, ("b", 11), ("a", 11), 0, 0]
+    host[("kt", 21)] = rand_residues(rng, q[: specs[11][0] + 1], N)
+    dev = {k: M.to_dev(v) for k, v in host.items()}
+    items = [(lv, dev[kt], dkeys[k], dev[b], dev[a], bm, am) for lv, kt, k, b, a, bm, am in specs]
+    for lv, kt, k, b, a, bm, am in specs:
+        host[b], host[a] = octx.keyswitch(lv, host[kt], keys[k], host[b], host[a], bm, am)
+    ctx.keyswitch_batch(items)
+    for name, v in host.items():
+        assert np.array_equal(M.from_dev(dev[name]), v), name
+
+
 @pytest.mark.parametrize("N,nq", [(1 << 12, 6), (1 << 15, 13), (1 << 16, 8)])
 def test_rescale_and_elementwise_match_oracle(N, nq):
     M, ctx, octx, q, p, rng = _random_setup(N, nq, 2, 2, qbits=40 + nq % 7, seed=nq)

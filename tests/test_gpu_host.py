@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import digest, host, load_json, load_npz
+from helpers import digest, host, load_json, load_npz, lt_diags
 
 pytestmark = pytest.mark.gpu
 
@@ -349,6 +349,7 @@ def _bt10_context(H):
     bp.sparse_secret_hamming_weight = 16
     bp.linear_transform_log_plaintext_scales = [[[25], [25], [25]], [[25], [25], [25]]]
     bp.linear_transform_depths = [3, 3]
+    bp.linear_transform_use_qp = [True, True]
     c.bootstrap_params = bp
     ctx = H.CKKSContext(H.CKKSParams(c))
     P = ctx.getParams()
@@ -439,6 +440,44 @@ def test_host_dense_sparse_switches_and_modraise_match_reference():
     H.ckks_modraise_inplace(ctx, ct, o["modraise_level"])
     assert ct.getLevel() == o["modraise_level"]
     assert digest(np.concatenate([ct.getPoly(0), ct.getPoly(1)])) == o["modraise"]
+
+
+def test_host_linear_transform_bsgs_matches_reference_lq10():
+    """ckks_apply_linear_transform_bsgs (Q evaluator, normal-form diagonals,
+    rescale after) with the reference's keys reproduces its output limbs,
+    level and scale; its missing-key error names the reference's message."""
+    H = host()
+    d = load_npz("lq10.npz")
+    o = load_json("lq10_outputs.json")
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 3
+    c.q0_bit = 50
+    c.q_bit = 40
+    c.p_bits = [60, 60]
+    c.dnum = 2
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [1, 2, 3, 6]
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    q = P.getModuli()
+    assert list(q) == list(d["q"]) and list(P.getPModuli()) == list(d["p"])
+    for r in (1, 2, 3, 6):
+        ctx.setRotationKey(r, d[f"key_rot{r}"])
+    N, L = 1 << 10, len(q)
+    W = L * N
+    scale = float(d["ct_meta"][2:3].view(np.float64)[0])
+    ct = H.CKKSCiphertext(ctx, [d["ct"][:W], d["ct"][W:]], scale, L - 1)
+    e = o["bsgs"]
+    diags = lt_diags(N, q, 14, seed_base=e["diag_seed_base"]).reshape(14, W)
+    out = H.ckks_apply_linear_transform_bsgs(ctx, ct, [diags[t] for t in e["terms"]], o["pt_scale"], False,
+                                             e["diag_indices"], e["baby_steps"])
+    assert digest(np.concatenate([out.getPoly(0), out.getPoly(1)])) == e["digest"]
+    assert out.getLevel() == e["level"] and out.getScale() == e["scale"]
+    with pytest.raises(ValueError, match="missing giant-step rotation key"):
+        H.ckks_apply_linear_transform_bsgs(ctx, ct, [diags[0]], o["pt_scale"], False, [12], 3)
 
 
 def _bits_bootstrap_context(H):

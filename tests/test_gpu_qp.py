@@ -1,8 +1,8 @@
 """GPU parity of hoisting, the QP-basis key switches, the BSGS linear-transform
 stage, ModRaise and the level-0 q0p0 switch, through the C-ABI.
 
-Bit-exact against the reference's own outputs (lt10 / bt10 golden fixtures,
-captured keys) and against the oracle on seeded random inputs at larger ring
+Bit-exact against the reference's own outputs (lt10 / lq10 / bt10 golden
+fixtures, captured keys) and against the oracle on seeded random inputs at larger ring
 sizes, with random residues standing in for keys and diagonals."""
 import numpy as np
 import pytest
@@ -59,6 +59,50 @@ def test_qp_path_matches_reference_lt10():
     assert digest(r.ravel()) == o["lt_rescaled"]
     # inputs untouched
     assert np.array_equal(M.from_dev(da), a) and np.array_equal(M.from_dev(db), b)
+
+
+def test_q_evaluator_matches_reference_lq10():
+    """The Q-evaluator BSGS stage (owned stage through a plan with
+    Montgomery-form diagonals; the stage-view API with normal-form diagonals
+    and a rescale) reproduces the reference's outputs; a batch of the same
+    ciphertext three times (once in place) gives the same limbs."""
+    d = load_npz("lq10.npz")
+    o = load_json("lq10_outputs.json")
+    M = product()
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    L = len(q)
+    lv = L - 1
+    ctx = M.Context(N, q, p, 2)
+    b, a = d["ct"].reshape(2, L * N)
+    keys = {r: M.to_dev(d[f"key_rot{r}"]) for r in (1, 2, 3, 6)}
+    diags = lt_diags(N, q, 14, seed_base=o["ltq"]["diag_seed_base"]).reshape(14, L * N)
+    for name in ("ltq", "bsgs"):
+        e = o[name]
+        dd = [M.to_dev(diags[t]) for t in e["terms"]]
+        da, db = M.to_dev(a), M.to_dev(b)
+        ao, bo = _zeros(M, L * N), _zeros(M, L * N)
+        ctx.linear_transform_q(lv, da, db, e["diag_indices"], dd, e["baby_steps"], keys, ao, bo,
+                               pt_montgomery=e["montgomery"])
+        out = np.concatenate([M.from_dev(bo), M.from_dev(ao)])
+        if e["rescale"]:
+            pair = M.to_dev(out)
+            ctx.rescale(lv, pair, 2)
+            out = M.from_dev(pair).reshape(2, L * N)[:, : (L - 1) * N].ravel()
+        assert digest(out) == e["digest"], name
+        if name == "ltq":
+            want = out
+            cts = [(M.to_dev(a), M.to_dev(b)) for _ in range(3)]
+            outs = [(_zeros(M, L * N), _zeros(M, L * N)), cts[1], (_zeros(M, L * N), _zeros(M, L * N))]
+            ctx.linear_transform_qp_batch(lv, cts, e["diag_indices"], dd, e["baby_steps"], keys, outs,
+                                          pt_montgomery=True, q_evaluator=True)
+            for ao, bo in outs:
+                assert np.array_equal(np.concatenate([M.from_dev(bo), M.from_dev(ao)]), want)
+        assert np.array_equal(M.from_dev(da), a) and np.array_equal(M.from_dev(db), b)
+    with pytest.raises(M.MiError, match="duplicate diagonal"):
+        ctx.linear_transform_q(lv, da, db, [1, 1], [dd[0], dd[1]], 3, keys, ao, bo)
+    with pytest.raises(M.MiError, match="exceeds slot count"):
+        ctx.linear_transform_q(lv, da, db, [0, 9], [dd[0], dd[1]], 3, keys, ao, bo, slots=8)
 
 
 def test_bootstrap_pieces_match_reference_bt10():
@@ -178,6 +222,44 @@ def test_linear_transform_many_terms_per_giant():
     """More terms in one giant step than one inner-product launch takes."""
     M, ctx, octx, q, p, rng = _setup(1 << 12, 6, 2, 2, seed=5)
     _lt_case(M, ctx, octx, q, p, rng, 1 << 12, 5, list(range(40)) + [64, 70], 64)
+
+
+def _lt_batch_case(M, ctx, octx, q, p, rng, N, level, diag_idx, bs, n, in_place=()):
+    L = level + 1
+    qp = list(q[:L]) + list(p)
+    parts = octx.parts(len(q) - 1)
+    diags = [rand_residues(rng, qp, N) for _ in diag_idx]
+    rots = {d % bs for d in diag_idx if d % bs} | {(d // bs) * bs for d in diag_idx if d // bs}
+    keys = {r: _key(rng, q, p, N, parts) for r in sorted(rots)}
+    dkeys = {r: M.to_dev(k) for r, k in keys.items()}
+    dd = [M.to_dev(x) for x in diags]
+    cts, outs, want = [], [], []
+    for i in range(n):
+        a = rand_residues(rng, q[:L], N)
+        b = rand_residues(rng, q[:L], N)
+        want.append(octx.lt_stage_qp(level, a, b, diag_idx, np.concatenate(diags), bs, keys))
+        da, db = M.to_dev(a), M.to_dev(b)
+        cts.append((da, db))
+        outs.append((da, db) if i in in_place else (_zeros(M, L * N), _zeros(M, L * N)))
+    ctx.linear_transform_qp_batch(level, cts, diag_idx, dd, bs, dkeys, outs)
+    for i, ((ao, bo), (oa, ob)) in enumerate(zip(outs, want)):
+        assert np.array_equal(M.from_dev(ao), oa), (level, i)
+        assert np.array_equal(M.from_dev(bo), ob), (level, i)
+
+
+def test_linear_transform_batch_matches_oracle():
+    """Several ciphertexts through one stage in one call, some in place."""
+    M, ctx, octx, q, p, rng = _setup(1 << 13, 9, 3, 3, seed=41)
+    _lt_batch_case(M, ctx, octx, q, p, rng, 1 << 13, 8, [0, 1, 3, 5, 6, 9, 12, 14], 4, 5, in_place={1, 4})
+    _lt_batch_case(M, ctx, octx, q, p, rng, 1 << 13, 4, [5, 9, 10], 4, 3)
+
+
+def test_linear_transform_batch_beyond_one_launch():
+    """More ciphertexts than one launch sequence takes (32), and more baby
+    steps than the inner-product kernel holds in registers (16)."""
+    M, ctx, octx, q, p, rng = _setup(1 << 12, 6, 2, 2, seed=43)
+    _lt_batch_case(M, ctx, octx, q, p, rng, 1 << 12, 3, [0, 2, 7, 9], 4, 35, in_place={0, 33})
+    _lt_batch_case(M, ctx, octx, q, p, rng, 1 << 12, 5, list(range(20)) + [32, 41], 32, 3)
 
 
 @pytest.mark.parametrize("N,nq,mont", [(1 << 14, 10, True), (1 << 16, 24, False)])
