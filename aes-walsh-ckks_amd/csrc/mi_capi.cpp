@@ -556,6 +556,41 @@ int mi_linear_transform_q(mi_context* ctx, int level, const uint64_t* a, const u
                                        &b_out, stream);
 }
 
+int mi_set_encoding_tables(mi_context* ctx, const double* twiddles, const uint64_t* perm) {
+    return guard("ckks_context", [&] {
+        check_ctx(ctx);
+        if (!twiddles || !perm) throw mi::Error(MI_EINVAL, "null pointer argument");
+        mi::set_encoding_tables(*ctx->c, twiddles, perm);
+    });
+}
+
+int mi_encode_slots(mi_context* ctx, const double* values, int count, double scale, int limbs, int eval,
+                    uint64_t* out, int64_t poly_stride, void* stream) {
+    return guard("ckks_encode", [&] {
+        check_ctx(ctx);
+        if (!values || !out) throw mi::Error(MI_EINVAL, "null pointer argument");
+        mi::encode_slots(*ctx->c, values, count, scale, limbs, eval != 0, out, (long long)poly_stride, S(stream));
+    });
+}
+
+int mi_sample_uniform(mi_context* ctx, uint64_t* out, int limbs, int basis0, uint64_t seed, uint64_t counter,
+                      void* stream) {
+    return guard("ckks_uniform_mask_polys", [&] {
+        check_ctx(ctx);
+        check_ptr(out);
+        mi::sample_uniform(*ctx->c, out, limbs, basis0, seed, counter, S(stream));
+    });
+}
+
+int mi_sample_gaussian(mi_context* ctx, uint64_t* out, int limbs, double sigma, uint64_t seed, uint64_t counter,
+                       void* stream) {
+    return guard("ckks_gaussian_noise_polys", [&] {
+        check_ctx(ctx);
+        check_ptr(out);
+        mi::sample_gaussian(*ctx->c, out, limbs, sigma, seed, counter, S(stream));
+    });
+}
+
 int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint64_t* out,
                 void* stream) {
     return guard("modraise_eval_poly_from_q0", [&] {

@@ -361,6 +361,9 @@ void context_destroy(Context* c) {
     (void)hipFree(c->d_bc);
     (void)hipFree(c->d_blob);
     (void)hipFree(c->d_half);
+    (void)hipFree(c->d_enc_tw);
+    (void)hipFree(c->d_enc_inv);
+    (void)hipFree(c->d_gauss_cdf);
     for (auto& kv : c->ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     delete c;

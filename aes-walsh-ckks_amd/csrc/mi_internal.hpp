@@ -73,6 +73,12 @@ struct Context {
     std::vector<RescaleTables> rescale;           // [level] (level >= 1)
     u64* d_half = nullptr;                        // rescale half_ntt storage
     u64 q0p0_inv = 0, q0p0_inv_shoup = 0;         // p_0^-1 mod q_0 (level-0 switch)
+    double* d_enc_tw = nullptr;                   // encoding FFT twiddles [N] complex
+    int* d_enc_inv = nullptr;                     // encoding slot placement [N]
+    std::mutex gauss_mu;                          // discrete Gaussian table of the last sigma
+    double gauss_sigma = 0.0;
+    double* d_gauss_cdf = nullptr;
+    int gauss_bound = 0, gauss_n = 0;
     size_t table_bytes = 0;
     std::mutex ws_mu;
     std::map<hipStream_t, Workspace> ws;
@@ -244,6 +250,19 @@ void linear_transform_qp_batch(Context& c, const LtStage& st, int n, const u64* 
 // rotated by a full key switch.
 void linear_transform_q_batch(Context& c, const LtStage& st, int n, const u64* const* a,
                               const u64* const* b, u64* const* a_out, u64* const* b_out, hipStream_t s);
+
+// Slot encoding on the device (mi_encode.hip): twiddles = N complex doubles
+// (re, im), perm = the N/2-entry slot permutation; values = count x N/2
+// complex doubles on the host; out poly b at out + b*stride.
+void set_encoding_tables(Context& c, const double* twiddles, const u64* perm);
+void encode_slots(Context& c, const double* values, int count, double scale, int limbs, bool eval, u64* out,
+                  long long stride, hipStream_t s);
+
+// Samplers (mi_encode.hip), counter-based (Philox4x32-10): a limb range of
+// uniform residues (basis indices basis0..), and one discrete Gaussian
+// integer per coefficient as residues of limbs 0..limbs-1.
+void sample_uniform(Context& c, u64* out, int limbs, int basis0, u64 seed, u64 counter, hipStream_t s);
+void sample_gaussian(Context& c, u64* out, int limbs, double sigma, u64 seed, u64 counter, hipStream_t s);
 
 // Bootstrapping pieces (mi_boot.hip).
 void modraise(Context& c, int target_level, const u64* poly_q0, u64* out, hipStream_t s);

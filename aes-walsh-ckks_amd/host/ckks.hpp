@@ -260,6 +260,7 @@ public:
     MyVector<uint64_t> getPlaintext() const;
     const uint64_t* data() const { return buf.data(); }
     bool isMontgomeryForm() const { return montgomery_form; }
+    const CKKSContext& getContext() const { return *ctx; }
     long double getScale() const { return scale; }
     size_t getLevel() const { return level; }
     size_t getN() const { return N; }
@@ -513,6 +514,14 @@ CKKSCiphertext ckks_evalchebyshev_preconditioned(const CKKSContext& context, con
 
 CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& context,
                                                          const CKKSBootstrapDftMatrixLiteral& literal);
+// A one-stage plan from caller diagonals (index -> N/2 slot values), encoded
+// as the DFT stages are (encode_sparse_dft_stage, ckks_bootstrap_dft.cpp:
+// 789-917: pre-rotated by the giant offset, Q u P basis, Montgomery form when
+// the context's keys are): the QP evaluator for arbitrary matrices.
+CKKSLinearTransformPlan ckks_encode_linear_transform_plan(const CKKSContext& context,
+                                                          const std::map<size_t, MyVector<Complex>>& diagonals,
+                                                          size_t level, size_t baby_step_count, double plaintext_scale,
+                                                          bool use_qp_evaluator, bool rescale_after = true);
 CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& context, const CKKSCiphertext& ct,
                                                           const CKKSLinearTransformOwnedStage& stage);
 CKKSCiphertext ckks_apply_linear_transform_owned_stage_q(const CKKSContext& context, const CKKSCiphertext& ct,
@@ -551,4 +560,15 @@ uint64_t conjugation_automorphism_index(size_t N);
 
 // Throws std::runtime_error (MI_EHIP / MI_ERUNTIME) or std::invalid_argument
 // (MI_EINVAL) with mi_last_error() for a failed C-ABI call.
+// ---- byte formats (ckks_serialize.cpp).  The reference has no wire or disk
+// format; a blob records the context fingerprint (N and the moduli) and a
+// checksum, and loads only into a context with the same ring and chain.
+std::vector<uint8_t> ckks_serialize_ciphertext(const CKKSCiphertext& ct);
+CKKSCiphertext ckks_deserialize_ciphertext(const CKKSContext& context, std::span<const uint8_t> bytes);
+std::vector<uint8_t> ckks_serialize_encoding(const CKKSEncoding& pt);
+CKKSEncoding ckks_deserialize_encoding(const CKKSContext& context, std::span<const uint8_t> bytes);
+// Relinearisation, conjugation, rotation and sparse-secret switching keys.
+std::vector<uint8_t> ckks_serialize_evaluation_keys(const CKKSContext& context);
+void ckks_load_evaluation_keys(CKKSContext& context, std::span<const uint8_t> bytes);
+
 void mi_check(int rc, const char* fn);

@@ -365,6 +365,22 @@ MyVector<uint64_t> CKKSLinearTransformOwnedStage::downloadDiagonalQp(const CKKSC
     return h;
 }
 
+CKKSLinearTransformPlan ckks_encode_linear_transform_plan(const CKKSContext& c,
+                                                          const std::map<size_t, MyVector<Complex>>& diagonals,
+                                                          size_t level, size_t baby_step_count, double plaintext_scale,
+                                                          bool use_qp_evaluator, bool rescale_after) {
+    CKKSBootstrapDftMatrixLiteral lit;
+    lit.type = CKKSBootstrapDftType::CoeffsToSlots;  // output stays slot-encoded
+    lit.level = level;
+    lit.baby_step_count = baby_step_count;
+    lit.plaintext_scale = plaintext_scale;
+    lit.rescale_after_each_stage = rescale_after;
+    CKKSLinearTransformPlan plan;
+    plan.use_qp_evaluator = use_qp_evaluator;
+    plan.stages.push_back(encode_stage(c, lit, diagonals, true));
+    return plan;
+}
+
 CKKSLinearTransformPlan ckks_generate_bootstrap_dft_plan(const CKKSContext& c, const CKKSBootstrapDftMatrixLiteral& lit) {
     const auto& P = c.getParams();
     const size_t slots = active_slot_count(c, lit);

@@ -473,6 +473,12 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
     mi_check(mi_context_create(&mi, N, q.data(), (int)nq, pm.data(), (int)pm.size(), (int)dnum),
              "ckks_context");
     try {
+        {
+            static_assert(sizeof(Complex) == 2 * sizeof(double), "complex<double> layout");
+            std::vector<uint64_t> perm(perm_table.begin(), perm_table.end());
+            mi_check(mi_set_encoding_tables(mi, reinterpret_cast<const double*>(twiddle_fft.data()), perm.data()),
+                     "ckks_context");
+        }
         Rng rng(params.getSeed());
         secret_coeffs = ckks_host::hamming_weight_ternary(N, params.getSecretKeyHammingWeight(), rng);
         secret_qp = secret_qp_from_coeffs(secret_coeffs);
@@ -588,8 +594,7 @@ DeviceBuffer CKKSContext::generate_q0p0_key(const uint64_t* src_qp, const uint64
     DeviceBuffer key(mi, 4 * N), tmp(mi, 2 * N);
     uint64_t* bq = key.data();
     uint64_t* bp = bq + N;
-    uint64_t* aq = bp + N;
-    uint64_t* ap = aq + N;
+    uint64_t* aq = bp + N;  // a(p0) follows at aq + N
     const uint64_t* sq_src = src_qp;
     const uint64_t* sq_tgt = tgt_qp;
     const uint64_t* sp_tgt = tgt_qp + nq * N;

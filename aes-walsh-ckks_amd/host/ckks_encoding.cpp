@@ -1,10 +1,13 @@
 // Encode / encrypt / decrypt (ckks_encoding.cpp:75-580, fft.cpp:82-117).
 //
-// The floating-point parts stay on the host with the reference's operation
-// order (special FFT with merged twists, llrint of value * scale / N, centred
-// CRT of the first limbs whose product fits 128 bits, division by the scale in
-// long double); built with -ffp-contract=off so no product is fused.  The ring
-// parts run on the device: NTTs, the mask product a*s and the sums.
+// Slot encoding runs on the device (mi_encode_slots: the special FFT in double
+// with the reference's operation order, llrint of value * scale / N, signed
+// residues, NTT).  Decoding's floating-point parts stay on the host (centred
+// CRT of the first limbs whose product fits 128 bits, division by the scale
+// in long double, the inverse FFT), as does encode_slots_custom_basis for
+// callers that want host residues; built with -ffp-contract=off so no
+// product is fused.  The ring parts run on the device: NTTs, the mask product
+// a*s and the sums.
 #include <bit>
 #include <cmath>
 
@@ -95,15 +98,21 @@ MyVector<uint64_t> encode_slots_custom_basis(size_t N, const MyVector<Complex>& 
     return out;
 }
 
+// ckks_encode (ckks_encoding.cpp:332-366): the encoding itself runs on the
+// device (mi_encode_slots: placement, FFT, rounding, residues, NTT).
 CKKSEncoding ckks_encode(const CKKSContext& c, const MyVector<Complex>& values, double scale, size_t level,
                          bool eval) {
     if (!(scale > 0.0)) throw std::invalid_argument("ckks_encode: scale must be positive");
     check_level(c, level, "ckks_encode");
-    const auto& q = c.getParams().getModuli();
-    MyVector<uint64_t> active(q.begin(), q.begin() + level + 1);
-    auto h = encode_slots_custom_basis(c.getParams().getN(), values, scale, active, c.getTwiddleFft(),
-                                       c.getPermTable());
-    return CKKSEncoding(c, upload_plaintext(c, h, level + 1, eval), scale, level, false);
+    const size_t N = c.getParams().getN();
+    if (values.size() != N / 2)
+        throw std::invalid_argument("encode_slots_custom_basis: expected exactly N/2 slot values");
+    const size_t words = (level + 1) * N;
+    DeviceBuffer d(c.handle(), words);
+    mi_check(mi_encode_slots(c.handle(), reinterpret_cast<const double*>(values.data()), 1, scale, (int)(level + 1),
+                             eval ? 1 : 0, d.data(), (int64_t)words, c.stream()),
+             "ckks_encode");
+    return CKKSEncoding(c, std::move(d), scale, level, false);
 }
 
 CKKSEncoding ckks_encode(const CKKSContext& c, const MyVector<Complex>& values, size_t level, bool eval) {
@@ -162,19 +171,11 @@ CKKSCiphertext ckks_encrypt(const CKKSContext& c, const CKKSEncoding& encd, bool
         mi_check(mi_stream_synchronize(mi, c.stream()), "ckks_encrypt(ptxt)");
     }
     if (!eval) mi_check(mi_ntt_forward(mi, b, 1, (int)L, W, 0, c.stream()), "ckks_encrypt(ptxt)");
-    ckks_host::Rng rng(c.nextRandomnessSeed());
-    ckks_host::GaussianSampler gauss(P.getSigma());
-    const auto& q = P.getModuli();
-    MyVector<uint64_t> h(L * N);
-    for (size_t l = 0; l < L; ++l)
-        for (size_t i = 0; i < N; ++i) h[l * N + i] = rng.below(q[l]);
-    ct.setPoly(1, h);
-    for (size_t i = 0; i < N; ++i) {
-        const int64_t e = gauss.sample(rng);
-        for (size_t l = 0; l < L; ++l) h[l * N + i] = signed_to_residue(e, q[l]);
-    }
+    // a uniform (evaluation domain) and e Gaussian, drawn on the device
+    const uint64_t seed = c.nextRandomnessSeed();
+    mi_check(mi_sample_uniform(mi, a, (int)L, 0, seed, 0, c.stream()), "ckks_encrypt(ptxt)");
     DeviceBuffer tmp(mi, L * N);
-    mi_check(mi_memcpy(mi, tmp.data(), h.data(), h.size() * 8, MI_COPY_H2D, c.stream()), "ckks_encrypt(ptxt)");
+    mi_check(mi_sample_gaussian(mi, tmp.data(), (int)L, P.getSigma(), seed, 1, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_ntt_forward(mi, tmp.data(), 1, (int)L, W, 0, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_poly_op(mi, MI_OP_ADD, b, tmp.data(), (int)L, 0, 1, W, c.stream()), "ckks_encrypt(ptxt)");
     mi_check(mi_memcpy(mi, tmp.data(), a, L * N * 8, MI_COPY_D2D, c.stream()), "ckks_encrypt(ptxt)");

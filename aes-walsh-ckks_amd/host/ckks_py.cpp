@@ -302,7 +302,37 @@ PYBIND11_MODULE(ckks_py, m) {
                  return p.stages.at(i);
              },
              py::return_value_policy::reference_internal);
+    auto as_bytes = [](const std::vector<uint8_t>& v) {
+        return py::bytes(reinterpret_cast<const char*>(v.data()), v.size());
+    };
+    auto span_of = [](const py::bytes& b) {
+        std::string_view sv(b);
+        return std::span<const uint8_t>(reinterpret_cast<const uint8_t*>(sv.data()), sv.size());
+    };
+    m.def("ckks_serialize_ciphertext", [as_bytes](const CKKSCiphertext& ct) { return as_bytes(ckks_serialize_ciphertext(ct)); });
+    m.def("ckks_deserialize_ciphertext",
+          [span_of](const CKKSContext& c, const py::bytes& b) { return ckks_deserialize_ciphertext(c, span_of(b)); },
+          py::keep_alive<0, 1>());
+    m.def("ckks_serialize_encoding", [as_bytes](const CKKSEncoding& pt) { return as_bytes(ckks_serialize_encoding(pt)); });
+    m.def("ckks_deserialize_encoding",
+          [span_of](const CKKSContext& c, const py::bytes& b) { return ckks_deserialize_encoding(c, span_of(b)); },
+          py::keep_alive<0, 1>());
+    m.def("ckks_serialize_evaluation_keys",
+          [as_bytes](const CKKSContext& c) { return as_bytes(ckks_serialize_evaluation_keys(c)); });
+    m.def("ckks_load_evaluation_keys",
+          [span_of](CKKSContext& c, const py::bytes& b) { ckks_load_evaluation_keys(c, span_of(b)); });
     m.def("ckks_generate_bootstrap_dft_plan", &ckks_generate_bootstrap_dft_plan, py::keep_alive<0, 1>());
+    m.def("ckks_encode_linear_transform_plan",
+          [](const CKKSContext& c, const std::map<size_t, std::vector<std::complex<double>>>& diags, size_t level,
+             size_t baby_step_count, double plaintext_scale, bool use_qp, bool rescale_after) {
+              std::map<size_t, MyVector<Complex>> m;
+              for (const auto& [i, v] : diags) m[i] = MyVector<Complex>(v.begin(), v.end());
+              return ckks_encode_linear_transform_plan(c, m, level, baby_step_count, plaintext_scale, use_qp,
+                                                       rescale_after);
+          },
+          py::arg("context"), py::arg("diagonals"), py::arg("level"), py::arg("baby_step_count"),
+          py::arg("plaintext_scale"), py::arg("use_qp_evaluator") = true, py::arg("rescale_after") = true,
+          py::keep_alive<0, 1>());
     m.def("ckks_apply_linear_transform_plan",
           [](const CKKSContext& c, const CKKSCiphertext& ct, const CKKSLinearTransformPlan& plan) {
               return ckks_apply_linear_transform_plan(c, ct.clone(), plan);

@@ -65,6 +65,10 @@ _SIGS = {
                                      cint, vp, vp, vp]),
     "mi_linear_transform_q_batch": (cint, [vp, cint, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint,
                                            vp, vp, cint, vp, vp, vp]),
+    "mi_set_encoding_tables": (cint, [vp, vp, vp]),
+    "mi_encode_slots": (cint, [vp, vp, cint, ctypes.c_double, cint, cint, vp, i64, vp]),
+    "mi_sample_uniform": (cint, [vp, vp, cint, cint, u64, u64, vp]),
+    "mi_sample_gaussian": (cint, [vp, vp, cint, ctypes.c_double, u64, u64, vp]),
     "mi_modraise": (cint, [vp, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint, vp]),
 }
@@ -321,6 +325,13 @@ class Context:
             ctypes.addressof(dp), int(pt_montgomery), int(baby_steps), int(slots), len(rots),
             ctypes.addressof(kr), ctypes.addressof(kp), int(montgomery), ctypes.addressof(oa),
             ctypes.addressof(ob), _stream(stream)))
+
+    # ---- sampling and encoding
+    def sample_uniform(self, out, limbs, seed, counter=0, basis0=0, stream=None):
+        check(lib().mi_sample_uniform(self.h, ptr(out), limbs, basis0, seed, counter, _stream(stream)))
+
+    def sample_gaussian(self, out, limbs, sigma, seed, counter=0, stream=None):
+        check(lib().mi_sample_gaussian(self.h, ptr(out), limbs, float(sigma), seed, counter, _stream(stream)))
 
     # ---- bootstrapping pieces
     def modraise(self, target_level, poly_q0, out, stream=None):

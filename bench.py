@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -335,12 +335,103 @@ def cpu_baseline_lt(N, q, p, level, diag_idx, bs, rots, parts):
             "kind": "port", "sample": "oracle BSGS stage on one ciphertext (same shape)"}
 
 
+def bench_chain(args, M, torch):
+    """The plain CKKS primitive chain through the C++ host API: N=2^14, 8 chain
+    limbs of 50-bit primes, P = 2 x 50-bit, dnum 4, Hamming weight 192,
+    sigma 3.2; 2^13 complex slots uniform in [-1, 1].  One step: encrypt two
+    ciphertexts at level 7, HMult, relinearise, rescale, rotate by 1,
+    decrypt (host encode/decode included)."""
+    sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
+    import ckks_py as H
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 14
+    c.mult_depth = 7
+    c.q0_bit = 50
+    c.q_bit = 50
+    c.p_bits = [50, 50]
+    c.dnum = 4
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [1]
+    c.seed = 11
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    N, lv = 1 << 14, 7
+    rng = np.random.default_rng(2024)
+    m1 = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    m2 = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    want = np.roll(m1 * m2, -1)
+
+    def evaluate(ct1, ct2):
+        H.ckks_mul_inplace(ctx, ct1, ct2)
+        H.ckks_relin_hybrid_inplace(ctx, ct1)
+        H.ckks_rescale(ctx, ct1)
+        H.ckks_rotate_inplace(ctx, ct1, 1)
+
+    def step():
+        ct1 = H.ckks_encrypt(ctx, m1, lv)
+        ct2 = H.ckks_encrypt(ctx, m2, lv)
+        evaluate(ct1, ct2)
+        return H.ckks_decrypt(ctx, ct1)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dec = step()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    err = float(np.max(np.abs(dec - want)))
+    # the evaluation part alone (device-resident inputs)
+    cts = [(H.ckks_encrypt(ctx, m1, lv), H.ckks_encrypt(ctx, m2, lv)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in cts:
+        evaluate(a, b)
+    torch.cuda.synchronize()
+    eval_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    cpu = cpu_baseline_chain(ctx, N) if args.cpu_baseline else None
+    return {
+        "metric": "CKKS primitive chain (encrypt x2, HMult, relinearise, rescale, rotate, decrypt), N=2^14",
+        "value": 1e3 / ms, "unit": "chain/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (exact mod-q integer); f64 encode/decode", "data": "synthetic (seeded slots)",
+        "config": {"workload": "one chain per step, N=2^14, 8 + 2 limbs of 50 bits, dnum 4",
+                   "evaluation_only_ms": eval_ms, "max_abs_error_vs_slot_product": err},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_chain(ctx, N):
+    """The oracle's tensor, relinearisation, rescale and rotation at the same
+    shape with the context's keys, repeated for ~10 s (evaluation part)."""
+    O = _oracle()
+    P = ctx.getParams()
+    q, p = np.array(P.getModuli(), np.uint64), np.array(P.getPModuli(), np.uint64)
+    octx = O.Ctx(N, q, p, 4)
+    rng = np.random.default_rng(3)
+    lv = 7
+    rnd = lambda: np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in q[: lv + 1]])
+    relin, rot1 = ctx.getRelinKey(), ctx.getRotationKey(1)
+    b1, a1, b2, a2 = rnd(), rnd(), rnd(), rnd()
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        ob, oa, oc2 = octx.tensor(lv, b1, a1, b2, a2)
+        ob, oa = octx.relin(lv, ob, oa, oc2, relin)
+        r = octx.rescale(lv, np.stack([ob, oa]))
+        octx.rotate(lv - 1, r[0], r[1], 1, rot1)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "chain/s (evaluation part)", "cores": int(O.lib().oc_omp_threads()),
+            "kind": "port", "sample": "oracle HMult + relinearise + rescale + rotate at N=2^14, repeated for 10 s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -354,7 +445,8 @@ def main():
     M.lib()
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
-    out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt}[args.workload](args, M, torch)
+    out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt,
+           "chain": bench_chain}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 

@@ -205,6 +205,34 @@ int mi_keyswitch_qp(mi_context* ctx, int level, const uint64_t* key_term, const 
  * limbs of qp hold the Q result, the P limbs are overwritten. */
 int mi_moddown_qp(mi_context* ctx, int level, uint64_t* qp, void* stream);
 
+/* ---- slot encoding (encode_slots_custom_basis, ckks_encoding.cpp:368-410;
+ * fft_dif, fft.cpp:75-95) ----------------------------------------------------
+ * mi_set_encoding_tables: the context's FFT twiddles (N complex doubles as
+ * re, im pairs: the conjugates of twiddle_ifft_generation, fft.cpp:37-73,
+ * ckks_context.cpp:610-614) and its slot
+ * permutation (N/2 entries; perm_table_generation, ckks_context.cpp:56-68).
+ * mi_encode_slots: count vectors of N/2 complex doubles (host, re/im pairs)
+ * -> count plaintexts of `limbs` limbs (q_0..), poly b at out + b*poly_stride;
+ * eval != 0 also applies the forward NTT.  The FFT runs in double with the
+ * reference's operation order, so the rounded coefficients equal its own. */
+int mi_set_encoding_tables(mi_context* ctx, const double* twiddles, const uint64_t* perm);
+int mi_encode_slots(mi_context* ctx, const double* values, int count, double scale, int limbs, int eval,
+                    uint64_t* out, int64_t poly_stride, void* stream);
+
+/* ---- sampling (ckks_uniform_mask_polys / ckks_gaussian_noise_polys,
+ * ckks_encoding.hpp:11-15; the reference draws from std::random_device-seeded
+ * mt19937_64, ckks_encoding.cpp:211-223) -------------------------------------
+ * Counter-based (Philox4x32-10): the output is a function of (seed, counter)
+ * only.  mi_sample_uniform: limbs x N residues uniform in [0, q_b) for basis
+ * indices b = basis0.. .  mi_sample_gaussian: one discrete Gaussian integer
+ * per coefficient (cumulative table over |x| <= ceil(12.0061 sigma)), written
+ * as residues of limbs q_0..q_{limbs-1} (coefficient domain).  Not a
+ * cryptographic generator, like the reference's. */
+int mi_sample_uniform(mi_context* ctx, uint64_t* out, int limbs, int basis0, uint64_t seed, uint64_t counter,
+                      void* stream);
+int mi_sample_gaussian(mi_context* ctx, uint64_t* out, int limbs, double sigma, uint64_t seed, uint64_t counter,
+                       void* stream);
+
 /* ---- BSGS linear-transform stage, QP evaluator ---------------------------
  * ckks_apply_linear_transform_owned_stage_qp (ckks_linear_transform.cpp:
  * 1174-1407) without its final rescale: a hoisted baby step per needed

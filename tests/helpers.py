@@ -106,3 +106,27 @@ def host():
         _host = mod
     return _host
 
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 (Salmon et al., SC 2011) on Python ints: ctr = 4 words,
+    key = 2 words; used to check the device samplers."""
+    c0, c1, c2, c3 = ctr
+    k0, k1 = key
+    M = 0xFFFFFFFF
+    for r in range(10):
+        if r:
+            k0 = (k0 + 0x9E3779B9) & M
+            k1 = (k1 + 0xBB67AE85) & M
+        p0 = 0xD2511F53 * c0
+        p1 = 0xCD9E8D57 * c2
+        c0, c1, c2, c3 = ((p1 >> 32) ^ c1 ^ k0) & M, p1 & M, ((p0 >> 32) ^ c3 ^ k1) & M, p0 & M
+    return c0, c1, c2, c3
+
+
+def sampler_draw(seed, counter, limb, index):
+    """The 128 random bits the device samplers use for (seed, counter, limb, index)."""
+    v = philox4x32_10((index, limb, counter & 0xFFFFFFFF, counter >> 32), (seed & 0xFFFFFFFF, seed >> 32))
+    lo = (v[1] << 32) | v[0]
+    hi = (v[3] << 32) | v[2]
+    return lo, hi

@@ -245,3 +245,52 @@ def test_invalid_arguments_raise():
         ctx.keyswitch(4, x, x, x, x, 0, 1)
     with pytest.raises(M.MiError, match="must differ"):
         ctx.automorphism(4, 5, x, x)
+
+
+def test_device_samplers_match_philox_and_their_distributions():
+    """mi_sample_uniform / mi_sample_gaussian equal the Philox4x32-10 draws
+    recomputed on the host (uniform: floor(r q / 2^128) of the 128-bit draw;
+    Gaussian: inversion of the cumulative table), are reproducible for one
+    (seed, counter) and differ across counters; at N = 2^16 the Gaussian has
+    mean ~0 and variance ~sigma^2 and the uniform limbs mean ~q/2."""
+    import math
+    from helpers import sampler_draw
+    N = 1 << 12
+    M, ctx, octx, q, p, rng = _random_setup(N, 3, 1, 1, seed=17)
+    out = M.to_dev(np.zeros(3 * N, np.uint64))
+    ctx.sample_uniform(out, 3, seed=0x1234567890ABCDEF, counter=5)
+    got = M.from_dev(out)
+    for l in range(3):
+        for i in range(0, N, 97):
+            lo, hi = sampler_draw(0x1234567890ABCDEF, 5, l, i)
+            assert int(got[l * N + i]) == ((hi << 64) + lo) * int(q[l]) >> 128, (l, i)
+    again = M.to_dev(np.zeros(3 * N, np.uint64))
+    ctx.sample_uniform(again, 3, seed=0x1234567890ABCDEF, counter=5)
+    assert np.array_equal(M.from_dev(again), got)
+    ctx.sample_uniform(again, 3, seed=0x1234567890ABCDEF, counter=6)
+    assert not np.array_equal(M.from_dev(again), got)
+    sigma = 3.2
+    bound = math.ceil(12.00610553538285 * sigma)
+    w = np.cumsum([math.exp(-(x * x) / (2 * sigma * sigma)) for x in range(-bound, bound + 1)])
+    cdf = w / w[-1]
+    ctx.sample_gaussian(out, 3, sigma, seed=99, counter=1)
+    g = M.from_dev(out)
+    for i in range(0, N, 31):
+        lo, _ = sampler_draw(99, 1, 0xFFFFFFFF, i)
+        u = (lo >> 11) * 2.0 ** -53
+        e = int(np.searchsorted(cdf, u, side="left")) - bound
+        for l in range(3):
+            assert int(g[l * N + i]) == e % int(q[l]), (i, l)
+    # distributions at N = 2^16
+    N2 = 1 << 16
+    M2, ctx2, _, q2, _, _ = _random_setup(N2, 2, 1, 1, seed=19)
+    big = M2.to_dev(np.zeros(2 * N2, np.uint64))
+    ctx2.sample_gaussian(big, 2, sigma, seed=7, counter=3)
+    v = M2.from_dev(big)[:N2].astype(np.int64)
+    v = np.where(v > int(q2[0]) // 2, v - int(q2[0]), v)
+    assert abs(v.mean()) < 0.1 and abs(v.var() / sigma ** 2 - 1) < 0.05
+    ctx2.sample_uniform(big, 2, seed=7, counter=4)
+    u = M2.from_dev(big)
+    for l in range(2):
+        m = u[l * N2:(l + 1) * N2].astype(np.float64).mean() / float(q2[l])
+        assert abs(m - 0.5) < 0.01

@@ -480,6 +480,47 @@ def test_host_linear_transform_bsgs_matches_reference_lq10():
         H.ckks_apply_linear_transform_bsgs(ctx, ct, [diags[0]], o["pt_scale"], False, [12], 3)
 
 
+def test_host_bsgs_stage_error_vs_fp64_matvec_at_full_size():
+    """The BSGS linear-transform workload at full size: N=2^16, 2^15 slots,
+    level 24 of Q = 30 x 50-bit, P = 10 x 50-bit, dnum 3; 128 diagonals
+    {16 g + b : g < 8, b < 16} with real and imaginary parts uniform in
+    [-1, 1], encoded at scale 2^40.  Through both evaluators the decrypted
+    result matches the FP64 matrix-vector product y_i = sum_d diag_d[i] x[i+d]
+    within 1e-6 (the stated tolerance; the observed error is printed)."""
+    H = host()
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 16
+    c.mult_depth = 29
+    c.q0_bit = 50
+    c.q_bit = 50
+    c.p_bits = [50] * 10
+    c.dnum = 3
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = list(range(1, 16)) + [16 * g for g in range(1, 8)]
+    c.seed = 7
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    n = 1 << 15
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    diags = {16 * g + b: rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n) for g in range(8) for b in range(16)}
+    want = sum(v * np.roll(x, -d) for d, v in diags.items())
+    ct = H.ckks_encrypt(ctx, x, 24)
+    outs = []
+    for qp in (True, False):
+        plan = H.ckks_encode_linear_transform_plan(ctx, diags, 24, 16, 2.0 ** 40, qp)
+        out = H.ckks_apply_linear_transform_plan(ctx, ct, plan)
+        assert out.getLevel() == 23
+        dec = H.ckks_decrypt(ctx, out)
+        err = float(np.max(np.abs(dec - want)))
+        print(f"BSGS stage N=2^16 level 24, 128 diagonals, {'QP' if qp else 'Q'} evaluator: "
+              f"max |error| vs FP64 mat-vec = {err:.3e}")
+        assert err < 1e-6, (qp, err)
+        outs.append(dec)
+    assert np.max(np.abs(outs[0] - outs[1])) < 1e-6
+
+
 def _bits_bootstrap_context(H):
     """N=2^10 bootstrapping chain: q0 41 bits (one bit above the 2^40 scale, the
     relation the EvalMod C2S scaling assumes: bit b sits at b * q0 / 2 after
@@ -534,3 +575,78 @@ def test_host_bootstrap_refreshes_encrypted_bits():
     d1 = H.ckks_decrypt(ctx, ct1)
     assert ct0.getLevel() > 0 and ct0.getScale() == P.getScale()
     assert np.max(np.abs(d0.real - b0)) < 0.05 and np.max(np.abs(d1.real - b1)) < 0.05
+
+
+def test_host_serialization_round_trips_and_rejects_corruption():
+    """Ciphertexts, plaintexts and evaluation keys survive a byte round trip
+    bit for bit (keys loaded into a second context with another seed switch
+    exactly as they do in the first); blobs from another chain, truncated or
+    altered blobs are rejected."""
+    H = host()
+    c = _config1(H, seed=21)
+    c.N = 1 << 12
+    c.rotate_values = [1, 3]
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    rng = np.random.default_rng(5)
+    m = rng.uniform(-1, 1, 1 << 11) + 1j * rng.uniform(-1, 1, 1 << 11)
+    ct = H.ckks_encrypt(ctx, m, 6)
+    H.ckks_mul_inplace(ctx, ct, H.ckks_encrypt(ctx, m, 6))  # three polys
+    blob = H.ckks_serialize_ciphertext(ct)
+    back = H.ckks_deserialize_ciphertext(ctx, blob)
+    assert back.getNumPolys() == 3 and back.getLevel() == 6 and back.getScale() == ct.getScale()
+    for k in range(3):
+        assert np.array_equal(back.getPoly(k), ct.getPoly(k))
+    pt = H.ckks_encode(ctx, m, 4, eval=True)
+    pback = H.ckks_deserialize_encoding(ctx, H.ckks_serialize_encoding(pt))
+    assert np.array_equal(pback.getPlaintext(), pt.getPlaintext()) and pback.getLevel() == 4
+    # keys: a second context (other seed, so other keys) given the first
+    # context's keys switches exactly as the first one does
+    keys = H.ckks_serialize_evaluation_keys(ctx)
+    c2 = _config1(H, seed=22)
+    c2.N = 1 << 12
+    c2.rotate_values = [1, 3]
+    ctx2 = H.CKKSContext(H.CKKSParams(c2))
+    H.ckks_load_evaluation_keys(ctx2, keys)
+    x1 = H.ckks_encrypt(ctx, m, 5)
+    x2 = H.ckks_deserialize_ciphertext(ctx2, H.ckks_serialize_ciphertext(x1))
+    H.ckks_rotate_inplace(ctx, x1, 3)
+    H.ckks_rotate_inplace(ctx2, x2, 3)
+    assert np.array_equal(x1.getPoly(0), x2.getPoly(0)) and np.array_equal(x1.getPoly(1), x2.getPoly(1))
+    with pytest.raises(ValueError, match="checksum"):
+        H.ckks_deserialize_ciphertext(ctx, blob[:100] + bytes([blob[100] ^ 1]) + blob[101:])
+    with pytest.raises(ValueError, match="checksum|truncated"):
+        H.ckks_deserialize_ciphertext(ctx, blob[: len(blob) // 2])
+    other = _config1(H, seed=21)
+    other.N = 1 << 12
+    other.q_bit = 45
+    ctx3 = H.CKKSContext(H.CKKSParams(other))
+    with pytest.raises(ValueError, match="different ring or modulus chain"):
+        H.ckks_deserialize_ciphertext(ctx3, blob)
+    with pytest.raises(ValueError, match="wrong magic"):
+        H.ckks_deserialize_encoding(ctx, blob)
+
+
+@pytest.mark.parametrize("logn,mult_depth", [(10, 3), (13, 6), (16, 20)])
+def test_host_device_encode_matches_host_restatement(logn, mult_depth):
+    """ckks_encode runs the special FFT, rounding and residues on the device;
+    its limbs equal the host restatement of encode_slots_custom_basis (which
+    the reference's golden encodings pin) for small and large magnitudes and
+    two scales."""
+    H = host()
+    c = H.CKKSParamsConfig()
+    c.N = 1 << logn
+    c.mult_depth = mult_depth
+    c.q0_bit = 55
+    c.q_bit = 45
+    c.p_bits = [60] * (6 if mult_depth > 10 else 3)
+    c.dnum = 3
+    c.secure = False
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    q = list(ctx.getParams().getModuli())
+    rng = np.random.default_rng(logn)
+    n = 1 << (logn - 1)
+    for mag, scale, level in [(1.0, 2.0 ** 45, mult_depth), (1e3, 2.0 ** 30, 1), (1e-6, 2.0 ** 55, 0)]:
+        v = mag * (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n))
+        pt = H.ckks_encode(ctx, v, level, scale=scale, eval=False)
+        want = H.encode_slots_custom_basis(ctx, v, scale, q[: level + 1])
+        assert np.array_equal(pt.getPlaintext(), want), (mag, scale, level)
