@@ -105,6 +105,11 @@ __device__ __forceinline__ u64 redc128(u64 tl, u64 th, u64 q, u64 qninv) {
     u64 u = th + mq_hi + (tl != 0 ? 1 : 0);
     return u >= q ? u - q : u;
 }
+// REDC of any 128-bit t: the high word is brought below q first (t keeps its
+// residue), so the REDC precondition t < q*2^64 holds.
+__device__ __forceinline__ u64 redc128_any(u64 tl, u64 th, u64 q, u64 qninv, u64 mu_hi) {
+    return redc128(tl, reduce64(th, q, mu_hi), q, qninv);
+}
 __device__ __forceinline__ u64 mont_mul(u64 normal, u64 mont, u64 q, u64 qninv) {
     return redc128(normal * mont, __umul64hi(normal, mont), q, qninv);
 }

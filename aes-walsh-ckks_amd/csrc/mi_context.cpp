@@ -200,7 +200,8 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
                     u64 v = 1;
                     for (int j = 0; j < ns; ++j)
                         if (j != i) v = mulm(v, q[s0 + j] % dst[t], dst[t]);
-                    qmd[(size_t)t * ns + i] = v;
+                    // Montgomery form (x 2^64): the kernel's sum ends in one REDC
+                    qmd[(size_t)t * ns + i] = (u64)(((unsigned __int128)v << 64) % dst[t]);
                 }
             }
             for (int t = 0; t < nd; ++t) {
@@ -224,7 +225,8 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
         const int L = level + 1;
         std::vector<u64> pinv((size_t)L * np);
         for (int i = 0; i < L; ++i)
-            for (int j = 0; j < np; ++j) pinv[(size_t)i * np + j] = invm(p[j] % q[i], q[i]);
+            for (int j = 0; j < np; ++j)  // Montgomery form, as above
+                pinv[(size_t)i * np + j] = (u64)(((unsigned __int128)invm(p[j] % q[i], q[i]) << 64) % q[i]);
         md_off[level] = blob.add(pinv.data(), 8 * pinv.size());
     }
     struct RS { size_t ip, ips, hm; };

@@ -31,14 +31,14 @@ struct ModUpTables {
     const u64* qhat_inv;        // [ns]
     const u64* qhat_inv_shoup;  // [ns]
     const double* src_inv;      // [ns]
-    const u64* qhat_mod_dst;    // [nd][ns]
+    const u64* qhat_mod_dst;    // [nd][ns], Montgomery form (x 2^64 mod m_dst)
     const u64* alpha_corr;      // [nd][ns+1]
     const int* dst_basis;       // [nd]
 };
 
 // Folded ModDown (ckks_keyswitch.cpp:521-607) tables at one level.
 struct ModDownTables {
-    const u64* pinv;            // [(level+1)][np] p_j^-1 mod q_i
+    const u64* pinv;            // [(level+1)][np] p_j^-1 mod q_i, Montgomery form
 };
 
 struct RescaleTables {

@@ -87,7 +87,7 @@ template <int NS>
 __global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
     __shared__ u64 s_row[MODUP_DCH][NS];       // qhat_i mod m_j
     __shared__ u64 s_corr[MODUP_DCH][NS + 1];  // alpha * Q_digit mod m_j
-    __shared__ u64 s_dq[MODUP_DCH], s_mulo[MODUP_DCH], s_muhi[MODUP_DCH];
+    __shared__ u64 s_dq[MODUP_DCH], s_ninv[MODUP_DCH], s_muhi[MODUP_DCH];
     __shared__ int s_slot[MODUP_DCH];
     __shared__ u64 s_sq[NS], s_qinv[NS], s_qinvs[NS];
     __shared__ double s_sinv[NS];
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
         const int j = j0 + threadIdx.x;
         const BasisConst b = a.bc[t.dst_basis[j]];
         s_dq[threadIdx.x] = b.q;
-        s_mulo[threadIdx.x] = b.mu_lo;
+        s_ninv[threadIdx.x] = b.qninv;
         s_muhi[threadIdx.x] = b.mu_hi;
         s_slot[threadIdx.x] =
             j < t.s0 ? j : (j < t.s0 + (a.L - t.s1) ? t.s1 + (j - t.s0) : a.L + (j - (a.L - NS)));
@@ -140,12 +140,13 @@ __global__ void __launch_bounds__(KS_THREADS) k_modup(ModUpArgs a) {
     u64 al = (u64)__dadd_rn(est, 0.5);
     if (al > (u64)NS) al = (u64)NS;
     for (int jj = 0; jj < nj; ++jj) {
-        // sum of NS products < 16 * 2^124 fits in 128 bits
+        // sum of NS products < 16 * 2^124 fits in 128 bits; the rows are in
+        // Montgomery form, so one REDC gives sum y_i qhat_i mod m
         U128 acc{0, 0};
 #pragma unroll
         for (int i = 0; i < NS; ++i) mac128(acc, y[i], s_row[jj][i]);
         const u64 q = s_dq[jj];
-        u64 v = reduce128(acc.lo, acc.hi, q, s_mulo[jj], s_muhi[jj]);
+        u64 v = redc128_any(acc.lo, acc.hi, q, s_ninv[jj], s_muhi[jj]);
         v = sub_mod(v, s_corr[jj][al], q);
         D[(long long)s_slot[jj] * N + c] = v;
     }
@@ -276,32 +277,23 @@ __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
     const long long src = a.galois ? (long long)galois_src((u32)i, a.galois, a.logN) : i;
     const u64* digits = a.digits + it * a.digits_stride;
     const u64* key = a.key[it];
-    // Lazy 128-bit sums of the digit products, brought below q every three
-    // products (3 q^2 + q < q 2^64 for q < 2^62) and reduced once at the end:
-    // REDC for Montgomery-form keys, Barrett otherwise.  The result is the
-    // canonical residue of the per-product sum.
+    // Exact 128-bit sums of the digit products: at most 16 digits of
+    // residues below q < 2^62 keep the sum below 2^128.  One reduction at the
+    // end (REDC for Montgomery-form keys, Barrett otherwise) gives the
+    // canonical residue of the sum.
     U128 sb{0, 0}, sa{0, 0};
     for (int d = 0; d < a.parts; ++d) {
-        u64 x = digits[((long long)d * slots + slot) * N + src];
+        const u64 x = reduce_4q(digits[((long long)d * slots + slot) * N + src], b.q);
         const u64* kb = key + ((long long)d * 2 * key_limbs + basis) * N;
         const u64* ka = kb + (long long)key_limbs * N;
-        u64 kbv = kb[src], kav = ka[src];
-        if (!a.montgomery) {
-            x = reduce_4q(x, b.q);
-            kbv = reduce_4q(kbv, b.q);
-            kav = reduce_4q(kav, b.q);
-        }
+        const u64 kbv = reduce_4q(kb[src], b.q), kav = reduce_4q(ka[src], b.q);
         mac128(sb, x, kbv);
         mac128(sa, x, kav);
-        if (d % 3 == 2 && d + 1 < a.parts) {
-            sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
-            sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
-        }
     }
     u64 acc_b, acc_a;
     if (a.montgomery) {
-        acc_b = redc128(sb.lo, sb.hi, b.q, b.qninv);
-        acc_a = redc128(sa.lo, sa.hi, b.q, b.qninv);
+        acc_b = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
+        acc_a = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
     } else {
         acc_b = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
         acc_a = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
@@ -330,6 +322,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
 
 void ks_mac(const Context& c, const MacArgs& args, int level, hipStream_t s) {
     if (args.items < 1 || args.items > KS_MAX_ITEMS) throw Error(-1, "key product batch size out of range");
+    if (args.parts < 1 || args.parts > 16) throw Error(-1, "too many digits");  // 128-bit sums hold 16 products
     const int tps = (int)(c.N / KS_THREADS);
     const unsigned blocks = tile_grid(args.items, tps, level + 1 + c.np);
     hipLaunchKernelGGL(k_mac, dim3(blocks), dim3(KS_THREADS), 0, s, args);
@@ -365,7 +358,7 @@ struct MdContribArgs {
 template <int K>
 __global__ void __launch_bounds__(KS_THREADS) k_moddown_contrib(MdContribArgs a) {
     __shared__ u64 s_pinv[MODDOWN_TCH][K];
-    __shared__ u64 s_tq[MODDOWN_TCH], s_mulo[MODDOWN_TCH], s_muhi[MODDOWN_TCH];
+    __shared__ u64 s_tq[MODDOWN_TCH], s_ninv[MODDOWN_TCH], s_muhi[MODDOWN_TCH];
     __shared__ u64 s_pq[K];
     __shared__ double s_pinvd[K];
     const u64 N = (u64)1 << a.logN;
@@ -378,7 +371,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_moddown_contrib(MdContribArgs a)
     if (threadIdx.x < nt) {
         const BasisConst b = a.bc[t0 + threadIdx.x];
         s_tq[threadIdx.x] = b.q;
-        s_mulo[threadIdx.x] = b.mu_lo;
+        s_ninv[threadIdx.x] = b.qninv;
         s_muhi[threadIdx.x] = b.mu_hi;
     }
     if (threadIdx.x >= 64 && threadIdx.x < 64 + K) {
@@ -402,9 +395,9 @@ __global__ void __launch_bounds__(KS_THREADS) k_moddown_contrib(MdContribArgs a)
     for (int tt = 0; tt < nt; ++tt) {
         U128 acc{0, 0};
 #pragma unroll
-        for (int j = 0; j < K; ++j) mac128(acc, x[j], s_pinv[tt][j]);
+        for (int j = 0; j < K; ++j) mac128(acc, x[j], s_pinv[tt][j]);  // Montgomery-form p_j^-1
         const u64 q = s_tq[tt];
-        const u64 v = reduce128(acc.lo, acc.hi, q, s_mulo[tt], s_muhi[tt]);
+        const u64 v = redc128_any(acc.lo, acc.hi, q, s_ninv[tt], s_muhi[tt]);
         a.tmp[((long long)poly * a.L + t0 + tt) * N + i] = sub_mod(v, al, q);
     }
 }

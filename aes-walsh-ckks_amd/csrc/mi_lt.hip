@@ -76,28 +76,24 @@ __global__ void __launch_bounds__(LT_THREADS) k_lt_inner_all(LtAllArgs a) {
         }
     }
     for (int g = 0; g < a.ng; ++g) {
-        // lazy 128-bit sums, brought below q every three products and reduced
-        // once (REDC for Montgomery-form plaintexts, Barrett otherwise)
+        // exact 128-bit sums: at most LT_NB = 16 products of residues below
+        // 2^62 (plaintexts brought below q first) stay below 2^128; one
+        // reduction at the end (REDC for Montgomery-form plaintexts, Barrett
+        // otherwise) gives the canonical residue of the sum
         U128 sa{0, 0}, sb{0, 0};
-        int pending = 0;
 #pragma unroll
         for (int r = 0; r < LT_NB; ++r) {
             const u64* dp = a.diag[g * LT_NB + r];
             if (r < a.nb && dp) {
-                const u64 p = dp[o];
+                const u64 p = reduce_4q(dp[o], b.q);
                 mac128(sa, ba[r], p);
                 mac128(sb, bb[r], p);
-                if (++pending == 3) {
-                    sa = U128{reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi), 0};
-                    sb = U128{reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi), 0};
-                    pending = 0;
-                }
             }
         }
         u64 ra, rb;
         if (a.pt_mont) {
-            ra = redc128(sa.lo, sa.hi, b.q, b.qninv);
-            rb = redc128(sb.lo, sb.hi, b.q, b.qninv);
+            ra = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
+            rb = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
         } else {
             ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
             rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);

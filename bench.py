@@ -277,6 +277,9 @@ def bench_lt(args, M, torch):
     ref = torch.cat(outs).clone()
     eager_ms = timed(step, args.steps)
     plain_ms = timed(step_plain, 1) if args.plain else None
+    if args.eager:
+        return _lt_result(args, eager_ms, eager_ms, plain_ms, None, nct, N, q, p, L, K, diag_idx, rots, parts, level,
+                          bs)
     # the batch captured once in a HIP graph and replayed
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph, stream=stream):
@@ -296,6 +299,10 @@ def bench_lt(args, M, torch):
     torch.cuda.synchronize()
     if not torch.equal(torch.cat(outs), ref):
         raise RuntimeError("graph replay differs from the eager batch")
+    return _lt_result(args, ms, eager_ms, plain_ms, replay_ok, nct, N, q, p, L, K, diag_idx, rots, parts, level, bs)
+
+
+def _lt_result(args, ms, eager_ms, plain_ms, replay_ok, nct, N, q, p, L, K, diag_idx, rots, parts, level, bs):
     T = L + K
     # per ciphertext: ct in/out, diagonals and keys amortised over the batch
     algo = nct * (2 * L * 8 * N * 2) + len(diag_idx) * T * 8 * N + len(rots) * parts * 2 * (len(q) + K) * 8 * N
@@ -308,7 +315,8 @@ def bench_lt(args, M, torch):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
         "config": {"workload": "32 ct x BSGS stage (16 baby x 8 giant), QP diagonals, 22 rotation keys, "
-                               "one batched call captured in a HIP graph",
+                               + ("one batched call (eager)" if replay_ok is None else
+                                  "one batched call captured in a HIP graph"),
                    "algorithmic_GBps": algo / (ms / 1e3) / 1e9, "eager_batched_ms": eager_ms,
                    "plain_one_ct_per_call_ms": plain_ms,
                    "batched_over_plain": (plain_ms / ms) if plain_ms else None,
@@ -435,6 +443,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
+    ap.add_argument("--eager", action="store_true",
+                    help="lt: time the eager batched call, no HIP graph (counter collection does not "
+                         "follow graph-launched kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
