@@ -108,6 +108,18 @@ void* Context::workspace(hipStream_t s, size_t bytes) {
     return w.ptr;
 }
 
+const std::vector<hipStream_t>& Context::side_streams() {
+    std::lock_guard<std::mutex> g(side_mu);
+    if (side.empty()) {
+        for (int i = 0; i < 4; ++i) {
+            hipStream_t st = nullptr;
+            MI_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            side.push_back(st);
+        }
+    }
+    return side;
+}
+
 struct Blob {
     std::vector<unsigned char> host;
     size_t add(const void* p, size_t bytes) {
@@ -368,6 +380,7 @@ void context_destroy(Context* c) {
     (void)hipFree(c->d_gauss_cdf);
     for (auto& kv : c->ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (hipStream_t st : c->side) (void)hipStreamDestroy(st);
     delete c;
 }
 

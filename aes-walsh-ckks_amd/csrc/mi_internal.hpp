@@ -82,6 +82,10 @@ struct Context {
     size_t table_bytes = 0;
     std::mutex ws_mu;
     std::map<hipStream_t, Workspace> ws;
+    // Side streams (default priority) that independent groups of a batch
+    // run on concurrently; created on first use, joined back with events.
+    std::mutex side_mu;
+    std::vector<hipStream_t> side;
     // Device memory pool for ciphertexts, plaintexts and keys: freed blocks
     // are cached by size and reused; byte counts feed admission decisions.
     std::mutex pool_mu;
@@ -97,6 +101,7 @@ struct Context {
         return pp < dnum ? pp : dnum;
     }
     void* workspace(hipStream_t s, size_t bytes);
+    const std::vector<hipStream_t>& side_streams();
     void* pool_alloc(size_t bytes);
     void pool_release(void* p);
     void pool_trim();

@@ -575,8 +575,36 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         std::vector<Span> writes, reads;
     };
     std::vector<Group> open;
+    // Open groups are independent of each other: one group runs on the
+    // caller's stream, the others on side streams forked from it and joined
+    // back, so small levels share the GPU instead of queueing.
     auto flush_all = [&]() {
-        for (auto& g : open) keyswitch_items(c, g.v.data(), (int)g.v.size(), montgomery, s);
+        if (open.size() == 1) {
+            keyswitch_items(c, open[0].v.data(), (int)open[0].v.size(), montgomery, s);
+        } else if (open.size() > 1) {
+            const auto& side = c.side_streams();
+            hipEvent_t fork;
+            MI_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            MI_HIP(hipEventRecord(fork, s));
+            std::vector<hipEvent_t> joins;
+            for (size_t g = 0; g < open.size(); ++g) {
+                hipStream_t st = g == 0 ? s : side[(g - 1) % side.size()];
+                if (st != s) MI_HIP(hipStreamWaitEvent(st, fork, 0));
+                keyswitch_items(c, open[g].v.data(), (int)open[g].v.size(), montgomery, st);
+                if (st != s && (g + side.size() >= open.size())) {
+                    // last group on this side stream: join it back
+                    hipEvent_t j;
+                    MI_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+                    MI_HIP(hipEventRecord(j, st));
+                    joins.push_back(j);
+                }
+            }
+            for (hipEvent_t j : joins) {
+                MI_HIP(hipStreamWaitEvent(s, j, 0));
+                MI_HIP(hipEventDestroy(j));
+            }
+            MI_HIP(hipEventDestroy(fork));
+        }
         open.clear();
     };
     for (int i = 0; i < n; ++i) {
