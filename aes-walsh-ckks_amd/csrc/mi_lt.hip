@@ -23,12 +23,14 @@ constexpr int LT_THREADS = 256;
 constexpr int LT_NB = 16;  // baby-step slots held in registers per launch (slot 0 = the input)
 constexpr int LT_NG = 16;  // giant steps per launch
 
-// Inner products of every giant step in one pass over the baby steps: each
-// thread holds its coefficient of up to 16 baby steps (a and b) in registers
-// and runs through the diagonals of up to 16 giant steps, so a baby step is
-// read once per ciphertext and a diagonal tile once per XCD for all
-// ciphertexts of the batch (map_tile places a tile's blocks for every item
-// consecutively on one XCD).
+// Inner products of every giant step.  One block computes one giant step's
+// sum for 256 coefficients of one limb of one ciphertext, streaming its (up
+// to 16) baby/diagonal pairs: few registers, and a block touches only the
+// memory of its own terms.  Block order (map_tile_g) puts the blocks of every
+// giant step and ciphertext for one coefficient tile consecutively on one
+// XCD, so the tile's baby steps (read once per giant step) and diagonals (read
+// once per ciphertext) are served from that XCD's L2: HBM sees each baby step
+// once per ciphertext and each diagonal once per batch.
 struct LtAllArgs {
     const u64* diag[LT_NG * LT_NB];     // [giant][baby slot], null: no term
     const u64* in_a[KS_MAX_ITEMS];      // baby slot 0 (Q limbs; its P limbs are zero)
@@ -38,7 +40,7 @@ struct LtAllArgs {
     long long baby_slot_stride;
     u64* out[LT_NG];                    // giant g of item i: a at out[g] + i*out_stride[g], b T*N later
     long long out_stride[LT_NG];
-    int slot0;                          // baby slot of this launch's first register (multiple of 16)
+    int slot0;                          // baby slot of this launch's first term (multiple of 16)
     int nb, ng, accumulate;
     int items, L, K, nq, logN, pt_mont;
     const BasisConst* bc;
@@ -48,61 +50,53 @@ __global__ void __launch_bounds__(LT_THREADS) k_lt_inner_all(LtAllArgs a) {
     const u64 N = (u64)1 << a.logN;
     const int T = a.L + a.K;
     const int tps = (int)(N / LT_THREADS);
-    const TileMap m = map_tile(blockIdx.x, a.items, tps, T);
-    if (!m.valid) return;
-    const int it = m.item, slot = m.slot;
-    const long long i = (long long)m.tile * LT_THREADS + threadIdx.x;
+    // block -> (giant, item, slot, tile): giant fastest, then item, per XCD
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int g = local % a.ng, rest = local / a.ng;
+    const int it = rest % a.items, tl = rest / a.items;
+    const int tile = tl * 8 + xcd;
+    const int slot = tile / tps;
+    if (slot >= T) return;
+    const long long i = (long long)(tile - slot * tps) * LT_THREADS + threadIdx.x;
     const int basis = slot < a.L ? slot : a.nq + (slot - a.L);
     const BasisConst b = a.bc[basis];
     const long long o = (long long)slot * N + i;
     const long long TN = (long long)T * N;
-    u64 ba[LT_NB], bb[LT_NB];
-#pragma unroll
-    for (int r = 0; r < LT_NB; ++r) {
-        ba[r] = 0;
-        bb[r] = 0;
-        if (r < a.nb) {
-            const int sl = a.slot0 + r;
-            if (sl == 0) {
-                if (slot < a.L) {
-                    ba[r] = a.in_a[it][o];
-                    bb[r] = a.in_b[it][o];
-                }
-            } else {
-                const u64* p = a.babies + it * a.baby_item_stride + (long long)(sl - 1) * a.baby_slot_stride + o;
-                ba[r] = p[0];
-                bb[r] = p[TN];
-            }
-        }
-    }
-    for (int g = 0; g < a.ng; ++g) {
-        // exact 128-bit sums: at most LT_NB = 16 products of residues below
-        // 2^62 (plaintexts brought below q first) stay below 2^128; one
-        // reduction at the end (REDC for Montgomery-form plaintexts, Barrett
-        // otherwise) gives the canonical residue of the sum
-        U128 sa{0, 0}, sb{0, 0};
-#pragma unroll
-        for (int r = 0; r < LT_NB; ++r) {
-            const u64* dp = a.diag[g * LT_NB + r];
-            if (r < a.nb && dp) {
-                const u64 p = reduce_4q(dp[o], b.q);
-                mac128(sa, ba[r], p);
-                mac128(sb, bb[r], p);
-            }
-        }
-        u64 ra, rb;
-        if (a.pt_mont) {
-            ra = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
-            rb = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
+    // exact 128-bit sums: at most LT_NB = 16 products of residues below
+    // 2^62 (plaintexts brought below q first) stay below 2^128; one reduction
+    // at the end (REDC for Montgomery-form plaintexts, Barrett otherwise)
+    // gives the canonical residue of the sum
+    U128 sa{0, 0}, sb{0, 0};
+    for (int r = 0; r < a.nb; ++r) {
+        const u64* dp = a.diag[g * LT_NB + r];
+        if (!dp) continue;
+        const int sl = a.slot0 + r;
+        u64 xa, xb;
+        if (sl == 0) {
+            if (slot >= a.L) continue;  // the input's P limbs are zero
+            xa = a.in_a[it][o];
+            xb = a.in_b[it][o];
         } else {
-            ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
-            rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+            const u64* p = a.babies + it * a.baby_item_stride + (long long)(sl - 1) * a.baby_slot_stride + o;
+            xa = p[0];
+            xb = p[TN];
         }
-        u64* pa = a.out[g] + it * a.out_stride[g] + o;
-        u64* pb = pa + TN;
-        *pa = a.accumulate ? add_mod(*pa, ra, b.q) : ra;
-        *pb = a.accumulate ? add_mod(*pb, rb, b.q) : rb;
+        const u64 d = reduce_4q(dp[o], b.q);
+        mac128(sa, xa, d);
+        mac128(sb, xb, d);
     }
+    u64 ra, rb;
+    if (a.pt_mont) {
+        ra = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
+        rb = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
+    } else {
+        ra = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
+        rb = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+    }
+    u64* pa = a.out[g] + it * a.out_stride[g] + o;
+    u64* pb = pa + TN;
+    *pa = a.accumulate ? add_mod(*pa, ra, b.q) : ra;
+    *pb = a.accumulate ? add_mod(*pb, rb, b.q) : rb;
 }
 
 static u64 powmod_u64(u64 a, u64 e, u64 m) {
@@ -245,7 +239,7 @@ void launch_inner(Context& c, const Compiled& k, const LtStage& st, int m, int K
             ia.logN = c.logN;
             ia.pt_mont = st.pt_montgomery ? 1 : 0;
             ia.bc = c.d_bc;
-            hipLaunchKernelGGL(k_lt_inner_all, dim3(tile_grid(m, tps, T)), dim3(LT_THREADS), 0, s, ia);
+            hipLaunchKernelGGL(k_lt_inner_all, dim3(tile_grid(m * ia.ng, tps, T)), dim3(LT_THREADS), 0, s, ia);
             MI_HIP(hipGetLastError());
         }
     }
