@@ -292,8 +292,15 @@ __global__ void __launch_bounds__(KS_THREADS) k_mac(MacArgs a) {
     }
     u64 acc_b, acc_a;
     if (a.montgomery) {
-        acc_b = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
-        acc_a = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
+        if (a.parts <= 4) {
+            // at most 4 products of residues below q: the sum is below 4 q^2 <
+            // q 2^64 (q < 2^62), REDC's precondition, without reducing the high word
+            acc_b = redc128(sb.lo, sb.hi, b.q, b.qninv);
+            acc_a = redc128(sa.lo, sa.hi, b.q, b.qninv);
+        } else {
+            acc_b = redc128_any(sb.lo, sb.hi, b.q, b.qninv, b.mu_hi);
+            acc_a = redc128_any(sa.lo, sa.hi, b.q, b.qninv, b.mu_hi);
+        }
     } else {
         acc_b = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
         acc_a = reduce128(sa.lo, sa.hi, b.q, b.mu_lo, b.mu_hi);
