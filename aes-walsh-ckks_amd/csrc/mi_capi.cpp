@@ -448,6 +448,23 @@ int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, vo
     });
 }
 
+int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
+                            const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
+                            uint64_t* const* a_out, int montgomery, void* stream) {
+    return guard("ckks_hybrid_key_switch_hoisted", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(digits); check_ptr(b);
+        if (count < 1 || !galois || !keys || !b_out || !a_out) throw mi::Error(MI_EINVAL, "invalid batch");
+        for (int i = 0; i < count; ++i) {
+            check_galois(galois[i], false);
+            check_ptr(keys[i]); check_ptr(b_out[i]); check_ptr(a_out[i]);
+        }
+        mi::rotate_hoisted_batch(*ctx->c, level, digits, b, count, galois, keys, b_out, a_out, montgomery != 0,
+                                 S(stream));
+    });
+}
+
 int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
                          uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
                          int montgomery, void* stream) {

@@ -225,6 +225,12 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
 void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s);
 void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key, u64* b_out,
                        u64* a_out, int b_mode, int a_mode, u64 galois, bool montgomery, hipStream_t s);
+// Rotations i < n of one hoisted ciphertext (digits of a, b): (b_out[i],
+// a_out[i]) = g_i(b + KS_i(a)) with galois[i] and keys[i]; outputs must not
+// overlap digits or b.
+void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
+                          const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
+                          hipStream_t s);
 void keyswitch_hoisted_qp(Context& c, int level, const u64* digits, const u64* key, const u64* b_q,
                           u64* b_qp, u64* a_qp, u64 galois, bool montgomery, hipStream_t s);
 void keyswitch_qp(Context& c, int level, const u64* key_term, const u64* key, u64* b_qp, u64* a_qp,

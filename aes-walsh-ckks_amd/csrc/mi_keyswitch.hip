@@ -19,6 +19,7 @@
 // residue unchanged.
 #include "mi_ntt_kernels.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace mi {
@@ -677,6 +678,85 @@ void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key,
         automorphism(c, L, galois, a_out, tmp + W, 1, W, W, s);
         MI_HIP(hipMemcpyAsync(b_out, tmp, sizeof(u64) * W, hipMemcpyDeviceToDevice, s));
         MI_HIP(hipMemcpyAsync(a_out, tmp + W, sizeof(u64) * W, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+// dst_a[i], dst_b[i] = automorphism g_i of item i's (a, b) pair at src + i*2*L*N
+struct AutoItemsArgs {
+    const u64* src;
+    u64 galois[KS_MAX_ITEMS];
+    u64* dst_a[KS_MAX_ITEMS];
+    u64* dst_b[KS_MAX_ITEMS];
+    int L, logN;
+};
+
+__global__ void __launch_bounds__(KS_THREADS) k_automorphism_items(AutoItemsArgs a) {
+    const u64 N = (u64)1 << a.logN;
+    const int tps = (int)(N / KS_THREADS);
+    const int tile = blockIdx.x % tps, rest = blockIdx.x / tps;
+    const int l = rest % a.L, poly = (rest / a.L) & 1, it = rest / (2 * a.L);
+    const u32 i = (u32)tile * KS_THREADS + threadIdx.x;
+    const u32 sidx = galois_src(i, a.galois[it], a.logN);
+    const u64* src = a.src + ((long long)it * 2 + poly) * a.L * N + (long long)l * N;
+    u64* dst = (poly == 0 ? a.dst_a[it] : a.dst_b[it]) + (long long)l * N;
+    dst[i] = src[sidx];
+}
+
+void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
+                          const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
+                          hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
+    if (n < 1) throw Error(-1, "need at least one rotation");
+    const int L = level + 1, K = c.np;
+    const u64 N = c.N;
+    const long long LN = (long long)L * (long long)N, KN = (long long)K * (long long)N;
+    const int chunk = std::min(n, KS_MAX_ITEMS);
+    // per rotation: P accumulators (a, b) | switched pair (a, b) | ModDown temp
+    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (2 * KN + 4 * LN));
+    u64* AP = ws;
+    u64* SW = AP + chunk * 2 * KN;
+    u64* Tm = SW + chunk * 2 * LN;
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int m = std::min(chunk, n - i0);
+        // key products of the shared hoisted digits, b added in the Q limbs
+        // (ckks_hybrid_key_switch_hoisted_inplace with b Add, a Assign)
+        MacArgs ma = mac_args(c, level, digits, keys[i0], montgomery);
+        ma.items = m;
+        ma.digits_stride = 0;
+        ma.accp = AP;
+        ma.accp_stride = 2 * KN;
+        ma.b_mode = 1;
+        ma.a_mode = 1;
+        for (int it = 0; it < m; ++it) {
+            ma.key[it] = keys[i0 + it];
+            ma.a_out[it] = SW + it * 2 * LN;
+            ma.b_out[it] = SW + it * 2 * LN + LN;
+            ma.init_b[it] = b;
+        }
+        ks_mac(c, ma, level, s);
+        u64* p[2 * KS_MAX_ITEMS];
+        const u64* base[2 * KS_MAX_ITEMS];
+        u64* out[2 * KS_MAX_ITEMS];
+        for (int it = 0; it < m; ++it) {
+            p[2 * it] = AP + it * 2 * KN;
+            p[2 * it + 1] = AP + it * 2 * KN + KN;
+            base[2 * it] = out[2 * it] = SW + it * 2 * LN;
+            base[2 * it + 1] = out[2 * it + 1] = SW + it * 2 * LN + LN;
+        }
+        ks_moddown_n(c, level, 2 * m, p, base, out, Tm, s);
+        // the automorphisms (the rotation's NTT-domain map)
+        AutoItemsArgs aa{};
+        aa.src = SW;
+        for (int it = 0; it < m; ++it) {
+            aa.galois[it] = galois[i0 + it] % (2 * N);
+            aa.dst_a[it] = a_out[i0 + it];
+            aa.dst_b[it] = b_out[i0 + it];
+        }
+        aa.L = L;
+        aa.logN = c.logN;
+        hipLaunchKernelGGL(k_automorphism_items, dim3((unsigned)(m * 2 * L * (N / KS_THREADS))), dim3(KS_THREADS),
+                           0, s, aa);
+        MI_HIP(hipGetLastError());
     }
 }
 

@@ -55,6 +55,7 @@ _SIGS = {
     "mi_hoist": (cint, [vp, cint, vp, vp, vp]),
     "mi_keyswitch_hoisted": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, u64, cint, vp]),
     "mi_keyswitch_hoisted_qp": (cint, [vp, cint, vp, vp, vp, vp, vp, u64, cint, vp]),
+    "mi_rotate_hoisted_batch": (cint, [vp, cint, vp, vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_keyswitch_qp": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_moddown_qp": (cint, [vp, cint, vp, vp]),
     "mi_linear_transform_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
@@ -264,6 +265,17 @@ class Context:
                           stream=None):
         check(lib().mi_keyswitch_hoisted(self.h, level, ptr(digits), ptr(key), ptr(b_out), ptr(a_out), b_mode,
                                          a_mode, int(galois), int(montgomery), _stream(stream)))
+
+    def rotate_hoisted_batch(self, level, digits, b, galois, keys, outs, montgomery=True, stream=None):
+        """outs: list of (b_out, a_out) device tensors, one per Galois element."""
+        n = len(galois)
+        g = (u64 * max(n, 1))(*[int(x) for x in galois])
+        kp = (vp * max(n, 1))(*[k.data_ptr() for k in keys])
+        bo = (vp * max(n, 1))(*[o[0].data_ptr() for o in outs])
+        ao = (vp * max(n, 1))(*[o[1].data_ptr() for o in outs])
+        check(lib().mi_rotate_hoisted_batch(self.h, level, ptr(digits), ptr(b), n, ctypes.addressof(g),
+                                            ctypes.addressof(kp), ctypes.addressof(bo), ctypes.addressof(ao),
+                                            int(montgomery), _stream(stream)))
 
     def keyswitch_hoisted_qp(self, level, digits, key, b, b_qp, a_qp, galois, montgomery=True, stream=None):
         check(lib().mi_keyswitch_hoisted_qp(self.h, level, ptr(digits), ptr(key), ptr(b), ptr(b_qp), ptr(a_qp),

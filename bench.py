@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -39,10 +39,17 @@ def _oracle():
     return oracle
 
 
+def _host():
+    sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
+    import ckks_py
+    return ckks_py
+
+
 def _primes(p_bits, q_bits, N):
-    """Reference prime search (restated in the product's host layer is not
-    exposed; the oracle's restatement is pinned to the reference's lists)."""
-    return _oracle().find_primes(p_bits, q_bits, N)
+    """The reference's prime search (find_p_and_q_primes, random.cpp:66-131),
+    from the product's host layer: (P primes, Q primes) as uint64 arrays."""
+    p, q = _host().find_p_and_q_primes(list(p_bits), list(q_bits), N)
+    return np.array(p, dtype=np.uint64), np.array(q, dtype=np.uint64)
 
 
 def _rand_limbs(torch, q, N, count_per_limb, seed):
@@ -343,14 +350,96 @@ def cpu_baseline_lt(N, q, p, level, diag_idx, bs, rots, parts):
             "kind": "port", "sample": "oracle BSGS stage on one ciphertext (same shape)"}
 
 
+def bench_hoisted(args, M, torch):
+    """Hoisted rotations of one ciphertext at the walsh15 shape: N=2^15, the
+    walsh15 chain (13 limbs: 36, 61, 42, 42, 38 x 7, 50, 50 bits) at its top
+    level, P = 4 x 61 bits, dnum 3; 58 rotations (the Walsh S-box unpack
+    count) by 512 s for s = 1..29 and their negatives, keys as seeded
+    residues.  One step = hoist the a-part once, then all 58 rotations."""
+    N = 1 << 15
+    p, q = _primes([61] * 4, [36, 61, 42, 42] + [38] * 7 + [50, 50], N)
+    ctx = M.Context(N, q, p, 3)
+    L, K = len(q), len(p)
+    level = L - 1
+    parts = ctx.parts(level)
+    slots = N // 2
+    rots = [512 * s for s in range(1, 30)] + [slots - 512 * s for s in range(1, 30)]
+    gal = [pow(5, r, 2 * N) for r in rots]
+    keys = [_rand_limbs(torch, list(q) + list(p), N, 2 * parts, seed=300 + i) for i in range(len(rots))]
+    ct = _rand_limbs(torch, q, N, 2, seed=77)
+    W = L * N
+    b, a = ct[:W], ct[W:]
+    dig = torch.empty(parts * (L + K) * N, dtype=torch.int64, device="cuda")
+    outs = [(torch.empty(W, dtype=torch.int64, device="cuda"), torch.empty(W, dtype=torch.int64, device="cuda"))
+            for _ in rots]
+
+    def step():
+        ctx.hoist(level, a, dig)
+        ctx.rotate_hoisted_batch(level, dig, b, gal, keys, outs)
+
+    def step_plain():
+        ctx.hoist(level, a, dig)
+        for g, k, (bo, ao) in zip(gal, keys, outs):
+            bo.copy_(b)
+            ctx.keyswitch_hoisted(level, dig, k, bo, ao, 0, 1, galois=g)
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    for _ in range(args.warmup):
+        step()
+    ms = timed(step, args.steps)
+    ref = [torch.cat(o).clone() for o in outs]
+    plain_ms = None
+    if args.plain:
+        step_plain()
+        plain_ms = timed(step_plain, max(1, args.steps // 4))
+        if not all(torch.equal(torch.cat(o), r) for o, r in zip(outs, ref)):
+            raise RuntimeError("batched and one-by-one hoisted rotations differ")
+    cpu = cpu_baseline_hoisted(N, q, p, level, gal, parts) if args.cpu_baseline else None
+    return {
+        "metric": "hoisted rotations/s of one ciphertext, walsh15 shape (N=2^15, 13 + 4 limbs, dnum 3)",
+        "value": len(rots) / (ms / 1e3), "unit": "rotation/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
+        "config": {"workload": "hoist once + 58 rotations (512 s, s = +-1..29) in one batched call",
+                   "plain_one_rotation_per_call_ms": plain_ms,
+                   "batched_over_plain": (plain_ms / ms) if plain_ms else None},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_hoisted(N, q, p, level, gal, parts):
+    """The oracle's hoist and hoisted key switches (a bounded sample of 8
+    rotations, scaled to rotations/s)."""
+    O = _oracle()
+    octx = O.Ctx(N, q, p, 3)
+    rng = np.random.default_rng(5)
+    L = level + 1
+    rnd = lambda mods: np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in mods])
+    a, b = rnd(q[:L]), rnd(q[:L])
+    key = np.concatenate([rnd(list(q) + list(p)) for _ in range(2 * parts)])
+    t0 = time.perf_counter()
+    dig = octx.hoist(level, a)
+    for g in gal[:8]:
+        octx.keyswitch_hoisted(level, dig, key, b, np.zeros(L * N, np.uint64), 0, 1, galois=g)
+    dt = time.perf_counter() - t0
+    return {"value": 8 / dt, "unit": "rotation/s", "cores": int(O.lib().oc_omp_threads()), "kind": "port",
+            "sample": "oracle hoist + 8 hoisted rotations at the same shape"}
+
+
 def bench_chain(args, M, torch):
     """The plain CKKS primitive chain through the C++ host API: N=2^14, 8 chain
     limbs of 50-bit primes, P = 2 x 50-bit, dnum 4, Hamming weight 192,
     sigma 3.2; 2^13 complex slots uniform in [-1, 1].  One step: encrypt two
     ciphertexts at level 7, HMult, relinearise, rescale, rotate by 1,
     decrypt (host encode/decode included)."""
-    sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
-    import ckks_py as H
+    H = _host()
     c = H.CKKSParamsConfig()
     c.N = 1 << 14
     c.mult_depth = 7
@@ -439,7 +528,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -456,8 +545,8 @@ def main():
     M.lib()
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
-    out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt,
-           "chain": bench_chain}[args.workload](args, M, torch)
+    out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
+           "hoisted": bench_hoisted}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 

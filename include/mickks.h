@@ -190,6 +190,16 @@ int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, vo
 int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
                          uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
                          int montgomery, void* stream);
+/* count rotations of one hoisted ciphertext in one launch sequence (the
+ * Walsh S-box unpacks one ciphertext through dozens of hoisted rotations,
+ * ckks_walsh_sbox.cpp:521-546): for each i, ckks_hybrid_key_switch_hoisted_inplace
+ * with b Add and a Assign and the map of galois[i], i.e.
+ * (b_out[i], a_out[i]) = g_i(b + KS_i(a)) with digits = mi_hoist of a.
+ * keys[i], b_out[i], a_out[i]: device pointers (host arrays); outputs must
+ * not overlap digits or b. */
+int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
+                            const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
+                            uint64_t* const* a_out, int montgomery, void* stream);
 /* ckks_hybrid_key_switch_hoisted_qp_rotated (ckks_keyswitch.cpp:675-852):
  * b_qp = g([b ; 0] + sum_d digit_d key_b,d), a_qp = g(sum_d digit_d key_a,d),
  * no ModDown.  b: level+1 eval limbs; b_qp, a_qp: level+1+np limbs. */

@@ -170,6 +170,49 @@ def test_hoisted_and_qp_switches_match_oracle(N, nq, np_, dnum, mont):
         assert np.array_equal(M.from_dev(dx)[: L * N], octx.moddown_qp_to_q(level, x)), level
 
 
+def test_rotate_hoisted_batch_matches_single_rotations_and_reference():
+    """mi_rotate_hoisted_batch equals one mi_keyswitch_hoisted (b Add, a
+    Assign, galois) per rotation, bit for bit, for 37 rotations (more than
+    one launch sequence takes); on lt10 it reproduces the reference's hoisted
+    rotation by 1."""
+    d = load_npz("lt10.npz")
+    o = load_json("lt10_outputs.json")
+    M = product()
+    N = 1 << 10
+    q, p = d["q"], d["p"]
+    L, K = len(q), len(p)
+    ctx = M.Context(N, q, p, 2)
+    lv = L - 1
+    b, a = d["ct"].reshape(2, L * N)
+    dig = _zeros(M, ctx.parts(lv) * (L + K) * N)
+    ctx.hoist(lv, M.to_dev(a), dig)
+    db = M.to_dev(b)
+    key1 = M.to_dev(d["key_rot1"])
+    outs = [(_zeros(M, L * N), _zeros(M, L * N))]
+    ctx.rotate_hoisted_batch(lv, dig, db, [O.galois_rotation(N, 1)], [key1], outs)
+    assert digest(np.concatenate([M.from_dev(outs[0][0]), M.from_dev(outs[0][1])])) == o["hoisted_rot1"]
+    # many rotations with random keys at a larger ring
+    M, ctx, octx, q, p, rng = _setup(1 << 13, 9, 3, 3, seed=29)
+    N, lv = 1 << 13, 7
+    L, T = lv + 1, lv + 1 + len(p)
+    a = rand_residues(rng, q[:L], N)
+    b = rand_residues(rng, q[:L], N)
+    dig = _zeros(M, ctx.parts(lv) * T * N)
+    ctx.hoist(lv, M.to_dev(a), dig)
+    keys = [M.to_dev(_key(rng, q, p, N, octx.parts(len(q) - 1))) for _ in range(5)]
+    rots = [1 + 3 * i for i in range(37)]
+    gal = [O.galois_rotation(N, r) for r in rots]
+    ks = [keys[i % 5] for i in range(37)]
+    outs = [(_zeros(M, L * N), _zeros(M, L * N)) for _ in rots]
+    db = M.to_dev(b)
+    ctx.rotate_hoisted_batch(lv, dig, db, gal, ks, outs)
+    for i, (g, k) in enumerate(zip(gal, ks)):
+        hb, ha = M.to_dev(b), _zeros(M, L * N)
+        ctx.keyswitch_hoisted(lv, dig, k, hb, ha, 0, 1, galois=g)
+        assert np.array_equal(M.from_dev(outs[i][0]), M.from_dev(hb)), i
+        assert np.array_equal(M.from_dev(outs[i][1]), M.from_dev(ha)), i
+
+
 def _lt_case(M, ctx, octx, q, p, rng, N, level, diag_idx, bs, mont=True, pt_mont=True, in_place=False):
     L, T = level + 1, level + 1 + len(p)
     qp = list(q[:L]) + list(p)
