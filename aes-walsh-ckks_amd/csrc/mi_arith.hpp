@@ -41,6 +41,13 @@ __device__ __forceinline__ void mac128(U128& acc, u64 a, u64 b) {
 __device__ __forceinline__ u64 shoup_lazy(u64 a, u64 w, u64 ws, u64 q) {
     return a * w - __umul64hi(a, ws) * q;
 }
+// The same lazy product with negq = 2^64 - q: a*w + qhat*negq is a*w - qhat*q
+// modulo 2^64, so the subtraction rides in the low word's multiply-add.
+// negq must come from memory (a table field): from 0 - q the compiler
+// rebuilds the subtraction.
+__device__ __forceinline__ u64 shoup_lazy_neg(u64 a, u64 w, u64 ws, u64 negq) {
+    return a * w + __umul64hi(a, ws) * negq;
+}
 __device__ __forceinline__ u64 shoup(u64 a, u64 w, u64 ws, u64 q) {
     u64 r = shoup_lazy(a, w, ws, q);
     return r >= q ? r - q : r;

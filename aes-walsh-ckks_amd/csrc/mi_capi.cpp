@@ -11,6 +11,11 @@ struct mi_context {
     mi::Context* c;
 };
 
+struct mi_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -278,7 +283,7 @@ int mi_memset_zero(mi_context* ctx, void* ptr, size_t bytes, void* stream) {
         check_ctx(ctx);
         if (bytes == 0) return;
         check_ptr(ptr);
-        MI_HIP(hipMemsetAsync(ptr, 0, bytes, S(stream)));
+        mi::fill_zero(ptr, bytes, S(stream));
     });
 }
 
@@ -623,6 +628,59 @@ int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t*
         check_ctx(ctx);
         check_ptr(b); check_ptr(a); check_ptr(key);
         mi::q0p0_keyswitch(*ctx->c, b, a, key, montgomery != 0, S(stream));
+    });
+}
+
+int mi_graph_begin(mi_context* ctx, void* stream) {
+    return guard("mi_graph_begin", [&] {
+        check_ctx(ctx);
+        if (!stream) throw mi::Error(MI_EINVAL, "capture needs a created stream, not the null stream");
+        MI_HIP(hipStreamBeginCapture(S(stream), hipStreamCaptureModeRelaxed));
+        ctx->c->capture_begin();
+    });
+}
+
+int mi_graph_end(mi_context* ctx, void* stream, mi_graph** out) {
+    return guard("mi_graph_end", [&] {
+        check_ctx(ctx);
+        if (!stream || !out) throw mi::Error(MI_EINVAL, "null argument");
+        *out = nullptr;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        MI_HIP(hipStreamIsCapturing(S(stream), &cap));
+        if (cap == hipStreamCaptureStatusNone) throw mi::Error(MI_EINVAL, "stream is not capturing");
+        mi_graph* g = new mi_graph;
+        hipError_t e = hipStreamEndCapture(S(stream), &g->graph);
+        if (e == hipSuccess && g->graph) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess || !g->exec) {
+            if (g->graph) (void)hipGraphDestroy(g->graph);
+            delete g;
+            ctx->c->capture_end(false);
+            (void)hipGetLastError();
+            throw mi::Error(MI_EHIP, std::string("capture did not produce a graph: ") + hipGetErrorString(e) +
+                                         " (a call inside the capture synchronised, allocated a workspace or"
+                                         " failed; its own error was returned then)");
+        }
+        ctx->c->capture_end(true);
+        *out = g;
+    });
+}
+
+int mi_graph_launch(mi_context* ctx, mi_graph* graph, void* stream) {
+    return guard("mi_graph_launch", [&] {
+        check_ctx(ctx);
+        if (!graph || !graph->exec) throw mi::Error(MI_EINVAL, "null graph");
+        MI_HIP(hipGraphLaunch(graph->exec, S(stream)));
+    });
+}
+
+int mi_graph_destroy(mi_context* ctx, mi_graph* graph) {
+    return guard("mi_graph_destroy", [&] {
+        check_ctx(ctx);
+        if (!graph) return;
+        (void)hipGraphExecDestroy(graph->exec);
+        (void)hipGraphDestroy(graph->graph);
+        delete graph;
+        ctx->c->graph_destroyed();
     });
 }
 

@@ -88,6 +88,7 @@ static void make_twiddles(u64 N, int logN, u64 q, u64* fwd, u64* fws, u64* inv, 
     bc.n_inv_shoup = shoup_of(bc.n_inv, q);
     bc.r_mod_q = (u64)((((u128)1) << 64) % q);
     bc.inv_d = 1.0 / (double)q;
+    bc.negq = (u64)0 - q;
     (void)logN;
 }
 
@@ -95,13 +96,30 @@ void* Context::workspace(hipStream_t s, size_t bytes) {
     std::lock_guard<std::mutex> g(ws_mu);
     Workspace& w = ws[s];
     if (w.bytes < bytes) {
+        // Under stream capture the buffer can be neither synchronised nor
+        // reallocated: the captured calls must have run once before.
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        MI_HIP(hipStreamIsCapturing(s, &cap));
+        if (cap != hipStreamCaptureStatusNone)
+            throw Error(-2, "stream workspace holds " + std::to_string(w.bytes) + " bytes, the call needs " +
+                                std::to_string(bytes) +
+                                "; run the captured calls once outside the capture first so the workspace is sized");
+        const size_t nb = std::max(bytes, w.bytes + w.bytes / 2);
+        int graphs = 0;
+        {
+            std::lock_guard<std::mutex> gp(pool_mu);
+            graphs = live_graphs;
+        }
         if (w.ptr) {
-            MI_HIP(hipStreamSynchronize(s));
-            MI_HIP(hipFree(w.ptr));
+            if (graphs > 0) {
+                retired.push_back(w.ptr);  // a graph may still replay into it
+            } else {
+                MI_HIP(hipStreamSynchronize(s));
+                MI_HIP(hipFree(w.ptr));
+            }
             w.ptr = nullptr;
             w.bytes = 0;
         }
-        size_t nb = std::max(bytes, w.bytes * 2);
         MI_HIP(hipMalloc(&w.ptr, nb));
         w.bytes = nb;
     }
@@ -340,6 +358,7 @@ void* Context::pool_alloc(size_t bytes) {
         }
     }
     pool_live[p] = got;
+    if (capturing > 0) captured_blocks.insert(p);
     pool_live_bytes += got;
     pool_peak_bytes = std::max(pool_peak_bytes, pool_live_bytes);
     return p;
@@ -351,9 +370,48 @@ void Context::pool_release(void* p) {
     auto it = pool_live.find(p);
     if (it == pool_live.end()) throw Error(-1, "pointer was not allocated by this context");
     pool_live_bytes -= it->second;
-    pool_cached_bytes += it->second;
-    pool_free.emplace(it->second, p);
+    if (capturing > 0 || captured_blocks.count(p)) {
+        graph_held.emplace_back(it->second, p);
+    } else {
+        pool_cached_bytes += it->second;
+        pool_free.emplace(it->second, p);
+    }
     pool_live.erase(it);
+}
+
+void Context::capture_begin() {
+    std::lock_guard<std::mutex> g(pool_mu);
+    ++capturing;
+}
+
+void Context::capture_end(bool made_graph) {
+    std::lock_guard<std::mutex> g(pool_mu);
+    --capturing;
+    if (made_graph) ++live_graphs;
+    if (capturing > 0 || live_graphs > 0) return;
+    for (auto& h : graph_held) {
+        pool_cached_bytes += h.first;
+        pool_free.emplace(h.first, h.second);
+    }
+    graph_held.clear();
+    captured_blocks.clear();
+}
+
+void Context::graph_destroyed() {
+    {
+        std::lock_guard<std::mutex> g(pool_mu);
+        --live_graphs;
+        if (capturing > 0 || live_graphs > 0) return;
+        for (auto& h : graph_held) {
+            pool_cached_bytes += h.first;
+            pool_free.emplace(h.first, h.second);
+        }
+        graph_held.clear();
+        captured_blocks.clear();
+    }
+    std::lock_guard<std::mutex> g(ws_mu);
+    for (void* r : retired) (void)hipFree(r);
+    retired.clear();
 }
 
 void Context::pool_trim() {
@@ -368,6 +426,7 @@ void context_destroy(Context* c) {
     (void)hipDeviceSynchronize();
     for (auto& kv : c->pool_free) (void)hipFree(kv.second);
     for (auto& kv : c->pool_live) (void)hipFree(kv.first);
+    for (auto& kv : c->graph_held) (void)hipFree(kv.second);
     (void)hipFree(c->d_fwd);
     (void)hipFree(c->d_fwd_shoup);
     (void)hipFree(c->d_inv);
@@ -380,6 +439,7 @@ void context_destroy(Context* c) {
     (void)hipFree(c->d_gauss_cdf);
     for (auto& kv : c->ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (void* r : c->retired) (void)hipFree(r);
     for (hipStream_t st : c->side) (void)hipStreamDestroy(st);
     delete c;
 }

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <set>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -21,6 +22,7 @@ struct BasisConst {
     u64 n_inv, n_inv_shoup;
     u64 r_mod_q;        // 2^64 mod q
     double inv_d;       // 1.0 / (double)q (HPS estimates)
+    u64 negq;           // 2^64 - q
 };
 
 // HPS ModUp tables of one (level, digit): source limbs are the digit's chain
@@ -82,6 +84,9 @@ struct Context {
     size_t table_bytes = 0;
     std::mutex ws_mu;
     std::map<hipStream_t, Workspace> ws;
+    // Workspaces replaced while a graph is alive (a graph may replay into
+    // them); freed when the last graph is destroyed, or with the context.
+    std::vector<void*> retired;
     // Side streams (default priority) that independent groups of a batch
     // run on concurrently; created on first use, joined back with events.
     std::mutex side_mu;
@@ -93,6 +98,12 @@ struct Context {
     std::map<void*, size_t> pool_live;
     size_t pool_live_bytes = 0, pool_peak_bytes = 0, pool_cached_bytes = 0;
     size_t pool_limit = 0;  // 0 = no limit
+    // Graph capture (pool_mu): blocks allocated or released while a capture
+    // is open can be addressed by the graph, so once released they are held
+    // back from reuse until no capture is open and no graph is alive.
+    int capturing = 0, live_graphs = 0;
+    std::set<void*> captured_blocks;
+    std::vector<std::pair<size_t, void*>> graph_held;
 
     int nb() const { return nq + np; }
     int parts(int level) const {
@@ -105,6 +116,10 @@ struct Context {
     void* pool_alloc(size_t bytes);
     void pool_release(void* p);
     void pool_trim();
+    void capture_begin();
+    // made_graph: the capture produced an instantiated graph
+    void capture_end(bool made_graph);
+    void graph_destroyed();
 };
 
 struct Error : std::runtime_error {
@@ -155,6 +170,8 @@ void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* 
 void automorphism_acc(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                       long long sstride, long long dstride, bool accumulate, hipStream_t s);
 void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s);
+// bytes of zeros at p by a kernel (no memset node, so safe inside a captured graph)
+void fill_zero(void* p, size_t bytes, hipStream_t s);
 void add_plain_mont(const Context& c, int limbs, u64* b, const u64* pt, hipStream_t s);
 
 // ---- key switching (mi_keyswitch.hip) ----

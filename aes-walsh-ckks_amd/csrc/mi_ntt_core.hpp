@@ -22,22 +22,28 @@ constexpr int NTT_LDS_WORDS = NTT_THREADS * E + NTT_THREADS;
 
 __device__ __forceinline__ int lpad(int p) { return p + (p >> LOGE); }
 
+// Modulus constants of one limb for the butterflies: q2 = 2q, and the
+// negations negq = 2^64 - q (read from the basis table) and negq2 = 2^64 - 2q,
+// so conditional subtractions become selects of a 64-bit add.
+struct BfConst {
+    u64 q2, negq, negq2;
+};
+__device__ __forceinline__ BfConst bf_const(u64 q, u64 negq) { return BfConst{q << 1, negq, negq << 1}; }
+
 // Forward (DIT, Harvey lazy): x0,x1 in [0,4q) -> [0,4q).
-__device__ __forceinline__ void bf_fwd(u64& x0, u64& x1, u64 w, u64 ws, u64 q, u64 q2) {
-    u64 t = x0;
-    if (t >= q2) t -= q2;
-    u64 u = shoup_lazy(x1, w, ws, q);
+__device__ __forceinline__ void bf_fwd(u64& x0, u64& x1, u64 w, u64 ws, const BfConst& k) {
+    const u64 t = x0 >= k.q2 ? x0 + k.negq2 : x0;
+    const u64 u = shoup_lazy_neg(x1, w, ws, k.negq);
     x0 = t + u;
-    x1 = t + q2 - u;
+    x1 = t + k.q2 - u;
 }
 
 // Inverse (DIF): x0,x1 in [0,2q) -> [0,2q).
-__device__ __forceinline__ void bf_inv(u64& x0, u64& x1, u64 w, u64 ws, u64 q, u64 q2) {
-    u64 t = x0, u = x1;
-    u64 s = t + u;
-    if (s >= q2) s -= q2;
-    x1 = shoup_lazy(t - u + q2, w, ws, q);
-    x0 = s;
+__device__ __forceinline__ void bf_inv(u64& x0, u64& x1, u64 w, u64 ws, const BfConst& k) {
+    const u64 t = x0, u = x1;
+    const u64 s = t + u;
+    x1 = shoup_lazy_neg(t - u + k.q2, w, ws, k.negq);
+    x0 = s >= k.q2 ? s + k.negq2 : s;
 }
 
 template <int LOGR>
@@ -73,11 +79,10 @@ __device__ __forceinline__ void to_strided(u64 (&x)[E], int t, u64* lds) {
 // Forward group transform.  In: strided layout; out: strided layout.
 // idx(s, i) gives the global twiddle index of group sub-stage s, block i.
 template <int LOGR, class IDX>
-__device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, u64 q,
+__device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, const BfConst& kq,
                                           const u64* __restrict__ W, const u64* __restrict__ WS,
                                           IDX idx) {
     using G = Grp<LOGR>;
-    const u64 q2 = q << 1;
 #pragma unroll
     for (int s = 0; s < LOGE; ++s) {
         const int dj = E >> (s + 1);
@@ -85,7 +90,7 @@ __device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, u64 q,
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & dj) continue;
             const int k = idx(s, j0 >> (LOGE - s));
-            bf_fwd(x[j0], x[j0 + dj], W[k], WS[k], q, q2);
+            bf_fwd(x[j0], x[j0 + dj], W[k], WS[k], kq);
         }
     }
     __syncthreads();
@@ -97,7 +102,7 @@ __device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, u64 q,
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & d) continue;
             const int k = idx(s, (t * E + j0) >> (LOGR - s));
-            bf_fwd(x[j0], x[j0 + d], W[k], WS[k], q, q2);
+            bf_fwd(x[j0], x[j0 + d], W[k], WS[k], kq);
         }
     }
     __syncthreads();
@@ -107,11 +112,10 @@ __device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, u64 q,
 // Inverse group transform.  In: strided layout; out: strided layout.
 // idx(u, i): global twiddle index for group sub-stage u (distance 2^u), block i.
 template <int LOGR, class IDX>
-__device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, u64 q,
+__device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, const BfConst& kq,
                                           const u64* __restrict__ W, const u64* __restrict__ WS,
                                           IDX idx) {
     using G = Grp<LOGR>;
-    const u64 q2 = q << 1;
     __syncthreads();
     to_contiguous<LOGR>(x, t, lds);
 #pragma unroll
@@ -121,7 +125,7 @@ __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, u64 q,
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & d) continue;
             const int k = idx(u, (t * E + j0) >> (u + 1));
-            bf_inv(x[j0], x[j0 + d], W[k], WS[k], q, q2);
+            bf_inv(x[j0], x[j0 + d], W[k], WS[k], kq);
         }
     }
     __syncthreads();
@@ -133,7 +137,7 @@ __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, u64 q,
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & dj) continue;
             const int k = idx(u, j0 >> (u + 1 - G::LOGTG));
-            bf_inv(x[j0], x[j0 + dj], W[k], WS[k], q, q2);
+            bf_inv(x[j0], x[j0 + dj], W[k], WS[k], kq);
         }
     }
 }

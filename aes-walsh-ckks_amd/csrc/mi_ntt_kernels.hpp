@@ -68,11 +68,13 @@ k_ntt_cols(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
     if constexpr (FWD) {
         const u64* W = tb.fwd + (long long)v.basis * tb.N;
         const u64* WS = tb.fwd_shoup + (long long)v.basis * tb.N;
-        group_fwd<LOGR>(x, t, my, bc.q, W, WS, [](int s, int i) { return (1 << s) + i; });
+        group_fwd<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
+                        [](int s, int i) { return (1 << s) + i; });
     } else {
         const u64* W = tb.inv + (long long)v.basis * tb.N;
         const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR>(x, t, my, bc.q, W, WS, [n1](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
+        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
+                        [n1](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
 #pragma unroll
         for (int j = 0; j < E; ++j) x[j] = shoup(x[j], bc.n_inv, bc.n_inv_shoup, bc.q);
     }
@@ -100,14 +102,14 @@ k_ntt_rows(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
     if constexpr (FWD) {
         const u64* W = tb.fwd + (long long)v.basis * tb.N;
         const u64* WS = tb.fwd_shoup + (long long)v.basis * tb.N;
-        group_fwd<LOGR>(x, t, my, bc.q, W, WS,
+        group_fwd<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
                         [n1, row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
 #pragma unroll
         for (int j = 0; j < E; ++j) x[j] = reduce_4q(x[j], bc.q);
     } else {
         const u64* W = tb.inv + (long long)v.basis * tb.N;
         const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR>(x, t, my, bc.q, W, WS, [n, n2, row](int u, int i) {
+        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS, [n, n2, row](int u, int i) {
             return (1 << (n - 1 - u)) + (row << (n2 - 1 - u)) + i;
         });
     }

@@ -49,6 +49,34 @@ void ew_binary(const Context& c, int op, u64* x, const u64* y, int limbs, int np
     MI_HIP(hipGetLastError());
 }
 
+// Zero fill: whole 8-byte words at an 8-byte aligned base, bytes for the
+// unaligned head and the tail.
+__global__ void __launch_bounds__(EW_THREADS) k_fill_zero_words(u64* p, long long words) {
+    const long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i < words) p[i] = 0;
+}
+__global__ void __launch_bounds__(64) k_fill_zero_bytes(unsigned char* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+void fill_zero(void* ptr, size_t bytes, hipStream_t s) {
+    unsigned char* p = (unsigned char*)ptr;
+    const size_t head = (8 - ((uintptr_t)p & 7)) & 7;
+    if (head >= bytes) {
+        hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(64), 0, s, p, (int)bytes);
+        MI_HIP(hipGetLastError());
+        return;
+    }
+    if (head) hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(64), 0, s, p, (int)head);
+    const long long words = (long long)((bytes - head) / 8);
+    const int tail = (int)((bytes - head) % 8);
+    if (words)
+        hipLaunchKernelGGL(k_fill_zero_words, dim3(ew_blocks(words)), dim3(EW_THREADS), 0, s, (u64*)(p + head),
+                           words);
+    if (tail) hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(64), 0, s, p + head + 8 * words, tail);
+    MI_HIP(hipGetLastError());
+}
+
 struct Scalars {
     u64 v[MAX_LIMBS];
 };

@@ -72,6 +72,10 @@ _SIGS = {
     "mi_sample_gaussian": (cint, [vp, vp, cint, ctypes.c_double, u64, u64, vp]),
     "mi_modraise": (cint, [vp, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint, vp]),
+    "mi_graph_begin": (cint, [vp, vp]),
+    "mi_graph_end": (cint, [vp, vp, vp]),
+    "mi_graph_launch": (cint, [vp, vp, vp]),
+    "mi_graph_destroy": (cint, [vp, vp]),
 }
 
 EXPORTED = sorted(_SIGS)
@@ -351,6 +355,32 @@ class Context:
 
     def q0p0_keyswitch(self, b, a, key, montgomery=True, stream=None):
         check(lib().mi_q0p0_keyswitch(self.h, ptr(b), ptr(a), ptr(key), int(montgomery), _stream(stream)))
+
+    # ---- HIP graphs
+    def graph_begin(self, stream):
+        """Start recording the calls made on `stream` (a created torch stream)."""
+        check(lib().mi_graph_begin(self.h, _stream(stream)))
+
+    def graph_end(self, stream):
+        """Stop recording on `stream`; returns a Graph to replay."""
+        g = vp()
+        check(lib().mi_graph_end(self.h, _stream(stream), ctypes.byref(g)))
+        return Graph(self, g)
+
+
+class Graph:
+    """An instantiated capture of context calls (mi_graph_*)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    def launch(self, stream):
+        check(lib().mi_graph_launch(self.ctx.h, self.h, _stream(stream)))
+
+    def destroy(self):
+        if self.h:
+            check(lib().mi_graph_destroy(self.ctx.h, self.h))
+            self.h = None
 
 
 def to_dev(arr):

@@ -303,6 +303,26 @@ int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint
 int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
                       int montgomery, void* stream);
 
+/* ---- HIP graph capture ----
+ * No counterpart in the reference, whose evaluator runs eagerly on host
+ * threads; these let a caller record a fixed sequence of the calls above (an
+ * S-box, a bootstrap stage, a batch of key switches) once and replay it as
+ * one HIP graph.  mi_graph_begin starts capturing `stream` (a created
+ * stream, not the null stream); every call made on it until mi_graph_end is
+ * recorded, not run.  Calls that synchronise the host (encode, sampling
+ * table changes, host copies) cannot be captured, and a call whose stream
+ * workspace would have to grow fails: run the sequence once eagerly first.
+ * Replays read and write the buffers that were passed during capture.
+ * Device blocks that the context allocates or frees during a capture are
+ * kept from reuse until every graph of the context is destroyed.  A replay
+ * shares the context's workspaces with eager calls: do not replay on one
+ * stream while calls run on another. */
+typedef struct mi_graph mi_graph;
+int mi_graph_begin(mi_context* ctx, void* stream);
+int mi_graph_end(mi_context* ctx, void* stream, mi_graph** out);
+int mi_graph_launch(mi_context* ctx, mi_graph* graph, void* stream);
+int mi_graph_destroy(mi_context* ctx, mi_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif
