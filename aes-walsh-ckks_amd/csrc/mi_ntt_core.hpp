@@ -17,8 +17,9 @@ namespace mi {
 constexpr int E = 16;
 constexpr int LOGE = 4;
 constexpr int NTT_THREADS = 256;
-// LDS words per workgroup: NTT_THREADS*E data words plus one pad per 16.
-constexpr int NTT_LDS_WORDS = NTT_THREADS * E + NTT_THREADS;
+// LDS words per workgroup: NTT_THREADS*E data words, one pad per 16, and one
+// more word per group (at most 128 groups) so that the group stride is odd.
+constexpr int NTT_LDS_WORDS = NTT_THREADS * E + NTT_THREADS + 128;
 
 __device__ __forceinline__ int lpad(int p) { return p + (p >> LOGE); }
 
@@ -51,7 +52,11 @@ struct Grp {
     static constexpr int R = 1 << LOGR;
     static constexpr int TG = R / E;
     static constexpr int LOGTG = LOGR - LOGE;
-    static constexpr int LDS = R + R / E;  // padded words per group
+    // Padded words per group.  The stride is odd: with R + R/E alone (272
+    // words at R = 256, a multiple of 16) the same position of the 16 groups
+    // that a column-pass wave spans falls in one LDS bank.
+    static constexpr int LDS = R + R / E + 1;
+    static_assert((NTT_THREADS / TG) * LDS <= NTT_LDS_WORDS, "groups of a workgroup exceed its LDS");
     static_assert(LOGR >= LOGE + 1 && LOGR <= 2 * LOGE, "group size 32..256");
 };
 
