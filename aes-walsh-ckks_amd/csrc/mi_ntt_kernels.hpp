@@ -50,9 +50,21 @@ struct PlainJob {
     __device__ bool skip(int) const { return false; }
 };
 
-template <int LOGR, bool FWD, class Job>
+// Pass geometry, fixed at compile time by log2 N: n1 = n/2 column stages and
+// n2 = n - n1 row stages.  A pass whose groups have 2^logr elements and which
+// has 2^logother groups per limb runs workgroups of pass_threads() threads
+// holding pass_groups() groups, pass_wpl() workgroups per limb.
+constexpr int pass_threads(int logr, int logother) {
+    return ((1 << (logr - LOGE)) << logother) > NTT_THREADS ? NTT_THREADS : ((1 << (logr - LOGE)) << logother);
+}
+constexpr int pass_groups(int logr, int logother) { return pass_threads(logr, logother) >> (logr - LOGE); }
+constexpr int pass_wpl(int logr, int logother) { return (1 << logother) / pass_groups(logr, logother); }
+
+template <int LOGN, bool FWD, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
-k_ntt_cols(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
+k_ntt_cols(NttTabs tb, Job job) {
+    constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n1;
+    constexpr int G = pass_groups(n1, n2), wpl = pass_wpl(n1, n2);
     __shared__ u64 lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
     const int limb = blockIdx.x / wpl, w = blockIdx.x - limb * wpl;
@@ -74,7 +86,7 @@ k_ntt_cols(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
         const u64* W = tb.inv + (long long)v.basis * tb.N;
         const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
         group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                        [n1](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
+                        [](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
 #pragma unroll
         for (int j = 0; j < E; ++j) x[j] = shoup(x[j], bc.n_inv, bc.n_inv_shoup, bc.q);
     }
@@ -82,9 +94,11 @@ k_ntt_cols(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
     for (int j = 0; j < E; ++j) v.store(((t + Gr::TG * j) << n2) + col, x[j]);
 }
 
-template <int LOGR, bool FWD, class Job>
+template <int LOGN, bool FWD, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
-k_ntt_rows(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
+k_ntt_rows(NttTabs tb, Job job) {
+    constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n2;
+    constexpr int G = pass_groups(n2, n1), wpl = pass_wpl(n2, n1);
     __shared__ u64 lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
     const int limb = blockIdx.x / wpl, w = blockIdx.x - limb * wpl;
@@ -98,64 +112,55 @@ k_ntt_rows(NttTabs tb, Job job, int n1, int n2, int G, int wpl) {
 #pragma unroll
     for (int j = 0; j < E; ++j) x[j] = v.load(off + t + Gr::TG * j);
     u64* my = lds + g * Gr::LDS;
-    const int n = n1 + n2;
     if constexpr (FWD) {
         const u64* W = tb.fwd + (long long)v.basis * tb.N;
         const u64* WS = tb.fwd_shoup + (long long)v.basis * tb.N;
         group_fwd<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                        [n1, row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
+                        [row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
 #pragma unroll
         for (int j = 0; j < E; ++j) x[j] = reduce_4q(x[j], bc.q);
     } else {
         const u64* W = tb.inv + (long long)v.basis * tb.N;
         const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS, [n, n2, row](int u, int i) {
-            return (1 << (n - 1 - u)) + (row << (n2 - 1 - u)) + i;
+        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS, [row](int u, int i) {
+            return (1 << (LOGN - 1 - u)) + (row << (n2 - 1 - u)) + i;
         });
     }
 #pragma unroll
     for (int j = 0; j < E; ++j) v.store(off + t + Gr::TG * j, x[j]);
 }
 
-// Workgroup shape of a pass whose groups have 2^logr elements and which has
-// 2^logother groups per limb.
-inline void pass_shape(int logr, int logother, int& threads, int& G, int& wpl) {
-    int tg = 1 << (logr - LOGE);
-    threads = tg << logother;
-    if (threads > NTT_THREADS) threads = NTT_THREADS;
-    G = threads / tg;
-    wpl = (1 << logother) / G;
-}
+#define MI_NTT_LOGN_CASES(K)                                                                          \
+    switch (c.logN) {                                                                                 \
+        case 10: K(10); break;                                                                        \
+        case 11: K(11); break;                                                                        \
+        case 12: K(12); break;                                                                        \
+        case 13: K(13); break;                                                                        \
+        case 14: K(14); break;                                                                        \
+        case 15: K(15); break;                                                                        \
+        case 16: K(16); break;                                                                        \
+        default: throw Error(-1, "ntt: unsupported ring dimension");                                  \
+    }
 
 template <bool FWD, class Job>
 void launch_cols(const Context& c, const Job& job, int count, hipStream_t s) {
-    int threads, G, wpl;
-    pass_shape(c.n1, c.n2, threads, G, wpl);
-    dim3 grid((unsigned)count * wpl), block(threads);
-    NttTabs tb = tabs_of(c);
-    switch (c.n1) {
-        case 5: hipLaunchKernelGGL((k_ntt_cols<5, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 6: hipLaunchKernelGGL((k_ntt_cols<6, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 7: hipLaunchKernelGGL((k_ntt_cols<7, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 8: hipLaunchKernelGGL((k_ntt_cols<8, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        default: throw Error(-1, "ntt: unsupported ring dimension");
-    }
+    const NttTabs tb = tabs_of(c);
+#define MI_COLS(LN)                                                                                   \
+    hipLaunchKernelGGL((k_ntt_cols<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN / 2, LN - LN / 2)), \
+                       dim3(pass_threads(LN / 2, LN - LN / 2)), 0, s, tb, job)
+    MI_NTT_LOGN_CASES(MI_COLS)
+#undef MI_COLS
     MI_HIP(hipGetLastError());
 }
 
 template <bool FWD, class Job>
 void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
-    int threads, G, wpl;
-    pass_shape(c.n2, c.n1, threads, G, wpl);
-    dim3 grid((unsigned)count * wpl), block(threads);
-    NttTabs tb = tabs_of(c);
-    switch (c.n2) {
-        case 5: hipLaunchKernelGGL((k_ntt_rows<5, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 6: hipLaunchKernelGGL((k_ntt_rows<6, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 7: hipLaunchKernelGGL((k_ntt_rows<7, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        case 8: hipLaunchKernelGGL((k_ntt_rows<8, FWD, Job>), grid, block, 0, s, tb, job, c.n1, c.n2, G, wpl); break;
-        default: throw Error(-1, "ntt: unsupported ring dimension");
-    }
+    const NttTabs tb = tabs_of(c);
+#define MI_ROWS(LN)                                                                                   \
+    hipLaunchKernelGGL((k_ntt_rows<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN - LN / 2, LN / 2)), \
+                       dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job)
+    MI_NTT_LOGN_CASES(MI_ROWS)
+#undef MI_ROWS
     MI_HIP(hipGetLastError());
 }
 
