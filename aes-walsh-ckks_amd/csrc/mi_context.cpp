@@ -89,6 +89,8 @@ static void make_twiddles(u64 N, int logN, u64 q, u64* fwd, u64* fws, u64* inv, 
     bc.r_mod_q = (u64)((((u128)1) << 64) % q);
     bc.inv_d = 1.0 / (double)q;
     bc.negq = (u64)0 - q;
+    bc.w1_ninv = (u64)(((u128)inv[1] * bc.n_inv) % q);
+    bc.w1_ninv_shoup = shoup_of(bc.w1_ninv, q);
     (void)logN;
 }
 

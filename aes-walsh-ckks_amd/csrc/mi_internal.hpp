@@ -23,6 +23,8 @@ struct BasisConst {
     u64 r_mod_q;        // 2^64 mod q
     double inv_d;       // 1.0 / (double)q (HPS estimates)
     u64 negq;           // 2^64 - q
+    u64 w1_ninv, w1_ninv_shoup;  // inverse twiddle 1 times N^-1: the last inverse
+                                 //   stage fused with the scaling
 };
 
 // HPS ModUp tables of one (level, digit): source limbs are the digit's chain

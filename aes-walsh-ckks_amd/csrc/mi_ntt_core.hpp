@@ -116,7 +116,8 @@ __device__ __forceinline__ void group_fwd(u64 (&x)[E], int t, u64* lds, const Bf
 
 // Inverse group transform.  In: strided layout; out: strided layout.
 // idx(u, i): global twiddle index for group sub-stage u (distance 2^u), block i.
-template <int LOGR, class IDX>
+// LAST = false leaves the last sub-stage (pairs j, j + E/2) to the caller.
+template <int LOGR, bool LAST = true, class IDX>
 __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, const BfConst& kq,
                                           const u64* __restrict__ W, const u64* __restrict__ WS,
                                           IDX idx) {
@@ -136,7 +137,7 @@ __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, const Bf
     __syncthreads();
     to_strided<LOGR>(x, t, lds);
 #pragma unroll
-    for (int u = LOGE; u < LOGR; ++u) {
+    for (int u = LOGE; u < (LAST ? LOGR : LOGR - 1); ++u) {
         const int dj = 1 << (u - G::LOGTG);
 #pragma unroll
         for (int j0 = 0; j0 < E; ++j0) {

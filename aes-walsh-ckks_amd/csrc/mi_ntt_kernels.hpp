@@ -85,10 +85,17 @@ k_ntt_cols(NttTabs tb, Job job) {
     } else {
         const u64* W = tb.inv + (long long)v.basis * tb.N;
         const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                        [](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
+        group_inv<LOGR, false>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
+                               [](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
+        // last stage (distance N/2, twiddle index 1) with the N^-1 scaling:
+        // x0 = (t + u) N^-1, x1 = (t - u) w_1 N^-1, canonical
+        const u64 q2 = bc.q << 1;
 #pragma unroll
-        for (int j = 0; j < E; ++j) x[j] = shoup(x[j], bc.n_inv, bc.n_inv_shoup, bc.q);
+        for (int j = 0; j < E / 2; ++j) {
+            const u64 a = x[j], b = x[j + E / 2];
+            x[j] = shoup(a + b, bc.n_inv, bc.n_inv_shoup, bc.q);
+            x[j + E / 2] = shoup(a - b + q2, bc.w1_ninv, bc.w1_ninv_shoup, bc.q);
+        }
     }
 #pragma unroll
     for (int j = 0; j < E; ++j) v.store(((t + Gr::TG * j) << n2) + col, x[j]);
