@@ -59,11 +59,11 @@ void modraise(Context& c, int target_level, const u64* poly_q0, u64* out, hipStr
     u64* S = (u64*)c.workspace(s, sizeof(u64) * N);
     {
         OneLimbCopy in{poly_q0, S, 0};
-        PlainJob o{LimbBatch{S, (long long)N, 1, 0, 1}, N};
+        const PlainJob o = plain_job(LimbBatch{S, (long long)N, 1, 0, 1}, N);
         run_ntt_inv(c, in, o, 1, s);
     }
     LiftJob lift{S, out, N, c.q[0], c.d_bc};
-    PlainJob rows{LimbBatch{out, (long long)L * (long long)N, L, 0, L}, N};
+    const PlainJob rows = plain_job(LimbBatch{out, (long long)L * (long long)N, L, 0, L}, N);
     run_ntt_fwd(c, lift, rows, L, s);
 }
 
@@ -157,12 +157,12 @@ void q0p0_keyswitch(Context& c, u64* b, u64* a, const u64* key, bool montgomery,
     u64* ct = S + 6 * N;
     {
         OneLimbCopy in{a, coeff, 0};
-        PlainJob o{LimbBatch{coeff, (long long)N, 1, 0, 1}, N};
+        const PlainJob o = plain_job(LimbBatch{coeff, (long long)N, 1, 0, 1}, N);
         run_ntt_inv(c, in, o, 1, s);
     }
     {
         ChangeJob in{coeff, moved, c.nq, c.q[0], c.p[0], c.h_bc[c.nq].mu_hi};
-        PlainJob o{LimbBatch{moved, (long long)N, 1, c.nq, 1}, N};
+        const PlainJob o = plain_job(LimbBatch{moved, (long long)N, 1, c.nq, 1}, N);
         run_ntt_fwd(c, in, o, 1, s);
     }
     Q0P0Args g{};

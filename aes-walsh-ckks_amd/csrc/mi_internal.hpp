@@ -136,6 +136,20 @@ struct Error : std::runtime_error {
             throw ::mi::Error(-3, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Division of a uniform index k < 2^20 by d < 2^20 as a multiply and shift
+// with m = ceil(2^40 / d): k*m / 2^40 exceeds k/d by less than 2^-20, which is
+// below the gap 1/d between frac(k/d) and 1, so the floor is exact.  Block
+// index arithmetic in the NTT jobs uses it so the quotient stays on the
+// scalar unit (a runtime '/' is expanded on the vector unit).
+struct FastDiv {
+    u64 m;
+    __host__ __device__ __forceinline__ int div(int k) const { return (int)(((u64)(unsigned)k * m) >> 40); }
+};
+inline FastDiv fast_div(int d) {
+    if (d < 1 || d >= (1 << 20)) throw Error(-1, "batch dimension out of range");
+    return FastDiv{((1ull << 40) + (u64)d - 1) / (u64)d};
+}
+
 // ---- NTT launchers (mi_ntt.hip) ----
 // A batch of limbs: limb k of the batch lives at base + (k / lpp) * stride +
 // (k % lpp) * N and uses basis index basis0 + (k % lpp).

@@ -36,6 +36,7 @@ struct CopyListJob {
     long long dstride;
     int n;
     u64 N;
+    FastDiv per_limb;  // n
     struct View {
         int basis;
         const u64* s;
@@ -44,7 +45,7 @@ struct CopyListJob {
         __device__ void store(int i, u64 v) const { d[i] = v; }
     };
     __device__ View view(int k) const {
-        const int l = k / n, it = k - l * n;
+        const int l = per_limb.div(k), it = k - l * n;
         return View{l, src[it] + (long long)l * N, dst + it * dstride + (long long)l * N};
     }
     __device__ bool skip(int) const { return false; }
@@ -55,6 +56,7 @@ struct ListJob {
     u64* p[2 * KS_MAX_ITEMS];
     int npoly, basis0;
     u64 N;
+    FastDiv per_limb;  // npoly
     struct View {
         int basis;
         u64* p;
@@ -62,7 +64,7 @@ struct ListJob {
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
     __device__ View view(int k) const {
-        const int l = k / npoly, poly = k - l * npoly;
+        const int l = per_limb.div(k), poly = k - l * npoly;
         return View{basis0 + l, p[poly] + (long long)l * N};
     }
     __device__ bool skip(int) const { return false; }
@@ -161,6 +163,7 @@ struct DigitJob {
     long long dstride;
     int L, K, nq, alpha, parts, n;
     u64 N;
+    FastDiv per_slot, per_item;  // parts * n, parts
     struct View {
         int basis;
         u64* p;
@@ -169,10 +172,9 @@ struct DigitJob {
     };
     // slot-major: every item's and digit's slot 0, then slot 1, ... (shared twiddles)
     __device__ void split(int k, int& l, int& it, int& d) const {
-        const int per = parts * n;
-        l = k / per;
-        const int r = k - l * per;
-        it = r / parts;
+        l = per_slot.div(k);
+        const int r = k - l * (parts * n);
+        it = per_item.div(r);
         d = r - it * parts;
     }
     __device__ View view(int k) const {
@@ -203,7 +205,8 @@ void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* 
         in.dstride = (long long)L * (long long)N;
         in.n = n;
         in.N = N;
-        PlainJob out{LimbBatch{C, (long long)L * (long long)N, L, 0, L * n}, N};
+        in.per_limb = fast_div(n);
+        const PlainJob out = plain_job(LimbBatch{C, (long long)L * (long long)N, L, 0, L * n}, N);
         run_ntt_inv(c, in, out, L * n, s);
     }
     int maxnd = 0;
@@ -245,7 +248,7 @@ void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* 
             d0 = d1;
         }
     }
-    DigitJob j{D, dstride, L, K, c.nq, c.alpha, parts, n, N};
+    DigitJob j{D, dstride, L, K, c.nq, c.alpha, parts, n, N, fast_div(parts * n), fast_div(parts)};
     run_ntt_fwd(c, j, j, parts * (L + K) * n, s);
 }
 
@@ -415,6 +418,7 @@ struct MdInJob {
     u64* tmp;
     int L, npoly;
     u64 N;
+    FastDiv per_limb;  // npoly
     struct View {
         int basis;
         u64* p;
@@ -422,7 +426,7 @@ struct MdInJob {
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
     __device__ View view(int k) const {
-        const int t = k / npoly, poly = k - t * npoly;
+        const int t = per_limb.div(k), poly = k - t * npoly;
         return View{t, tmp + ((long long)poly * L + t) * N};
     }
     __device__ bool skip(int) const { return false; }
@@ -435,6 +439,7 @@ struct MdOutJob {
     int L, npoly;
     u64 N;
     const BasisConst* bc;
+    FastDiv per_limb;  // npoly
     struct View {
         int basis;
         const u64* src;
@@ -445,7 +450,7 @@ struct MdOutJob {
         __device__ void store(int i, u64 v) const { out[i] = sub_mod(base[i], v, q); }
     };
     __device__ View view(int k) const {
-        const int t = k / npoly, poly = k - t * npoly;
+        const int t = per_limb.div(k), poly = k - t * npoly;
         return View{t, tmp + ((long long)poly * L + t) * N, base[poly] + (long long)t * N,
                     out[poly] + (long long)t * N, bc[t].q};
     }
@@ -462,6 +467,7 @@ void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* co
         ListJob j{};
         for (int k = 0; k < npoly; ++k) j.p[k] = p[k];
         j.npoly = npoly;
+        j.per_limb = fast_div(npoly);
         j.basis0 = c.nq;
         j.N = N;
         run_ntt_inv(c, j, j, K * npoly, s);
@@ -484,7 +490,7 @@ void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* co
         default: throw Error(-1, "more than 16 special moduli");
     }
     MI_HIP(hipGetLastError());
-    MdInJob in{T, L, npoly, N};
+    MdInJob in{T, L, npoly, N, fast_div(npoly)};
     MdOutJob o{};
     o.tmp = T;
     for (int k = 0; k < npoly; ++k) {
@@ -493,6 +499,7 @@ void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* co
     }
     o.L = L;
     o.npoly = npoly;
+    o.per_limb = fast_div(npoly);
     o.N = N;
     o.bc = c.d_bc;
     run_ntt_fwd(c, in, o, npoly * L, s);

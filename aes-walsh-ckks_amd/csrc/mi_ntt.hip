@@ -15,13 +15,13 @@ namespace mi {
 
 void ntt_forward(const Context& c, const LimbBatch& b, hipStream_t s) {
     if (b.count <= 0) return;
-    PlainJob j{b, c.N};
+    const PlainJob j = plain_job(b, c.N);
     run_ntt_fwd(c, j, j, b.count, s);
 }
 
 void ntt_inverse(const Context& c, const LimbBatch& b, hipStream_t s) {
     if (b.count <= 0) return;
-    PlainJob j{b, c.N};
+    const PlainJob j = plain_job(b, c.N);
     run_ntt_inv(c, j, j, b.count, s);
 }
 

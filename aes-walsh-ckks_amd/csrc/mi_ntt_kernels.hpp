@@ -36,6 +36,8 @@ inline NttTabs tabs_of(const Context& c) {
 struct PlainJob {
     LimbBatch b;
     u64 N;
+    int npolys;        // polys per modulus: b.count / b.lpp
+    FastDiv per_limb;  // npolys
     struct View {
         int basis;
         u64* p;
@@ -43,12 +45,14 @@ struct PlainJob {
         __device__ void store(int i, u64 v) const { p[i] = v; }
     };
     __device__ View view(int k) const {
-        const int npolys = b.count / b.lpp;
-        const int l = k / npolys, poly = k - l * npolys;
+        const int l = per_limb.div(k), poly = k - l * npolys;
         return View{b.basis0 + l, b.base + (long long)poly * b.stride + (long long)l * N};
     }
     __device__ bool skip(int) const { return false; }
 };
+inline PlainJob plain_job(const LimbBatch& b, u64 N) {
+    return PlainJob{b, N, b.count / b.lpp, fast_div(b.count / b.lpp)};
+}
 
 // Pass geometry, fixed at compile time by log2 N: n1 = n/2 column stages and
 // n2 = n - n1 row stages.  A pass whose groups have 2^logr elements and which
@@ -151,6 +155,7 @@ k_ntt_rows(NttTabs tb, Job job) {
 
 template <bool FWD, class Job>
 void launch_cols(const Context& c, const Job& job, int count, hipStream_t s) {
+    if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
 #define MI_COLS(LN)                                                                                   \
     hipLaunchKernelGGL((k_ntt_cols<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN / 2, LN - LN / 2)), \
@@ -162,6 +167,7 @@ void launch_cols(const Context& c, const Job& job, int count, hipStream_t s) {
 
 template <bool FWD, class Job>
 void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
+    if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
 #define MI_ROWS(LN)                                                                                   \
     hipLaunchKernelGGL((k_ntt_rows<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN - LN / 2, LN / 2)), \

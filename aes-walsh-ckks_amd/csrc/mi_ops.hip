@@ -18,15 +18,22 @@ static unsigned ew_blocks(long long work) {
     return (unsigned)(b > 0 ? b : 1);
 }
 
+// Element-wise launches over polys of `limbs` limbs: blockIdx.y is the poly,
+// blockIdx.x walks its limbs * N words.  N is a multiple of EW_THREADS, so a
+// block lies inside one limb: the limb index, its basis constants and its
+// scalar are uniform over the block (scalar loads, no per-thread division).
+static dim3 ew_grid(const Context& c, int limbs, int npolys) {
+    if (npolys > 65535) throw Error(-1, "too many polynomials in one element-wise launch");
+    return dim3((unsigned)(((long long)limbs << c.logN) / EW_THREADS), (unsigned)npolys);
+}
+__device__ __forceinline__ int ew_limb(int logN) { return (int)(((long long)blockIdx.x * EW_THREADS) >> logN); }
+
 __global__ void __launch_bounds__(EW_THREADS)
 k_ew_binary(const BasisConst* __restrict__ bc, int op, u64* x, const u64* __restrict__ y,
-            int logN, int limbs, long long xstride, long long ystride, long long total) {
-    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
-    if (i >= total) return;
-    const long long per = (long long)limbs << logN;
-    const long long k = i / per, r = i - k * per;
-    const int l = (int)(r >> logN);
-    const BasisConst b = bc[l];  // bc is pre-offset to the first limb's basis
+            int logN, long long xstride, long long ystride) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const long long k = blockIdx.y;
+    const BasisConst b = bc[ew_limb(logN)];  // bc is pre-offset to the first limb's basis
     u64* px = x + k * xstride + r;
     const u64 vy = y[k * ystride + r];
     const u64 vx = *px;
@@ -42,10 +49,9 @@ k_ew_binary(const BasisConst* __restrict__ bc, int op, u64* x, const u64* __rest
 
 void ew_binary(const Context& c, int op, u64* x, const u64* y, int limbs, int npolys,
                long long xstride, long long ystride, hipStream_t s, int basis0) {
-    long long total = (long long)npolys * limbs * (long long)c.N;
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_ew_binary, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
-                       y, c.logN, limbs, xstride, ystride, total);
+    if (npolys <= 0 || limbs <= 0) return;
+    hipLaunchKernelGGL(k_ew_binary, ew_grid(c, limbs, npolys), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
+                       y, c.logN, xstride, ystride);
     MI_HIP(hipGetLastError());
 }
 
@@ -82,13 +88,10 @@ struct Scalars {
 };
 
 __global__ void __launch_bounds__(EW_THREADS)
-k_ew_scalar(const BasisConst* __restrict__ bc, int op, u64* x, Scalars sc, int logN, int limbs,
-            long long stride, long long total) {
-    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
-    if (i >= total) return;
-    const long long per = (long long)limbs << logN;
-    const long long k = i / per, r = i - k * per;
-    const int l = (int)(r >> logN);
+k_ew_scalar(const BasisConst* __restrict__ bc, int op, u64* x, Scalars sc, int logN, long long stride) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const long long k = blockIdx.y;
+    const int l = ew_limb(logN);
     const BasisConst b = bc[l];
     u64* px = x + k * stride + r;
     const u64 s = sc.v[l];
@@ -99,12 +102,10 @@ k_ew_scalar(const BasisConst* __restrict__ bc, int op, u64* x, Scalars sc, int l
 // (fused_multiply_by_real_scalar_then_add_inplace, ckks_evalpoly.cpp:352-398)
 __global__ void __launch_bounds__(EW_THREADS)
 k_ew_scalar_mac(const BasisConst* __restrict__ bc, u64* dst, const u64* __restrict__ src, Scalars sc,
-                int logN, int limbs, long long dstride, long long sstride, long long total) {
-    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
-    if (i >= total) return;
-    const long long per = (long long)limbs << logN;
-    const long long k = i / per, r = i - k * per;
-    const int l = (int)(r >> logN);
+                int logN, long long dstride, long long sstride) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const long long k = blockIdx.y;
+    const int l = ew_limb(logN);
     const BasisConst b = bc[l];
     u64* pd = dst + k * dstride + r;
     *pd = add_mod(*pd, mul_mod(src[k * sstride + r], sc.v[l], b.q, b.mu_lo, b.mu_hi), b.q);
@@ -118,10 +119,9 @@ void ew_scalar_mac(const Context& c, u64* dst, const u64* src, const u64* scalar
         if (scalars_host[l] >= c.h_bc[l].q) throw Error(-1, "scalar residue is not reduced");
         sc.v[l] = scalars_host[l];
     }
-    long long total = (long long)npolys * limbs * (long long)c.N;
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_ew_scalar_mac, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, dst, src,
-                       sc, c.logN, limbs, dstride, sstride, total);
+    if (npolys <= 0 || limbs <= 0) return;
+    hipLaunchKernelGGL(k_ew_scalar_mac, ew_grid(c, limbs, npolys), dim3(EW_THREADS), 0, s, c.d_bc, dst, src,
+                       sc, c.logN, dstride, sstride);
     MI_HIP(hipGetLastError());
 }
 
@@ -133,10 +133,9 @@ void ew_scalar(const Context& c, int op, u64* x, const u64* scalars_host, int li
         if (scalars_host[l] >= c.h_bc[basis0 + l].q) throw Error(-1, "scalar residue is not reduced");
         sc.v[l] = scalars_host[l];
     }
-    long long total = (long long)npolys * limbs * (long long)c.N;
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_ew_scalar, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
-                       sc, c.logN, limbs, stride, total);
+    if (npolys <= 0 || limbs <= 0) return;
+    hipLaunchKernelGGL(k_ew_scalar, ew_grid(c, limbs, npolys), dim3(EW_THREADS), 0, s, c.d_bc + basis0, op, x,
+                       sc, c.logN, stride);
     MI_HIP(hipGetLastError());
 }
 
@@ -146,7 +145,7 @@ k_tensor(const BasisConst* __restrict__ bc, int logN, long long total, u64* b1, 
          const u64* __restrict__ b2, const u64* __restrict__ a2, u64* c2) {
     long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= total) return;
-    const BasisConst b = bc[i >> logN];
+    const BasisConst b = bc[ew_limb(logN)];
     const u64 vb1 = b1[i], va1 = a1[i], vb2 = b2[i], va2 = a2[i];
     c2[i] = mul_mod(va1, va2, b.q, b.mu_lo, b.mu_hi);
     b1[i] = mul_mod(vb1, vb2, b.q, b.mu_lo, b.mu_hi);
@@ -166,7 +165,7 @@ __global__ void __launch_bounds__(EW_THREADS)
 k_square(const BasisConst* __restrict__ bc, int logN, long long total, u64* b, u64* a, u64* c2) {
     long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= total) return;
-    const BasisConst k = bc[i >> logN];
+    const BasisConst k = bc[ew_limb(logN)];
     const u64 vb = b[i], va = a[i];
     c2[i] = mul_mod(va, va, k.q, k.mu_lo, k.mu_hi);
     b[i] = mul_mod(vb, vb, k.q, k.mu_lo, k.mu_hi);
@@ -187,7 +186,7 @@ k_add_plain_mont(const BasisConst* __restrict__ bc, int logN, long long total, u
                  const u64* __restrict__ pt) {
     long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= total) return;
-    const BasisConst k = bc[i >> logN];
+    const BasisConst k = bc[ew_limb(logN)];
     b[i] = add_mod(b[i], redc128(pt[i], 0, k.q, k.qninv), k.q);
 }
 
@@ -202,13 +201,11 @@ void add_plain_mont(const Context& c, int limbs, u64* b, const u64* pt, hipStrea
 __device__ __forceinline__ u32 brev_n(u32 x, int logN) { return __brev(x) >> (32 - logN); }
 
 __global__ void __launch_bounds__(EW_THREADS)
-k_automorphism(int logN, u64 galois, int limbs, const u64* __restrict__ src, u64* __restrict__ dst,
-               long long sstride, long long dstride, long long total) {
-    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
-    if (i >= total) return;
-    const long long per = (long long)limbs << logN;
-    const long long k = i / per, r = i - k * per;
-    const long long limb_off = r & ~(((long long)1 << logN) - 1);
+k_automorphism(int logN, u64 galois, const u64* __restrict__ src, u64* __restrict__ dst, long long sstride,
+               long long dstride) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const long long k = blockIdx.y;
+    const long long limb_off = (long long)ew_limb(logN) << logN;
     const u32 idx = (u32)(r & (((long long)1 << logN) - 1));
     const u32 mask2n = (2u << logN) - 1;
     const u32 d = brev_n(idx, logN);
@@ -219,15 +216,12 @@ k_automorphism(int logN, u64 galois, int limbs, const u64* __restrict__ src, u64
 
 // dst (+)= automorphism(src) over Q limbs 0..limbs-1 (canonical residues)
 __global__ void __launch_bounds__(EW_THREADS)
-k_automorphism_acc(const BasisConst* __restrict__ bc, int logN, u64 galois, int limbs,
-                   const u64* __restrict__ src, u64* __restrict__ dst, long long sstride, long long dstride,
-                   int accumulate, long long total) {
-    long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
-    if (i >= total) return;
-    const long long per = (long long)limbs << logN;
-    const long long k = i / per, r = i - k * per;
-    const int l = (int)(r >> logN);
-    const long long limb_off = r & ~(((long long)1 << logN) - 1);
+k_automorphism_acc(const BasisConst* __restrict__ bc, int logN, u64 galois, const u64* __restrict__ src,
+                   u64* __restrict__ dst, long long sstride, long long dstride, int accumulate) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const long long k = blockIdx.y;
+    const int l = ew_limb(logN);
+    const long long limb_off = (long long)l << logN;
     const u32 idx = (u32)(r & (((long long)1 << logN) - 1));
     const u32 mask2n = (2u << logN) - 1;
     const u32 d = brev_n(idx, logN);
@@ -241,10 +235,9 @@ k_automorphism_acc(const BasisConst* __restrict__ bc, int logN, u64 galois, int 
 void automorphism_acc(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                       long long sstride, long long dstride, bool accumulate, hipStream_t s) {
     if ((galois & 1) == 0) throw Error(-1, "Galois element must be odd");
-    long long total = (long long)npolys * limbs * (long long)c.N;
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_automorphism_acc, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.d_bc, c.logN,
-                       galois % (2 * c.N), limbs, src, dst, sstride, dstride, accumulate ? 1 : 0, total);
+    if (npolys <= 0 || limbs <= 0) return;
+    hipLaunchKernelGGL(k_automorphism_acc, ew_grid(c, limbs, npolys), dim3(EW_THREADS), 0, s, c.d_bc, c.logN,
+                       galois % (2 * c.N), src, dst, sstride, dstride, accumulate ? 1 : 0);
     MI_HIP(hipGetLastError());
 }
 
@@ -252,10 +245,9 @@ void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* 
                   long long sstride, long long dstride, hipStream_t s) {
     if (src == dst) throw Error(-1, "source and destination must differ");
     if ((galois & 1) == 0) throw Error(-1, "Galois element must be odd");
-    long long total = (long long)npolys * limbs * (long long)c.N;
-    if (total == 0) return;
-    hipLaunchKernelGGL(k_automorphism, dim3(ew_blocks(total)), dim3(EW_THREADS), 0, s, c.logN,
-                       galois % (2 * c.N), limbs, src, dst, sstride, dstride, total);
+    if (npolys <= 0 || limbs <= 0) return;
+    hipLaunchKernelGGL(k_automorphism, ew_grid(c, limbs, npolys), dim3(EW_THREADS), 0, s, c.logN,
+                       galois % (2 * c.N), src, dst, sstride, dstride);
     MI_HIP(hipGetLastError());
 }
 
@@ -270,6 +262,7 @@ struct RescaleInJob {
     u64 N;
     u64 p, half;
     const BasisConst* bc;
+    FastDiv per_limb;  // npolys
     struct View {
         int basis;
         const u64* top;
@@ -283,7 +276,7 @@ struct RescaleInJob {
         __device__ void store(int i, u64 v) const { out[i] = v; }
     };
     __device__ View view(int k) const {
-        const int i = k / npolys, poly = k - i * npolys;  // modulus-major
+        const int i = per_limb.div(k), poly = k - i * npolys;  // modulus-major
         const BasisConst b = bc[i];
         return View{i, polys + poly * stride + (long long)level * N,
                     scratch + ((long long)poly * level + i) * N, p, half, b.q, b.mu_hi};
@@ -302,6 +295,7 @@ struct RescaleOutJob {
     const u64* inv_p;
     const u64* inv_p_shoup;
     const BasisConst* bc;
+    FastDiv per_limb;  // npolys
     struct View {
         int basis;
         const u64* src;
@@ -315,7 +309,7 @@ struct RescaleOutJob {
         }
     };
     __device__ View view(int k) const {
-        const int i = k / npolys, poly = k - i * npolys;  // modulus-major
+        const int i = per_limb.div(k), poly = k - i * npolys;  // modulus-major
         return View{i, scratch + ((long long)poly * level + i) * N,
                     polys + poly * stride + (long long)i * N, half_ntt + (long long)i * N,
                     bc[i].q, inv_p[i], inv_p_shoup[i]};
@@ -333,9 +327,9 @@ void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hi
     u64* scratch = (u64*)c.workspace(s, sizeof(u64) * (size_t)npolys * level * N);
     const u64 p = c.q[level];
     const RescaleTables& rt = c.rescale[level];
-    RescaleInJob in{polys, stride, scratch, level, npolys, N, p, p >> 1, c.d_bc};
+    RescaleInJob in{polys, stride, scratch, level, npolys, N, p, p >> 1, c.d_bc, fast_div(npolys)};
     RescaleOutJob out{polys, stride, scratch, level, npolys, N, rt.half_ntt, rt.inv_p, rt.inv_p_shoup,
-                      c.d_bc};
+                      c.d_bc, fast_div(npolys)};
     run_ntt_fwd(c, in, out, npolys * level, s);
 }
 
