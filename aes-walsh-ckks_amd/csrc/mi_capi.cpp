@@ -631,6 +631,28 @@ int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t*
     });
 }
 
+int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
+                         const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
+                         const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,
+                         int pt_montgomery, uint64_t* acc_b, uint64_t* acc_a, uint64_t* raw_b, uint64_t* raw_a,
+                         uint64_t* raw_c2, void* stream) {
+    return guard("walsh_sum_for_output_pair", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (!low_b || !low_a || !high_b || !high_a || !cr || !ci || !const_abs || !const_neg)
+            throw mi::Error(MI_EINVAL, "null table");
+        check_ptr(pt); check_ptr(acc_b); check_ptr(acc_a); check_ptr(raw_b); check_ptr(raw_a); check_ptr(raw_c2);
+        const uint64_t* outs[5] = {acc_b, acc_a, raw_b, raw_a, raw_c2};
+        for (int m = 0; m < 15; ++m)
+            for (const uint64_t* o : outs)
+                if (o == low_b[m] || o == low_a[m] || o == high_b[m] || o == high_a[m] || o == pt)
+                    throw mi::Error(MI_EINVAL, "outputs must not alias the inputs");
+        mi::walsh_product_sum(*ctx->c, level, low_b, low_a, high_b, high_a, (const short*)cr, (const short*)ci,
+                              const_abs, (const int*)const_neg, pt, pt_montgomery != 0, acc_b, acc_a, raw_b,
+                              raw_a, raw_c2, S(stream));
+    });
+}
+
 int mi_graph_begin(mi_context* ctx, void* stream) {
     return guard("mi_graph_begin", [&] {
         check_ctx(ctx);

@@ -150,6 +150,15 @@ inline FastDiv fast_div(int d) {
     return FastDiv{((1ull << 40) + (u64)d - 1) / (u64)d};
 }
 
+// ---- Walsh S-box product sum (mi_walsh.hip) ----
+// 15 sign ciphertexts per nibble (masks 1..15), coefficient tables [16][16]
+// (int16, [high mask][low mask]), rounded constants |V| (lo, hi) and signs
+// for (h, real) at 2h and (h, imag) at 2h + 1.
+void walsh_product_sum(const Context& c, int level, const u64* const* low_b, const u64* const* low_a,
+                       const u64* const* high_b, const u64* const* high_a, const short* cr, const short* ci,
+                       const u64* v_abs, const int* v_neg, const u64* pt, bool pt_mont, u64* acc_b, u64* acc_a,
+                       u64* raw_b, u64* raw_a, u64* raw_c2, hipStream_t s);
+
 // ---- NTT launchers (mi_ntt.hip) ----
 // A batch of limbs: limb k of the batch lives at base + (k / lpp) * stride +
 // (k % lpp) * N and uses basis index basis0 + (k % lpp).

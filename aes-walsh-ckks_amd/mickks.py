@@ -72,6 +72,7 @@ _SIGS = {
     "mi_sample_gaussian": (cint, [vp, vp, cint, ctypes.c_double, u64, u64, vp]),
     "mi_modraise": (cint, [vp, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch": (cint, [vp, vp, vp, vp, cint, vp]),
+    "mi_walsh_product_sum": (cint, [vp, cint, vp, vp, vp, vp, vp, vp, vp, vp, vp, cint, vp, vp, vp, vp, vp, vp]),
     "mi_graph_begin": (cint, [vp, vp]),
     "mi_graph_end": (cint, [vp, vp, vp]),
     "mi_graph_launch": (cint, [vp, vp, vp]),
@@ -355,6 +356,35 @@ class Context:
 
     def q0p0_keyswitch(self, b, a, key, montgomery=True, stream=None):
         check(lib().mi_q0p0_keyswitch(self.h, ptr(b), ptr(a), ptr(key), int(montgomery), _stream(stream)))
+
+    # ---- Walsh S-box
+    def walsh_product_sum(self, level, low, high, cr, ci, consts, pt, acc, raw, pt_montgomery=True, stream=None):
+        """low, high: 15 (b, a) device tensors each (sign masks 1..15); cr, ci:
+        int16 [16][16] ([high mask][low mask]); consts: 16 pairs (V_real, V_imag)
+        of Python ints, the rounded scaled constants; pt: encoded i; acc: (b, a)
+        outputs; raw: (b, a, c2) outputs."""
+        import numpy as np
+        if len(low) != 15 or len(high) != 15:
+            raise ValueError("15 low and 15 high sign ciphertexts")
+        P15 = vp * 15
+        lb, la = P15(*[t[0].data_ptr() for t in low]), P15(*[t[1].data_ptr() for t in low])
+        hb, ha = P15(*[t[0].data_ptr() for t in high]), P15(*[t[1].data_ptr() for t in high])
+        cr = np.ascontiguousarray(np.asarray(cr, dtype=np.int16).reshape(16, 16))
+        ci = np.ascontiguousarray(np.asarray(ci, dtype=np.int16).reshape(16, 16))
+        absv = np.zeros(64, dtype=np.uint64)
+        neg = np.zeros(32, dtype=np.int32)
+        for h in range(16):
+            for r in range(2):
+                v = int(consts[h][r])
+                t = 2 * h + r
+                absv[2 * t] = abs(v) & (2**64 - 1)
+                absv[2 * t + 1] = abs(v) >> 64
+                neg[t] = 1 if v < 0 else 0
+        check(lib().mi_walsh_product_sum(self.h, level, ctypes.addressof(lb), ctypes.addressof(la),
+                                         ctypes.addressof(hb), ctypes.addressof(ha), cr.ctypes.data, ci.ctypes.data,
+                                         absv.ctypes.data, neg.ctypes.data, ptr(pt), int(pt_montgomery),
+                                         ptr(acc[0]), ptr(acc[1]), ptr(raw[0]), ptr(raw[1]), ptr(raw[2]),
+                                         _stream(stream)))
 
     # ---- HIP graphs
     def graph_begin(self, stream):

@@ -303,6 +303,28 @@ int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint
 int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
                       int montgomery, void* stream);
 
+/* ---- Walsh S-box ----
+ * The linear combinations and raw 4+4 products of one output pair of the
+ * Walsh S-box (walsh_sum_for_output_pair, ckks_walsh_sbox.cpp:984-1043, up
+ * to its relinearisation), fused into one pass over the signs.  All
+ * ciphertexts are at `level` (level+1 limbs, limb-major, evaluation domain).
+ * low_b/low_a and high_b/high_a: 15 pointers each, the sign ciphertexts of
+ * nibble masks 1..15.  cr, ci: int16 tables [16][16] indexed [high mask]
+ * [low mask] (the real and imaginary output bits' Walsh coefficients); the
+ * column [h][0] is unused: the constant term enters as the rounded scaled
+ * integer V (ckks_add_integer_constant_inplace) given as |V| in two words
+ * (lo, hi) in const_abs[4h + 2r .. 4h + 2r + 1] and its sign in
+ * const_neg[2h + r] (r = 0 real, 1 imaginary).  pt: the encoded i (Montgomery
+ * form if pt_montgomery).  With L(c) = sum_m c[m] low_m + V on b and
+ * Z_h = L(cr[h]) + pt * L(ci[h]):  acc = Z_0 (b, a) and
+ * raw = sum_{h=1..15} Z_h (x) high_h (b, a, c2).  Outputs must not alias
+ * the inputs. */
+int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
+                         const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
+                         const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,
+                         int pt_montgomery, uint64_t* acc_b, uint64_t* acc_a, uint64_t* raw_b, uint64_t* raw_a,
+                         uint64_t* raw_c2, void* stream);
+
 /* ---- HIP graph capture ----
  * No counterpart in the reference, whose evaluator runs eagerly on host
  * threads; these let a caller record a fixed sequence of the calls above (an

@@ -250,6 +250,43 @@ class Ctx:
         lib().oc_add_scalar_residues(self.h, level, _p(b), _p(res))
         return b
 
+    def walsh_product_sum(self, level, low, high, cr, ci, consts, pt, pt_mont=True, montgomery=True):
+        """walsh_sum_for_output_pair (ckks_walsh_sbox.cpp:984-1043) up to its
+        relinearisation, composed from this oracle's primitives in the
+        reference's order: linear_combination_with_constant (:862-905; the
+        constant by ckks_add_integer_constant_inplace's residues, each sign
+        term by a scaled add, :827-860), combine_real_imag (:931-958: the
+        imaginary sum times the encoded i, then added), mul_raw (:297-321: the
+        tensor) and the raw-product accumulation (add_term_inplace, :288-295).
+        low, high: 15 arrays (2, L*N) [b, a] for masks 1..15; cr, ci [16][16];
+        consts: 16 pairs (V_real, V_imag) of integers.  Returns (acc (2, L*N),
+        raw (3, L*N))."""
+        L = level + 1
+        W = L * self.N
+
+        def lc(coeffs, v):
+            out = np.zeros((2, W), dtype=np.uint64)
+            if v:
+                out[0] = self.add_scalar_residues(level, out[0], [v % int(self.q[l]) for l in range(L)])
+            for m in range(1, 16):
+                c = int(coeffs[m])
+                if c:
+                    out = self.add(level, out, self.mul_int(level, low[m - 1], c, montgomery))
+            return out
+
+        def z(h):
+            real = lc(cr[h], consts[h][0])
+            imag = self.mul_plain(level, lc(ci[h], consts[h][1]), pt, pt_mont)
+            return self.add(level, real, imag)
+
+        acc = z(0)
+        raw = None
+        for h in range(1, 16):
+            zb, za = z(h)
+            prod = np.stack(self.tensor(level, zb, za, high[h - 1][0], high[h - 1][1]))
+            raw = prod if raw is None else self.add(level, raw, prod)
+        return acc, raw
+
 
 def _ctx_extra():
     """Methods for hoisting, QP key switching, BSGS, ModRaise and q0p0."""

@@ -376,3 +376,33 @@ def test_lt_stage_zero_diagonals():
     keys = {1: d["key_rot1"], 2: d["key_rot2"], 4: d["key_rot4"]}
     ao, bo = ctx.lt_stage_qp(lv, a, b, list(range(6)), zeros, 2, keys)
     assert not ao.any() and not bo.any()
+
+
+def test_walsh_product_sum_composition_special_cases():
+    """The oracle's Walsh product sum (composed from the pinned scaled-add,
+    constant, plaintext and tensor primitives) reduces to a single tensor for
+    a unit coefficient table and is linear in the real coefficients."""
+    N = 1 << 10
+    p, q = O.find_primes([45], [40, 40, 40], N)
+    ctx = O.Ctx(N, q, p, 1)
+    rng = np.random.default_rng(17)
+    lv = 2
+    rnd = lambda: np.stack([rand_residues(rng, q, N), rand_residues(rng, q, N)])
+    low = [rnd() for _ in range(15)]
+    high = [rnd() for _ in range(15)]
+    pt = rand_residues(rng, q, N)
+    zero = np.zeros((16, 16), dtype=np.int64)
+    consts0 = [(0, 0)] * 16
+    cr = zero.copy()
+    cr[5][9] = 1
+    acc, raw = ctx.walsh_product_sum(lv, low, high, cr, zero, consts0, pt)
+    tb, ta, tc = ctx.tensor(lv, low[8][0], low[8][1], high[4][0], high[4][1])
+    assert not acc.any()
+    assert np.array_equal(raw, np.stack([tb, ta, tc]))
+    c1 = rng.integers(-64, 65, (16, 16))
+    c2 = rng.integers(-64, 65, (16, 16))
+    a1, r1 = ctx.walsh_product_sum(lv, low, high, c1, zero, consts0, pt)
+    a2, r2 = ctx.walsh_product_sum(lv, low, high, c2, zero, consts0, pt)
+    a12, r12 = ctx.walsh_product_sum(lv, low, high, c1 + c2, zero, consts0, pt)
+    assert np.array_equal(a12, ctx.add(lv, a1, a2))
+    assert np.array_equal(r12, ctx.add(lv, r1, r2))
