@@ -502,6 +502,27 @@ void ckks_rotate_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t
 CKKSCiphertext ckks_conjugate(const CKKSContext& context, const CKKSCiphertext& ct);
 void ckks_conjugate_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 
+// ------------------------------------------------------------ Walsh S-box
+// ckks_walsh_sbox.cpp.  Walsh coefficients / 4 of the AES S-box output bits,
+// [output bit][low mask][high mask] (build_walsh_table, :130-162).
+using CKKSWalshTable = std::array<std::array<std::array<int16_t, 16>, 16>, 8>;
+const CKKSWalshTable& ckks_aes_walsh_table();
+// walsh_sum_for_output_pair (:984-1043): the Walsh sum of output bits
+// (real_out_bit, imag_out_bit) over the 16 + 16 nibble sign ciphertexts (mask
+// 0 is the constant 1); the linear combinations, i-combination and raw 4+4
+// products run as one device pass, then one relinearisation.  All signs at
+// one level; the high signs' products must share one scale.
+CKKSCiphertext ckks_walsh_sum_for_output_pair(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
+                                              const MyVector<CKKSCiphertext>& high_signs, size_t real_out_bit,
+                                              size_t imag_out_bit, const CKKSEncoding& encoded_i);
+// walsh_sum_to_bit_pair (:1045-1078): z = (1/2 + i/2) - D / 128.
+CKKSCiphertext ckks_walsh_sum_to_bit_pair(const CKKSContext& context, CKKSCiphertext walsh_sum);
+// recombine_mask_packed_signs_to_paired_outputs (:1096-1150): the four output
+// pairs (bit p real, bit p + 4 imaginary) of one S-box.
+MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKSContext& context,
+                                                                      const MyVector<CKKSCiphertext>& low_signs,
+                                                                      const MyVector<CKKSCiphertext>& high_signs);
+
 // Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
 // coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].
 struct CKKSEvalPolyOptions {

@@ -383,6 +383,37 @@ PYBIND11_MODULE(ckks_py, m) {
           },
           py::keep_alive<0, 1>());
     m.def("ckks_batch_bootstrap_evalmod_inplace", &ckks_batch_bootstrap_evalmod_inplace);
+    // Walsh S-box helpers (lists of ciphertexts are cloned into the call)
+    auto ct_list = [](const py::list& l) {
+        MyVector<CKKSCiphertext> v;
+        v.reserve(l.size());
+        for (const auto& h : l) v.push_back(h.cast<const CKKSCiphertext&>().clone());
+        return v;
+    };
+    m.def("ckks_aes_walsh_table", []() { return ckks_aes_walsh_table(); });
+    m.def("ckks_walsh_sum_for_output_pair",
+          [ct_list](const CKKSContext& c, const py::list& low, const py::list& high, size_t real_bit, size_t imag_bit,
+                    const CKKSEncoding& encoded_i) {
+              return ckks_walsh_sum_for_output_pair(c, ct_list(low), ct_list(high), real_bit, imag_bit, encoded_i);
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_walsh_sum_to_bit_pair",
+          [](const CKKSContext& c, const CKKSCiphertext& ct) { return ckks_walsh_sum_to_bit_pair(c, ct.clone()); },
+          py::keep_alive<0, 1>());
+    // each returned ciphertext keeps the context alive (its device blocks
+    // are freed through it), as keep_alive<0, 1> does for single results
+    m.def("ckks_walsh_recombine_signs_to_paired_outputs",
+          [ct_list](const py::object& context, const py::list& low, const py::list& high) {
+              const CKKSContext& c = context.cast<const CKKSContext&>();
+              auto out = ckks_walsh_recombine_signs_to_paired_outputs(c, ct_list(low), ct_list(high));
+              py::list r;
+              for (auto& x : out) {
+                  py::object o = py::cast(std::move(x));
+                  py::detail::keep_alive_impl(o, context);
+                  r.append(o);
+              }
+              return r;
+          });
     m.def("rotation_automorphism_index", &rotation_automorphism_index);
     m.def("ckks_evalchebyshev_preconditioned",
           [](const CKKSContext& c, const CKKSCiphertext& x, std::vector<double> coeffs, double target_scale) {

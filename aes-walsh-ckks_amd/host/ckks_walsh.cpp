@@ -1,0 +1,234 @@
+// The Walsh S-box's output-pair sums (ckks_walsh_sbox.cpp): the AES S-box
+// Walsh table, the sum over 4+4 sign products of one output pair (its linear
+// combinations, i-combination and raw products fused on the device by
+// mi_walsh_product_sum), the map of a Walsh sum to a pair of output bits, and
+// the four pairs of one S-box.
+#include <cmath>
+#include <cstring>
+
+#include "ckks.hpp"
+
+namespace {
+
+constexpr size_t kMasks = 16;  // nibble values (ckks_walsh_sbox.cpp:38)
+
+// scales_close (ckks_walsh_sbox.cpp:164-172)
+bool scales_close(long double lhs, long double rhs, long double rel_tol = 0x1p-45L) {
+    if (!std::isfinite(lhs) || !std::isfinite(rhs)) return false;
+    const long double scale = std::max<long double>({1.0L, std::abs(lhs), std::abs(rhs)});
+    return std::abs(lhs - rhs) <= rel_tol * scale;
+}
+
+// GF(2^8) modulo x^8 + x^4 + x^3 + x + 1.
+uint8_t gf_mul(uint8_t a, uint8_t b) {
+    uint8_t r = 0;
+    while (b) {
+        if (b & 1) r ^= a;
+        a = (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1b : 0));
+        b >>= 1;
+    }
+    return r;
+}
+
+// The AES S-box from its definition (FIPS-197 5.1.1): the multiplicative
+// inverse (0 -> 0) followed by the affine map b_i ^ b_{i+4} ^ b_{i+5} ^
+// b_{i+6} ^ b_{i+7} ^ 0x63_i.
+std::array<uint8_t, 256> aes_sbox() {
+    std::array<uint8_t, 256> s{};
+    for (int x = 0; x < 256; ++x) {
+        uint8_t inv = 0;
+        for (int y = 1; y < 256 && x; ++y)
+            if (gf_mul((uint8_t)x, (uint8_t)y) == 1) {
+                inv = (uint8_t)y;
+                break;
+            }
+        uint8_t out = 0x63;
+        for (int i = 0; i < 8; ++i) {
+            const int bit = ((inv >> i) ^ (inv >> ((i + 4) & 7)) ^ (inv >> ((i + 5) & 7)) ^ (inv >> ((i + 6) & 7)) ^
+                             (inv >> ((i + 7) & 7))) & 1;
+            out ^= (uint8_t)(bit << i);
+        }
+        s[x] = out;
+    }
+    return s;
+}
+
+int parity(unsigned v) { return __builtin_popcount(v) & 1; }
+
+// ckks_add_integer_constant_inplace's rounded constant (ckks_addition.cpp:
+// 17-31) as an exact integer: |V| in two words and its sign.
+void rounded_constant(int64_t value, long double scale, uint64_t* abs_words, int32_t* neg) {
+    const long double v = std::round((long double)value * scale);
+    if (!std::isfinite(v) || std::abs(v) >= 0x1p127L)
+        throw std::invalid_argument("ckks_walsh_sbox: scaled constant out of range");
+    const long double a = std::abs(v);
+    const uint64_t hi = (uint64_t)(a / 0x1p64L);
+    const uint64_t lo = (uint64_t)(a - (long double)hi * 0x1p64L);
+    abs_words[0] = lo;
+    abs_words[1] = hi;
+    *neg = v < 0 ? 1 : 0;
+}
+
+bool ciphertext_is_zero(const CKKSCiphertext& ct) {
+    for (size_t k = 0; k < ct.getNumPolys(); ++k)
+        for (uint64_t w : ct.getPoly(k))
+            if (w) return false;
+    return true;
+}
+
+// scale_up_ciphertext_to_match_inplace with multiply_by_integer_scalar_inplace
+// (ckks_walsh_sbox.cpp:228-270): the ciphertext's scale is raised to
+// `target` by an integer factor.  The reference first tests for an all-zero
+// ciphertext (which only takes the new scale); a zero ciphertext times the
+// integer is zero too, so the test (a device-to-host copy here) is made only
+// where the integer path would throw.
+void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double target) {
+    if (scales_close(ct.getScale(), target)) {
+        ct.setScale(target);
+        return;
+    }
+    const char* err = nullptr;
+    const long double scalar = target / ct.getScale();
+    const long double rounded = std::round(scalar);
+    if (ct.getScale() > target)
+        err = "ckks_walsh_sbox: cannot scale down ciphertext for addition";
+    else if (rounded < 1.0L ||
+             std::abs(scalar - rounded) > std::max<long double>(1.0L, std::abs(scalar)) * 0x1p-20L)
+        err = "ckks_walsh_sbox: non-integral scalar scale-up";
+    else if (rounded > (long double)std::numeric_limits<int64_t>::max())
+        err = "ckks_walsh_sbox: scalar scale-up is too large";
+    if (err) {
+        if (!ciphertext_is_zero(ct)) throw std::invalid_argument(err);
+        ct.setScale(target);
+        return;
+    }
+    const int64_t s = (int64_t)rounded;
+    if (s != 1) ckks_mul_inplace(context, ct, s);
+    ct.setScale(target);
+}
+
+CKKSEncoding constant_encoding(const CKKSContext& context, Complex value, long double scale, size_t level) {
+    MyVector<Complex> slots(context.getParams().getSlots(), value);
+    return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
+}
+
+}  // namespace
+
+const CKKSWalshTable& ckks_aes_walsh_table() {
+    static const CKKSWalshTable table = [] {
+        const auto sbox = aes_sbox();
+        CKKSWalshTable t{};
+        for (size_t bit = 0; bit < 8; ++bit)
+            for (size_t lo = 0; lo < kMasks; ++lo)
+                for (size_t hi = 0; hi < kMasks; ++hi) {
+                    const unsigned mask = (unsigned)(lo | (hi << 4));
+                    int w = 0;
+                    for (unsigned x = 0; x < 256; ++x)
+                        w += ((sbox[x] >> bit) & 1) == parity(mask & x) ? 1 : -1;
+                    if (w % 4 != 0)
+                        throw std::runtime_error("ckks_walsh_sbox: AES Walsh coefficient is not divisible by four");
+                    t[bit][lo][hi] = (int16_t)(w / 4);
+                }
+        return t;
+    }();
+    return table;
+}
+
+CKKSCiphertext ckks_walsh_sum_for_output_pair(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
+                                              const MyVector<CKKSCiphertext>& high_signs, size_t real_out_bit,
+                                              size_t imag_out_bit, const CKKSEncoding& encoded_i) {
+    if (low_signs.size() != kMasks || high_signs.size() != kMasks)
+        throw std::invalid_argument("ckks_walsh_sbox: expected 16 low and 16 high sign ciphertexts");
+    if (real_out_bit >= 8 || imag_out_bit >= 8) throw std::invalid_argument("ckks_walsh_sbox: invalid output bit");
+    const CKKSCiphertext& ref = low_signs.front();
+    const size_t level = ref.getLevel();
+    if (level == 0) throw std::invalid_argument("ckks_walsh_sbox: insufficient level for 4+4 Walsh product");
+    // linear_combination_with_constant's result takes signs.front()'s metadata;
+    // mul_raw sets the low sum's scale to low_signs.front()'s (:297-321)
+    const long double S = ref.getScale();
+    // each product's scale; their sum keeps the largest (add_term_inplace)
+    long double P = 0.0L;
+    for (size_t m = 1; m < kMasks; ++m) {
+        const CKKSCiphertext& y = high_signs[m];
+        P = std::max(P, S * (scales_close(y.getScale(), S) ? y.getScale() : S));
+    }
+    for (size_t m = 1; m < kMasks; ++m) {
+        const CKKSCiphertext& x = low_signs[m];
+        const CKKSCiphertext& y = high_signs[m];
+        if (x.getLevel() != level || y.getLevel() != level || x.getNumPolys() != 2 || y.getNumPolys() != 2)
+            throw std::invalid_argument("ckks_walsh_sbox: sign ciphertexts must be 2-polynomial at one level");
+        if (x.getSecretOwner() != ref.getSecretOwner() || y.getSecretOwner() != ref.getSecretOwner() ||
+            x.getMessageEncodingState() != ref.getMessageEncodingState() ||
+            y.getMessageEncodingState() != ref.getMessageEncodingState())
+            throw std::invalid_argument("ckks_walsh_sbox: sign ciphertext metadata mismatch");
+        const long double pm = S * (scales_close(y.getScale(), S) ? y.getScale() : S);
+        if (!scales_close(pm, P))
+            throw std::invalid_argument("ckks_walsh_sbox: high sign scales differ (raw products would need alignment)");
+    }
+    if (encoded_i.getLevel() != level || encoded_i.getN() != ref.getN())
+        throw std::invalid_argument("ckks_walsh_sbox: encoded i level mismatch");
+
+    const auto& table = ckks_aes_walsh_table();
+    int16_t cr[kMasks][kMasks], ci[kMasks][kMasks];
+    uint64_t vabs[4 * kMasks];
+    int32_t vneg[2 * kMasks];
+    for (size_t h = 0; h < kMasks; ++h) {
+        for (size_t m = 0; m < kMasks; ++m) {
+            cr[h][m] = table[real_out_bit][m][h];
+            ci[h][m] = table[imag_out_bit][m][h];
+        }
+        rounded_constant(cr[h][0], S, &vabs[4 * h], &vneg[2 * h]);
+        rounded_constant(ci[h][0], S, &vabs[4 * h + 2], &vneg[2 * h + 1]);
+    }
+    const uint64_t *lb[15], *la[15], *hb[15], *ha[15];
+    for (size_t m = 1; m < kMasks; ++m) {
+        lb[m - 1] = low_signs[m].polyData(0);
+        la[m - 1] = low_signs[m].polyData(1);
+        hb[m - 1] = high_signs[m].polyData(0);
+        ha[m - 1] = high_signs[m].polyData(1);
+    }
+    CKKSCiphertext acc(context, 2, S, level, ref.getSecretOwner(), ref.getMessageEncodingState());
+    CKKSCiphertext raw(context, 3, P, level, ref.getSecretOwner(), ref.getMessageEncodingState());
+    mi_check(mi_walsh_product_sum(context.handle(), (int)level, lb, la, hb, ha, &cr[0][0], &ci[0][0], vabs, vneg,
+                                  encoded_i.data(), encoded_i.isMontgomeryForm() ? 1 : 0, acc.polyData(0),
+                                  acc.polyData(1), raw.polyData(0), raw.polyData(1), raw.polyData(2),
+                                  context.stream()),
+             "ckks_walsh_sbox");
+    // relin_inplace then add_term_inplace (:288-295, :323-328)
+    ckks_relin_hybrid_inplace(context, raw);
+    const long double target = std::max(acc.getScale(), raw.getScale());
+    scale_up_to(context, acc, target);
+    scale_up_to(context, raw, target);
+    ckks_add_inplace(context, acc, raw);
+    return acc;
+}
+
+CKKSCiphertext ckks_walsh_sum_to_bit_pair(const CKKSContext& context, CKKSCiphertext walsh_sum) {
+    if (walsh_sum.getLevel() == 0)
+        throw std::invalid_argument("ckks_walsh_sbox: Walsh sum must have one level for final normalization");
+    ckks_rescale(context, walsh_sum);
+    const long double output_scale = walsh_sum.getScale() * 128.0L;
+    walsh_sum.setScale(output_scale);
+    ckks_mul_inplace(context, walsh_sum, int64_t{-1});
+    const auto half = constant_encoding(context, Complex(0.5, 0.5), output_scale, walsh_sum.getLevel());
+    ckks_add_inplace(context, walsh_sum, half);
+    return walsh_sum;
+}
+
+MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKSContext& context,
+                                                                      const MyVector<CKKSCiphertext>& low_signs,
+                                                                      const MyVector<CKKSCiphertext>& high_signs) {
+    if (low_signs.size() != kMasks || high_signs.size() != kMasks ||
+        low_signs[1].getLevel() != high_signs[1].getLevel() ||
+        !scales_close(low_signs[1].getScale(), high_signs[1].getScale()))
+        throw std::runtime_error("ckks_walsh_sbox: low/high sign metadata mismatch");
+    if (low_signs[1].getLevel() == 0)
+        throw std::runtime_error("ckks_walsh_sbox: sign output level is too low for 4+4 products");
+    const auto encoded_i = constant_encoding(context, Complex(0.0, 1.0), 1.0L, low_signs[1].getLevel());
+    MyVector<CKKSCiphertext> outputs;
+    outputs.reserve(4);
+    for (size_t pair = 0; pair < 4; ++pair)
+        outputs.emplace_back(ckks_walsh_sum_to_bit_pair(
+            context, ckks_walsh_sum_for_output_pair(context, low_signs, high_signs, pair, pair + 4, encoded_i)));
+    return outputs;
+}

@@ -76,3 +76,126 @@ def test_walsh_product_sum_rejects_bad_arguments():
     with pytest.raises(M.MiError, match="level"):
         ctx.walsh_product_sum(2, low, high, cr, cr, consts, pt, acc, raw)
     torch.cuda.synchronize()
+
+
+def _walsh_context(H, seed=5):
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 12
+    c.mult_depth = 4
+    c.q0_bit = 60
+    c.q_bit = 40
+    c.p_bits = [60, 60, 60]
+    c.dnum = 2
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [1]
+    c.seed = seed
+    return H.CKKSContext(H.CKKSParams(c))
+
+
+def _signs(H, ctx, nibbles, level):
+    """16 sign ciphertexts of one nibble per slot: mask 0 the constant 1
+    (zero_like + 1, as unpack_mask_packed_signs builds it), mask m the
+    encryption of (-1)^parity(m & nibble)."""
+    out = []
+    for m in range(16):
+        vals = (1 - 2 * np.array([bin(m & int(v)).count("1") & 1 for v in nibbles])).astype(np.complex128)
+        ct = H.ckks_encrypt(ctx, vals, level)
+        if m == 0:
+            H.ckks_mul_int_inplace(ctx, ct, 0)
+            H.ckks_add_integer_constant_inplace(ctx, ct, 1)
+        out.append(ct)
+    return out
+
+
+def test_walsh_sum_for_output_pair_matches_the_reference_sequence():
+    """The fused output-pair sum equals, bit for bit and in scale, the
+    reference's sequence of host operations: linear combinations by scaled
+    additions and constants, the imaginary part times the encoded i, raw
+    tensors summed, one relinearisation, the integer scale-up of the h = 0
+    term and the final addition (ckks_walsh_sbox.cpp:288-328, 862-1043)."""
+    from helpers import host
+    H = host()
+    ctx = _walsh_context(H)
+    level = 3
+    slots = ctx.getParams().getSlots()
+    rng = np.random.default_rng(9)
+    x = rng.integers(0, 256, slots)
+    low, high = _signs(H, ctx, x & 15, level), _signs(H, ctx, x >> 4, level)
+    enc_i = H.ckks_encode(ctx, np.full(slots, 1j), level, 1.0, True)
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    rb, ib = 2, 6
+    fused = H.ckks_walsh_sum_for_output_pair(ctx, low, high, rb, ib, enc_i)
+
+    S = low[0].getScale()
+
+    def lc(coeffs):
+        out = low[0].clone()
+        H.ckks_mul_int_inplace(ctx, out, 0)
+        if coeffs[0]:
+            H.ckks_add_integer_constant_inplace(ctx, out, int(coeffs[0]))
+        for m in range(1, 16):
+            if coeffs[m]:
+                term = low[m].clone()
+                H.ckks_mul_int_inplace(ctx, term, int(coeffs[m]))
+                H.ckks_add_inplace(ctx, out, term)
+        return out
+
+    def z(h):
+        real, imag = lc(table[rb][:, h]), lc(table[ib][:, h])
+        H.ckks_mul_plain_inplace(ctx, imag, enc_i)
+        imag.setScale(S)
+        H.ckks_add_inplace(ctx, real, imag)
+        return real
+
+    acc = z(0)
+    raw = None
+    for h in range(1, 16):
+        zh = z(h)
+        zh.setScale(S)
+        H.ckks_mul_inplace(ctx, zh, high[h])
+        if raw is None:
+            raw = zh
+        else:
+            H.ckks_add_inplace(ctx, raw, zh)
+    H.ckks_relin_hybrid_inplace(ctx, raw)
+    factor = raw.getScale() / acc.getScale()
+    assert factor == float(int(factor))
+    H.ckks_mul_int_inplace(ctx, acc, int(factor))
+    acc.setScale(raw.getScale())
+    H.ckks_add_inplace(ctx, acc, raw)
+
+    assert fused.getNumPolys() == 2 and fused.getLevel() == level
+    assert fused.getScale() == acc.getScale()
+    for k in range(2):
+        assert np.array_equal(fused.getPoly(k), acc.getPoly(k)), k
+
+
+def test_walsh_recombine_decrypts_to_sbox_bits():
+    """Signs of random bytes through the four output pairs decrypt to the
+    AES S-box output bits (real: bit p, imaginary: bit p + 4)."""
+    from helpers import host
+    H = host()
+    ctx = _walsh_context(H, seed=6)
+    level = 3
+    slots = ctx.getParams().getSlots()
+    rng = np.random.default_rng(10)
+    x = rng.integers(0, 256, slots)
+    low, high = _signs(H, ctx, x & 15, level), _signs(H, ctx, x >> 4, level)
+    outs = H.ckks_walsh_recombine_signs_to_paired_outputs(ctx, low, high)
+    assert len(outs) == 4
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    # S(x) from the table (pinned against FIPS-197 values on the CPU)
+    par = np.array([[bin(m & int(v)).count("1") & 1 for v in range(256)] for m in range(256)])
+    masks = np.arange(256)
+    f = np.stack([4 * table[b][masks & 15, masks >> 4] for b in range(8)]) @ (1 - 2 * par)
+    sbox = ((f < 0).astype(np.int64) * (1 << np.arange(8))[:, None]).sum(axis=0)
+    sx = sbox[x]
+    for p, ct in enumerate(outs):
+        assert ct.getLevel() == level - 1
+        dec = H.ckks_decrypt(ctx, ct)
+        want = ((sx >> p) & 1) + 1j * ((sx >> (p + 4)) & 1)
+        err = np.max(np.abs(dec - want))
+        print(f"pair {p}: max error {err:.2e}", flush=True)
+        assert err < 0.02, (p, err)
