@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -433,6 +433,155 @@ def cpu_baseline_hoisted(N, q, p, level, gal, parts):
             "sample": "oracle hoist + 8 hoisted rotations at the same shape"}
 
 
+def bench_walsh(args, M, torch):
+    """The four output pairs of one Walsh S-box at the walsh15 shape: N=2^15,
+    the walsh15 chain with the sign ciphertexts at level 2 (36-, 61- and
+    42-bit limbs), P = 4 x 61 bits, dnum 3; 15 + 15 sign ciphertexts (seeded
+    residues), the AES Walsh table, the encoded i in Montgomery form.  One
+    step = for each pair, the fused product sum (linear combinations,
+    i-combination, 15 raw 4+4 products) and one relinearisation, captured in
+    a HIP graph.  The comparison runs the reference's structure of the same
+    pairs (ckks_walsh_sbox.cpp:862-1043: one scaled addition per sign term,
+    a plaintext product per imaginary part, a tensor and an addition per
+    product), also captured in a graph, and checks both agree bit for bit."""
+    import ctypes
+    H = _host()
+    N = 1 << 15
+    p, q = _primes([61] * 4, [36, 61, 42, 42] + [38] * 7 + [50, 50], N)
+    ctx = M.Context(N, q, p, 3)
+    level = 2
+    L = level + 1
+    W = L * N
+    parts = ctx.parts(level)
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    signs = _rand_limbs(torch, q[:L], N, 60, seed=901).reshape(60, W)
+    low = [(signs[2 * m], signs[2 * m + 1]) for m in range(15)]
+    high = [(signs[30 + 2 * m], signs[31 + 2 * m]) for m in range(15)]
+    pt = _rand_limbs(torch, q[:L], N, 1, seed=902)
+    key = _rand_limbs(torch, list(q) + list(p), N, 2 * parts, seed=903)
+    scale = 2.0 ** 38
+    pairs = []
+    for pr in range(4):
+        cr = table[pr].T.copy()        # [h][m]
+        ci = table[pr + 4].T.copy()
+        consts = [(int(round(float(cr[h][0]) * scale)), int(round(float(ci[h][0]) * scale))) for h in range(16)]
+        pairs.append((cr, ci, consts))
+    new = lambda n: torch.empty(n * W, dtype=torch.int64, device="cuda")
+    fused_out = [(new(2), new(3)) for _ in range(4)]
+    ref_out = [(new(2), new(3)) for _ in range(4)]
+    tmp_r, tmp_i, c2 = new(2), new(2), new(1)
+    stream = torch.cuda.Stream()
+    lib = M.lib()
+
+    def res(v):
+        return np.array([int(v) % int(qi) for qi in q[:L]], dtype=np.uint64)
+
+    def fused():
+        for (cr, ci, consts), (acc, raw) in zip(pairs, fused_out):
+            ctx.walsh_product_sum(level, low, high, cr, ci, consts, pt, (acc[:W], acc[W:]),
+                                  (raw[:W], raw[W:2 * W], raw[2 * W:]), pt_montgomery=True, stream=stream)
+            ctx.relin(level, raw[:W], raw[W:2 * W], raw[2 * W:], key, stream=stream)
+
+    def lc(out, coeffs, const):
+        M.check(lib.mi_memset_zero(ctx.h, M.ptr(out), out.numel() * 8, M._stream(stream)))
+        if const:
+            ctx.add_scalar(level, out[:W], res(const), stream=stream)
+        for m in range(1, 16):
+            c = int(coeffs[m])
+            if c:
+                src = signs[2 * (m - 1)]  # (b, a) of mask m are consecutive rows
+                r = res(c)
+                M.check(lib.mi_mul_scalar_add(ctx.h, level, M.ptr(out), M.ptr(src), 2, W, r.ctypes.data,
+                                              M._stream(stream)))
+
+    def reference_structure():
+        for (cr, ci, consts), (acc, raw) in zip(pairs, ref_out):
+            for h in range(16):
+                lc(tmp_r, cr[h], consts[h][0])
+                lc(tmp_i, ci[h], consts[h][1])
+                ctx.mul_plain(level, tmp_i, 2, pt, pt_montgomery=True, stride=W, stream=stream)
+                ctx.add(level, tmp_r, tmp_i, 2, stride=W, stream=stream)
+                if h == 0:
+                    acc.copy_(tmp_r)
+                    continue
+                hb, ha = high[h - 1]
+                ctx.tensor(level, tmp_r[:W], tmp_r[W:], hb, ha, c2, stream=stream)
+                if h == 1:
+                    raw[:2 * W].copy_(tmp_r)
+                    raw[2 * W:].copy_(c2)
+                else:
+                    ctx.add(level, raw[:2 * W], tmp_r, 2, stride=W, stream=stream)
+                    ctx.add(level, raw[2 * W:], c2, 1, stride=W, stream=stream)
+            ctx.relin(level, raw[:W], raw[W:2 * W], raw[2 * W:], key, stream=stream)
+
+    # rows 2m, 2m+1 of `signs` are low mask m+1 (b, a): lc reads them as one 2-poly block
+    assert all(low[m][0].data_ptr() + W * 8 == low[m][1].data_ptr() for m in range(15))
+
+    def capture(fn):
+        with torch.cuda.stream(stream):
+            fn()                      # eager once: sizes the workspaces
+        stream.synchronize()
+        ctx.graph_begin(stream)
+        with torch.cuda.stream(stream):
+            fn()
+        return ctx.graph_end(stream)
+
+    g_fused = capture(fused)
+    if args.plain:
+        g_ref = capture(reference_structure)
+        stream.synchronize()
+        for (a1, r1), (a2, r2) in zip(fused_out, ref_out):
+            if not (torch.equal(a1, a2) and torch.equal(r1[:2 * W], r2[:2 * W])):
+                raise RuntimeError("fused and reference-structured Walsh sums differ")
+
+    def timed(g, n):
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            g.launch(stream)
+        stream.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    for _ in range(args.warmup):
+        g_fused.launch(stream)
+    ms = timed(g_fused, args.steps)
+    ref_ms = timed(g_ref, max(1, args.steps // 4)) if args.plain else None
+    g_fused.destroy()
+    if args.plain:
+        g_ref.destroy()
+    cpu = cpu_baseline_walsh(N, q, p, level, pairs[0]) if args.cpu_baseline else None
+    return {
+        "metric": "Walsh S-box output-pair sums/s, walsh15 shape (N=2^15, signs at level 2)",
+        "value": 4 / (ms / 1e3), "unit": "output-pair/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
+        "config": {"workload": "4 output pairs of one S-box: fused product sums + relinearisations, HIP graph",
+                   "sbox_per_s": 1 / (ms / 1e3),
+                   "reference_structure_ms": ref_ms,
+                   "fused_over_reference_structure": (ref_ms / ms) if ref_ms else None},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_walsh(N, q, p, level, pair):
+    """The oracle's output-pair product sum (one pair, the same shape)."""
+    O = _oracle()
+    octx = O.Ctx(N, q, p, 3)
+    rng = np.random.default_rng(6)
+    L = level + 1
+    rnd = lambda: np.stack([np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in q[:L]])
+                            for _ in range(2)])
+    low = [rnd() for _ in range(15)]
+    high = [rnd() for _ in range(15)]
+    pt = rnd()[0]
+    cr, ci, consts = pair
+    t0 = time.perf_counter()
+    octx.walsh_product_sum(level, low, high, cr, ci, consts, pt, True)
+    dt = time.perf_counter() - t0
+    return {"value": 1 / dt, "unit": "output-pair/s", "cores": int(O.lib().oc_omp_threads()), "kind": "port",
+            "sample": "oracle product sum of one output pair (no relinearisation) at the same shape"}
+
+
 def bench_chain(args, M, torch):
     """The plain CKKS primitive chain through the C++ host API: N=2^14, 8 chain
     limbs of 50-bit primes, P = 2 x 50-bit, dnum 4, Hamming weight 192,
@@ -528,7 +677,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -546,7 +695,7 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
-           "hoisted": bench_hoisted}[args.workload](args, M, torch)
+           "hoisted": bench_hoisted, "walsh": bench_walsh}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 
