@@ -53,10 +53,19 @@ __device__ __forceinline__ u64 walsh_lc(const short* c, const int (&x0)[WALSH_MA
     return reduce128(lo, hi, b.q, b.mu_lo, b.mu_hi);
 }
 
+// A block holds 64 coefficients; its 4 waves each take 4 of the 16 high
+// masks (wave 0: h = 0..3, whose h = 0 is the acc term) for all 64, so a pair
+// launches 4 blocks per 256 coefficients.  The waves' partial sums of the raw
+// products meet in LDS.
+constexpr int WALSH_COEFFS = WALSH_THREADS / 4;
+
 __global__ void __launch_bounds__(WALSH_THREADS) k_walsh_product_sum(WalshArgs a) {
     __shared__ u64 s_const[2 * WALSH_MASKS];
-    const int l = (int)(((long long)blockIdx.x * WALSH_THREADS) >> a.logN);  // uniform: a block is in one limb
-    const long long i = (long long)blockIdx.x * WALSH_THREADS + threadIdx.x;
+    __shared__ u64 s_part[3][4][WALSH_COEFFS];
+    const int l = (int)(((long long)blockIdx.x * WALSH_COEFFS) >> a.logN);  // uniform: a block is in one limb
+    const int lane = threadIdx.x % WALSH_COEFFS;
+    const int quarter = __builtin_amdgcn_readfirstlane(threadIdx.x / WALSH_COEFFS);  // one wave per quarter
+    const long long i = (long long)blockIdx.x * WALSH_COEFFS + lane;
     const BasisConst b = a.bc[l];
     if (threadIdx.x < 2 * WALSH_MASKS) {
         const int t = threadIdx.x;
@@ -86,27 +95,35 @@ __global__ void __launch_bounds__(WALSH_THREADS) k_walsh_product_sum(WalshArgs a
         zb = add_mod(rb, pb, b.q);
         za = add_mod(ra, pa, b.q);
     };
-    {
-        u64 zb, za;
-        z_pair(0, zb, za);
-        a.acc_b[i] = zb;
-        a.acc_a[i] = za;
-    }
-    // 15 canonical products per 128-bit sum: 15 q^2 < 2^128 for q < 2^62
+    // at most 4 canonical products per 128-bit sum here
     U128 sb{0, 0}, sab{0, 0}, sba{0, 0}, sc{0, 0};
-    for (int h = 1; h < WALSH_MASKS; ++h) {
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        const int h = 4 * quarter + k;
         u64 zb, za;
         z_pair(h, zb, za);
+        if (h == 0) {
+            a.acc_b[i] = zb;
+            a.acc_a[i] = za;
+            continue;
+        }
         const u64 hb = a.high_b[h - 1][i], ha = a.high_a[h - 1][i];
         mac128(sb, zb, hb);
         mac128(sab, zb, ha);
         mac128(sba, za, hb);
         mac128(sc, za, ha);
     }
-    a.raw_b[i] = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
-    a.raw_a[i] = add_mod(reduce128(sab.lo, sab.hi, b.q, b.mu_lo, b.mu_hi),
-                         reduce128(sba.lo, sba.hi, b.q, b.mu_lo, b.mu_hi), b.q);
-    a.raw_c2[i] = reduce128(sc.lo, sc.hi, b.q, b.mu_lo, b.mu_hi);
+    s_part[0][quarter][lane] = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
+    s_part[1][quarter][lane] = add_mod(reduce128(sab.lo, sab.hi, b.q, b.mu_lo, b.mu_hi),
+                                       reduce128(sba.lo, sba.hi, b.q, b.mu_lo, b.mu_hi), b.q);
+    s_part[2][quarter][lane] = reduce128(sc.lo, sc.hi, b.q, b.mu_lo, b.mu_hi);
+    __syncthreads();
+    if (quarter < 3) {  // wave w sums raw poly w over the quarters
+        const u64 v = add_mod(add_mod(s_part[quarter][0][lane], s_part[quarter][1][lane], b.q),
+                              add_mod(s_part[quarter][2][lane], s_part[quarter][3][lane], b.q), b.q);
+        u64* out = quarter == 0 ? a.raw_b : quarter == 1 ? a.raw_a : a.raw_c2;
+        out[i] = v;
+    }
 }
 
 void walsh_product_sum(const Context& c, int level, const u64* const* low_b, const u64* const* low_a,
@@ -142,8 +159,8 @@ void walsh_product_sum(const Context& c, int level, const u64* const* low_b, con
     a.raw_c2 = raw_c2;
     a.logN = c.logN;
     a.bc = c.d_bc;
-    const long long words = (long long)(level + 1) << c.logN;  // a multiple of WALSH_THREADS
-    hipLaunchKernelGGL(k_walsh_product_sum, dim3((unsigned)(words / WALSH_THREADS)), dim3(WALSH_THREADS), 0, s,
+    const long long words = (long long)(level + 1) << c.logN;  // a multiple of WALSH_COEFFS
+    hipLaunchKernelGGL(k_walsh_product_sum, dim3((unsigned)(words / WALSH_COEFFS)), dim3(WALSH_THREADS), 0, s,
                        a);
     MI_HIP(hipGetLastError());
 }

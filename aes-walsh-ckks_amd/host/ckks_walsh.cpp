@@ -134,9 +134,15 @@ const CKKSWalshTable& ckks_aes_walsh_table() {
     return table;
 }
 
-CKKSCiphertext ckks_walsh_sum_for_output_pair(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
-                                              const MyVector<CKKSCiphertext>& high_signs, size_t real_out_bit,
-                                              size_t imag_out_bit, const CKKSEncoding& encoded_i) {
+namespace {
+
+// The device part of walsh_sum_for_output_pair: acc (2 polys, the h = 0 term)
+// and raw (3 polys, the raw products' sum) of one output pair.
+std::pair<CKKSCiphertext, CKKSCiphertext> walsh_product_sum(const CKKSContext& context,
+                                                            const MyVector<CKKSCiphertext>& low_signs,
+                                                            const MyVector<CKKSCiphertext>& high_signs,
+                                                            size_t real_out_bit, size_t imag_out_bit,
+                                                            const CKKSEncoding& encoded_i) {
     if (low_signs.size() != kMasks || high_signs.size() != kMasks)
         throw std::invalid_argument("ckks_walsh_sbox: expected 16 low and 16 high sign ciphertexts");
     if (real_out_bit >= 8 || imag_out_bit >= 8) throw std::invalid_argument("ckks_walsh_sbox: invalid output bit");
@@ -194,13 +200,26 @@ CKKSCiphertext ckks_walsh_sum_for_output_pair(const CKKSContext& context, const 
                                   acc.polyData(1), raw.polyData(0), raw.polyData(1), raw.polyData(2),
                                   context.stream()),
              "ckks_walsh_sbox");
-    // relin_inplace then add_term_inplace (:288-295, :323-328)
-    ckks_relin_hybrid_inplace(context, raw);
+    return {std::move(acc), std::move(raw)};
+}
+
+// add_term_inplace(acc, relinearised raw) (:288-295)
+CKKSCiphertext add_raw_term(const CKKSContext& context, CKKSCiphertext acc, CKKSCiphertext raw) {
     const long double target = std::max(acc.getScale(), raw.getScale());
     scale_up_to(context, acc, target);
     scale_up_to(context, raw, target);
     ckks_add_inplace(context, acc, raw);
     return acc;
+}
+
+}  // namespace
+
+CKKSCiphertext ckks_walsh_sum_for_output_pair(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
+                                              const MyVector<CKKSCiphertext>& high_signs, size_t real_out_bit,
+                                              size_t imag_out_bit, const CKKSEncoding& encoded_i) {
+    auto [acc, raw] = walsh_product_sum(context, low_signs, high_signs, real_out_bit, imag_out_bit, encoded_i);
+    ckks_relin_hybrid_inplace(context, raw);  // relin_inplace (:323-328)
+    return add_raw_term(context, std::move(acc), std::move(raw));
 }
 
 CKKSCiphertext ckks_walsh_sum_to_bit_pair(const CKKSContext& context, CKKSCiphertext walsh_sum) {
@@ -225,10 +244,27 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
     if (low_signs[1].getLevel() == 0)
         throw std::runtime_error("ckks_walsh_sbox: sign output level is too low for 4+4 products");
     const auto encoded_i = constant_encoding(context, Complex(0.0, 1.0), 1.0L, low_signs[1].getLevel());
+    // the four pairs are independent: their product sums first, then their
+    // relinearisations as one key-switch batch (one launch per phase)
+    std::vector<std::pair<CKKSCiphertext, CKKSCiphertext>> sums;
+    sums.reserve(4);
+    for (size_t pair = 0; pair < 4; ++pair)
+        sums.push_back(walsh_product_sum(context, low_signs, high_signs, pair, pair + 4, encoded_i));
+    if (low_signs.front().getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_relin: sparse-secret relinearization keys are not available");
+    mi_keyswitch_item items[4];
+    for (size_t pair = 0; pair < 4; ++pair) {
+        CKKSCiphertext& raw = sums[pair].second;
+        items[pair] = mi_keyswitch_item{raw.polyData(2), context.getRelinKeyDevice(), raw.polyData(0),
+                                        raw.polyData(1), (int32_t)raw.getLevel(), 0, 0, 0};
+    }
+    mi_check(mi_keyswitch_batch(context.handle(), items, 4, context.getParams().montgomery ? 1 : 0, context.stream()),
+             "ckks_relin");
     MyVector<CKKSCiphertext> outputs;
     outputs.reserve(4);
-    for (size_t pair = 0; pair < 4; ++pair)
-        outputs.emplace_back(ckks_walsh_sum_to_bit_pair(
-            context, ckks_walsh_sum_for_output_pair(context, low_signs, high_signs, pair, pair + 4, encoded_i)));
+    for (auto& [acc, raw] : sums) {
+        raw.resizePolys(2);
+        outputs.emplace_back(ckks_walsh_sum_to_bit_pair(context, add_raw_term(context, std::move(acc), std::move(raw))));
+    }
     return outputs;
 }
