@@ -16,7 +16,6 @@
 
 namespace mi {
 
-constexpr int WALSH_THREADS = 256;
 constexpr int WALSH_MASKS = 16;
 
 struct WalshArgs {
@@ -26,8 +25,8 @@ struct WalshArgs {
     const u64* high_a[WALSH_MASKS - 1];
     const u64* pt;                      // L limbs, evaluation domain
     u64 *acc_b, *acc_a, *raw_b, *raw_a, *raw_c2;
-    short cr[WALSH_MASKS][WALSH_MASKS];  // [h][m]
-    short ci[WALSH_MASKS][WALSH_MASKS];
+    int cr[WALSH_MASKS][WALSH_MASKS];  // [h][m] (32-bit: scalar loads)
+    int ci[WALSH_MASKS][WALSH_MASKS];
     u64 v_lo[2 * WALSH_MASKS], v_hi[2 * WALSH_MASKS];  // |V| of (h, real) at 2h, (h, imag) at 2h+1
     unsigned v_neg;                                     // sign bit per entry
     int logN, pt_mont;
@@ -38,7 +37,7 @@ struct WalshArgs {
 // halves: two signed 32x32+64 multiply-adds per term, |partial sums| < 2^50,
 // shifted non-negative by off (a multiple of q above 2^50), then one 128-bit
 // reduction of (A1 + off) 2^31 + (A0 + off).
-__device__ __forceinline__ u64 walsh_lc(const short* c, const int (&x0)[WALSH_MASKS - 1],
+__device__ __forceinline__ u64 walsh_lc(const int* c, const int (&x0)[WALSH_MASKS - 1],
                                         const int (&x1)[WALSH_MASKS - 1], u64 off, const BasisConst& b) {
     long long a0 = 0, a1 = 0;
 #pragma unroll
@@ -53,18 +52,22 @@ __device__ __forceinline__ u64 walsh_lc(const short* c, const int (&x0)[WALSH_MA
     return reduce128(lo, hi, b.q, b.mu_lo, b.mu_hi);
 }
 
-// A block holds 64 coefficients; its 4 waves each take 4 of the 16 high
-// masks (wave 0: h = 0..3, whose h = 0 is the acc term) for all 64, so a pair
-// launches 4 blocks per 256 coefficients.  The waves' partial sums of the raw
-// products meet in LDS.
-constexpr int WALSH_COEFFS = WALSH_THREADS / 4;
+// A block holds 64 coefficients in 8 waves: side (b or a) x quarter of the
+// 16 high masks (quarter 0: h = 0..3, whose h = 0 is the acc term).  A wave
+// forms its side of Z_h (b: L(cr) + const + pt L(ci), a: the same without the
+// constants) for its 4 masks and the products that side owns:
+// b-side Z_b H_b (raw b) and Z_b H_a, a-side Z_a H_b and Z_a H_a (raw c2);
+// the partial sums meet in LDS.
+constexpr int WALSH_COEFFS = 64;
+constexpr int WALSH_BLOCK = 8 * WALSH_COEFFS;
 
-__global__ void __launch_bounds__(WALSH_THREADS) k_walsh_product_sum(WalshArgs a) {
+__global__ void __launch_bounds__(WALSH_BLOCK) k_walsh_product_sum(WalshArgs a) {
     __shared__ u64 s_const[2 * WALSH_MASKS];
-    __shared__ u64 s_part[3][4][WALSH_COEFFS];
+    __shared__ u64 s_part[2][2][4][WALSH_COEFFS];  // [side][product][quarter][coefficient]
     const int l = (int)(((long long)blockIdx.x * WALSH_COEFFS) >> a.logN);  // uniform: a block is in one limb
     const int lane = threadIdx.x % WALSH_COEFFS;
-    const int quarter = __builtin_amdgcn_readfirstlane(threadIdx.x / WALSH_COEFFS);  // one wave per quarter
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WALSH_COEFFS);
+    const int side = wave >> 2, quarter = wave & 3;
     const long long i = (long long)blockIdx.x * WALSH_COEFFS + lane;
     const BasisConst b = a.bc[l];
     if (threadIdx.x < 2 * WALSH_MASKS) {
@@ -74,56 +77,45 @@ __global__ void __launch_bounds__(WALSH_THREADS) k_walsh_product_sum(WalshArgs a
     }
     // q * ceil(2^50 / q): a multiple of q above every partial sum's magnitude
     const u64 off = b.q * (((1ull << 50) + b.q - 1) / b.q);
-    int lb0[WALSH_MASKS - 1], lb1[WALSH_MASKS - 1], la0[WALSH_MASKS - 1], la1[WALSH_MASKS - 1];
+    int x0[WALSH_MASKS - 1], x1[WALSH_MASKS - 1];
 #pragma unroll
     for (int m = 0; m < WALSH_MASKS - 1; ++m) {
-        const u64 xb = a.low_b[m][i], xa = a.low_a[m][i];
-        lb0[m] = (int)(xb & 0x7fffffffu);
-        lb1[m] = (int)(xb >> 31);
-        la0[m] = (int)(xa & 0x7fffffffu);
-        la1[m] = (int)(xa >> 31);
+        const u64 x = (side ? a.low_a[m] : a.low_b[m])[i];
+        x0[m] = (int)(x & 0x7fffffffu);
+        x1[m] = (int)(x >> 31);
     }
     const u64 pt = a.pt[i];
     __syncthreads();
-    auto z_pair = [&](int h, u64& zb, u64& za) {
-        const u64 rb = add_mod(walsh_lc(a.cr[h], lb0, lb1, off, b), s_const[2 * h], b.q);
-        const u64 ra = walsh_lc(a.cr[h], la0, la1, off, b);
-        const u64 ib = add_mod(walsh_lc(a.ci[h], lb0, lb1, off, b), s_const[2 * h + 1], b.q);
-        const u64 ia = walsh_lc(a.ci[h], la0, la1, off, b);
-        const u64 pb = a.pt_mont ? mont_mul(ib, pt, b.q, b.qninv) : mul_mod(ib, pt, b.q, b.mu_lo, b.mu_hi);
-        const u64 pa = a.pt_mont ? mont_mul(ia, pt, b.q, b.qninv) : mul_mod(ia, pt, b.q, b.mu_lo, b.mu_hi);
-        zb = add_mod(rb, pb, b.q);
-        za = add_mod(ra, pa, b.q);
-    };
     // at most 4 canonical products per 128-bit sum here
-    U128 sb{0, 0}, sab{0, 0}, sba{0, 0}, sc{0, 0};
+    U128 s_h0{0, 0}, s_h1{0, 0};  // side's Z times H_b, times H_a
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         const int h = 4 * quarter + k;
-        u64 zb, za;
-        z_pair(h, zb, za);
+        u64 r = walsh_lc(a.cr[h], x0, x1, off, b);
+        u64 im = walsh_lc(a.ci[h], x0, x1, off, b);
+        if (side == 0) {
+            r = add_mod(r, s_const[2 * h], b.q);
+            im = add_mod(im, s_const[2 * h + 1], b.q);
+        }
+        const u64 p = a.pt_mont ? mont_mul(im, pt, b.q, b.qninv) : mul_mod(im, pt, b.q, b.mu_lo, b.mu_hi);
+        const u64 z = add_mod(r, p, b.q);
         if (h == 0) {
-            a.acc_b[i] = zb;
-            a.acc_a[i] = za;
+            (side ? a.acc_a : a.acc_b)[i] = z;
             continue;
         }
-        const u64 hb = a.high_b[h - 1][i], ha = a.high_a[h - 1][i];
-        mac128(sb, zb, hb);
-        mac128(sab, zb, ha);
-        mac128(sba, za, hb);
-        mac128(sc, za, ha);
+        mac128(s_h0, z, a.high_b[h - 1][i]);
+        mac128(s_h1, z, a.high_a[h - 1][i]);
     }
-    s_part[0][quarter][lane] = reduce128(sb.lo, sb.hi, b.q, b.mu_lo, b.mu_hi);
-    s_part[1][quarter][lane] = add_mod(reduce128(sab.lo, sab.hi, b.q, b.mu_lo, b.mu_hi),
-                                       reduce128(sba.lo, sba.hi, b.q, b.mu_lo, b.mu_hi), b.q);
-    s_part[2][quarter][lane] = reduce128(sc.lo, sc.hi, b.q, b.mu_lo, b.mu_hi);
+    s_part[side][0][quarter][lane] = reduce128(s_h0.lo, s_h0.hi, b.q, b.mu_lo, b.mu_hi);
+    s_part[side][1][quarter][lane] = reduce128(s_h1.lo, s_h1.hi, b.q, b.mu_lo, b.mu_hi);
     __syncthreads();
-    if (quarter < 3) {  // wave w sums raw poly w over the quarters
-        const u64 v = add_mod(add_mod(s_part[quarter][0][lane], s_part[quarter][1][lane], b.q),
-                              add_mod(s_part[quarter][2][lane], s_part[quarter][3][lane], b.q), b.q);
-        u64* out = quarter == 0 ? a.raw_b : quarter == 1 ? a.raw_a : a.raw_c2;
-        out[i] = v;
-    }
+    auto sum4 = [&](int sd, int pr) {
+        return add_mod(add_mod(s_part[sd][pr][0][lane], s_part[sd][pr][1][lane], b.q),
+                       add_mod(s_part[sd][pr][2][lane], s_part[sd][pr][3][lane], b.q), b.q);
+    };
+    if (wave == 0) a.raw_b[i] = sum4(0, 0);                                  // Z_b H_b
+    else if (wave == 1) a.raw_a[i] = add_mod(sum4(0, 1), sum4(1, 0), b.q);   // Z_b H_a + Z_a H_b
+    else if (wave == 2) a.raw_c2[i] = sum4(1, 1);                            // Z_a H_a
 }
 
 void walsh_product_sum(const Context& c, int level, const u64* const* low_b, const u64* const* low_a,
@@ -160,8 +152,7 @@ void walsh_product_sum(const Context& c, int level, const u64* const* low_b, con
     a.logN = c.logN;
     a.bc = c.d_bc;
     const long long words = (long long)(level + 1) << c.logN;  // a multiple of WALSH_COEFFS
-    hipLaunchKernelGGL(k_walsh_product_sum, dim3((unsigned)(words / WALSH_COEFFS)), dim3(WALSH_THREADS), 0, s,
-                       a);
+    hipLaunchKernelGGL(k_walsh_product_sum, dim3((unsigned)(words / WALSH_COEFFS)), dim3(WALSH_BLOCK), 0, s, a);
     MI_HIP(hipGetLastError());
 }
 
