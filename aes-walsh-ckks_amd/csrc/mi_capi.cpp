@@ -453,6 +453,26 @@ int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, vo
     });
 }
 
+int mi_rotate_batch(mi_context* ctx, int level, int count, uint64_t* const* b, uint64_t* const* a,
+                    const uint64_t* galois, const uint64_t* const* keys, int montgomery, void* stream) {
+    return guard("ckks_rotate", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1 || !b || !a || !galois || !keys) throw mi::Error(MI_EINVAL, "invalid batch");
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        for (int i = 0; i < count; ++i) {
+            check_galois(galois[i], false);
+            check_ptr(b[i]); check_ptr(a[i]); check_ptr(keys[i]);
+            for (int j = 0; j < i; ++j)  // rotated in place: the ciphertexts must be distinct
+                for (const uint64_t* x : {b[i], a[i]})
+                    for (const uint64_t* y : {b[j], a[j]})
+                        if (x < y + W && y < x + W) throw mi::Error(MI_EINVAL, "ciphertexts of a batch overlap");
+            if (b[i] < a[i] + W && a[i] < b[i] + W) throw mi::Error(MI_EINVAL, "b and a overlap");
+        }
+        mi::rotate_batch(*ctx->c, level, count, b, a, galois, keys, montgomery != 0, S(stream));
+    });
+}
+
 int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
                             const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
                             uint64_t* const* a_out, int montgomery, void* stream) {

@@ -270,6 +270,8 @@ void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key,
 // Rotations i < n of one hoisted ciphertext (digits of a, b): (b_out[i],
 // a_out[i]) = g_i(b + KS_i(a)) with galois[i] and keys[i]; outputs must not
 // overlap digits or b.
+void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* galois,
+                  const u64* const* keys, bool montgomery, hipStream_t s);
 void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
                           const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
                           hipStream_t s);

@@ -767,6 +767,47 @@ void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b
     }
 }
 
+// In-place rotations of independent ciphertexts at one level
+// (ckks_rotate_inplace, ckks_rotation.cpp:41-96, for each): per chunk of up to
+// KS_MAX_ITEMS, b is staged next to the switched a, the key switches of all a
+// run as one batch (b Add, a Assign), and one launch applies each rotation's
+// automorphism from the staging area back into the ciphertexts.
+void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* galois,
+                  const u64* const* keys, bool montgomery, hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
+    if (n < 1) throw Error(-1, "need at least one rotation");
+    const int L = level + 1;
+    const u64 N = c.N;
+    const long long LN = (long long)L * (long long)N;
+    const int chunk = std::min(n, KS_MAX_ITEMS);
+    const size_t kw = ks_item_words(c, level);
+    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (kw + 2 * LN));
+    u64* SW = ws + (size_t)chunk * kw;  // per item: a' then b'
+    for (int i0 = 0; i0 < n; i0 += chunk) {
+        const int m = std::min(chunk, n - i0);
+        KsItem items[KS_MAX_ITEMS];
+        for (int it = 0; it < m; ++it) {
+            u64* a_sw = SW + it * 2 * LN;
+            u64* b_sw = a_sw + LN;
+            MI_HIP(hipMemcpyAsync(b_sw, b[i0 + it], sizeof(u64) * LN, hipMemcpyDeviceToDevice, s));
+            items[it] = KsItem{a[i0 + it], keys[i0 + it], b_sw, a_sw, level, 0, 1};
+        }
+        keyswitch_items_with(c, items, m, montgomery, ws, s);
+        AutoItemsArgs aa{};
+        aa.src = SW;
+        for (int it = 0; it < m; ++it) {
+            aa.galois[it] = galois[i0 + it] % (2 * N);
+            aa.dst_a[it] = a[i0 + it];
+            aa.dst_b[it] = b[i0 + it];
+        }
+        aa.L = L;
+        aa.logN = c.logN;
+        hipLaunchKernelGGL(k_automorphism_items, dim3((unsigned)(m * 2 * L * (N / KS_THREADS))), dim3(KS_THREADS),
+                           0, s, aa);
+        MI_HIP(hipGetLastError());
+    }
+}
+
 void keyswitch_hoisted_qp(Context& c, int level, const u64* digits, const u64* key, const u64* b_q,
                           u64* b_qp, u64* a_qp, u64 galois, bool montgomery, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");

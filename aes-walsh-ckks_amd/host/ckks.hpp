@@ -523,6 +523,19 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
                                                                       const MyVector<CKKSCiphertext>& low_signs,
                                                                       const MyVector<CKKSCiphertext>& high_signs);
 
+// build_mask_packed_parity_input (:521-546): the 30 nibble parity sums of
+// the 8 input bits, each rotated by its segment * block_count (one batched
+// rotation) and summed.  The caller bootstraps the result next, so the
+// reference's requirement that the bits sit at the S2C entry level is left
+// to it.
+CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& context,
+                                                         const MyVector<CKKSCiphertext>& input, size_t block_count);
+// unpack_mask_packed_signs (:551-596): the 16 low and 16 high sign
+// ciphertexts (mask 0: the constant 1) from the packed signs, with one hoist
+// and the 29 segment rotations as one batch.
+std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_mask_packed_signs(
+    const CKKSContext& context, const CKKSCiphertext& packed, size_t block_count);
+
 // Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
 // coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].
 struct CKKSEvalPolyOptions {

@@ -402,17 +402,31 @@ PYBIND11_MODULE(ckks_py, m) {
           py::keep_alive<0, 1>());
     // each returned ciphertext keeps the context alive (its device blocks
     // are freed through it), as keep_alive<0, 1> does for single results
+    auto ct_pylist = [](MyVector<CKKSCiphertext>& v, const py::object& context) {
+        py::list r;
+        for (auto& x : v) {
+            py::object o = py::cast(std::move(x));
+            py::detail::keep_alive_impl(o, context);
+            r.append(o);
+        }
+        return r;
+    };
     m.def("ckks_walsh_recombine_signs_to_paired_outputs",
-          [ct_list](const py::object& context, const py::list& low, const py::list& high) {
+          [ct_list, ct_pylist](const py::object& context, const py::list& low, const py::list& high) {
               const CKKSContext& c = context.cast<const CKKSContext&>();
               auto out = ckks_walsh_recombine_signs_to_paired_outputs(c, ct_list(low), ct_list(high));
-              py::list r;
-              for (auto& x : out) {
-                  py::object o = py::cast(std::move(x));
-                  py::detail::keep_alive_impl(o, context);
-                  r.append(o);
-              }
-              return r;
+              return ct_pylist(out, context);
+          });
+    m.def("ckks_walsh_build_mask_packed_parity_input",
+          [ct_list](const CKKSContext& c, const py::list& input, size_t block_count) {
+              return ckks_walsh_build_mask_packed_parity_input(c, ct_list(input), block_count);
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_walsh_unpack_mask_packed_signs",
+          [ct_pylist](const py::object& context, const CKKSCiphertext& packed, size_t block_count) {
+              const CKKSContext& c = context.cast<const CKKSContext&>();
+              auto [low, high] = ckks_walsh_unpack_mask_packed_signs(c, packed, block_count);
+              return py::make_tuple(ct_pylist(low, context), ct_pylist(high, context));
           });
     m.def("rotation_automorphism_index", &rotation_automorphism_index);
     m.def("ckks_evalchebyshev_preconditioned",

@@ -268,3 +268,141 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
     }
     return outputs;
 }
+
+namespace {
+
+constexpr size_t kSegments = 30;  // kPackedWalshSignSegments: 15 low + 15 high masks
+
+// packed_walsh_segment (ckks_walsh_sbox.cpp:476-482)
+size_t packed_segment(bool high_nibble, size_t mask) { return (high_nibble ? kMasks - 1 : 0) + (mask - 1); }
+
+// parity_sum (ckks_walsh_sbox.cpp:363-383): the sum of the masked input bits
+CKKSCiphertext parity_sum(const CKKSContext& context, const MyVector<CKKSCiphertext>& input, size_t bit_offset,
+                          size_t mask) {
+    std::optional<CKKSCiphertext> out;
+    for (size_t bit = 0; bit < 4; ++bit) {
+        if (((mask >> bit) & 1) == 0) continue;
+        const CKKSCiphertext& term = input[bit_offset + bit];
+        if (!out) out.emplace(term.clone());
+        else ckks_add_inplace(context, *out, term);
+    }
+    if (!out) throw std::invalid_argument("ckks_walsh_sbox: parity mask must be nonzero");
+    return std::move(*out);
+}
+
+CKKSCiphertext constant_one_like(const CKKSContext& context, const CKKSCiphertext& ref) {
+    auto out = ref.clone();  // zero_like (:216-220), then the constant 1
+    ckks_mul_inplace(context, out, int64_t{0});
+    ckks_add_integer_constant_inplace(context, out, int64_t{1});
+    return out;
+}
+
+void check_segment_keys(const CKKSContext& context, size_t block_count) {
+    const size_t slots = context.getParams().getSlots();
+    if (block_count == 0) throw std::invalid_argument("ckks_walsh_sbox: mask-packed block count must be nonzero");
+    if (block_count * kSegments > slots)
+        throw std::invalid_argument("ckks_walsh_sbox: mask-packed block count exceeds slot capacity");
+    for (size_t segment = 1; segment < kSegments; ++segment) {
+        const int64_t off = (int64_t)(segment * block_count);
+        if (!context.hasRotationKey(off) || !context.hasRotationKey(-off))
+            throw std::invalid_argument("ckks_walsh_sbox: missing mask-packed segment rotation key");
+    }
+}
+
+}  // namespace
+
+CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& context,
+                                                         const MyVector<CKKSCiphertext>& input, size_t block_count) {
+    if (input.size() != 8) throw std::invalid_argument("ckks_walsh_sbox: input must contain 8 bit ciphertexts");
+    const CKKSCiphertext& ref = input.front();
+    if (ref.getN() != context.getParams().getN())
+        throw std::invalid_argument("ckks_walsh_sbox: ciphertext/context size mismatch");
+    if (ref.getNumPolys() != 2 || ref.getSecretOwner() != CKKSSecretOwner::Dense ||
+        ref.getMessageEncodingState() != CKKSMessageEncodingState::Slots)
+        throw std::invalid_argument("ckks_walsh_sbox: expected dense slot-encoded 2-polynomial inputs");
+    for (const auto& bit : input)
+        if (bit.getN() != ref.getN() || bit.getLevel() != ref.getLevel() || bit.getNumPolys() != 2 ||
+            bit.getSecretOwner() != ref.getSecretOwner() ||
+            bit.getMessageEncodingState() != ref.getMessageEncodingState() ||
+            !scales_close(bit.getScale(), ref.getScale()))
+            throw std::invalid_argument("ckks_walsh_sbox: inconsistent input ciphertexts");
+    check_segment_keys(context, block_count);
+    const size_t N = context.getParams().getN();
+    // the 30 parity sums in the reference's order (low mask m, then high mask m)
+    MyVector<CKKSCiphertext> terms;
+    std::vector<size_t> segments;
+    terms.reserve(kSegments);
+    for (size_t mask = 1; mask < kMasks; ++mask)
+        for (int high = 0; high < 2; ++high) {
+            terms.push_back(parity_sum(context, input, high ? 4 : 0, mask));
+            segments.push_back(packed_segment(high != 0, mask));
+        }
+    // rotate_segment_inplace (:484-495) of every term but segment 0, as one batch
+    std::vector<uint64_t*> b, a;
+    std::vector<uint64_t> galois;
+    std::vector<const uint64_t*> keys;
+    for (size_t t = 0; t < terms.size(); ++t) {
+        if (segments[t] == 0) continue;
+        const int64_t r = normalize_rotation_value(N, (int64_t)(segments[t] * block_count));
+        b.push_back(terms[t].polyData(0));
+        a.push_back(terms[t].polyData(1));
+        galois.push_back(rotation_automorphism_index(N, r));
+        keys.push_back(context.getRotationKeyDevice(r));
+    }
+    mi_check(mi_rotate_batch(context.handle(), (int)ref.getLevel(), (int)b.size(), b.data(), a.data(), galois.data(),
+                             keys.data(), context.getParams().montgomery ? 1 : 0, context.stream()),
+             "ckks_rotate");
+    // add_rotated_segment_term_inplace's accumulation (:497-519)
+    CKKSCiphertext packed = std::move(terms[0]);
+    for (size_t t = 1; t < terms.size(); ++t) ckks_add_inplace(context, packed, terms[t]);
+    return packed;
+}
+
+std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_mask_packed_signs(
+    const CKKSContext& context, const CKKSCiphertext& packed, size_t block_count) {
+    if (packed.getNumPolys() != 2 || packed.getSecretOwner() != CKKSSecretOwner::Dense)
+        throw std::invalid_argument("ckks_walsh_sbox: invalid hoisted rotation source");
+    check_segment_keys(context, block_count);
+    const auto& P = context.getParams();
+    const size_t N = P.getN(), level = packed.getLevel();
+    MyVector<CKKSCiphertext> low, high;
+    low.reserve(kMasks);
+    high.reserve(kMasks);
+    low.push_back(constant_one_like(context, packed));
+    high.push_back(constant_one_like(context, packed));
+    // one hoist of a (ckks_hoist_ciphertext_a_qp_eval), then every segment's
+    // rotation by -offset from it (unpack_segment_from_hoist, :498-509) in one batch
+    int64_t info[6];
+    mi_check(mi_context_info(context.handle(), info), "ckks_walsh_sbox");
+    const size_t L = level + 1, K = P.getPModuli().size(), alpha = (size_t)info[4];
+    const size_t parts = std::min<size_t>(P.getDnum(), (L + alpha - 1) / alpha);
+    DeviceBuffer digits(context.handle(), parts * (L + K) * N);
+    mi_check(mi_hoist(context.handle(), (int)level, packed.polyData(1), digits.data(), context.stream()),
+             "ckks_walsh_sbox");
+    std::vector<uint64_t*> b_out, a_out;
+    std::vector<uint64_t> galois;
+    std::vector<const uint64_t*> keys;
+    for (size_t mask = 1; mask < kMasks; ++mask)
+        for (int hi = 0; hi < 2; ++hi) {
+            auto& dst = hi ? high : low;
+            const size_t segment = packed_segment(hi != 0, mask);
+            if (segment == 0) {
+                dst.push_back(packed.clone());
+                continue;
+            }
+            const int64_t r = normalize_rotation_value(N, -(int64_t)(segment * block_count));
+            if (!context.hasRotationKey(r))
+                throw std::invalid_argument("ckks_walsh_sbox: missing hoisted unpack rotation key");
+            dst.emplace_back(context, 2, packed.getScale(), level, packed.getSecretOwner(),
+                             packed.getMessageEncodingState());
+            b_out.push_back(dst.back().polyData(0));
+            a_out.push_back(dst.back().polyData(1));
+            galois.push_back(rotation_automorphism_index(N, r));
+            keys.push_back(context.getRotationKeyDevice(r));
+        }
+    mi_check(mi_rotate_hoisted_batch(context.handle(), (int)level, digits.data(), packed.polyData(0),
+                                     (int)galois.size(), galois.data(), keys.data(), b_out.data(), a_out.data(),
+                                     P.montgomery ? 1 : 0, context.stream()),
+             "ckks_walsh_sbox");
+    return {std::move(low), std::move(high)};
+}

@@ -56,6 +56,7 @@ _SIGS = {
     "mi_keyswitch_hoisted": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, u64, cint, vp]),
     "mi_keyswitch_hoisted_qp": (cint, [vp, cint, vp, vp, vp, vp, vp, u64, cint, vp]),
     "mi_rotate_hoisted_batch": (cint, [vp, cint, vp, vp, cint, vp, vp, vp, vp, cint, vp]),
+    "mi_rotate_batch": (cint, [vp, cint, cint, vp, vp, vp, vp, cint, vp]),
     "mi_keyswitch_qp": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_moddown_qp": (cint, [vp, cint, vp, vp]),
     "mi_linear_transform_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
@@ -270,6 +271,16 @@ class Context:
                           stream=None):
         check(lib().mi_keyswitch_hoisted(self.h, level, ptr(digits), ptr(key), ptr(b_out), ptr(a_out), b_mode,
                                          a_mode, int(galois), int(montgomery), _stream(stream)))
+
+    def rotate_batch(self, level, cts, galois, keys, montgomery=True, stream=None):
+        """In-place rotations of independent ciphertexts: cts = [(b, a), ...]."""
+        n = len(cts)
+        P = vp * n
+        b, a = P(*[c[0].data_ptr() for c in cts]), P(*[c[1].data_ptr() for c in cts])
+        g = (u64 * n)(*[int(x) for x in galois])
+        k = P(*[t.data_ptr() for t in keys])
+        check(lib().mi_rotate_batch(self.h, level, n, ctypes.addressof(b), ctypes.addressof(a), ctypes.addressof(g),
+                                    ctypes.addressof(k), int(montgomery), _stream(stream)))
 
     def rotate_hoisted_batch(self, level, digits, b, galois, keys, outs, montgomery=True, stream=None):
         """outs: list of (b_out, a_out) device tensors, one per Galois element."""

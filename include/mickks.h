@@ -200,6 +200,14 @@ int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, con
 int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
                             const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
                             uint64_t* const* a_out, int montgomery, void* stream);
+/* ckks_rotate_inplace (ckks_rotation.cpp:41-96) for `count` independent
+ * ciphertexts at one level, as the mask-packed parity input rotates its 29
+ * parity sums (add_rotated_segment_term_inplace, ckks_walsh_sbox.cpp:
+ * 484-546): each (b[i], a[i]) <- g_i(b[i] + KS_i(a[i]), KS_i(a[i])) with
+ * keys[i], in place.  The key switches of up to 32 ciphertexts go through
+ * each phase as one launch.  Ciphertexts must not overlap. */
+int mi_rotate_batch(mi_context* ctx, int level, int count, uint64_t* const* b, uint64_t* const* a,
+                    const uint64_t* galois, const uint64_t* const* keys, int montgomery, void* stream);
 /* ckks_hybrid_key_switch_hoisted_qp_rotated (ckks_keyswitch.cpp:675-852):
  * b_qp = g([b ; 0] + sum_d digit_d key_b,d), a_qp = g(sum_d digit_d key_a,d),
  * no ModDown.  b: level+1 eval limbs; b_qp, a_qp: level+1+np limbs. */

@@ -294,3 +294,24 @@ def test_device_samplers_match_philox_and_their_distributions():
     for l in range(2):
         m = u[l * N2:(l + 1) * N2].astype(np.float64).mean() / float(q2[l])
         assert abs(m - 0.5) < 0.01
+
+
+def test_rotate_batch_matches_oracle():
+    """mi_rotate_batch: 35 independent ciphertexts (two chunks of key
+    switches) rotated in place, each equal to the oracle's rotation; batches
+    whose ciphertexts overlap are rejected."""
+    N = 1 << 12
+    M, ctx, octx, q, p, rng = _random_setup(N, 6, 2, 3, seed=21)
+    level = 4
+    L = level + 1
+    keys = [_rand_key(rng, q, p, N, octx.parts(5)) for _ in range(5)]
+    dkeys = [M.to_dev(k) for k in keys]
+    rots = [1, 3, N // 2 - 7, 100, N // 2 - 1] * 7
+    host_cts = [(rand_residues(rng, q[:L], N), rand_residues(rng, q[:L], N)) for _ in rots]
+    dev = [(M.to_dev(b), M.to_dev(a)) for b, a in host_cts]
+    ctx.rotate_batch(level, dev, [O.galois_rotation(N, r) for r in rots], [dkeys[i % 5] for i in range(len(rots))])
+    for i, ((b, a), (db, da), r) in enumerate(zip(host_cts, dev, rots)):
+        ob, oa = octx.rotate(level, b, a, r, keys[i % 5])
+        assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa), i
+    with pytest.raises(M.MiError, match="overlap"):
+        ctx.rotate_batch(level, [dev[0], (dev[0][0], dev[1][1])], [5, 5], [dkeys[0], dkeys[0]])

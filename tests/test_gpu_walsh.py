@@ -199,3 +199,69 @@ def test_walsh_recombine_decrypts_to_sbox_bits():
         err = np.max(np.abs(dec - want))
         print(f"pair {p}: max error {err:.2e}", flush=True)
         assert err < 0.02, (p, err)
+
+
+def _packing_context(H, bc, seed=7):
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 12
+    c.mult_depth = 2
+    c.q0_bit = 60
+    c.q_bit = 40
+    c.p_bits = [60, 60]
+    c.dnum = 2
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.rotate_values = [bc * s for s in range(1, 30)] + [-bc * s for s in range(1, 30)]
+    c.seed = seed
+    return H.CKKSContext(H.CKKSParams(c))
+
+
+def test_walsh_parity_packing_matches_rotations_and_unpacks():
+    """The mask-packed parity input (30 parity sums, 29 batched rotations,
+    summed) equals the reference's sequence of clones, additions and
+    one-at-a-time rotations bit for bit; unpacking a packed ciphertext (one
+    hoist, 29 batched hoisted rotations) decrypts to each segment rotated
+    back, with the constant-1 signs at mask 0."""
+    from helpers import host
+    H = host()
+    bc = 64
+    ctx = _packing_context(H, bc)
+    level = 2
+    slots = ctx.getParams().getSlots()
+    rng = np.random.default_rng(12)
+    vals = rng.integers(0, 2, (8, slots)).astype(np.float64)
+    bits = [H.ckks_encrypt(ctx, v.astype(np.complex128), level) for v in vals]
+    packed = H.ckks_walsh_build_mask_packed_parity_input(ctx, bits, bc)
+
+    ref, want = None, np.zeros(slots, np.complex128)
+    for mask in range(1, 16):
+        for hi in (0, 1):
+            sel = [4 * hi + k for k in range(4) if (mask >> k) & 1]
+            t = bits[sel[0]].clone()
+            for k in sel[1:]:
+                H.ckks_add_inplace(ctx, t, bits[k])
+            seg = 15 * hi + mask - 1
+            if seg:
+                H.ckks_rotate_inplace(ctx, t, seg * bc)
+            want += np.roll(vals[sel].sum(axis=0), -seg * bc)
+            if ref is None:
+                ref = t
+            else:
+                H.ckks_add_inplace(ctx, ref, t)
+    for k in range(2):
+        assert np.array_equal(packed.getPoly(k), ref.getPoly(k)), k
+    assert np.max(np.abs(H.ckks_decrypt(ctx, packed) - want)) < 1e-6
+
+    v = rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots)
+    src = H.ckks_encrypt(ctx, v, level)
+    low, high = H.ckks_walsh_unpack_mask_packed_signs(ctx, src, bc)
+    assert len(low) == 16 and len(high) == 16
+    for signs in (low, high):
+        assert np.max(np.abs(H.ckks_decrypt(ctx, signs[0]) - 1.0)) < 1e-9
+    assert np.array_equal(low[1].getPoly(0), src.getPoly(0))
+    for mask in range(1, 16):
+        for hi, signs in ((0, low), (1, high)):
+            seg = 15 * hi + mask - 1
+            err = np.max(np.abs(H.ckks_decrypt(ctx, signs[mask]) - np.roll(v, seg * bc)))
+            assert err < 1e-6, (hi, mask, err)
