@@ -651,6 +651,30 @@ int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t*
     });
 }
 
+int mi_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, void* stream) {
+    return guard("ckks_rescale", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (npolys < 1 || !polys) throw mi::Error(MI_EINVAL, "invalid poly list");
+        for (int k = 0; k < npolys; ++k) check_ptr(polys[k]);
+        mi::rescale_list(*ctx->c, level, polys, npolys, S(stream));
+    });
+}
+
+int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, const uint64_t* pt,
+                              int pt_montgomery, int rescale, void* stream) {
+    return guard("ckks_mul_inplace", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(pt);
+        if (npolys < 1 || !polys) throw mi::Error(MI_EINVAL, "invalid poly list");
+        for (int k = 0; k < npolys; ++k) check_ptr(polys[k]);
+        if (rescale && level < 1) throw mi::Error(MI_EINVAL, "invalid level");
+        mi::mul_plain_list(*ctx->c, level, polys, npolys, pt, pt_montgomery != 0, S(stream));
+        if (rescale) mi::rescale_list(*ctx->c, level, polys, npolys, S(stream));
+    });
+}
+
 int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
                          const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
                          const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,

@@ -195,6 +195,11 @@ void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* 
 void automorphism_acc(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                       long long sstride, long long dstride, bool accumulate, hipStream_t s);
 void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s);
+// the same for listed polys (any number; launches of up to 64)
+void rescale_list(Context& c, int level, u64* const* polys, int npolys, hipStream_t s);
+// polys[k] *= pt over limbs 0..level (pt Montgomery form if mont)
+void mul_plain_list(const Context& c, int level, u64* const* polys, int npolys, const u64* pt, bool mont,
+                    hipStream_t s);
 // bytes of zeros at p by a kernel (no memset node, so safe inside a captured graph)
 void fill_zero(void* p, size_t bytes, hipStream_t s);
 void add_plain_mont(const Context& c, int limbs, u64* b, const u64* pt, hipStream_t s);

@@ -252,11 +252,27 @@ void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* 
 }
 
 // ---------------------------------------------------------------- rescale
+// Polys of one launch are named by a pointer list (up to RS_MAX_POLYS).
+constexpr int RS_MAX_POLYS = 64;
+
+// In-place NTT of one limb (basis `basis`) of each listed poly.
+struct TopLimbJob {
+    u64* p[RS_MAX_POLYS];
+    int basis;
+    struct View {
+        int basis;
+        u64* p;
+        __device__ u64 load(int i) const { return p[i]; }
+        __device__ void store(int i, u64 v) const { p[i] = v; }
+    };
+    __device__ View view(int k) const { return View{basis, p[k]}; }
+    __device__ bool skip(int) const { return false; }
+};
+
 // Column pass input: rem_i = ((x_l + floor(p/2)) mod p) mod q_i from the
 // coefficient-domain top limb; output to scratch.
 struct RescaleInJob {
-    const u64* polys;
-    long long stride;
+    const u64* polys[RS_MAX_POLYS];
     u64* scratch;     // [npolys][level][N]
     int level, npolys;
     u64 N;
@@ -278,16 +294,15 @@ struct RescaleInJob {
     __device__ View view(int k) const {
         const int i = per_limb.div(k), poly = k - i * npolys;  // modulus-major
         const BasisConst b = bc[i];
-        return View{i, polys + poly * stride + (long long)level * N,
-                    scratch + ((long long)poly * level + i) * N, p, half, b.q, b.mu_hi};
+        return View{i, polys[poly] + (long long)level * N, scratch + ((long long)poly * level + i) * N, p, half,
+                    b.q, b.mu_hi};
     }
     __device__ bool skip(int) const { return false; }
 };
 
 // Row pass output: x_i <- (x_i + NTT(half) - NTT(rem)) * p^-1 mod q_i
 struct RescaleOutJob {
-    u64* polys;
-    long long stride;
+    u64* polys[RS_MAX_POLYS];
     const u64* scratch;
     int level, npolys;
     u64 N;
@@ -310,27 +325,79 @@ struct RescaleOutJob {
     };
     __device__ View view(int k) const {
         const int i = per_limb.div(k), poly = k - i * npolys;  // modulus-major
-        return View{i, scratch + ((long long)poly * level + i) * N,
-                    polys + poly * stride + (long long)i * N, half_ntt + (long long)i * N,
-                    bc[i].q, inv_p[i], inv_p_shoup[i]};
+        return View{i, scratch + ((long long)poly * level + i) * N, polys[poly] + (long long)i * N,
+                    half_ntt + (long long)i * N, bc[i].q, inv_p[i], inv_p_shoup[i]};
     }
     __device__ bool skip(int) const { return false; }
 };
 
-void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s) {
+void rescale_list(Context& c, int level, u64* const* polys, int npolys, hipStream_t s) {
     if (level < 1 || level >= c.nq) throw Error(-1, "invalid level");
     if (npolys <= 0) return;
     const u64 N = c.N;
-    // 1. top limb of every poly to the coefficient domain (in place)
-    ntt_inverse(c, LimbBatch{polys + (long long)level * N, stride, 1, level, npolys}, s);
-    // 2. fused remainder -> NTT -> finish
-    u64* scratch = (u64*)c.workspace(s, sizeof(u64) * (size_t)npolys * level * N);
     const u64 p = c.q[level];
     const RescaleTables& rt = c.rescale[level];
-    RescaleInJob in{polys, stride, scratch, level, npolys, N, p, p >> 1, c.d_bc, fast_div(npolys)};
-    RescaleOutJob out{polys, stride, scratch, level, npolys, N, rt.half_ntt, rt.inv_p, rt.inv_p_shoup,
-                      c.d_bc, fast_div(npolys)};
-    run_ntt_fwd(c, in, out, npolys * level, s);
+    for (int k0 = 0; k0 < npolys; k0 += RS_MAX_POLYS) {
+        const int n = std::min(RS_MAX_POLYS, npolys - k0);
+        // 1. top limb of every poly to the coefficient domain (in place)
+        TopLimbJob top{};
+        top.basis = level;
+        for (int k = 0; k < n; ++k) top.p[k] = polys[k0 + k] + (long long)level * N;
+        run_ntt_inv(c, top, top, n, s);
+        // 2. fused remainder -> NTT -> finish
+        u64* scratch = (u64*)c.workspace(s, sizeof(u64) * (size_t)n * level * N);
+        RescaleInJob in{};
+        RescaleOutJob out{};
+        for (int k = 0; k < n; ++k) {
+            in.polys[k] = polys[k0 + k];
+            out.polys[k] = polys[k0 + k];
+        }
+        in.scratch = scratch;
+        in.level = out.level = level;
+        in.npolys = out.npolys = n;
+        in.N = out.N = N;
+        in.p = p;
+        in.half = p >> 1;
+        in.bc = out.bc = c.d_bc;
+        in.per_limb = out.per_limb = fast_div(n);
+        out.scratch = scratch;
+        out.half_ntt = rt.half_ntt;
+        out.inv_p = rt.inv_p;
+        out.inv_p_shoup = rt.inv_p_shoup;
+        run_ntt_fwd(c, in, out, n * level, s);
+    }
+}
+
+void rescale(Context& c, int level, u64* polys, int npolys, long long stride, hipStream_t s) {
+    if (level < 1 || level >= c.nq) throw Error(-1, "invalid level");
+    std::vector<u64*> list(npolys > 0 ? npolys : 0);
+    for (int k = 0; k < npolys; ++k) list[k] = polys + (long long)k * stride;
+    rescale_list(c, level, list.data(), npolys, s);
+}
+
+// x[k] *= pt (limb-wise, Barrett or Montgomery plaintext) for listed polys
+struct PolyList {
+    u64* p[RS_MAX_POLYS];
+};
+__global__ void __launch_bounds__(EW_THREADS)
+k_mul_plain_list(const BasisConst* __restrict__ bc, PolyList list, const u64* __restrict__ pt, int logN, int mont) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const BasisConst b = bc[ew_limb(logN)];
+    u64* px = list.p[blockIdx.y] + r;
+    *px = mont ? mont_mul(*px, pt[r], b.q, b.qninv) : mul_mod(*px, pt[r], b.q, b.mu_lo, b.mu_hi);
+}
+
+void mul_plain_list(const Context& c, int level, u64* const* polys, int npolys, const u64* pt, bool mont,
+                    hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
+    for (int k0 = 0; k0 < npolys; k0 += RS_MAX_POLYS) {
+        const int n = std::min(RS_MAX_POLYS, npolys - k0);
+        PolyList list{};
+        for (int k = 0; k < n; ++k) list.p[k] = polys[k0 + k];
+        hipLaunchKernelGGL(k_mul_plain_list, ew_grid(c, level + 1, n), dim3(EW_THREADS), 0, s, c.d_bc, list, pt,
+                           c.logN, mont ? 1 : 0);
+        MI_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace mi

@@ -536,6 +536,16 @@ CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& cont
 std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_mask_packed_signs(
     const CKKSContext& context, const CKKSCiphertext& packed, size_t block_count);
 
+// The AES-CTR masking loops (ckks_aes_ctr_walsh_sbox.cpp:217-329): every
+// state ciphertext times the active-slot mask (encoded at q_level) and
+// rescaled back to its scale; the final refresh mask (scale q_level and a
+// rescale while the level is above minimum_level_to_keep, scale 1
+// otherwise).  All ciphertexts go through one batched product and rescale.
+void ckks_aes_mask_inactive_slots_inplace(const CKKSContext& context, MyVector<CKKSCiphertext>& state,
+                                          size_t block_count);
+void ckks_aes_mask_final_refresh_slots_inplace(const CKKSContext& context, MyVector<CKKSCiphertext>& bits,
+                                               size_t active_slot_count, size_t minimum_level_to_keep);
+
 // Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
 // coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].
 struct CKKSEvalPolyOptions {

@@ -107,8 +107,9 @@ void ckks_rescale(const CKKSContext& c, CKKSCiphertext& ct) {
     if (lv == 0 || lv > c.getParams().getMaxLevel()) throw std::invalid_argument("ckks_rescale: invalid level");
     check_ct(c, ct, "ckks_rescale");
     if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_rescale: empty ciphertext");
-    for (size_t i = 0; i < ct.getNumPolys(); ++i)
-        mi_check(mi_rescale(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), c.stream()), "ckks_rescale");
+    std::vector<uint64_t*> polys(ct.getNumPolys());
+    for (size_t i = 0; i < ct.getNumPolys(); ++i) polys[i] = ct.polyData(i);
+    mi_check(mi_rescale_list(c.handle(), (int)lv, (int)polys.size(), polys.data(), c.stream()), "ckks_rescale");
     const uint64_t p = c.getParams().getModuli()[lv];
     ct.setScale(ct.getScale() / (long double)p);
     ct.setLevel(lv - 1);

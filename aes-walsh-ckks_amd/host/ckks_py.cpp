@@ -417,6 +417,36 @@ PYBIND11_MODULE(ckks_py, m) {
               auto out = ckks_walsh_recombine_signs_to_paired_outputs(c, ct_list(low), ct_list(high));
               return ct_pylist(out, context);
           });
+    // in-place functions over a list: the ciphertexts are moved into a
+    // vector for the call and moved back into their Python objects
+    auto inplace_list = [](const py::list& l, auto&& fn) {
+        MyVector<CKKSCiphertext> v;
+        v.reserve(l.size());
+        for (const auto& h : l) v.push_back(std::move(h.cast<CKKSCiphertext&>()));
+        auto put_back = [&]() {
+            size_t i = 0;
+            for (const auto& h : l) h.cast<CKKSCiphertext&>() = std::move(v[i++]);
+        };
+        try {
+            fn(v);
+        } catch (...) {
+            put_back();
+            throw;
+        }
+        put_back();
+    };
+    m.def("ckks_aes_mask_inactive_slots_inplace",
+          [inplace_list](const CKKSContext& c, const py::list& state, size_t block_count) {
+              inplace_list(state, [&](MyVector<CKKSCiphertext>& v) {
+                  ckks_aes_mask_inactive_slots_inplace(c, v, block_count);
+              });
+          });
+    m.def("ckks_aes_mask_final_refresh_slots_inplace",
+          [inplace_list](const CKKSContext& c, const py::list& bits, size_t active, size_t minimum_level_to_keep) {
+              inplace_list(bits, [&](MyVector<CKKSCiphertext>& v) {
+                  ckks_aes_mask_final_refresh_slots_inplace(c, v, active, minimum_level_to_keep);
+              });
+          });
     m.def("ckks_walsh_build_mask_packed_parity_input",
           [ct_list](const CKKSContext& c, const py::list& input, size_t block_count) {
               return ckks_walsh_build_mask_packed_parity_input(c, ct_list(input), block_count);

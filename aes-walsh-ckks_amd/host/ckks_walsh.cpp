@@ -406,3 +406,75 @@ std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_
              "ckks_walsh_sbox");
     return {std::move(low), std::move(high)};
 }
+
+namespace {
+
+// encode_active_slot_mask (ckks_aes_ctr_walsh_sbox.cpp:199-215): 1 in the
+// first `count` slots, 0 elsewhere, at `scale`, evaluation domain.
+CKKSEncoding active_slot_mask(const CKKSContext& context, size_t count, size_t level, long double scale) {
+    MyVector<Complex> slots(context.getParams().getSlots(), Complex(0.0, 0.0));
+    for (size_t i = 0; i < count; ++i) slots[i] = Complex(1.0, 0.0);
+    return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
+}
+
+// aes_scales_close (ckks_aes_ctr_walsh_sbox.cpp:42-49)
+bool aes_scales_close(long double lhs, long double rhs) { return scales_close(lhs, rhs, 0x1p-45L); }
+
+// Every ciphertext times the mask, then (rescale) one rescale of all their
+// polys: one batched call instead of the reference's OpenMP loop.
+void mask_all(const CKKSContext& context, MyVector<CKKSCiphertext>& cts, const CKKSEncoding& mask, bool rescale,
+              const char* level_msg) {
+    std::vector<uint64_t*> polys;
+    for (auto& ct : cts) {
+        if (ct.getLevel() != mask.getLevel()) throw std::invalid_argument(level_msg);
+        for (size_t k = 0; k < ct.getNumPolys(); ++k) polys.push_back(ct.polyData(k));
+    }
+    const size_t level = mask.getLevel();
+    mi_check(mi_mul_plain_rescale_list(context.handle(), (int)level, (int)polys.size(), polys.data(), mask.data(),
+                                       mask.isMontgomeryForm() ? 1 : 0, rescale ? 1 : 0, context.stream()),
+             "ckks_mul_inplace");
+    const long double p = (long double)context.getParams().getModuli()[level];
+    for (auto& ct : cts) {
+        ct.setScale(ct.getScale() * mask.getScale());
+        if (rescale) {
+            ct.setScale(ct.getScale() / p);
+            ct.setLevel(level - 1);
+        }
+    }
+}
+
+}  // namespace
+
+void ckks_aes_mask_inactive_slots_inplace(const CKKSContext& context, MyVector<CKKSCiphertext>& state,
+                                          size_t block_count) {
+    if (state.empty()) return;
+    const size_t level = state.front().getLevel();
+    const size_t s2c_level = context.getParams().getBootstrapS2CDepth();
+    if (level <= s2c_level)
+        throw std::invalid_argument(
+            "ckks_aes_walsh: active slot mask requires a selection limb above S2C input level");
+    const long double target_scale = state.front().getScale();
+    const auto mask =
+        active_slot_mask(context, block_count, level, (long double)context.getParams().getModuli()[level]);
+    for (const auto& ct : state)
+        if (!aes_scales_close(ct.getScale(), target_scale))
+            throw std::invalid_argument("ckks_aes_walsh: active slot mask scale mismatch");
+    mask_all(context, state, mask, true, "ckks_aes_walsh: active slot mask level mismatch");
+    for (auto& ct : state) ct.setScale(target_scale);
+}
+
+void ckks_aes_mask_final_refresh_slots_inplace(const CKKSContext& context, MyVector<CKKSCiphertext>& bits,
+                                               size_t active_slot_count, size_t minimum_level_to_keep) {
+    if (bits.empty()) return;
+    const auto& params = context.getParams();
+    if (active_slot_count > params.getSlots())
+        throw std::invalid_argument("ckks_aes_walsh: final refresh mask active slot count exceeds CKKS slots");
+    const size_t level = bits.front().getLevel();
+    const bool consume_selection_limb = level > minimum_level_to_keep;
+    const long double target_scale = bits.front().getScale();
+    const long double mask_scale = consume_selection_limb ? (long double)params.getModuli()[level] : 1.0L;
+    const auto mask = active_slot_mask(context, active_slot_count, level, mask_scale);
+    mask_all(context, bits, mask, consume_selection_limb, "ckks_aes_walsh: final refresh output mask level mismatch");
+    if (consume_selection_limb)
+        for (auto& ct : bits) ct.setScale(target_scale);
+}

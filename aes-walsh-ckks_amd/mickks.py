@@ -57,6 +57,8 @@ _SIGS = {
     "mi_keyswitch_hoisted_qp": (cint, [vp, cint, vp, vp, vp, vp, vp, u64, cint, vp]),
     "mi_rotate_hoisted_batch": (cint, [vp, cint, vp, vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_rotate_batch": (cint, [vp, cint, cint, vp, vp, vp, vp, cint, vp]),
+    "mi_rescale_list": (cint, [vp, cint, cint, vp, vp]),
+    "mi_mul_plain_rescale_list": (cint, [vp, cint, cint, vp, vp, cint, cint, vp]),
     "mi_keyswitch_qp": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_moddown_qp": (cint, [vp, cint, vp, vp]),
     "mi_linear_transform_qp": (cint, [vp, cint, vp, vp, cint, vp, vp, cint, cint, cint, cint, vp, vp,
@@ -271,6 +273,13 @@ class Context:
                           stream=None):
         check(lib().mi_keyswitch_hoisted(self.h, level, ptr(digits), ptr(key), ptr(b_out), ptr(a_out), b_mode,
                                          a_mode, int(galois), int(montgomery), _stream(stream)))
+
+    def mul_plain_rescale_list(self, level, polys, pt, pt_montgomery=False, rescale=True, stream=None):
+        """polys[k] *= pt, then (rescale) ckks_rescale of every listed poly."""
+        P = vp * len(polys)
+        arr = P(*[t.data_ptr() for t in polys])
+        check(lib().mi_mul_plain_rescale_list(self.h, level, len(polys), ctypes.addressof(arr), ptr(pt),
+                                              int(pt_montgomery), int(rescale), _stream(stream)))
 
     def rotate_batch(self, level, cts, galois, keys, montgomery=True, stream=None):
         """In-place rotations of independent ciphertexts: cts = [(b, a), ...]."""

@@ -311,6 +311,19 @@ int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint
 int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
                       int montgomery, void* stream);
 
+/* ckks_rescale (ckks_rescale.cpp:68-148) of `npolys` listed polys at one
+ * level (the polys of one or more ciphertexts), in place; the top limb of
+ * every poly goes through one inverse-NTT launch pair and the remainder /
+ * finish through one forward pair per 64 polys. */
+int mi_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, void* stream);
+/* The AES-CTR masking loops (mask_inactive_slots_inplace /
+ * mask_final_refresh_slots_inplace, ckks_aes_ctr_walsh_sbox.cpp:217-329):
+ * every listed poly times one plaintext (ckks_mul_inplace(ct, pt)), then,
+ * if rescale, ckks_rescale of all of them; one launch (pair) per 64 polys
+ * instead of a loop over ciphertexts. */
+int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, const uint64_t* pt,
+                              int pt_montgomery, int rescale, void* stream);
+
 /* ---- Walsh S-box ----
  * The linear combinations and raw 4+4 products of one output pair of the
  * Walsh S-box (walsh_sum_for_output_pair, ckks_walsh_sbox.cpp:984-1043, up

@@ -265,3 +265,77 @@ def test_walsh_parity_packing_matches_rotations_and_unpacks():
             seg = 15 * hi + mask - 1
             err = np.max(np.abs(H.ckks_decrypt(ctx, signs[mask]) - np.roll(v, seg * bc)))
             assert err < 1e-6, (hi, mask, err)
+
+
+def test_aes_masking_matches_per_ciphertext_sequence():
+    """The AES-CTR masking loops (one batched product and rescale over all
+    ciphertexts) equal the reference's per-ciphertext sequence (encode the
+    active-slot mask, multiply, rescale, restore the scale) bit for bit, in
+    level and in scale (ckks_aes_ctr_walsh_sbox.cpp:217-329)."""
+    from helpers import host
+    H = host()
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 1
+    c.q0_bit = 41
+    c.q_bit = 40
+    c.p_bits = [60, 60, 60]
+    c.dnum = 4
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.bts_keys = True   # the S2C depth is read from the bootstrap parameters
+    c.seed = 31
+    bp = H.CKKSBootstrapParams()
+    bp.sparse_secret_hamming_weight = 16
+    bp.linear_transform_log_plaintext_scales = [[[30], [30], [30]], [[30], [30], [30]]]
+    bp.linear_transform_depths = [3, 3]
+    bp.evalmod_depth = 7
+    c.bootstrap_params = bp
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    q = list(P.getModuli())
+    slots = P.getSlots()
+    rng = np.random.default_rng(14)
+    level = 6
+    vals = [rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots) for _ in range(5)]
+    state = [H.ckks_encrypt(ctx, v, level) for v in vals]
+    ref = [s.clone() for s in state]
+    target = state[0].getScale()
+
+    def mask(count, lv, scale):
+        m = np.zeros(slots, np.complex128)
+        m[:count] = 1
+        return H.ckks_encode(ctx, m, lv, scale, True)
+
+    H.ckks_aes_mask_inactive_slots_inplace(ctx, state, 100)
+    mk = mask(100, level, float(q[level]))
+    for r in ref:
+        H.ckks_mul_plain_inplace(ctx, r, mk)
+        H.ckks_rescale(ctx, r)
+        r.setScale(target)
+    for s, r in zip(state, ref):
+        assert s.getLevel() == r.getLevel() == level - 1 and s.getScale() == r.getScale() == target
+        for k in range(2):
+            assert np.array_equal(s.getPoly(k), r.getPoly(k))
+    want = vals[0].copy()
+    want[100:] = 0
+    assert np.max(np.abs(H.ckks_decrypt(ctx, state[0]) - want)) < 1e-6
+
+    # the final refresh, consuming the selection limb, then keeping the level
+    for minimum, consume in ((2, True), (level - 1, False)):
+        H.ckks_aes_mask_final_refresh_slots_inplace(ctx, state, 50, minimum)
+        lv = ref[0].getLevel()
+        mk = mask(50, lv, float(q[lv]) if consume else 1.0)
+        for r in ref:
+            H.ckks_mul_plain_inplace(ctx, r, mk)
+            if consume:
+                H.ckks_rescale(ctx, r)
+                r.setScale(target)
+        for s, r in zip(state, ref):
+            assert s.getLevel() == r.getLevel() and s.getScale() == r.getScale()
+            for k in range(2):
+                assert np.array_equal(s.getPoly(k), r.getPoly(k)), (minimum, k)
+    low = [H.ckks_encrypt(ctx, vals[0], 3)]
+    with pytest.raises(ValueError, match="selection limb"):
+        H.ckks_aes_mask_inactive_slots_inplace(ctx, low, 10)
