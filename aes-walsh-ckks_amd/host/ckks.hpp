@@ -536,6 +536,15 @@ CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& cont
 std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_mask_packed_signs(
     const CKKSContext& context, const CKKSCiphertext& packed, size_t block_count);
 
+// ckks_walsh_sbox_complex_blocks_mask_packed (ckks_walsh_sbox.cpp:1280-1323):
+// the AES S-box on 8 bit ciphertexts (slot j: real = bit of block lane 0,
+// imaginary = bit of lane 1, at the S2C entry level): parity packing, one
+// complex-packed bootstrap to parity bits, signs, unpacking, the Walsh
+// output pairs of both lanes and their split into the 8 output bits.
+MyVector<CKKSCiphertext> ckks_walsh_sbox_complex_blocks_mask_packed(CKKSContext& context,
+                                                                    MyVector<CKKSCiphertext> input,
+                                                                    size_t block_count);
+
 // The AES-CTR masking loops (ckks_aes_ctr_walsh_sbox.cpp:217-329): every
 // state ciphertext times the active-slot mask (encoded at q_level) and
 // rescaled back to its scale; the final refresh mask (scale q_level and a

@@ -447,6 +447,12 @@ PYBIND11_MODULE(ckks_py, m) {
                   ckks_aes_mask_final_refresh_slots_inplace(c, v, active, minimum_level_to_keep);
               });
           });
+    m.def("ckks_walsh_sbox_complex_blocks_mask_packed",
+          [ct_list, ct_pylist](const py::object& context, const py::list& input, size_t block_count) {
+              CKKSContext& c = context.cast<CKKSContext&>();
+              auto out = ckks_walsh_sbox_complex_blocks_mask_packed(c, ct_list(input), block_count);
+              return ct_pylist(out, context);
+          });
     m.def("ckks_walsh_build_mask_packed_parity_input",
           [ct_list](const CKKSContext& c, const py::list& input, size_t block_count) {
               return ckks_walsh_build_mask_packed_parity_input(c, ct_list(input), block_count);

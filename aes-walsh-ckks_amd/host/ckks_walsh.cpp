@@ -478,3 +478,75 @@ void ckks_aes_mask_final_refresh_slots_inplace(const CKKSContext& context, MyVec
     if (consume_selection_limb)
         for (auto& ct : bits) ct.setScale(target_scale);
 }
+
+namespace {
+
+// bit_to_sign_inplace (ckks_walsh_sbox.cpp:385-388): b -> 1 - 2b
+void bit_to_sign(const CKKSContext& context, CKKSCiphertext& bit) {
+    ckks_mul_inplace(context, bit, int64_t{-2});
+    ckks_add_integer_constant_inplace(context, bit, int64_t{1});
+}
+
+// multiply_by_i (:913-924) and multiply_by_cached_i_inplace without a cache
+// (:926-935): times the constant +-i at plaintext scale 1, scale kept
+CKKSCiphertext times_i(const CKKSContext& context, const CKKSCiphertext& ct, double sign) {
+    auto out = ckks_mul(context, ct, Complex(0.0, sign), 1.0);
+    out.setScale(ct.getScale());
+    return out;
+}
+
+// split_real_imag_pair (:1080-1095): (z + conj z) and -i (z - conj z), both
+// read at twice the scale
+std::pair<CKKSCiphertext, CKKSCiphertext> split_real_imag_pair(const CKKSContext& context,
+                                                               const CKKSCiphertext& pair) {
+    auto conjugated = ckks_conjugate(context, pair);
+    auto real = pair.clone();
+    ckks_add_inplace(context, real, conjugated);
+    real.setScale(pair.getScale() * 2.0L);
+    auto imag = pair.clone();
+    ckks_sub_inplace(context, imag, conjugated);
+    imag = times_i(context, imag, -1.0);
+    imag.setScale(pair.getScale() * 2.0L);
+    return {std::move(real), std::move(imag)};
+}
+
+// combine_real_imag (:937-958) for two ciphertexts of the same metadata
+CKKSCiphertext combine_real_imag(const CKKSContext& context, CKKSCiphertext real, const CKKSCiphertext& imag) {
+    auto shifted = times_i(context, imag, 1.0);
+    if (real.getLevel() != shifted.getLevel() || real.getNumPolys() != shifted.getNumPolys() ||
+        !scales_close(real.getScale(), shifted.getScale()))
+        throw std::invalid_argument("ckks_walsh_sbox: real/imaginary lanes do not align");
+    ckks_add_inplace(context, real, shifted);
+    return real;
+}
+
+}  // namespace
+
+MyVector<CKKSCiphertext> ckks_walsh_sbox_complex_blocks_mask_packed(CKKSContext& context,
+                                                                    MyVector<CKKSCiphertext> input,
+                                                                    size_t block_count) {
+    if (input.size() != 8) throw std::invalid_argument("ckks_walsh_sbox: input must contain 8 bit ciphertexts");
+    if (input.front().getLevel() != context.getParams().getBootstrapS2CDepth())
+        throw std::invalid_argument("ckks_walsh_sbox: input level must equal bootstrap S2C entry depth");
+    auto packed = ckks_walsh_build_mask_packed_parity_input(context, input, block_count);
+    // bootstrap_complex_packed_parity_lanes: the parity sums refresh to parity bits
+    auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(context, std::move(packed));
+    bit_to_sign(context, lanes.first);
+    bit_to_sign(context, lanes.second);
+    auto unpacked0 = ckks_walsh_unpack_mask_packed_signs(context, lanes.first, block_count);
+    auto unpacked1 = ckks_walsh_unpack_mask_packed_signs(context, lanes.second, block_count);
+    auto paired0 = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked0.first, unpacked0.second);
+    auto paired1 = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked1.first, unpacked1.second);
+    // combine_paired_lane_outputs_as_complex_bits (:1154-1193)
+    std::array<std::optional<CKKSCiphertext>, 8> bits;
+    for (size_t pair = 0; pair < 4; ++pair) {
+        auto [real_low, real_high] = split_real_imag_pair(context, paired0[pair]);
+        auto [imag_low, imag_high] = split_real_imag_pair(context, paired1[pair]);
+        bits[pair] = combine_real_imag(context, std::move(real_low), imag_low);
+        bits[pair + 4] = combine_real_imag(context, std::move(real_high), imag_high);
+    }
+    MyVector<CKKSCiphertext> out;
+    out.reserve(8);
+    for (auto& b : bits) out.push_back(std::move(*b));
+    return out;
+}

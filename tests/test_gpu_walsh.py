@@ -339,3 +339,65 @@ def test_aes_masking_matches_per_ciphertext_sequence():
     low = [H.ckks_encrypt(ctx, vals[0], 3)]
     with pytest.raises(ValueError, match="selection limb"):
         H.ckks_aes_mask_inactive_slots_inplace(ctx, low, 10)
+
+
+def test_walsh_sbox_end_to_end_on_a_bootstrapping_chain():
+    """The whole mask-packed S-box (parity packing, complex-packed bootstrap
+    to parity bits, signs, unpacking, Walsh output pairs of both lanes, split
+    into 8 bits) on the N = 2^10 bootstrapping chain of the host bootstrap
+    test: 16 blocks per lane, random bytes; the decrypted outputs round to
+    the AES S-box bits of every block in both lanes.  The EvalMod polynomial
+    is the test's own Chebyshev interpolant (the reference fits its
+    coefficients with MPFR)."""
+    from helpers import host
+    H = host()
+    bc = 16
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 10
+    c.mult_depth = 1
+    c.q0_bit = 41
+    c.q_bit = 40
+    c.p_bits = [60, 60, 60]
+    c.dnum = 4
+    c.secret_key_hamming_weight = 192
+    c.sigma = 3.2
+    c.secure = False
+    c.bts_keys = True
+    c.seed = 99
+    c.rotate_values = [bc * s for s in range(1, 30)] + [-bc * s for s in range(1, 30)]
+    bp = H.CKKSBootstrapParams()
+    bp.sparse_secret_hamming_weight = 16
+    bp.linear_transform_log_plaintext_scales = [[[30], [30], [30]], [[30], [30], [30]]]
+    bp.linear_transform_depths = [3, 3]
+    bp.evalmod_depth = 7
+    c.bootstrap_params = bp
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    P = ctx.getParams()
+    K = P.getBootstrapEvalModK()
+    coeffs = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), 100)
+    coeffs[1::2] = 0.0
+    ctx.setEvalModPolynomial(list(coeffs))
+    slots = P.getSlots()
+    rng = np.random.default_rng(23)
+    x0, x1 = rng.integers(0, 256, bc), rng.integers(0, 256, bc)
+    level = P.getBootstrapS2CDepth()
+    bits = []
+    for k in range(8):
+        v = np.zeros(slots, np.complex128)
+        v[:bc] = ((x0 >> k) & 1) + 1j * ((x1 >> k) & 1)
+        bits.append(H.ckks_encrypt(ctx, v, level))
+    out = H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, bits, bc)
+    assert len(out) == 8
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    par = np.array([[bin(m & int(v)).count("1") & 1 for v in range(256)] for m in range(256)])
+    masks = np.arange(256)
+    f = np.stack([4 * table[b][masks & 15, masks >> 4] for b in range(8)]) @ (1 - 2 * par)
+    sbox = ((f < 0).astype(np.int64) * (1 << np.arange(8))[:, None]).sum(axis=0)
+    worst = 0.0
+    for k, ct in enumerate(out):
+        dec = H.ckks_decrypt(ctx, ct)[:bc]
+        want = ((sbox[x0] >> k) & 1) + 1j * ((sbox[x1] >> k) & 1)
+        err = np.max(np.abs(dec - want))
+        worst = max(worst, err)
+        print(f"bit {k}: level {ct.getLevel()} max error {err:.3e}", flush=True)
+    assert worst < 1e-4, worst
