@@ -56,6 +56,9 @@ PYBIND11_MODULE(ckks_py, m) {
         .def_readwrite("evalmod_depth", &CKKSBootstrapParams::evalmod_depth)
         .def_readwrite("evalmod_bit_size", &CKKSBootstrapParams::evalmod_bit_size)
         .def_readwrite("evalmod_k", &CKKSBootstrapParams::evalmod_k)
+        .def_property("evalmod_k_failure_probability_log2",
+                      [](const CKKSBootstrapParams& b) { return (double)b.evalmod_k_failure_probability_log2; },
+                      [](CKKSBootstrapParams& b, double v) { b.evalmod_k_failure_probability_log2 = v; })
         .def_readwrite("evalmod_double_angle", &CKKSBootstrapParams::evalmod_double_angle)
         .def_readwrite("linear_transform_bsgs_baby_step_counts", &CKKSBootstrapParams::linear_transform_bsgs_baby_step_counts)
         .def_readwrite("linear_transform_log_bsgs_ratios", &CKKSBootstrapParams::linear_transform_log_bsgs_ratios)

@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh|sbox]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -585,6 +585,101 @@ def cpu_baseline_walsh(N, q, p, level, pair):
             "sample": "oracle product sum of one output pair (no relinearisation) at the same shape"}
 
 
+def walsh15_context(H, seed=15):
+    """The reference's AESWalsh15 parameters (ckks_aes_params.hpp:35-82) in
+    the host API: N=2^15, q0 36 bits, mult limbs 42 x 2, scale 2^35, dnum 3,
+    P = 4 x 61 bits, Hamming weight 256; bootstrap: sparse secret of weight
+    32, S2C one stage of scale groups 18+20+23, C2S two stages of 25+25,
+    EvalMod 7 x 38 bits with K from the BinBoot bound at 2^-16, degree 100,
+    QP evaluators; the mask-packed S-box's 58 segment rotation keys
+    (block_count 512).  The EvalMod coefficients are a Chebyshev interpolant
+    of 0.5 - 0.5 cos(2 pi K u) on [-1, 1] (the reference fits its own with
+    MPFR)."""
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 15
+    c.mult_depth = 2
+    c.q0_bit = 36
+    c.q_mult_bits = [42, 42]
+    c.scale = 2.0 ** 35
+    c.dnum = 3
+    c.p_bits = [61, 61, 61, 61]
+    c.secret_key_hamming_weight = 256
+    c.secure = False
+    c.bts_keys = True
+    c.seed = seed
+    c.rotate_values = [512 * s for s in range(1, 30)] + [-512 * s for s in range(1, 30)]
+    bp = H.CKKSBootstrapParams()
+    bp.sparse_secret_hamming_weight = 32
+    bp.linear_transform_log_plaintext_scales = [[[18, 20, 23]], [[25, 25], [25, 25]]]
+    bp.linear_transform_depths = [1, 2]
+    bp.evalmod_depth = 7
+    bp.evalmod_bit_size = 38
+    bp.evalmod_k_failure_probability_log2 = -16.0
+    bp.evalmod_polynomial_degree = 100
+    bp.linear_transform_log_bsgs_ratios = [1, 1]
+    bp.linear_transform_use_qp = [True, True]
+    c.bootstrap_params = bp
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    K = ctx.getParams().getBootstrapEvalModK()
+    coeffs = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), 100)
+    coeffs[1::2] = 0.0
+    ctx.setEvalModPolynomial(list(coeffs))
+    return ctx
+
+
+def bench_sbox(args, M, torch):
+    """One AES S-box of the mask-packed Walsh circuit at the walsh15
+    parameters: 8 bit ciphertexts at the S2C entry level, 512 blocks per
+    lane, two lanes (real and imaginary slots); parity packing, the
+    complex-packed bootstrap, signs, unpacking, the Walsh output pairs of both
+    lanes and the split into 8 output bits, through the C++ host API.  The
+    outputs are decrypted and checked against the AES S-box."""
+    H = _host()
+    t0 = time.perf_counter()
+    ctx = walsh15_context(H)
+    setup_s = time.perf_counter() - t0
+    P = ctx.getParams()
+    slots, bc = P.getSlots(), 512
+    rng = np.random.default_rng(29)
+    x0, x1 = rng.integers(0, 256, bc), rng.integers(0, 256, bc)
+    level = P.getBootstrapS2CDepth()
+    bits = []
+    for k in range(8):
+        v = np.zeros(slots, np.complex128)
+        v[:bc] = ((x0 >> k) & 1) + 1j * ((x1 >> k) & 1)
+        bits.append(H.ckks_encrypt(ctx, v, level))
+    for _ in range(args.warmup):
+        out = H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, bits, bc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, bits, bc)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the AES S-box from the Walsh table (its inverse transform)
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    par = np.array([[bin(m & int(v)).count("1") & 1 for v in range(256)] for m in range(256)])
+    masks = np.arange(256)
+    f = np.stack([4 * table[b][masks & 15, masks >> 4] for b in range(8)]) @ (1 - 2 * par)
+    sbox = ((f < 0).astype(np.int64) * (1 << np.arange(8))[:, None]).sum(axis=0)
+    worst, wrong = 0.0, 0
+    for k, ct in enumerate(out):
+        dec = H.ckks_decrypt(ctx, ct)[:bc]
+        want = ((sbox[x0] >> k) & 1) + 1j * ((sbox[x1] >> k) & 1)
+        worst = max(worst, float(np.max(np.abs(dec - want))))
+        wrong += int(np.sum(np.round(dec.real) != want.real) + np.sum(np.round(dec.imag) != want.imag))
+    return {
+        "metric": "walsh15 AES S-box evaluations/s (mask-packed Walsh circuit, 512 blocks x 2 lanes)",
+        "value": 1e3 / ms, "unit": "sbox/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (exact mod-q integer)", "data": "synthetic (random bytes, seeded keys)",
+        "config": {"workload": "one S-box: parity packing, bootstrap, unpacking, Walsh pairs, bit split",
+                   "byte_substitutions_per_s": 2 * bc * 1e3 / ms, "output_level": out[0].getLevel(),
+                   "max_abs_error": worst, "wrong_bits": wrong, "context_setup_s": setup_s},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def bench_chain(args, M, torch):
     """The plain CKKS primitive chain through the C++ host API: N=2^14, 8 chain
     limbs of 50-bit primes, P = 2 x 50-bit, dnum 4, Hamming weight 192,
@@ -680,7 +775,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -698,7 +793,7 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
-           "hoisted": bench_hoisted, "walsh": bench_walsh}[args.workload](args, M, torch)
+           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 
