@@ -60,8 +60,10 @@ struct CKKSBootstrapParams {
     size_t evalmod_polynomial_degree = 120;
     size_t evalmod_double_angle = 0;
     size_t evalmod_log_message_ratio = 0;
-    // fit controls of the reference's MPFR EvalMod fit (kept for the API; the
-    // coefficients are supplied with CKKSContext::setEvalModPolynomial)
+    // fit controls of the reference's MPFR EvalMod fit (kept for the API; a
+    // context with bootstrap keys starts from the degree-d Chebyshev
+    // interpolant of 0.5 - 0.5 cos(2 pi K u), replaceable with
+    // CKKSContext::setEvalModPolynomial)
     double evalmod_coefficient_prune_threshold = 1e-10;
     double evalmod_discrete_interval_radius = 0.0;
     size_t evalmod_discrete_validation_points_per_interval = 0;
@@ -73,7 +75,17 @@ struct CKKSBootstrapParams {
 };
 
 // CKKSParamsConfig (ckks_params.hpp:85-117).
+// CKKSAesWalshParams (ckks_params.hpp:72-81): the AES Walsh path's active
+// blocks per bit ciphertext (also the packed segment stride) and the number
+// of segments to provision rotation keys for (0: slots / block_count).
+struct CKKSAesWalshParams {
+    bool enabled = false;
+    size_t block_count = 0;
+    size_t packed_segment_count = 0;
+};
+
 struct CKKSParamsConfig {
+    CKKSAesWalshParams aes_walsh_params;
     double scale = 0.0;
     double sigma = 3.19;
     size_t N = size_t{1} << 15;
@@ -98,6 +110,15 @@ struct CKKSParamsConfig {
     uint64_t seed = 0x5eed;
 };
 
+// AESWalsh15BootstrapParams / AESWalsh15Config (ckks_aes_params.hpp:35-82):
+// N = 2^15, q0 36 bits, two 42-bit mult limbs, scale 2^35, dnum 3, P = 4 x
+// 61 bits, Hamming weight 256, 512 AES blocks per lane with 32 packed
+// segments; bootstrap: sparse weight 32, S2C one stage of 18+20+23, C2S two
+// stages of 25+25, EvalMod 7 x 38 bits with K at failure probability 2^-16,
+// degree 100, QP evaluators for both transforms.
+CKKSBootstrapParams aes_walsh15_bootstrap_params();
+CKKSParamsConfig aes_walsh15_config();
+
 // find_p_and_q_primes (random.cpp:66-131): P bit sizes first, then Q; one
 // descending cursor per bit size from 2^bits/(2N); q = 2kN+1; Miller-Rabin.
 std::pair<std::vector<uint64_t>, std::vector<uint64_t>> find_p_and_q_primes(
@@ -106,6 +127,8 @@ std::pair<std::vector<uint64_t>, std::vector<uint64_t>> find_p_and_q_primes(
 class CKKSParams {
 public:
     explicit CKKSParams(CKKSParamsConfig config);
+    // CKKSParams::AESWalsh15 (ckks_aes_params.hpp:213-215)
+    static CKKSParams AESWalsh15();
     size_t getN() const { return N; }
     size_t getSlots() const { return slots; }
     size_t getDepth() const { return depth; }
@@ -137,6 +160,10 @@ public:
     bool usesSparseBootstrapSlots() const { return getBootstrapActiveSlotCount() != slots; }
     size_t getBootstrapEvalModK() const;
     size_t getEvalModLogScaleBits() const;
+    // ckks_params.cpp:184-205
+    bool usesAesWalsh() const { return aes_walsh.enabled; }
+    size_t getAesWalshBlockCount() const { return aes_walsh.block_count; }
+    size_t getAesWalshPackedSegmentCount() const;
 
     bool bts_keys = false;
     bool secure = true;
@@ -151,6 +178,7 @@ private:
     std::vector<uint64_t> q_bits, p_bits;
     MyVector<uint64_t> moduli, pmoduli;
     MyVector<int64_t> rotate_values;
+    CKKSAesWalshParams aes_walsh;
     uint64_t seed = 0;
 };
 
@@ -554,6 +582,51 @@ void ckks_aes_mask_inactive_slots_inplace(const CKKSContext& context, MyVector<C
                                           size_t block_count);
 void ckks_aes_mask_final_refresh_slots_inplace(const CKKSContext& context, MyVector<CKKSCiphertext>& bits,
                                                size_t active_slot_count, size_t minimum_level_to_keep);
+
+// ------------------------------------------------------------ AES-128 CTR
+// aes.cpp (math/aes): FIPS-197 key expansion and plain AES-128 of one block,
+// the host-side reference for the homomorphic path.
+std::array<uint8_t, 176> aes128_expand_key(const std::array<uint8_t, 16>& key);
+std::array<uint8_t, 16> aes128_encrypt_block_expanded(const std::array<uint8_t, 176>& expanded_round_keys,
+                                                      const std::array<uint8_t, 16>& block);
+std::array<uint8_t, 16> aes128_encrypt_block(const std::array<uint8_t, 16>& key, const std::array<uint8_t, 16>& block);
+std::array<uint8_t, 16> aes128_ctr_crypt_block(const std::array<uint8_t, 16>& key,
+                                               const std::array<uint8_t, 16>& counter_block,
+                                               const std::array<uint8_t, 16>& input_block);
+// The default EvalMod polynomial of a bootstrapping context (Chebyshev
+// coefficients, degree evalmod_polynomial_degree).
+std::vector<double> default_evalmod_polynomial(const CKKSParams& params);
+const std::array<uint8_t, 256>& aes_sbox_table();
+
+namespace ckks_detail {
+// scale_up_ciphertext_to_match_inplace (ckks_walsh_sbox.cpp:228-270): the
+// scale raised to `target` by an integer factor (an all-zero ciphertext only
+// takes the new scale).
+void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double target);
+}  // namespace ckks_detail
+
+// ckks_aes_ctr_walsh_sbox.hpp:16-21 / .cpp:979-1035: the 1408 expanded
+// round-key bits, each encrypted in the first `active_complex_block_count`
+// slots (real and imaginary lanes) at the AES state entry level (S2C depth
+// + 1); rounds 1..9 at scale / 4, rounds 0 and 10 at the scale.
+MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(
+    const CKKSContext& context, const std::array<uint8_t, 176>& expanded_round_keys,
+    size_t active_complex_block_count);
+
+struct CKKSAesCtrWalshSboxOptions {
+    bool parallelize_sbox_pairs = true;
+};
+
+// ckks_aes_ctr_walsh_sbox.hpp:23-35 / .cpp:936-977, 1037-1051: AES-128 CTR
+// decryption with the mask-packed Walsh S-box.  Block s of the first half
+// of each span rides in the real lane of slot s, block n/2 + s in the
+// imaginary lane; the 128 returned bit ciphertexts have been through the
+// final binary refresh.
+MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(
+    CKKSContext& context, const MyVector<CKKSCiphertext>& encrypted_expanded_round_key_bits,
+    std::span<const std::array<uint8_t, 16>> counter_blocks,
+    std::span<const std::array<uint8_t, 16>> ciphertext_blocks, const CKKSAesCtrWalshSboxOptions& options = {});
+
 
 // Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
 // coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].

@@ -1,0 +1,358 @@
+// AES-128 CTR decryption over CKKS with the mask-packed Walsh S-box
+// (ckks_aes_ctr_walsh_sbox.cpp), and the plain AES-128 it is checked against
+// (math/aes/aes.cpp).
+//
+// The state is 128 bit ciphertexts (byte b, bit k at 8b + k; byte b = column
+// b / 4, row b % 4).  XOR is a lazy addition: bits are sums of 0/1 values
+// and the S-box's parity packing only reads them mod 2, so AddRoundKey,
+// MixColumns and the public counter / ciphertext bits are exact additions of
+// ciphertexts (with the scale raised by an integer factor where the operands'
+// scales differ).  Every step is exact arithmetic mod q in the same order of
+// operations as the reference, so each limb equals the reference's.
+#include <algorithm>
+#include <cmath>
+
+#include "ckks.hpp"
+
+namespace {
+
+constexpr size_t kBlockBytes = 16;
+constexpr size_t kBlockBits = 128;
+constexpr size_t kRounds = 10;
+constexpr size_t kExpandedKeyBits = 176 * 8;
+constexpr size_t kMinimumPackedSegments = 30;  // kWalshMinimumPackedSegments
+
+using Block = std::array<uint8_t, 16>;
+using CKKSByte = MyVector<CKKSCiphertext>;
+
+uint8_t xtime(uint8_t v) { return (uint8_t)((v << 1) ^ ((v & 0x80) ? 0x1b : 0)); }
+
+}  // namespace
+
+// FIPS-197 5.2: words w[0..43], w[i] = w[i-4] ^ (i % 4 == 0 ?
+// SubWord(RotWord(w[i-1])) ^ Rcon[i/4] : w[i-1]); bytes in word order.
+std::array<uint8_t, 176> aes128_expand_key(const std::array<uint8_t, 16>& key) {
+    const auto& sbox = aes_sbox_table();
+    std::array<uint8_t, 176> w{};
+    std::copy(key.begin(), key.end(), w.begin());
+    uint8_t rcon = 1;
+    for (size_t i = 4; i < 44; ++i) {
+        uint8_t t[4] = {w[4 * i - 4], w[4 * i - 3], w[4 * i - 2], w[4 * i - 1]};
+        if (i % 4 == 0) {
+            const uint8_t t0 = t[0];
+            t[0] = (uint8_t)(sbox[t[1]] ^ rcon);
+            t[1] = sbox[t[2]];
+            t[2] = sbox[t[3]];
+            t[3] = sbox[t0];
+            rcon = xtime(rcon);
+        }
+        for (size_t k = 0; k < 4; ++k) w[4 * i + k] = (uint8_t)(w[4 * i - 16 + k] ^ t[k]);
+    }
+    return w;
+}
+
+// FIPS-197 5.1: state byte 4c + r is row r of column c.
+std::array<uint8_t, 16> aes128_encrypt_block_expanded(const std::array<uint8_t, 176>& rk,
+                                                      const std::array<uint8_t, 16>& in) {
+    const auto& sbox = aes_sbox_table();
+    Block s;
+    for (size_t i = 0; i < 16; ++i) s[i] = (uint8_t)(in[i] ^ rk[i]);
+    for (size_t round = 1; round <= kRounds; ++round) {
+        Block t;
+        for (size_t c = 0; c < 4; ++c)
+            for (size_t r = 0; r < 4; ++r) t[4 * c + r] = sbox[s[4 * ((c + r) % 4) + r]];
+        if (round != kRounds)
+            for (size_t c = 0; c < 4; ++c) {
+                const uint8_t a0 = t[4 * c], a1 = t[4 * c + 1], a2 = t[4 * c + 2], a3 = t[4 * c + 3];
+                t[4 * c + 0] = (uint8_t)(xtime(a0 ^ a1) ^ a1 ^ a2 ^ a3);
+                t[4 * c + 1] = (uint8_t)(xtime(a1 ^ a2) ^ a2 ^ a3 ^ a0);
+                t[4 * c + 2] = (uint8_t)(xtime(a2 ^ a3) ^ a3 ^ a0 ^ a1);
+                t[4 * c + 3] = (uint8_t)(xtime(a3 ^ a0) ^ a0 ^ a1 ^ a2);
+            }
+        for (size_t i = 0; i < 16; ++i) s[i] = (uint8_t)(t[i] ^ rk[16 * round + i]);
+    }
+    return s;
+}
+
+// aes128_encrypt_block / aes128_ctr_crypt_block (math/aes/aes.cpp:100-125)
+std::array<uint8_t, 16> aes128_encrypt_block(const std::array<uint8_t, 16>& key, const std::array<uint8_t, 16>& in) {
+    return aes128_encrypt_block_expanded(aes128_expand_key(key), in);
+}
+
+std::array<uint8_t, 16> aes128_ctr_crypt_block(const std::array<uint8_t, 16>& key, const std::array<uint8_t, 16>& counter,
+                                               const std::array<uint8_t, 16>& input) {
+    const auto keystream = aes128_encrypt_block(key, counter);
+    std::array<uint8_t, 16> out;
+    for (size_t i = 0; i < 16; ++i) out[i] = (uint8_t)(input[i] ^ keystream[i]);
+    return out;
+}
+
+namespace {
+
+// pad_ciphertext_with_zero_polys (ckks_aes_ctr_walsh_sbox.cpp:58-66)
+void pad_with_zero_polys(const CKKSContext& context, CKKSCiphertext& ct, size_t target) {
+    if (target < ct.getNumPolys()) throw std::invalid_argument("ckks_aes_walsh: cannot shrink ciphertext degree");
+    while (ct.getNumPolys() < target) {
+        ct.addPoly();
+        mi_check(mi_memset_zero(context.handle(), ct.polyData(ct.getNumPolys() - 1),
+                                (ct.getLevel() + 1) * ct.getN() * sizeof(uint64_t), context.stream()),
+                 "ckks_aes_walsh");
+    }
+}
+
+// align_for_add_exact / add_term_inplace (:128-147): both operands to the
+// lower level, the higher scale (an integer scale-up) and the larger degree,
+// then one addition.
+void add_term(const CKKSContext& context, CKKSCiphertext& acc, CKKSCiphertext term) {
+    const size_t level = std::min(acc.getLevel(), term.getLevel());
+    const long double scale = std::max(acc.getScale(), term.getScale());
+    const size_t polys = std::max(acc.getNumPolys(), term.getNumPolys());
+    for (CKKSCiphertext* ct : {&acc, &term}) {
+        ckks_drop_to_level_inplace(*ct, level);
+        pad_with_zero_polys(context, *ct, polys);
+        ckks_detail::scale_up_to(context, *ct, scale);
+    }
+    ckks_add_inplace(context, acc, term);
+}
+
+CKKSCiphertext add_bits(const CKKSContext& context, const CKKSCiphertext& lhs, const CKKSCiphertext& rhs) {
+    auto out = lhs.clone();
+    add_term(context, out, rhs.clone());
+    return out;
+}
+
+// walsh_packing_block_count (:154-171)
+size_t packing_block_count(const CKKSContext& context, size_t active) {
+    const auto& P = context.getParams();
+    const size_t pack = P.usesAesWalsh() ? P.getAesWalshBlockCount() : P.getSlots() / kMinimumPackedSegments;
+    const size_t segments = P.usesAesWalsh() ? P.getAesWalshPackedSegmentCount() : kMinimumPackedSegments;
+    if (active == 0 || pack == 0 || segments < kMinimumPackedSegments || active > pack ||
+        segments > P.getSlots() / pack)
+        throw std::invalid_argument("ckks_aes_walsh: active blocks exceed padded Walsh packing capacity");
+    return pack;
+}
+
+// aes_walsh_state_entry_level (:173-175): one selection limb above S2C's input
+size_t state_entry_level(const CKKSContext& context) { return context.getParams().getBootstrapS2CDepth() + 1; }
+
+// encode_active_complex_block_public_bit (:177-197): bit `bit` of block s in
+// the real lane and of block n + s in the imaginary lane of slot s.
+CKKSEncoding encode_public_bit(const CKKSContext& context, std::span<const Block> blocks, size_t bit, size_t level,
+                               long double scale) {
+    const size_t n = blocks.size() / 2;
+    MyVector<Complex> slots(context.getParams().getSlots(), Complex(0.0, 0.0));
+    for (size_t s = 0; s < n; ++s)
+        slots[s] = Complex((double)((blocks[s][bit / 8] >> (bit % 8)) & 1),
+                           (double)((blocks[n + s][bit / 8] >> (bit % 8)) & 1));
+    return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
+}
+
+// In-place rotations of many ciphertexts of one level as one batch
+// (each equal to ckks_rotate_inplace).
+void rotate_batch(const CKKSContext& context, std::vector<CKKSCiphertext*>& cts, const std::vector<int64_t>& rot) {
+    if (cts.empty()) return;
+    const size_t N = context.getParams().getN();
+    std::vector<uint64_t*> b, a;
+    std::vector<uint64_t> galois;
+    std::vector<const uint64_t*> keys;
+    for (size_t i = 0; i < cts.size(); ++i) {
+        const int64_t r = normalize_rotation_value(N, rot[i]);
+        if (!context.hasRotationKey(r)) throw std::invalid_argument("ckks_aes_walsh: missing refresh rotation key");
+        if (cts[i]->getNumPolys() != 2 || cts[i]->getLevel() != cts[0]->getLevel())
+            throw std::invalid_argument("ckks_aes_walsh: refresh rotation inputs differ in shape");
+        b.push_back(cts[i]->polyData(0));
+        a.push_back(cts[i]->polyData(1));
+        galois.push_back(rotation_automorphism_index(N, r));
+        keys.push_back(context.getRotationKeyDevice(r));
+    }
+    mi_check(mi_rotate_batch(context.handle(), (int)cts[0]->getLevel(), (int)b.size(), b.data(), a.data(),
+                             galois.data(), keys.data(), context.getParams().montgomery ? 1 : 0, context.stream()),
+             "ckks_rotate");
+}
+
+// final_binary_refresh_complex_block_output_bits_inplace (:406-490): mask,
+// pack slots / n bits per ciphertext (bit first + s rotated by s n), refresh
+// the packed real and imaginary lanes with one complex-packed bootstrap,
+// recombine them, rotate each bit back and mask again.
+void final_binary_refresh(const CKKSContext& context, MyVector<CKKSCiphertext>& bits, size_t active) {
+    if (bits.size() != kBlockBits)
+        throw std::invalid_argument("ckks_aes_walsh: expected 128 output bits for final refresh");
+    const auto& P = context.getParams();
+    const size_t stride = active;
+    const size_t per_batch = stride == 0 ? 0 : P.getSlots() / stride;
+    if (stride == 0 || per_batch == 0 || stride * per_batch > P.getSlots())
+        throw std::invalid_argument("ckks_aes_walsh: invalid final refresh packing layout");
+    ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, P.getBootstrapS2CDepth());
+    const size_t batches = (kBlockBits + per_batch - 1) / per_batch;
+    MyVector<CKKSCiphertext> refreshed;
+    refreshed.reserve(kBlockBits);
+    for (size_t batch = 0; batch < batches; ++batch) {
+        const size_t first = batch * per_batch, count = std::min(per_batch, kBlockBits - first);
+        // pack_final_refresh_lane (:347-367): zero_like + each rotated term
+        MyVector<CKKSCiphertext> terms;
+        terms.reserve(count);
+        std::vector<CKKSCiphertext*> to_rotate;
+        std::vector<int64_t> rot;
+        for (size_t s = 0; s < count; ++s) terms.push_back(bits[first + s].clone());
+        for (size_t s = 1; s < count; ++s) {
+            to_rotate.push_back(&terms[s]);
+            rot.push_back((int64_t)(s * stride));
+        }
+        rotate_batch(context, to_rotate, rot);
+        auto packed = std::move(terms[0]);
+        for (size_t s = 1; s < count; ++s) ckks_add_inplace(context, packed, terms[s]);
+        auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(context, std::move(packed));
+        // combine_real_imag_for_final_refresh (:392-404)
+        auto imag_i = ckks_mul(context, lanes.second, Complex(0.0, 1.0), 1.0);
+        imag_i.setScale(lanes.second.getScale());
+        auto combined = std::move(lanes.first);
+        ckks_add_inplace(context, combined, imag_i);
+        // unpack_final_refresh_lane_inplace (:369-390)
+        to_rotate.clear();
+        rot.clear();
+        const size_t base = refreshed.size();
+        for (size_t s = 0; s < count; ++s) refreshed.push_back(combined.clone());
+        for (size_t s = 1; s < count; ++s) {
+            to_rotate.push_back(&refreshed[base + s]);
+            rot.push_back(-(int64_t)(s * stride));
+        }
+        rotate_batch(context, to_rotate, rot);
+    }
+    bits = std::move(refreshed);
+    ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, /*minimum_level_to_keep=*/0);
+}
+
+// shift_rows_bits_inplace (:723-739): new byte 4c + r <- old byte 4((c + r) % 4) + r
+void shift_rows(MyVector<CKKSCiphertext>& state) {
+    MyVector<CKKSCiphertext> old = std::move(state);
+    state.clear();
+    state.reserve(kBlockBits);
+    for (size_t c = 0; c < 4; ++c)
+        for (size_t r = 0; r < 4; ++r) {
+            const size_t src = 4 * ((c + r) % 4) + r;
+            for (size_t k = 0; k < 8; ++k) state.push_back(std::move(old[src * 8 + k]));
+        }
+}
+
+// fmix_column_byte (:611-641): xtime(b0 + b1) + b1 + b2 + b3, where
+// xtime(x) = (x7, x0 + x7, x1, x2 + x7, x3 + x7, x4, x5, x6).
+CKKSByte mix_column_byte(const CKKSContext& context, const MyVector<CKKSCiphertext>& st, size_t i0, size_t i1,
+                         size_t i2, size_t i3) {
+    CKKSByte sum;
+    sum.reserve(8);
+    for (size_t k = 0; k < 8; ++k) sum.push_back(add_bits(context, st[8 * i0 + k], st[8 * i1 + k]));
+    CKKSByte out;
+    out.reserve(8);
+    out.push_back(sum[7].clone());
+    out.push_back(add_bits(context, sum[0], sum[7]));
+    out.push_back(sum[1].clone());
+    out.push_back(add_bits(context, sum[2], sum[7]));
+    out.push_back(add_bits(context, sum[3], sum[7]));
+    out.push_back(sum[4].clone());
+    out.push_back(sum[5].clone());
+    out.push_back(sum[6].clone());
+    for (size_t idx : {i1, i2, i3})
+        for (size_t k = 0; k < 8; ++k) add_term(context, out[k], st[8 * idx + k].clone());
+    return out;
+}
+
+// fmix_columns_bits_inplace (:741-783)
+void mix_columns(const CKKSContext& context, MyVector<CKKSCiphertext>& state) {
+    std::array<CKKSByte, kBlockBytes> bytes;
+    for (size_t c = 0; c < 4; ++c) {
+        const size_t b0 = 4 * c, b1 = b0 + 1, b2 = b0 + 2, b3 = b0 + 3;
+        bytes[b0] = mix_column_byte(context, state, b0, b1, b2, b3);
+        bytes[b1] = mix_column_byte(context, state, b1, b2, b3, b0);
+        bytes[b2] = mix_column_byte(context, state, b2, b3, b0, b1);
+        bytes[b3] = mix_column_byte(context, state, b3, b0, b1, b2);
+    }
+    state.clear();
+    for (auto& byte : bytes)
+        for (auto& bit : byte) state.push_back(std::move(bit));
+}
+
+// add_round_key_lazy_inplace (:785-817)
+void add_round_key(const CKKSContext& context, MyVector<CKKSCiphertext>& state, const MyVector<CKKSCiphertext>& keys,
+                   size_t round) {
+    for (size_t bit = 0; bit < kBlockBits; ++bit) add_term(context, state[bit], keys[round * kBlockBits + bit].clone());
+}
+
+// sub_bytes_walsh_complex_blocks_mask_packed (:659-721): the 16 S-boxes
+MyVector<CKKSCiphertext> sub_bytes(CKKSContext& context, const MyVector<CKKSCiphertext>& state, size_t pack) {
+    MyVector<CKKSCiphertext> out;
+    out.reserve(kBlockBits);
+    for (size_t byte = 0; byte < kBlockBytes; ++byte) {
+        MyVector<CKKSCiphertext> input;
+        input.reserve(8);
+        for (size_t k = 0; k < 8; ++k) input.push_back(state[8 * byte + k].clone());
+        auto bits = ckks_walsh_sbox_complex_blocks_mask_packed(context, std::move(input), pack);
+        if (bits.size() != 8) throw std::runtime_error("ckks_aes_walsh: complex-block Walsh S-box returned wrong size");
+        for (auto& b : bits) out.push_back(std::move(b));
+    }
+    return out;
+}
+
+}  // namespace
+
+MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(
+    const CKKSContext& context, const std::array<uint8_t, 176>& rk, size_t active) {
+    const auto& P = context.getParams();
+    packing_block_count(context, active);
+    const size_t level = state_entry_level(context);
+    if (level > P.getMaxLevel())
+        throw std::invalid_argument("ckks_aes_walsh: selection key level exceeds parameter chain");
+    MyVector<CKKSCiphertext> out;
+    out.reserve(kExpandedKeyBits);
+    for (size_t bit = 0; bit < kExpandedKeyBits; ++bit) {
+        const double v = ((rk[bit / 8] >> (bit % 8)) & 1) ? 1.0 : 0.0;
+        MyVector<Complex> slots(P.getSlots(), Complex(0.0, 0.0));
+        for (size_t s = 0; s < active; ++s) slots[s] = Complex(v, v);
+        const double scale = (bit < kBlockBits || bit >= kRounds * kBlockBits) ? P.getScale() : P.getScale() / 4.0;
+        out.push_back(ckks_encrypt(context, ckks_encode(context, slots, scale, level, /*eval=*/true), /*eval=*/true));
+    }
+    return out;
+}
+
+MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& context,
+                                                                const MyVector<CKKSCiphertext>& keys,
+                                                                std::span<const Block> counters,
+                                                                std::span<const Block> ciphertexts,
+                                                                const CKKSAesCtrWalshSboxOptions&) {
+    // validate_complex_block_inputs (:905-934)
+    if (keys.size() != kExpandedKeyBits)
+        throw std::invalid_argument("ckks_aes_walsh: expanded round key must have 1408 bits");
+    if (counters.empty() || counters.size() != ciphertexts.size() || counters.size() % 2 != 0)
+        throw std::invalid_argument(
+            "ckks_aes_walsh: complex-block counter/ciphertext spans must be non-empty, equal, and even");
+    const size_t active = counters.size() / 2;
+    const size_t pack = packing_block_count(context, active);
+    const size_t key_level = state_entry_level(context);
+    if (key_level > context.getParams().getMaxLevel())
+        throw std::invalid_argument("ckks_aes_walsh: Walsh path requires a selection limb above S2C input level");
+    for (const auto& ct : keys)
+        if (ct.getN() != context.getParams().getN() || ct.getNumPolys() != 2 || ct.getLevel() != key_level)
+            throw std::invalid_argument(
+                "ckks_aes_walsh: complex-block path received a round key bit at the wrong level");
+    // make_initial_counter_complex_block_state (:534-559): round key 0 + counter
+    MyVector<CKKSCiphertext> state;
+    state.reserve(kBlockBits);
+    for (size_t bit = 0; bit < kBlockBits; ++bit) {
+        state.push_back(keys[bit].clone());
+        ckks_add_inplace(context, state.back(),
+                         encode_public_bit(context, counters, bit, state.back().getLevel(), state.back().getScale()));
+    }
+    // evaluate_aes128_rounds_walsh_complex_blocks_impl (:819-903)
+    for (size_t round = 1; round <= kRounds; ++round) {
+        ckks_aes_mask_inactive_slots_inplace(context, state, pack);
+        state = sub_bytes(context, state, pack);
+        shift_rows(state);
+        if (round != kRounds) mix_columns(context, state);
+        add_round_key(context, state, keys, round);
+    }
+    // CTR: plaintext = ciphertext + AES(counter) (add_public_complex_block_bits_lazy_inplace, :492-532)
+    for (size_t bit = 0; bit < kBlockBits; ++bit)
+        ckks_add_inplace(context, state[bit],
+                         encode_public_bit(context, ciphertexts, bit, state[bit].getLevel(), state[bit].getScale()));
+    final_binary_refresh(context, state, active);
+    return state;
+}

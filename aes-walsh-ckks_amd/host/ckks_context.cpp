@@ -175,6 +175,7 @@ CKKSParams::CKKSParams(CKKSParamsConfig c) {
     sigma = c.sigma;
     scale = c.scale > 0.0 ? c.scale : std::exp2((double)c.q_bit);
     rotate_values = c.rotate_values;
+    aes_walsh = c.aes_walsh_params;
     secure = c.secure;
     fold_hybrid_keyswitch = c.fold_hybrid_keyswitch;
     montgomery = c.montgomery;
@@ -231,6 +232,11 @@ const CKKSBootstrapParams& CKKSParams::getBootstrapParams() const {
 static size_t lt_depth(const CKKSBootstrapParams& b, size_t i) {
     const auto& s = b.linear_transform_log_plaintext_scales[i];
     return s.empty() ? b.linear_transform_depths[i] : s.size();
+}
+
+size_t CKKSParams::getAesWalshPackedSegmentCount() const {
+    if (aes_walsh.packed_segment_count != 0) return aes_walsh.packed_segment_count;
+    return aes_walsh.block_count == 0 ? 0 : slots / aes_walsh.block_count;
 }
 
 size_t CKKSParams::getBootstrapS2CDepth() const { return lt_depth(getBootstrapParams(), 0); }
@@ -496,7 +502,25 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
             const int64_t n = normalize_rotation_value(N, r);
             if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
         }
-        if (params.bts_keys) build_bootstrap_material(rng);
+        // append_aes_walsh_rotation_values (ckks_context.cpp:274-305):
+        // +-segment * block_count for every provisioned segment
+        if (params.usesAesWalsh()) {
+            const size_t blocks = params.getAesWalshBlockCount(), segments = params.getAesWalshPackedSegmentCount();
+            if (blocks == 0) throw std::invalid_argument("ckks_context: AES Walsh block count must be positive");
+            if (segments < 30)
+                throw std::invalid_argument("ckks_context: AES Walsh packed segment count must be at least 30");
+            if (segments > params.getSlots() / blocks)
+                throw std::invalid_argument("ckks_context: AES Walsh block count exceeds packed slot capacity");
+            for (size_t s = 1; s < segments; ++s)
+                for (int64_t r : {(int64_t)(s * blocks), -(int64_t)(s * blocks)}) {
+                    const int64_t n = normalize_rotation_value(N, r);
+                    if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
+                }
+        }
+        if (params.bts_keys) {
+            build_bootstrap_material(rng);
+            evalmod_coeffs = default_evalmod_polynomial(params);
+        }
     } catch (...) {
         release_all();
         mi_context_destroy(mi);
@@ -692,6 +716,67 @@ const CKKSLinearTransformPlan& CKKSContext::getScaledS2CPlan(uint64_t ratio) con
     const CKKSLinearTransformPlan& ref = *plan;
     scaled_s2c.emplace(ratio, std::move(plan));
     return ref;
+}
+
+CKKSBootstrapParams aes_walsh15_bootstrap_params() {
+    CKKSBootstrapParams b;
+    b.sparse_secret_hamming_weight = 32;
+    b.linear_transform_log_plaintext_scales = {CKKSBootstrapDftScaleSchedule{{18, 20, 23}},
+                                               CKKSBootstrapDftScaleSchedule{{25, 25}, {25, 25}}};
+    b.linear_transform_depths = {1, 2};
+    b.evalmod_depth = 7;
+    b.evalmod_bit_size = 38;
+    b.evalmod_k_failure_probability_log2 = -16.0L;
+    b.evalmod_polynomial_degree = 100;
+    b.evalmod_double_angle = 0;
+    b.evalmod_coefficient_prune_threshold = 1e-10;
+    b.evalmod_discrete_interval_radius = std::nextafter(0.25, 0.0);
+    b.evalmod_discrete_validation_points_per_interval = 128;
+    b.linear_transform_log_bsgs_ratios = {1, 1};
+    b.linear_transform_use_qp = {true, true};
+    return b;
+}
+
+CKKSParamsConfig aes_walsh15_config() {
+    CKKSParamsConfig c;
+    c.secure = true;
+    c.bts_keys = true;
+    c.N = size_t{1} << 15;
+    c.mult_depth = 2;
+    c.q0_bit = 36;
+    c.q_mult_bits = {42, 42};
+    c.scale = std::exp2(35.0);
+    c.dnum = 3;
+    c.pmoduli_count = 4;
+    c.p_bits = {61, 61, 61, 61};
+    c.secret_key_hamming_weight = 256;
+    c.aes_walsh_params.enabled = true;
+    c.aes_walsh_params.block_count = 512;
+    c.aes_walsh_params.packed_segment_count = 32;
+    c.bootstrap_params = aes_walsh15_bootstrap_params();
+    return c;
+}
+
+CKKSParams CKKSParams::AESWalsh15() { return CKKSParams(aes_walsh15_config()); }
+
+// The degree-d Chebyshev interpolant (points cos(pi (j + 1/2) / (d + 1)))
+// of 0.5 - 0.5 cos(2 pi K u) on [-1, 1], odd terms zero (the function is even).
+std::vector<double> default_evalmod_polynomial(const CKKSParams& params) {
+    const size_t d = params.getBootstrapEvalModPolynomialDegree(), n = d + 1;
+    const long double K = (long double)params.getBootstrapEvalModK();
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> f(n);
+    for (size_t j = 0; j < n; ++j) {
+        const long double x = std::cos(pi * ((long double)j + 0.5L) / (long double)n);
+        f[j] = 0.5L - 0.5L * std::cos(2.0L * pi * K * x);
+    }
+    std::vector<double> c(n, 0.0);
+    for (size_t k = 0; k < n; k += 2) {
+        long double acc = 0.0L;
+        for (size_t j = 0; j < n; ++j) acc += f[j] * std::cos(pi * (long double)k * ((long double)j + 0.5L) / (long double)n);
+        c[k] = (double)(acc * (k == 0 ? 1.0L : 2.0L) / (long double)n);
+    }
+    return c;
 }
 
 void CKKSContext::setEvalModPolynomial(const std::vector<double>& c) {

@@ -1,6 +1,8 @@
 // Python binding of the C++ host API (tests and the benchmark drive the
 // library through it).  Device-resident objects stay on the device; polys,
 // plaintexts and keys cross into numpy only through explicit downloads.
+#include <cstring>
+
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -33,6 +35,22 @@ py::array_t<std::complex<double>> complex_np(const MyVector<Complex>& v) {
 }
 
 }  // namespace
+
+// The 1408 encrypted round-key bits, held on the C++ side so that each AES
+// call reads them without copying them into Python objects.
+struct AesRoundKeyBits {
+    explicit AesRoundKeyBits(MyVector<CKKSCiphertext> b) : bits(std::move(b)) {}
+    AesRoundKeyBits(const AesRoundKeyBits&) = delete;
+    AesRoundKeyBits(AesRoundKeyBits&&) = default;
+    MyVector<CKKSCiphertext> bits;
+};
+
+std::vector<std::array<uint8_t, 16>> blocks_from_array(const py::array_t<uint8_t, py::array::c_style>& a) {
+    if (a.ndim() != 2 || a.shape(1) != 16) throw std::invalid_argument("blocks must be a uint8 array of shape [n, 16]");
+    std::vector<std::array<uint8_t, 16>> out((size_t)a.shape(0));
+    for (size_t i = 0; i < out.size(); ++i) std::memcpy(out[i].data(), a.data(i, 0), 16);
+    return out;
+}
 
 PYBIND11_MODULE(ckks_py, m) {
     m.doc() = "C++ host API of the MI355X CKKS library";
@@ -67,6 +85,12 @@ PYBIND11_MODULE(ckks_py, m) {
         .def_readwrite("evalmod_polynomial_degree", &CKKSBootstrapParams::evalmod_polynomial_degree)
         .def_readwrite("evalmod_log_message_ratio", &CKKSBootstrapParams::evalmod_log_message_ratio);
 
+    py::class_<CKKSAesWalshParams>(m, "CKKSAesWalshParams")
+        .def(py::init<>())
+        .def_readwrite("enabled", &CKKSAesWalshParams::enabled)
+        .def_readwrite("block_count", &CKKSAesWalshParams::block_count)
+        .def_readwrite("packed_segment_count", &CKKSAesWalshParams::packed_segment_count);
+
     py::class_<CKKSParamsConfig>(m, "CKKSParamsConfig")
         .def_readwrite("bts_keys", &CKKSParamsConfig::bts_keys)
         .def_readwrite("bootstrap_params", &CKKSParamsConfig::bootstrap_params)
@@ -86,12 +110,19 @@ PYBIND11_MODULE(ckks_py, m) {
         .def_readwrite("q_bits", &CKKSParamsConfig::q_bits)
         .def_readwrite("rotate_values", &CKKSParamsConfig::rotate_values)
         .def_readwrite("secure", &CKKSParamsConfig::secure)
+        .def_readwrite("aes_walsh_params", &CKKSParamsConfig::aes_walsh_params)
         .def_readwrite("fold_hybrid_keyswitch", &CKKSParamsConfig::fold_hybrid_keyswitch)
         .def_readwrite("montgomery", &CKKSParamsConfig::montgomery)
         .def_readwrite("seed", &CKKSParamsConfig::seed);
 
+    m.def("aes_walsh15_config", &aes_walsh15_config);
+    m.def("aes_walsh15_bootstrap_params", &aes_walsh15_bootstrap_params);
     py::class_<CKKSParams>(m, "CKKSParams")
         .def(py::init<CKKSParamsConfig>())
+        .def_static("AESWalsh15", &CKKSParams::AESWalsh15)
+        .def("usesAesWalsh", &CKKSParams::usesAesWalsh)
+        .def("getAesWalshBlockCount", &CKKSParams::getAesWalshBlockCount)
+        .def("getAesWalshPackedSegmentCount", &CKKSParams::getAesWalshPackedSegmentCount)
         .def("getN", &CKKSParams::getN)
         .def("getSlots", &CKKSParams::getSlots)
         .def("getMaxLevel", &CKKSParams::getMaxLevel)
@@ -476,4 +507,28 @@ PYBIND11_MODULE(ckks_py, m) {
           },
           py::arg("context"), py::arg("x"), py::arg("coefficients"), py::arg("target_scale") = 0.0,
           py::keep_alive<0, 1>());
+
+    py::class_<AesRoundKeyBits>(m, "CKKSAesRoundKeyBits")
+        .def("__len__", [](const AesRoundKeyBits& k) { return k.bits.size(); })
+        .def("get", [](const AesRoundKeyBits& k, size_t i) { return k.bits.at(i).clone(); }, py::keep_alive<0, 1>());
+    m.def("aes128_expand_key", &aes128_expand_key);
+    m.def("aes128_encrypt_block", &aes128_encrypt_block);
+    m.def("aes128_encrypt_block_expanded", &aes128_encrypt_block_expanded);
+    m.def("aes128_ctr_crypt_block", &aes128_ctr_crypt_block);
+    m.def("default_evalmod_polynomial", &default_evalmod_polynomial);
+    m.def("aes_sbox_table", []() { return aes_sbox_table(); });
+    m.def("ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks",
+          [](const CKKSContext& c, const std::array<uint8_t, 176>& rk, size_t active) {
+              return AesRoundKeyBits(ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(c, rk, active));
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_aes_ctr_walsh_sbox_complex_blocks",
+          [ct_pylist](const py::object& context, const AesRoundKeyBits& keys,
+                      const py::array_t<uint8_t, py::array::c_style>& counters,
+                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts) {
+              CKKSContext& c = context.cast<CKKSContext&>();
+              const auto ctr = blocks_from_array(counters), cts = blocks_from_array(ciphertexts);
+              auto out = ckks_aes_ctr_walsh_sbox_complex_blocks(c, keys.bits, ctr, cts);
+              return ct_pylist(out, context);
+          });
 }

@@ -114,9 +114,20 @@ CKKSEncoding constant_encoding(const CKKSContext& context, Complex value, long d
 
 }  // namespace
 
+const std::array<uint8_t, 256>& aes_sbox_table() {
+    static const std::array<uint8_t, 256> table = aes_sbox();
+    return table;
+}
+
+namespace ckks_detail {
+void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double target) {
+    ::scale_up_to(context, ct, target);
+}
+}  // namespace ckks_detail
+
 const CKKSWalshTable& ckks_aes_walsh_table() {
     static const CKKSWalshTable table = [] {
-        const auto sbox = aes_sbox();
+        const auto& sbox = aes_sbox_table();
         CKKSWalshTable t{};
         for (size_t bit = 0; bit < 8; ++bit)
             for (size_t lo = 0; lo < kMasks; ++lo)

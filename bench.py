@@ -6,7 +6,7 @@ Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
 of the whole batch in one launch pair followed by the inverse NTT.  The
 plain form (one limb per launch) is timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh|sbox]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh|sbox|aes]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -680,6 +680,77 @@ def bench_sbox(args, M, torch):
     }
 
 
+AES_KEY = bytes.fromhex("2b7e151628aed2a6abf7158809cf4f3c")
+
+
+def aes_bench_blocks(H, count):
+    """The blocks of the reference's AES bench (aes_bench.cpp:29-101): counter
+    f0f1..ff + i (big-endian), plaintext 0 the SP 800-38A block and block i
+    bytes (0x5a + 17 i + 29 b) & 0xff, ciphertext = AES-CTR under the test key."""
+    base = int.from_bytes(bytes.fromhex("f0f1f2f3f4f5f6f7f8f9fafbfcfdfeff"), "big")
+    counters = np.array([list((base + i).to_bytes(16, "big")) for i in range(count)], np.uint8)
+    plains = np.array([list(bytes.fromhex("6bc1bee22e409f96e93d7e117393172a")) if i == 0 else
+                       [(0x5A + 17 * i + 29 * b) & 0xFF for b in range(16)] for i in range(count)], np.uint8)
+    cts = np.array([H.aes128_ctr_crypt_block(list(AES_KEY), list(c), list(p)) for c, p in zip(counters, plains)],
+                   np.uint8)
+    return counters, plains, cts
+
+
+def aes_validate(H, ctx, bits, expected, cb):
+    """validate_walsh_outputs (aes_bench.cpp:156-196): mismatched blocks, max
+    and average error of the decrypted bits."""
+    decoded = np.zeros((2 * cb, 16), np.uint8)
+    errs = []
+    for bit in range(128):
+        v = np.asarray(H.ckks_decrypt(ctx, bits[bit]))[:cb]
+        for lane, vals in enumerate((v.real, v.imag)):
+            rows = np.arange(cb) + lane * cb
+            want = (expected[rows, bit // 8] >> (bit % 8)) & 1
+            decoded[rows, bit // 8] |= (vals >= 0.5).astype(np.uint8) << (bit % 8)
+            errs.append(np.abs(vals - want))
+    errs = np.concatenate(errs)
+    return int(np.sum(np.any(decoded != expected, axis=1))), float(errs.max()), float(errs.mean())
+
+
+def bench_aes(args, M, torch):
+    """walsh15 AES-CTR (aes_bench.cpp:247-292): 1024 blocks (512 complex
+    slots x 2 lanes) decrypted homomorphically per step through
+    ckks_aes_ctr_walsh_sbox_complex_blocks; the context build and round-key
+    encryption are outside the timed region, as in the reference."""
+    H = _host()
+    t0 = time.perf_counter()
+    ctx = H.CKKSContext(H.CKKSParams.AESWalsh15())
+    cb = ctx.getParams().getAesWalshBlockCount()
+    counters, plains, cts = aes_bench_blocks(H, 2 * cb)
+    keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(list(AES_KEY)), cb)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"# walsh15 context and round keys: {setup_s:.1f} s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+    torch.cuda.synchronize()
+    lat = []
+    for step in range(args.steps):
+        t1 = time.perf_counter()
+        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+        print(f"# step {step}: {lat[-1]:.3f} s", file=sys.stderr, flush=True)
+    ms = 1e3 * sum(lat) / len(lat)
+    mism, max_err, avg_err = aes_validate(H, ctx, out, plains, cb)
+    return {
+        "metric": "walsh15 AES-CTR blocks/s (homomorphic CTR decryption, mask-packed Walsh S-box)",
+        "value": 2 * cb * 1e3 / ms, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 (exact mod-q integer)", "data": "synthetic (the reference bench's counters and plaintexts)",
+        "config": {"workload": "walsh15 AES-CTR, 1024 blocks per evaluation", "blocks": 2 * cb,
+                   "mismatches": mism, "max_error": max_err, "average_error": avg_err,
+                   "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                   "context_and_key_setup_s": setup_s},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def bench_chain(args, M, torch):
     """The plain CKKS primitive chain through the C++ host API: N=2^14, 8 chain
     limbs of 50-bit primes, P = 2 x 50-bit, dnum 4, Hamming weight 192,
@@ -775,7 +846,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox"])
+    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
@@ -793,7 +864,7 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
-           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox}[args.workload](args, M, torch)
+           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox, "aes": bench_aes}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 

@@ -1,0 +1,70 @@
+"""Host AES-128 (the plain reference the homomorphic CTR path is checked
+against), the walsh15 parameter set and the default EvalMod polynomial, on
+the CPU."""
+import numpy as np
+
+from helpers import host
+
+KEY = bytes.fromhex("2b7e151628aed2a6abf7158809cf4f3c")
+
+
+def test_aes128_fips197_known_answers():
+    H = host()
+    rk = bytes(H.aes128_expand_key(list(KEY)))
+    # FIPS-197 Appendix A.1: w[40..43]
+    assert rk[160:].hex() == "d014f9a8c9ee2589e13f0cc8b6630ca6"
+    assert rk[16:20].hex() == "a0fafe17"
+    # FIPS-197 Appendix B
+    pt = list(bytes.fromhex("3243f6a8885a308d313198a2e0370734"))
+    assert bytes(H.aes128_encrypt_block(list(KEY), pt)).hex() == "3925841d02dc09fbdc118597196a0b32"
+    assert bytes(H.aes128_encrypt_block_expanded(list(rk), pt)).hex() == "3925841d02dc09fbdc118597196a0b32"
+    # FIPS-197 Appendix C.1 (another key)
+    k2 = list(range(16))
+    p2 = list(bytes.fromhex("00112233445566778899aabbccddeeff"))
+    assert bytes(H.aes128_encrypt_block(k2, p2)).hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+
+
+def test_aes128_ctr_sp800_38a_first_block():
+    """NIST SP 800-38A F.5.1 CTR-AES128.Encrypt, block 1: the counter and
+    plaintext the reference bench starts from (aes_bench.cpp:29-39)."""
+    H = host()
+    ctr = list(bytes.fromhex("f0f1f2f3f4f5f6f7f8f9fafbfcfdfeff"))
+    pt = list(bytes.fromhex("6bc1bee22e409f96e93d7e117393172a"))
+    ct = bytes(H.aes128_ctr_crypt_block(list(KEY), ctr, pt))
+    assert ct.hex() == "874d6191b620e3261bef6864990db6ce"
+    assert bytes(H.aes128_ctr_crypt_block(list(KEY), ctr, list(ct))) == bytes(pt)
+
+
+def test_aes_sbox_table_from_its_definition():
+    H = host()
+    s = H.aes_sbox_table()
+    assert s[0x00] == 0x63 and s[0x01] == 0x7C and s[0x53] == 0xED and s[0xFF] == 0x16
+    assert sorted(s) == list(range(256))
+
+
+def test_walsh15_parameters():
+    """AESWalsh15 (ckks_aes_params.hpp:35-82): the chain, P, K and packing."""
+    H = host()
+    P = H.CKKSParams.AESWalsh15()
+    assert P.getN() == 1 << 15 and P.getSlots() == 1 << 14
+    assert P.getMaxLevel() == 12 and P.getDnum() == 3
+    assert [int(q).bit_length() for q in P.getModuli()] == [36, 61, 42, 42] + [38] * 7 + [50, 50]
+    assert [int(p).bit_length() for p in P.getPModuli()] == [61] * 4
+    assert P.getScale() == 2.0 ** 35
+    assert P.usesAesWalsh() and P.getAesWalshBlockCount() == 512 and P.getAesWalshPackedSegmentCount() == 32
+    assert P.getBootstrapS2CDepth() == 1 and P.getBootstrapEvalModK() == 11
+
+
+def test_default_evalmod_polynomial_is_the_chebyshev_interpolant():
+    H = host()
+    P = H.CKKSParams.AESWalsh15()
+    K = P.getBootstrapEvalModK()
+    c = np.array(H.default_evalmod_polynomial(P))
+    want = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), 100)
+    want[1::2] = 0.0
+    assert c.shape == (101,)
+    np.testing.assert_allclose(c, want, atol=1e-12)
+    # near every half-integer / K (parity 1) and integer / K (parity 0) the fit is the mod-2 map
+    u = np.concatenate([(np.arange(-K, K + 1) + d) / K for d in (-0.02, 0.0, 0.02)])
+    u = u[np.abs(u) <= 1]
+    np.testing.assert_allclose(np.polynomial.chebyshev.chebval(u, c), 0.5 - 0.5 * np.cos(2 * np.pi * K * u), atol=1e-6)
