@@ -156,6 +156,25 @@ int mi_memcpy(mi_context* ctx, void* dst, const void* src, size_t bytes, int kin
     });
 }
 
+int mi_stream_create(mi_context* ctx, void** out) {
+    return guard("mi_stream_create", [&] {
+        check_ctx(ctx);
+        if (!out) throw mi::Error(-1, "null output");
+        hipStream_t st = nullptr;
+        MI_HIP(hipStreamCreate(&st));  // default priority, synchronises with the null stream
+        *out = st;
+    });
+}
+
+int mi_stream_destroy(mi_context* ctx, void* stream) {
+    return guard("mi_stream_destroy", [&] {
+        check_ctx(ctx);
+        if (!stream) throw mi::Error(-1, "null stream");
+        MI_HIP(hipStreamSynchronize(S(stream)));
+        MI_HIP(hipStreamDestroy(S(stream)));
+    });
+}
+
 int mi_stream_synchronize(mi_context* ctx, void* stream) {
     return guard("mi_stream_synchronize", [&] {
         check_ctx(ctx);

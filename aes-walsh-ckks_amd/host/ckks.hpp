@@ -384,7 +384,10 @@ public:
 
     const CKKSParams& getParams() const { return params; }
     mi_context* handle() const { return mi; }
-    void* stream() const { return nullptr; }
+    // The context's own stream: every evaluation of this context is ordered
+    // on it; it synchronises with the null stream, not with other contexts'
+    // streams, so contexts on separate host threads overlap on the device.
+    void* stream() const { return stream_; }
     const MyVector<Complex>& getTwiddleFft() const { return twiddle_fft; }
     const MyVector<Complex>& getTwiddleIfft() const { return twiddle_ifft; }
     const MyVector<size_t>& getPermTable() const { return perm_table; }
@@ -455,6 +458,7 @@ private:
 
     CKKSParams params;
     mi_context* mi = nullptr;
+    void* stream_ = nullptr;
     MyVector<Complex> twiddle_fft, twiddle_ifft;
     MyVector<size_t> perm_table;
     MyVector<int8_t> secret_coeffs;
@@ -613,8 +617,13 @@ MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_compl
     const CKKSContext& context, const std::array<uint8_t, 176>& expanded_round_keys,
     size_t active_complex_block_count);
 
+// ckks_aes_ctr_walsh_sbox.hpp:12-14.  parallelize_sbox_pairs: the 16
+// S-boxes of a round run concurrently, one host thread per context, over the
+// main context and `sbox_contexts` (contexts built from the same parameters
+// and seed, hence the same keys, each with its own stream and memory pool).
 struct CKKSAesCtrWalshSboxOptions {
     bool parallelize_sbox_pairs = true;
+    std::vector<CKKSContext*> sbox_contexts;
 };
 
 // ckks_aes_ctr_walsh_sbox.hpp:23-35 / .cpp:936-977, 1037-1051: AES-128 CTR

@@ -11,6 +11,8 @@
 // operations as the reference, so each limb equals the reference's.
 #include <algorithm>
 #include <cmath>
+#include <exception>
+#include <thread>
 
 #include "ckks.hpp"
 
@@ -277,17 +279,89 @@ void add_round_key(const CKKSContext& context, MyVector<CKKSCiphertext>& state, 
     for (size_t bit = 0; bit < kBlockBits; ++bit) add_term(context, state[bit], keys[round * kBlockBits + bit].clone());
 }
 
-// sub_bytes_walsh_complex_blocks_mask_packed (:659-721): the 16 S-boxes
-MyVector<CKKSCiphertext> sub_bytes(CKKSContext& context, const MyVector<CKKSCiphertext>& state, size_t pack) {
+// A copy of `src` owned by `dst` (its memory and stream).  The copy is
+// ordered on dst's stream only: src's stream must have finished writing it.
+CKKSCiphertext copy_to_context(const CKKSContext& dst, const CKKSCiphertext& src) {
+    if (&src.getContext() == &dst) return src.clone();
+    CKKSCiphertext out(dst, src.getNumPolys(), src.getScale(), src.getLevel(), src.getSecretOwner(),
+                       src.getMessageEncodingState());
+    const size_t bytes = (src.getLevel() + 1) * src.getN() * sizeof(uint64_t);
+    for (size_t k = 0; k < src.getNumPolys(); ++k)
+        mi_check(mi_memcpy(dst.handle(), out.polyData(k), src.polyData(k), bytes, MI_COPY_D2D, dst.stream()),
+                 "ckks_aes_walsh");
+    return out;
+}
+
+// The worker contexts must hold the same keys as the main one: the same
+// parameters and seed (hence the same secret and switching keys) and the same
+// EvalMod polynomial, so that every S-box result is identical whichever
+// context evaluates it.
+void check_workers(const CKKSContext& context, const std::vector<CKKSContext*>& workers) {
+    const auto& P = context.getParams();
+    for (const CKKSContext* w : workers) {
+        if (!w) throw std::invalid_argument("ckks_aes_walsh: null S-box context");
+        const auto& Q = w->getParams();
+        if (Q.getN() != P.getN() || Q.getSeed() != P.getSeed() || Q.getModuli() != P.getModuli() ||
+            Q.getPModuli() != P.getPModuli() || Q.getDnum() != P.getDnum() ||
+            Q.getSecretKeyHammingWeight() != P.getSecretKeyHammingWeight() ||
+            Q.getAesWalshBlockCount() != P.getAesWalshBlockCount() ||
+            Q.getAesWalshPackedSegmentCount() != P.getAesWalshPackedSegmentCount() ||
+            w->getEvalModPolynomial() != context.getEvalModPolynomial())
+            throw std::invalid_argument("ckks_aes_walsh: S-box contexts must share the main context's parameters and keys");
+    }
+}
+
+// sub_bytes_walsh_complex_blocks_mask_packed (:659-721): the 16 S-boxes.
+// The reference runs them in parallel over OpenMP threads; here each host
+// thread drives one context (its own stream and memory pool), byte b going
+// to context b mod the number of contexts, so the S-boxes' small kernels
+// overlap on the device.  Without extra contexts they run one after another
+// on the main stream.
+MyVector<CKKSCiphertext> sub_bytes(CKKSContext& context, const MyVector<CKKSCiphertext>& state, size_t pack,
+                                   const std::vector<CKKSContext*>& extra) {
+    std::vector<CKKSContext*> ctxs{&context};
+    ctxs.insert(ctxs.end(), extra.begin(), extra.end());
+    std::array<MyVector<CKKSCiphertext>, kBlockBytes> bytes;
+    auto run = [&](size_t w) {
+        CKKSContext& wc = *ctxs[w];
+        for (size_t byte = w; byte < kBlockBytes; byte += ctxs.size()) {
+            MyVector<CKKSCiphertext> input;
+            input.reserve(8);
+            for (size_t k = 0; k < 8; ++k) input.push_back(copy_to_context(wc, state[8 * byte + k]));
+            bytes[byte] = ckks_walsh_sbox_complex_blocks_mask_packed(wc, std::move(input), pack);
+            if (bytes[byte].size() != 8)
+                throw std::runtime_error("ckks_aes_walsh: complex-block Walsh S-box returned wrong size");
+        }
+        mi_check(mi_stream_synchronize(wc.handle(), wc.stream()), "ckks_aes_walsh");
+    };
+    if (ctxs.size() == 1) {
+        run(0);
+    } else {
+        // the state's producers on the main stream finish before other streams read it
+        mi_check(mi_stream_synchronize(context.handle(), context.stream()), "ckks_aes_walsh");
+        std::vector<std::exception_ptr> errors(ctxs.size());
+        std::vector<std::thread> threads;
+        for (size_t w = 0; w < ctxs.size(); ++w)
+            threads.emplace_back([&, w] {
+                try {
+                    run(w);
+                } catch (...) {
+                    errors[w] = std::current_exception();
+                }
+            });
+        for (auto& t : threads) t.join();
+        for (auto& e : errors)
+            if (e) std::rethrow_exception(e);
+    }
     MyVector<CKKSCiphertext> out;
     out.reserve(kBlockBits);
-    for (size_t byte = 0; byte < kBlockBytes; ++byte) {
-        MyVector<CKKSCiphertext> input;
-        input.reserve(8);
-        for (size_t k = 0; k < 8; ++k) input.push_back(state[8 * byte + k].clone());
-        auto bits = ckks_walsh_sbox_complex_blocks_mask_packed(context, std::move(input), pack);
-        if (bits.size() != 8) throw std::runtime_error("ckks_aes_walsh: complex-block Walsh S-box returned wrong size");
-        for (auto& b : bits) out.push_back(std::move(b));
+    for (auto& byte : bytes)
+        for (auto& bit : byte)
+            out.push_back(&bit.getContext() == &context ? std::move(bit) : copy_to_context(context, bit));
+    if (ctxs.size() > 1) {
+        // the copies finish before the workers' buffers return to their pools
+        mi_check(mi_stream_synchronize(context.handle(), context.stream()), "ckks_aes_walsh");
+        for (auto& byte : bytes) byte.clear();
     }
     return out;
 }
@@ -317,7 +391,10 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
                                                                 const MyVector<CKKSCiphertext>& keys,
                                                                 std::span<const Block> counters,
                                                                 std::span<const Block> ciphertexts,
-                                                                const CKKSAesCtrWalshSboxOptions&) {
+                                                                const CKKSAesCtrWalshSboxOptions& options) {
+    const std::vector<CKKSContext*> no_workers;
+    const auto& workers = options.parallelize_sbox_pairs ? options.sbox_contexts : no_workers;
+    check_workers(context, workers);
     // validate_complex_block_inputs (:905-934)
     if (keys.size() != kExpandedKeyBits)
         throw std::invalid_argument("ckks_aes_walsh: expanded round key must have 1408 bits");
@@ -344,7 +421,7 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
     // evaluate_aes128_rounds_walsh_complex_blocks_impl (:819-903)
     for (size_t round = 1; round <= kRounds; ++round) {
         ckks_aes_mask_inactive_slots_inplace(context, state, pack);
-        state = sub_bytes(context, state, pack);
+        state = sub_bytes(context, state, pack, workers);
         shift_rows(state);
         if (round != kRounds) mix_columns(context, state);
         add_round_key(context, state, keys, round);

@@ -479,6 +479,7 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
     mi_check(mi_context_create(&mi, N, q.data(), (int)nq, pm.data(), (int)pm.size(), (int)dnum),
              "ckks_context");
     try {
+        mi_check(mi_stream_create(mi, &stream_), "ckks_context");
         {
             static_assert(sizeof(Complex) == 2 * sizeof(double), "complex<double> layout");
             std::vector<uint64_t> perm(perm_table.begin(), perm_table.end());
@@ -523,6 +524,7 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
         }
     } catch (...) {
         release_all();
+        if (stream_) mi_stream_destroy(mi, stream_);
         mi_context_destroy(mi);
         throw;
     }
@@ -792,6 +794,7 @@ const std::vector<double>& CKKSContext::getEvalModPolynomial() const {
 
 CKKSContext::~CKKSContext() {
     release_all();
+    if (mi && stream_) mi_stream_destroy(mi, stream_);
     if (mi) mi_context_destroy(mi);
 }
 

@@ -525,10 +525,18 @@ PYBIND11_MODULE(ckks_py, m) {
     m.def("ckks_aes_ctr_walsh_sbox_complex_blocks",
           [ct_pylist](const py::object& context, const AesRoundKeyBits& keys,
                       const py::array_t<uint8_t, py::array::c_style>& counters,
-                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts) {
+                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts, const py::list& sbox_contexts) {
               CKKSContext& c = context.cast<CKKSContext&>();
               const auto ctr = blocks_from_array(counters), cts = blocks_from_array(ciphertexts);
-              auto out = ckks_aes_ctr_walsh_sbox_complex_blocks(c, keys.bits, ctr, cts);
+              CKKSAesCtrWalshSboxOptions options;
+              for (const auto& w : sbox_contexts) options.sbox_contexts.push_back(&w.cast<CKKSContext&>());
+              MyVector<CKKSCiphertext> out;
+              {
+                  py::gil_scoped_release unlocked;
+                  out = ckks_aes_ctr_walsh_sbox_complex_blocks(c, keys.bits, ctr, cts, options);
+              }
               return ct_pylist(out, context);
-          });
+          },
+          py::arg("context"), py::arg("keys"), py::arg("counter_blocks"), py::arg("ciphertext_blocks"),
+          py::arg("sbox_contexts") = py::list());
 }

@@ -719,7 +719,8 @@ def bench_aes(args, M, torch):
     encryption are outside the timed region, as in the reference."""
     H = _host()
     t0 = time.perf_counter()
-    ctx = H.CKKSContext(H.CKKSParams.AESWalsh15())
+    ctxs = [H.CKKSContext(H.CKKSParams.AESWalsh15()) for _ in range(max(1, args.aes_contexts))]
+    ctx, workers = ctxs[0], ctxs[1:]
     cb = ctx.getParams().getAesWalshBlockCount()
     counters, plains, cts = aes_bench_blocks(H, 2 * cb)
     keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(list(AES_KEY)), cb)
@@ -727,17 +728,28 @@ def bench_aes(args, M, torch):
     setup_s = time.perf_counter() - t0
     print(f"# walsh15 context and round keys: {setup_s:.1f} s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=workers)
     torch.cuda.synchronize()
     lat = []
     for step in range(args.steps):
         t1 = time.perf_counter()
-        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=workers)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
         print(f"# step {step}: {lat[-1]:.3f} s", file=sys.stderr, flush=True)
     ms = 1e3 * sum(lat) / len(lat)
     mism, max_err, avg_err = aes_validate(H, ctx, out, plains, cb)
+    plain_s = None
+    if workers and args.plain:
+        # the plain form: the same evaluation with the S-boxes one after another
+        t1 = time.perf_counter()
+        plain = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t1
+        same = all(np.array_equal(np.asarray(plain[b].getPoly(k)), np.asarray(out[b].getPoly(k)))
+                   for b in range(128) for k in range(2))
+        if not same:
+            raise SystemExit("concurrent and sequential S-box evaluations differ")
     return {
         "metric": "walsh15 AES-CTR blocks/s (homomorphic CTR decryption, mask-packed Walsh S-box)",
         "value": 2 * cb * 1e3 / ms, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -746,6 +758,9 @@ def bench_aes(args, M, torch):
         "config": {"workload": "walsh15 AES-CTR, 1024 blocks per evaluation", "blocks": 2 * cb,
                    "mismatches": mism, "max_error": max_err, "average_error": avg_err,
                    "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                   "sbox_contexts": len(ctxs),
+                   "plain_blocks_per_s": None if plain_s is None else 2 * cb / plain_s,
+                   "plain_bit_exact": None if plain_s is None else True,
                    "context_and_key_setup_s": setup_s},
         "roofline": None, "cpu_baseline": None,
     }
@@ -848,6 +863,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--aes-contexts", type=int, default=4,
+                    help="aes: contexts (host threads and streams) the S-boxes of a round are spread over")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
     ap.add_argument("--eager", action="store_true",

@@ -74,6 +74,12 @@ enum { MI_COPY_H2D = 0, MI_COPY_D2H = 1, MI_COPY_D2D = 2 };
  * stream-ordered. */
 int mi_memcpy(mi_context* ctx, void* dst, const void* src, size_t bytes, int kind, void* stream);
 int mi_stream_synchronize(mi_context* ctx, void* stream);
+/* A stream of default priority that synchronises with the null stream (no
+ * counterpart in the reference, whose evaluator runs on host threads): each
+ * C++ context keeps one, so that contexts driven from separate host threads
+ * run concurrently.  Destroying it waits for its work first. */
+int mi_stream_create(mi_context* ctx, void** out);
+int mi_stream_destroy(mi_context* ctx, void* stream);
 
 /* ---- element-wise over a basis range (limb l uses basis index basis0 + l) ---
  * pointwise_multiply*_dispatch and poly_add/sub_modq (ntt.cpp:450-579,

@@ -80,3 +80,36 @@ def test_aes_ctr_rejects_bad_block_spans():
         H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, c[:4], x[:2])
     with pytest.raises(Exception, match="packing capacity"):
         H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), 513)
+
+
+def test_aes_ctr_concurrent_sboxes_are_bit_exact():
+    """The 16 S-boxes of every round spread over the main context and two
+    more built from the same parameters and seed (one host thread and stream
+    each): every limb of the 128 output bits equals the sequential run's."""
+    H = host()
+    ctxs = [H.CKKSContext(H.CKKSParams.AESWalsh15()) for _ in range(3)]
+    ctx = ctxs[0]
+    cb = ctx.getParams().getAesWalshBlockCount()
+    counters, plains, cts = make_blocks(H, 2 * cb)
+    keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), cb)
+    seq = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+    par = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=ctxs[1:])
+    for bit in range(128):
+        assert par[bit].getLevel() == seq[bit].getLevel() and par[bit].getScale() == seq[bit].getScale()
+        for k in range(2):
+            assert np.array_equal(np.asarray(par[bit].getPoly(k)), np.asarray(seq[bit].getPoly(k))), (bit, k)
+    mismatches, max_err, _ = validate(H, ctx, par, plains, cb)
+    print(f"concurrent S-boxes: mismatches {mismatches} max error {max_err:.3e}", flush=True)
+    assert mismatches == 0
+
+
+def test_aes_ctr_rejects_a_context_with_other_keys():
+    H = host()
+    ctx = H.CKKSContext(H.CKKSParams.AESWalsh15())
+    c = H.aes_walsh15_config()
+    c.seed = 12345
+    other = H.CKKSContext(H.CKKSParams(c))
+    keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), 512)
+    counters, _, cts = make_blocks(H, 1024)
+    with pytest.raises(Exception, match="share the main context"):
+        H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=[other])
