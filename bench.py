@@ -1,17 +1,21 @@
 """Benchmark of the CKKS evaluation hot path on one MI355X.
 
-Default workload (the batched NTT configuration): N = 2^16, 40 RNS limbs of
-60-bit NTT primes (the reference prime search), 64 ciphertexts x 2 polys =
-5120 limbs, residues uniform in [0, q_i) from a seed.  One step = forward NTT
-of the whole batch in one launch pair followed by the inverse NTT.  The
-plain form (one limb per launch) is timed beside it.
+Default workload: walsh15 AES-CTR (the reference's AESWalsh15 parameters,
+N = 2^15), 1024 blocks per evaluation (512 complex slots x 2 lanes) on the
+reference bench's counters and plaintexts; one step = one homomorphic CTR
+decryption of all 1024 blocks (10 AES rounds of 16 mask-packed Walsh
+S-boxes, each with its bootstrap, then the final binary refresh), with the
+S-boxes of a round spread over 16 contexts of the same keys (one host thread
+and stream each).  The plain form (the S-boxes one after another on one
+stream) is timed beside it and checked bit for bit; the outputs are
+decrypted and compared with the plaintexts (0 mismatched blocks).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ntt|keyswitch|lt|chain|hoisted|walsh|sbox|aes]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload aes|ntt|keyswitch|lt|chain|hoisted|walsh|sbox]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
 stream the library launches on.  The CPU baseline times the oracle (a C
-restatement of the reference's NTT, OpenMP over limbs) on a bounded sample.
+restatement of the reference's arithmetic, OpenMP) on a bounded sample.
 """
 import argparse
 import importlib.util
@@ -739,6 +743,7 @@ def bench_aes(args, M, torch):
         print(f"# step {step}: {lat[-1]:.3f} s", file=sys.stderr, flush=True)
     ms = 1e3 * sum(lat) / len(lat)
     mism, max_err, avg_err = aes_validate(H, ctx, out, plains, cb)
+    roofline = aes_ntt_roofline(M, torch, ctx.getParams())
     plain_s = None
     if workers and args.plain:
         # the plain form: the same evaluation with the S-boxes one after another
@@ -762,8 +767,60 @@ def bench_aes(args, M, torch):
                    "plain_blocks_per_s": None if plain_s is None else 2 * cb / plain_s,
                    "plain_bit_exact": None if plain_s is None else True,
                    "context_and_key_setup_s": setup_s},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline_aes(H, ctx.getParams(), 2 * cb) if args.cpu_baseline else None,
     }
+
+
+def aes_ntt_roofline(M, torch, P):
+    """The NTT passes are the largest kernel family of the AES evaluation
+    (about 40 % of its kernel time; the rest is spread over key-switch inner
+    products, element-wise ops, encoding FFT stages and copies).  Measured
+    here with HIP events on their stream, right after the timed region: the
+    forward NTT (column + row pass) of one ciphertext over QP at the top
+    level, 2 polys x 17 limbs at N=2^15, the shape of the key switches'
+    ModUp / ModDown transforms; 16 N bytes per limb algorithmically."""
+    N = P.getN()
+    q, p = [int(x) for x in P.getModuli()], [int(x) for x in P.getPModuli()]
+    ctx = M.Context(N, q, p, P.getDnum())
+    limbs = len(q) + len(p)
+    data = _rand_limbs(torch, q + p, N, 2, seed=77)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        ctx.ntt_forward(data, 2, limbs)
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.ntt_forward(data, 2, limbs)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    algo = 16 * N * 2 * limbs
+    achieved = algo / (us / 1e6)
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": _profile_traffic("ntt15_qp_forward"),
+            "kernel": "forward NTT (column + row pass), N=2^15, 2 polys x 17 limbs (QP ciphertext)",
+            "launch_pair_us": us}
+
+
+def cpu_baseline_aes(H, P, blocks):
+    """The oracle's product sum of one Walsh output pair at the walsh15 shape
+    (signs at level 2).  One AES evaluation of `blocks` blocks holds 640 such
+    pairs (10 rounds x 16 S-boxes x 4) besides its 164 bootstraps, which the
+    sample leaves out, so blocks x rate / 640 bounds the CPU port's blocks/s
+    from above."""
+    N = P.getN()
+    q, p = [int(x) for x in P.getModuli()], [int(x) for x in P.getPModuli()]
+    table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
+    cr, ci = table[0].T.copy(), table[4].T.copy()
+    scale = 2.0 ** 38
+    consts = [(int(round(float(cr[h][0]) * scale)), int(round(float(ci[h][0]) * scale))) for h in range(16)]
+    pair = cpu_baseline_walsh(N, q, p, 2, (cr, ci, consts))
+    return {"value": pair["value"] * blocks / 640, "unit": "blocks/s (upper bound)", "cores": pair["cores"],
+            "kind": "port",
+            "sample": f"oracle product sum of one Walsh output pair at the walsh15 shape ({1 / pair['value']:.2f} s); "
+                      "scaled by the 640 pairs of one 1024-block evaluation, bootstraps left out"}
 
 
 def bench_chain(args, M, torch):
@@ -861,9 +918,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ntt", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes"])
+    ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    ap.add_argument("--aes-contexts", type=int, default=4,
+    ap.add_argument("--aes-contexts", type=int, default=16,
                     help="aes: contexts (host threads and streams) the S-boxes of a round are spread over")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
