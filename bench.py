@@ -10,7 +10,7 @@ and stream each).  The plain form (the S-boxes one after another on one
 stream) is timed beside it and checked bit for bit; the outputs are
 decrypted and compared with the plaintexts (0 mismatched blocks).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload aes|ntt|keyswitch|lt|chain|hoisted|walsh|sbox]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload aes|serve|ntt|keyswitch|lt|chain|hoisted|walsh|sbox]
 
 Prints one JSON line.  Timing: W untimed steps, then K timed steps bracketed
 by torch.cuda.synchronize(); per-launch durations come from HIP events on the
@@ -772,6 +772,79 @@ def bench_aes(args, M, torch):
     }
 
 
+def bench_serve(args, M, torch):
+    """Served walsh15 AES-CTR (aes-walsh-ckks_amd/aes_serve.py): Poisson
+    arrivals at --rate requests/s from a seed, each of 1-32 blocks (random
+    plaintexts, AES-CTR under the test key), served by continuous batching
+    over 2 lanes of 8 contexts; blocks/s over the whole run (first arrival to
+    last result) and p50/p99 of arrival -> decrypted result.  The plain form
+    evaluates one request per evaluation on one lane."""
+    sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
+    import aes_serve
+    H = _host()
+    rng = np.random.default_rng(55)
+    base = int.from_bytes(bytes.fromhex("f0f1f2f3f4f5f6f7f8f9fafbfcfdfeff"), "big")
+    reqs, first = [], 0
+    for _ in range(args.requests):
+        n = int(rng.integers(1, 33))
+        c = np.array([list((base + first + i).to_bytes(16, "big")) for i in range(n)], np.uint8)
+        p = rng.integers(0, 256, (n, 16), dtype=np.uint8)
+        x = np.array([H.aes128_ctr_crypt_block(list(AES_KEY), list(a), list(b)) for a, b in zip(c, p)], np.uint8)
+        reqs.append((c, x, p))
+        first += n
+    gaps = rng.exponential(1.0 / args.rate, args.requests)
+    t0 = time.perf_counter()
+    srv = aes_serve.AesCtrServer(AES_KEY, lanes=2, contexts_per_lane=8)
+    setup_s = time.perf_counter() - t0
+    print(f"# server: {len(srv.lanes)} lanes, footprints {[f // 2**20 for f in srv.footprints]} MiB, "
+          f"set-up {setup_s:.1f} s", file=sys.stderr, flush=True)
+    live = []
+    start = time.perf_counter()
+    due = start
+    for (c, x, _), g in zip(reqs, gaps):
+        due += g
+        while time.perf_counter() < due:
+            time.sleep(min(0.005, max(0.0, due - time.perf_counter())))
+        live.append(srv.submit(aes_serve.Request(c, x)))
+    for r in live:
+        r.done.wait()
+    end = max(r.finish for r in live)
+    evaluations = srv.evaluations
+    srv.close()
+    errors = [r.error for r in live if r.error is not None]
+    if errors:
+        raise errors[0]
+    mism = sum(int(np.sum(np.any(r.plaintexts != p, axis=1))) for r, (_, _, p) in zip(live, reqs))
+    blocks = sum(r.blocks for r in live)
+    lat = np.array([r.latency for r in live])
+    plain = None
+    if args.plain:
+        one = aes_serve.AesCtrServer(AES_KEY, lanes=1, contexts_per_lane=8)
+        t1 = time.perf_counter()
+        nb = 0
+        for c, x, p in reqs[:3]:
+            r = one.submit(aes_serve.Request(c, x))
+            r.done.wait()
+            nb += r.blocks
+            if r.error is not None or not np.array_equal(r.plaintexts, p):
+                raise RuntimeError("plain serving decrypted a wrong block")
+        plain = nb / (time.perf_counter() - t1)
+        one.close()
+    return {
+        "metric": "served walsh15 AES-CTR blocks/s with p50/p99 latency (continuous batching)",
+        "value": blocks / (end - start), "unit": "blocks/s", "n_gpus": 1, "steps": args.requests, "warmup": 0,
+        "ms_per_step": 1e3 * (end - start) / args.requests, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 (exact mod-q integer)",
+        "data": "synthetic (Poisson arrivals, random plaintexts, seed 55)",
+        "config": {"workload": "served walsh15 AES-CTR, requests of 1-32 blocks", "requests": args.requests,
+                   "rate_per_s": args.rate, "blocks": blocks, "mismatched_blocks": mism,
+                   "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                   "evaluations": evaluations, "lanes": 2, "contexts_per_lane": 8,
+                   "plain_one_request_per_evaluation_blocks_per_s": plain, "server_setup_s": setup_s},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def aes_ntt_roofline(M, torch, P):
     """The NTT passes are the largest kernel family of the AES evaluation
     (about 40 % of its kernel time; the rest is spread over key-switch inner
@@ -918,8 +991,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes"])
+    ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes", "serve"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--requests", type=int, default=240, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--aes-contexts", type=int, default=16,
                     help="aes: contexts (host threads and streams) the S-boxes of a round are spread over")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
@@ -938,7 +1013,7 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
-           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox, "aes": bench_aes}[args.workload](args, M, torch)
+           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox, "aes": bench_aes, "serve": bench_serve}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 
