@@ -474,6 +474,32 @@ private:
     mutable std::mutex scaled_mu;
     mutable std::map<uint64_t, std::unique_ptr<CKKSLinearTransformPlan>> scaled_s2c;
     std::vector<double> evalmod_coeffs;
+
+public:
+    // An encoding that depends only on `key` (what is encoded, its scale and
+    // level), made by `make` on first use and kept for the context's
+    // lifetime (at most 256 entries; past that each call makes its own).
+    // Used for the constant-slot and slot-mask plaintexts the evaluator
+    // multiplies by, which repeat in every S-box and bootstrap.
+    template <class Make>
+    const CKKSEncoding* cachedEncoding(const std::array<uint64_t, 4>& key, Make make,
+                                       std::optional<CKKSEncoding>& fallback) const {
+        std::lock_guard<std::mutex> g(enc_mu);
+        auto it = enc_cache.find(key);
+        if (it != enc_cache.end()) return it->second.get();
+        if (enc_cache.size() >= 256) {
+            fallback.emplace(make());
+            return &*fallback;
+        }
+        auto e = std::make_unique<CKKSEncoding>(make());
+        const CKKSEncoding* ptr = e.get();
+        enc_cache.emplace(key, std::move(e));
+        return ptr;
+    }
+
+private:
+    mutable std::mutex enc_mu;
+    mutable std::map<std::array<uint64_t, 4>, std::unique_ptr<CKKSEncoding>> enc_cache;
 };
 
 // ------------------------------------------------------------ encode / crypt

@@ -117,9 +117,24 @@ void add_term(const CKKSContext& context, CKKSCiphertext& acc, CKKSCiphertext te
     ckks_add_inplace(context, acc, term);
 }
 
+// add_term on an operand the caller keeps: when it already has acc's level,
+// degree and (within 2^-45) scale, the alignment changes nothing but acc's
+// scale label, so it is added in place of a copy.
+void add_term_ref(const CKKSContext& context, CKKSCiphertext& acc, const CKKSCiphertext& term) {
+    const long double sa = acc.getScale(), st = term.getScale();
+    const long double m = std::max<long double>({1.0L, std::abs(sa), std::abs(st)});
+    if (term.getLevel() == acc.getLevel() && term.getNumPolys() == acc.getNumPolys() && std::isfinite(sa) &&
+        std::isfinite(st) && std::abs(sa - st) <= 0x1p-45L * m) {
+        acc.setScale(std::max(sa, st));
+        ckks_add_inplace(context, acc, term);
+        return;
+    }
+    add_term(context, acc, term.clone());
+}
+
 CKKSCiphertext add_bits(const CKKSContext& context, const CKKSCiphertext& lhs, const CKKSCiphertext& rhs) {
     auto out = lhs.clone();
-    add_term(context, out, rhs.clone());
+    add_term_ref(context, out, rhs);
     return out;
 }
 
@@ -254,7 +269,7 @@ CKKSByte mix_column_byte(const CKKSContext& context, const MyVector<CKKSCipherte
     out.push_back(sum[5].clone());
     out.push_back(sum[6].clone());
     for (size_t idx : {i1, i2, i3})
-        for (size_t k = 0; k < 8; ++k) add_term(context, out[k], st[8 * idx + k].clone());
+        for (size_t k = 0; k < 8; ++k) add_term_ref(context, out[k], st[8 * idx + k]);
     return out;
 }
 
@@ -276,7 +291,7 @@ void mix_columns(const CKKSContext& context, MyVector<CKKSCiphertext>& state) {
 // add_round_key_lazy_inplace (:785-817)
 void add_round_key(const CKKSContext& context, MyVector<CKKSCiphertext>& state, const MyVector<CKKSCiphertext>& keys,
                    size_t round) {
-    for (size_t bit = 0; bit < kBlockBits; ++bit) add_term(context, state[bit], keys[round * kBlockBits + bit].clone());
+    for (size_t bit = 0; bit < kBlockBits; ++bit) add_term_ref(context, state[bit], keys[round * kBlockBits + bit]);
 }
 
 // A copy of `src` owned by `dst` (its memory and stream).  The copy is

@@ -531,6 +531,7 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
 }
 
 void CKKSContext::release_all() {
+    enc_cache.clear();
     scaled_s2c.clear();
     s2c_plan.reset();
     c2s_plan.reset();

@@ -1,3 +1,4 @@
+#include <cstring>
 // Evaluator free functions over device-resident ciphertexts: metadata checks
 // and scale bookkeeping of the reference (ckks_mul.cpp:27-288,
 // ckks_rescale.cpp:42-148, ckks_addition.cpp:17-139, ckks_relin.cpp:8-44,
@@ -91,9 +92,18 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t scalar) 
 }
 
 void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, Complex value, double plaintext_scale) {
-    MyVector<Complex> slots(c.getParams().getSlots(), value);
-    CKKSEncoding pt = ckks_encode(c, slots, plaintext_scale, ct.getLevel(), /*eval=*/true);
-    ckks_mul_inplace(c, ct, pt);
+    // the same constant encoding every time: made once per (value, scale, level)
+    const double re = value.real(), im = value.imag();
+    uint64_t kr, ki, ks;
+    std::memcpy(&kr, &re, 8);
+    std::memcpy(&ki, &im, 8);
+    std::memcpy(&ks, &plaintext_scale, 8);
+    std::optional<CKKSEncoding> own;
+    const CKKSEncoding* pt = c.cachedEncoding({(uint64_t{1} << 32) | ct.getLevel(), kr, ki, ks}, [&] {
+        MyVector<Complex> slots(c.getParams().getSlots(), value);
+        return ckks_encode(c, slots, plaintext_scale, ct.getLevel(), /*eval=*/true);
+    }, own);
+    ckks_mul_inplace(c, ct, *pt);
 }
 
 CKKSCiphertext ckks_mul(const CKKSContext& c, const CKKSCiphertext& ct, Complex value, double plaintext_scale) {

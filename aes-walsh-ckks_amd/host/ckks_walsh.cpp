@@ -107,9 +107,24 @@ void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double tar
     ct.setScale(target);
 }
 
-CKKSEncoding constant_encoding(const CKKSContext& context, Complex value, long double scale, size_t level) {
-    MyVector<Complex> slots(context.getParams().getSlots(), value);
-    return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
+uint64_t bits_of(double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    return b;
+}
+
+// The all-slots constant `value` at `scale`, from the context's encoding
+// cache (`own` holds it when the cache is full).
+const CKKSEncoding& constant_encoding(const CKKSContext& context, Complex value, long double scale, size_t level,
+                                      std::optional<CKKSEncoding>& own) {
+    const double sc = (double)scale;
+    return *context.cachedEncoding({(uint64_t{1} << 32) | level, bits_of(value.real()), bits_of(value.imag()),
+                                    bits_of(sc)},
+                                   [&] {
+                                       MyVector<Complex> slots(context.getParams().getSlots(), value);
+                                       return ckks_encode(context, slots, sc, level, /*eval=*/true);
+                                   },
+                                   own);
 }
 
 }  // namespace
@@ -240,7 +255,8 @@ CKKSCiphertext ckks_walsh_sum_to_bit_pair(const CKKSContext& context, CKKSCipher
     const long double output_scale = walsh_sum.getScale() * 128.0L;
     walsh_sum.setScale(output_scale);
     ckks_mul_inplace(context, walsh_sum, int64_t{-1});
-    const auto half = constant_encoding(context, Complex(0.5, 0.5), output_scale, walsh_sum.getLevel());
+    std::optional<CKKSEncoding> own;
+    const CKKSEncoding& half = constant_encoding(context, Complex(0.5, 0.5), output_scale, walsh_sum.getLevel(), own);
     ckks_add_inplace(context, walsh_sum, half);
     return walsh_sum;
 }
@@ -254,7 +270,8 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
         throw std::runtime_error("ckks_walsh_sbox: low/high sign metadata mismatch");
     if (low_signs[1].getLevel() == 0)
         throw std::runtime_error("ckks_walsh_sbox: sign output level is too low for 4+4 products");
-    const auto encoded_i = constant_encoding(context, Complex(0.0, 1.0), 1.0L, low_signs[1].getLevel());
+    std::optional<CKKSEncoding> own;
+    const CKKSEncoding& encoded_i = constant_encoding(context, Complex(0.0, 1.0), 1.0L, low_signs[1].getLevel(), own);
     // the four pairs are independent: their product sums first, then their
     // relinearisations as one key-switch batch (one launch per phase)
     std::vector<std::pair<CKKSCiphertext, CKKSCiphertext>> sums;
@@ -422,10 +439,16 @@ namespace {
 
 // encode_active_slot_mask (ckks_aes_ctr_walsh_sbox.cpp:199-215): 1 in the
 // first `count` slots, 0 elsewhere, at `scale`, evaluation domain.
-CKKSEncoding active_slot_mask(const CKKSContext& context, size_t count, size_t level, long double scale) {
-    MyVector<Complex> slots(context.getParams().getSlots(), Complex(0.0, 0.0));
-    for (size_t i = 0; i < count; ++i) slots[i] = Complex(1.0, 0.0);
-    return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
+const CKKSEncoding& active_slot_mask(const CKKSContext& context, size_t count, size_t level, long double scale,
+                                     std::optional<CKKSEncoding>& own) {
+    const double sc = (double)scale;
+    return *context.cachedEncoding({(uint64_t{2} << 32) | level, (uint64_t)count, 0, bits_of(sc)},
+                                   [&] {
+                                       MyVector<Complex> slots(context.getParams().getSlots(), Complex(0.0, 0.0));
+                                       for (size_t i = 0; i < count; ++i) slots[i] = Complex(1.0, 0.0);
+                                       return ckks_encode(context, slots, sc, level, /*eval=*/true);
+                                   },
+                                   own);
 }
 
 // aes_scales_close (ckks_aes_ctr_walsh_sbox.cpp:42-49)
@@ -465,8 +488,9 @@ void ckks_aes_mask_inactive_slots_inplace(const CKKSContext& context, MyVector<C
         throw std::invalid_argument(
             "ckks_aes_walsh: active slot mask requires a selection limb above S2C input level");
     const long double target_scale = state.front().getScale();
-    const auto mask =
-        active_slot_mask(context, block_count, level, (long double)context.getParams().getModuli()[level]);
+    std::optional<CKKSEncoding> own;
+    const CKKSEncoding& mask =
+        active_slot_mask(context, block_count, level, (long double)context.getParams().getModuli()[level], own);
     for (const auto& ct : state)
         if (!aes_scales_close(ct.getScale(), target_scale))
             throw std::invalid_argument("ckks_aes_walsh: active slot mask scale mismatch");
@@ -484,7 +508,8 @@ void ckks_aes_mask_final_refresh_slots_inplace(const CKKSContext& context, MyVec
     const bool consume_selection_limb = level > minimum_level_to_keep;
     const long double target_scale = bits.front().getScale();
     const long double mask_scale = consume_selection_limb ? (long double)params.getModuli()[level] : 1.0L;
-    const auto mask = active_slot_mask(context, active_slot_count, level, mask_scale);
+    std::optional<CKKSEncoding> own;
+    const CKKSEncoding& mask = active_slot_mask(context, active_slot_count, level, mask_scale, own);
     mask_all(context, bits, mask, consume_selection_limb, "ckks_aes_walsh: final refresh output mask level mismatch");
     if (consume_selection_limb)
         for (auto& ct : bits) ct.setScale(target_scale);
