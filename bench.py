@@ -777,8 +777,11 @@ def bench_serve(args, M, torch):
     arrivals at --rate requests/s from a seed, each of 1-32 blocks (random
     plaintexts, AES-CTR under the test key), served by continuous batching
     over 2 lanes of 8 contexts; blocks/s over the whole run (first arrival to
-    last result) and p50/p99 of arrival -> decrypted result.  The plain form
-    evaluates one request per evaluation on one lane."""
+    last result) and p50/p99 of arrival -> decrypted result.  Default: one
+    lane of 16 contexts (two lanes of 8 overlap whole evaluations but serve
+    fewer blocks/s, since the S-boxes of one evaluation already keep the
+    device's queues busy).  The plain form evaluates one request per
+    evaluation on one lane."""
     sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
     import aes_serve
     H = _host()
@@ -794,7 +797,7 @@ def bench_serve(args, M, torch):
         first += n
     gaps = rng.exponential(1.0 / args.rate, args.requests)
     t0 = time.perf_counter()
-    srv = aes_serve.AesCtrServer(AES_KEY, lanes=2, contexts_per_lane=8)
+    srv = aes_serve.AesCtrServer(AES_KEY, lanes=args.lanes, contexts_per_lane=args.lane_contexts)
     setup_s = time.perf_counter() - t0
     print(f"# server: {len(srv.lanes)} lanes, footprints {[f // 2**20 for f in srv.footprints]} MiB, "
           f"set-up {setup_s:.1f} s", file=sys.stderr, flush=True)
@@ -819,7 +822,7 @@ def bench_serve(args, M, torch):
     lat = np.array([r.latency for r in live])
     plain = None
     if args.plain:
-        one = aes_serve.AesCtrServer(AES_KEY, lanes=1, contexts_per_lane=8)
+        one = aes_serve.AesCtrServer(AES_KEY, lanes=1, contexts_per_lane=args.lane_contexts)
         t1 = time.perf_counter()
         nb = 0
         for c, x, p in reqs[:3]:
@@ -839,10 +842,22 @@ def bench_serve(args, M, torch):
         "config": {"workload": "served walsh15 AES-CTR, requests of 1-32 blocks", "requests": args.requests,
                    "rate_per_s": args.rate, "blocks": blocks, "mismatched_blocks": mism,
                    "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-                   "evaluations": evaluations, "lanes": 2, "contexts_per_lane": 8,
+                   "evaluations": evaluations, "lanes": len(srv.lanes), "contexts_per_lane": args.lane_contexts,
                    "plain_one_request_per_evaluation_blocks_per_s": plain, "server_setup_s": setup_s},
         "roofline": None, "cpu_baseline": None,
     }
+
+
+def bench_ntt_qp(args, M, torch):
+    """The AES workload's roofline kernel alone: the forward NTT of one QP
+    ciphertext at walsh15 (2 polys x 17 limbs, N=2^15), for counter passes."""
+    H = _host()
+    r = aes_ntt_roofline(M, torch, H.CKKSParams.AESWalsh15())
+    return {"metric": "forward NTT launch pair, walsh15 QP ciphertext (2 polys x 17 limbs, N=2^15)",
+            "value": 1e6 / r["launch_pair_us"], "unit": "launch-pairs/s", "n_gpus": 1, "steps": 50, "warmup": 3,
+            "ms_per_step": r["launch_pair_us"] / 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
+            "config": {"workload": "forward NTT, N=2^15, 34 limbs"}, "roofline": r, "cpu_baseline": None}
 
 
 def aes_ntt_roofline(M, torch, P):
@@ -991,9 +1006,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes", "serve"])
+    ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes", "serve", "ntt_qp"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--requests", type=int, default=240, help="serve: number of requests")
+    ap.add_argument("--lanes", type=int, default=1, help="serve: evaluation lanes")
+    ap.add_argument("--lane-contexts", type=int, default=16, help="serve: contexts per lane")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--aes-contexts", type=int, default=16,
                     help="aes: contexts (host threads and streams) the S-boxes of a round are spread over")
@@ -1013,7 +1030,7 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     out = {"ntt": bench_ntt, "keyswitch": bench_keyswitch, "lt": bench_lt, "chain": bench_chain,
-           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox, "aes": bench_aes, "serve": bench_serve}[args.workload](args, M, torch)
+           "hoisted": bench_hoisted, "walsh": bench_walsh, "sbox": bench_sbox, "aes": bench_aes, "serve": bench_serve, "ntt_qp": bench_ntt_qp}[args.workload](args, M, torch)
     print(json.dumps(out))
 
 
