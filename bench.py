@@ -776,7 +776,7 @@ def bench_serve(args, M, torch):
     """Served walsh15 AES-CTR (aes-walsh-ckks_amd/aes_serve.py): Poisson
     arrivals at --rate requests/s from a seed, each of 1-32 blocks (random
     plaintexts, AES-CTR under the test key), served by continuous batching
-    over 2 lanes of 8 contexts; blocks/s over the whole run (first arrival to
+    over --lanes lanes of --lane-contexts contexts; blocks/s over the whole run (first arrival to
     last result) and p50/p99 of arrival -> decrypted result.  Default: one
     lane of 16 contexts (two lanes of 8 overlap whole evaluations but serve
     fewer blocks/s, since the S-boxes of one evaluation already keep the
