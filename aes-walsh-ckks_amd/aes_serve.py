@@ -92,8 +92,8 @@ class Lane:
                                                             sbox_contexts=self.ctxs[1:])
         cb = self.complex_blocks
         vals = np.empty((2 * cb, 128))
-        for bit in range(128):
-            v = np.asarray(self.H.ckks_decrypt(self.ctx, out[bit]))[:cb]
+        for bit, v in enumerate(self.H.ckks_decrypt_batch(self.ctx, out)):
+            v = np.asarray(v)[:cb]
             vals[:cb, bit] = v.real
             vals[cb:, bit] = v.imag
         return vals
@@ -111,7 +111,7 @@ class AesCtrServer:
     contexts each.  submit() queues a request; completed requests carry their
     plaintexts, decrypted values and finish time."""
 
-    def __init__(self, aes_key, lanes=2, contexts_per_lane=8, memory_budget=None, params_fn=None):
+    def __init__(self, aes_key, lanes=1, contexts_per_lane=16, memory_budget=None, params_fn=None):
         self.H = _host()
         params_fn = params_fn or self.H.CKKSParams.AESWalsh15
         self.lanes = []

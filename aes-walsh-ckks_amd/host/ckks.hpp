@@ -523,6 +523,13 @@ CKKSCiphertext ckks_encrypt(const CKKSContext& context, const CKKSEncoding& encd
 MyVector<Complex> ckks_decrypt(const CKKSContext& context, const CKKSCiphertext& ct,
                                CKKSDecryptDomain domain = CKKSDecryptDomain::Slots,
                                bool use_low_secret = false);
+// ckks_decrypt of many ciphertexts: the device parts in order on the
+// context's stream, the host decodes (CRT and FFT) on up to 16 threads; each
+// result equals ckks_decrypt's.  No counterpart in the reference (its bench
+// decrypts one ciphertext at a time, aes_bench.cpp:156-196).
+std::vector<MyVector<Complex>> ckks_decrypt_batch(const CKKSContext& context,
+                                                  const std::vector<const CKKSCiphertext*>& cts,
+                                                  CKKSDecryptDomain domain = CKKSDecryptDomain::Slots);
 // Decode of a coefficient-domain integer plaintext (first limbs whose product
 // fits 128 bits, centred) at `scale` (ckks_encoding.cpp:75-170, 240-329).
 MyVector<Complex> ckks_decode_plaintext(const CKKSContext& context, const MyVector<uint64_t>& coeffs,

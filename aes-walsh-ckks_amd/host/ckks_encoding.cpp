@@ -1,3 +1,6 @@
+#include <thread>
+#include <exception>
+#include <algorithm>
 // Encode / encrypt / decrypt (ckks_encoding.cpp:75-580, fft.cpp:82-117).
 //
 // Slot encoding runs on the device (mi_encode_slots: the special FFT in double
@@ -256,8 +259,46 @@ MyVector<Complex> decode_plaintext_custom_basis(size_t N, const MyVector<uint64_
 
 // ckks_decrypt_with_secret_eval (ckks_encoding.cpp:96-170): m = b + a*s (+ c2*s^2)
 // on the device, INTT, then the host decode.
+namespace {
+// The device part of ckks_decrypt: b + a s (+ c2 s^2), inverse NTT, copied to the host.
+MyVector<uint64_t> decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret);
+}  // namespace
+
 MyVector<Complex> ckks_decrypt(const CKKSContext& c, const CKKSCiphertext& ct, CKKSDecryptDomain domain,
                                bool use_low_secret) {
+    auto h = decrypt_coefficients(c, ct, use_low_secret);
+    return ckks_decode_plaintext(c, h, ct.getLevel() + 1, ct.getScale(), domain);
+}
+
+std::vector<MyVector<Complex>> ckks_decrypt_batch(const CKKSContext& c, const std::vector<const CKKSCiphertext*>& cts,
+                                                  CKKSDecryptDomain domain) {
+    std::vector<MyVector<uint64_t>> coeffs;
+    coeffs.reserve(cts.size());
+    for (const CKKSCiphertext* ct : cts) {
+        if (!ct) throw std::invalid_argument("ckks_decrypt: null ciphertext");
+        coeffs.push_back(decrypt_coefficients(c, *ct, false));
+    }
+    std::vector<MyVector<Complex>> out(cts.size());
+    const size_t workers = std::max<size_t>(1, std::min<size_t>(cts.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::exception_ptr> errors(workers);
+    std::vector<std::thread> threads;
+    for (size_t w = 0; w < workers; ++w)
+        threads.emplace_back([&, w] {
+            try {
+                for (size_t i = w; i < cts.size(); i += workers)
+                    out[i] = ckks_decode_plaintext(c, coeffs[i], cts[i]->getLevel() + 1, cts[i]->getScale(), domain);
+            } catch (...) {
+                errors[w] = std::current_exception();
+            }
+        });
+    for (auto& t : threads) t.join();
+    for (auto& e : errors)
+        if (e) std::rethrow_exception(e);
+    return out;
+}
+
+namespace {
+MyVector<uint64_t> decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret) {
     if (use_low_secret || ct.getSecretOwner() == CKKSSecretOwner::BootstrapSparse)
         throw std::runtime_error("ckks_context: sparse bootstrap secret key is unavailable");
     const size_t polys = ct.getNumPolys();
@@ -281,5 +322,6 @@ MyVector<Complex> ckks_decrypt(const CKKSContext& c, const CKKSCiphertext& ct, C
     mi_check(mi_ntt_inverse(mi, t.data(), 1, (int)L, W, 0, c.stream()), "ckks_decrypt");
     MyVector<uint64_t> h(L * N);
     mi_check(mi_memcpy(mi, h.data(), t.data(), h.size() * 8, MI_COPY_D2H, c.stream()), "ckks_decrypt");
-    return ckks_decode_plaintext(c, h, L, ct.getScale(), domain);
+    return h;
 }
+}  // namespace

@@ -255,6 +255,20 @@ PYBIND11_MODULE(ckks_py, m) {
               return complex_np(ckks_decrypt(c, ct, d));
           },
           py::arg("context"), py::arg("ct"), py::arg("domain") = CKKSDecryptDomain::Slots);
+    m.def("ckks_decrypt_batch",
+          [](const CKKSContext& c, const py::list& cts, CKKSDecryptDomain d) {
+              std::vector<const CKKSCiphertext*> ptrs;
+              for (const auto& o : cts) ptrs.push_back(&o.cast<const CKKSCiphertext&>());
+              std::vector<MyVector<Complex>> out;
+              {
+                  py::gil_scoped_release unlocked;
+                  out = ckks_decrypt_batch(c, ptrs, d);
+              }
+              py::list res;
+              for (auto& v : out) res.append(complex_np(v));
+              return res;
+          },
+          py::arg("context"), py::arg("cts"), py::arg("domain") = CKKSDecryptDomain::Slots);
     m.def("ckks_decode_plaintext",
           [](const CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> coeffs, size_t limbs,
              double scale, CKKSDecryptDomain d) {

@@ -67,6 +67,10 @@ def test_aes_ctr_walsh15_decrypts_every_block():
           f"level {out[0].getLevel()}", flush=True)
     assert mismatches == 0
     assert max_err < 0.05, max_err
+    # the batched decryption equals the one-at-a-time decryption, value for value
+    batch = H.ckks_decrypt_batch(ctx, out)
+    for bit in (0, 57, 127):
+        assert np.array_equal(np.asarray(batch[bit]), np.asarray(H.ckks_decrypt(ctx, out[bit])))
 
 
 def test_aes_ctr_rejects_bad_block_spans():
