@@ -694,6 +694,25 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
     });
 }
 
+int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
+                    void* stream) {
+    return guard(op == MI_LIST_COPY ? "ckks_ciphertext_clone" : op == MI_LIST_ADD ? "ckks_add_inplace(ct,ct)"
+                                                                                  : "ckks_sub_inplace(ct,ct)",
+                 [&] {
+                     check_ctx(ctx);
+                     check_level(*ctx->c, level);
+                     if (op != MI_LIST_COPY && op != MI_LIST_ADD && op != MI_LIST_SUB)
+                         throw mi::Error(MI_EINVAL, "invalid list operation");
+                     if (npolys < 1 || !x || !y) throw mi::Error(MI_EINVAL, "invalid poly list");
+                     for (int k = 0; k < npolys; ++k) {
+                         check_ptr(x[k]);
+                         check_ptr(y[k]);
+                         if (op == MI_LIST_COPY && x[k] == y[k]) throw mi::Error(MI_EINVAL, "copy onto itself");
+                     }
+                     mi::pair_list_op(*ctx->c, op, level, x, y, npolys, S(stream));
+                 });
+}
+
 int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
                          const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
                          const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,

@@ -387,6 +387,43 @@ k_mul_plain_list(const BasisConst* __restrict__ bc, PolyList list, const u64* __
     *px = mont ? mont_mul(*px, pt[r], b.q, b.qninv) : mul_mod(*px, pt[r], b.q, b.mu_lo, b.mu_hi);
 }
 
+// x[k] = y[k] (op 0), x[k] + y[k] (op 1) or x[k] - y[k] (op 2) mod q for
+// listed poly pairs: one launch per RS_MAX_POLYS pairs.
+struct PolyPairList {
+    u64* x[RS_MAX_POLYS];
+    const u64* y[RS_MAX_POLYS];
+};
+template <int OP>
+__global__ void __launch_bounds__(EW_THREADS) k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, int logN) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    u64* px = l.x[blockIdx.y] + r;
+    const u64 v = l.y[blockIdx.y][r];
+    if constexpr (OP == 0) {
+        *px = v;
+    } else {
+        const u64 q = bc[ew_limb(logN)].q;
+        *px = OP == 1 ? add_mod(*px, v, q) : sub_mod(*px, v, q);
+    }
+}
+
+void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64* const* y, int npolys, hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
+    if (op < 0 || op > 2) throw Error(-1, "invalid list operation");
+    for (int k0 = 0; k0 < npolys; k0 += RS_MAX_POLYS) {
+        const int n = std::min(RS_MAX_POLYS, npolys - k0);
+        PolyPairList l{};
+        for (int k = 0; k < n; ++k) {
+            l.x[k] = x[k0 + k];
+            l.y[k] = y[k0 + k];
+        }
+        const dim3 g = ew_grid(c, level + 1, n);
+        if (op == 0) hipLaunchKernelGGL(k_pair_list<0>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
+        else if (op == 1) hipLaunchKernelGGL(k_pair_list<1>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
+        else hipLaunchKernelGGL(k_pair_list<2>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
+        MI_HIP(hipGetLastError());
+    }
+}
+
 void mul_plain_list(const Context& c, int level, u64* const* polys, int npolys, const u64* pt, bool mont,
                     hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");

@@ -339,7 +339,16 @@ CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uin
 
 CKKSCiphertext CKKSCiphertext::clone() const {
     CKKSCiphertext out(*ctx, polys.size(), scale, level, secret_owner, message_encoding_state);
-    for (size_t i = 0; i < polys.size(); ++i) copy_d2d(*ctx, out.polys[i].data(), polys[i].data(), (level + 1) * N);
+    if (polys.empty()) return out;
+    std::vector<uint64_t*> dst(polys.size());
+    std::vector<const uint64_t*> src(polys.size());
+    for (size_t i = 0; i < polys.size(); ++i) {
+        dst[i] = out.polys[i].data();
+        src[i] = polys[i].data();
+    }
+    mi_check(mi_poly_list_op(ctx->handle(), MI_LIST_COPY, (int)level, (int)polys.size(), dst.data(), src.data(),
+                             ctx->stream()),
+             "ckks_ciphertext");
     return out;
 }
 

@@ -149,12 +149,17 @@ static void add_or_sub(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiph
     if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
         ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
         throw std::invalid_argument(std::string(fn) + ": ciphertext metadata mismatch");
-    const size_t lv = ct1.getLevel();
-    for (size_t i = 0; i < ct1.getNumPolys(); ++i) {
-        const int rc = sub ? mi_sub(c.handle(), (int)lv, ct1.polyData(i), ct2.polyData(i), 1, words(c, lv), c.stream())
-                           : mi_add(c.handle(), (int)lv, ct1.polyData(i), ct2.polyData(i), 1, words(c, lv), c.stream());
-        mi_check(rc, fn);
+    const size_t lv = ct1.getLevel(), n = ct1.getNumPolys();
+    if (n == 0) return;
+    std::vector<uint64_t*> x(n);
+    std::vector<const uint64_t*> y(n);
+    for (size_t i = 0; i < n; ++i) {
+        x[i] = ct1.polyData(i);
+        y[i] = ct2.polyData(i);
     }
+    mi_check(mi_poly_list_op(c.handle(), sub ? MI_LIST_SUB : MI_LIST_ADD, (int)lv, (int)n, x.data(), y.data(),
+                             c.stream()),
+             fn);
 }
 
 void ckks_add_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {

@@ -329,6 +329,15 @@ int mi_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* pol
  * instead of a loop over ciphertexts. */
 int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, const uint64_t* pt,
                               int pt_montgomery, int rescale, void* stream);
+/* x[k] = y[k] (MI_LIST_COPY), x[k] + y[k] (MI_LIST_ADD) or x[k] - y[k]
+ * (MI_LIST_SUB) mod q over limbs 0..level of `npolys` listed poly pairs,
+ * one launch per 64 pairs: the per-poly loops of ckks_add_inplace /
+ * ckks_sub_inplace (ckks_addition.cpp:35-139) and of a ciphertext copy
+ * (CKKSCiphertext::clone, ckks_ciphertext.hpp:18-108) as one launch.
+ * A copy's source and destination must not overlap. */
+enum { MI_LIST_COPY = 0, MI_LIST_ADD = 1, MI_LIST_SUB = 2 };
+int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
+                    void* stream);
 
 /* ---- Walsh S-box ----
  * The linear combinations and raw 4+4 products of one output pair of the

@@ -1,0 +1,56 @@
+"""mi_poly_list_op (copy / add / sub of listed poly pairs in one launch)
+against numpy on the same residues, including a list longer than one
+launch (70 pairs), and its argument checks."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import product, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+
+def _arr(ptrs):
+    return (ctypes.c_void_p * len(ptrs))(*ptrs)
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_poly_list_op_matches_numpy(op):
+    M = product()
+    import torch
+    N, level = 1 << 11, 3
+    p, q = O.find_primes([60], [50, 45, 40, 55], N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(40 + op)
+    L = level + 1
+    count = 70
+    xs = [rand_residues(rng, q[:L], N) for _ in range(count)]
+    ys = [rand_residues(rng, q[:L], N) for _ in range(count)]
+    dx = [M.to_dev(x) for x in xs]
+    dy = [M.to_dev(y) for y in ys]
+    print(f"list op {op}: {count} pairs, N={N}, level {level}", flush=True)
+    M.check(M.lib().mi_poly_list_op(ctx.h, op, level, count, _arr([M.ptr(d) for d in dx]),
+                                    _arr([M.ptr(d) for d in dy]), M._stream(torch.cuda.current_stream())))
+    torch.cuda.synchronize()
+    qv = np.repeat(np.array(q[:L], dtype=object), N)
+    for x, y, d in zip(xs, ys, dx):
+        xo, yo = x.astype(object), y.astype(object)
+        want = yo if op == 0 else (xo + yo) % qv if op == 1 else (xo - yo) % qv
+        assert np.array_equal(M.from_dev(d), np.array(want, dtype=np.uint64))
+
+
+def test_poly_list_op_rejects_bad_arguments():
+    M = product()
+    import torch
+    N = 1 << 11
+    p, q = O.find_primes([60], [50, 45], N)
+    ctx = M.Context(N, q, p, 1)
+    d = M.to_dev(np.zeros(2 * N, np.uint64))
+    s = M._stream(torch.cuda.current_stream())
+    one = _arr([M.ptr(d)])
+    assert M.lib().mi_poly_list_op(ctx.h, 0, 1, 1, one, one, s) < 0       # copy onto itself
+    assert M.lib().mi_poly_list_op(ctx.h, 3, 1, 1, one, one, s) < 0       # unknown operation
+    assert M.lib().mi_poly_list_op(ctx.h, 1, 2, 1, one, one, s) < 0       # level beyond the chain
+    assert M.lib().mi_poly_list_op(ctx.h, 1, 1, 0, one, one, s) < 0       # empty list
