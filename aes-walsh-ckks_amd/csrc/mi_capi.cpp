@@ -695,22 +695,22 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
 }
 
 int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
-                    void* stream) {
-    return guard(op == MI_LIST_COPY ? "ckks_ciphertext_clone" : op == MI_LIST_ADD ? "ckks_add_inplace(ct,ct)"
-                                                                                  : "ckks_sub_inplace(ct,ct)",
-                 [&] {
-                     check_ctx(ctx);
-                     check_level(*ctx->c, level);
-                     if (op != MI_LIST_COPY && op != MI_LIST_ADD && op != MI_LIST_SUB)
-                         throw mi::Error(MI_EINVAL, "invalid list operation");
-                     if (npolys < 1 || !x || !y) throw mi::Error(MI_EINVAL, "invalid poly list");
-                     for (int k = 0; k < npolys; ++k) {
-                         check_ptr(x[k]);
-                         check_ptr(y[k]);
-                         if (op == MI_LIST_COPY && x[k] == y[k]) throw mi::Error(MI_EINVAL, "copy onto itself");
-                     }
-                     mi::pair_list_op(*ctx->c, op, level, x, y, npolys, S(stream));
-                 });
+                    const uint64_t* residues, void* stream) {
+    static const char* const names[] = {"ckks_ciphertext_clone", "ckks_add_inplace(ct,ct)", "ckks_sub_inplace(ct,ct)",
+                                        "ckks_mul_inplace(ct,int)", "ckks_evalchebyshev"};
+    return guard(op >= 0 && op <= 4 ? names[op] : "mi_poly_list_op", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (op < MI_LIST_COPY || op > MI_LIST_MUL_SCALAR_ADD) throw mi::Error(MI_EINVAL, "invalid list operation");
+        if (npolys < 1 || !x || (op != MI_LIST_MUL_SCALAR && !y)) throw mi::Error(MI_EINVAL, "invalid poly list");
+        if (op >= MI_LIST_MUL_SCALAR && !residues) throw mi::Error(MI_EINVAL, "null residues");
+        for (int k = 0; k < npolys; ++k) {
+            check_ptr(x[k]);
+            if (op != MI_LIST_MUL_SCALAR) check_ptr(y[k]);
+            if (op == MI_LIST_COPY && x[k] == y[k]) throw mi::Error(MI_EINVAL, "copy onto itself");
+        }
+        mi::pair_list_op(*ctx->c, op, level, x, y, npolys, residues, S(stream));
+    });
 }
 
 int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,

@@ -393,33 +393,60 @@ struct PolyPairList {
     u64* x[RS_MAX_POLYS];
     const u64* y[RS_MAX_POLYS];
 };
+// per-limb scalar residues of ops 3 (x *= r) and 4 (x += y * r)
+struct LimbResidues {
+    u64 r[MAX_LIMBS];
+};
 template <int OP>
-__global__ void __launch_bounds__(EW_THREADS) k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, int logN) {
+__global__ void __launch_bounds__(EW_THREADS)
+k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, LimbResidues res, int logN) {
     const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     u64* px = l.x[blockIdx.y] + r;
+    if constexpr (OP == 3) {
+        const int limb = ew_limb(logN);
+        const BasisConst b = bc[limb];
+        *px = mul_mod(*px, res.r[limb], b.q, b.mu_lo, b.mu_hi);
+        return;
+    }
     const u64 v = l.y[blockIdx.y][r];
     if constexpr (OP == 0) {
         *px = v;
+    } else if constexpr (OP == 4) {
+        const int limb = ew_limb(logN);
+        const BasisConst b = bc[limb];
+        *px = add_mod(*px, mul_mod(v, res.r[limb], b.q, b.mu_lo, b.mu_hi), b.q);
     } else {
         const u64 q = bc[ew_limb(logN)].q;
         *px = OP == 1 ? add_mod(*px, v, q) : sub_mod(*px, v, q);
     }
 }
 
-void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64* const* y, int npolys, hipStream_t s) {
+void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64* const* y, int npolys,
+                  const u64* residues, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
-    if (op < 0 || op > 2) throw Error(-1, "invalid list operation");
+    if (op < 0 || op > 4) throw Error(-1, "invalid list operation");
+    LimbResidues res{};
+    if (op >= 3)
+        for (int i = 0; i <= level; ++i) {
+            if (residues[i] >= c.h_bc[i].q) throw Error(-1, "scalar residue not below its modulus");
+            res.r[i] = residues[i];
+        }
     for (int k0 = 0; k0 < npolys; k0 += RS_MAX_POLYS) {
         const int n = std::min(RS_MAX_POLYS, npolys - k0);
         PolyPairList l{};
         for (int k = 0; k < n; ++k) {
             l.x[k] = x[k0 + k];
-            l.y[k] = y[k0 + k];
+            l.y[k] = op == 3 ? nullptr : y[k0 + k];
         }
         const dim3 g = ew_grid(c, level + 1, n);
-        if (op == 0) hipLaunchKernelGGL(k_pair_list<0>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
-        else if (op == 1) hipLaunchKernelGGL(k_pair_list<1>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
-        else hipLaunchKernelGGL(k_pair_list<2>, g, dim3(EW_THREADS), 0, s, c.d_bc, l, c.logN);
+        const dim3 b(EW_THREADS);
+        switch (op) {
+            case 0: hipLaunchKernelGGL(k_pair_list<0>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 1: hipLaunchKernelGGL(k_pair_list<1>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 2: hipLaunchKernelGGL(k_pair_list<2>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 3: hipLaunchKernelGGL(k_pair_list<3>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            default: hipLaunchKernelGGL(k_pair_list<4>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+        }
         MI_HIP(hipGetLastError());
     }
 }

@@ -636,6 +636,10 @@ std::vector<double> default_evalmod_polynomial(const CKKSParams& params);
 const std::array<uint8_t, 256>& aes_sbox_table();
 
 namespace ckks_detail {
+// Every poly of `ct` through mi_poly_list_op `op` (y = the same polys of
+// `other` where the op reads one, residues per limb where it scales).
+void poly_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const CKKSCiphertext* other,
+                  const uint64_t* residues, const char* fn);
 // scale_up_ciphertext_to_match_inplace (ckks_walsh_sbox.cpp:228-270): the
 // scale raised to `target` by an integer factor (an all-zero ciphertext only
 // takes the new scale).

@@ -39,10 +39,7 @@ void mul_real_to_scale(const CKKSContext& c, CKKSCiphertext& ct, double scalar, 
     const long double ss = output_scale / ct.getScale();
     std::vector<uint64_t> r(lv + 1);
     for (size_t l = 0; l <= lv; ++l) r[l] = scaled_real_mod_q(scalar, ss, q[l]);
-    for (size_t i = 0; i < ct.getNumPolys(); ++i)
-        mi_check(mi_mul_scalar(c.handle(), (int)lv, ct.polyData(i), 1, (int64_t)((lv + 1) * ct.getN()), r.data(),
-                               c.stream()),
-                 "evalmod_binary_inplace");
+    ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR, ct, nullptr, r.data(), "evalmod_binary_inplace");
     ct.setScale((double)output_scale);
 }
 

@@ -347,7 +347,7 @@ CKKSCiphertext CKKSCiphertext::clone() const {
         src[i] = polys[i].data();
     }
     mi_check(mi_poly_list_op(ctx->handle(), MI_LIST_COPY, (int)level, (int)polys.size(), dst.data(), src.data(),
-                             ctx->stream()),
+                             nullptr, ctx->stream()),
              "ckks_ciphertext");
     return out;
 }

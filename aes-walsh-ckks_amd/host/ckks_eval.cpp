@@ -45,10 +45,11 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncodi
         throw std::invalid_argument("ckks_mul_inplace: ciphertext/plaintext size mismatch");
     if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_mul_inplace: empty ciphertext");
     const size_t lv = ct.getLevel();
-    for (size_t i = 0; i < ct.getNumPolys(); ++i)
-        mi_check(mi_mul_plain(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), pt.data(),
-                              pt.isMontgomeryForm() ? 1 : 0, c.stream()),
-                 "ckks_mul_inplace");
+    std::vector<uint64_t*> polys(ct.getNumPolys());
+    for (size_t i = 0; i < polys.size(); ++i) polys[i] = ct.polyData(i);
+    mi_check(mi_mul_plain_rescale_list(c.handle(), (int)lv, (int)polys.size(), polys.data(), pt.data(),
+                                       pt.isMontgomeryForm() ? 1 : 0, 0, c.stream()),
+             "ckks_mul_inplace");
     ct.setScale(ct.getScale() * pt.getScale());
 }
 
@@ -86,9 +87,7 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t scalar) 
     const auto& q = c.getParams().getModuli();
     std::vector<uint64_t> r(lv + 1);
     for (size_t l = 0; l <= lv; ++l) r[l] = reduce_i64(scalar, q[l]);
-    for (size_t i = 0; i < ct.getNumPolys(); ++i)
-        mi_check(mi_mul_scalar(c.handle(), (int)lv, ct.polyData(i), 1, words(c, lv), r.data(), c.stream()),
-                 "ckks_mul_inplace(ct,int)");
+    ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR, ct, nullptr, r.data(), "ckks_mul_inplace(ct,int)");
 }
 
 void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, Complex value, double plaintext_scale) {
@@ -149,16 +148,20 @@ static void add_or_sub(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiph
     if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
         ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
         throw std::invalid_argument(std::string(fn) + ": ciphertext metadata mismatch");
-    const size_t lv = ct1.getLevel(), n = ct1.getNumPolys();
+    ckks_detail::poly_list_op(c, sub ? MI_LIST_SUB : MI_LIST_ADD, ct1, &ct2, nullptr, fn);
+}
+
+void ckks_detail::poly_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const CKKSCiphertext* other,
+                               const uint64_t* residues, const char* fn) {
+    const size_t n = ct.getNumPolys();
     if (n == 0) return;
     std::vector<uint64_t*> x(n);
-    std::vector<const uint64_t*> y(n);
+    std::vector<const uint64_t*> y(n, nullptr);
     for (size_t i = 0; i < n; ++i) {
-        x[i] = ct1.polyData(i);
-        y[i] = ct2.polyData(i);
+        x[i] = ct.polyData(i);
+        if (other) y[i] = other->polyData(i);
     }
-    mi_check(mi_poly_list_op(c.handle(), sub ? MI_LIST_SUB : MI_LIST_ADD, (int)lv, (int)n, x.data(), y.data(),
-                             c.stream()),
+    mi_check(mi_poly_list_op(c.handle(), op, (int)ct.getLevel(), (int)n, x.data(), y.data(), residues, c.stream()),
              fn);
 }
 

@@ -173,10 +173,7 @@ void mul_integer(const CKKSContext& c, CKKSCiphertext& ct, ld scalar, ld out_sca
         const size_t lv = ct.getLevel();
         std::vector<uint64_t> res(lv + 1);
         for (size_t l = 0; l <= lv; ++l) res[l] = s % q[l];
-        for (size_t i = 0; i < ct.getNumPolys(); ++i)
-            mi_check(mi_mul_scalar(c.handle(), (int)lv, ct.polyData(i), 1, (int64_t)((lv + 1) * ct.getN()), res.data(),
-                                   c.stream()),
-                     "ckks_evalpoly");
+        ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR, ct, nullptr, res.data(), "ckks_evalpoly");
     }
     ct.setScale((double)out_scale);
 }
@@ -192,10 +189,7 @@ void mul_real_then_add(const CKKSContext& c, const CKKSCiphertext& src, double s
     const size_t lv = src.getLevel();
     std::vector<uint64_t> r(lv + 1);
     for (size_t l = 0; l <= lv; ++l) r[l] = scaled_constant_mod_q(scalar, scalar_scale, q[l]);
-    for (size_t i = 0; i < src.getNumPolys(); ++i)
-        mi_check(mi_mul_scalar_add(c.handle(), (int)lv, dst.polyData(i), src.polyData(i), 1,
-                                   (int64_t)((lv + 1) * src.getN()), r.data(), c.stream()),
-                 "ckks_evalpoly");
+    ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR_ADD, dst, &src, r.data(), "ckks_evalpoly");
 }
 
 void raise_scale_to(const CKKSContext& c, CKKSCiphertext& ct, ld target) {

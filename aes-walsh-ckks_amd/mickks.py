@@ -32,7 +32,7 @@ _SIGS = {
     "mi_memcpy": (cint, [vp, vp, vp, ctypes.c_size_t, cint, vp]),
     "mi_stream_synchronize": (cint, [vp, vp]),
     "mi_stream_create": (cint, [vp, ctypes.POINTER(vp)]),
-    "mi_poly_list_op": (cint, [vp, cint, cint, cint, vp, vp, vp]),
+    "mi_poly_list_op": (cint, [vp, cint, cint, cint, vp, vp, vp, vp]),
     "mi_stream_destroy": (cint, [vp, vp]),
     "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),
     "mi_mul_scalar_add": (cint, [vp, cint, vp, vp, cint, i64, vp, vp]),

@@ -329,15 +329,19 @@ int mi_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* pol
  * instead of a loop over ciphertexts. */
 int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* const* polys, const uint64_t* pt,
                               int pt_montgomery, int rescale, void* stream);
-/* x[k] = y[k] (MI_LIST_COPY), x[k] + y[k] (MI_LIST_ADD) or x[k] - y[k]
- * (MI_LIST_SUB) mod q over limbs 0..level of `npolys` listed poly pairs,
- * one launch per 64 pairs: the per-poly loops of ckks_add_inplace /
- * ckks_sub_inplace (ckks_addition.cpp:35-139) and of a ciphertext copy
- * (CKKSCiphertext::clone, ckks_ciphertext.hpp:18-108) as one launch.
- * A copy's source and destination must not overlap. */
-enum { MI_LIST_COPY = 0, MI_LIST_ADD = 1, MI_LIST_SUB = 2 };
+/* Listed poly pairs, one launch per 64 pairs, over limbs 0..level:
+ * x[k] = y[k] (MI_LIST_COPY), x[k] + y[k] (MI_LIST_ADD), x[k] - y[k]
+ * (MI_LIST_SUB), x[k] * r (MI_LIST_MUL_SCALAR, y unused) or x[k] + y[k] * r
+ * (MI_LIST_MUL_SCALAR_ADD) mod q, r = residues[limb] (host array, each below
+ * its modulus).  The per-poly loops of a ciphertext copy
+ * (CKKSCiphertext::clone, ckks_ciphertext.hpp:18-108), ckks_add_inplace /
+ * ckks_sub_inplace (ckks_addition.cpp:35-139), ckks_mul_inplace(ct, int)
+ * (ckks_mul.cpp:37-272) and fused_multiply_by_real_scalar_then_add_inplace
+ * (ckks_evalpoly.cpp:352-398) as one launch.  A copy's source and
+ * destination must not overlap. */
+enum { MI_LIST_COPY = 0, MI_LIST_ADD = 1, MI_LIST_SUB = 2, MI_LIST_MUL_SCALAR = 3, MI_LIST_MUL_SCALAR_ADD = 4 };
 int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
-                    void* stream);
+                    const uint64_t* residues, void* stream);
 
 /* ---- Walsh S-box ----
  * The linear combinations and raw 4+4 products of one output pair of the

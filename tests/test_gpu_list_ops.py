@@ -32,7 +32,7 @@ def test_poly_list_op_matches_numpy(op):
     dy = [M.to_dev(y) for y in ys]
     print(f"list op {op}: {count} pairs, N={N}, level {level}", flush=True)
     M.check(M.lib().mi_poly_list_op(ctx.h, op, level, count, _arr([M.ptr(d) for d in dx]),
-                                    _arr([M.ptr(d) for d in dy]), M._stream(torch.cuda.current_stream())))
+                                    _arr([M.ptr(d) for d in dy]), None, M._stream(torch.cuda.current_stream())))
     torch.cuda.synchronize()
     qv = np.repeat(np.array(q[:L], dtype=object), N)
     for x, y, d in zip(xs, ys, dx):
@@ -50,7 +50,32 @@ def test_poly_list_op_rejects_bad_arguments():
     d = M.to_dev(np.zeros(2 * N, np.uint64))
     s = M._stream(torch.cuda.current_stream())
     one = _arr([M.ptr(d)])
-    assert M.lib().mi_poly_list_op(ctx.h, 0, 1, 1, one, one, s) < 0       # copy onto itself
-    assert M.lib().mi_poly_list_op(ctx.h, 3, 1, 1, one, one, s) < 0       # unknown operation
-    assert M.lib().mi_poly_list_op(ctx.h, 1, 2, 1, one, one, s) < 0       # level beyond the chain
-    assert M.lib().mi_poly_list_op(ctx.h, 1, 1, 0, one, one, s) < 0       # empty list
+    assert M.lib().mi_poly_list_op(ctx.h, 0, 1, 1, one, one, None, s) < 0       # copy onto itself
+    assert M.lib().mi_poly_list_op(ctx.h, 5, 1, 1, one, one, None, s) < 0       # unknown operation
+    assert M.lib().mi_poly_list_op(ctx.h, 3, 1, 1, one, one, None, s) < 0       # scaling without residues
+    assert M.lib().mi_poly_list_op(ctx.h, 1, 2, 1, one, one, None, s) < 0       # level beyond the chain
+    assert M.lib().mi_poly_list_op(ctx.h, 1, 1, 0, one, one, None, s) < 0       # empty list
+
+
+@pytest.mark.parametrize("op", [3, 4])
+def test_poly_list_scalar_ops_match_numpy(op):
+    M = product()
+    import torch
+    N, level = 1 << 11, 2
+    p, q = O.find_primes([60], [50, 45, 40], N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(60 + op)
+    L = level + 1
+    xs = [rand_residues(rng, q[:L], N) for _ in range(3)]
+    ys = [rand_residues(rng, q[:L], N) for _ in range(3)]
+    r = np.array([int(rng.integers(0, int(m))) for m in q[:L]], dtype=np.uint64)
+    dx, dy = [M.to_dev(x) for x in xs], [M.to_dev(y) for y in ys]
+    M.check(M.lib().mi_poly_list_op(ctx.h, op, level, 3, _arr([M.ptr(d) for d in dx]), _arr([M.ptr(d) for d in dy]),
+                                    r.ctypes.data_as(ctypes.c_void_p), M._stream(torch.cuda.current_stream())))
+    torch.cuda.synchronize()
+    qv = np.repeat(np.array(q[:L], dtype=object), N)
+    rv = np.repeat(r.astype(object), N)
+    for x, y, d in zip(xs, ys, dx):
+        xo, yo = x.astype(object), y.astype(object)
+        want = (xo * rv) % qv if op == 3 else (xo + yo * rv) % qv
+        assert np.array_equal(M.from_dev(d), np.array(want, dtype=np.uint64))
