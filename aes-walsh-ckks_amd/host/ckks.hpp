@@ -238,6 +238,9 @@ public:
     CKKSCiphertext& operator=(CKKSCiphertext&&) noexcept = default;
 
     CKKSCiphertext clone() const;
+    // clone() followed by ckks_drop_to_level_inplace(out, level), copying
+    // only limbs 0..level.
+    CKKSCiphertext cloneAtLevel(size_t level) const;
 
     long double getScale() const { return scale; }
     size_t getLevel() const { return level; }

@@ -337,7 +337,11 @@ CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uin
     for (size_t i = 0; i < host.size(); ++i) setPoly(i, host[i]);
 }
 
-CKKSCiphertext CKKSCiphertext::clone() const {
+CKKSCiphertext CKKSCiphertext::clone() const { return cloneAtLevel(level); }
+
+CKKSCiphertext CKKSCiphertext::cloneAtLevel(size_t lv) const {
+    if (lv > level) throw std::invalid_argument("ckks_drop_to_level_inplace: target level above current level");
+    const size_t level = lv;
     CKKSCiphertext out(*ctx, polys.size(), scale, level, secret_owner, message_encoding_state);
     if (polys.empty()) return out;
     std::vector<uint64_t*> dst(polys.size());

@@ -381,15 +381,16 @@ private:
         const bool rb = gen_unrescaled(b);
         if (ra) rescale_entry(a);
         if (rb) rescale_entry(b);
-        CKKSCiphertext out = ct_.at(a).clone();
-        const size_t level = a == b ? out.getLevel() : std::min(out.getLevel(), ct_.at(b).getLevel());
-        ckks_drop_to_level_inplace(out, level);
+        const CKKSCiphertext& lhs_src = ct_.at(a);
+        const size_t level = a == b ? lhs_src.getLevel() : std::min(lhs_src.getLevel(), ct_.at(b).getLevel());
+        CKKSCiphertext out = lhs_src.cloneAtLevel(level);
         const ScaleTerm prod = scale_product(sc_.at(a), sc_.at(b));
         if (a == b) {
             ckks_square_inplace(c_, out);
+        } else if (ct_.at(b).getLevel() == level) {
+            ckks_mul_inplace(c_, out, ct_.at(b));   // the reference's clone at the same level changes nothing
         } else {
-            CKKSCiphertext rhs = ct_.at(b).clone();
-            ckks_drop_to_level_inplace(rhs, level);
+            CKKSCiphertext rhs = ct_.at(b).cloneAtLevel(level);
             ckks_mul_inplace(c_, out, rhs);
         }
         ckks_relin_hybrid_inplace(c_, out);
