@@ -129,7 +129,7 @@ void add_term_ref(const CKKSContext& context, CKKSCiphertext& acc, const CKKSCip
         ckks_add_inplace(context, acc, term);
         return;
     }
-    add_term(context, acc, term.clone());
+    add_term(context, acc, term.cloneAtLevel(std::min(acc.getLevel(), term.getLevel())));
 }
 
 CKKSCiphertext add_bits(const CKKSContext& context, const CKKSCiphertext& lhs, const CKKSCiphertext& rhs) {
@@ -300,9 +300,15 @@ CKKSCiphertext copy_to_context(const CKKSContext& dst, const CKKSCiphertext& src
     if (&src.getContext() == &dst) return src.clone();
     CKKSCiphertext out(dst, src.getNumPolys(), src.getScale(), src.getLevel(), src.getSecretOwner(),
                        src.getMessageEncodingState());
-    const size_t bytes = (src.getLevel() + 1) * src.getN() * sizeof(uint64_t);
-    for (size_t k = 0; k < src.getNumPolys(); ++k)
-        mi_check(mi_memcpy(dst.handle(), out.polyData(k), src.polyData(k), bytes, MI_COPY_D2D, dst.stream()),
+    std::vector<uint64_t*> to(src.getNumPolys());
+    std::vector<const uint64_t*> from(src.getNumPolys());
+    for (size_t k = 0; k < to.size(); ++k) {
+        to[k] = out.polyData(k);
+        from[k] = src.polyData(k);
+    }
+    if (!to.empty())
+        mi_check(mi_poly_list_op(dst.handle(), MI_LIST_COPY, (int)src.getLevel(), (int)to.size(), to.data(),
+                                 from.data(), nullptr, dst.stream()),
                  "ckks_aes_walsh");
     return out;
 }

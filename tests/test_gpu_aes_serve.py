@@ -60,3 +60,12 @@ def test_request_is_the_same_alone_and_in_the_mix():
         assert srv.evaluations >= 3   # the warm-ups are not counted; the burst needs two evaluations
     finally:
         srv.close()
+
+
+def test_lanes_are_admitted_by_hbm_footprint():
+    """A second lane opens only if its measured pool footprint fits the budget."""
+    srv = aes_serve.AesCtrServer(KEY, lanes=2, contexts_per_lane=1, memory_budget=1)
+    try:
+        assert len(srv.lanes) == 1 and srv.footprints[0] > 0
+    finally:
+        srv.close()
