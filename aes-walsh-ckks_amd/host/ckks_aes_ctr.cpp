@@ -187,11 +187,45 @@ void rotate_batch(const CKKSContext& context, std::vector<CKKSCiphertext*>& cts,
              "ckks_rotate");
 }
 
+CKKSCiphertext copy_to_context(const CKKSContext& dst, const CKKSCiphertext& src);
+
+// Jobs 0..n-1 on contexts[j mod count], one host thread per context (the
+// calling thread's stream is synchronised first, each worker synchronises
+// its own stream at the end); `job(j, context)` runs on its context only.
+template <class Job>
+void run_on_contexts(CKKSContext& main, const std::vector<CKKSContext*>& ctxs, size_t n, Job job) {
+    auto run = [&](size_t w) {
+        for (size_t j = w; j < n; j += ctxs.size()) job(j, *ctxs[w]);
+        mi_check(mi_stream_synchronize(ctxs[w]->handle(), ctxs[w]->stream()), "ckks_aes_walsh");
+    };
+    if (ctxs.size() == 1) {
+        run(0);
+        return;
+    }
+    mi_check(mi_stream_synchronize(main.handle(), main.stream()), "ckks_aes_walsh");
+    std::vector<std::exception_ptr> errors(ctxs.size());
+    std::vector<std::thread> threads;
+    for (size_t w = 0; w < ctxs.size(); ++w)
+        threads.emplace_back([&, w] {
+            try {
+                run(w);
+            } catch (...) {
+                errors[w] = std::current_exception();
+            }
+        });
+    for (auto& t : threads) t.join();
+    for (auto& e : errors)
+        if (e) std::rethrow_exception(e);
+}
+
 // final_binary_refresh_complex_block_output_bits_inplace (:406-490): mask,
 // pack slots / n bits per ciphertext (bit first + s rotated by s n), refresh
 // the packed real and imaginary lanes with one complex-packed bootstrap,
-// recombine them, rotate each bit back and mask again.
-void final_binary_refresh(const CKKSContext& context, MyVector<CKKSCiphertext>& bits, size_t active) {
+// recombine them, rotate each bit back and mask again.  The packs are
+// independent (the reference runs them on OpenMP threads): they run over
+// the main context and `extra` as the S-boxes do.
+void final_binary_refresh(CKKSContext& context, MyVector<CKKSCiphertext>& bits, size_t active,
+                          const std::vector<CKKSContext*>& extra) {
     if (bits.size() != kBlockBits)
         throw std::invalid_argument("ckks_aes_walsh: expected 128 output bits for final refresh");
     const auto& P = context.getParams();
@@ -201,39 +235,50 @@ void final_binary_refresh(const CKKSContext& context, MyVector<CKKSCiphertext>& 
         throw std::invalid_argument("ckks_aes_walsh: invalid final refresh packing layout");
     ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, P.getBootstrapS2CDepth());
     const size_t batches = (kBlockBits + per_batch - 1) / per_batch;
-    MyVector<CKKSCiphertext> refreshed;
-    refreshed.reserve(kBlockBits);
-    for (size_t batch = 0; batch < batches; ++batch) {
+    std::vector<CKKSContext*> ctxs{&context};
+    ctxs.insert(ctxs.end(), extra.begin(), extra.end());
+    std::vector<MyVector<CKKSCiphertext>> done(batches);
+    run_on_contexts(context, ctxs, batches, [&](size_t batch, CKKSContext& wc) {
         const size_t first = batch * per_batch, count = std::min(per_batch, kBlockBits - first);
         // pack_final_refresh_lane (:347-367): zero_like + each rotated term
         MyVector<CKKSCiphertext> terms;
         terms.reserve(count);
         std::vector<CKKSCiphertext*> to_rotate;
         std::vector<int64_t> rot;
-        for (size_t s = 0; s < count; ++s) terms.push_back(bits[first + s].clone());
+        for (size_t s = 0; s < count; ++s) terms.push_back(copy_to_context(wc, bits[first + s]));
         for (size_t s = 1; s < count; ++s) {
             to_rotate.push_back(&terms[s]);
             rot.push_back((int64_t)(s * stride));
         }
-        rotate_batch(context, to_rotate, rot);
+        rotate_batch(wc, to_rotate, rot);
         auto packed = std::move(terms[0]);
-        for (size_t s = 1; s < count; ++s) ckks_add_inplace(context, packed, terms[s]);
-        auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(context, std::move(packed));
+        for (size_t s = 1; s < count; ++s) ckks_add_inplace(wc, packed, terms[s]);
+        auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(wc, std::move(packed));
         // combine_real_imag_for_final_refresh (:392-404)
-        auto imag_i = ckks_mul(context, lanes.second, Complex(0.0, 1.0), 1.0);
+        auto imag_i = ckks_mul(wc, lanes.second, Complex(0.0, 1.0), 1.0);
         imag_i.setScale(lanes.second.getScale());
         auto combined = std::move(lanes.first);
-        ckks_add_inplace(context, combined, imag_i);
+        ckks_add_inplace(wc, combined, imag_i);
         // unpack_final_refresh_lane_inplace (:369-390)
+        auto& out = done[batch];
         to_rotate.clear();
         rot.clear();
-        const size_t base = refreshed.size();
-        for (size_t s = 0; s < count; ++s) refreshed.push_back(combined.clone());
+        for (size_t s = 0; s < count; ++s) out.push_back(combined.clone());
         for (size_t s = 1; s < count; ++s) {
-            to_rotate.push_back(&refreshed[base + s]);
+            to_rotate.push_back(&out[s]);
             rot.push_back(-(int64_t)(s * stride));
         }
-        rotate_batch(context, to_rotate, rot);
+        rotate_batch(wc, to_rotate, rot);
+    });
+    MyVector<CKKSCiphertext> refreshed;
+    refreshed.reserve(kBlockBits);
+    for (auto& batch : done)
+        for (auto& ct : batch)
+            refreshed.push_back(&ct.getContext() == &context ? std::move(ct) : copy_to_context(context, ct));
+    if (ctxs.size() > 1) {
+        // the copies finish before the workers' buffers return to their pools
+        mi_check(mi_stream_synchronize(context.handle(), context.stream()), "ckks_aes_walsh");
+        done.clear();
     }
     bits = std::move(refreshed);
     ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, /*minimum_level_to_keep=*/0);
@@ -451,6 +496,6 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
     for (size_t bit = 0; bit < kBlockBits; ++bit)
         ckks_add_inplace(context, state[bit],
                          encode_public_bit(context, ciphertexts, bit, state[bit].getLevel(), state[bit].getScale()));
-    final_binary_refresh(context, state, active);
+    final_binary_refresh(context, state, active, workers);
     return state;
 }
