@@ -713,6 +713,27 @@ int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* co
     });
 }
 
+int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out, int nterms,
+                      const uint64_t* const* in, const uint32_t* coef, void* stream) {
+    return guard("ckks_aes_walsh", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (nout < 1 || !out || !in || !coef) throw mi::Error(MI_EINVAL, "invalid output list");
+        if (nterms < 1 || nterms > 16) throw mi::Error(MI_EINVAL, "1..16 terms per output");
+        for (int k = 0; k < nout; ++k) {
+            check_ptr(out[k]);
+            for (int t = 0; t < nterms; ++t) {
+                const size_t i = (size_t)k * nterms + t;
+                if (!coef[i]) continue;
+                check_ptr(in[i]);
+                for (int j = 0; j < nout; ++j)
+                    if (in[i] == out[j]) throw mi::Error(MI_EINVAL, "an output aliases an input");
+            }
+        }
+        mi::linear_combine(*ctx->c, level, nout, out, nterms, in, coef, S(stream));
+    });
+}
+
 int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
                          const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
                          const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,

@@ -451,6 +451,51 @@ void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64*
     }
 }
 
+// out[k] = sum_t coef[k][t] * in[k][t] mod q for listed output polys (coef 0:
+// term unused); LC_ROWS outputs of up to LC_TERMS terms per launch.
+constexpr int LC_ROWS = 12;
+constexpr int LC_TERMS = 16;
+struct LinCombArgs {
+    u64* out[LC_ROWS];
+    const u64* in[LC_ROWS][LC_TERMS];
+    unsigned coef[LC_ROWS][LC_TERMS];
+};
+__global__ void __launch_bounds__(EW_THREADS)
+k_lincomb(const BasisConst* __restrict__ bc, LinCombArgs a, int nterms, int logN) {
+    const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const int k = blockIdx.y;
+    const BasisConst b = bc[ew_limb(logN)];
+    u64 acc = 0;
+    for (int t = 0; t < nterms; ++t) {
+        const unsigned cf = a.coef[k][t];
+        if (cf == 0) continue;
+        acc = add_mod(acc, mul_mod(a.in[k][t][r], (u64)cf, b.q, b.mu_lo, b.mu_hi), b.q);
+    }
+    a.out[k][r] = acc;
+}
+
+void linear_combine(const Context& c, int level, int nout, u64* const* out, int nterms, const u64* const* in,
+                    const unsigned* coef, hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
+    if (nterms < 1 || nterms > LC_TERMS) throw Error(-1, "1..16 terms per output");
+    for (int l = 0; l <= level; ++l)
+        for (int i = 0; i < nout * nterms; ++i)
+            if ((u64)coef[i] >= c.h_bc[l].q) throw Error(-1, "coefficient not below the modulus");
+    for (int k0 = 0; k0 < nout; k0 += LC_ROWS) {
+        const int n = std::min(LC_ROWS, nout - k0);
+        LinCombArgs a{};
+        for (int k = 0; k < n; ++k) {
+            a.out[k] = out[k0 + k];
+            for (int t = 0; t < nterms; ++t) {
+                a.in[k][t] = in[(size_t)(k0 + k) * nterms + t];
+                a.coef[k][t] = coef[(size_t)(k0 + k) * nterms + t];
+            }
+        }
+        hipLaunchKernelGGL(k_lincomb, ew_grid(c, level + 1, n), dim3(EW_THREADS), 0, s, c.d_bc, a, nterms, c.logN);
+        MI_HIP(hipGetLastError());
+    }
+}
+
 void mul_plain_list(const Context& c, int level, u64* const* polys, int npolys, const u64* pt, bool mont,
                     hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");

@@ -664,6 +664,9 @@ MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_compl
 struct CKKSAesCtrWalshSboxOptions {
     bool parallelize_sbox_pairs = true;
     std::vector<CKKSContext*> sbox_contexts;
+    // MixColumns + AddRoundKey of a round as one mi_linear_combine (equal
+    // limb for limb to the per-operation sequence, which runs when false)
+    bool fuse_linear_layer = true;
 };
 
 // ckks_aes_ctr_walsh_sbox.hpp:23-35 / .cpp:936-977, 1037-1051: AES-128 CTR

@@ -377,6 +377,105 @@ void check_workers(const CKKSContext& context, const std::vector<CKKSContext*>& 
     }
 }
 
+// The integer factor scale_up_to would multiply a ciphertext of scale `from`
+// by to reach `to` (1 when the scales are within 2^-45), or 0 where it would
+// throw (the per-operation path then raises the reference's error).
+uint64_t scale_up_factor(long double from, long double to) {
+    const long double m = std::max<long double>({1.0L, std::abs(from), std::abs(to)});
+    if (std::isfinite(from) && std::isfinite(to) && std::abs(from - to) <= 0x1p-45L * m) return 1;
+    if (!(from <= to) || !(from > 0)) return 0;
+    const long double ratio = to / from, rounded = std::round(ratio);
+    if (rounded < 1.0L || std::abs(ratio - rounded) > std::max<long double>(1.0L, ratio) * 0x1p-20L ||
+        rounded > 4294967295.0L)
+        return 0;
+    return (uint64_t)rounded;
+}
+
+// ShiftRows has run; MixColumns (unless the last round) and AddRoundKey as
+// one linear map: every new bit is a sum of S-box output bits with small
+// multiplicities plus its round-key bit times the key's scale-up factor,
+// derived by running the per-operation sequence (mix_column_byte, add_term)
+// on coefficient vectors; one mi_linear_combine evaluates all 128 bits.
+// Returns false (state untouched) where the per-operation path must run:
+// bits of differing level, scale or degree, or a scale-up the reference
+// would refuse.
+bool fused_linear_layer(const CKKSContext& context, MyVector<CKKSCiphertext>& state,
+                        const MyVector<CKKSCiphertext>& keys, size_t round, bool mix) {
+    const CKKSCiphertext& s0 = state[0];
+    for (const auto& ct : state)
+        if (ct.getLevel() != s0.getLevel() || ct.getNumPolys() != 2 || ct.getScale() != s0.getScale()) return false;
+    const CKKSCiphertext& k0 = keys[round * kBlockBits];
+    for (size_t b = 0; b < kBlockBits; ++b) {
+        const auto& k = keys[round * kBlockBits + b];
+        if (k.getLevel() != k0.getLevel() || k.getNumPolys() != 2 || k.getScale() != k0.getScale()) return false;
+    }
+    using Vec = std::array<uint32_t, 2 * kBlockBits>;  // [S-box outputs after ShiftRows | round-key bits]
+    std::array<Vec, kBlockBits> v{};
+    for (size_t b = 0; b < kBlockBits; ++b) v[b][b] = 1;
+    auto add = [](Vec a, const Vec& b) {
+        for (size_t i = 0; i < a.size(); ++i) a[i] += b[i];
+        return a;
+    };
+    if (mix) {
+        std::array<Vec, kBlockBits> m{};
+        for (size_t c = 0; c < 4; ++c)
+            for (size_t o = 0; o < 4; ++o) {
+                const size_t i0 = 4 * c + o, i1 = 4 * c + (o + 1) % 4, i2 = 4 * c + (o + 2) % 4,
+                             i3 = 4 * c + (o + 3) % 4;
+                std::array<Vec, 8> sum, out;
+                for (size_t k = 0; k < 8; ++k) sum[k] = add(v[8 * i0 + k], v[8 * i1 + k]);
+                out = {sum[7], add(sum[0], sum[7]), sum[1], add(sum[2], sum[7]), add(sum[3], sum[7]),
+                       sum[4], sum[5], sum[6]};
+                for (size_t k = 0; k < 8; ++k) m[8 * i0 + k] = add(add(add(out[k], v[8 * i1 + k]), v[8 * i2 + k]), v[8 * i3 + k]);
+            }
+        v = m;
+    }
+    const long double target = std::max(s0.getScale(), k0.getScale());
+    const uint64_t fs = scale_up_factor(s0.getScale(), target), fk = scale_up_factor(k0.getScale(), target);
+    if (!fs || !fk) return false;
+    const size_t level = std::min(s0.getLevel(), k0.getLevel());
+    int nterms = 0;
+    for (size_t b = 0; b < kBlockBits; ++b) {
+        v[b][kBlockBits + b] += 1;
+        int n = 0;
+        for (size_t i = 0; i < v[b].size(); ++i) {
+            const uint64_t c = (uint64_t)v[b][i] * (i < kBlockBits ? fs : fk);
+            if (c > 0xffffffffull) return false;
+            n += v[b][i] != 0;
+        }
+        nterms = std::max(nterms, n);
+    }
+    if (nterms > 16) return false;
+    MyVector<CKKSCiphertext> out;
+    out.reserve(kBlockBits);
+    std::vector<uint64_t*> dst;
+    std::vector<const uint64_t*> src;
+    std::vector<uint32_t> coef;
+    for (size_t b = 0; b < kBlockBits; ++b) {
+        out.emplace_back(context, 2, target, level, s0.getSecretOwner(), s0.getMessageEncodingState());
+        for (size_t p = 0; p < 2; ++p) {
+            dst.push_back(out.back().polyData(p));
+            int n = 0;
+            for (size_t i = 0; i < v[b].size(); ++i) {
+                if (!v[b][i]) continue;
+                const CKKSCiphertext& term = i < kBlockBits ? state[i] : keys[round * kBlockBits + (i - kBlockBits)];
+                src.push_back(term.polyData(p));
+                coef.push_back((uint32_t)((uint64_t)v[b][i] * (i < kBlockBits ? fs : fk)));
+                ++n;
+            }
+            for (; n < nterms; ++n) {
+                src.push_back(src.back());
+                coef.push_back(0);
+            }
+        }
+    }
+    mi_check(mi_linear_combine(context.handle(), (int)level, (int)dst.size(), dst.data(), nterms, src.data(),
+                               coef.data(), context.stream()),
+             "ckks_aes_walsh");
+    state = std::move(out);
+    return true;
+}
+
 // sub_bytes_walsh_complex_blocks_mask_packed (:659-721): the 16 S-boxes.
 // The reference runs them in parallel over OpenMP threads; here each host
 // thread drives one context (its own stream and memory pool), byte b going
@@ -489,8 +588,10 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
         ckks_aes_mask_inactive_slots_inplace(context, state, pack);
         state = sub_bytes(context, state, pack, workers);
         shift_rows(state);
-        if (round != kRounds) mix_columns(context, state);
-        add_round_key(context, state, keys, round);
+        if (!options.fuse_linear_layer || !fused_linear_layer(context, state, keys, round, round != kRounds)) {
+            if (round != kRounds) mix_columns(context, state);
+            add_round_key(context, state, keys, round);
+        }
     }
     // CTR: plaintext = ciphertext + AES(counter) (add_public_complex_block_bits_lazy_inplace, :492-532)
     for (size_t bit = 0; bit < kBlockBits; ++bit)

@@ -539,11 +539,13 @@ PYBIND11_MODULE(ckks_py, m) {
     m.def("ckks_aes_ctr_walsh_sbox_complex_blocks",
           [ct_pylist](const py::object& context, const AesRoundKeyBits& keys,
                       const py::array_t<uint8_t, py::array::c_style>& counters,
-                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts, const py::list& sbox_contexts) {
+                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts, const py::list& sbox_contexts,
+                      bool fuse_linear_layer) {
               CKKSContext& c = context.cast<CKKSContext&>();
               const auto ctr = blocks_from_array(counters), cts = blocks_from_array(ciphertexts);
               CKKSAesCtrWalshSboxOptions options;
               for (const auto& w : sbox_contexts) options.sbox_contexts.push_back(&w.cast<CKKSContext&>());
+              options.fuse_linear_layer = fuse_linear_layer;
               MyVector<CKKSCiphertext> out;
               {
                   py::gil_scoped_release unlocked;
@@ -552,5 +554,5 @@ PYBIND11_MODULE(ckks_py, m) {
               return ct_pylist(out, context);
           },
           py::arg("context"), py::arg("keys"), py::arg("counter_blocks"), py::arg("ciphertext_blocks"),
-          py::arg("sbox_contexts") = py::list());
+          py::arg("sbox_contexts") = py::list(), py::arg("fuse_linear_layer") = true);
 }

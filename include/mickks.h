@@ -339,6 +339,16 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
  * (ckks_mul.cpp:37-272) and fused_multiply_by_real_scalar_then_add_inplace
  * (ckks_evalpoly.cpp:352-398) as one launch.  A copy's source and
  * destination must not overlap. */
+/* out[k] = sum_t coef[k * nterms + t] * in[k * nterms + t] mod q over limbs
+ * 0..level (coef 0: term unused; 1 <= nterms <= 16; coefficients below
+ * every modulus; no output may alias an input), 12 outputs per launch: the
+ * AES round's lazy-XOR layer (MixColumns and AddRoundKey as sums of S-box
+ * output and round-key bit ciphertexts with the reference's integer
+ * scale-ups, ckks_aes_ctr_walsh_sbox.cpp:575-641, 741-817) in a few
+ * launches.  Every sum is exact mod q, so the result equals the
+ * reference's sequence of additions. */
+int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out, int nterms,
+                      const uint64_t* const* in, const uint32_t* coef, void* stream);
 enum { MI_LIST_COPY = 0, MI_LIST_ADD = 1, MI_LIST_SUB = 2, MI_LIST_MUL_SCALAR = 3, MI_LIST_MUL_SCALAR_ADD = 4 };
 int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
                     const uint64_t* residues, void* stream);

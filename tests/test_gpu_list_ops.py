@@ -79,3 +79,36 @@ def test_poly_list_scalar_ops_match_numpy(op):
         xo, yo = x.astype(object), y.astype(object)
         want = (xo * rv) % qv if op == 3 else (xo + yo * rv) % qv
         assert np.array_equal(M.from_dev(d), np.array(want, dtype=np.uint64))
+
+
+def test_linear_combine_matches_numpy_and_checks_arguments():
+    M = product()
+    import torch
+    N, level = 1 << 11, 2
+    p, q = O.find_primes([60], [50, 45, 40], N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(77)
+    L = level + 1
+    ins = [rand_residues(rng, q[:L], N) for _ in range(6)]
+    din = [M.to_dev(x) for x in ins]
+    nout, nterms = 14, 4     # two launches of up to 12 outputs
+    dout = [M.to_dev(np.zeros(L * N, np.uint64)) for _ in range(nout)]
+    idx = rng.integers(0, 6, (nout, nterms))
+    coef = rng.integers(0, 9, (nout, nterms)).astype(np.uint32)
+    src = _arr([M.ptr(din[i]) for i in idx.reshape(-1)])
+    s = M._stream(torch.cuda.current_stream())
+    print(f"linear combine: {nout} outputs x {nterms} terms", flush=True)
+    M.check(M.lib().mi_linear_combine(ctx.h, level, nout, _arr([M.ptr(d) for d in dout]), nterms, src,
+                                      coef.ctypes.data_as(ctypes.c_void_p), s))
+    torch.cuda.synchronize()
+    qv = np.repeat(np.array(q[:L], dtype=object), N)
+    for k in range(nout):
+        want = sum(int(coef[k, t]) * ins[idx[k, t]].astype(object) for t in range(nterms)) % qv
+        assert np.array_equal(M.from_dev(dout[k]), np.array(want, dtype=np.uint64))
+    one = _arr([M.ptr(din[0])])
+    c1 = np.ones(1, np.uint32)
+    assert M.lib().mi_linear_combine(ctx.h, level, 1, one, 1, one, c1.ctypes.data_as(ctypes.c_void_p), s) < 0  # aliasing
+    c17 = np.ones(17, np.uint32)
+    src17 = _arr([M.ptr(din[1])] * 17)
+    assert M.lib().mi_linear_combine(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 17, src17,
+                                     c17.ctypes.data_as(ctypes.c_void_p), s) < 0                                # > 16 terms
