@@ -664,8 +664,9 @@ MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_compl
 struct CKKSAesCtrWalshSboxOptions {
     bool parallelize_sbox_pairs = true;
     std::vector<CKKSContext*> sbox_contexts;
-    // MixColumns + AddRoundKey of a round as one mi_linear_combine (equal
-    // limb for limb to the per-operation sequence, which runs when false)
+    // MixColumns + AddRoundKey of a round as one mi_linear_combine, and the
+    // 128 public counter / ciphertext bit encodings as one batched encoding
+    // (equal limb for limb to the per-operation sequence, which runs when false)
     bool fuse_linear_layer = true;
 };
 

@@ -164,6 +164,48 @@ CKKSEncoding encode_public_bit(const CKKSContext& context, std::span<const Block
     return ckks_encode(context, slots, (double)scale, level, /*eval=*/true);
 }
 
+// encode_active_complex_block_public_bit + ckks_add_inplace(ct, pt) for all
+// 128 bits (add_public_complex_block_bits_lazy_inplace, :492-532, and the
+// counter half of make_initial_counter_complex_block_state, :534-559): one
+// batched device encoding of the 128 slot vectors at the bits' common level
+// and scale, then one launch adding each to its bit's b poly (the encodings
+// are not in Montgomery form, so the addition is the plain one
+// ckks_add_inplace makes).  Bits of differing level or scale take the
+// per-bit path.
+void add_public_bits(const CKKSContext& context, MyVector<CKKSCiphertext>& bits, std::span<const Block> blocks,
+                     bool batched) {
+    const size_t level = bits[0].getLevel();
+    const long double scale = bits[0].getScale();
+    bool uniform = batched;
+    for (const auto& b : bits) uniform = uniform && b.getLevel() == level && b.getScale() == scale;
+    if (!uniform || !(scale > 0)) {
+        for (size_t bit = 0; bit < bits.size(); ++bit)
+            ckks_add_inplace(context, bits[bit],
+                             encode_public_bit(context, blocks, bit, bits[bit].getLevel(), bits[bit].getScale()));
+        return;
+    }
+    const auto& P = context.getParams();
+    const size_t slots = P.getSlots(), N = P.getN(), words = (level + 1) * N, n = blocks.size() / 2;
+    std::vector<Complex> vals(bits.size() * slots, Complex(0.0, 0.0));
+    for (size_t bit = 0; bit < bits.size(); ++bit)
+        for (size_t s = 0; s < n; ++s)
+            vals[bit * slots + s] = Complex((double)((blocks[s][bit / 8] >> (bit % 8)) & 1),
+                                            (double)((blocks[n + s][bit / 8] >> (bit % 8)) & 1));
+    DeviceBuffer enc(context.handle(), bits.size() * words);
+    mi_check(mi_encode_slots(context.handle(), reinterpret_cast<const double*>(vals.data()), (int)bits.size(),
+                             (double)scale, (int)(level + 1), 1, enc.data(), (int64_t)words, context.stream()),
+             "ckks_encode");
+    std::vector<uint64_t*> x(bits.size());
+    std::vector<const uint64_t*> y(bits.size());
+    for (size_t bit = 0; bit < bits.size(); ++bit) {
+        x[bit] = bits[bit].polyData(0);
+        y[bit] = enc.data() + bit * words;
+    }
+    mi_check(mi_poly_list_op(context.handle(), MI_LIST_ADD, (int)level, (int)bits.size(), x.data(), y.data(), nullptr,
+                             context.stream()),
+             "ckks_add_inplace(ct,pt)");
+}
+
 // In-place rotations of many ciphertexts of one level as one batch
 // (each equal to ckks_rotate_inplace).
 void rotate_batch(const CKKSContext& context, std::vector<CKKSCiphertext*>& cts, const std::vector<int64_t>& rot) {
@@ -578,11 +620,8 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
     // make_initial_counter_complex_block_state (:534-559): round key 0 + counter
     MyVector<CKKSCiphertext> state;
     state.reserve(kBlockBits);
-    for (size_t bit = 0; bit < kBlockBits; ++bit) {
-        state.push_back(keys[bit].clone());
-        ckks_add_inplace(context, state.back(),
-                         encode_public_bit(context, counters, bit, state.back().getLevel(), state.back().getScale()));
-    }
+    for (size_t bit = 0; bit < kBlockBits; ++bit) state.push_back(keys[bit].clone());
+    add_public_bits(context, state, counters, options.fuse_linear_layer);
     // evaluate_aes128_rounds_walsh_complex_blocks_impl (:819-903)
     for (size_t round = 1; round <= kRounds; ++round) {
         ckks_aes_mask_inactive_slots_inplace(context, state, pack);
@@ -594,9 +633,7 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
         }
     }
     // CTR: plaintext = ciphertext + AES(counter) (add_public_complex_block_bits_lazy_inplace, :492-532)
-    for (size_t bit = 0; bit < kBlockBits; ++bit)
-        ckks_add_inplace(context, state[bit],
-                         encode_public_bit(context, ciphertexts, bit, state[bit].getLevel(), state[bit].getScale()));
+    add_public_bits(context, state, ciphertexts, options.fuse_linear_layer);
     final_binary_refresh(context, state, active, workers);
     return state;
 }
