@@ -1,3 +1,4 @@
+#include <algorithm>
 // extern "C" entry points of libmickks.so (declared in include/mickks.h).
 #include <cstring>
 #include <string>
@@ -460,6 +461,42 @@ int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uin
         MI_HIP(hipMemcpyAsync(b, tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
         MI_HIP(hipMemcpyAsync(a, tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
         mi::keyswitch(c, mi::KsItem{a, key, b, a, level, 0, 1}, montgomery != 0, S(stream));
+    });
+}
+
+int mi_conjugate_batch(mi_context* ctx, int level, int count, uint64_t* const* b, uint64_t* const* a,
+                       const uint64_t* key, int montgomery, void* stream) {
+    return guard("ckks_conjugate", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(key);
+        if (count < 1 || !b || !a) throw mi::Error(MI_EINVAL, "invalid ciphertext list");
+        std::vector<const uint64_t*> seen;
+        for (int k = 0; k < count; ++k) {
+            check_ptr(b[k]);
+            check_ptr(a[k]);
+            seen.push_back(b[k]);
+            seen.push_back(a[k]);
+        }
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            throw mi::Error(MI_EINVAL, "ciphertexts of a batch must not share polys");
+        mi::Context& c = *ctx->c;
+        const int L = level + 1;
+        const long long words = (long long)L * (long long)c.N;
+        const uint64_t g = 2 * c.N - 1;
+        uint64_t* tmp = (uint64_t*)c.workspace(S(stream), sizeof(uint64_t) * 2 * words);
+        std::vector<mi::KsItem> items;
+        for (int k = 0; k < count; ++k) {
+            // as mi_conjugate: the automorphism first, then the key switch of the new a
+            mi::automorphism(c, L, g, b[k], tmp, 1, words, words, S(stream));
+            mi::automorphism(c, L, g, a[k], tmp + words, 1, words, words, S(stream));
+            MI_HIP(hipMemcpyAsync(b[k], tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+            MI_HIP(hipMemcpyAsync(a[k], tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
+            items.push_back(mi::KsItem{a[k], key, b[k], a[k], level, 0, 1});
+        }
+        for (int k0 = 0; k0 < count; k0 += 32)
+            mi::keyswitch_batch(c, items.data() + k0, std::min(32, count - k0), montgomery != 0, S(stream));
     });
 }
 

@@ -568,6 +568,9 @@ void ckks_relin_hybrid_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 CKKSCiphertext ckks_rotate(const CKKSContext& context, const CKKSCiphertext& ct, int64_t rotation);
 void ckks_rotate_inplace(const CKKSContext& context, CKKSCiphertext& ct, int64_t rotation);
 CKKSCiphertext ckks_conjugate(const CKKSContext& context, const CKKSCiphertext& ct);
+// ckks_conjugate of each ciphertext (one level, 2 polys), the key switches as
+// one batch (mi_conjugate_batch); each result equals ckks_conjugate's.
+MyVector<CKKSCiphertext> ckks_conjugate_batch(const CKKSContext& context, const MyVector<CKKSCiphertext>& cts);
 void ckks_conjugate_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 
 // ------------------------------------------------------------ Walsh S-box

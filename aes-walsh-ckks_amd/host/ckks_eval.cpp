@@ -228,6 +228,25 @@ void ckks_conjugate_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
              "ckks_conjugate");
 }
 
+MyVector<CKKSCiphertext> ckks_conjugate_batch(const CKKSContext& c, const MyVector<CKKSCiphertext>& cts) {
+    MyVector<CKKSCiphertext> out;
+    if (cts.empty()) return out;
+    out.reserve(cts.size());
+    std::vector<uint64_t*> b, a;
+    for (const auto& ct : cts) {
+        check_ct(c, ct, "ckks_conjugate");
+        if (ct.getNumPolys() != 2 || ct.getLevel() != cts[0].getLevel())
+            throw std::invalid_argument("ckks_conjugate: batch ciphertexts must have 2 polys and one level");
+        out.push_back(ct.clone());
+        b.push_back(out.back().polyData(0));
+        a.push_back(out.back().polyData(1));
+    }
+    mi_check(mi_conjugate_batch(c.handle(), (int)cts[0].getLevel(), (int)cts.size(), b.data(), a.data(),
+                                c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+             "ckks_conjugate");
+    return out;
+}
+
 CKKSCiphertext ckks_conjugate(const CKKSContext& c, const CKKSCiphertext& ct) {
     CKKSCiphertext out = ct.clone();
     ckks_conjugate_inplace(c, out);

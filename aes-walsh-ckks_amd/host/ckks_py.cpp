@@ -459,6 +459,10 @@ PYBIND11_MODULE(ckks_py, m) {
         }
         return r;
     };
+    m.def("ckks_conjugate_batch", [ct_list, ct_pylist](const py::object& context, const py::list& cts) {
+        auto out = ckks_conjugate_batch(context.cast<const CKKSContext&>(), ct_list(cts));
+        return ct_pylist(out, context);
+    });
     m.def("ckks_walsh_recombine_signs_to_paired_outputs",
           [ct_list, ct_pylist](const py::object& context, const py::list& low, const py::list& high) {
               const CKKSContext& c = context.cast<const CKKSContext&>();

@@ -533,9 +533,8 @@ CKKSCiphertext times_i(const CKKSContext& context, const CKKSCiphertext& ct, dou
 
 // split_real_imag_pair (:1080-1095): (z + conj z) and -i (z - conj z), both
 // read at twice the scale
-std::pair<CKKSCiphertext, CKKSCiphertext> split_real_imag_pair(const CKKSContext& context,
-                                                               const CKKSCiphertext& pair) {
-    auto conjugated = ckks_conjugate(context, pair);
+std::pair<CKKSCiphertext, CKKSCiphertext> split_real_imag_pair(const CKKSContext& context, const CKKSCiphertext& pair,
+                                                               const CKKSCiphertext& conjugated) {
     auto real = pair.clone();
     ckks_add_inplace(context, real, conjugated);
     real.setScale(pair.getScale() * 2.0L);
@@ -575,9 +574,21 @@ MyVector<CKKSCiphertext> ckks_walsh_sbox_complex_blocks_mask_packed(CKKSContext&
     auto paired1 = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked1.first, unpacked1.second);
     // combine_paired_lane_outputs_as_complex_bits (:1154-1193)
     std::array<std::optional<CKKSCiphertext>, 8> bits;
+    // the eight conjugations of split_real_imag_pair as one key-switch batch
+    MyVector<CKKSCiphertext> all;
+    for (size_t pair = 0; pair < 4; ++pair) all.push_back(paired0[pair].clone());
+    for (size_t pair = 0; pair < 4; ++pair) all.push_back(paired1[pair].clone());
+    bool one_level = true;
+    for (const auto& ct : all) one_level = one_level && ct.getLevel() == all[0].getLevel() && ct.getNumPolys() == 2;
+    MyVector<CKKSCiphertext> conj;
+    if (one_level) {
+        conj = ckks_conjugate_batch(context, all);
+    } else {
+        for (const auto& ct : all) conj.push_back(ckks_conjugate(context, ct));
+    }
     for (size_t pair = 0; pair < 4; ++pair) {
-        auto [real_low, real_high] = split_real_imag_pair(context, paired0[pair]);
-        auto [imag_low, imag_high] = split_real_imag_pair(context, paired1[pair]);
+        auto [real_low, real_high] = split_real_imag_pair(context, paired0[pair], conj[pair]);
+        auto [imag_low, imag_high] = split_real_imag_pair(context, paired1[pair], conj[4 + pair]);
         bits[pair] = combine_real_imag(context, std::move(real_low), imag_low);
         bits[pair + 4] = combine_real_imag(context, std::move(real_high), imag_high);
     }

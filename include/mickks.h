@@ -190,6 +190,12 @@ int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uin
  * ckks_hoist_ciphertext_a_qp_eval (ckks_keyswitch.cpp:344-373): ModUp of every
  * digit of a (level+1 eval limbs) to the other Q limbs and P. */
 int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, void* stream);
+/* ckks_conjugate_inplace (ckks_conjugate.cpp:16-61) of `count` ciphertexts at
+ * one level: each automorphism as in mi_conjugate, then the key switches of
+ * all of them as one batch (up to 32 per launch sequence); each result
+ * equals mi_conjugate's.  The ciphertexts must not share polys. */
+int mi_conjugate_batch(mi_context* ctx, int level, int count, uint64_t* const* b, uint64_t* const* a,
+                       const uint64_t* key, int montgomery, void* stream);
 /* ckks_hybrid_key_switch_hoisted_inplace (ckks_keyswitch.cpp:1072-1328): inner
  * product with the key, folded ModDown, combine (mode 0 = Add, 1 = Assign),
  * then the automorphism g of both outputs (galois 0 = none). */

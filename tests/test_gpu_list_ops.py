@@ -112,3 +112,28 @@ def test_linear_combine_matches_numpy_and_checks_arguments():
     src17 = _arr([M.ptr(din[1])] * 17)
     assert M.lib().mi_linear_combine(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 17, src17,
                                      c17.ctypes.data_as(ctypes.c_void_p), s) < 0                                # > 16 terms
+
+
+def test_conjugate_batch_equals_one_at_a_time():
+    """ckks_conjugate_batch (mi_conjugate_batch) against ckks_conjugate of each
+    ciphertext, limb for limb, on 5 ciphertexts."""
+    from helpers import host
+    H = host()
+    c = H.CKKSParamsConfig()
+    c.N = 1 << 12
+    c.mult_depth = 3
+    c.q0_bit = 50
+    c.q_bit = 40
+    c.p_bits = [60, 60]
+    c.dnum = 2
+    c.secure = False
+    c.seed = 81
+    ctx = H.CKKSContext(H.CKKSParams(c))
+    rng = np.random.default_rng(82)
+    slots = ctx.getParams().getSlots()
+    cts = [H.ckks_encrypt(ctx, rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots), 2) for _ in range(5)]
+    batch = H.ckks_conjugate_batch(ctx, cts)
+    for ct, got in zip(cts, batch):
+        want = H.ckks_conjugate(ctx, ct)
+        for k in range(2):
+            assert np.array_equal(np.asarray(got.getPoly(k)), np.asarray(want.getPoly(k)))
