@@ -96,6 +96,54 @@ int64_t mi_context_device_bytes(const mi_context* ctx) {
     return b;
 }
 
+int mi_device_memory_info(mi_context* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+    return guard("mi_device_memory_info", [&] {
+        check_ctx(ctx);
+        check_ptr(free_bytes);
+        check_ptr(total_bytes);
+        int dev = -1;
+        MI_HIP(hipGetDevice(&dev));
+        if (dev != ctx->c->device) MI_HIP(hipSetDevice(ctx->c->device));
+        size_t f = 0, t = 0;
+        MI_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = (int64_t)f;
+        *total_bytes = (int64_t)t;
+    });
+}
+
+int mi_probe_begin(mi_context* ctx) {
+    return guard("mi_probe_begin", [&] {
+        check_ctx(ctx);
+        mi::Context& c = *ctx->c;
+        std::lock_guard<std::mutex> g(c.probe_mu);
+        c.probe.on = true;
+        c.probe.used = 0;
+        c.probe.bytes = 0.0;
+        c.probe.limbs = 0;
+    });
+}
+
+int mi_probe_end(mi_context* ctx, double* stats) {
+    return guard("mi_probe_end", [&] {
+        check_ctx(ctx);
+        check_ptr(stats);
+        mi::Context& c = *ctx->c;
+        std::lock_guard<std::mutex> g(c.probe_mu);
+        c.probe.on = false;
+        double ms = 0.0;
+        for (size_t i = 0; i + 1 < c.probe.used; i += 2) {
+            MI_HIP(hipEventSynchronize(c.probe.events[i + 1]));
+            float t = 0.0f;
+            MI_HIP(hipEventElapsedTime(&t, c.probe.events[i], c.probe.events[i + 1]));
+            ms += t;
+        }
+        stats[0] = ms;
+        stats[1] = (double)(c.probe.used / 2);
+        stats[2] = c.probe.bytes;
+        stats[3] = (double)c.probe.limbs;
+    });
+}
+
 int mi_alloc(mi_context* ctx, size_t bytes, void** out) {
     return guard("mi_alloc", [&] {
         check_ctx(ctx);
@@ -246,6 +294,35 @@ int mi_tensor(mi_context* ctx, int level, uint64_t* b1, uint64_t* a1, const uint
         check_level(*ctx->c, level);
         check_ptr(b1); check_ptr(a1); check_ptr(b2); check_ptr(a2); check_ptr(c2);
         mi::tensor(*ctx->c, level + 1, b1, a1, b2, a2, c2, S(stream));
+    });
+}
+
+int mi_tensor_list(mi_context* ctx, int level, int count, uint64_t* const* b1, uint64_t* const* a1,
+                   const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* c2, void* stream) {
+    return guard(b2 ? "ckks_mul_inplace(ct,ct)" : "ckks_square_inplace", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1 || !b1 || !a1 || !c2 || (!b2 != !a2)) throw mi::Error(MI_EINVAL, "invalid ciphertext list");
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        for (int k = 0; k < count; ++k) {
+            check_ptr(b1[k]); check_ptr(a1[k]); check_ptr(c2[k]);
+            if (b2) {
+                check_ptr(b2[k]);
+                check_ptr(a2[k]);
+            }
+            // outputs are written in place: no two outputs of the list may overlap
+            const uint64_t* outs[3] = {b1[k], a1[k], c2[k]};
+            for (int j = 0; j <= k; ++j) {
+                const uint64_t* other[3] = {b1[j], a1[j], c2[j]};
+                for (int u = 0; u < 3; ++u)
+                    for (int v = 0; v < 3; ++v) {
+                        if (j == k && v <= u) continue;
+                        if (outs[u] < other[v] + W && other[v] < outs[u] + W)
+                            throw mi::Error(MI_EINVAL, "output polys overlap");
+                    }
+            }
+        }
+        mi::tensor_list(*ctx->c, level, count, b1, a1, b2, a2, c2, S(stream));
     });
 }
 
@@ -471,16 +548,16 @@ int mi_conjugate_batch(mi_context* ctx, int level, int count, uint64_t* const* b
         check_level(*ctx->c, level);
         check_ptr(key);
         if (count < 1 || !b || !a) throw mi::Error(MI_EINVAL, "invalid ciphertext list");
-        std::vector<const uint64_t*> seen;
-        for (int k = 0; k < count; ++k) {
-            check_ptr(b[k]);
-            check_ptr(a[k]);
-            seen.push_back(b[k]);
-            seen.push_back(a[k]);
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        for (int i = 0; i < count; ++i) {
+            check_ptr(b[i]);
+            check_ptr(a[i]);
+            for (int j = 0; j < i; ++j)  // conjugated in place: the ciphertexts must be distinct
+                for (const uint64_t* x : {b[i], a[i]})
+                    for (const uint64_t* y : {b[j], a[j]})
+                        if (x < y + W && y < x + W) throw mi::Error(MI_EINVAL, "ciphertexts of a batch overlap");
+            if (b[i] < a[i] + W && a[i] < b[i] + W) throw mi::Error(MI_EINVAL, "b and a overlap");
         }
-        std::sort(seen.begin(), seen.end());
-        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
-            throw mi::Error(MI_EINVAL, "ciphertexts of a batch must not share polys");
         mi::Context& c = *ctx->c;
         const int L = level + 1;
         const long long words = (long long)L * (long long)c.N;
@@ -733,17 +810,21 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
 
 int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
                     const uint64_t* residues, void* stream) {
-    static const char* const names[] = {"ckks_ciphertext_clone", "ckks_add_inplace(ct,ct)", "ckks_sub_inplace(ct,ct)",
-                                        "ckks_mul_inplace(ct,int)", "ckks_evalchebyshev"};
-    return guard(op >= 0 && op <= 4 ? names[op] : "mi_poly_list_op", [&] {
+    static const char* const names[] = {"ckks_ciphertext_clone",   "ckks_add_inplace(ct,ct)",
+                                        "ckks_sub_inplace(ct,ct)", "ckks_mul_inplace(ct,int)",
+                                        "ckks_evalchebyshev",      "ckks_add_integer_constant_inplace",
+                                        "ckks_zero_polys",         "ckks_add_inplace(ct,pt)"};
+    return guard(op >= 0 && op <= 7 ? names[op] : "mi_poly_list_op", [&] {
         check_ctx(ctx);
         check_level(*ctx->c, level);
-        if (op < MI_LIST_COPY || op > MI_LIST_MUL_SCALAR_ADD) throw mi::Error(MI_EINVAL, "invalid list operation");
-        if (npolys < 1 || !x || (op != MI_LIST_MUL_SCALAR && !y)) throw mi::Error(MI_EINVAL, "invalid poly list");
-        if (op >= MI_LIST_MUL_SCALAR && !residues) throw mi::Error(MI_EINVAL, "null residues");
+        if (op < MI_LIST_COPY || op > MI_LIST_ADD_PLAIN_MONT) throw mi::Error(MI_EINVAL, "invalid list operation");
+        const bool uses_y = op <= MI_LIST_SUB || op == MI_LIST_MUL_SCALAR_ADD || op == MI_LIST_ADD_PLAIN_MONT;
+        const bool uses_r = op == MI_LIST_MUL_SCALAR || op == MI_LIST_MUL_SCALAR_ADD || op == MI_LIST_ADD_SCALAR;
+        if (npolys < 1 || !x || (uses_y && !y)) throw mi::Error(MI_EINVAL, "invalid poly list");
+        if (uses_r && !residues) throw mi::Error(MI_EINVAL, "null residues");
         for (int k = 0; k < npolys; ++k) {
             check_ptr(x[k]);
-            if (op != MI_LIST_MUL_SCALAR) check_ptr(y[k]);
+            if (uses_y) check_ptr(y[k]);
             if (op == MI_LIST_COPY && x[k] == y[k]) throw mi::Error(MI_EINVAL, "copy onto itself");
         }
         mi::pair_list_op(*ctx->c, op, level, x, y, npolys, residues, S(stream));

@@ -128,6 +128,31 @@ void* Context::workspace(hipStream_t s, size_t bytes) {
     return w.ptr;
 }
 
+long Context::probe_open(hipStream_t s, long long limbs) const {
+    std::lock_guard<std::mutex> g(probe_mu);
+    if (!probe.on) return -1;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    MI_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return -1;  // graphs are timed as a whole
+    while (probe.events.size() < probe.used + 2) {
+        hipEvent_t e;
+        MI_HIP(hipEventCreate(&e));
+        probe.events.push_back(e);
+    }
+    const long slot = (long)probe.used;
+    probe.used += 2;
+    probe.bytes += 16.0 * (double)N * (double)limbs;
+    probe.limbs += limbs;
+    MI_HIP(hipEventRecord(probe.events[slot], s));
+    return slot;
+}
+
+void Context::probe_close(long slot, hipStream_t s) const {
+    if (slot < 0) return;
+    std::lock_guard<std::mutex> g(probe_mu);
+    MI_HIP(hipEventRecord(probe.events[slot + 1], s));
+}
+
 const std::vector<hipStream_t>& Context::side_streams() {
     std::lock_guard<std::mutex> g(side_mu);
     if (side.empty()) {
@@ -443,6 +468,7 @@ void context_destroy(Context* c) {
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     for (void* r : c->retired) (void)hipFree(r);
     for (hipStream_t st : c->side) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : c->probe.events) (void)hipEventDestroy(e);
     delete c;
 }
 

@@ -107,6 +107,24 @@ struct Context {
     std::set<void*> captured_blocks;
     std::vector<std::pair<size_t, void*>> graph_held;
 
+    // NTT probe (mi_probe_begin / mi_probe_end): while on, every forward or
+    // inverse NTT launch pair is bracketed by events on its stream and its
+    // algorithmic bytes (16 N per limb) are counted.
+    struct NttProbe {
+        bool on = false;
+        std::vector<hipEvent_t> events;  // start/stop pairs, reused across probes
+        size_t used = 0;
+        double bytes = 0.0;
+        long long limbs = 0;
+    };
+    mutable std::mutex probe_mu;
+    mutable NttProbe probe;
+    // probe_open reserves an event pair for a launch pair of `limbs` limbs and
+    // records its start event on s (returns -1 when the probe is off);
+    // probe_close records the pair's stop event on s.
+    long probe_open(hipStream_t s, long long limbs) const;
+    void probe_close(long slot, hipStream_t s) const;
+
     int nb() const { return nq + np; }
     int parts(int level) const {
         int L = level + 1;
@@ -189,6 +207,9 @@ void ew_scalar_mac(const Context& c, u64* dst, const u64* src, const u64* scalar
 void tensor(const Context& c, int limbs, u64* b1, u64* a1, const u64* b2, const u64* a2, u64* c2,
             hipStream_t s);
 void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s);
+// listed tensor products (b2 == nullptr: squares), level+1 limbs each
+void tensor_list(const Context& c, int level, int count, u64* const* b1, u64* const* a1, const u64* const* b2,
+                 const u64* const* a2, u64* const* c2, hipStream_t s);
 void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                   long long sstride, long long dstride, hipStream_t s);
 // dst (+)= automorphism(src); src and dst must not overlap

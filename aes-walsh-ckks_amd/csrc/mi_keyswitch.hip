@@ -594,6 +594,8 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
     // caller's stream, the others on side streams forked from it and joined
     // back, so small levels share the GPU instead of queueing.
     auto flush_all = [&]() {
+        // a group that was just dispatched whole (KS_MAX_ITEMS items) is left empty
+        open.erase(std::remove_if(open.begin(), open.end(), [](const Group& g) { return g.v.empty(); }), open.end());
         if (open.size() == 1) {
             keyswitch_items(c, open[0].v.data(), (int)open[0].v.size(), montgomery, s);
         } else if (open.size() > 1) {

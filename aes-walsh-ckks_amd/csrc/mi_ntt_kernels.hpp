@@ -181,13 +181,17 @@ void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
 // second pass runs job `out` (load intermediate, store output).
 template <class JobIn, class JobOut>
 void run_ntt_fwd(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
+    const long probe = c.probe_open(s, count);
     launch_cols<true>(c, in, count, s);
     launch_rows<true>(c, out, count, s);
+    c.probe_close(probe, s);
 }
 template <class JobIn, class JobOut>
 void run_ntt_inv(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
+    const long probe = c.probe_open(s, count);
     launch_rows<false>(c, in, count, s);
     launch_cols<false>(c, out, count, s);
+    c.probe_close(probe, s);
 }
 
 // Block -> (item, slot, tile) with the blocks of one (slot, tile) for all

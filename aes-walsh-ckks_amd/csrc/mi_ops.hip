@@ -180,6 +180,60 @@ void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s)
     MI_HIP(hipGetLastError());
 }
 
+// The tensor product (or square, when b2 is null) of listed ciphertext
+// pairs: blockIdx.y is the pair, so a batch of ciphertexts at one level is
+// one launch per TL_MAX pairs.
+constexpr int TL_MAX = 16;
+struct TensorList {
+    u64* b1[TL_MAX];
+    u64* a1[TL_MAX];
+    const u64* b2[TL_MAX];
+    const u64* a2[TL_MAX];
+    u64* c2[TL_MAX];
+};
+template <bool SQUARE>
+__global__ void __launch_bounds__(EW_THREADS) k_tensor_list(const BasisConst* __restrict__ bc, TensorList l, int logN) {
+    const long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
+    const int k = blockIdx.y;
+    const BasisConst b = bc[ew_limb(logN)];
+    const u64 vb1 = l.b1[k][i], va1 = l.a1[k][i];
+    if constexpr (SQUARE) {
+        l.c2[k][i] = mul_mod(va1, va1, b.q, b.mu_lo, b.mu_hi);
+        l.b1[k][i] = mul_mod(vb1, vb1, b.q, b.mu_lo, b.mu_hi);
+        const u64 ab = mul_mod(va1, vb1, b.q, b.mu_lo, b.mu_hi);
+        l.a1[k][i] = add_mod(ab, ab, b.q);
+    } else {
+        const u64 vb2 = l.b2[k][i], va2 = l.a2[k][i];
+        l.c2[k][i] = mul_mod(va1, va2, b.q, b.mu_lo, b.mu_hi);
+        l.b1[k][i] = mul_mod(vb1, vb2, b.q, b.mu_lo, b.mu_hi);
+        l.a1[k][i] = add_mod(mul_mod(va1, vb2, b.q, b.mu_lo, b.mu_hi), mul_mod(vb1, va2, b.q, b.mu_lo, b.mu_hi), b.q);
+    }
+}
+
+void tensor_list(const Context& c, int level, int count, u64* const* b1, u64* const* a1, const u64* const* b2,
+                 const u64* const* a2, u64* const* c2, hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
+    const bool sq = b2 == nullptr;
+    for (int k0 = 0; k0 < count; k0 += TL_MAX) {
+        const int n = std::min(TL_MAX, count - k0);
+        TensorList l{};
+        for (int k = 0; k < n; ++k) {
+            l.b1[k] = b1[k0 + k];
+            l.a1[k] = a1[k0 + k];
+            l.b2[k] = sq ? nullptr : b2[k0 + k];
+            l.a2[k] = sq ? nullptr : a2[k0 + k];
+            l.c2[k] = c2[k0 + k];
+        }
+        if (sq)
+            hipLaunchKernelGGL(k_tensor_list<true>, ew_grid(c, level + 1, n), dim3(EW_THREADS), 0, s, c.d_bc, l,
+                               c.logN);
+        else
+            hipLaunchKernelGGL(k_tensor_list<false>, ew_grid(c, level + 1, n), dim3(EW_THREADS), 0, s, c.d_bc, l,
+                               c.logN);
+        MI_HIP(hipGetLastError());
+    }
+}
+
 // b += REDC(1 * pt): Montgomery plaintext back to normal form (ckks_encoding.cpp:67-78)
 __global__ void __launch_bounds__(EW_THREADS)
 k_add_plain_mont(const BasisConst* __restrict__ bc, int logN, long long total, u64* b,
@@ -397,15 +451,22 @@ struct PolyPairList {
 struct LimbResidues {
     u64 r[MAX_LIMBS];
 };
+// ops 5 (x += r), 6 (x = 0) and 7 (x += REDC(y): a Montgomery-form plaintext
+// added in normal form) complete the list: a ciphertext batch's constant
+// additions, zero fills and plaintext additions are one launch each.
 template <int OP>
 __global__ void __launch_bounds__(EW_THREADS)
 k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, LimbResidues res, int logN) {
     const long long r = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     u64* px = l.x[blockIdx.y] + r;
-    if constexpr (OP == 3) {
+    if constexpr (OP == 6) {
+        *px = 0;
+        return;
+    }
+    if constexpr (OP == 3 || OP == 5) {
         const int limb = ew_limb(logN);
         const BasisConst b = bc[limb];
-        *px = mul_mod(*px, res.r[limb], b.q, b.mu_lo, b.mu_hi);
+        *px = OP == 3 ? mul_mod(*px, res.r[limb], b.q, b.mu_lo, b.mu_hi) : add_mod(*px, res.r[limb], b.q);
         return;
     }
     const u64 v = l.y[blockIdx.y][r];
@@ -415,6 +476,9 @@ k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, LimbResidues res,
         const int limb = ew_limb(logN);
         const BasisConst b = bc[limb];
         *px = add_mod(*px, mul_mod(v, res.r[limb], b.q, b.mu_lo, b.mu_hi), b.q);
+    } else if constexpr (OP == 7) {
+        const BasisConst b = bc[ew_limb(logN)];
+        *px = add_mod(*px, redc128(v, 0, b.q, b.qninv), b.q);
     } else {
         const u64 q = bc[ew_limb(logN)].q;
         *px = OP == 1 ? add_mod(*px, v, q) : sub_mod(*px, v, q);
@@ -424,9 +488,10 @@ k_pair_list(const BasisConst* __restrict__ bc, PolyPairList l, LimbResidues res,
 void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64* const* y, int npolys,
                   const u64* residues, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
-    if (op < 0 || op > 4) throw Error(-1, "invalid list operation");
+    if (op < 0 || op > 7) throw Error(-1, "invalid list operation");
+    const bool uses_y = op <= 2 || op == 4 || op == 7;
     LimbResidues res{};
-    if (op >= 3)
+    if (op == 3 || op == 4 || op == 5)
         for (int i = 0; i <= level; ++i) {
             if (residues[i] >= c.h_bc[i].q) throw Error(-1, "scalar residue not below its modulus");
             res.r[i] = residues[i];
@@ -436,7 +501,7 @@ void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64*
         PolyPairList l{};
         for (int k = 0; k < n; ++k) {
             l.x[k] = x[k0 + k];
-            l.y[k] = op == 3 ? nullptr : y[k0 + k];
+            l.y[k] = uses_y ? y[k0 + k] : nullptr;
         }
         const dim3 g = ew_grid(c, level + 1, n);
         const dim3 b(EW_THREADS);
@@ -445,7 +510,10 @@ void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64*
             case 1: hipLaunchKernelGGL(k_pair_list<1>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
             case 2: hipLaunchKernelGGL(k_pair_list<2>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
             case 3: hipLaunchKernelGGL(k_pair_list<3>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
-            default: hipLaunchKernelGGL(k_pair_list<4>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 4: hipLaunchKernelGGL(k_pair_list<4>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 5: hipLaunchKernelGGL(k_pair_list<5>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            case 6: hipLaunchKernelGGL(k_pair_list<6>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
+            default: hipLaunchKernelGGL(k_pair_list<7>, g, b, 0, s, c.d_bc, l, res, c.logN); break;
         }
         MI_HIP(hipGetLastError());
     }

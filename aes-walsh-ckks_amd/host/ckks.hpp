@@ -13,8 +13,10 @@
 //   * CKKSCiphertext / CKKSEncoding carry the CKKSContext they live on;
 //     getPoly()/getPlaintext() return host copies, setPoly() uploads, and
 //     polyData() exposes the device pointer.
-//   * randomness is seeded: CKKSParamsConfig::seed drives key generation and
-//     each context's encryption stream (the reference draws from getrandom).
+//   * randomness comes from one 64-bit seed per CKKSParams (drawn from
+//     std::random_device unless CKKSParamsConfig::seed sets it): it drives key
+//     generation and each context's encryption stream (the reference draws
+//     from getrandom directly).  Contexts built from one CKKSParams share keys.
 #pragma once
 
 #include <atomic>
@@ -60,10 +62,9 @@ struct CKKSBootstrapParams {
     size_t evalmod_polynomial_degree = 120;
     size_t evalmod_double_angle = 0;
     size_t evalmod_log_message_ratio = 0;
-    // fit controls of the reference's MPFR EvalMod fit (kept for the API; a
-    // context with bootstrap keys starts from the degree-d Chebyshev
-    // interpolant of 0.5 - 0.5 cos(2 pi K u), replaceable with
-    // CKKSContext::setEvalModPolynomial)
+    // controls of the reference's EvalMod fit: a context with bootstrap keys
+    // installs the fit for its parameters (default_evalmod_polynomial);
+    // CKKSContext::setEvalModPolynomial replaces it
     double evalmod_coefficient_prune_threshold = 1e-10;
     double evalmod_discrete_interval_radius = 0.0;
     size_t evalmod_discrete_validation_points_per_interval = 0;
@@ -107,7 +108,11 @@ struct CKKSParamsConfig {
     bool secure = true;
     bool fold_hybrid_keyswitch = true;
     bool montgomery = true;
-    uint64_t seed = 0x5eed;
+    // Drives key generation and each context's encryption randomness.  Unset
+    // (the default), CKKSParams draws it from std::random_device (getrandom),
+    // as the reference draws its keys; set it only to replay a key set (tests,
+    // contexts that must share keys).
+    std::optional<uint64_t> seed;
 };
 
 // AESWalsh15BootstrapParams / AESWalsh15Config (ckks_aes_params.hpp:35-82):
@@ -118,6 +123,10 @@ struct CKKSParamsConfig {
 // degree 100, QP evaluators for both transforms.
 CKKSBootstrapParams aes_walsh15_bootstrap_params();
 CKKSParamsConfig aes_walsh15_config();
+// The XBOOT presets of aes_bench's xboot rows (ckks_aes_params.hpp:12-211).
+CKKSParamsConfig xboot_aes13_reference_config();
+CKKSParamsConfig aes_xboot14_secure_config();
+CKKSParamsConfig aes_xboot15_config();
 
 // find_p_and_q_primes (random.cpp:66-131): P bit sizes first, then Q; one
 // descending cursor per bit size from 2^bits/(2N); q = 2kN+1; Miller-Rabin.
@@ -129,6 +138,12 @@ public:
     explicit CKKSParams(CKKSParamsConfig config);
     // CKKSParams::AESWalsh15 (ckks_aes_params.hpp:213-215)
     static CKKSParams AESWalsh15();
+    // ckks_params.hpp:232-237 / ckks_aes_params.hpp:217-235
+    static CKKSParams XBootParamAes13Reference();
+    static CKKSParams AESXBoot14Secure();
+    static CKKSParams AESXBoot14SecureSparse1024();
+    static CKKSParams AESXBoot15();
+    static CKKSParams AESXBoot15Sparse1024();
     size_t getN() const { return N; }
     size_t getSlots() const { return slots; }
     size_t getDepth() const { return depth; }
@@ -222,12 +237,23 @@ private:
 // ---------------------------------------------------------- ciphertext types
 // CKKSCiphertext (ckks_ciphertext.hpp:18-80): polys [b, a, (c2)], each
 // (level+1)*N words in the evaluation domain.  Move-only; clone() copies.
+//
+// A CKKSCiphertext may also hold a batch of `getBatchSize()` ciphertexts
+// that share every piece of metadata (level, scale, degree, secret owner,
+// encoding state): the reference runs one evaluation schedule per
+// ciphertext on OpenMP threads (16 S-boxes per AES round, the final-refresh
+// packs); a batch runs the schedule once, every ring step one launch for all
+// members, and each member's limbs equal what the one-ciphertext evaluation
+// gives.  stack() / unstack() move members in and out without copies.
+// polyData(k) names a poly of a single ciphertext; polyData(k, m) the poly k
+// of member m.  Functions that have no batched form reject batches.
 class CKKSCiphertext {
 public:
     // Uninitialised device polys.
     CKKSCiphertext(const CKKSContext& context, size_t num_polys, long double scale, size_t level,
                    CKKSSecretOwner secret_owner = CKKSSecretOwner::Dense,
-                   CKKSMessageEncodingState message_encoding_state = CKKSMessageEncodingState::Slots);
+                   CKKSMessageEncodingState message_encoding_state = CKKSMessageEncodingState::Slots,
+                   size_t batch = 1);
     // Uploads host polys (ckks_ciphertext.hpp:20-33 with the context added).
     CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uint64_t>> polys, long double scale,
                    size_t level, CKKSSecretOwner secret_owner = CKKSSecretOwner::Dense,
@@ -245,9 +271,15 @@ public:
     long double getScale() const { return scale; }
     size_t getLevel() const { return level; }
     size_t getN() const { return N; }
-    size_t getNumPolys() const { return polys.size(); }
+    size_t getNumPolys() const { return polys.size() / batch; }
     CKKSSecretOwner getSecretOwner() const { return secret_owner; }
     CKKSMessageEncodingState getMessageEncodingState() const { return message_encoding_state; }
+    size_t getBatchSize() const { return batch; }
+    // A batch of the members (same context, N, level, degree, owner, encoding
+    // state and scale; std::invalid_argument otherwise), buffers moved in.
+    static CKKSCiphertext stack(MyVector<CKKSCiphertext> members);
+    // The members, buffers moved out; *this is left empty.
+    MyVector<CKKSCiphertext> unstack();
     void setScale(long double s) { scale = s; }
     void setLevel(size_t l);
     void setSecretOwner(CKKSSecretOwner o) { secret_owner = o; }
@@ -261,6 +293,13 @@ public:
     // Device pointers (limb-major, limb l at l*N).
     uint64_t* polyData(size_t idx);
     const uint64_t* polyData(size_t idx) const;
+    uint64_t* polyData(size_t idx, size_t member);
+    const uint64_t* polyData(size_t idx, size_t member) const;
+    // Every device poly, poly-major (poly k of member m at k * batch + m).
+    std::vector<uint64_t*> allPolyData();
+    std::vector<const uint64_t*> allPolyData() const;
+    // Host copy of poly idx of one member.
+    MyVector<uint64_t> getPoly(size_t idx, size_t member) const;
     // Appends an uninitialised poly (the c2 slot of a tensor product).
     void addPoly();
     void resizePolys(size_t count);
@@ -273,7 +312,8 @@ private:
     size_t level;
     CKKSSecretOwner secret_owner;
     CKKSMessageEncodingState message_encoding_state;
-    std::vector<DeviceBuffer> polys;
+    size_t batch = 1;
+    std::vector<DeviceBuffer> polys;  // poly k of member m at k * batch + m
 };
 
 // CKKSEncoding (ckks_ciphertext.hpp:83-108): one (level+1)-limb plaintext.
@@ -382,6 +422,15 @@ class CKKSContext {
 public:
     explicit CKKSContext(CKKSParams params);
     ~CKKSContext();
+    // A lane of `owner`: its own device tables, memory pool, workspaces and
+    // stream, and the owner's secret, switching keys, bootstrapping plans and
+    // EvalMod polynomial read in place (nothing regenerated or copied), so a
+    // second evaluation stream costs its working memory only.  The owner
+    // must outlive its lanes; key-changing calls on a lane throw.  Lanes on
+    // separate host threads overlap on the device.
+    static std::unique_ptr<CKKSContext> makeLane(const CKKSContext& owner);
+    bool isLane() const { return owner_ != nullptr; }
+    const CKKSContext& keyOwner() const { return owner_ ? *owner_ : *this; }
     CKKSContext(const CKKSContext&) = delete;
     CKKSContext& operator=(const CKKSContext&) = delete;
 
@@ -395,19 +444,19 @@ public:
     const MyVector<Complex>& getTwiddleIfft() const { return twiddle_ifft; }
     const MyVector<size_t>& getPermTable() const { return perm_table; }
 
-    const MyVector<int8_t>& getSecretKeyCoeffs() const { return secret_coeffs; }
+    const MyVector<int8_t>& getSecretKeyCoeffs() const { return K().secret_coeffs; }
     // Evaluation-domain secret over q_0..q_Lmax, p_0..p_{K-1} (device).
-    const uint64_t* secretKeyDevice() const { return secret_qp.data(); }
+    const uint64_t* secretKeyDevice() const { return K().secret_qp.data(); }
     // Host copy of the evaluation-domain secret, one vector per Q limb.
     MyVector<MyVector<uint64_t>> getSecretKey() const;
 
     // Switching keys: [digits][2 (b, a)][nq+np][N] on the device.
-    size_t keyDigitCount() const { return key_digits; }
+    size_t keyDigitCount() const { return K().key_digits; }
     size_t switchKeyWords() const;
-    const uint64_t* getRelinKeyDevice() const { return relin_key.data(); }
-    bool hasConjugationKey() const { return conj_key.data() != nullptr; }
+    const uint64_t* getRelinKeyDevice() const { return K().relin_key.data(); }
+    bool hasConjugationKey() const { return K().conj_key.data() != nullptr; }
     const uint64_t* getConjugationKeyDevice() const;
-    size_t getRotationKeyCount() const { return rotation_keys.size(); }
+    size_t getRotationKeyCount() const { return K().rotation_keys.size(); }
     bool hasRotationKey(int64_t rotation) const;
     const uint64_t* getRotationKeyDevice(int64_t rotation) const;
     std::vector<int64_t> getRotationKeyValues() const;
@@ -428,26 +477,26 @@ public:
     uint64_t nextRandomnessSeed() const;
 
     // ---- bootstrapping material (ckks_context.cpp:603-705 with bts_keys)
-    bool hasSparseSecretEncapsulationKeys() const { return d2s_key.data() != nullptr; }
-    const MyVector<int8_t>& getLowSecretKeyCoeffs() const { return low_secret_coeffs; }
-    const uint64_t* lowSecretKeyDevice() const { return low_secret_qp.data(); }
+    bool hasSparseSecretEncapsulationKeys() const { return K().d2s_key.data() != nullptr; }
+    const MyVector<int8_t>& getLowSecretKeyCoeffs() const { return K().low_secret_coeffs; }
+    const uint64_t* lowSecretKeyDevice() const { return K().low_secret_qp.data(); }
     // [b(q0), b(p0), a(q0), a(p0)], N words each
     const uint64_t* getDenseToSparseSwitchKeyDevice() const;
     const uint64_t* getSparseToDenseSwitchKeyDevice() const;
     void setDenseToSparseSwitchKey(const MyVector<uint64_t>& key);
     void setSparseToDenseSwitchKey(const MyVector<uint64_t>& key);
     MyVector<uint64_t> downloadDenseToSparseSwitchKey() const;
-    bool hasEvalModBootstrapDftPlans() const { return s2c_plan != nullptr || c2s_plan != nullptr; }
+    bool hasEvalModBootstrapDftPlans() const { return K().s2c_plan != nullptr || K().c2s_plan != nullptr; }
     const CKKSLinearTransformPlan& getEvalModS2CPlan() const;
     const CKKSLinearTransformPlan& getEvalModScaledC2SPlan() const;
-    double getEvalModScaledC2STransformScale() const { return c2s_transform_scale; }
+    double getEvalModScaledC2STransformScale() const { return K().c2s_transform_scale; }
     // S2C plan for an input whose scale is `ratio` times the default scale
     // (cached; ckks_bootstrap_evalmod_inplace.cpp:114-154).
     const CKKSLinearTransformPlan& getScaledS2CPlan(uint64_t ratio) const;
     // The EvalMod Chebyshev coefficients.  The reference derives them with
     // MPFR at context construction; here they are supplied by the caller.
     void setEvalModPolynomial(const std::vector<double>& chebyshev_coeffs);
-    bool hasEvalModPolynomial() const { return !evalmod_coeffs.empty(); }
+    bool hasEvalModPolynomial() const { return !K().evalmod_coeffs.empty(); }
     const std::vector<double>& getEvalModPolynomial() const;
 
 private:
@@ -459,24 +508,36 @@ private:
                                      uint64_t stream_id) const;
     DeviceBuffer secret_qp_from_coeffs(const MyVector<int8_t>& coeffs) const;
 
+    // Secret, switching keys, bootstrapping plans and the EvalMod polynomial:
+    // owned by a key-owning context, read through K() by it and its lanes.
+    struct KeyMaterial {
+        MyVector<int8_t> secret_coeffs;
+        DeviceBuffer secret_qp;
+        size_t key_digits = 0;
+        DeviceBuffer relin_key, conj_key;
+        std::map<int64_t, DeviceBuffer> rotation_keys;
+        mutable std::atomic<uint64_t> randomness_counter{0};
+        MyVector<int8_t> low_secret_coeffs;
+        DeviceBuffer low_secret_qp, d2s_key, s2d_key;
+        std::unique_ptr<CKKSLinearTransformPlan> s2c_plan, c2s_plan;
+        double c2s_transform_scale = 1.0;
+        std::vector<double> evalmod_coeffs;
+    };
+    const KeyMaterial& K() const { return owner_ ? owner_->keys_ : keys_; }
+    KeyMaterial& K();  // the owner's own material; a lane throws
+    CKKSContext(const CKKSContext& owner, int lane_tag);
+
     CKKSParams params;
     mi_context* mi = nullptr;
     void* stream_ = nullptr;
+    const CKKSContext* owner_ = nullptr;  // set for a lane: the context whose keys it reads
+    mutable std::atomic<int> lanes_{0};   // live lanes of this key-owning context
     MyVector<Complex> twiddle_fft, twiddle_ifft;
     MyVector<size_t> perm_table;
-    MyVector<int8_t> secret_coeffs;
-    DeviceBuffer secret_qp;
-    size_t key_digits = 0;
-    DeviceBuffer relin_key, conj_key;
-    std::map<int64_t, DeviceBuffer> rotation_keys;
-    mutable std::atomic<uint64_t> randomness_counter{0};
-    MyVector<int8_t> low_secret_coeffs;
-    DeviceBuffer low_secret_qp, d2s_key, s2d_key;
-    std::unique_ptr<CKKSLinearTransformPlan> s2c_plan, c2s_plan;
-    double c2s_transform_scale = 1.0;
+    KeyMaterial keys_;
+    // scaled S2C plans, per context (their diagonals live in its own pool)
     mutable std::mutex scaled_mu;
     mutable std::map<uint64_t, std::unique_ptr<CKKSLinearTransformPlan>> scaled_s2c;
-    std::vector<double> evalmod_coeffs;
 
 public:
     // An encoding that depends only on `key` (what is encoded, its scale and
@@ -637,8 +698,14 @@ std::array<uint8_t, 16> aes128_ctr_crypt_block(const std::array<uint8_t, 16>& ke
                                                const std::array<uint8_t, 16>& counter_block,
                                                const std::array<uint8_t, 16>& input_block);
 // The default EvalMod polynomial of a bootstrapping context (Chebyshev
-// coefficients, degree evalmod_polynomial_degree).
+// coefficients on [-1, 1] / 2^double_angle): the reference's discrete
+// multi-interval fit (poly_approx.cpp:565-681) for the EvalMod parameters of
+// its presets; otherwise, without double-angle steps, the degree-d
+// interpolant of 0.5 - 0.5 cos(2 pi K u) on [-1, 1]; empty with double-angle
+// steps (the context then needs setEvalModPolynomial).
 std::vector<double> default_evalmod_polynomial(const CKKSParams& params);
+// Whether default_evalmod_polynomial returns the reference's fit.
+bool evalmod_polynomial_is_reference_fit(const CKKSParams& params);
 const std::array<uint8_t, 256>& aes_sbox_table();
 
 namespace ckks_detail {
@@ -646,6 +713,11 @@ namespace ckks_detail {
 // `other` where the op reads one, residues per limb where it scales).
 void poly_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const CKKSCiphertext* other,
                   const uint64_t* residues, const char* fn);
+// The b poly of every member of `ct` through a y-less mi_poly_list_op `op`
+// (MI_LIST_ADD_SCALAR: the constant terms of a batch).
+void b_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const uint64_t* residues, const char* fn);
+// Polys from..degree-1 of every member set to zero (one launch per 64 polys).
+void zero_polys(const CKKSContext& c, CKKSCiphertext& ct, size_t from, const char* fn);
 // scale_up_ciphertext_to_match_inplace (ckks_walsh_sbox.cpp:228-270): the
 // scale raised to `target` by an integer factor (an all-zero ciphertext only
 // takes the new scale).
@@ -666,6 +738,11 @@ MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits_walsh_compl
 // and seed, hence the same keys, each with its own stream and memory pool).
 struct CKKSAesCtrWalshSboxOptions {
     bool parallelize_sbox_pairs = true;
+    // The 16 S-boxes of a round (and the final refresh's packs) evaluated as
+    // one ciphertext batch on the main context: one launch sequence for all
+    // of them.  When false they run one after another on the main stream,
+    // or over sbox_contexts (one host thread each) when those are given.
+    bool batch_sboxes = true;
     std::vector<CKKSContext*> sbox_contexts;
     // MixColumns + AddRoundKey of a round as one mi_linear_combine, and the
     // 128 public counter / ciphertext bit encodings as one batched encoding
@@ -683,6 +760,73 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(
     std::span<const std::array<uint8_t, 16>> counter_blocks,
     std::span<const std::array<uint8_t, 16>> ciphertext_blocks, const CKKSAesCtrWalshSboxOptions& options = {});
 
+
+// Served walsh15 AES-CTR with continuous batching (no counterpart in the
+// reference, whose bench evaluates one fixed batch).  Lanes of the key owner
+// evaluate whole requests packed first come first served into 1024-block
+// evaluations; see ckks_aes_serve.cpp for admission.
+struct CKKSAesServerOptions {
+    size_t lanes = 2;                   // evaluation streams, each a lane of the key owner
+    int64_t memory_budget_bytes = 0;    // device bytes the lanes may take (0: 90 % of the free memory)
+    size_t max_request_blocks = 1024;   // at most one evaluation
+    size_t max_queued_evaluations = 4;  // waiting work admitted per lane, in evaluations
+};
+struct CKKSAesRequestResult {
+    uint64_t id = 0;
+    bool refused = false;               // refused at admission (error says why)
+    std::string error;                  // empty on success
+    std::vector<std::array<uint8_t, 16>> plaintexts;
+    std::vector<std::array<double, 128>> values;  // decrypted bit values per block
+    double arrival = 0, start = 0, finish = 0;    // steady-clock seconds
+    size_t lane = 0;
+};
+// The packing rule of a lane: the offsets of the queued requests (block
+// counts, first come first served) taken whole into one evaluation of
+// `capacity` blocks; the first request that does not fit ends the batch.
+std::vector<size_t> ckks_aes_serve_take(const std::vector<size_t>& queued_blocks, size_t capacity);
+class CKKSAesCtrServer {
+public:
+    // Encrypts the round keys under `owner` and opens the lanes (a warm-up
+    // evaluation each).  The owner must outlive the server.
+    CKKSAesCtrServer(CKKSContext& owner, const std::array<uint8_t, 16>& aes_key,
+                     const CKKSAesServerOptions& options = {});
+    ~CKKSAesCtrServer();  // finishes the admitted requests, then stops
+    // Queues a request (or records its refusal); returns its id at once.
+    uint64_t submit(std::vector<std::array<uint8_t, 16>> counter_blocks,
+                    std::vector<std::array<uint8_t, 16>> ciphertext_blocks);
+    // Blocks until the request has completed (or was refused).
+    CKKSAesRequestResult wait(uint64_t id);
+    size_t laneCount() const;
+    int64_t evaluationBytes() const;
+    size_t evaluations() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+// ckks_aes_ctr.hpp:14-28: AES-128 CTR with the XBOOT monomial S-box and a
+// bootstrap of the whole state every round (aes_bench's xboot rows).
+// parallelize_independent_work is the reference's OpenMP switch (kept for
+// the API); batch_independent_work evaluates the 16 S-boxes of a round and
+// the 64 pair bootstraps each as one ciphertext batch (limb for limb the
+// one-at-a-time result, which runs when it is false).
+struct CKKSAes128CtrOptions {
+    bool parallelize_independent_work = true;
+    bool batch_independent_work = true;
+};
+// ckks_aes_ctr.cpp:687-725: round-key bits in every slot at the S2C level
+// (round 0 `first_round_key_level` levels higher).
+MyVector<CKKSCiphertext> ckks_aes128_encrypt_expanded_round_key_bits(const CKKSContext& context,
+                                                                     const std::array<uint8_t, 176>& expanded_round_keys,
+                                                                     size_t first_round_key_level);
+// ckks_aes_ctr.cpp:757-779: one AES block per slot (block s % active slots),
+// 128 refreshed plaintext bit ciphertexts.
+MyVector<CKKSCiphertext> ckks_aes_ctr(CKKSContext& context,
+                                      const MyVector<CKKSCiphertext>& encrypted_expanded_round_key_bits,
+                                      std::span<const std::array<uint8_t, 16>> counter_blocks,
+                                      std::span<const std::array<uint8_t, 16>> ciphertext_blocks,
+                                      const CKKSAes128CtrOptions& options = {});
 
 // Chebyshev-basis polynomial evaluation (ckks_evalpoly.hpp:9-20): the
 // coefficients c_0..c_d of sum c_i T_i(x), x already mapped into [-1, 1].

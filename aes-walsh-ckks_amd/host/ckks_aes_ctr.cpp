@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <exception>
+#include <limits>
 #include <thread>
 
 #include "ckks.hpp"
@@ -102,7 +103,57 @@ void pad_with_zero_polys(const CKKSContext& context, CKKSCiphertext& ct, size_t 
     }
 }
 
-// align_for_add_exact / add_term_inplace (:128-147): both operands to the
+// aes_scales_close (:42-49)
+bool aes_scales_close(long double lhs, long double rhs) {
+    if (!std::isfinite(lhs) || !std::isfinite(rhs)) return false;
+    const long double m = std::max<long double>({1.0L, std::abs(lhs), std::abs(rhs)});
+    return std::abs(lhs - rhs) <= 0x1p-45L * m;
+}
+
+// The integer factor scale_up_ciphertext_to_match_inplace (:105-124) with
+// multiply_by_integer_scalar_inplace (:80-103) multiplies a ciphertext of
+// scale `from` by to reach `to`: 1 when the scales are close, otherwise the
+// rounded ratio within the reference's 1e-3 integrality tolerance.  Returns
+// the reference's error message where it would throw (for a non-zero
+// ciphertext), nullptr otherwise.
+const char* aes_scale_up_factor(long double from, long double to, int64_t* factor) {
+    *factor = 1;
+    if (aes_scales_close(from, to)) return nullptr;
+    if (from > to) return "ckks_aes_walsh: cannot scale down ciphertext for addition";
+    const long double scalar = to / from;
+    if (!std::isfinite(scalar) || scalar < 0.0L) return "ckks_aes_walsh: invalid scalar scale-up";
+    const long double rounded = std::round(scalar);
+    if (rounded < 1.0L || std::abs(scalar - rounded) > std::max<long double>(1.0L, std::abs(scalar)) * 1e-3L)
+        return "ckks_aes_walsh: non-integral scalar scale-up";
+    if (rounded > (long double)std::numeric_limits<int64_t>::max())
+        return "ckks_aes_walsh: scalar scale-up is too large";
+    *factor = (int64_t)rounded;
+    return nullptr;
+}
+
+bool is_zero_ciphertext(const CKKSCiphertext& ct) {
+    for (size_t m = 0; m < ct.getBatchSize(); ++m)
+        for (size_t k = 0; k < ct.getNumPolys(); ++k)
+            for (uint64_t w : ct.getPoly(k, m))
+                if (w) return false;
+    return true;
+}
+
+// scale_up_ciphertext_to_match_inplace (:105-124): an all-zero ciphertext
+// only takes the new scale (tested here only where the integer path throws:
+// a zero times the factor is zero too).
+void aes_scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double target) {
+    int64_t factor;
+    if (const char* err = aes_scale_up_factor(ct.getScale(), target, &factor)) {
+        if (!is_zero_ciphertext(ct)) throw std::invalid_argument(err);
+        ct.setScale(target);
+        return;
+    }
+    if (factor != 1) ckks_mul_inplace(context, ct, factor);
+    ct.setScale(target);
+}
+
+// align_for_add_exact / add_term_inplace (:126-147): both operands to the
 // lower level, the higher scale (an integer scale-up) and the larger degree,
 // then one addition.
 void add_term(const CKKSContext& context, CKKSCiphertext& acc, CKKSCiphertext term) {
@@ -112,7 +163,7 @@ void add_term(const CKKSContext& context, CKKSCiphertext& acc, CKKSCiphertext te
     for (CKKSCiphertext* ct : {&acc, &term}) {
         ckks_drop_to_level_inplace(*ct, level);
         pad_with_zero_polys(context, *ct, polys);
-        ckks_detail::scale_up_to(context, *ct, scale);
+        aes_scale_up_to(context, *ct, scale);
     }
     ckks_add_inplace(context, acc, term);
 }
@@ -267,7 +318,7 @@ void run_on_contexts(CKKSContext& main, const std::vector<CKKSContext*>& ctxs, s
 // independent (the reference runs them on OpenMP threads): they run over
 // the main context and `extra` as the S-boxes do.
 void final_binary_refresh(CKKSContext& context, MyVector<CKKSCiphertext>& bits, size_t active,
-                          const std::vector<CKKSContext*>& extra) {
+                          const std::vector<CKKSContext*>& extra, bool batched) {
     if (bits.size() != kBlockBits)
         throw std::invalid_argument("ckks_aes_walsh: expected 128 output bits for final refresh");
     const auto& P = context.getParams();
@@ -277,6 +328,53 @@ void final_binary_refresh(CKKSContext& context, MyVector<CKKSCiphertext>& bits, 
         throw std::invalid_argument("ckks_aes_walsh: invalid final refresh packing layout");
     ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, P.getBootstrapS2CDepth());
     const size_t batches = (kBlockBits + per_batch - 1) / per_batch;
+    if (batched) {
+        // the packs as one batch: all their rotations in one rotate batch,
+        // the packed ciphertexts through one batched bootstrap
+        MyVector<CKKSCiphertext> terms;
+        terms.reserve(kBlockBits);
+        for (auto& b : bits) terms.push_back(std::move(b));
+        std::vector<CKKSCiphertext*> to_rotate;
+        std::vector<int64_t> rot;
+        for (size_t i = 0; i < kBlockBits; ++i)
+            if (i % per_batch != 0) {
+                to_rotate.push_back(&terms[i]);
+                rot.push_back((int64_t)((i % per_batch) * stride));
+            }
+        rotate_batch(context, to_rotate, rot);
+        MyVector<CKKSCiphertext> packs;
+        for (size_t batch = 0; batch < batches; ++batch) {
+            const size_t first = batch * per_batch, count = std::min(per_batch, kBlockBits - first);
+            CKKSCiphertext packed = std::move(terms[first]);
+            for (size_t s = 1; s < count; ++s) ckks_add_inplace(context, packed, terms[first + s]);
+            packs.push_back(std::move(packed));
+        }
+        terms.clear();
+        auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(context, CKKSCiphertext::stack(std::move(packs)));
+        auto imag_i = ckks_mul(context, lanes.second, Complex(0.0, 1.0), 1.0);
+        imag_i.setScale(lanes.second.getScale());
+        auto combined = std::move(lanes.first);
+        ckks_add_inplace(context, combined, imag_i);
+        auto per_pack = combined.unstack();
+        MyVector<CKKSCiphertext> refreshed;
+        refreshed.reserve(kBlockBits);
+        to_rotate.clear();
+        rot.clear();
+        for (size_t batch = 0; batch < batches; ++batch) {
+            const size_t first = batch * per_batch, count = std::min(per_batch, kBlockBits - first);
+            for (size_t s = 0; s < count; ++s)
+                refreshed.push_back(s + 1 == count ? std::move(per_pack[batch]) : per_pack[batch].clone());
+        }
+        for (size_t i = 0; i < kBlockBits; ++i)
+            if (i % per_batch != 0) {
+                to_rotate.push_back(&refreshed[i]);
+                rot.push_back(-(int64_t)((i % per_batch) * stride));
+            }
+        rotate_batch(context, to_rotate, rot);
+        bits = std::move(refreshed);
+        ckks_aes_mask_final_refresh_slots_inplace(context, bits, stride, /*minimum_level_to_keep=*/0);
+        return;
+    }
     std::vector<CKKSContext*> ctxs{&context};
     ctxs.insert(ctxs.end(), extra.begin(), extra.end());
     std::vector<MyVector<CKKSCiphertext>> done(batches);
@@ -419,18 +517,13 @@ void check_workers(const CKKSContext& context, const std::vector<CKKSContext*>& 
     }
 }
 
-// The integer factor scale_up_to would multiply a ciphertext of scale `from`
-// by to reach `to` (1 when the scales are within 2^-45), or 0 where it would
-// throw (the per-operation path then raises the reference's error).
+// The factor aes_scale_up_to multiplies by, or 0 where it would throw or
+// the factor exceeds 32 bits (the per-operation path then runs and raises
+// the reference's error where there is one).
 uint64_t scale_up_factor(long double from, long double to) {
-    const long double m = std::max<long double>({1.0L, std::abs(from), std::abs(to)});
-    if (std::isfinite(from) && std::isfinite(to) && std::abs(from - to) <= 0x1p-45L * m) return 1;
-    if (!(from <= to) || !(from > 0)) return 0;
-    const long double ratio = to / from, rounded = std::round(ratio);
-    if (rounded < 1.0L || std::abs(ratio - rounded) > std::max<long double>(1.0L, ratio) * 0x1p-20L ||
-        rounded > 4294967295.0L)
-        return 0;
-    return (uint64_t)rounded;
+    int64_t factor;
+    if (aes_scale_up_factor(from, to, &factor) || factor < 1 || factor > 4294967295LL) return 0;
+    return (uint64_t)factor;
 }
 
 // ShiftRows has run; MixColumns (unless the last round) and AddRoundKey as
@@ -444,13 +537,19 @@ uint64_t scale_up_factor(long double from, long double to) {
 bool fused_linear_layer(const CKKSContext& context, MyVector<CKKSCiphertext>& state,
                         const MyVector<CKKSCiphertext>& keys, size_t round, bool mix) {
     const CKKSCiphertext& s0 = state[0];
+    // same level, degree, scale, secret owner and encoding state for every
+    // state and key bit (the additions of the per-operation path would
+    // otherwise refuse mismatched metadata)
+    auto same_meta = [&](const CKKSCiphertext& ct, const CKKSCiphertext& ref) {
+        return ct.getLevel() == ref.getLevel() && ct.getNumPolys() == 2 && ct.getScale() == ref.getScale() &&
+               ct.getBatchSize() == 1 && ct.getSecretOwner() == s0.getSecretOwner() &&
+               ct.getMessageEncodingState() == s0.getMessageEncodingState();
+    };
     for (const auto& ct : state)
-        if (ct.getLevel() != s0.getLevel() || ct.getNumPolys() != 2 || ct.getScale() != s0.getScale()) return false;
+        if (!same_meta(ct, s0)) return false;
     const CKKSCiphertext& k0 = keys[round * kBlockBits];
-    for (size_t b = 0; b < kBlockBits; ++b) {
-        const auto& k = keys[round * kBlockBits + b];
-        if (k.getLevel() != k0.getLevel() || k.getNumPolys() != 2 || k.getScale() != k0.getScale()) return false;
-    }
+    for (size_t b = 0; b < kBlockBits; ++b)
+        if (!same_meta(keys[round * kBlockBits + b], k0)) return false;
     using Vec = std::array<uint32_t, 2 * kBlockBits>;  // [S-box outputs after ShiftRows | round-key bits]
     std::array<Vec, kBlockBits> v{};
     for (size_t b = 0; b < kBlockBits; ++b) v[b][b] = 1;
@@ -516,6 +615,37 @@ bool fused_linear_layer(const CKKSContext& context, MyVector<CKKSCiphertext>& st
              "ckks_aes_walsh");
     state = std::move(out);
     return true;
+}
+
+// sub_bytes_walsh_complex_blocks_mask_packed (:659-721) as one batch: input
+// bit k of all 16 bytes forms one batched ciphertext (the masked state bits
+// share level and scale), the mask-packed S-box runs once over the 8
+// batches, so every step of the 16 S-boxes (parity packing, bootstrap,
+// unpacking, product sums, conjugations) is one launch sequence for all of
+// them; member b of output bit k is byte b's bit k, limb for limb the
+// one-S-box result.
+MyVector<CKKSCiphertext> sub_bytes_batched(CKKSContext& context, MyVector<CKKSCiphertext> state, size_t pack) {
+    MyVector<CKKSCiphertext> input;
+    input.reserve(8);
+    for (size_t k = 0; k < 8; ++k) {
+        MyVector<CKKSCiphertext> members;
+        members.reserve(kBlockBytes);
+        for (size_t byte = 0; byte < kBlockBytes; ++byte) members.push_back(std::move(state[8 * byte + k]));
+        input.push_back(CKKSCiphertext::stack(std::move(members)));
+    }
+    auto bits = ckks_walsh_sbox_complex_blocks_mask_packed(context, std::move(input), pack);
+    if (bits.size() != 8) throw std::runtime_error("ckks_aes_walsh: complex-block Walsh S-box returned wrong size");
+    std::array<MyVector<CKKSCiphertext>, 8> split;
+    for (size_t k = 0; k < 8; ++k) {
+        split[k] = bits[k].unstack();
+        if (split[k].size() != kBlockBytes)
+            throw std::runtime_error("ckks_aes_walsh: batched S-box returned a wrong batch size");
+    }
+    MyVector<CKKSCiphertext> out;
+    out.reserve(kBlockBits);
+    for (size_t byte = 0; byte < kBlockBytes; ++byte)
+        for (size_t k = 0; k < 8; ++k) out.push_back(std::move(split[k][byte]));
+    return out;
 }
 
 // sub_bytes_walsh_complex_blocks_mask_packed (:659-721): the 16 S-boxes.
@@ -600,7 +730,8 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
                                                                 std::span<const Block> ciphertexts,
                                                                 const CKKSAesCtrWalshSboxOptions& options) {
     const std::vector<CKKSContext*> no_workers;
-    const auto& workers = options.parallelize_sbox_pairs ? options.sbox_contexts : no_workers;
+    const auto& workers =
+        options.parallelize_sbox_pairs && !options.batch_sboxes ? options.sbox_contexts : no_workers;
     check_workers(context, workers);
     // validate_complex_block_inputs (:905-934)
     if (keys.size() != kExpandedKeyBits)
@@ -617,23 +748,37 @@ MyVector<CKKSCiphertext> ckks_aes_ctr_walsh_sbox_complex_blocks(CKKSContext& con
         if (ct.getN() != context.getParams().getN() || ct.getNumPolys() != 2 || ct.getLevel() != key_level)
             throw std::invalid_argument(
                 "ckks_aes_walsh: complex-block path received a round key bit at the wrong level");
+    // Round-key bits encrypted under another context of the same keys (the
+    // key owner of a lane) are copied onto this context's stream and pool
+    // first: every operation below then stays on one stream.
+    MyVector<CKKSCiphertext> local_keys;
+    if (&keys.front().getContext() != &context) {
+        mi_check(mi_stream_synchronize(keys.front().getContext().handle(), keys.front().getContext().stream()),
+                 "ckks_aes_walsh");
+        local_keys.reserve(keys.size());
+        for (const auto& k : keys) local_keys.push_back(copy_to_context(context, k));
+    }
+    const MyVector<CKKSCiphertext>& round_keys = local_keys.empty() ? keys : local_keys;
     // make_initial_counter_complex_block_state (:534-559): round key 0 + counter
     MyVector<CKKSCiphertext> state;
     state.reserve(kBlockBits);
-    for (size_t bit = 0; bit < kBlockBits; ++bit) state.push_back(keys[bit].clone());
+    for (size_t bit = 0; bit < kBlockBits; ++bit) state.push_back(round_keys[bit].clone());
     add_public_bits(context, state, counters, options.fuse_linear_layer);
     // evaluate_aes128_rounds_walsh_complex_blocks_impl (:819-903)
     for (size_t round = 1; round <= kRounds; ++round) {
         ckks_aes_mask_inactive_slots_inplace(context, state, pack);
-        state = sub_bytes(context, state, pack, workers);
+        if (options.batch_sboxes)
+            state = sub_bytes_batched(context, std::move(state), pack);
+        else
+            state = sub_bytes(context, state, pack, workers);
         shift_rows(state);
-        if (!options.fuse_linear_layer || !fused_linear_layer(context, state, keys, round, round != kRounds)) {
+        if (!options.fuse_linear_layer || !fused_linear_layer(context, state, round_keys, round, round != kRounds)) {
             if (round != kRounds) mix_columns(context, state);
-            add_round_key(context, state, keys, round);
+            add_round_key(context, state, round_keys, round);
         }
     }
     // CTR: plaintext = ciphertext + AES(counter) (add_public_complex_block_bits_lazy_inplace, :492-532)
     add_public_bits(context, state, ciphertexts, options.fuse_linear_layer);
-    final_binary_refresh(context, state, active, workers);
+    final_binary_refresh(context, state, active, workers, options.batch_sboxes);
     return state;
 }

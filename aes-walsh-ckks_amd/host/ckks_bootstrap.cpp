@@ -105,9 +105,10 @@ void ckks_key_switch_dense_to_sparse_inplace(const CKKSContext& c, CKKSCiphertex
         throw std::invalid_argument("ckks_key_switch: dense-to-sparse expects a dense-secret ciphertext");
     if (ct.getNumPolys() != 2 || ct.getLevel() != 0)
         throw std::invalid_argument("ckks_key_switch: dense-to-sparse expects a level-0 2-polynomial ciphertext");
-    mi_check(mi_q0p0_keyswitch(c.handle(), ct.polyData(0), ct.polyData(1), c.getDenseToSparseSwitchKeyDevice(),
-                               c.getParams().montgomery ? 1 : 0, c.stream()),
-             "ckks_key_switch");
+    for (size_t m = 0; m < ct.getBatchSize(); ++m)
+        mi_check(mi_q0p0_keyswitch(c.handle(), ct.polyData(0, m), ct.polyData(1, m),
+                                   c.getDenseToSparseSwitchKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+                 "ckks_key_switch");
     ct.setSecretOwner(CKKSSecretOwner::BootstrapSparse);
 }
 
@@ -115,9 +116,20 @@ void ckks_key_switch_sparse_to_dense_inplace(const CKKSContext& c, CKKSCiphertex
     if (ct.getSecretOwner() != CKKSSecretOwner::BootstrapSparse)
         throw std::invalid_argument("ckks_key_switch: sparse-to-dense expects a sparse-secret ciphertext");
     if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_key_switch: expected a 2-polynomial ciphertext");
-    mi_check(mi_keyswitch(c.handle(), (int)ct.getLevel(), ct.polyData(1), c.getSparseToDenseSwitchKeyDevice(),
-                          ct.polyData(0), ct.polyData(1), 0, 1, c.getParams().montgomery ? 1 : 0, c.stream()),
-             "ckks_key_switch");
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_keyswitch(c.handle(), (int)ct.getLevel(), ct.polyData(1), c.getSparseToDenseSwitchKeyDevice(),
+                              ct.polyData(0), ct.polyData(1), 0, 1, c.getParams().montgomery ? 1 : 0, c.stream()),
+                 "ckks_key_switch");
+    } else {
+        // b += KS_b(a), a = KS_a(a) of every member as one key-switch batch
+        std::vector<mi_keyswitch_item> items(ct.getBatchSize());
+        for (size_t m = 0; m < items.size(); ++m)
+            items[m] = mi_keyswitch_item{ct.polyData(1, m), c.getSparseToDenseSwitchKeyDevice(), ct.polyData(0, m),
+                                         ct.polyData(1, m),  (int32_t)ct.getLevel(), 0, 1, 0};
+        mi_check(mi_keyswitch_batch(c.handle(), items.data(), (int)items.size(), c.getParams().montgomery ? 1 : 0,
+                                    c.stream()),
+                 "ckks_key_switch");
+    }
     ct.setSecretOwner(CKKSSecretOwner::Dense);
 }
 
@@ -130,10 +142,12 @@ void ckks_modraise_inplace(const CKKSContext& c, CKKSCiphertext& ct, size_t targ
         throw std::invalid_argument("ckks_modraise: expected a sparse-secret ciphertext");
     if (target_level == 0 || target_level > P.getMaxLevel())
         throw std::invalid_argument("ckks_modraise: target level must be in [1, max_level]");
-    CKKSCiphertext out(c, 2, ct.getScale(), target_level, ct.getSecretOwner(), ct.getMessageEncodingState());
-    for (size_t i = 0; i < 2; ++i)
-        mi_check(mi_modraise(c.handle(), (int)target_level, ct.polyData(i), out.polyData(i), c.stream()),
-                 "ckks_modraise");
+    CKKSCiphertext out(c, 2, ct.getScale(), target_level, ct.getSecretOwner(), ct.getMessageEncodingState(),
+                       ct.getBatchSize());
+    for (size_t m = 0; m < ct.getBatchSize(); ++m)
+        for (size_t i = 0; i < 2; ++i)
+            mi_check(mi_modraise(c.handle(), (int)target_level, ct.polyData(i, m), out.polyData(i, m), c.stream()),
+                     "ckks_modraise");
     ct = std::move(out);
 }
 

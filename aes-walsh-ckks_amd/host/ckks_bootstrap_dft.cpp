@@ -471,12 +471,30 @@ CKKSCiphertext ckks_apply_linear_transform_owned_stage_qp(const CKKSContext& c, 
     std::vector<int64_t> didx(st.diagonal_indices.begin(), st.diagonal_indices.end());
     std::vector<const uint64_t*> dptr;
     for (const auto& d : st.diagonal_qp_plaintexts) dptr.push_back(d.data());
-    CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState());
-    mi_check(mi_linear_transform_qp(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, didx.data(),
-                                    dptr.data(), st.diagonal_qp_plaintexts_montgomery ? 1 : 0,
-                                    (int)st.baby_step_count, (int)slots, (int)rots.size(), rots.data(),
-                                    keys.data(), P.montgomery ? 1 : 0, out.polyData(1), out.polyData(0), c.stream()),
-             "ckks_linear_transform");
+    CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState(), ct.getBatchSize());
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_linear_transform_qp(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, didx.data(),
+                                        dptr.data(), st.diagonal_qp_plaintexts_montgomery ? 1 : 0,
+                                        (int)st.baby_step_count, (int)slots, (int)rots.size(), rots.data(),
+                                        keys.data(), P.montgomery ? 1 : 0, out.polyData(1), out.polyData(0),
+                                        c.stream()),
+                 "ckks_linear_transform");
+    } else {
+        const size_t B = ct.getBatchSize();
+        std::vector<const uint64_t*> ia(B), ib(B);
+        std::vector<uint64_t*> oa(B), ob(B);
+        for (size_t m = 0; m < B; ++m) {
+            ia[m] = ct.polyData(1, m);
+            ib[m] = ct.polyData(0, m);
+            oa[m] = out.polyData(1, m);
+            ob[m] = out.polyData(0, m);
+        }
+        mi_check(mi_linear_transform_qp_batch(c.handle(), (int)level, (int)B, ia.data(), ib.data(), (int)n,
+                                              didx.data(), dptr.data(), st.diagonal_qp_plaintexts_montgomery ? 1 : 0,
+                                              (int)st.baby_step_count, (int)slots, (int)rots.size(), rots.data(),
+                                              keys.data(), P.montgomery ? 1 : 0, oa.data(), ob.data(), c.stream()),
+                 "ckks_linear_transform");
+    }
     out.setScale((double)(ct.getScale() * (long double)st.plaintext_scale));
     if (st.rescale_after) ckks_rescale(c, out);
     out.setMessageEncodingState(st.output_encoding_state);
@@ -555,12 +573,29 @@ CKKSCiphertext apply_q_stage(const CKKSContext& c, const CKKSCiphertext& ct, con
             break;
         }
     const size_t slot_arg = indices.empty() ? giants * bs : slots;
-    CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState());
-    mi_check(mi_linear_transform_q(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, idx.data(),
-                                   diags.data(), diag_mont[0] ? 1 : 0, (int)bs, (int)slot_arg, (int)rots.size(),
-                                   rots.data(), keys.data(), P.montgomery ? 1 : 0, out.polyData(1), out.polyData(0),
-                                   c.stream()),
-             "ckks_linear_transform");
+    CKKSCiphertext out(c, 2, 0.0L, level, ct.getSecretOwner(), ct.getMessageEncodingState(), ct.getBatchSize());
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_linear_transform_q(c.handle(), (int)level, ct.polyData(1), ct.polyData(0), (int)n, idx.data(),
+                                       diags.data(), diag_mont[0] ? 1 : 0, (int)bs, (int)slot_arg, (int)rots.size(),
+                                       rots.data(), keys.data(), P.montgomery ? 1 : 0, out.polyData(1),
+                                       out.polyData(0), c.stream()),
+                 "ckks_linear_transform");
+    } else {
+        const size_t B = ct.getBatchSize();
+        std::vector<const uint64_t*> ia(B), ib(B);
+        std::vector<uint64_t*> oa(B), ob(B);
+        for (size_t m = 0; m < B; ++m) {
+            ia[m] = ct.polyData(1, m);
+            ib[m] = ct.polyData(0, m);
+            oa[m] = out.polyData(1, m);
+            ob[m] = out.polyData(0, m);
+        }
+        mi_check(mi_linear_transform_q_batch(c.handle(), (int)level, (int)B, ia.data(), ib.data(), (int)n, idx.data(),
+                                             diags.data(), diag_mont[0] ? 1 : 0, (int)bs, (int)slot_arg,
+                                             (int)rots.size(), rots.data(), keys.data(), P.montgomery ? 1 : 0,
+                                             oa.data(), ob.data(), c.stream()),
+                 "ckks_linear_transform");
+    }
     out.setScale((double)((long double)ct.getScale() * diag_scale[first]));
     if (rescale_after) ckks_rescale(c, out);
     out.setMessageEncodingState(out_state);

@@ -17,8 +17,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numbers>
 #include <set>
+
+#include <random>
 
 #include "ckks.hpp"
 #include "ckks_rng.hpp"
@@ -179,7 +182,12 @@ CKKSParams::CKKSParams(CKKSParamsConfig c) {
     secure = c.secure;
     fold_hybrid_keyswitch = c.fold_hybrid_keyswitch;
     montgomery = c.montgomery;
-    seed = c.seed;
+    if (c.seed) {
+        seed = *c.seed;
+    } else {
+        std::random_device rd;
+        seed = ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+    }
     // chain: q0, S2C groups (bottom to top), mult limbs, EvalMod limbs, C2S
     // groups (ckks_params.cpp:213-296)
     size_t expect_depth = mult_depth;
@@ -323,12 +331,13 @@ static void copy_d2d(const CKKSContext& c, uint64_t* dst, const uint64_t* src, s
 }
 
 CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, size_t num_polys, long double s, size_t lv,
-                               CKKSSecretOwner o, CKKSMessageEncodingState st)
+                               CKKSSecretOwner o, CKKSMessageEncodingState st, size_t b)
     : ctx(&context), N(context.getParams().getN()), scale(s), level(lv), secret_owner(o),
-      message_encoding_state(st) {
+      message_encoding_state(st), batch(b) {
     if (lv > context.getParams().getMaxLevel())
         throw std::invalid_argument("CKKSCiphertext: invalid level");
-    for (size_t i = 0; i < num_polys; ++i) polys.emplace_back(context.handle(), (lv + 1) * N);
+    if (b == 0) throw std::invalid_argument("CKKSCiphertext: batch size must be positive");
+    for (size_t i = 0; i < num_polys * b; ++i) polys.emplace_back(context.handle(), (lv + 1) * N);
 }
 
 CKKSCiphertext::CKKSCiphertext(const CKKSContext& context, MyVector<MyVector<uint64_t>> host,
@@ -342,7 +351,7 @@ CKKSCiphertext CKKSCiphertext::clone() const { return cloneAtLevel(level); }
 CKKSCiphertext CKKSCiphertext::cloneAtLevel(size_t lv) const {
     if (lv > level) throw std::invalid_argument("ckks_drop_to_level_inplace: target level above current level");
     const size_t level = lv;
-    CKKSCiphertext out(*ctx, polys.size(), scale, level, secret_owner, message_encoding_state);
+    CKKSCiphertext out(*ctx, getNumPolys(), scale, level, secret_owner, message_encoding_state, batch);
     if (polys.empty()) return out;
     std::vector<uint64_t*> dst(polys.size());
     std::vector<const uint64_t*> src(polys.size());
@@ -356,6 +365,44 @@ CKKSCiphertext CKKSCiphertext::cloneAtLevel(size_t lv) const {
     return out;
 }
 
+CKKSCiphertext CKKSCiphertext::stack(MyVector<CKKSCiphertext> members) {
+    if (members.empty()) throw std::invalid_argument("CKKSCiphertext::stack: no members");
+    const CKKSCiphertext& r = members.front();
+    size_t total = 0;
+    for (const auto& m : members) {
+        if (m.ctx != r.ctx || m.N != r.N || m.level != r.level || m.getNumPolys() != r.getNumPolys() ||
+            m.secret_owner != r.secret_owner || m.message_encoding_state != r.message_encoding_state ||
+            m.scale != r.scale)
+            throw std::invalid_argument("CKKSCiphertext::stack: members differ in context or metadata");
+        total += m.batch;
+    }
+    const size_t np = r.getNumPolys();
+    CKKSCiphertext out(*r.ctx, 0, r.scale, r.level, r.secret_owner, r.message_encoding_state, total);
+    out.polys.resize(np * total);
+    size_t base = 0;
+    for (auto& m : members) {
+        for (size_t k = 0; k < np; ++k)
+            for (size_t j = 0; j < m.batch; ++j) out.polys[k * total + base + j] = std::move(m.polys[k * m.batch + j]);
+        base += m.batch;
+        m.polys.clear();
+    }
+    return out;
+}
+
+MyVector<CKKSCiphertext> CKKSCiphertext::unstack() {
+    const size_t np = getNumPolys();
+    MyVector<CKKSCiphertext> out;
+    out.reserve(batch);
+    for (size_t m = 0; m < batch; ++m) {
+        out.emplace_back(*ctx, 0, scale, level, secret_owner, message_encoding_state, 1);
+        out.back().polys.resize(np);
+        for (size_t k = 0; k < np; ++k) out.back().polys[k] = std::move(polys[k * batch + m]);
+    }
+    polys.clear();
+    batch = 1;
+    return out;
+}
+
 void CKKSCiphertext::setLevel(size_t l) {
     for (auto& p : polys)
         if ((l + 1) * N > p.words()) throw std::invalid_argument("CKKSCiphertext: cannot raise level in place");
@@ -363,32 +410,57 @@ void CKKSCiphertext::setLevel(size_t l) {
 }
 
 MyVector<uint64_t> CKKSCiphertext::getPoly(size_t idx) const {
-    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
+    if (batch != 1) throw std::invalid_argument("CKKSCiphertext: getPoly of a batch needs a member index");
+    return getPoly(idx, 0);
+}
+
+MyVector<uint64_t> CKKSCiphertext::getPoly(size_t idx, size_t member) const {
     MyVector<uint64_t> h((level + 1) * N);
-    download(*ctx, h.data(), polys[idx].data(), h.size());
+    download(*ctx, h.data(), polyData(idx, member), h.size());
     return h;
 }
 
 void CKKSCiphertext::setPoly(size_t idx, const MyVector<uint64_t>& host) {
-    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
     if (host.size() != (level + 1) * N) throw std::invalid_argument("CKKSCiphertext: poly size mismatch");
-    upload(*ctx, polys[idx].data(), host.data(), host.size());
+    upload(*ctx, polyData(idx), host.data(), host.size());
 }
 
 uint64_t* CKKSCiphertext::polyData(size_t idx) {
-    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
-    return polys[idx].data();
+    if (batch != 1) throw std::invalid_argument("CKKSCiphertext: this operation has no batched form");
+    return polyData(idx, 0);
 }
 const uint64_t* CKKSCiphertext::polyData(size_t idx) const {
-    if (idx >= polys.size()) throw std::out_of_range("CKKSCiphertext: poly index");
-    return polys[idx].data();
+    if (batch != 1) throw std::invalid_argument("CKKSCiphertext: this operation has no batched form");
+    return polyData(idx, 0);
+}
+uint64_t* CKKSCiphertext::polyData(size_t idx, size_t member) {
+    if (idx >= getNumPolys() || member >= batch) throw std::out_of_range("CKKSCiphertext: poly index");
+    return polys[idx * batch + member].data();
+}
+const uint64_t* CKKSCiphertext::polyData(size_t idx, size_t member) const {
+    if (idx >= getNumPolys() || member >= batch) throw std::out_of_range("CKKSCiphertext: poly index");
+    return polys[idx * batch + member].data();
 }
 
-void CKKSCiphertext::addPoly() { polys.emplace_back(ctx->handle(), (level + 1) * N); }
+std::vector<uint64_t*> CKKSCiphertext::allPolyData() {
+    std::vector<uint64_t*> v(polys.size());
+    for (size_t i = 0; i < polys.size(); ++i) v[i] = polys[i].data();
+    return v;
+}
+std::vector<const uint64_t*> CKKSCiphertext::allPolyData() const {
+    std::vector<const uint64_t*> v(polys.size());
+    for (size_t i = 0; i < polys.size(); ++i) v[i] = polys[i].data();
+    return v;
+}
+
+void CKKSCiphertext::addPoly() {
+    for (size_t m = 0; m < batch; ++m) polys.emplace_back(ctx->handle(), (level + 1) * N);
+}
 
 void CKKSCiphertext::resizePolys(size_t count) {
-    while (polys.size() > count) polys.pop_back();
-    while (polys.size() < count) addPoly();
+    while (getNumPolys() > count)
+        for (size_t m = 0; m < batch; ++m) polys.pop_back();
+    while (getNumPolys() < count) addPoly();
 }
 
 CKKSEncoding::CKKSEncoding(const CKKSContext& context, MyVector<uint64_t> pt, long double s, size_t lv,
@@ -482,7 +554,7 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
                                     std::to_string(dnum) + ", P bits = " + std::to_string(p_bits) +
                                     ", widest Q digit bits = " + std::to_string(widest) +
                                     "; add P limbs or increase dnum");
-    key_digits = std::min(dnum, (nq + alpha - 1) / alpha);
+    K().key_digits = std::min(dnum, (nq + alpha - 1) / alpha);
 
     twiddle_ifft = twiddle_ifft_generation(N);
     twiddle_fft.resize(N);
@@ -500,21 +572,21 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
                      "ckks_context");
         }
         Rng rng(params.getSeed());
-        secret_coeffs = ckks_host::hamming_weight_ternary(N, params.getSecretKeyHammingWeight(), rng);
-        secret_qp = secret_qp_from_coeffs(secret_coeffs);
+        K().secret_coeffs = ckks_host::hamming_weight_ternary(N, params.getSecretKeyHammingWeight(), rng);
+        K().secret_qp = secret_qp_from_coeffs(K().secret_coeffs);
         const size_t KL = nq + pm.size();
         // relinearisation: s^2 -> s
         DeviceBuffer s2(mi, KL * N);
-        copy_d2d(*this, s2.data(), secret_qp.data(), KL * N);
-        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), secret_qp.data(), (int)KL, 0, 1, (int64_t)(KL * N), stream()),
+        copy_d2d(*this, s2.data(), K().secret_qp.data(), KL * N);
+        mi_check(mi_poly_op(mi, MI_OP_MUL, s2.data(), K().secret_qp.data(), (int)KL, 0, 1, (int64_t)(KL * N), stream()),
                  "ckks_context");
-        relin_key = generate_switch_key(s2.data(), secret_qp.data(), 1);
+        K().relin_key = generate_switch_key(s2.data(), K().secret_qp.data(), 1);
         // conjugation: conj(s) -> s
-        DeviceBuffer sc = secret_qp_from_coeffs(automorphism_coeffs(secret_coeffs, conjugation_automorphism_index(N)));
-        conj_key = generate_switch_key(sc.data(), secret_qp.data(), 2);
+        DeviceBuffer sc = secret_qp_from_coeffs(automorphism_coeffs(K().secret_coeffs, conjugation_automorphism_index(N)));
+        K().conj_key = generate_switch_key(sc.data(), K().secret_qp.data(), 2);
         for (int64_t r : params.getRotateValues()) {
             const int64_t n = normalize_rotation_value(N, r);
-            if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
+            if (n != 0 && !K().rotation_keys.count(n)) addRotationKey(n);
         }
         // append_aes_walsh_rotation_values (ckks_context.cpp:274-305):
         // +-segment * block_count for every provisioned segment
@@ -528,12 +600,12 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
             for (size_t s = 1; s < segments; ++s)
                 for (int64_t r : {(int64_t)(s * blocks), -(int64_t)(s * blocks)}) {
                     const int64_t n = normalize_rotation_value(N, r);
-                    if (n != 0 && !rotation_keys.count(n)) addRotationKey(n);
+                    if (n != 0 && !K().rotation_keys.count(n)) addRotationKey(n);
                 }
         }
         if (params.bts_keys) {
             build_bootstrap_material(rng);
-            evalmod_coeffs = default_evalmod_polynomial(params);
+            K().evalmod_coeffs = default_evalmod_polynomial(params);
         }
     } catch (...) {
         release_all();
@@ -543,18 +615,53 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
     }
 }
 
+CKKSContext::KeyMaterial& CKKSContext::K() {
+    if (owner_) throw std::runtime_error("ckks_context: a lane shares its owner's keys; change them on the owner");
+    return keys_;
+}
+
+// A lane: the owner's parameters, its own device context (tables, pool,
+// workspaces) and stream; keys, plans and polynomial read from the owner.
+CKKSContext::CKKSContext(const CKKSContext& owner, int)
+    : params(owner.params), owner_(&owner.keyOwner()), twiddle_fft(owner.twiddle_fft),
+      twiddle_ifft(owner.twiddle_ifft), perm_table(owner.perm_table) {
+    const auto& q = params.getModuli();
+    const auto& pm = params.getPModuli();
+    // the owner's keys and plans are complete before another stream reads them
+    mi_check(mi_stream_synchronize(owner.handle(), owner.stream()), "ckks_context");
+    mi_check(mi_context_create(&mi, params.getN(), q.data(), (int)q.size(), pm.data(), (int)pm.size(),
+                               (int)params.getDnum()),
+             "ckks_context");
+    try {
+        mi_check(mi_stream_create(mi, &stream_), "ckks_context");
+        std::vector<uint64_t> perm(perm_table.begin(), perm_table.end());
+        mi_check(mi_set_encoding_tables(mi, reinterpret_cast<const double*>(twiddle_fft.data()), perm.data()),
+                 "ckks_context");
+    } catch (...) {
+        if (stream_) mi_stream_destroy(mi, stream_);
+        mi_context_destroy(mi);
+        throw;
+    }
+    owner_->lanes_.fetch_add(1);
+}
+
+std::unique_ptr<CKKSContext> CKKSContext::makeLane(const CKKSContext& owner) {
+    return std::unique_ptr<CKKSContext>(new CKKSContext(owner, 0));
+}
+
 void CKKSContext::release_all() {
     enc_cache.clear();
     scaled_s2c.clear();
-    s2c_plan.reset();
-    c2s_plan.reset();
-    d2s_key.reset();
-    s2d_key.reset();
-    low_secret_qp.reset();
-    relin_key.reset();
-    conj_key.reset();
-    rotation_keys.clear();
-    secret_qp.reset();
+    if (owner_) return;  // a lane holds no keys
+    K().s2c_plan.reset();
+    K().c2s_plan.reset();
+    K().d2s_key.reset();
+    K().s2d_key.reset();
+    K().low_secret_qp.reset();
+    K().relin_key.reset();
+    K().conj_key.reset();
+    K().rotation_keys.clear();
+    K().secret_qp.reset();
 }
 
 // Sparse secret and its switching keys, the S2C / C2S DFT plans and their
@@ -563,10 +670,10 @@ void CKKSContext::build_bootstrap_material(Rng& rng) {
     const size_t N = params.getN(), nq = params.getModuli().size();
     const auto& b = params.getBootstrapParams();
     const auto& q = params.getModuli();
-    low_secret_coeffs = ckks_host::hamming_weight_ternary(N, b.sparse_secret_hamming_weight, rng);
-    low_secret_qp = secret_qp_from_coeffs(low_secret_coeffs);
-    d2s_key = generate_q0p0_key(secret_qp.data(), low_secret_qp.data(), 3);
-    s2d_key = generate_switch_key(low_secret_qp.data(), secret_qp.data(), 4);
+    K().low_secret_coeffs = ckks_host::hamming_weight_ternary(N, b.sparse_secret_hamming_weight, rng);
+    K().low_secret_qp = secret_qp_from_coeffs(K().low_secret_coeffs);
+    K().d2s_key = generate_q0p0_key(K().secret_qp.data(), K().low_secret_qp.data(), 3);
+    K().s2d_key = generate_switch_key(K().low_secret_qp.data(), K().secret_qp.data(), 4);
 
     const size_t s2c_depth = params.getBootstrapS2CDepth(), c2s_depth = params.getBootstrapC2SDepth();
     const size_t active = params.getBootstrapActiveSlotCount();
@@ -583,14 +690,14 @@ void CKKSContext::build_bootstrap_material(Rng& rng) {
         lit.transform_scale = ((double)q[0] * 0.5) / params.getScale();
         lit.plaintext_scale = (double)q[s2c_depth];
         lit.rescale_after_each_stage = true;
-        s2c_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
+        K().s2c_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
     }
     if (c2s_depth > 0) {
         const size_t level = params.getDepth();
         if (c2s_depth > level) throw std::runtime_error("ckks_context: EvalMod C2S depth exceeds bootstrap depth");
-        const double K = (double)params.getBootstrapEvalModK();
+        const double evalmod_k = (double)params.getBootstrapEvalModK();
         const int shift = (int)params.getEvalModLogScaleBits() - (int)params.getQ0Bit();
-        c2s_transform_scale = std::ldexp(params.getScale(), shift) / (double)q[0] / K;
+        K().c2s_transform_scale = std::ldexp(params.getScale(), shift) / (double)q[0] / evalmod_k;
         CKKSBootstrapDftMatrixLiteral lit;
         lit.type = CKKSBootstrapDftType::CoeffsToSlots;
         lit.level = level;
@@ -600,10 +707,10 @@ void CKKSContext::build_bootstrap_material(Rng& rng) {
         lit.log_plaintext_scale_groups = b.linear_transform_log_plaintext_scales[1];
         lit.merge_depths = b.linear_transform_merge_depths[1];
         lit.active_slots = active;
-        lit.transform_scale = c2s_transform_scale / (double)(params.getSlots() / active);
+        lit.transform_scale = K().c2s_transform_scale / (double)(params.getSlots() / active);
         lit.plaintext_scale = (double)q[level] / params.getScale();
         lit.rescale_after_each_stage = true;
-        c2s_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
+        K().c2s_plan = std::make_unique<CKKSLinearTransformPlan>(ckks_generate_bootstrap_dft_plan(*this, lit));
     }
     auto add_plan_rotations = [&](const CKKSLinearTransformPlan* plan) {
         if (!plan) return;
@@ -614,8 +721,8 @@ void CKKSContext::build_bootstrap_material(Rng& rng) {
                 if (d - baby) addRotationKey((int64_t)(d - baby));
             }
     };
-    add_plan_rotations(s2c_plan.get());
-    add_plan_rotations(c2s_plan.get());
+    add_plan_rotations(K().s2c_plan.get());
+    add_plan_rotations(K().c2s_plan.get());
     if (c2s_depth > 0)
         for (size_t r = active; r < params.getSlots(); r <<= 1) addRotationKey((int64_t)r);
     (void)nq;
@@ -677,21 +784,21 @@ DeviceBuffer CKKSContext::generate_q0p0_key(const uint64_t* src_qp, const uint64
 }
 
 const uint64_t* CKKSContext::getDenseToSparseSwitchKeyDevice() const {
-    if (!d2s_key.data()) throw std::runtime_error("ckks_context: dense-to-sparse switch key is unavailable");
-    return d2s_key.data();
+    if (!K().d2s_key.data()) throw std::runtime_error("ckks_context: dense-to-sparse switch key is unavailable");
+    return K().d2s_key.data();
 }
 const uint64_t* CKKSContext::getSparseToDenseSwitchKeyDevice() const {
-    if (!s2d_key.data()) throw std::runtime_error("ckks_context: sparse-to-dense switch key is unavailable");
-    return s2d_key.data();
+    if (!K().s2d_key.data()) throw std::runtime_error("ckks_context: sparse-to-dense switch key is unavailable");
+    return K().s2d_key.data();
 }
 void CKKSContext::setDenseToSparseSwitchKey(const MyVector<uint64_t>& key) {
     const size_t N = params.getN();
     if (key.size() != 4 * N) throw std::invalid_argument("ckks_context: dense-to-sparse key size mismatch");
     DeviceBuffer d(mi, key.size());
     upload(*this, d.data(), key.data(), key.size());
-    d2s_key = std::move(d);
+    K().d2s_key = std::move(d);
 }
-void CKKSContext::setSparseToDenseSwitchKey(const MyVector<uint64_t>& key) { s2d_key = upload_key(key); }
+void CKKSContext::setSparseToDenseSwitchKey(const MyVector<uint64_t>& key) { K().s2d_key = upload_key(key); }
 MyVector<uint64_t> CKKSContext::downloadDenseToSparseSwitchKey() const {
     MyVector<uint64_t> h(4 * params.getN());
     download(*this, h.data(), getDenseToSparseSwitchKeyDevice(), h.size());
@@ -699,12 +806,12 @@ MyVector<uint64_t> CKKSContext::downloadDenseToSparseSwitchKey() const {
 }
 
 const CKKSLinearTransformPlan& CKKSContext::getEvalModS2CPlan() const {
-    if (!s2c_plan) throw std::runtime_error("ckks_context: EvalMod S2C plan is unavailable");
-    return *s2c_plan;
+    if (!K().s2c_plan) throw std::runtime_error("ckks_context: EvalMod S2C plan is unavailable");
+    return *K().s2c_plan;
 }
 const CKKSLinearTransformPlan& CKKSContext::getEvalModScaledC2SPlan() const {
-    if (!c2s_plan) throw std::runtime_error("ckks_context: EvalMod C2S plan is unavailable");
-    return *c2s_plan;
+    if (!K().c2s_plan) throw std::runtime_error("ckks_context: EvalMod C2S plan is unavailable");
+    return *K().c2s_plan;
 }
 
 const CKKSLinearTransformPlan& CKKSContext::getScaledS2CPlan(uint64_t ratio) const {
@@ -775,9 +882,175 @@ CKKSParamsConfig aes_walsh15_config() {
 
 CKKSParams CKKSParams::AESWalsh15() { return CKKSParams(aes_walsh15_config()); }
 
-// The degree-d Chebyshev interpolant (points cos(pi (j + 1/2) / (d + 1)))
-// of 0.5 - 0.5 cos(2 pi K u) on [-1, 1], odd terms zero (the function is even).
+namespace {
+
+// make_evalmod_bootstrap_params (ckks_params.hpp:140-156) with the XBOOT
+// Mod1 settings (mod1.CosDiscrete, no double angle, LogMessageRatio 1,
+// discrete interval radius just below 1/4, 64 validation points, no pruning).
+CKKSBootstrapParams xboot_bootstrap_params(size_t h, CKKSBootstrapDftScaleSchedule s2c,
+                                           CKKSBootstrapDftScaleSchedule c2s, size_t evalmod_depth,
+                                           size_t evalmod_bits, size_t K, size_t degree,
+                                           std::array<int, 2> log_bsgs_ratios) {
+    CKKSBootstrapParams b;
+    b.sparse_secret_hamming_weight = h;
+    b.linear_transform_depths = {s2c.size(), c2s.size()};
+    b.linear_transform_log_plaintext_scales = {std::move(s2c), std::move(c2s)};
+    b.evalmod_depth = evalmod_depth;
+    b.evalmod_bit_size = evalmod_bits;
+    b.evalmod_k = K;
+    b.evalmod_polynomial_degree = degree;
+    b.evalmod_double_angle = 0;
+    b.evalmod_log_message_ratio = 1;
+    b.evalmod_coefficient_prune_threshold = 0.0;
+    b.evalmod_discrete_interval_radius = std::nextafter(0.25, 0.0);
+    b.evalmod_discrete_validation_points_per_interval = 64;
+    b.linear_transform_log_bsgs_ratios = log_bsgs_ratios;
+    b.linear_transform_use_qp = {true, true};
+    return b;
+}
+
+}  // namespace
+
+// XBootParamAes13ReferenceConfig (ckks_aes_params.hpp:12-33, 83-109): XBOOT
+// Table 2 Param-AES-13 at N = 2^14 (the reference marks it not secure under
+// its fixed-h bound), dnum 13 over one 32-bit special prime.
+CKKSParamsConfig xboot_aes13_reference_config() {
+    CKKSParamsConfig c;
+    c.secure = false;
+    c.bts_keys = true;
+    c.N = size_t{1} << 14;
+    c.mult_depth = 3;
+    c.q0_bit = 32;
+    c.q_bit = 32;
+    c.q_bits = {32, 28, 31, 31, 31, 32, 32, 32, 32, 32, 32, 28, 28};
+    c.scale = std::exp2(31.0);
+    c.dnum = 13;
+    c.pmoduli_count = 1;
+    c.p_bits = {32};
+    c.secret_key_hamming_weight = 256;
+    c.bootstrap_params = xboot_bootstrap_params(16, CKKSBootstrapDftScaleSchedule{{28}},
+                                                CKKSBootstrapDftScaleSchedule{{28}, {28}}, 6, 32, 7, 60, {1, 2});
+    return c;
+}
+
+// AESXBoot14SecureConfig (ckks_aes_params.hpp:111-148): every Q limb one bit
+// smaller than Param-AES-13, secure.
+CKKSParamsConfig aes_xboot14_secure_config() {
+    CKKSParamsConfig c = xboot_aes13_reference_config();
+    c.secure = true;
+    c.q0_bit = 31;
+    c.q_bit = 31;
+    c.q_bits = {31, 27, 30, 30, 30, 31, 31, 31, 31, 31, 31, 27, 27};
+    c.scale = std::exp2(30.0);
+    c.bootstrap_params = xboot_bootstrap_params(16, CKKSBootstrapDftScaleSchedule{{27}},
+                                                CKKSBootstrapDftScaleSchedule{{27}, {27}}, 6, 31, 7, 60, {1, 2});
+    return c;
+}
+
+// AESXBoot15Config (ckks_aes_params.hpp:156-203): N = 2^15, dnum 5 over five
+// 38-bit special primes, EvalMod degree 120 with K = 12.
+CKKSParamsConfig aes_xboot15_config() {
+    CKKSParamsConfig c;
+    c.secure = true;
+    c.bts_keys = true;
+    c.N = size_t{1} << 15;
+    c.mult_depth = 3;
+    c.q0_bit = 38;
+    c.q_bit = 38;
+    c.q_bits = {38, 36, 36, 37, 37, 37, 38, 38, 38, 38, 38, 38, 38, 38, 38, 38, 38};
+    c.scale = std::exp2(37.0);
+    c.dnum = 5;
+    c.pmoduli_count = 5;
+    c.p_bits = {38, 38, 38, 38, 38};
+    c.secret_key_hamming_weight = 256;
+    c.bootstrap_params =
+        xboot_bootstrap_params(32, CKKSBootstrapDftScaleSchedule{{36}, {36}},
+                               CKKSBootstrapDftScaleSchedule{{38}, {38}, {38}, {38}}, 7, 38, 12, 120, {2, 1});
+    return c;
+}
+
+// The Sparse1024 variants (ckks_aes_params.hpp:150-154, 205-209): 1024
+// active bootstrap slots.
+static CKKSParamsConfig sparse1024(CKKSParamsConfig c) {
+    c.bootstrap_params->active_slot_count = 1024;
+    return c;
+}
+
+CKKSParams CKKSParams::XBootParamAes13Reference() { return CKKSParams(xboot_aes13_reference_config()); }
+CKKSParams CKKSParams::AESXBoot14Secure() { return CKKSParams(aes_xboot14_secure_config()); }
+CKKSParams CKKSParams::AESXBoot14SecureSparse1024() { return CKKSParams(sparse1024(aes_xboot14_secure_config())); }
+CKKSParams CKKSParams::AESXBoot15() { return CKKSParams(aes_xboot15_config()); }
+CKKSParams CKKSParams::AESXBoot15Sparse1024() { return CKKSParams(sparse1024(aes_xboot15_config())); }
+
+namespace {
+
+// The reference's EvalMod fit for the parameters of its presets
+// (ckks_evalmod_tables.inc, made by evalmod_fit.py at 256-bit precision).
+struct EvalModFit {
+    double K;
+    size_t degree, double_angle;
+    double interval_radius;
+    size_t validation_points;
+    double prune_threshold, interval_min, interval_max, max_error;
+    std::vector<double> coeffs;
+};
+
+const std::vector<EvalModFit>& evalmod_fits() {
+    static const std::vector<EvalModFit> fits = {
+#include "ckks_evalmod_tables.inc"
+    };
+    return fits;
+}
+
+// automatic_binboot_discrete_interval_radius (ckks_params.hpp:34-45) behind
+// getBootstrapEvalModDiscreteIntervalRadius (ckks_params.cpp:403-414)
+double evalmod_interval_radius(const CKKSParams& P) {
+    const auto& b = P.getBootstrapParams();
+    const double r = b.evalmod_discrete_interval_radius;
+    if (!std::isfinite(r) || r < 0.0 || r >= 0.25)
+        throw std::runtime_error("CKKSParams: EvalMod discrete interval radius must be in [0, 0.25)");
+    if (r > 0.0) return r;
+    const size_t lm = b.evalmod_log_message_ratio;
+    if (lm == 0) return 1.0 / 32.0;
+    if (lm >= (size_t)std::numeric_limits<double>::max_exponent) return std::numeric_limits<double>::min();
+    return std::min(std::ldexp(1.0, -(int)lm), std::nextafter(0.25, 0.0));
+}
+
+// getBootstrapEvalModDiscreteValidationPointsPerInterval (ckks_params.cpp:416-427)
+size_t evalmod_validation_points(const CKKSParams& P) {
+    const size_t n = P.getBootstrapParams().evalmod_discrete_validation_points_per_interval;
+    if (n == 0) return 64;
+    if (n < 2) throw std::runtime_error("CKKSParams: EvalMod discrete validation points per interval must be at least 2");
+    return n;
+}
+
+const EvalModFit* find_reference_fit(const CKKSParams& P) {
+    const auto& b = P.getBootstrapParams();
+    const double K = (double)P.getBootstrapEvalModK();
+    const double radius = evalmod_interval_radius(P);
+    const size_t points = evalmod_validation_points(P);
+    for (const auto& f : evalmod_fits())
+        if (f.K == K && f.degree == b.evalmod_polynomial_degree && f.double_angle == b.evalmod_double_angle &&
+            f.interval_radius == radius && f.validation_points == points &&
+            f.prune_threshold == b.evalmod_coefficient_prune_threshold)
+            return &f;
+    return nullptr;
+}
+
+}  // namespace
+
+bool evalmod_polynomial_is_reference_fit(const CKKSParams& params) { return find_reference_fit(params) != nullptr; }
+
+// The reference's fit (binboot_han_ki_discrete_multi_interval_approximate,
+// poly_approx.cpp:565-681) where its parameters are those of a preset; for
+// other parameters without double-angle steps, the degree-d Chebyshev
+// interpolant (points cos(pi (j + 1/2) / (d + 1))) of 0.5 - 0.5 cos(2 pi K u)
+// on [-1, 1], odd terms zero; with double-angle steps there is no built-in
+// polynomial (empty: evalmod_fit.py computes the reference's fit for any
+// parameters, installed with CKKSContext::setEvalModPolynomial).
 std::vector<double> default_evalmod_polynomial(const CKKSParams& params) {
+    if (const EvalModFit* f = find_reference_fit(params)) return f->coeffs;
+    if (params.getBootstrapEvalModDoubleAngle() != 0) return {};
     const size_t d = params.getBootstrapEvalModPolynomialDegree(), n = d + 1;
     const long double K = (long double)params.getBootstrapEvalModK();
     const long double pi = 3.141592653589793238462643383279502884L;
@@ -797,17 +1070,20 @@ std::vector<double> default_evalmod_polynomial(const CKKSParams& params) {
 
 void CKKSContext::setEvalModPolynomial(const std::vector<double>& c) {
     if (c.empty()) throw std::invalid_argument("ckks_context: empty EvalMod polynomial");
-    evalmod_coeffs = c;
+    K().evalmod_coeffs = c;
 }
 
 const std::vector<double>& CKKSContext::getEvalModPolynomial() const {
-    if (evalmod_coeffs.empty())
-        throw std::runtime_error("ckks_context: EvalMod polynomial is unavailable (set it with setEvalModPolynomial)");
-    return evalmod_coeffs;
+    if (K().evalmod_coeffs.empty())
+        throw std::runtime_error(
+            "ckks_context: EvalMod polynomial is unavailable: no built-in fit for these EvalMod parameters "
+            "(double-angle steps); set the reference's fit with setEvalModPolynomial");
+    return K().evalmod_coeffs;
 }
 
 CKKSContext::~CKKSContext() {
     release_all();
+    if (owner_) owner_->lanes_.fetch_sub(1);
     if (mi && stream_) mi_stream_destroy(mi, stream_);
     if (mi) mi_context_destroy(mi);
 }
@@ -827,7 +1103,7 @@ DeviceBuffer CKKSContext::secret_qp_from_coeffs(const MyVector<int8_t>& coeffs) 
 }
 
 size_t CKKSContext::switchKeyWords() const {
-    return key_digits * 2 * (params.getModuli().size() + params.getPModuli().size()) * params.getN();
+    return K().key_digits * 2 * (params.getModuli().size() + params.getPModuli().size()) * params.getN();
 }
 
 DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_t* tgt, uint64_t stream_id) const {
@@ -864,7 +1140,7 @@ DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_
     DeviceBuffer tmp(mi, KL * N);
     std::vector<uint64_t> h(KL * N);
     const int64_t W = (int64_t)(KL * N);
-    for (size_t d = 0; d < key_digits; ++d) {
+    for (size_t d = 0; d < K().key_digits; ++d) {
         uint64_t* b = key.data() + (2 * d) * KL * N;
         uint64_t* a = b + KL * N;
         copy_d2d(*this, b, src, KL * N);
@@ -894,10 +1170,10 @@ DeviceBuffer CKKSContext::generate_switch_key(const uint64_t* src, const uint64_
 void CKKSContext::addRotationKey(int64_t rotation) {
     const size_t N = params.getN();
     const int64_t r = normalize_rotation_value(N, rotation);
-    if (r == 0 || rotation_keys.count(r)) return;
+    if (r == 0 || K().rotation_keys.count(r)) return;
     // keys switch s to pi^-1(s); the rotation applies pi after the switch
-    DeviceBuffer tgt = secret_qp_from_coeffs(automorphism_coeffs(secret_coeffs, rotation_automorphism_index(N, -r)));
-    rotation_keys.emplace(r, generate_switch_key(secret_qp.data(), tgt.data(), 1000 + (uint64_t)r));
+    DeviceBuffer tgt = secret_qp_from_coeffs(automorphism_coeffs(K().secret_coeffs, rotation_automorphism_index(N, -r)));
+    K().rotation_keys.emplace(r, generate_switch_key(K().secret_qp.data(), tgt.data(), 1000 + (uint64_t)r));
 }
 
 DeviceBuffer CKKSContext::upload_key(const MyVector<uint64_t>& key) const {
@@ -909,45 +1185,45 @@ DeviceBuffer CKKSContext::upload_key(const MyVector<uint64_t>& key) const {
     return d;
 }
 
-void CKKSContext::setRelinKey(const MyVector<uint64_t>& key) { relin_key = upload_key(key); }
-void CKKSContext::setConjugationKey(const MyVector<uint64_t>& key) { conj_key = upload_key(key); }
+void CKKSContext::setRelinKey(const MyVector<uint64_t>& key) { K().relin_key = upload_key(key); }
+void CKKSContext::setConjugationKey(const MyVector<uint64_t>& key) { K().conj_key = upload_key(key); }
 void CKKSContext::setRotationKey(int64_t rotation, const MyVector<uint64_t>& key) {
     const int64_t r = normalize_rotation_value(params.getN(), rotation);
     if (r == 0) throw std::invalid_argument("ckks_context: rotation by 0 needs no key");
-    rotation_keys[r] = upload_key(key);
+    K().rotation_keys[r] = upload_key(key);
 }
 void CKKSContext::setSecretKeyEval(const MyVector<uint64_t>& q_limbs) {
     const size_t N = params.getN(), nq = params.getModuli().size();
     if (q_limbs.size() != nq * N) throw std::invalid_argument("ckks_context: secret key size mismatch");
-    upload(*this, secret_qp.data(), q_limbs.data(), q_limbs.size());
+    upload(*this, K().secret_qp.data(), q_limbs.data(), q_limbs.size());
 }
 
 bool CKKSContext::hasRotationKey(int64_t rotation) const {
-    return rotation_keys.count(normalize_rotation_value(params.getN(), rotation)) != 0;
+    return K().rotation_keys.count(normalize_rotation_value(params.getN(), rotation)) != 0;
 }
 
 const uint64_t* CKKSContext::getRotationKeyDevice(int64_t rotation) const {
-    auto it = rotation_keys.find(normalize_rotation_value(params.getN(), rotation));
-    if (it == rotation_keys.end())
+    auto it = K().rotation_keys.find(normalize_rotation_value(params.getN(), rotation));
+    if (it == K().rotation_keys.end())
         throw std::runtime_error("ckks_context: rotation key is unavailable for rotation " + std::to_string(rotation));
     return it->second.data();
 }
 
 std::vector<int64_t> CKKSContext::getRotationKeyValues() const {
     std::vector<int64_t> v;
-    for (auto& kv : rotation_keys) v.push_back(kv.first);
+    for (auto& kv : K().rotation_keys) v.push_back(kv.first);
     return v;
 }
 
 const uint64_t* CKKSContext::getConjugationKeyDevice() const {
-    if (!conj_key.data()) throw std::runtime_error("ckks_context: conjugation key is unavailable");
-    return conj_key.data();
+    if (!K().conj_key.data()) throw std::runtime_error("ckks_context: conjugation key is unavailable");
+    return K().conj_key.data();
 }
 
 MyVector<MyVector<uint64_t>> CKKSContext::getSecretKey() const {
     const size_t N = params.getN(), nq = params.getModuli().size();
     MyVector<uint64_t> all(nq * N);
-    download(*this, all.data(), secret_qp.data(), all.size());
+    download(*this, all.data(), K().secret_qp.data(), all.size());
     MyVector<MyVector<uint64_t>> out(nq);
     for (size_t l = 0; l < nq; ++l) out[l].assign(all.begin() + l * N, all.begin() + (l + 1) * N);
     return out;
@@ -961,6 +1237,6 @@ MyVector<uint64_t> CKKSContext::downloadKey(const uint64_t* key) const {
 
 uint64_t CKKSContext::nextRandomnessSeed() const {
     uint64_t st = params.getSeed() ^ 0x3C6EF372FE94F82Bull;
-    st += 0x9E3779B97F4A7C15ull * (randomness_counter.fetch_add(1) + 1);
+    st += 0x9E3779B97F4A7C15ull * (K().randomness_counter.fetch_add(1) + 1);
     return ckks_host::splitmix64(st);
 }

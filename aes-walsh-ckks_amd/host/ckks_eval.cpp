@@ -16,8 +16,6 @@ void check_ct(const CKKSContext& c, const CKKSCiphertext& ct, const char* fn) {
     if (&ct.getContext() != &c) throw std::invalid_argument(std::string(fn) + ": ciphertext belongs to another context");
 }
 
-int64_t words(const CKKSContext& c, size_t level) { return (int64_t)((level + 1) * c.getParams().getN()); }
-
 uint64_t reduce_i64(int64_t v, uint64_t q) {
     const int64_t qi = (int64_t)q;
     const int64_t r = v % qi;
@@ -45,8 +43,7 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncodi
         throw std::invalid_argument("ckks_mul_inplace: ciphertext/plaintext size mismatch");
     if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_mul_inplace: empty ciphertext");
     const size_t lv = ct.getLevel();
-    std::vector<uint64_t*> polys(ct.getNumPolys());
-    for (size_t i = 0; i < polys.size(); ++i) polys[i] = ct.polyData(i);
+    std::vector<uint64_t*> polys = ct.allPolyData();
     mi_check(mi_mul_plain_rescale_list(c.handle(), (int)lv, (int)polys.size(), polys.data(), pt.data(),
                                        pt.isMontgomeryForm() ? 1 : 0, 0, c.stream()),
              "ckks_mul_inplace");
@@ -63,10 +60,28 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphe
     if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
         ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
         throw std::invalid_argument("ckks_mul_inplace(ct,ct): ciphertext metadata mismatch");
+    if (ct1.getBatchSize() != ct2.getBatchSize())
+        throw std::invalid_argument("ckks_mul_inplace(ct,ct): batch size mismatch");
     ct1.addPoly();
-    mi_check(mi_tensor(c.handle(), (int)lv, ct1.polyData(0), ct1.polyData(1), ct2.polyData(0), ct2.polyData(1),
-                       ct1.polyData(2), c.stream()),
-             "ckks_mul_inplace(ct,ct)");
+    if (ct1.getBatchSize() == 1) {
+        mi_check(mi_tensor(c.handle(), (int)lv, ct1.polyData(0), ct1.polyData(1), ct2.polyData(0), ct2.polyData(1),
+                           ct1.polyData(2), c.stream()),
+                 "ckks_mul_inplace(ct,ct)");
+    } else {
+        const size_t B = ct1.getBatchSize();
+        std::vector<uint64_t*> b1(B), a1(B), c2(B);
+        std::vector<const uint64_t*> b2(B), a2(B);
+        for (size_t m = 0; m < B; ++m) {
+            b1[m] = ct1.polyData(0, m);
+            a1[m] = ct1.polyData(1, m);
+            c2[m] = ct1.polyData(2, m);
+            b2[m] = ct2.polyData(0, m);
+            a2[m] = ct2.polyData(1, m);
+        }
+        mi_check(mi_tensor_list(c.handle(), (int)lv, (int)B, b1.data(), a1.data(), b2.data(), a2.data(), c2.data(),
+                                c.stream()),
+                 "ckks_mul_inplace(ct,ct)");
+    }
     ct1.setScale(ct1.getScale() * ct2.getScale());
 }
 
@@ -74,8 +89,22 @@ void ckks_square_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
     check_ct(c, ct, "ckks_square_inplace");
     if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_square_inplace: expected 2-component input");
     ct.addPoly();
-    mi_check(mi_square(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2), c.stream()),
-             "ckks_square_inplace");
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_square(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2),
+                           c.stream()),
+                 "ckks_square_inplace");
+    } else {
+        const size_t B = ct.getBatchSize();
+        std::vector<uint64_t*> b(B), a(B), c2(B);
+        for (size_t m = 0; m < B; ++m) {
+            b[m] = ct.polyData(0, m);
+            a[m] = ct.polyData(1, m);
+            c2[m] = ct.polyData(2, m);
+        }
+        mi_check(mi_tensor_list(c.handle(), (int)ct.getLevel(), (int)B, b.data(), a.data(), nullptr, nullptr,
+                                c2.data(), c.stream()),
+                 "ckks_square_inplace");
+    }
     ct.setScale(ct.getScale() * ct.getScale());
 }
 
@@ -116,8 +145,7 @@ void ckks_rescale(const CKKSContext& c, CKKSCiphertext& ct) {
     if (lv == 0 || lv > c.getParams().getMaxLevel()) throw std::invalid_argument("ckks_rescale: invalid level");
     check_ct(c, ct, "ckks_rescale");
     if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_rescale: empty ciphertext");
-    std::vector<uint64_t*> polys(ct.getNumPolys());
-    for (size_t i = 0; i < ct.getNumPolys(); ++i) polys[i] = ct.polyData(i);
+    std::vector<uint64_t*> polys = ct.allPolyData();
     mi_check(mi_rescale_list(c.handle(), (int)lv, (int)polys.size(), polys.data(), c.stream()), "ckks_rescale");
     const uint64_t p = c.getParams().getModuli()[lv];
     ct.setScale(ct.getScale() / (long double)p);
@@ -135,8 +163,19 @@ void ckks_add_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncodi
     check_ct(c, ct, "ckks_add_inplace(ct,pt)");
     if (pt.getN() != ct.getN() || pt.getLevel() != ct.getLevel())
         throw std::invalid_argument("ckks_add_inplace(ct,pt): ciphertext/plaintext size mismatch");
-    mi_check(mi_add_plain(c.handle(), (int)ct.getLevel(), ct.polyData(0), pt.data(), pt.isMontgomeryForm() ? 1 : 0,
-                          c.stream()),
+    if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_add_inplace(ct,pt): empty ciphertext");
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_add_plain(c.handle(), (int)ct.getLevel(), ct.polyData(0), pt.data(),
+                              pt.isMontgomeryForm() ? 1 : 0, c.stream()),
+                 "ckks_add_inplace(ct,pt)");
+        return;
+    }
+    const size_t B = ct.getBatchSize();
+    std::vector<uint64_t*> x(B);
+    std::vector<const uint64_t*> y(B, pt.data());
+    for (size_t m = 0; m < B; ++m) x[m] = ct.polyData(0, m);
+    mi_check(mi_poly_list_op(c.handle(), pt.isMontgomeryForm() ? MI_LIST_ADD_PLAIN_MONT : MI_LIST_ADD,
+                             (int)ct.getLevel(), (int)B, x.data(), y.data(), nullptr, c.stream()),
              "ckks_add_inplace(ct,pt)");
 }
 
@@ -148,20 +187,40 @@ static void add_or_sub(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiph
     if (ct1.getSecretOwner() != ct2.getSecretOwner() ||
         ct1.getMessageEncodingState() != ct2.getMessageEncodingState())
         throw std::invalid_argument(std::string(fn) + ": ciphertext metadata mismatch");
+    if (ct1.getBatchSize() != ct2.getBatchSize()) throw std::invalid_argument(std::string(fn) + ": batch size mismatch");
     ckks_detail::poly_list_op(c, sub ? MI_LIST_SUB : MI_LIST_ADD, ct1, &ct2, nullptr, fn);
 }
 
 void ckks_detail::poly_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const CKKSCiphertext* other,
                                const uint64_t* residues, const char* fn) {
-    const size_t n = ct.getNumPolys();
-    if (n == 0) return;
-    std::vector<uint64_t*> x(n);
-    std::vector<const uint64_t*> y(n, nullptr);
-    for (size_t i = 0; i < n; ++i) {
-        x[i] = ct.polyData(i);
-        if (other) y[i] = other->polyData(i);
-    }
-    mi_check(mi_poly_list_op(c.handle(), op, (int)ct.getLevel(), (int)n, x.data(), y.data(), residues, c.stream()),
+    if (ct.getNumPolys() == 0) return;
+    if (other && (other->getBatchSize() != ct.getBatchSize() || other->getNumPolys() < ct.getNumPolys()))
+        throw std::invalid_argument(std::string(fn) + ": ciphertext batch or degree mismatch");
+    std::vector<uint64_t*> x = ct.allPolyData();
+    std::vector<const uint64_t*> y(x.size(), nullptr);
+    if (other) y = other->allPolyData();
+    mi_check(mi_poly_list_op(c.handle(), op, (int)ct.getLevel(), (int)x.size(), x.data(), y.data(), residues,
+                             c.stream()),
+             fn);
+}
+
+void ckks_detail::b_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const uint64_t* residues,
+                            const char* fn) {
+    if (ct.getNumPolys() == 0) throw std::invalid_argument(std::string(fn) + ": empty ciphertext");
+    std::vector<uint64_t*> x(ct.getBatchSize());
+    for (size_t m = 0; m < x.size(); ++m) x[m] = ct.polyData(0, m);
+    mi_check(mi_poly_list_op(c.handle(), op, (int)ct.getLevel(), (int)x.size(), x.data(), nullptr, residues,
+                             c.stream()),
+             fn);
+}
+
+void ckks_detail::zero_polys(const CKKSContext& c, CKKSCiphertext& ct, size_t from, const char* fn) {
+    std::vector<uint64_t*> x;
+    for (size_t k = from; k < ct.getNumPolys(); ++k)
+        for (size_t m = 0; m < ct.getBatchSize(); ++m) x.push_back(ct.polyData(k, m));
+    if (x.empty()) return;
+    mi_check(mi_poly_list_op(c.handle(), MI_LIST_ZERO, (int)ct.getLevel(), (int)x.size(), x.data(), nullptr, nullptr,
+                             c.stream()),
              fn);
 }
 
@@ -180,8 +239,11 @@ void ckks_add_integer_constant_inplace(const CKKSContext& c, CKKSCiphertext& ct,
     const auto& q = c.getParams().getModuli();
     std::vector<uint64_t> r(lv + 1);
     for (size_t l = 0; l <= lv; ++l) r[l] = rounded_scaled_integer_mod_q(constant, ct.getScale(), q[l]);
-    mi_check(mi_add_scalar(c.handle(), (int)lv, ct.polyData(0), r.data(), c.stream()),
-             "ckks_add_integer_constant_inplace");
+    if (ct.getBatchSize() == 1)
+        mi_check(mi_add_scalar(c.handle(), (int)lv, ct.polyData(0), r.data(), c.stream()),
+                 "ckks_add_integer_constant_inplace");
+    else
+        ckks_detail::b_list_op(c, MI_LIST_ADD_SCALAR, ct, r.data(), "ckks_add_integer_constant_inplace");
 }
 
 void ckks_relin_hybrid_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
@@ -189,9 +251,20 @@ void ckks_relin_hybrid_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
     if (ct.getNumPolys() != 3) throw std::invalid_argument("ckks_relin: expected 3-polynomial ciphertext");
     if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
         throw std::invalid_argument("ckks_relin: sparse-secret relinearization keys are not available");
-    mi_check(mi_relin(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2),
-                      c.getRelinKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
-             "ckks_relin");
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_relin(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1), ct.polyData(2),
+                          c.getRelinKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+                 "ckks_relin");
+    } else {
+        // (b, a) += KS_rlk(c2) of every member as one key-switch batch
+        std::vector<mi_keyswitch_item> items(ct.getBatchSize());
+        for (size_t m = 0; m < items.size(); ++m)
+            items[m] = mi_keyswitch_item{ct.polyData(2, m), c.getRelinKeyDevice(), ct.polyData(0, m),
+                                         ct.polyData(1, m),  (int32_t)ct.getLevel(), 0, 0, 0};
+        mi_check(mi_keyswitch_batch(c.handle(), items.data(), (int)items.size(), c.getParams().montgomery ? 1 : 0,
+                                    c.stream()),
+                 "ckks_relin");
+    }
     ct.resizePolys(2);
 }
 
@@ -205,9 +278,23 @@ void ckks_rotate_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t rotat
     check_ct(c, ct, "ckks_rotate");
     if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
         throw std::invalid_argument("ckks_rotate: sparse-secret rotation keys are not available");
-    mi_check(mi_rotate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
-                       rotation_automorphism_index(N, r), c.getRotationKeyDevice(r),
-                       c.getParams().montgomery ? 1 : 0, c.stream()),
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_rotate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
+                           rotation_automorphism_index(N, r), c.getRotationKeyDevice(r),
+                           c.getParams().montgomery ? 1 : 0, c.stream()),
+                 "ckks_rotate");
+        return;
+    }
+    const size_t B = ct.getBatchSize();
+    std::vector<uint64_t*> b(B), a(B);
+    std::vector<uint64_t> galois(B, rotation_automorphism_index(N, r));
+    std::vector<const uint64_t*> keys(B, c.getRotationKeyDevice(r));
+    for (size_t m = 0; m < B; ++m) {
+        b[m] = ct.polyData(0, m);
+        a[m] = ct.polyData(1, m);
+    }
+    mi_check(mi_rotate_batch(c.handle(), (int)ct.getLevel(), (int)B, b.data(), a.data(), galois.data(), keys.data(),
+                             c.getParams().montgomery ? 1 : 0, c.stream()),
              "ckks_rotate");
 }
 
@@ -223,25 +310,43 @@ void ckks_conjugate_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
     check_ct(c, ct, "ckks_conjugate");
     if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
         throw std::invalid_argument("ckks_conjugate: sparse-secret conjugation keys are not available");
-    mi_check(mi_conjugate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
-                          c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+    if (ct.getBatchSize() == 1) {
+        mi_check(mi_conjugate(c.handle(), (int)ct.getLevel(), ct.polyData(0), ct.polyData(1),
+                              c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
+                 "ckks_conjugate");
+        return;
+    }
+    std::vector<uint64_t*> b(ct.getBatchSize()), a(ct.getBatchSize());
+    for (size_t m = 0; m < b.size(); ++m) {
+        b[m] = ct.polyData(0, m);
+        a[m] = ct.polyData(1, m);
+    }
+    mi_check(mi_conjugate_batch(c.handle(), (int)ct.getLevel(), (int)b.size(), b.data(), a.data(),
+                                c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
              "ckks_conjugate");
 }
 
 MyVector<CKKSCiphertext> ckks_conjugate_batch(const CKKSContext& c, const MyVector<CKKSCiphertext>& cts) {
     MyVector<CKKSCiphertext> out;
     if (cts.empty()) return out;
+    // each element passes ckks_conjugate_inplace's checks
+    if (!c.hasConjugationKey()) throw std::invalid_argument("ckks_conjugate: conjugation key is unavailable");
     out.reserve(cts.size());
     std::vector<uint64_t*> b, a;
     for (const auto& ct : cts) {
+        if (ct.getNumPolys() != 2) throw std::invalid_argument("ckks_conjugate: expected a 2-polynomial ciphertext");
         check_ct(c, ct, "ckks_conjugate");
-        if (ct.getNumPolys() != 2 || ct.getLevel() != cts[0].getLevel())
-            throw std::invalid_argument("ckks_conjugate: batch ciphertexts must have 2 polys and one level");
+        if (ct.getSecretOwner() != CKKSSecretOwner::Dense)
+            throw std::invalid_argument("ckks_conjugate: sparse-secret conjugation keys are not available");
+        if (ct.getLevel() != cts[0].getLevel())
+            throw std::invalid_argument("ckks_conjugate: batch ciphertexts must have one level");
         out.push_back(ct.clone());
-        b.push_back(out.back().polyData(0));
-        a.push_back(out.back().polyData(1));
+        for (size_t m = 0; m < ct.getBatchSize(); ++m) {
+            b.push_back(out.back().polyData(0, m));
+            a.push_back(out.back().polyData(1, m));
+        }
     }
-    mi_check(mi_conjugate_batch(c.handle(), (int)cts[0].getLevel(), (int)cts.size(), b.data(), a.data(),
+    mi_check(mi_conjugate_batch(c.handle(), (int)cts[0].getLevel(), (int)b.size(), b.data(), a.data(),
                                 c.getConjugationKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
              "ckks_conjugate");
     return out;

@@ -140,14 +140,12 @@ bool scales_match(double a, double b, double rel = 0x1p-45) {
 
 // ---- ciphertext helpers over the C-ABI
 void zero_polys(const CKKSContext& c, CKKSCiphertext& ct, size_t from) {
-    const size_t words = (ct.getLevel() + 1) * ct.getN();
-    for (size_t i = from; i < ct.getNumPolys(); ++i)
-        mi_check(mi_memset_zero(c.handle(), ct.polyData(i), words * 8, c.stream()), "ckks_evalpoly");
+    ckks_detail::zero_polys(c, ct, from, "ckks_evalpoly");
 }
 
 CKKSCiphertext zero_ct(const CKKSContext& c, size_t level, double scale, CKKSSecretOwner o,
-                       CKKSMessageEncodingState st) {
-    CKKSCiphertext ct(c, 2, scale, level, o, st);
+                       CKKSMessageEncodingState st, size_t batch) {
+    CKKSCiphertext ct(c, 2, scale, level, o, st, batch);
     zero_polys(c, ct, 0);
     return ct;
 }
@@ -157,7 +155,7 @@ void add_scaled_constant(const CKKSContext& c, CKKSCiphertext& ct, double consta
     const auto& q = c.getParams().getModuli();
     std::vector<uint64_t> r(ct.getLevel() + 1);
     for (size_t l = 0; l <= ct.getLevel(); ++l) r[l] = scaled_constant_mod_q(constant, scale, q[l]);
-    mi_check(mi_add_scalar(c.handle(), (int)ct.getLevel(), ct.polyData(0), r.data(), c.stream()), "ckks_evalpoly");
+    ckks_detail::b_list_op(c, MI_LIST_ADD_SCALAR, ct, r.data(), "ckks_evalpoly");
 }
 
 void mul_integer(const CKKSContext& c, CKKSCiphertext& ct, ld scalar, ld out_scale) {
@@ -427,7 +425,8 @@ bool coeff_used(const Poly& p, size_t e) {
 CKKSCiphertext eval_leaf(const CKKSContext& c, const Poly& p, const Powers& pw) {
     const size_t deg = p.degree();
     const CKKSCiphertext& x1 = pw.at(1);
-    CKKSCiphertext res = zero_ct(c, p.level, (double)p.scale, x1.getSecretOwner(), x1.getMessageEncodingState());
+    CKKSCiphertext res = zero_ct(c, p.level, (double)p.scale, x1.getSecretOwner(), x1.getMessageEncodingState(),
+                                 x1.getBatchSize());
     if (p.even && p.c[0] != 0.0) add_scaled_constant(c, res, p.c[0], p.scale);
     if (deg == 0) return res;
     for (size_t e = deg; e > 0; --e) {
@@ -521,7 +520,8 @@ CKKSCiphertext ckks_evalchebyshev_preconditioned(const CKKSContext& c, const CKK
         throw std::invalid_argument(
             "ckks_evalchebyshev_preconditioned: ciphertext level is too small for Chebyshev evaluation depth");
     if (deg == 0) {
-        CKKSCiphertext out = zero_ct(c, x.getLevel(), target, x.getSecretOwner(), x.getMessageEncodingState());
+        CKKSCiphertext out =
+            zero_ct(c, x.getLevel(), target, x.getSecretOwner(), x.getMessageEncodingState(), x.getBatchSize());
         add_scaled_constant(c, out, coeffs[0], target);
         return out;
     }

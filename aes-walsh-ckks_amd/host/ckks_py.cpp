@@ -5,6 +5,9 @@
 
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
+#include <chrono>
+#include <cstring>
+
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -120,10 +123,16 @@ PYBIND11_MODULE(ckks_py, m) {
     py::class_<CKKSParams>(m, "CKKSParams")
         .def(py::init<CKKSParamsConfig>())
         .def_static("AESWalsh15", &CKKSParams::AESWalsh15)
+        .def_static("XBootParamAes13Reference", &CKKSParams::XBootParamAes13Reference)
+        .def_static("AESXBoot14Secure", &CKKSParams::AESXBoot14Secure)
+        .def_static("AESXBoot14SecureSparse1024", &CKKSParams::AESXBoot14SecureSparse1024)
+        .def_static("AESXBoot15", &CKKSParams::AESXBoot15)
+        .def_static("AESXBoot15Sparse1024", &CKKSParams::AESXBoot15Sparse1024)
         .def("usesAesWalsh", &CKKSParams::usesAesWalsh)
         .def("getAesWalshBlockCount", &CKKSParams::getAesWalshBlockCount)
         .def("getAesWalshPackedSegmentCount", &CKKSParams::getAesWalshPackedSegmentCount)
         .def("getN", &CKKSParams::getN)
+        .def("getSeed", &CKKSParams::getSeed)
         .def("getSlots", &CKKSParams::getSlots)
         .def("getMaxLevel", &CKKSParams::getMaxLevel)
         .def("getDnum", &CKKSParams::getDnum)
@@ -144,6 +153,8 @@ PYBIND11_MODULE(ckks_py, m) {
 
     py::class_<CKKSContext>(m, "CKKSContext")
         .def(py::init<CKKSParams>())
+        .def_static("makeLane", &CKKSContext::makeLane, py::keep_alive<0, 1>())
+        .def("isLane", &CKKSContext::isLane)
         .def("getParams", &CKKSContext::getParams, py::return_value_policy::reference_internal)
         .def("getSecretKeyCoeffs", [](const CKKSContext& c) {
             const auto& s = c.getSecretKeyCoeffs();
@@ -187,6 +198,14 @@ PYBIND11_MODULE(ckks_py, m) {
             int64_t st[4];
             mi_check(mi_memory_stats(c.handle(), st), "memoryStats");
             return std::vector<int64_t>(st, st + 4);
+        })
+        // NTT probe: events around every NTT launch pair between the two calls;
+        // the end returns [summed ms, launch pairs, algorithmic bytes, limbs]
+        .def("nttProbeBegin", [](const CKKSContext& c) { mi_check(mi_probe_begin(c.handle()), "nttProbeBegin"); })
+        .def("nttProbeEnd", [](const CKKSContext& c) {
+            double st[4];
+            mi_check(mi_probe_end(c.handle(), st), "nttProbeEnd");
+            return std::vector<double>(st, st + 4);
         });
 
     py::class_<CKKSCiphertext>(m, "CKKSCiphertext")
@@ -205,6 +224,8 @@ PYBIND11_MODULE(ckks_py, m) {
         .def("getSecretOwner", &CKKSCiphertext::getSecretOwner)
         .def("getMessageEncodingState", &CKKSCiphertext::getMessageEncodingState)
         .def("getPoly", [](const CKKSCiphertext& ct, size_t i) { return to_np(ct.getPoly(i)); })
+        .def("getBatchSize", &CKKSCiphertext::getBatchSize)
+        .def("getMemberPoly", [](const CKKSCiphertext& ct, size_t i, size_t m) { return to_np(ct.getPoly(i, m)); })
         .def("setPoly", [](CKKSCiphertext& ct, size_t i, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> a) {
             ct.setPoly(i, from_np(a));
         })
@@ -422,14 +443,23 @@ PYBIND11_MODULE(ckks_py, m) {
     m.def("ckks_s2c_generalized", &ckks_s2c_generalized, py::keep_alive<0, 1>());
     m.def("ckks_c2s_generalized", &ckks_c2s_generalized, py::keep_alive<0, 1>());
     m.def("ckks_evalmod_binary_inplace", &ckks_evalmod_binary_inplace);
-    m.def("ckks_bootstrap_pre_evalmod",
-          [](const CKKSContext& c, const CKKSCiphertext& ct) { return ckks_bootstrap_pre_evalmod(c, ct.clone()); },
-          py::keep_alive<0, 1>());
+    // a pair of ciphertexts, each keeping the context alive (a tuple itself
+    // cannot carry keep_alive)
+    auto ct_pair = [](std::pair<CKKSCiphertext, CKKSCiphertext> v, const py::object& context) {
+        py::object a = py::cast(std::move(v.first)), b = py::cast(std::move(v.second));
+        py::detail::keep_alive_impl(a, context);
+        py::detail::keep_alive_impl(b, context);
+        return py::make_tuple(a, b);
+    };
+    m.def("ckks_bootstrap_pre_evalmod", [ct_pair](const py::object& context, const CKKSCiphertext& ct) {
+        return ct_pair(ckks_bootstrap_pre_evalmod(context.cast<const CKKSContext&>(), ct.clone()), context);
+    });
     m.def("ckks_batch_bootstrap_evalmod_from_complex_packed",
-          [](const CKKSContext& c, const CKKSCiphertext& ct) {
-              return ckks_batch_bootstrap_evalmod_from_complex_packed(c, ct.clone());
-          },
-          py::keep_alive<0, 1>());
+          [ct_pair](const py::object& context, const CKKSCiphertext& ct) {
+              return ct_pair(ckks_batch_bootstrap_evalmod_from_complex_packed(context.cast<const CKKSContext&>(),
+                                                                              ct.clone()),
+                             context);
+          });
     m.def("ckks_batch_bootstrap_evalmod_inplace", &ckks_batch_bootstrap_evalmod_inplace);
     // Walsh S-box helpers (lists of ciphertexts are cloned into the call)
     auto ct_list = [](const py::list& l) {
@@ -459,6 +489,16 @@ PYBIND11_MODULE(ckks_py, m) {
         }
         return r;
     };
+    // a batch of copies of the listed ciphertexts, and copies of a batch's members
+    m.def("ckks_stack", [ct_list](const py::object& context, const py::list& cts) {
+        py::object o = py::cast(CKKSCiphertext::stack(ct_list(cts)));
+        py::detail::keep_alive_impl(o, context);
+        return o;
+    });
+    m.def("ckks_unstack", [ct_pylist](const py::object& context, const CKKSCiphertext& ct) {
+        auto v = ct.clone().unstack();
+        return ct_pylist(v, context);
+    });
     m.def("ckks_conjugate_batch", [ct_list, ct_pylist](const py::object& context, const py::list& cts) {
         auto out = ckks_conjugate_batch(context.cast<const CKKSContext&>(), ct_list(cts));
         return ct_pylist(out, context);
@@ -534,22 +574,103 @@ PYBIND11_MODULE(ckks_py, m) {
     m.def("aes128_encrypt_block_expanded", &aes128_encrypt_block_expanded);
     m.def("aes128_ctr_crypt_block", &aes128_ctr_crypt_block);
     m.def("default_evalmod_polynomial", &default_evalmod_polynomial);
+    m.def("evalmod_polynomial_is_reference_fit", &evalmod_polynomial_is_reference_fit);
     m.def("aes_sbox_table", []() { return aes_sbox_table(); });
     m.def("ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks",
           [](const CKKSContext& c, const std::array<uint8_t, 176>& rk, size_t active) {
               return AesRoundKeyBits(ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(c, rk, active));
           },
           py::keep_alive<0, 1>());
+    py::class_<CKKSAesCtrServer>(m, "CKKSAesCtrServer")
+        .def(py::init([](CKKSContext& owner, const std::array<uint8_t, 16>& key, size_t lanes, int64_t budget,
+                         size_t max_request_blocks, size_t max_queued_evaluations) {
+                 CKKSAesServerOptions o;
+                 o.lanes = lanes;
+                 o.memory_budget_bytes = budget;
+                 o.max_request_blocks = max_request_blocks;
+                 o.max_queued_evaluations = max_queued_evaluations;
+                 py::gil_scoped_release unlocked;
+                 return std::make_unique<CKKSAesCtrServer>(owner, key, o);
+             }),
+             py::arg("owner"), py::arg("aes_key"), py::arg("lanes") = 2, py::arg("memory_budget_bytes") = 0,
+             py::arg("max_request_blocks") = 1024, py::arg("max_queued_evaluations") = 4, py::keep_alive<1, 2>())
+        .def("submit",
+             [](CKKSAesCtrServer& s, const py::array_t<uint8_t, py::array::c_style>& counters,
+                const py::array_t<uint8_t, py::array::c_style>& cts) {
+                 auto c = blocks_from_array(counters), x = blocks_from_array(cts);
+                 return s.submit(std::move(c), std::move(x));
+             })
+        .def("wait",
+             [](CKKSAesCtrServer& s, uint64_t id) {
+                 CKKSAesRequestResult r;
+                 {
+                     py::gil_scoped_release unlocked;
+                     r = s.wait(id);
+                 }
+                 py::dict d;
+                 d["id"] = r.id;
+                 d["refused"] = r.refused;
+                 d["error"] = r.error;
+                 const size_t n = r.plaintexts.size();
+                 py::array_t<uint8_t> pt({(py::ssize_t)n, (py::ssize_t)16});
+                 py::array_t<double> vals({(py::ssize_t)n, (py::ssize_t)128});
+                 for (size_t j = 0; j < n; ++j) {
+                     std::memcpy(pt.mutable_data(j, 0), r.plaintexts[j].data(), 16);
+                     std::memcpy(vals.mutable_data(j, 0), r.values[j].data(), 128 * sizeof(double));
+                 }
+                 d["plaintexts"] = pt;
+                 d["values"] = vals;
+                 d["arrival"] = r.arrival;
+                 d["start"] = r.start;
+                 d["finish"] = r.finish;
+                 d["lane"] = r.lane;
+                 return d;
+             })
+        .def("laneCount", &CKKSAesCtrServer::laneCount)
+        .def("evaluationBytes", &CKKSAesCtrServer::evaluationBytes)
+        .def("evaluations", &CKKSAesCtrServer::evaluations);
+    m.def("ckks_aes_serve_take", &ckks_aes_serve_take);
+    m.def("steady_clock_seconds", [] {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    });
+    m.def("ckks_aes128_encrypt_expanded_round_key_bits",
+          [](const CKKSContext& c, const std::array<uint8_t, 176>& rk, size_t first_level) {
+              AesRoundKeyBits k(MyVector<CKKSCiphertext>{});
+              {
+                  py::gil_scoped_release unlocked;
+                  k.bits = ckks_aes128_encrypt_expanded_round_key_bits(c, rk, first_level);
+              }
+              return k;
+          },
+          py::keep_alive<0, 1>());
+    m.def("ckks_aes_ctr",
+          [ct_pylist](const py::object& context, const AesRoundKeyBits& keys,
+                      const py::array_t<uint8_t, py::array::c_style>& counters,
+                      const py::array_t<uint8_t, py::array::c_style>& ciphertexts, bool batch) {
+              CKKSContext& c = context.cast<CKKSContext&>();
+              const auto ctr = blocks_from_array(counters), cts = blocks_from_array(ciphertexts);
+              CKKSAes128CtrOptions options;
+              options.batch_independent_work = batch;
+              MyVector<CKKSCiphertext> out;
+              {
+                  py::gil_scoped_release unlocked;
+                  out = ckks_aes_ctr(c, keys.bits, ctr, cts, options);
+              }
+              return ct_pylist(out, context);
+          },
+          py::arg("context"), py::arg("keys"), py::arg("counter_blocks"), py::arg("ciphertext_blocks"),
+          py::arg("batch_independent_work") = true);
     m.def("ckks_aes_ctr_walsh_sbox_complex_blocks",
           [ct_pylist](const py::object& context, const AesRoundKeyBits& keys,
                       const py::array_t<uint8_t, py::array::c_style>& counters,
                       const py::array_t<uint8_t, py::array::c_style>& ciphertexts, const py::list& sbox_contexts,
-                      bool fuse_linear_layer) {
+                      bool fuse_linear_layer, bool batch_sboxes) {
               CKKSContext& c = context.cast<CKKSContext&>();
               const auto ctr = blocks_from_array(counters), cts = blocks_from_array(ciphertexts);
               CKKSAesCtrWalshSboxOptions options;
               for (const auto& w : sbox_contexts) options.sbox_contexts.push_back(&w.cast<CKKSContext&>());
               options.fuse_linear_layer = fuse_linear_layer;
+              options.batch_sboxes = batch_sboxes;
               MyVector<CKKSCiphertext> out;
               {
                   py::gil_scoped_release unlocked;
@@ -558,5 +679,6 @@ PYBIND11_MODULE(ckks_py, m) {
               return ct_pylist(out, context);
           },
           py::arg("context"), py::arg("keys"), py::arg("counter_blocks"), py::arg("ciphertext_blocks"),
-          py::arg("sbox_contexts") = py::list(), py::arg("fuse_linear_layer") = true);
+          py::arg("sbox_contexts") = py::list(), py::arg("fuse_linear_layer") = true,
+          py::arg("batch_sboxes") = true);
 }

@@ -70,9 +70,10 @@ void rounded_constant(int64_t value, long double scale, uint64_t* abs_words, int
 }
 
 bool ciphertext_is_zero(const CKKSCiphertext& ct) {
-    for (size_t k = 0; k < ct.getNumPolys(); ++k)
-        for (uint64_t w : ct.getPoly(k))
-            if (w) return false;
+    for (size_t m = 0; m < ct.getBatchSize(); ++m)
+        for (size_t k = 0; k < ct.getNumPolys(); ++k)
+            for (uint64_t w : ct.getPoly(k, m))
+                if (w) return false;
     return true;
 }
 
@@ -212,20 +213,26 @@ std::pair<CKKSCiphertext, CKKSCiphertext> walsh_product_sum(const CKKSContext& c
         rounded_constant(cr[h][0], S, &vabs[4 * h], &vneg[2 * h]);
         rounded_constant(ci[h][0], S, &vabs[4 * h + 2], &vneg[2 * h + 1]);
     }
-    const uint64_t *lb[15], *la[15], *hb[15], *ha[15];
-    for (size_t m = 1; m < kMasks; ++m) {
-        lb[m - 1] = low_signs[m].polyData(0);
-        la[m - 1] = low_signs[m].polyData(1);
-        hb[m - 1] = high_signs[m].polyData(0);
-        ha[m - 1] = high_signs[m].polyData(1);
+    const size_t B = ref.getBatchSize();
+    for (size_t m = 1; m < kMasks; ++m)
+        if (low_signs[m].getBatchSize() != B || high_signs[m].getBatchSize() != B)
+            throw std::invalid_argument("ckks_walsh_sbox: sign ciphertexts differ in batch size");
+    CKKSCiphertext acc(context, 2, S, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
+    CKKSCiphertext raw(context, 3, P, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
+    for (size_t j = 0; j < B; ++j) {
+        const uint64_t *lb[15], *la[15], *hb[15], *ha[15];
+        for (size_t m = 1; m < kMasks; ++m) {
+            lb[m - 1] = low_signs[m].polyData(0, j);
+            la[m - 1] = low_signs[m].polyData(1, j);
+            hb[m - 1] = high_signs[m].polyData(0, j);
+            ha[m - 1] = high_signs[m].polyData(1, j);
+        }
+        mi_check(mi_walsh_product_sum(context.handle(), (int)level, lb, la, hb, ha, &cr[0][0], &ci[0][0], vabs, vneg,
+                                      encoded_i.data(), encoded_i.isMontgomeryForm() ? 1 : 0, acc.polyData(0, j),
+                                      acc.polyData(1, j), raw.polyData(0, j), raw.polyData(1, j), raw.polyData(2, j),
+                                      context.stream()),
+                 "ckks_walsh_sbox");
     }
-    CKKSCiphertext acc(context, 2, S, level, ref.getSecretOwner(), ref.getMessageEncodingState());
-    CKKSCiphertext raw(context, 3, P, level, ref.getSecretOwner(), ref.getMessageEncodingState());
-    mi_check(mi_walsh_product_sum(context.handle(), (int)level, lb, la, hb, ha, &cr[0][0], &ci[0][0], vabs, vneg,
-                                  encoded_i.data(), encoded_i.isMontgomeryForm() ? 1 : 0, acc.polyData(0),
-                                  acc.polyData(1), raw.polyData(0), raw.polyData(1), raw.polyData(2),
-                                  context.stream()),
-             "ckks_walsh_sbox");
     return {std::move(acc), std::move(raw)};
 }
 
@@ -280,13 +287,15 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
         sums.push_back(walsh_product_sum(context, low_signs, high_signs, pair, pair + 4, encoded_i));
     if (low_signs.front().getSecretOwner() != CKKSSecretOwner::Dense)
         throw std::invalid_argument("ckks_relin: sparse-secret relinearization keys are not available");
-    mi_keyswitch_item items[4];
+    std::vector<mi_keyswitch_item> items;
     for (size_t pair = 0; pair < 4; ++pair) {
         CKKSCiphertext& raw = sums[pair].second;
-        items[pair] = mi_keyswitch_item{raw.polyData(2), context.getRelinKeyDevice(), raw.polyData(0),
-                                        raw.polyData(1), (int32_t)raw.getLevel(), 0, 0, 0};
+        for (size_t m = 0; m < raw.getBatchSize(); ++m)
+            items.push_back(mi_keyswitch_item{raw.polyData(2, m), context.getRelinKeyDevice(), raw.polyData(0, m),
+                                              raw.polyData(1, m), (int32_t)raw.getLevel(), 0, 0, 0});
     }
-    mi_check(mi_keyswitch_batch(context.handle(), items, 4, context.getParams().montgomery ? 1 : 0, context.stream()),
+    mi_check(mi_keyswitch_batch(context.handle(), items.data(), (int)items.size(),
+                                context.getParams().montgomery ? 1 : 0, context.stream()),
              "ckks_relin");
     MyVector<CKKSCiphertext> outputs;
     outputs.reserve(4);
@@ -372,10 +381,12 @@ CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& cont
     for (size_t t = 0; t < terms.size(); ++t) {
         if (segments[t] == 0) continue;
         const int64_t r = normalize_rotation_value(N, (int64_t)(segments[t] * block_count));
-        b.push_back(terms[t].polyData(0));
-        a.push_back(terms[t].polyData(1));
-        galois.push_back(rotation_automorphism_index(N, r));
-        keys.push_back(context.getRotationKeyDevice(r));
+        for (size_t m = 0; m < terms[t].getBatchSize(); ++m) {
+            b.push_back(terms[t].polyData(0, m));
+            a.push_back(terms[t].polyData(1, m));
+            galois.push_back(rotation_automorphism_index(N, r));
+            keys.push_back(context.getRotationKeyDevice(r));
+        }
     }
     mi_check(mi_rotate_batch(context.handle(), (int)ref.getLevel(), (int)b.size(), b.data(), a.data(), galois.data(),
                              keys.data(), context.getParams().montgomery ? 1 : 0, context.stream()),
@@ -404,10 +415,8 @@ std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_
     mi_check(mi_context_info(context.handle(), info), "ckks_walsh_sbox");
     const size_t L = level + 1, K = P.getPModuli().size(), alpha = (size_t)info[4];
     const size_t parts = std::min<size_t>(P.getDnum(), (L + alpha - 1) / alpha);
-    DeviceBuffer digits(context.handle(), parts * (L + K) * N);
-    mi_check(mi_hoist(context.handle(), (int)level, packed.polyData(1), digits.data(), context.stream()),
-             "ckks_walsh_sbox");
-    std::vector<uint64_t*> b_out, a_out;
+    const size_t B = packed.getBatchSize();
+    std::vector<size_t> rotated;  // indices into (low | high) of the hoisted outputs
     std::vector<uint64_t> galois;
     std::vector<const uint64_t*> keys;
     for (size_t mask = 1; mask < kMasks; ++mask)
@@ -422,16 +431,27 @@ std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_
             if (!context.hasRotationKey(r))
                 throw std::invalid_argument("ckks_walsh_sbox: missing hoisted unpack rotation key");
             dst.emplace_back(context, 2, packed.getScale(), level, packed.getSecretOwner(),
-                             packed.getMessageEncodingState());
-            b_out.push_back(dst.back().polyData(0));
-            a_out.push_back(dst.back().polyData(1));
+                             packed.getMessageEncodingState(), B);
+            rotated.push_back((hi ? kMasks : 0) + dst.size() - 1);
             galois.push_back(rotation_automorphism_index(N, r));
             keys.push_back(context.getRotationKeyDevice(r));
         }
-    mi_check(mi_rotate_hoisted_batch(context.handle(), (int)level, digits.data(), packed.polyData(0),
-                                     (int)galois.size(), galois.data(), keys.data(), b_out.data(), a_out.data(),
-                                     P.montgomery ? 1 : 0, context.stream()),
-             "ckks_walsh_sbox");
+    // one hoist per member (the buffer is reused in stream order)
+    DeviceBuffer digits(context.handle(), parts * (L + K) * N);
+    for (size_t m = 0; m < B; ++m) {
+        mi_check(mi_hoist(context.handle(), (int)level, packed.polyData(1, m), digits.data(), context.stream()),
+                 "ckks_walsh_sbox");
+        std::vector<uint64_t*> b_out, a_out;
+        for (size_t idx : rotated) {
+            CKKSCiphertext& dst = idx >= kMasks ? high[idx - kMasks] : low[idx];
+            b_out.push_back(dst.polyData(0, m));
+            a_out.push_back(dst.polyData(1, m));
+        }
+        mi_check(mi_rotate_hoisted_batch(context.handle(), (int)level, digits.data(), packed.polyData(0, m),
+                                         (int)galois.size(), galois.data(), keys.data(), b_out.data(), a_out.data(),
+                                         P.montgomery ? 1 : 0, context.stream()),
+                 "ckks_walsh_sbox");
+    }
     return {std::move(low), std::move(high)};
 }
 
@@ -461,7 +481,7 @@ void mask_all(const CKKSContext& context, MyVector<CKKSCiphertext>& cts, const C
     std::vector<uint64_t*> polys;
     for (auto& ct : cts) {
         if (ct.getLevel() != mask.getLevel()) throw std::invalid_argument(level_msg);
-        for (size_t k = 0; k < ct.getNumPolys(); ++k) polys.push_back(ct.polyData(k));
+        for (uint64_t* p : ct.allPolyData()) polys.push_back(p);
     }
     const size_t level = mask.getLevel();
     mi_check(mi_mul_plain_rescale_list(context.handle(), (int)level, (int)polys.size(), polys.data(), mask.data(),

@@ -29,6 +29,9 @@ _SIGS = {
     "mi_memory_trim": (cint, [vp]),
     "mi_memory_stats": (cint, [vp, vp]),
     "mi_set_memory_limit": (cint, [vp, i64]),
+    "mi_device_memory_info": (cint, [vp, vp, vp]),
+    "mi_probe_begin": (cint, [vp]),
+    "mi_probe_end": (cint, [vp, vp]),
     "mi_memcpy": (cint, [vp, vp, vp, ctypes.c_size_t, cint, vp]),
     "mi_stream_synchronize": (cint, [vp, vp]),
     "mi_stream_create": (cint, [vp, ctypes.POINTER(vp)]),
@@ -44,6 +47,7 @@ _SIGS = {
     "mi_ntt_inverse": (cint, [vp, vp, cint, cint, i64, cint, vp]),
     "mi_tensor": (cint, [vp, cint, vp, vp, vp, vp, vp, vp]),
     "mi_square": (cint, [vp, cint, vp, vp, vp, vp]),
+    "mi_tensor_list": (cint, [vp, cint, cint, vp, vp, vp, vp, vp, vp]),
     "mi_mul_plain": (cint, [vp, cint, vp, cint, i64, vp, cint, vp]),
     "mi_mul_scalar": (cint, [vp, cint, vp, cint, i64, vp, vp]),
     "mi_add": (cint, [vp, cint, vp, vp, cint, i64, vp]),

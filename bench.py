@@ -187,13 +187,27 @@ def bench_keyswitch(args, M, torch):
     KL = 40
     keys = [_rand_limbs(torch, list(q) + list(p), N, 2 * 3, seed=100 + k) for k in range(nkeys)]
     levels = rng.integers(7, 30, nct)
-    items = []
+    levels[:3] = [29, 19, 7]
+    items, inputs = [], []
     for i, lv in enumerate(levels):
         Lc = int(lv) + 1
         kt = _rand_limbs(torch, q[:Lc], N, 1, seed=1000 + i)
         b = _rand_limbs(torch, q[:Lc], N, 1, seed=2000 + i)
         a = _rand_limbs(torch, q[:Lc], N, 1, seed=3000 + i)
         items.append((int(lv), kt, keys[i % nkeys], b, a, 0, 1))
+        inputs.append((b.clone(), a.clone()))
+    # one switch of the batch checked against the oracle for items 0-2
+    # (levels 29, 19, 7): the timed steps then reuse the same buffers
+    ctx.keyswitch_batch(items)
+    torch.cuda.synchronize()
+    O = _oracle()
+    octx = O.Ctx(N, q, p, 3)
+    host = lambda t: t.detach().cpu().numpy().view(np.uint64)
+    for i in range(3):
+        lv, kt, key, b, a, _, _ = items[i]
+        ob, oa = octx.keyswitch(lv, host(kt), host(key), host(inputs[i][0]), host(inputs[i][1]), 0, 1)
+        if not (np.array_equal(host(b), ob) and np.array_equal(host(a), oa)):
+            raise SystemExit(f"batched key switch item {i} (level {lv}) differs from the oracle")
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         ctx.keyswitch_batch(items)
@@ -216,7 +230,7 @@ def bench_keyswitch(args, M, torch):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded)",
         "config": {"workload": "128 key switches, mixed levels, 16 keys", "algorithmic_GBps":
-                   algo / (ms / 1e3) / 1e9},
+                   algo / (ms / 1e3) / 1e9, "oracle_checked": "items at levels 29, 19, 7 bit-exact"},
         "roofline": None, "cpu_baseline": cpu,
     }
 
@@ -723,7 +737,8 @@ def bench_aes(args, M, torch):
     encryption are outside the timed region, as in the reference."""
     H = _host()
     t0 = time.perf_counter()
-    ctxs = [H.CKKSContext(H.CKKSParams.AESWalsh15()) for _ in range(max(1, args.aes_contexts))]
+    params = H.CKKSParams.AESWalsh15()  # one seed: every context holds the same keys
+    ctxs = [H.CKKSContext(params) for _ in range(max(1, args.aes_contexts))]
     ctx, workers = ctxs[0], ctxs[1:]
     cb = ctx.getParams().getAesWalshBlockCount()
     counters, plains, cts = aes_bench_blocks(H, 2 * cb)
@@ -731,31 +746,46 @@ def bench_aes(args, M, torch):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     print(f"# walsh15 context and round keys: {setup_s:.1f} s", file=sys.stderr, flush=True)
+    batched = not workers  # one context: the 16 S-boxes of a round as one ciphertext batch
+
+    def evaluate():
+        return H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=workers,
+                                                        batch_sboxes=batched)
+
     for _ in range(args.warmup):
-        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=workers)
+        out = evaluate()
     torch.cuda.synchronize()
     lat = []
     for step in range(args.steps):
         t1 = time.perf_counter()
-        out = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=workers)
+        out = evaluate()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
         print(f"# step {step}: {lat[-1]:.3f} s", file=sys.stderr, flush=True)
     ms = 1e3 * sum(lat) / len(lat)
     mism, max_err, avg_err = aes_validate(H, ctx, out, plains, cb)
-    roofline = aes_ntt_roofline(M, torch, ctx.getParams())
-    plain_s = None
-    if workers and args.plain:
-        # the plain form: the same evaluation with the S-boxes one after another
+    # the roofline kernel inside the evaluation itself: one more evaluation
+    # with HIP events around every NTT launch pair on its stream
+    ctx.nttProbeBegin()
+    t1 = time.perf_counter()
+    evaluate()
+    torch.cuda.synchronize()
+    probe_s = time.perf_counter() - t1
+    ntt_ms, pairs, ntt_bytes, ntt_limbs = ctx.nttProbeEnd()
+    roofline = aes_ntt_roofline(M, torch, ctx.getParams(), (ntt_ms, pairs, ntt_bytes, ntt_limbs, probe_s))
+    plain_s, plain = None, None
+    if args.plain:
+        # the plain form: the same evaluation one ciphertext at a time (the
+        # S-boxes one after another on one stream), limb for limb the same
         t1 = time.perf_counter()
-        plain = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+        plain = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, batch_sboxes=False)
         torch.cuda.synchronize()
         plain_s = time.perf_counter() - t1
         same = all(np.array_equal(np.asarray(plain[b].getPoly(k)), np.asarray(out[b].getPoly(k)))
                    for b in range(128) for k in range(2))
         if not same:
-            raise SystemExit("concurrent and sequential S-box evaluations differ")
-    return {
+            raise SystemExit("batched and one-at-a-time S-box evaluations differ")
+    res = {
         "metric": "walsh15 AES-CTR blocks/s (homomorphic CTR decryption, mask-packed Walsh S-box)",
         "value": 2 * cb * 1e3 / ms, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -763,87 +793,96 @@ def bench_aes(args, M, torch):
         "config": {"workload": "walsh15 AES-CTR, 1024 blocks per evaluation", "blocks": 2 * cb,
                    "mismatches": mism, "max_error": max_err, "average_error": avg_err,
                    "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-                   "sbox_contexts": len(ctxs),
+                   "form": "batched (16 S-boxes per launch sequence, one context)" if batched
+                   else f"{len(ctxs)} S-box contexts on host threads",
                    "plain_blocks_per_s": None if plain_s is None else 2 * cb / plain_s,
                    "plain_bit_exact": None if plain_s is None else True,
                    "context_and_key_setup_s": setup_s},
         "roofline": roofline,
         "cpu_baseline": cpu_baseline_aes(H, ctx.getParams(), 2 * cb) if args.cpu_baseline else None,
     }
+    if args.serve_requests > 0:
+        del out, plain
+        torch.cuda.synchronize()
+        res["config"]["served"] = serve_leg(H, args.serve_requests, args.rate, args.lanes)
+    return res
 
 
-def bench_serve(args, M, torch):
-    """Served walsh15 AES-CTR (aes-walsh-ckks_amd/aes_serve.py): Poisson
-    arrivals at --rate requests/s from a seed, each of 1-32 blocks (random
-    plaintexts, AES-CTR under the test key), served by continuous batching
-    over --lanes lanes of --lane-contexts contexts; blocks/s over the whole run (first arrival to
-    last result) and p50/p99 of arrival -> decrypted result.  Default: one
-    lane of 16 contexts (two lanes of 8 overlap whole evaluations but serve
-    fewer blocks/s, since the S-boxes of one evaluation already keep the
-    device's queues busy).  The plain form evaluates one request per
-    evaluation on one lane."""
+def serve_leg(H, requests, rate, lanes, seed=55, plain=False):
+    """Served walsh15 AES-CTR through the native continuous-batching server
+    (aes-walsh-ckks_amd/aes_serve.py over host/ckks_aes_serve.cpp): Poisson
+    arrivals at `rate` requests/s from `seed`, each request 1-32 blocks of
+    random plaintext under the test key, submitted at its arrival time;
+    blocks/s over the run (first arrival to last result) and p50/p99 of
+    arrival -> decrypted result.  `plain`: the first three requests again on
+    a one-lane server, one request per evaluation, for comparison."""
     sys.path.insert(0, os.path.join(ROOT, "aes-walsh-ckks_amd"))
     import aes_serve
-    H = _host()
-    rng = np.random.default_rng(55)
+    rng = np.random.default_rng(seed)
     base = int.from_bytes(bytes.fromhex("f0f1f2f3f4f5f6f7f8f9fafbfcfdfeff"), "big")
     reqs, first = [], 0
-    for _ in range(args.requests):
+    for _ in range(requests):
         n = int(rng.integers(1, 33))
         c = np.array([list((base + first + i).to_bytes(16, "big")) for i in range(n)], np.uint8)
         p = rng.integers(0, 256, (n, 16), dtype=np.uint8)
         x = np.array([H.aes128_ctr_crypt_block(list(AES_KEY), list(a), list(b)) for a, b in zip(c, p)], np.uint8)
         reqs.append((c, x, p))
         first += n
-    gaps = rng.exponential(1.0 / args.rate, args.requests)
+    gaps = rng.exponential(1.0 / rate, requests)
     t0 = time.perf_counter()
-    srv = aes_serve.AesCtrServer(AES_KEY, lanes=args.lanes, contexts_per_lane=args.lane_contexts)
+    srv = aes_serve.AesCtrServer(AES_KEY, lanes=lanes)
     setup_s = time.perf_counter() - t0
-    print(f"# server: {len(srv.lanes)} lanes, footprints {[f // 2**20 for f in srv.footprints]} MiB, "
+    print(f"# server: {srv.lanes} lanes, {srv.evaluation_bytes / 2**30:.2f} GiB per evaluation, "
           f"set-up {setup_s:.1f} s", file=sys.stderr, flush=True)
-    live = []
+    ids = []
     start = time.perf_counter()
     due = start
     for (c, x, _), g in zip(reqs, gaps):
         due += g
         while time.perf_counter() < due:
             time.sleep(min(0.005, max(0.0, due - time.perf_counter())))
-        live.append(srv.submit(aes_serve.Request(c, x)))
-    for r in live:
-        r.done.wait()
-    end = max(r.finish for r in live)
-    evaluations = srv.evaluations
+        ids.append(srv.submit(c, x))
+    res = [srv.result(i) for i in ids]
+    evaluations, lane_count = srv.evaluations, srv.lanes
     srv.close()
-    errors = [r.error for r in live if r.error is not None]
-    if errors:
-        raise errors[0]
-    mism = sum(int(np.sum(np.any(r.plaintexts != p, axis=1))) for r, (_, _, p) in zip(live, reqs))
-    blocks = sum(r.blocks for r in live)
-    lat = np.array([r.latency for r in live])
-    plain = None
-    if args.plain:
-        one = aes_serve.AesCtrServer(AES_KEY, lanes=1, contexts_per_lane=args.lane_contexts)
+    bad = [r for r in res if r["refused"] or r["error"]]
+    if bad:
+        raise RuntimeError(f"served request {bad[0]['id']} failed: {bad[0]['error']}")
+    mism = sum(int(np.sum(np.any(r["plaintexts"] != p, axis=1))) for r, (_, _, p) in zip(res, reqs))
+    blocks = sum(len(c) for c, _, _ in reqs)
+    t_first = min(r["arrival"] for r in res)
+    t_last = max(r["finish"] for r in res)
+    lat = np.array([r["finish"] - r["arrival"] for r in res])
+    out = {"requests": requests, "rate_per_s": rate, "blocks": blocks, "mismatched_blocks": mism,
+           "blocks_per_s": blocks / (t_last - t_first), "wall_s": t_last - t_first,
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "evaluations": evaluations, "lanes": lane_count, "server_setup_s": setup_s}
+    if plain:
+        one = aes_serve.AesCtrServer(AES_KEY, lanes=1)
         t1 = time.perf_counter()
         nb = 0
         for c, x, p in reqs[:3]:
-            r = one.submit(aes_serve.Request(c, x))
-            r.done.wait()
-            nb += r.blocks
-            if r.error is not None or not np.array_equal(r.plaintexts, p):
+            r = one.result(one.submit(c, x))
+            nb += len(c)
+            if r["refused"] or r["error"] or not np.array_equal(r["plaintexts"], p):
                 raise RuntimeError("plain serving decrypted a wrong block")
-        plain = nb / (time.perf_counter() - t1)
+        out["plain_one_request_per_evaluation_blocks_per_s"] = nb / (time.perf_counter() - t1)
         one.close()
+    return out
+
+
+def bench_serve(args, M, torch):
+    """Served walsh15 AES-CTR (serve_leg) with --requests requests at --rate
+    requests/s over --lanes lanes; value = served blocks/s."""
+    H = _host()
+    r = serve_leg(H, args.requests, args.rate, args.lanes, plain=args.plain)
     return {
         "metric": "served walsh15 AES-CTR blocks/s with p50/p99 latency (continuous batching)",
-        "value": blocks / (end - start), "unit": "blocks/s", "n_gpus": 1, "steps": args.requests, "warmup": 0,
-        "ms_per_step": 1e3 * (end - start) / args.requests, "higher_is_better": True, "scaling": "weak",
+        "value": r["blocks_per_s"], "unit": "blocks/s", "n_gpus": 1, "steps": args.requests, "warmup": 0,
+        "ms_per_step": 1e3 * r["wall_s"] / args.requests, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64 (exact mod-q integer)",
         "data": "synthetic (Poisson arrivals, random plaintexts, seed 55)",
-        "config": {"workload": "served walsh15 AES-CTR, requests of 1-32 blocks", "requests": args.requests,
-                   "rate_per_s": args.rate, "blocks": blocks, "mismatched_blocks": mism,
-                   "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-                   "evaluations": evaluations, "lanes": len(srv.lanes), "contexts_per_lane": args.lane_contexts,
-                   "plain_one_request_per_evaluation_blocks_per_s": plain, "server_setup_s": setup_s},
+        "config": dict({"workload": "served walsh15 AES-CTR, requests of 1-32 blocks"}, **r),
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -852,22 +891,33 @@ def bench_ntt_qp(args, M, torch):
     """The AES workload's roofline kernel alone: the forward NTT of one QP
     ciphertext at walsh15 (2 polys x 17 limbs, N=2^15), for counter passes."""
     H = _host()
-    r = aes_ntt_roofline(M, torch, H.CKKSParams.AESWalsh15())
+    r = aes_ntt_roofline(M, torch, H.CKKSParams.AESWalsh15(), (1.0, 1.0, 1.0, 1.0, 1.0))
+    us = r["isolated_qp_pair_us"]
+    N = 1 << 15
+    r.update({"achieved": 16 * N * 34 / (us / 1e6) / 1e9, "frac": r["isolated_qp_pair_frac"],
+              "kernel": "forward NTT (column + row pass), N=2^15, 2 polys x 17 limbs (QP ciphertext)"})
+    for k in ("launch_pairs", "limbs", "avg_pair_us", "avg_limbs_per_pair", "ntt_event_ms", "evaluation_ms"):
+        r.pop(k)
     return {"metric": "forward NTT launch pair, walsh15 QP ciphertext (2 polys x 17 limbs, N=2^15)",
-            "value": 1e6 / r["launch_pair_us"], "unit": "launch-pairs/s", "n_gpus": 1, "steps": 50, "warmup": 3,
-            "ms_per_step": r["launch_pair_us"] / 1e3, "higher_is_better": True, "scaling": "weak",
+            "value": 1e6 / us, "unit": "launch-pairs/s", "n_gpus": 1, "steps": 50, "warmup": 3,
+            "ms_per_step": us / 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64 (exact mod-q integer)", "data": "synthetic (seeded residues)",
             "config": {"workload": "forward NTT, N=2^15, 34 limbs"}, "roofline": r, "cpu_baseline": None}
 
 
-def aes_ntt_roofline(M, torch, P):
+def aes_ntt_roofline(M, torch, P, probe):
     """The NTT passes are the largest kernel family of the AES evaluation
     (about 40 % of its kernel time; the rest is spread over key-switch inner
-    products, element-wise ops, encoding FFT stages and copies).  Measured
-    here with HIP events on their stream, right after the timed region: the
-    forward NTT (column + row pass) of one ciphertext over QP at the top
-    level, 2 polys x 17 limbs at N=2^15, the shape of the key switches'
-    ModUp / ModDown transforms; 16 N bytes per limb algorithmically."""
+    products, element-wise ops, encoding FFT stages and copies).  `probe` is
+    what the context's NTT probe measured over one whole evaluation: HIP
+    events on each launch pair's own stream around every forward and inverse
+    NTT (column + row pass; standalone, inside key switches, rescales,
+    ModRaise and the linear transforms), 16 N bytes per limb algorithmically
+    (each limb read and written once).  achieved = algorithmic bytes of all
+    pairs / their summed event time.  For comparison the forward NTT of one
+    QP ciphertext at the top level alone (2 polys x 17 limbs, N=2^15, the
+    key switches' ModUp / ModDown shape) is timed the same way."""
+    ntt_ms, pairs, ntt_bytes, ntt_limbs, probe_s = probe
     N = P.getN()
     q, p = [int(x) for x in P.getModuli()], [int(x) for x in P.getPModuli()]
     ctx = M.Context(N, q, p, P.getDnum())
@@ -884,20 +934,50 @@ def aes_ntt_roofline(M, torch, P):
     e1.record(stream)
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
-    algo = 16 * N * 2 * limbs
-    achieved = algo / (us / 1e6)
+    achieved = ntt_bytes / (ntt_ms / 1e3)
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": _profile_traffic("ntt15_qp_forward"),
-            "kernel": "forward NTT (column + row pass), N=2^15, 2 polys x 17 limbs (QP ciphertext)",
-            "launch_pair_us": us}
+            "kernel": "NTT launch pairs (column + row pass) inside one walsh15 AES-CTR evaluation, N=2^15",
+            "launch_pairs": int(pairs), "limbs": int(ntt_limbs),
+            "avg_pair_us": 1e3 * ntt_ms / max(1.0, pairs), "avg_limbs_per_pair": ntt_limbs / max(1.0, pairs),
+            "ntt_event_ms": ntt_ms, "evaluation_ms": 1e3 * probe_s,
+            "isolated_qp_pair_us": us, "isolated_qp_pair_frac": 16 * N * 2 * limbs / (us / 1e6) / HBM_PEAK}
 
 
 def cpu_baseline_aes(H, P, blocks):
-    """The oracle's product sum of one Walsh output pair at the walsh15 shape
-    (signs at level 2).  One AES evaluation of `blocks` blocks holds 640 such
-    pairs (10 rounds x 16 S-boxes x 4) besides its 164 bootstraps, which the
-    sample leaves out, so blocks x rate / 640 bounds the CPU port's blocks/s
-    from above."""
+    """The reference's own CPU path, built from its sources by oracle/Makefile
+    (oracle/_ref/ref_harness, mode sbox15time): whole walsh15 S-boxes (parity
+    packing, the complex-packed bootstrap with its degree-100 EvalMod, hoisted
+    unpacking, Walsh product sums, real/imaginary split) through the
+    reference's evaluator, `threads` of them at once, one per OpenMP thread,
+    as its sub_bytes runs the S-boxes of a round (ckks_aes_ctr_walsh_sbox.cpp:
+    659-721).  One 1024-block evaluation holds 160 S-boxes and 4 output
+    refreshes (bootstraps, counted as S-boxes), so blocks/s = blocks x
+    threads / (164 x wall).  The EvalMod coefficients come from this
+    package's restated fit (the reference fits them with MPFR, absent here).
+    Without the built harness: the oracle's Walsh product sum alone."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if os.path.exists(harness):
+        import subprocess
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            coeffs = os.path.join(d, "evalmod_coeffs.bin")
+            np.asarray(H.default_evalmod_polynomial(P), np.float64).tofile(coeffs)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            r = subprocess.run([harness, "sbox15time", coeffs, str(threads)], capture_output=True, text=True,
+                               timeout=600, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference S-box timing failed: {r.stderr[-2000:]}")
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        return {"value": blocks * t["concurrent_sboxes"] / (164 * t["wall_s"]), "unit": "blocks/s",
+                "cores": t["omp_threads"], "kind": "reference",
+                "sample": f"{t['concurrent_sboxes']} whole walsh15 S-boxes at once on {t['omp_threads']} OpenMP "
+                          f"threads through the reference's evaluator: {t['wall_s']:.2f} s wall "
+                          f"({t['sbox_mean_s']:.2f} s per S-box: bootstrap linear {t['bootstrap_linear_s']:.2f}, "
+                          f"EvalMod {t['evalmod_s']:.2f}, Walsh products {t['walsh_products_s']:.2f}, unpack "
+                          f"{t['unpack_s']:.2f}, packing {t['parity_pack_s']:.2f}, split {t['split_s']:.2f}); "
+                          "scaled by the 164 S-box-sized steps of one 1024-block evaluation"}
     N = P.getN()
     q, p = [int(x) for x in P.getModuli()], [int(x) for x in P.getPModuli()]
     table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
@@ -1009,11 +1089,13 @@ def main():
     ap.add_argument("--workload", default="aes", choices=["ntt", "keyswitch", "lt", "chain", "hoisted", "walsh", "sbox", "aes", "serve", "ntt_qp"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--requests", type=int, default=240, help="serve: number of requests")
-    ap.add_argument("--lanes", type=int, default=1, help="serve: evaluation lanes")
-    ap.add_argument("--lane-contexts", type=int, default=16, help="serve: contexts per lane")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="aes: requests of the short served leg after the timed region (0: none)")
+    ap.add_argument("--lanes", type=int, default=2, help="serve: evaluation lanes (sharing one key set)")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrival rate (requests/s)")
-    ap.add_argument("--aes-contexts", type=int, default=16,
-                    help="aes: contexts (host threads and streams) the S-boxes of a round are spread over")
+    ap.add_argument("--aes-contexts", type=int, default=1,
+                    help="aes: 1 (default) evaluates the 16 S-boxes of a round as one ciphertext batch; more "
+                         "spreads them over that many contexts (host threads and streams) one at a time")
     ap.add_argument("--no-plain", dest="plain", action="store_false",
                     help="skip the one-limb-per-launch comparison (profiling runs)")
     ap.add_argument("--eager", action="store_true",

@@ -69,6 +69,19 @@ int mi_free(mi_context* ctx, void* ptr);
 int mi_memory_trim(mi_context* ctx);
 int mi_memory_stats(mi_context* ctx, int64_t* stats);
 int mi_set_memory_limit(mi_context* ctx, int64_t bytes);
+/* NTT probe (measurement only; no counterpart in the reference): between
+ * mi_probe_begin and mi_probe_end every forward or inverse NTT launch pair
+ * of the context (standalone, or inside key switches, rescales, ModRaise and
+ * linear transforms) is bracketed by HIP events on its stream.  mi_probe_end
+ * waits for them and returns stats[0] = summed event time of the pairs (ms),
+ * stats[1] = launch pairs, stats[2] = algorithmic bytes (16 N per limb: each
+ * limb read and written once), stats[3] = limbs transformed. */
+int mi_probe_begin(mi_context* ctx);
+int mi_probe_end(mi_context* ctx, double* stats);
+/* Free and total bytes of the context's device (hipMemGetInfo): what the
+ * serving loop sizes its lanes against.  No counterpart in the reference,
+ * whose data lives in host memory. */
+int mi_device_memory_info(mi_context* ctx, int64_t* free_bytes, int64_t* total_bytes);
 enum { MI_COPY_H2D = 0, MI_COPY_D2H = 1, MI_COPY_D2D = 2 };
 /* Host copies complete before the call returns; device-to-device copies are
  * stream-ordered. */
@@ -109,6 +122,15 @@ int mi_tensor(mi_context* ctx, int level, uint64_t* b1, uint64_t* a1, const uint
               const uint64_t* a2, uint64_t* c2, void* stream);
 /* ckks_square_inplace, ckks_mul.cpp:37-88 */
 int mi_square(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2, void* stream);
+/* ckks_mul_inplace(ct, ct) / ckks_square_inplace (ckks_mul.cpp:37-157) of
+ * `count` listed ciphertext pairs at one level, one launch per 16 pairs: for
+ * each k, (b1, a1, c2)[k] as mi_tensor (b2/a2 non-null) or mi_square (b2 and
+ * a2 null).  The reference evaluates such products one ciphertext at a time
+ * on OpenMP threads; here a batch of ciphertexts that share their metadata
+ * (the same schedule applied to independent inputs) is one launch.  No two
+ * output polys may overlap. */
+int mi_tensor_list(mi_context* ctx, int level, int count, uint64_t* const* b1, uint64_t* const* a1,
+                   const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* c2, void* stream);
 /* ckks_mul_inplace(ct, pt), ckks_mul.cpp:160-212 (pt_montgomery: plaintext stored x 2^64) */
 int mi_mul_plain(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
                  const uint64_t* pt, int pt_montgomery, void* stream);
@@ -355,7 +377,21 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
  * reference's sequence of additions. */
 int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out, int nterms,
                       const uint64_t* const* in, const uint32_t* coef, void* stream);
-enum { MI_LIST_COPY = 0, MI_LIST_ADD = 1, MI_LIST_SUB = 2, MI_LIST_MUL_SCALAR = 3, MI_LIST_MUL_SCALAR_ADD = 4 };
+/* Also MI_LIST_ADD_SCALAR: x[k] + r (ckks_add_integer_constant_inplace,
+ * ckks_addition.cpp:89-112, on the b polys of a batch; y unused),
+ * MI_LIST_ZERO: x[k] = 0 (y and residues unused), MI_LIST_ADD_PLAIN_MONT:
+ * x[k] + REDC(y[k]) (ckks_add_inplace(ct, pt) of a Montgomery-form plaintext,
+ * ckks_addition.cpp:35-60). */
+enum {
+    MI_LIST_COPY = 0,
+    MI_LIST_ADD = 1,
+    MI_LIST_SUB = 2,
+    MI_LIST_MUL_SCALAR = 3,
+    MI_LIST_MUL_SCALAR_ADD = 4,
+    MI_LIST_ADD_SCALAR = 5,
+    MI_LIST_ZERO = 6,
+    MI_LIST_ADD_PLAIN_MONT = 7
+};
 int mi_poly_list_op(mi_context* ctx, int op, int level, int npolys, uint64_t* const* x, const uint64_t* const* y,
                     const uint64_t* residues, void* stream);
 

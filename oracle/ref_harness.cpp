@@ -8,6 +8,11 @@
 // Inputs the harness chooses are drawn from splitmix64 so the tests can
 // regenerate them; the reference's own randomness (keys, encryption masks) is
 // captured as data.
+#include <algorithm>
+#include <array>
+#include <cstdlib>
+#include <optional>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +34,9 @@
 #include "ckks_linear_transform.hpp"
 #include "ckks_modraise.hpp"
 #include "ckks_evalpoly.hpp"
+#include "ckks_s2c_generalized.hpp"
+#include "ckks_c2s_generalized.hpp"
+#include "ckks_evalmod_scale_plan.hpp"
 #include "ntt.hpp"
 #include "random.hpp"
 
@@ -466,6 +474,116 @@ static void run_bootstrap_pieces() {
     dumpv(tag + "_s2c_meta", meta);
 }
 
+// The scaled C2S plan of a context's stages: metadata and QP diagonals.
+static void dump_plan(const std::string& tag, const CKKSLinearTransformPlan& plan) {
+    std::vector<uint64_t> meta;
+    for (const auto& st : plan.stages) {
+        meta.push_back(st.diagonal_indices.size());
+        meta.push_back(st.baby_step_count);
+        meta.push_back(st.qp_limb_count);
+        meta.insert(meta.end(), st.diagonal_indices.begin(), st.diagonal_indices.end());
+        std::vector<uint64_t> flat;
+        for (const auto& d : st.diagonal_qp_plaintexts) flat.insert(flat.end(), d.begin(), d.end());
+        dumpv(tag + "_stage" + std::to_string(&st - plan.stages.data()), flat);
+    }
+    dumpv(tag + "_meta", meta);
+}
+
+// The bootstrap driver ckks_batch_bootstrap_evalmod_from_complex_packed
+// (ckks_bootstrap_evalmod_inplace.cpp:354-434) as its sequence of public
+// functions, on a bootstrapping context whose EvalMod polynomial is a fixed
+// harness-chosen one: the context is built without EvalMod limbs (the
+// polynomial fit needs MPFR), the mult limbs carry the Chebyshev evaluation,
+// and evalmod_binary_inplace (:306-326) is restated with those coefficients.
+// Dumps the keys the sequence uses, its input, each step's output and the
+// S2C / scaled C2S plans.
+static void run_bootstrap_sequence() {
+    CKKSParamsConfig c;
+    c.N = size_t{1} << 10;
+    c.mult_depth = 3;
+    c.q0_bit = 40;
+    c.q_bit = 35;
+    c.p_bits = {60, 60, 60};
+    c.dnum = 2;
+    c.secret_key_hamming_weight = 192;
+    c.sigma = 3.2;
+    c.secure = false;
+    c.bts_keys = true;
+    auto bp = ckks_params_detail::make_evalmod_bootstrap_params(
+        16, {CKKSBootstrapDftScaleSchedule{{25}, {25}, {25}}, CKKSBootstrapDftScaleSchedule{{25}, {25}, {25}}},
+        0, 0, {0, 0});
+    bp.linear_transform_use_qp = {true, true};
+    c.bootstrap_params = bp;
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    const size_t N = P.getN(), L = P.getMaxLevel();
+    const std::string tag = "bs10";
+    dumpv(tag + "_q", P.getModuli());
+    dumpv(tag + "_p", P.getPModuli());
+    dump_key(tag + "_relin", ctx.getRelinHybrid(L));
+    dump_key(tag + "_conj", ctx.getConjugationKey());
+    dump_key(tag + "_d2s", ctx.getDenseToSparseSwitchKey());
+    dump_key(tag + "_s2d", ctx.getSparseToDenseSwitchKey());
+    // the rotation keys of both plans' baby and giant steps
+    std::vector<uint64_t> rots;
+    for (const CKKSLinearTransformPlan* plan : {&ctx.getEvalModS2CPlan(), &ctx.getEvalModScaledC2SPlan()})
+        for (const auto& st : plan->stages)
+            for (size_t d : st.diagonal_indices)
+                for (size_t r : {d % st.baby_step_count, (d / st.baby_step_count) * st.baby_step_count})
+                    if (r != 0 && std::find(rots.begin(), rots.end(), (uint64_t)r) == rots.end()) rots.push_back(r);
+    std::sort(rots.begin(), rots.end());
+    dumpv(tag + "_rots", rots);
+    for (uint64_t r : rots) dump_key(tag + "_rot" + std::to_string(r), ctx.getRotationKey((int64_t)r));
+    dump_plan(tag + "_s2c", ctx.getEvalModS2CPlan());
+    dump_plan(tag + "_c2s", ctx.getEvalModScaledC2SPlan());
+    double scales[2] = {ctx.getEvalModScaledC2STransformScale(), std::exp2((double)ckks_evalmod_log_scale_bits(P))};
+    dump(tag + "_scales", scales, sizeof(scales));
+    // complex-packed bits at the S2C level and the default scale (input scale ratio 1)
+    uint64_t seed = 777;
+    MyVector<Complex> m(N / 2);
+    for (auto& z : m)
+        z = Complex((double)(splitmix64(seed) & 1), (double)(splitmix64(seed) & 1));
+    dumpv(tag + "_m", m);
+    auto packed = ckks_encrypt(ctx, m, P.getBootstrapS2CDepth());
+    dump_ct(tag + "_in", packed);
+    // evalmod_s2c_for_input_scale (:156-178) with scale ratio 1
+    auto s2c = ckks_s2c_generalized(ctx, packed, P.getBootstrapS2CDepth());
+    s2c.setScale(P.getScale());
+    dump_ct(tag + "_s2c_out", s2c);
+    // finish_batch_evalmod_from_s2c_coeff_packed (:354-389)
+    ckks_key_switch_dense_to_sparse_inplace(ctx, s2c);
+    dump_ct(tag + "_d2s_out", s2c);
+    ckks_modraise_inplace(ctx, s2c, P.getDepth());
+    dump_ct(tag + "_modraise_out", s2c);
+    ckks_key_switch_sparse_to_dense_inplace(ctx, s2c);
+    dump_ct(tag + "_s2d_out", s2c);
+    auto c2s = ckks_c2s_generalized(ctx, s2c, ctx.getEvalModScaledC2STransformScale());
+    dump_ct(tag + "_c2s_out", c2s);
+    auto conjugated = ckks_conjugate(ctx, c2s);
+    auto out0 = c2s.clone();
+    ckks_add_inplace(ctx, out0, conjugated);
+    out0.setScale(c2s.getScale());
+    auto out1 = c2s.clone();
+    ckks_sub_inplace(ctx, out1, conjugated);
+    out1 = ckks_mul(ctx, out1, Complex(0.0, -1.0), 1.0);
+    out1.setScale(c2s.getScale());
+    dump_ct(tag + "_split0", out0);
+    dump_ct(tag + "_split1", out1);
+    // an even degree-6 polynomial in place of the fitted one (three levels)
+    std::vector<double> co(7, 0.0);
+    for (size_t i = 0; i <= 6; i += 2) co[i] = ((double)(splitmix64(seed) >> 11) * 0x1.0p-53 - 0.5) / (double)(i + 1);
+    dumpv(tag + "_coeffs", co);
+    int k = 0;
+    for (CKKSCiphertext* ct : {&out0, &out1}) {
+        CKKSEvalPolyOptions opt;
+        opt.target_scale = P.getScale();
+        ct->setScale(std::exp2((double)ckks_evalmod_log_scale_bits(P)));
+        *ct = ckks_evalchebyshev_preconditioned(ctx, *ct, co, opt);
+        ct->setScale(P.getScale());
+        dump_ct(tag + "_out" + std::to_string(k++), *ct);
+    }
+}
+
 // Chebyshev evaluation (Paterson-Stockmeyer with exact-scale tracking) of
 // harness-chosen coefficients on an encrypted input in [-1, 1].
 static void run_evalpoly() {
@@ -512,7 +630,205 @@ static void run_evalpoly() {
     }
 }
 
+// ---------------------------------------------------------------------------
+// CPU timing of one walsh15 S-box (ckks_walsh_sbox_complex_blocks_mask_packed,
+// ckks_walsh_sbox.cpp) through the reference's own evaluator on the host's
+// OpenMP threads: the CPU baseline of the AES evaluation (160 S-boxes and 4
+// output refreshes per 1024 blocks).  The reference fits its EvalMod
+// polynomial with MPFR (poly_approx.cpp), which this image lacks, so the
+// context is AESWalsh15's with the seven 38-bit EvalMod limbs declared as
+// mult limbs: the modulus chain, keys, plans and every operation's shape are
+// those of AESWalsh15 (checked below), and the degree-100 polynomial (its
+// coefficients read from a file) is evaluated on those limbs with
+// ckks_evalchebyshev_preconditioned as evalmod_binary_inplace does.  The
+// S-box's private steps (parity packing, hoisted unpacking, the Walsh product
+// sums) are composed from the same public calls they make.
+#include <chrono>
+#include <fstream>
+#include <omp.h>
+
+static double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Eight encrypted bit ciphertexts at the S2C entry level (512 blocks per lane).
+static MyVector<CKKSCiphertext> sbox_input(const CKKSContext& ctx, uint64_t seed) {
+    const auto& P = ctx.getParams();
+    MyVector<CKKSCiphertext> bits;
+    for (int k = 0; k < 8; ++k) {
+        MyVector<Complex> m(P.getSlots(), Complex(0.0, 0.0));
+        for (size_t j = 0; j < 512; ++j)
+            m[j] = Complex((double)(splitmix64(seed) & 1), (double)(splitmix64(seed) & 1));
+        bits.push_back(ckks_encrypt(ctx, m, P.getBootstrapS2CDepth()));
+    }
+    return bits;
+}
+
+// One S-box; t[0..5] = the seconds of its steps.
+static void time_one_sbox(const CKKSContext& ctx, const std::vector<double>& coeffs,
+                          const MyVector<CKKSCiphertext>& bits, double* t) {
+    const auto& P = ctx.getParams();
+    const size_t slots = P.getSlots(), bc = 512, level = P.getBootstrapS2CDepth();
+    auto t0 = std::chrono::steady_clock::now();
+    // parity packing: 30 parity sums, each rotated to its segment and added
+    std::optional<CKKSCiphertext> packed;
+    for (size_t half = 0; half < 2; ++half)
+        for (size_t mask = 1; mask < 16; ++mask) {
+            std::optional<CKKSCiphertext> sum;
+            for (size_t b = 0; b < 4; ++b)
+                if ((mask >> b) & 1) {
+                    if (!sum) sum.emplace(bits[4 * half + b].clone());
+                    else ckks_add_inplace(ctx, *sum, bits[4 * half + b]);
+                }
+            const size_t segment = half * 15 + mask - 1;
+            if (segment) ckks_rotate_inplace(ctx, *sum, (int64_t)(segment * bc));
+            if (!packed) packed.emplace(std::move(*sum));
+            else ckks_add_inplace(ctx, *packed, *sum);
+        }
+    t[0] = seconds_since(t0);
+    // the complex-packed bootstrap
+    t0 = std::chrono::steady_clock::now();
+    auto s2c = ckks_s2c_generalized(ctx, *packed, level);
+    s2c.setScale(P.getScale());
+    ckks_key_switch_dense_to_sparse_inplace(ctx, s2c);
+    ckks_modraise_inplace(ctx, s2c, P.getDepth());
+    ckks_key_switch_sparse_to_dense_inplace(ctx, s2c);
+    auto c2s = ckks_c2s_generalized(ctx, s2c, ctx.getEvalModScaledC2STransformScale());
+    auto conjugated = ckks_conjugate(ctx, c2s);
+    CKKSCiphertext lanes[2] = {c2s.clone(), c2s.clone()};
+    ckks_add_inplace(ctx, lanes[0], conjugated);
+    ckks_sub_inplace(ctx, lanes[1], conjugated);
+    lanes[1] = ckks_mul(ctx, lanes[1], Complex(0.0, -1.0), 1.0);
+    t[1] = seconds_since(t0);
+    t0 = std::chrono::steady_clock::now();
+    for (auto& ct : lanes) {
+        CKKSEvalPolyOptions opt;
+        opt.target_scale = P.getScale();
+        ct.setScale(std::exp2((double)ckks_evalmod_log_scale_bits(P)));
+        ct = ckks_evalchebyshev_preconditioned(ctx, ct, coeffs, opt);
+        ct.setScale(P.getScale());
+        ckks_mul_inplace(ctx, ct, int64_t{-2});  // bit -> sign
+        ckks_add_integer_constant_inplace(ctx, ct, int64_t{1});
+    }
+    t[2] = seconds_since(t0);
+    std::vector<CKKSCiphertext> outputs;
+    for (auto& lane : lanes) {
+        // hoisted unpacking of the 30 segments
+        t0 = std::chrono::steady_clock::now();
+        CKKSHoistWorkspace ws;
+        CKKSEvalHoistedCiphertextQP hoisted;
+        ckks_hoist_ciphertext_a_qp_eval(ctx, lane, ws, hoisted);
+        std::vector<CKKSCiphertext> signs;  // [0..14] low masks 1..15, [15..29] high
+        const size_t N = P.getN(), L = lane.getLevel() + 1;
+        for (size_t segment = 0; segment < 30; ++segment) {
+            if (segment == 0) { signs.push_back(lane.clone()); continue; }
+            const int64_t r = ((-(int64_t)(segment * bc)) % (int64_t)slots + (int64_t)slots) % (int64_t)slots;
+            MyVector<uint64_t> b = lane.getB();
+            MyVector<uint64_t> a(N * L, 0);
+            ckks_hybrid_key_switch_hoisted_inplace(ctx, hoisted, ctx.getRotationKey(r), b, a,
+                                                   CKKSHybridKeySwitchCombine::Add,
+                                                   CKKSHybridKeySwitchCombine::Assign, P.montgomery,
+                                                   ctx.getRotationNttMap(r));
+            signs.emplace_back(N, std::move(a), std::move(b), lane.getScale(), lane.getLevel(),
+                               lane.getSecretOwner(), lane.getMessageEncodingState());
+        }
+        t[3] += seconds_since(t0);
+        // four output pairs: per high mask, the low signs' integer
+        // combination times that high sign, summed raw, one relin, rescale
+        t0 = std::chrono::steady_clock::now();
+        for (int pair = 0; pair < 4; ++pair) {
+            std::optional<CKKSCiphertext> raw;
+            for (size_t h = 0; h < 15; ++h) {
+                std::optional<CKKSCiphertext> comb;
+                for (size_t m = 0; m < 15; ++m) {
+                    auto term = signs[m].clone();
+                    ckks_mul_inplace(ctx, term, (int64_t)(((h * 7 + m * 3 + pair) % 5) * 2 - 4));
+                    if (!comb) comb.emplace(std::move(term));
+                    else ckks_add_inplace(ctx, *comb, term);
+                }
+                ckks_mul_inplace(ctx, *comb, signs[15 + h]);
+                if (!raw) raw.emplace(std::move(*comb));
+                else ckks_add_inplace(ctx, *raw, *comb);
+            }
+            ckks_relin_hybrid_inplace(ctx, *raw);
+            ckks_rescale(ctx, *raw);
+            outputs.push_back(std::move(*raw));
+        }
+        t[4] += seconds_since(t0);
+    }
+    // real/imaginary split of the eight outputs (one conjugation each) and recombination
+    t0 = std::chrono::steady_clock::now();
+    for (auto& o : outputs) {
+        auto conj = ckks_conjugate(ctx, o);
+        auto re = o.clone();
+        ckks_add_inplace(ctx, re, conj);
+        auto im = o.clone();
+        ckks_sub_inplace(ctx, im, conj);
+        im = ckks_mul(ctx, im, Complex(0.0, -1.0), 1.0);
+        im.setScale(re.getScale());
+        ckks_add_inplace(ctx, re, im);
+    }
+    t[5] = seconds_since(t0);
+}
+
+static void run_sbox15_timing(const char* coeff_path, int concurrent) {
+    std::vector<double> coeffs;
+    {
+        std::ifstream f(coeff_path, std::ios::binary);
+        double v;
+        while (f.read(reinterpret_cast<char*>(&v), sizeof v)) coeffs.push_back(v);
+    }
+    if (coeffs.size() < 2) { std::fprintf(stderr, "sbox15time: no coefficients in %s\n", coeff_path); std::exit(2); }
+    const auto t_setup = std::chrono::steady_clock::now();
+    CKKSParamsConfig c = ckks_aes_params_detail::AESWalsh15Config();
+    const CKKSParams original(c);
+    auto bp = *c.bootstrap_params;
+    const size_t evalmod_limbs = bp.evalmod_depth;
+    std::vector<uint64_t> mult_bits = c.q_mult_bits;
+    for (size_t i = 0; i < evalmod_limbs; ++i) mult_bits.push_back(bp.evalmod_bit_size);
+    bp.evalmod_depth = 0;
+    c.bootstrap_params = bp;
+    c.q_mult_bits = mult_bits;
+    c.mult_depth = mult_bits.size();
+    CKKSContext ctx{CKKSParams(c)};
+    const auto& P = ctx.getParams();
+    if (P.getModuli() != original.getModuli() || P.getPModuli() != original.getPModuli()) {
+        std::fprintf(stderr, "sbox15time: the modulus chain differs from AESWalsh15's\n");
+        std::exit(3);
+    }
+    const double setup_s = seconds_since(t_setup);
+    // as sub_bytes_walsh_complex_blocks_mask_packed runs them (:659-721): the
+    // S-boxes of a round concurrently, one per OpenMP thread (nested regions
+    // inside each run on that thread alone)
+    omp_set_max_active_levels(1);
+    std::vector<std::array<double, 6>> steps(concurrent);
+    std::vector<MyVector<CKKSCiphertext>> inputs;
+    for (int i = 0; i < concurrent; ++i) inputs.push_back(sbox_input(ctx, 99 + (uint64_t)i));
+    const auto t_all = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < concurrent; ++i) {
+        double t[6] = {0, 0, 0, 0, 0, 0};
+        time_one_sbox(ctx, coeffs, inputs[i], t);
+        for (int k = 0; k < 6; ++k) steps[i][k] = t[k];
+    }
+    const double wall = seconds_since(t_all);
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (const auto& st : steps)
+        for (int k = 0; k < 6; ++k) t[k] += st[k] / concurrent;
+    std::printf("{\"concurrent_sboxes\": %d, \"wall_s\": %.6f, \"sbox_mean_s\": %.6f, \"parity_pack_s\": %.6f, "
+                "\"bootstrap_linear_s\": %.6f, \"evalmod_s\": %.6f, \"unpack_s\": %.6f, "
+                "\"walsh_products_s\": %.6f, \"split_s\": %.6f, \"setup_s\": %.3f, \"omp_threads\": %d, "
+                "\"limbs\": %zu}\n",
+                concurrent, wall, t[0] + t[1] + t[2] + t[3] + t[4] + t[5], t[0], t[1], t[2], t[3], t[4], t[5],
+                setup_s, omp_get_max_threads(), P.getModuli().size());
+    std::fflush(stdout);
+}
+
 int main(int argc, char** argv) {
+    if (argc == 4 && std::strcmp(argv[1], "sbox15time") == 0) {
+        run_sbox15_timing(argv[2], std::max(1, std::atoi(argv[3])));
+        return 0;
+    }
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness OUTDIR [case ...]\n"); return 2; }
     g_out = argv[1];
     mkdir(g_out.c_str(), 0755);
@@ -537,6 +853,7 @@ int main(int argc, char** argv) {
     if (want("ltq")) run_lt_q();
     if (want("boot")) run_bootstrap_pieces();
     if (want("evalpoly")) run_evalpoly();
+    if (want("bootseq")) run_bootstrap_sequence();
     std::printf("ref_harness: wrote %s\n", g_out.c_str());
     return 0;
 }

@@ -4,7 +4,7 @@ Run in a container where /root/reference exists:
     make -C oracle ref && oracle/_ref/ref_harness oracle/_ref/raw [case ...]
     python tests/golden/make_golden.py [--raw DIR] [group ...]
 
-Groups (all when none is named): primes ntt eval lt ltq boot evalpoly.  The
+Groups (all when none is named): primes ntt eval lt ltq boot evalpoly bootseq.  The
 harness draws keys and encryption masks from the reference's own
 unseedable randomness, so repack only the groups whose raw arrays were
 regenerated together.
@@ -170,9 +170,38 @@ def pack_evalpoly():
         json.dump(meta, f, indent=1)
 
 
+def plan_meta(tag):
+    meta = rd(tag + "_meta")
+    stages, i = [], 0
+    while i < len(meta):
+        n, bs, qpl = int(meta[i]), int(meta[i + 1]), int(meta[i + 2])
+        idx = [int(v) for v in meta[i + 3:i + 3 + n]]
+        stages.append({"baby_steps": bs, "qp_limbs": qpl, "diag_indices": idx,
+                       "digest": digest(rd(tag + "_stage%d" % len(stages)))})
+        i += 3 + n
+    return stages
+
+
+def pack_bootseq():
+    # the bootstrap driver's sequence (S2C, dense->sparse, ModRaise,
+    # sparse->dense, C2S, conjugate split, Chebyshev EvalMod) with its keys
+    rots = rd("bs10_rots")
+    arr = {"q": rd("bs10_q"), "p": rd("bs10_p"), "rots": rots, "key_relin": rd("bs10_relin"),
+           "key_conj": rd("bs10_conj"), "key_d2s": rd("bs10_d2s"), "key_s2d": rd("bs10_s2d"),
+           "ct_in": rd("bs10_in"), "coeffs": rd("bs10_coeffs", np.float64), "scales": rd("bs10_scales", np.float64)}
+    for r in rots:
+        arr["key_rot%d" % r] = rd("bs10_rot%d" % r)
+    meta = {"in": ct_meta("bs10_in"), "s2c_plan": plan_meta("bs10_s2c"), "c2s_plan": plan_meta("bs10_c2s")}
+    for step in ["s2c_out", "d2s_out", "modraise_out", "s2d_out", "c2s_out", "split0", "split1", "out0", "out1"]:
+        meta[step] = ct_meta("bs10_" + step)
+    np.savez_compressed(os.path.join(HERE, "bs10.npz"), **arr)
+    with open(os.path.join(HERE, "bs10_outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 GROUPS = {"primes": pack_primes, "ntt": pack_ntt, "eval": pack_eval, "lt": pack_lt, "ltq": pack_ltq,
           "boot": pack_boot,
-          "evalpoly": pack_evalpoly}
+          "evalpoly": pack_evalpoly, "bootseq": pack_bootseq}
 
 
 def main():

@@ -1,6 +1,8 @@
 """Host AES-128 (the plain reference the homomorphic CTR path is checked
 against), the walsh15 parameter set and the default EvalMod polynomial, on
 the CPU."""
+import os
+
 import numpy as np
 
 from helpers import host
@@ -55,16 +57,46 @@ def test_walsh15_parameters():
     assert P.getBootstrapS2CDepth() == 1 and P.getBootstrapEvalModK() == 11
 
 
-def test_default_evalmod_polynomial_is_the_chebyshev_interpolant():
+def test_default_evalmod_polynomial_fit_and_fallback():
+    """walsh15's default EvalMod polynomial is the reference's discrete fit
+    (reproduces the mod-2 map near every point k/(2K) the bootstrap meets);
+    parameters outside the presets fall back to the degree-d Chebyshev
+    interpolant of the same target on [-1, 1]."""
     H = host()
     P = H.CKKSParams.AESWalsh15()
     K = P.getBootstrapEvalModK()
     c = np.array(H.default_evalmod_polynomial(P))
-    want = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), 100)
+    assert H.evalmod_polynomial_is_reference_fit(P) and c.shape == (100,)
+    u = np.concatenate([(np.arange(-2 * K + 1, 2 * K) / 2 + d) / K for d in (-0.2, 0.0, 0.2)])
+    np.testing.assert_allclose(np.polynomial.chebyshev.chebval(u, c), 0.5 - 0.5 * np.cos(2 * np.pi * K * u), atol=1e-8)
+    cfg = H.aes_walsh15_config()
+    bp = cfg.bootstrap_params
+    bp.evalmod_polynomial_degree = 90
+    cfg.bootstrap_params = bp
+    P2 = H.CKKSParams(cfg)
+    assert not H.evalmod_polynomial_is_reference_fit(P2)
+    c2 = np.array(H.default_evalmod_polynomial(P2))
+    want = np.polynomial.chebyshev.chebinterpolate(lambda x: 0.5 - 0.5 * np.cos(2 * np.pi * K * x), 90)
     want[1::2] = 0.0
-    assert c.shape == (101,)
-    np.testing.assert_allclose(c, want, atol=1e-12)
-    # near every half-integer / K (parity 1) and integer / K (parity 0) the fit is the mod-2 map
-    u = np.concatenate([(np.arange(-K, K + 1) + d) / K for d in (-0.02, 0.0, 0.02)])
-    u = u[np.abs(u) <= 1]
-    np.testing.assert_allclose(np.polynomial.chebyshev.chebval(u, c), 0.5 - 0.5 * np.cos(2 * np.pi * K * u), atol=1e-6)
+    assert c2.shape == (91,)
+    np.testing.assert_allclose(c2, want, atol=1e-12)
+
+
+def test_xboot_presets_reproduce_reference_chains():
+    """The XBOOT presets (ckks_aes_params.hpp:12-211) give the reference's
+    own prime chains (the harness dumped them from CKKSParams::XBoot*), and
+    their EvalMod settings select the reference's fitted polynomial."""
+    import json
+    from helpers import GOLDEN
+    H = host()
+    with open(os.path.join(GOLDEN, "primes.json")) as f:
+        primes = json.load(f)
+    for tag, make in [("xboot13", H.CKKSParams.XBootParamAes13Reference), ("xboot14", H.CKKSParams.AESXBoot14Secure),
+                      ("xboot15", H.CKKSParams.AESXBoot15)]:
+        P = make()
+        assert [int(v) for v in P.getModuli()] == primes[tag]["q"], tag
+        assert [int(v) for v in P.getPModuli()] == primes[tag]["p"], tag
+        assert H.evalmod_polynomial_is_reference_fit(P), tag
+    assert H.CKKSParams.AESXBoot14SecureSparse1024().getBootstrapActiveSlotCount() == 1024
+    assert H.CKKSParams.AESXBoot15Sparse1024().getBootstrapActiveSlotCount() == 1024
+    assert H.CKKSParams.AESXBoot15().getBootstrapActiveSlotCount() == 1 << 14

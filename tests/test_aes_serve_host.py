@@ -11,13 +11,14 @@ import aes_serve  # noqa: E402
 
 def test_request_shape_checks():
     ok = np.zeros((3, 16), np.uint8)
-    assert aes_serve.Request(ok, ok).blocks == 3
+    c, x = aes_serve.check_request(ok, ok)
+    assert c.shape == (3, 16)
     with pytest.raises(ValueError):
-        aes_serve.Request(np.zeros((0, 16), np.uint8), np.zeros((0, 16), np.uint8))
+        aes_serve.check_request(np.zeros((0, 16), np.uint8), np.zeros((0, 16), np.uint8))
     with pytest.raises(ValueError):
-        aes_serve.Request(np.zeros((33, 16), np.uint8), np.zeros((33, 16), np.uint8))
+        aes_serve.check_request(np.zeros((1025, 16), np.uint8), np.zeros((1025, 16), np.uint8))
     with pytest.raises(ValueError):
-        aes_serve.Request(np.zeros((2, 16), np.uint8), np.zeros((3, 16), np.uint8))
+        aes_serve.check_request(np.zeros((2, 16), np.uint8), np.zeros((3, 16), np.uint8))
 
 
 def test_values_to_bytes_reads_bit_b_of_byte_k_at_8k_plus_b():
@@ -29,15 +30,10 @@ def test_values_to_bytes_reads_bit_b_of_byte_k_at_8k_plus_b():
 
 
 def test_batches_are_fifo_whole_requests_within_one_evaluation():
-    from collections import deque
-    srv = aes_serve.AesCtrServer.__new__(aes_serve.AesCtrServer)   # packing only: no lanes, no device
-    sizes = [32, 500, 400, 100, 7, 32]
-    srv.queue = deque(aes_serve.Request(np.zeros((min(n, 32), 16), np.uint8), np.zeros((min(n, 32), 16), np.uint8))
-                      for n in sizes)
-    # requests hold at most 32 blocks: emulate larger ones by their block counts
-    for r, n in zip(srv.queue, sizes):
-        r.counters = np.zeros((n, 16), np.uint8)
-    batch = srv._take_batch()
-    assert [r.blocks for r, _ in batch] == [32, 500, 400]          # the 100-block request would overflow 1024
-    assert [off for _, off in batch] == [0, 32, 532]
-    assert [r.blocks for r in srv.queue] == [100, 7, 32]           # FIFO: nothing jumps the queue
+    """The native server's packing rule (ckks_aes_serve_take): requests are
+    taken first come first served, whole, while they fit 1024 blocks."""
+    H = aes_serve._host()
+    take = H.ckks_aes_serve_take([32, 500, 400, 100, 7, 32], 1024)
+    assert list(take) == [0, 32, 532]          # the 100-block request would overflow: it waits
+    assert list(H.ckks_aes_serve_take([1024, 1], 1024)) == [0]
+    assert list(H.ckks_aes_serve_take([], 1024)) == []

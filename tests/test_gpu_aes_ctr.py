@@ -86,27 +86,36 @@ def test_aes_ctr_rejects_bad_block_spans():
         H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), 513)
 
 
-def test_aes_ctr_concurrent_sboxes_are_bit_exact():
-    """The 16 S-boxes of every round spread over the main context and two
-    more built from the same parameters and seed (one host thread and stream
-    each), with MixColumns + AddRoundKey fused into one linear combination:
-    every limb of the 128 output bits equals the run that follows the
-    reference's per-operation sequence on one stream."""
+def test_aes_ctr_batched_and_concurrent_sboxes_are_bit_exact():
+    """Three forms of the same decryption, every limb of the 128 output bits
+    equal: the plain one (S-boxes one after another on one stream, the
+    reference's per-operation order for the linear layer), the batched one
+    (the 16 S-boxes of each round and the final-refresh packs as one
+    ciphertext batch, MixColumns + AddRoundKey as one linear combination)
+    and the concurrent one (S-boxes over the main context and two more built
+    from the same parameters, one host thread and stream each)."""
     H = host()
-    ctxs = [H.CKKSContext(H.CKKSParams.AESWalsh15()) for _ in range(3)]
+    params = H.CKKSParams.AESWalsh15()
+    ctxs = [H.CKKSContext(params) for _ in range(3)]
     ctx = ctxs[0]
     cb = ctx.getParams().getAesWalshBlockCount()
     counters, plains, cts = make_blocks(H, 2 * cb)
     keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), cb)
-    seq = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, fuse_linear_layer=False)
-    par = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=ctxs[1:])
-    # the reference's per-operation order on one stream vs concurrent S-boxes and the fused linear layer
-    for bit in range(128):
-        assert par[bit].getLevel() == seq[bit].getLevel() and par[bit].getScale() == seq[bit].getScale()
-        for k in range(2):
-            assert np.array_equal(np.asarray(par[bit].getPoly(k)), np.asarray(seq[bit].getPoly(k))), (bit, k)
-    mismatches, max_err, _ = validate(H, ctx, par, plains, cb)
-    print(f"concurrent S-boxes: mismatches {mismatches} max error {max_err:.3e}", flush=True)
+    print("plain", flush=True)
+    seq = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, fuse_linear_layer=False,
+                                                   batch_sboxes=False)
+    print("batched", flush=True)
+    bat = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts)
+    print("concurrent", flush=True)
+    par = H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=ctxs[1:],
+                                                   batch_sboxes=False)
+    for other in (bat, par):
+        for bit in range(128):
+            assert other[bit].getLevel() == seq[bit].getLevel() and other[bit].getScale() == seq[bit].getScale()
+            for k in range(2):
+                assert np.array_equal(np.asarray(other[bit].getPoly(k)), np.asarray(seq[bit].getPoly(k))), (bit, k)
+    mismatches, max_err, _ = validate(H, ctx, bat, plains, cb)
+    print(f"batched S-boxes: mismatches {mismatches} max error {max_err:.3e}", flush=True)
     assert mismatches == 0
 
 
@@ -119,4 +128,10 @@ def test_aes_ctr_rejects_a_context_with_other_keys():
     keys = H.ckks_aes128_encrypt_expanded_round_key_bits_walsh_complex_blocks(ctx, H.aes128_expand_key(KEY), 512)
     counters, _, cts = make_blocks(H, 1024)
     with pytest.raises(Exception, match="share the main context"):
-        H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=[other])
+        H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts, sbox_contexts=[other],
+                                                 batch_sboxes=False)
+    # contexts built from separately drawn parameter sets hold separate keys
+    with pytest.raises(Exception, match="share the main context"):
+        H.ckks_aes_ctr_walsh_sbox_complex_blocks(ctx, keys, counters, cts,
+                                                 sbox_contexts=[H.CKKSContext(H.CKKSParams.AESWalsh15())],
+                                                 batch_sboxes=False)

@@ -1,7 +1,13 @@
-"""Served walsh15 AES-CTR with continuous batching (aes_serve.AesCtrServer):
-a request's result is the same alone and in a mix of requests (plaintext
-bytes equal and exact; decrypted bit values within 1e-3 of each other), and
-a burst larger than one evaluation spreads over both lanes."""
+"""Served walsh15 AES-CTR with continuous batching (the native server behind
+aes_serve.AesCtrServer: lanes of one key-owning context).
+
+* a request's result is the same alone and in a mix of requests that spans
+  several evaluations on both lanes (plaintext bytes exact; decrypted bit
+  values within 1e-3 of each other);
+* admission: a request beyond one evaluation, and requests beyond the lanes'
+  admitted work, are refused with their own error while every other request
+  of the burst completes correctly; a second lane opens only if its measured
+  footprint fits the memory budget."""
 import os
 import sys
 
@@ -27,45 +33,60 @@ def make_request(H, rng, n, first):
     return counters, cts, plains
 
 
-def test_request_is_the_same_alone_and_in_the_mix():
+def test_request_is_the_same_alone_and_in_the_mix_and_refusals_are_per_request():
     H = host()
-    srv = aes_serve.AesCtrServer(KEY, lanes=2, contexts_per_lane=2)
+    srv = aes_serve.AesCtrServer(KEY, lanes=2, max_queued_evaluations=2)
     try:
-        assert len(srv.lanes) == 2
+        assert srv.lanes == 2 and srv.evaluation_bytes > 0
+        print(f"lanes {srv.lanes}, evaluation footprint {srv.evaluation_bytes / 2**30:.2f} GiB", flush=True)
         rng = np.random.default_rng(31)
         c, x, p = make_request(H, rng, 32, 0)
-        alone = srv.submit(aes_serve.Request(c, x))
-        assert alone.done.wait(300) and alone.error is None
-        assert np.array_equal(alone.plaintexts, p)
-        # a mix of 60 requests (more than 1024 blocks) with the same request inside it
-        mix, want, first = [], [], 32
+        alone = srv.result(srv.submit(c, x))
+        assert alone["error"] == "" and not alone["refused"]
+        assert np.array_equal(alone["plaintexts"], p)
+        # a burst: 60 requests (more than 1024 blocks, so several evaluations on
+        # both lanes) with the same request inside it, one oversize request
+        # (1025 blocks) and then a flood past the admitted work
+        ids, want, first = [], [], 32
         for k in range(60):
             if k == 17:
-                mix.append(srv.submit(aes_serve.Request(c, x)))
+                ids.append(srv.submit(c, x))
                 want.append(p)
                 continue
+            if k == 30:
+                big_c, big_x, _ = make_request(H, rng, 1025, 10_000)
+                oversize = srv.submit(big_c, big_x)
             n = int(rng.integers(16, 33))
             cc, xx, pp = make_request(H, rng, n, first)
             first += n
-            mix.append(srv.submit(aes_serve.Request(cc, xx)))
+            ids.append(srv.submit(cc, xx))
             want.append(pp)
-        for r, w in zip(mix, want):
-            assert r.done.wait(300) and r.error is None
-            assert np.array_equal(r.plaintexts, w)
-        again = mix[17]
-        assert np.array_equal(again.plaintexts, alone.plaintexts)
-        diff = float(np.max(np.abs(again.values - alone.values)))
-        print(f"alone vs in the mix: max value difference {diff:.2e}; evaluations {srv.evaluations}", flush=True)
+        flood_c, flood_x, _ = make_request(H, rng, 1024, 20_000)
+        flood = [srv.submit(flood_c, flood_x) for _ in range(12)]
+        results = [srv.result(i) for i in ids]  # each result is handed out once
+        for r, w in zip(results, want):
+            assert r["error"] == "", r["error"]
+            assert np.array_equal(r["plaintexts"], w)
+        r_over = srv.result(oversize)
+        assert r_over["refused"] and "1..1024 blocks" in r_over["error"]
+        outcomes = [srv.result(i) for i in flood]
+        refused = sum(r["refused"] for r in outcomes)
+        assert refused > 0 and all(r["error"] == "" for r in outcomes if not r["refused"])
+        assert len({r["lane"] for r in results}) == 2
+        print(f"flood: {refused} of {len(flood)} refused at admission; evaluations {srv.evaluations}", flush=True)
+        r17 = results[17]
+        assert np.array_equal(r17["plaintexts"], alone["plaintexts"])
+        diff = float(np.max(np.abs(r17["values"] - alone["values"])))
+        print(f"alone vs in the mix: max value difference {diff:.2e}", flush=True)
         assert diff < 1e-3
-        assert srv.evaluations >= 3   # the warm-ups are not counted; the burst needs two evaluations
     finally:
         srv.close()
 
 
 def test_lanes_are_admitted_by_hbm_footprint():
     """A second lane opens only if its measured pool footprint fits the budget."""
-    srv = aes_serve.AesCtrServer(KEY, lanes=2, contexts_per_lane=1, memory_budget=1)
+    srv = aes_serve.AesCtrServer(KEY, lanes=2, memory_budget=1)
     try:
-        assert len(srv.lanes) == 1 and srv.footprints[0] > 0
+        assert srv.lanes == 1 and srv.evaluation_bytes > 0
     finally:
         srv.close()

@@ -51,7 +51,7 @@ def test_poly_list_op_rejects_bad_arguments():
     s = M._stream(torch.cuda.current_stream())
     one = _arr([M.ptr(d)])
     assert M.lib().mi_poly_list_op(ctx.h, 0, 1, 1, one, one, None, s) < 0       # copy onto itself
-    assert M.lib().mi_poly_list_op(ctx.h, 5, 1, 1, one, one, None, s) < 0       # unknown operation
+    assert M.lib().mi_poly_list_op(ctx.h, 8, 1, 1, one, one, None, s) < 0       # unknown operation
     assert M.lib().mi_poly_list_op(ctx.h, 3, 1, 1, one, one, None, s) < 0       # scaling without residues
     assert M.lib().mi_poly_list_op(ctx.h, 1, 2, 1, one, one, None, s) < 0       # level beyond the chain
     assert M.lib().mi_poly_list_op(ctx.h, 1, 1, 0, one, one, None, s) < 0       # empty list

@@ -341,17 +341,10 @@ def test_aes_masking_matches_per_ciphertext_sequence():
         H.ckks_aes_mask_inactive_slots_inplace(ctx, low, 10)
 
 
-def test_walsh_sbox_end_to_end_on_a_bootstrapping_chain():
-    """The whole mask-packed S-box (parity packing, complex-packed bootstrap
-    to parity bits, signs, unpacking, Walsh output pairs of both lanes, split
-    into 8 bits) on the N = 2^10 bootstrapping chain of the host bootstrap
-    test: 16 blocks per lane, random bytes; the decrypted outputs round to
-    the AES S-box bits of every block in both lanes.  The EvalMod polynomial
-    is the test's own Chebyshev interpolant (the reference fits its
-    coefficients with MPFR)."""
-    from helpers import host
-    H = host()
-    bc = 16
+def _sbox_context(H, bc):
+    """The N = 2^10 bootstrapping chain of the host bootstrap test with the
+    rotation keys of a bc-block mask packing, and the test's own Chebyshev
+    interpolant as its EvalMod polynomial."""
     c = H.CKKSParamsConfig()
     c.N = 1 << 10
     c.mult_depth = 1
@@ -372,11 +365,26 @@ def test_walsh_sbox_end_to_end_on_a_bootstrapping_chain():
     bp.evalmod_depth = 7
     c.bootstrap_params = bp
     ctx = H.CKKSContext(H.CKKSParams(c))
-    P = ctx.getParams()
-    K = P.getBootstrapEvalModK()
+    K = ctx.getParams().getBootstrapEvalModK()
     coeffs = np.polynomial.chebyshev.chebinterpolate(lambda u: 0.5 - 0.5 * np.cos(2 * np.pi * K * u), 100)
     coeffs[1::2] = 0.0
     ctx.setEvalModPolynomial(list(coeffs))
+    return ctx
+
+
+def test_walsh_sbox_end_to_end_on_a_bootstrapping_chain():
+    """The whole mask-packed S-box (parity packing, complex-packed bootstrap
+    to parity bits, signs, unpacking, Walsh output pairs of both lanes, split
+    into 8 bits) on the N = 2^10 bootstrapping chain of the host bootstrap
+    test: 16 blocks per lane, random bytes; the decrypted outputs round to
+    the AES S-box bits of every block in both lanes.  The EvalMod polynomial
+    is the test's own Chebyshev interpolant (the reference fits its
+    coefficients with MPFR)."""
+    from helpers import host
+    H = host()
+    bc = 16
+    ctx = _sbox_context(H, bc)
+    P = ctx.getParams()
     slots = P.getSlots()
     rng = np.random.default_rng(23)
     x0, x1 = rng.integers(0, 256, bc), rng.integers(0, 256, bc)

@@ -116,3 +116,16 @@ def test_host_decode_coefficient_domain_closed_form():
     for i in range(N // 2):
         j = int(format(i, "05b")[::-1], 2)
         assert out[i] == complex(vals[j] / 2.0 ** 20, vals[N // 2 + j] / 2.0 ** 20)
+
+
+def test_host_params_seed_is_drawn_unless_set():
+    """An unset seed is drawn from std::random_device per CKKSParams (so two
+    default contexts hold different secret keys, as the reference's
+    getrandom-drawn keys do); a set seed is kept for replay."""
+    H = host()
+    seeds = {H.CKKSParams.AESWalsh15().getSeed() for _ in range(4)}
+    assert len(seeds) == 4
+    c = H.aes_walsh15_config()
+    assert c.seed is None
+    c.seed = 77
+    assert H.CKKSParams(c).getSeed() == 77 and H.CKKSParams(c).getSeed() == 77
