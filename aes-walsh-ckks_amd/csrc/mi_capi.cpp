@@ -472,6 +472,20 @@ int mi_keyswitch(mi_context* ctx, int level, const uint64_t* key_term, const uin
     });
 }
 
+int mi_keyswitch_noalloc(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                         uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
+                         void* stream) {
+    return guard("mi_keyswitch_noalloc", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(key_term); check_ptr(key); check_ptr(b_out); check_ptr(a_out);
+        if ((b_mode != 0 && b_mode != 1) || (a_mode != 0 && a_mode != 1))
+            throw mi::Error(MI_EINVAL, "invalid combine mode");
+        mi::keyswitch_noalloc(*ctx->c, mi::KsItem{key_term, key, b_out, a_out, level, b_mode, a_mode},
+                              montgomery != 0, S(stream));
+    });
+}
+
 int mi_keyswitch_batch(mi_context* ctx, const mi_keyswitch_item* items, int count, int montgomery,
                        void* stream) {
     return guard("ckks_hybrid_keyswitch", [&] {

@@ -198,6 +198,7 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     c->q.assign(q, q + nq);
     c->p.assign(p, p + np);
     MI_HIP(hipGetDevice(&c->device));
+    noalloc_setup(c->device);
     const int nb = nq + np;
     std::vector<u64> fwd(nb * N), fws(nb * N), inv(nb * N), ivs(nb * N);
     c->h_bc.resize(nb);

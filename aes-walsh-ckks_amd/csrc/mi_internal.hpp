@@ -292,6 +292,13 @@ void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomer
 // Whole batch: items are grouped by (level, modes) into launches of up to
 // KS_MAX_ITEMS, keeping the order of items whose buffers overlap.
 void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s);
+// One key switch with its scratch in the library's fixed device array
+// (mi_noalloc.hip): no allocation.  noalloc_setup takes that array's address
+// on `device` (current) once; keyswitch_noalloc refuses a switch whose
+// scratch exceeds noalloc_scratch_words().
+void noalloc_setup(int device);
+size_t noalloc_scratch_words();
+void keyswitch_noalloc(Context& c, const KsItem& it, bool montgomery, hipStream_t s);
 
 // Hoisting and QP-basis switches (ckks_keyswitch.cpp:344-373, 675-1328).
 void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s);

@@ -57,6 +57,7 @@ _SIGS = {
     "mi_automorphism": (cint, [vp, cint, u64, vp, vp, cint, i64, vp]),
     "mi_rescale": (cint, [vp, cint, vp, cint, i64, vp]),
     "mi_keyswitch": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, cint, vp]),
+    "mi_keyswitch_noalloc": (cint, [vp, cint, vp, vp, vp, vp, cint, cint, cint, vp]),
     "mi_keyswitch_batch": (cint, [vp, vp, cint, cint, vp]),
     "mi_relin": (cint, [vp, cint, vp, vp, vp, vp, cint, vp]),
     "mi_rotate": (cint, [vp, cint, vp, vp, u64, vp, cint, vp]),

@@ -174,6 +174,18 @@ int mi_rescale(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t 
 int mi_keyswitch(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
                  uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
                  void* stream);
+/* mi_keyswitch with no allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The switch's scratch is
+ * the library's fixed device array of 2^22 words (32 MiB) per GPU, not the
+ * stream workspace; a switch needing more (ks words = (L + parts (L + K) +
+ * 2K + 2L) N, e.g. level 29 of a 30 + 10-limb N = 2^16 chain) is refused
+ * with MI_EINVAL and nothing runs.  Arguments and results as mi_keyswitch.
+ * All calls on one GPU share the array: calls that overlap on one GPU
+ * (several streams or host threads) race. */
+int mi_keyswitch_noalloc(mi_context* ctx, int level, const uint64_t* key_term, const uint64_t* key,
+                         uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, int montgomery,
+                         void* stream);
 
 /* One key switch of a ragged batch (mixed levels, keys and combine modes):
  * the reference's OpenMP loops over independent switches
