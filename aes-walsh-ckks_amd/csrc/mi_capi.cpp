@@ -144,6 +144,37 @@ int mi_probe_end(mi_context* ctx, double* stats) {
     });
 }
 
+int mi_tune_report(mi_context* ctx, char* buf, size_t len) {
+    return guard("mi_tune_report", [&] {
+        check_ctx(ctx);
+        check_ptr(buf);
+        const mi::TunePlan& t = ctx->c->tune;
+        std::string r = "{\"device\": \"" + ctx->c->tune_key + "\", \"tuned\": " + (t.tuned ? "true" : "false") +
+                        ", \"from_cache\": " + (t.from_cache ? "true" : "false") + ", \"ntt_limbs_per_workgroup\": [";
+        for (int b = 0; b < mi::TUNE_BUCKETS; ++b) r += (b ? ", " : "") + std::to_string(t.ntt_lpb[b]);
+        r += "], \"ntt_bucket_limbs\": [0, 4, 16, 64, 256, 1024], \"keyswitch_group\": " +
+             std::to_string(t.ks_group) + ", \"ntt_forced\": " + std::to_string(ctx->c->ntt_lpb_forced) + "}";
+        if (r.size() + 1 > len) throw mi::Error(MI_EINVAL, "buffer too small");
+        std::memcpy(buf, r.c_str(), r.size() + 1);
+    });
+}
+
+int mi_tune_set(mi_context* ctx, const int* ntt_lpb, int keyswitch_group) {
+    return guard("mi_tune_set", [&] {
+        check_ctx(ctx);
+        check_ptr(ntt_lpb);
+        mi::TunePlan t = ctx->c->tune;
+        for (int b = 0; b < mi::TUNE_BUCKETS; ++b) {
+            if (ntt_lpb[b] != 1 && ntt_lpb[b] != 2) throw mi::Error(MI_EINVAL, "limbs per workgroup must be 1 or 2");
+            t.ntt_lpb[b] = ntt_lpb[b];
+        }
+        if (keyswitch_group != 8 && keyswitch_group != 16 && keyswitch_group != 32)
+            throw mi::Error(MI_EINVAL, "key-switch group must be 8, 16 or 32");
+        t.ks_group = keyswitch_group;
+        ctx->c->tune = t;
+    });
+}
+
 int mi_alloc(mi_context* ctx, size_t bytes, void** out) {
     return guard("mi_alloc", [&] {
         check_ctx(ctx);
@@ -572,22 +603,8 @@ int mi_conjugate_batch(mi_context* ctx, int level, int count, uint64_t* const* b
                         if (x < y + W && y < x + W) throw mi::Error(MI_EINVAL, "ciphertexts of a batch overlap");
             if (b[i] < a[i] + W && a[i] < b[i] + W) throw mi::Error(MI_EINVAL, "b and a overlap");
         }
-        mi::Context& c = *ctx->c;
-        const int L = level + 1;
-        const long long words = (long long)L * (long long)c.N;
-        const uint64_t g = 2 * c.N - 1;
-        uint64_t* tmp = (uint64_t*)c.workspace(S(stream), sizeof(uint64_t) * 2 * words);
-        std::vector<mi::KsItem> items;
-        for (int k = 0; k < count; ++k) {
-            // as mi_conjugate: the automorphism first, then the key switch of the new a
-            mi::automorphism(c, L, g, b[k], tmp, 1, words, words, S(stream));
-            mi::automorphism(c, L, g, a[k], tmp + words, 1, words, words, S(stream));
-            MI_HIP(hipMemcpyAsync(b[k], tmp, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
-            MI_HIP(hipMemcpyAsync(a[k], tmp + words, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, S(stream)));
-            items.push_back(mi::KsItem{a[k], key, b[k], a[k], level, 0, 1});
-        }
-        for (int k0 = 0; k0 < count; k0 += 32)
-            mi::keyswitch_batch(c, items.data() + k0, std::min(32, count - k0), montgomery != 0, S(stream));
+        // as mi_conjugate: the automorphism first, then the key switch of the new a
+        mi::conjugate_batch(*ctx->c, level, count, b, a, key, montgomery != 0, S(stream));
     });
 }
 
@@ -750,6 +767,28 @@ int mi_set_encoding_tables(mi_context* ctx, const double* twiddles, const uint64
         check_ctx(ctx);
         if (!twiddles || !perm) throw mi::Error(MI_EINVAL, "null pointer argument");
         mi::set_encoding_tables(*ctx->c, twiddles, perm);
+    });
+}
+
+int mi_set_decoding_tables(mi_context* ctx, const double* twiddles, const uint64_t* perm) {
+    return guard("ckks_decrypt", [&] {
+        check_ctx(ctx);
+        check_ptr(twiddles);
+        check_ptr(perm);
+        mi::set_decoding_tables(*ctx->c, twiddles, perm);
+    });
+}
+
+int mi_decode_batch(mi_context* ctx, int count, const uint64_t* const* coeffs, int limbs,
+                    const uint64_t* scale_mant, const int32_t* scale_exp, int coeff_domain, double* out,
+                    void* stream) {
+    return guard("ckks_decrypt", [&] {
+        check_ctx(ctx);
+        if (count < 1 || !coeffs || !scale_mant || !scale_exp) throw mi::Error(MI_EINVAL, "invalid decode batch");
+        check_ptr(out);
+        for (int i = 0; i < count; ++i) check_ptr(coeffs[i]);
+        mi::decode_batch(*ctx->c, count, coeffs, limbs, scale_mant, (const int*)scale_exp, coeff_domain != 0, out,
+                         S(stream));
     });
 }
 

@@ -1,3 +1,4 @@
+#include <cstdlib>
 // Context construction: host precompute of the device-resident tables.
 //
 // Everything here is exact modular arithmetic on 64-bit moduli (128-bit
@@ -128,6 +129,11 @@ void* Context::workspace(hipStream_t s, size_t bytes) {
     return w.ptr;
 }
 
+int Context::ntt_lpb(int count, bool, bool) const {
+    if (ntt_lpb_forced) return ntt_lpb_forced;
+    return tune.ntt_lpb[tune_bucket(count)];
+}
+
 long Context::probe_open(hipStream_t s, long long limbs) const {
     std::lock_guard<std::mutex> g(probe_mu);
     if (!probe.on) return -1;
@@ -199,6 +205,10 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     c->p.assign(p, p + np);
     MI_HIP(hipGetDevice(&c->device));
     noalloc_setup(c->device);
+    if (const char* e = std::getenv("MICKKS_NTT_LPB")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2) c->ntt_lpb_forced = v;
+    }
     const int nb = nq + np;
     std::vector<u64> fwd(nb * N), fws(nb * N), inv(nb * N), ivs(nb * N);
     c->h_bc.resize(nb);
@@ -354,6 +364,12 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     // level-0 dense-to-sparse switch: p_0^-1 mod q_0 (ckks_keyswitch.cpp:1767-1768)
     c->q0p0_inv = invm(p[0] % q[0], q[0]);
     c->q0p0_inv_shoup = shoup_of(c->q0p0_inv, q[0]);
+    try {
+        tune_context(*c);
+    } catch (const Error&) {
+        // tuning could not run (e.g. no memory for its scratch): the defaults
+        c->tune = TunePlan{};
+    }
     return c;
 }
 
@@ -464,6 +480,8 @@ void context_destroy(Context* c) {
     (void)hipFree(c->d_half);
     (void)hipFree(c->d_enc_tw);
     (void)hipFree(c->d_enc_inv);
+    (void)hipFree(c->d_dec_tw);
+    (void)hipFree(c->d_dec_perm);
     (void)hipFree(c->d_gauss_cdf);
     for (auto& kv : c->ws)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);

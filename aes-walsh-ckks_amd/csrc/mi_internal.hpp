@@ -57,6 +57,16 @@ struct Workspace {
     size_t bytes = 0;
 };
 
+// Tuned launch configuration of a context (mi_tune.hip): NTT limbs per
+// workgroup per launch-size bucket, and key switches per launch group.
+constexpr int TUNE_BUCKETS = 6;
+struct TunePlan {
+    int ntt_lpb[TUNE_BUCKETS] = {1, 1, 1, 1, 1, 1};
+    int ks_group = 32;
+    bool tuned = false;       // measured (now or in an earlier process)
+    bool from_cache = false;  // read from the tuning cache file
+};
+
 struct Context {
     u64 N = 0;
     int logN = 0, n1 = 0, n2 = 0;
@@ -79,6 +89,8 @@ struct Context {
     u64 q0p0_inv = 0, q0p0_inv_shoup = 0;         // p_0^-1 mod q_0 (level-0 switch)
     double* d_enc_tw = nullptr;                   // encoding FFT twiddles [N] complex
     int* d_enc_inv = nullptr;                     // encoding slot placement [N]
+    double* d_dec_tw = nullptr;                   // decoding (inverse special FFT) twiddles [N] complex
+    int* d_dec_perm = nullptr;                    // decoding slot permutation [N/2]
     std::mutex gauss_mu;                          // discrete Gaussian table of the last sigma
     double gauss_sigma = 0.0;
     double* d_gauss_cdf = nullptr;
@@ -124,6 +136,13 @@ struct Context {
     // probe_close records the pair's stop event on s.
     long probe_open(hipStream_t s, long long limbs) const;
     void probe_close(long slot, hipStream_t s) const;
+
+    // NTT plan: limbs per workgroup (1 or 2) of a pass over `count` limbs.
+    // ntt_lpb_forced (environment MICKKS_NTT_LPB) overrides the plan.
+    int ntt_lpb_forced = 0;
+    int ntt_lpb(int count, bool fwd, bool rows) const;
+    TunePlan tune;
+    std::string tune_key;  // "<gfx arch> <compute units> <log2 N>"
 
     int nb() const { return nq + np; }
     int parts(int level) const {
@@ -191,6 +210,16 @@ void ntt_forward(const Context& c, const LimbBatch& b, hipStream_t s);
 void ntt_inverse(const Context& c, const LimbBatch& b, hipStream_t s);
 
 Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int dnum);
+// Decoding on the device (mi_decode.hip): tables from the host context, then
+// CRT + scale division + inverse special FFT of `count` coefficient-domain
+// plaintexts (limbs x N each); out: count x N/2 complex (2 doubles each).
+void set_decoding_tables(Context& c, const double* twiddles, const u64* perm);
+void decode_batch(Context& c, int count, const u64* const* coeffs, int limbs, const u64* scale_mant,
+                  const int* scale_exp, bool coeff_domain, double* out, hipStream_t s);
+// Installs the tuned plan for the context's device and ring dimension,
+// measuring it first when neither this process nor the cache file has it.
+void tune_context(Context& c);
+int tune_bucket(int count);
 void context_destroy(Context* c);
 
 // ---- evaluator ops (mi_ops.hip) ----
@@ -236,6 +265,7 @@ struct KsItem {
     u64* b_out;
     u64* a_out;
     int level, b_mode, a_mode;
+    const u64* init_b = nullptr;  // optional L-limb term added to the switched b before it is combined
 };
 void keyswitch(Context& c, const KsItem& it, bool montgomery, hipStream_t s);
 
@@ -292,6 +322,11 @@ void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomer
 // Whole batch: items are grouped by (level, modes) into launches of up to
 // KS_MAX_ITEMS, keeping the order of items whose buffers overlap.
 void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s);
+// In-place conjugations of independent ciphertexts at one level: one
+// automorphism launch per KS_MAX_ITEMS ciphertexts into scratch, then their
+// key switches as one batch (the conjugated b enters as the switch's init_b).
+void conjugate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* key, bool montgomery,
+                     hipStream_t s);
 // One key switch with its scratch in the library's fixed device array
 // (mi_noalloc.hip): no allocation.  noalloc_setup takes that array's address
 // on `device` (current) once; keyswitch_noalloc refuses a switch whose

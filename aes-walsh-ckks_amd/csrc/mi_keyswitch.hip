@@ -447,7 +447,8 @@ struct MdOutJob {
         u64* out;
         u64 q;
         __device__ u64 load(int i) const { return src[i]; }
-        __device__ void store(int i, u64 v) const { out[i] = sub_mod(base[i], v, q); }
+        __device__ u64 pre(int i) const { return base[i]; }
+        __device__ void store_pre(int i, u64 v, u64 b) const { out[i] = sub_mod(b, v, q); }
     };
     __device__ View view(int k) const {
         const int t = per_limb.div(k), poly = k - t * npoly;
@@ -552,6 +553,7 @@ void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomer
         a.key[i] = items[i].key;
         a.b_out[i] = items[i].b_out;
         a.a_out[i] = items[i].a_out;
+        a.init_b[i] = items[i].init_b;
     }
     ks_mac(c, a, level, s);
     u64* p[2 * KS_MAX_ITEMS];
@@ -579,8 +581,8 @@ struct Span {
 bool overlaps(const Span& x, const Span& y) { return x.lo < y.hi && y.lo < x.hi; }
 }  // namespace
 
-// Groups of one (level, modes) whose buffers are independent, up to
-// KS_MAX_ITEMS each.  An item that touches a buffer an earlier item of the
+// Groups of one (level, modes) whose buffers are independent, up to the
+// tuned group size (at most KS_MAX_ITEMS) each.  An item that touches a buffer an earlier item of the
 // open group of its kind writes (or writes one it reads) closes that group
 // and every other open group, so the order of dependent switches is kept.
 void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hipStream_t s) {
@@ -628,11 +630,12 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         const KsItem& it = items[i];
         const size_t W = (size_t)(it.level + 1) * c.N;
         const Span rd{it.key_term, it.key_term + W};
+        const Span ri{it.init_b, it.init_b ? it.init_b + W : it.init_b};
         const Span wb{it.b_out, it.b_out + W}, wa{it.a_out, it.a_out + W};
         bool conflict = false;
         for (const auto& g : open) {
             for (const auto& w : g.writes)
-                if (overlaps(w, rd) || overlaps(w, wb) || overlaps(w, wa)) conflict = true;
+                if (overlaps(w, rd) || overlaps(w, ri) || overlaps(w, wb) || overlaps(w, wa)) conflict = true;
             for (const auto& r : g.reads)
                 if (overlaps(r, wb) || overlaps(r, wa)) conflict = true;
         }
@@ -648,7 +651,8 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         g->writes.push_back(wb);
         g->writes.push_back(wa);
         g->reads.push_back(rd);
-        if ((int)g->v.size() == KS_MAX_ITEMS) {
+        if (it.init_b) g->reads.push_back(Span{it.init_b, it.init_b + W});
+        if ((int)g->v.size() == c.tune.ks_group) {
             keyswitch_items(c, g->v.data(), (int)g->v.size(), montgomery, s);
             g->v.clear();
             g->writes.clear();
@@ -719,7 +723,7 @@ void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b
     const int L = level + 1, K = c.np;
     const u64 N = c.N;
     const long long LN = (long long)L * (long long)N, KN = (long long)K * (long long)N;
-    const int chunk = std::min(n, KS_MAX_ITEMS);
+    const int chunk = std::min(n, c.tune.ks_group);
     // per rotation: P accumulators (a, b) | switched pair (a, b) | ModDown temp
     u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (2 * KN + 4 * LN));
     u64* AP = ws;
@@ -771,9 +775,10 @@ void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b
 
 // In-place rotations of independent ciphertexts at one level
 // (ckks_rotate_inplace, ckks_rotation.cpp:41-96, for each): per chunk of up to
-// KS_MAX_ITEMS, b is staged next to the switched a, the key switches of all a
-// run as one batch (b Add, a Assign), and one launch applies each rotation's
-// automorphism from the staging area back into the ciphertexts.
+// KS_MAX_ITEMS, the key switches of all a run as one batch into a staging
+// area (b enters as the switch's init_b: staged b = b + switched b, a
+// assigned), and one launch applies each rotation's automorphism from the
+// staging area back into the ciphertexts.
 void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* galois,
                   const u64* const* keys, bool montgomery, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
@@ -781,7 +786,7 @@ void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, co
     const int L = level + 1;
     const u64 N = c.N;
     const long long LN = (long long)L * (long long)N;
-    const int chunk = std::min(n, KS_MAX_ITEMS);
+    const int chunk = std::min(n, c.tune.ks_group);
     const size_t kw = ks_item_words(c, level);
     u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (kw + 2 * LN));
     u64* SW = ws + (size_t)chunk * kw;  // per item: a' then b'
@@ -791,8 +796,7 @@ void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, co
         for (int it = 0; it < m; ++it) {
             u64* a_sw = SW + it * 2 * LN;
             u64* b_sw = a_sw + LN;
-            MI_HIP(hipMemcpyAsync(b_sw, b[i0 + it], sizeof(u64) * LN, hipMemcpyDeviceToDevice, s));
-            items[it] = KsItem{a[i0 + it], keys[i0 + it], b_sw, a_sw, level, 0, 1};
+            items[it] = KsItem{a[i0 + it], keys[i0 + it], b_sw, a_sw, level, 1, 1, b[i0 + it]};
         }
         keyswitch_items_with(c, items, m, montgomery, ws, s);
         AutoItemsArgs aa{};
@@ -808,6 +812,67 @@ void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, co
                            0, s, aa);
         MI_HIP(hipGetLastError());
     }
+}
+
+// dst[k] = automorphism g of src[k] (L limbs each), for up to 2 KS_MAX_ITEMS polys
+struct AutoListArgs {
+    const u64* src[2 * KS_MAX_ITEMS];
+    u64* dst[2 * KS_MAX_ITEMS];
+    u64 galois;
+    int L, logN;
+};
+
+__global__ void __launch_bounds__(KS_THREADS) k_automorphism_list(AutoListArgs a) {
+    const u64 N = (u64)1 << a.logN;
+    const int tps = (int)(N / KS_THREADS);
+    const int tile = blockIdx.x % tps, rest = blockIdx.x / tps;
+    const int l = rest % a.L, k = rest / a.L;
+    const u32 i = (u32)tile * KS_THREADS + threadIdx.x;
+    const long long off = (long long)l * N;
+    a.dst[k][off + i] = a.src[k][off + galois_src(i, a.galois, a.logN)];
+}
+
+void conjugate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* key, bool montgomery,
+                     hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
+    if (n < 1) return;
+    const int L = level + 1;
+    const u64 N = c.N;
+    const long long LN = (long long)L * (long long)N;
+    const u64 g = 2 * N - 1;
+    // conjugated (a, b) of every ciphertext, live across the key-switch batch
+    // (which uses the stream workspace itself)
+    u64* tmp = (u64*)c.pool_alloc(sizeof(u64) * (size_t)n * 2 * LN);
+    try {
+        for (int i0 = 0; i0 < n; i0 += KS_MAX_ITEMS) {
+            const int m = std::min(KS_MAX_ITEMS, n - i0);
+            AutoListArgs al{};
+            for (int it = 0; it < m; ++it) {
+                al.src[2 * it] = a[i0 + it];
+                al.src[2 * it + 1] = b[i0 + it];
+                al.dst[2 * it] = tmp + (long long)(i0 + it) * 2 * LN;
+                al.dst[2 * it + 1] = tmp + (long long)(i0 + it) * 2 * LN + LN;
+            }
+            al.galois = g;
+            al.L = L;
+            al.logN = c.logN;
+            hipLaunchKernelGGL(k_automorphism_list, dim3((unsigned)(2 * m * L * (N / KS_THREADS))), dim3(KS_THREADS),
+                               0, s, al);
+            MI_HIP(hipGetLastError());
+        }
+        // (b, a) <- (sigma(b) + KS_b(sigma(a)), KS_a(sigma(a)))
+        std::vector<KsItem> items;
+        items.reserve(n);
+        for (int k = 0; k < n; ++k) {
+            const u64* ca = tmp + (long long)k * 2 * LN;
+            items.push_back(KsItem{ca, key, b[k], a[k], level, 1, 1, ca + LN});
+        }
+        keyswitch_batch(c, items.data(), n, montgomery, s);
+    } catch (...) {
+        c.pool_release(tmp);
+        throw;
+    }
+    c.pool_release(tmp);
 }
 
 void keyswitch_hoisted_qp(Context& c, int level, const u64* digits, const u64* key, const u64* b_q,

@@ -148,6 +148,93 @@ __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, const Bf
     }
 }
 
+// ---- several limbs per workgroup ----
+// LPB limbs per workgroup: every thread holds E words of each limb at the same
+// positions, the limbs' butterflies are interleaved (independent dependency
+// chains for the scheduler) and their LDS exchanges run one after another
+// through the one LDS region, so a workgroup keeps LPB limbs in flight for the
+// LDS (and occupancy) of one.
+template <int LOGR, int LPB>
+__device__ __forceinline__ void to_contiguous_n(u64 (&x)[LPB][E], int t, u64* lds) {
+#pragma unroll
+    for (int i = 0; i < LPB; ++i) {
+        if (i) __syncthreads();  // the previous limb's reads are done
+        to_contiguous<LOGR>(x[i], t, lds);
+    }
+}
+template <int LOGR, int LPB>
+__device__ __forceinline__ void to_strided_n(u64 (&x)[LPB][E], int t, u64* lds) {
+#pragma unroll
+    for (int i = 0; i < LPB; ++i) {
+        if (i) __syncthreads();
+        to_strided<LOGR>(x[i], t, lds);
+    }
+}
+
+template <int LOGR, int LPB, class IDX>
+__device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, u64* lds, const BfConst (&kq)[LPB],
+                                            const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
+    using G = Grp<LOGR>;
+#pragma unroll
+    for (int s = 0; s < LOGE; ++s) {
+        const int dj = E >> (s + 1);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & dj) continue;
+            const int k = idx(s, j0 >> (LOGE - s));
+#pragma unroll
+            for (int i = 0; i < LPB; ++i) bf_fwd(x[i][j0], x[i][j0 + dj], W[i][k], WS[i][k], kq[i]);
+        }
+    }
+    __syncthreads();
+    to_contiguous_n<LOGR, LPB>(x, t, lds);
+#pragma unroll
+    for (int s = LOGE; s < LOGR; ++s) {
+        const int d = G::R >> (s + 1);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & d) continue;
+            const int k = idx(s, (t * E + j0) >> (LOGR - s));
+#pragma unroll
+            for (int i = 0; i < LPB; ++i) bf_fwd(x[i][j0], x[i][j0 + d], W[i][k], WS[i][k], kq[i]);
+        }
+    }
+    __syncthreads();
+    to_strided_n<LOGR, LPB>(x, t, lds);
+}
+
+template <int LOGR, bool LAST, int LPB, class IDX>
+__device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, u64* lds, const BfConst (&kq)[LPB],
+                                            const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
+    using G = Grp<LOGR>;
+    __syncthreads();
+    to_contiguous_n<LOGR, LPB>(x, t, lds);
+#pragma unroll
+    for (int u = 0; u < LOGE; ++u) {
+        const int d = 1 << u;
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & d) continue;
+            const int k = idx(u, (t * E + j0) >> (u + 1));
+#pragma unroll
+            for (int i = 0; i < LPB; ++i) bf_inv(x[i][j0], x[i][j0 + d], W[i][k], WS[i][k], kq[i]);
+        }
+    }
+    __syncthreads();
+    to_strided_n<LOGR, LPB>(x, t, lds);
+#pragma unroll
+    for (int u = LOGE; u < (LAST ? LOGR : LOGR - 1); ++u) {
+        const int dj = 1 << (u - G::LOGTG);
+#pragma unroll
+        for (int j0 = 0; j0 < E; ++j0) {
+            if (j0 & dj) continue;
+            const int k = idx(u, j0 >> (u + 1 - G::LOGTG));
+#pragma unroll
+            for (int i = 0; i < LPB; ++i) bf_inv(x[i][j0], x[i][j0 + dj], W[i][k], WS[i][k], kq[i]);
+        }
+    }
+}
+
 // Geometry of one pass.  Column passes: group = column (stride R2 words
 // between elements), thread (g = tid % G, t = tid / G).  Row passes: group =
 // contiguous block of R2 words, thread (g = tid / TG, t = tid % TG).

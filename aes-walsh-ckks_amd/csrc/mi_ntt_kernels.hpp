@@ -7,11 +7,17 @@
 //   int basis;                            // basis index of the limb
 //   __device__ u64 load(int i) const;     // coefficient / slot i
 //   __device__ void store(int i, u64 v) const;
+// A View may also offer `u64 pre(int i) const` and
+// `void store_pre(int i, u64 v, u64 pre) const`: the pass then reads pre(i)
+// for its output positions together with its input, before the butterflies,
+// and stores through store_pre, so an output that combines the transform
+// with another operand does not wait for that operand's loads at the end.
 // Forward column pass: load = job input, store = intermediate ([0,4q)).
 // Forward row pass: load = intermediate, store = canonical output.
 // Inverse row pass: load = input (canonical), store = intermediate ([0,2q)).
 // Inverse column pass: load = intermediate, store = canonical output.
 #pragma once
+#include <type_traits>
 #include "mi_internal.hpp"
 #include "mi_ntt_core.hpp"
 
@@ -64,81 +70,153 @@ constexpr int pass_threads(int logr, int logother) {
 constexpr int pass_groups(int logr, int logother) { return pass_threads(logr, logother) >> (logr - LOGE); }
 constexpr int pass_wpl(int logr, int logother) { return (1 << logother) / pass_groups(logr, logother); }
 
-template <int LOGN, bool FWD, class Job>
+template <class V, class = void>
+struct HasPre : std::false_type {};
+template <class V>
+struct HasPre<V, std::void_t<decltype(&V::pre)>> : std::true_type {};
+
+// LPB limbs per workgroup (1 or 2): workgroup b serves limbs LPB*(b / wpl) +
+// 0..LPB-1 of a launch over `count` limbs; a limb past `count` or skipped by
+// the job is left alone (uniform over the workgroup).
+template <int LPB, class Job, class V>
+__device__ __forceinline__ bool limb_views(const Job& job, int first, int count, V (&v)[LPB], bool (&live)[LPB]) {
+    bool any = false;
+    int k0 = -1;
+#pragma unroll
+    for (int i = 0; i < LPB; ++i) {
+        const int k = first + i;
+        live[i] = k < count && !job.skip(k);
+        if (live[i] && k0 < 0) k0 = k;
+        any = any || live[i];
+    }
+    if (!any) return false;
+#pragma unroll
+    for (int i = 0; i < LPB; ++i) v[i] = job.view(live[i] ? first + i : k0);  // a dead limb borrows a live view
+    return true;
+}
+
+template <int LOGN, bool FWD, int LPB, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
-k_ntt_cols(NttTabs tb, Job job) {
+k_ntt_cols(NttTabs tb, Job job, int count) {
     constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n1;
     constexpr int G = pass_groups(n1, n2), wpl = pass_wpl(n1, n2);
     __shared__ u64 lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
-    const int limb = blockIdx.x / wpl, w = blockIdx.x - limb * wpl;
-    if (job.skip(limb)) return;
+    using V = decltype(job.view(0));
+    const int blk = blockIdx.x / wpl, w = blockIdx.x - blk * wpl;
+    V v[LPB];
+    bool live[LPB];
+    if (!limb_views<LPB>(job, blk * LPB, count, v, live)) return;
     const int g = threadIdx.x % G, t = threadIdx.x / G;
-    const auto v = job.view(limb);
-    const BasisConst bc = tb.bc[v.basis];
     const int col = w * G + g;
-    u64 x[E];
+    constexpr bool PRE = HasPre<V>::value;
+    u64 x[LPB][E], pre[LPB][PRE ? E : 1];
+    BasisConst bc[LPB];
+    BfConst kq[LPB];
+    const u64* W[LPB];
+    const u64* WS[LPB];
 #pragma unroll
-    for (int j = 0; j < E; ++j) x[j] = v.load(((t + Gr::TG * j) << n2) + col);
+    for (int i = 0; i < LPB; ++i) {
+        if (live[i]) {
+#pragma unroll
+            for (int j = 0; j < E; ++j) x[i][j] = v[i].load(((t + Gr::TG * j) << n2) + col);
+            if constexpr (PRE) {
+#pragma unroll
+                for (int j = 0; j < E; ++j) pre[i][j] = v[i].pre(((t + Gr::TG * j) << n2) + col);
+            }
+        }
+        bc[i] = tb.bc[v[i].basis];
+        kq[i] = bf_const(bc[i].q, bc[i].negq);
+        W[i] = (FWD ? tb.fwd : tb.inv) + (long long)v[i].basis * tb.N;
+        WS[i] = (FWD ? tb.fwd_shoup : tb.inv_shoup) + (long long)v[i].basis * tb.N;
+    }
     u64* my = lds + g * Gr::LDS;
     if constexpr (FWD) {
-        const u64* W = tb.fwd + (long long)v.basis * tb.N;
-        const u64* WS = tb.fwd_shoup + (long long)v.basis * tb.N;
-        group_fwd<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                        [](int s, int i) { return (1 << s) + i; });
+        group_fwd_n<LOGR, LPB>(x, t, my, kq, W, WS, [](int s, int i) { return (1 << s) + i; });
     } else {
-        const u64* W = tb.inv + (long long)v.basis * tb.N;
-        const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR, false>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                               [](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
+        group_inv_n<LOGR, false, LPB>(x, t, my, kq, W, WS, [](int u, int i) { return (1 << (n1 - 1 - u)) + i; });
         // last stage (distance N/2, twiddle index 1) with the N^-1 scaling:
         // x0 = (t + u) N^-1, x1 = (t - u) w_1 N^-1, canonical
-        const u64 q2 = bc.q << 1;
 #pragma unroll
-        for (int j = 0; j < E / 2; ++j) {
-            const u64 a = x[j], b = x[j + E / 2];
-            x[j] = shoup(a + b, bc.n_inv, bc.n_inv_shoup, bc.q);
-            x[j + E / 2] = shoup(a - b + q2, bc.w1_ninv, bc.w1_ninv_shoup, bc.q);
+        for (int i = 0; i < LPB; ++i) {
+            const u64 q2 = bc[i].q << 1;
+#pragma unroll
+            for (int j = 0; j < E / 2; ++j) {
+                const u64 a = x[i][j], b = x[i][j + E / 2];
+                x[i][j] = shoup(a + b, bc[i].n_inv, bc[i].n_inv_shoup, bc[i].q);
+                x[i][j + E / 2] = shoup(a - b + q2, bc[i].w1_ninv, bc[i].w1_ninv_shoup, bc[i].q);
+            }
         }
     }
 #pragma unroll
-    for (int j = 0; j < E; ++j) v.store(((t + Gr::TG * j) << n2) + col, x[j]);
+    for (int i = 0; i < LPB; ++i) {
+        if (!live[i]) continue;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            if constexpr (PRE) v[i].store_pre(((t + Gr::TG * j) << n2) + col, x[i][j], pre[i][j]);
+            else v[i].store(((t + Gr::TG * j) << n2) + col, x[i][j]);
+        }
+    }
 }
 
-template <int LOGN, bool FWD, class Job>
+template <int LOGN, bool FWD, int LPB, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
-k_ntt_rows(NttTabs tb, Job job) {
+k_ntt_rows(NttTabs tb, Job job, int count) {
     constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n2;
     constexpr int G = pass_groups(n2, n1), wpl = pass_wpl(n2, n1);
     __shared__ u64 lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
-    const int limb = blockIdx.x / wpl, w = blockIdx.x - limb * wpl;
-    if (job.skip(limb)) return;
+    using V = decltype(job.view(0));
+    const int blk = blockIdx.x / wpl, w = blockIdx.x - blk * wpl;
+    V v[LPB];
+    bool live[LPB];
+    if (!limb_views<LPB>(job, blk * LPB, count, v, live)) return;
     const int g = threadIdx.x / Gr::TG, t = threadIdx.x % Gr::TG;
-    const auto v = job.view(limb);
-    const BasisConst bc = tb.bc[v.basis];
     const int row = w * G + g;
     const int off = row << n2;
-    u64 x[E];
+    constexpr bool PRE = HasPre<V>::value;
+    u64 x[LPB][E], pre[LPB][PRE ? E : 1];
+    BasisConst bc[LPB];
+    BfConst kq[LPB];
+    const u64* W[LPB];
+    const u64* WS[LPB];
 #pragma unroll
-    for (int j = 0; j < E; ++j) x[j] = v.load(off + t + Gr::TG * j);
+    for (int i = 0; i < LPB; ++i) {
+        if (live[i]) {
+#pragma unroll
+            for (int j = 0; j < E; ++j) x[i][j] = v[i].load(off + t + Gr::TG * j);
+            if constexpr (PRE) {
+#pragma unroll
+                for (int j = 0; j < E; ++j) pre[i][j] = v[i].pre(off + t + Gr::TG * j);
+            }
+        }
+        bc[i] = tb.bc[v[i].basis];
+        kq[i] = bf_const(bc[i].q, bc[i].negq);
+        W[i] = (FWD ? tb.fwd : tb.inv) + (long long)v[i].basis * tb.N;
+        WS[i] = (FWD ? tb.fwd_shoup : tb.inv_shoup) + (long long)v[i].basis * tb.N;
+    }
     u64* my = lds + g * Gr::LDS;
     if constexpr (FWD) {
-        const u64* W = tb.fwd + (long long)v.basis * tb.N;
-        const u64* WS = tb.fwd_shoup + (long long)v.basis * tb.N;
-        group_fwd<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS,
-                        [row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
+        group_fwd_n<LOGR, LPB>(x, t, my, kq, W, WS,
+                               [row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
 #pragma unroll
-        for (int j = 0; j < E; ++j) x[j] = reduce_4q(x[j], bc.q);
+        for (int i = 0; i < LPB; ++i)
+#pragma unroll
+            for (int j = 0; j < E; ++j) x[i][j] = reduce_4q(x[i][j], bc[i].q);
     } else {
-        const u64* W = tb.inv + (long long)v.basis * tb.N;
-        const u64* WS = tb.inv_shoup + (long long)v.basis * tb.N;
-        group_inv<LOGR>(x, t, my, bf_const(bc.q, bc.negq), W, WS, [row](int u, int i) {
+        group_inv_n<LOGR, true, LPB>(x, t, my, kq, W, WS, [row](int u, int i) {
             return (1 << (LOGN - 1 - u)) + (row << (n2 - 1 - u)) + i;
         });
     }
 #pragma unroll
-    for (int j = 0; j < E; ++j) v.store(off + t + Gr::TG * j, x[j]);
+    for (int i = 0; i < LPB; ++i) {
+        if (!live[i]) continue;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            if constexpr (PRE) v[i].store_pre(off + t + Gr::TG * j, x[i][j], pre[i][j]);
+            else v[i].store(off + t + Gr::TG * j, x[i][j]);
+        }
+    }
 }
 
 #define MI_NTT_LOGN_CASES(K)                                                                          \
@@ -153,44 +231,52 @@ k_ntt_rows(NttTabs tb, Job job) {
         default: throw Error(-1, "ntt: unsupported ring dimension");                                  \
     }
 
-template <bool FWD, class Job>
+template <bool FWD, int LPB, class Job>
 void launch_cols(const Context& c, const Job& job, int count, hipStream_t s) {
     if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
+    const unsigned blocks = (unsigned)((count + LPB - 1) / LPB);
 #define MI_COLS(LN)                                                                                   \
-    hipLaunchKernelGGL((k_ntt_cols<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN / 2, LN - LN / 2)), \
-                       dim3(pass_threads(LN / 2, LN - LN / 2)), 0, s, tb, job)
+    hipLaunchKernelGGL((k_ntt_cols<LN, FWD, LPB, Job>), dim3(blocks * pass_wpl(LN / 2, LN - LN / 2)),  \
+                       dim3(pass_threads(LN / 2, LN - LN / 2)), 0, s, tb, job, count)
     MI_NTT_LOGN_CASES(MI_COLS)
 #undef MI_COLS
     MI_HIP(hipGetLastError());
 }
 
-template <bool FWD, class Job>
+template <bool FWD, int LPB, class Job>
 void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
     if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
+    const unsigned blocks = (unsigned)((count + LPB - 1) / LPB);
 #define MI_ROWS(LN)                                                                                   \
-    hipLaunchKernelGGL((k_ntt_rows<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN - LN / 2, LN / 2)), \
-                       dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job)
+    hipLaunchKernelGGL((k_ntt_rows<LN, FWD, LPB, Job>), dim3(blocks * pass_wpl(LN - LN / 2, LN / 2)),  \
+                       dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job, count)
     MI_NTT_LOGN_CASES(MI_ROWS)
 #undef MI_ROWS
     MI_HIP(hipGetLastError());
 }
 
 // Forward NTT: first pass runs job `in` (load input, store intermediate),
-// second pass runs job `out` (load intermediate, store output).
+// second pass runs job `out` (load intermediate, store output).  The limbs
+// per workgroup of each pass come from the context's NTT plan for this
+// launch (ntt_lpb: the autotuned choice for the ring, limb count and pass).
 template <class JobIn, class JobOut>
 void run_ntt_fwd(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
     const long probe = c.probe_open(s, count);
-    launch_cols<true>(c, in, count, s);
-    launch_rows<true>(c, out, count, s);
+    if (c.ntt_lpb(count, true, false) == 2) launch_cols<true, 2>(c, in, count, s);
+    else launch_cols<true, 1>(c, in, count, s);
+    if (c.ntt_lpb(count, true, true) == 2) launch_rows<true, 2>(c, out, count, s);
+    else launch_rows<true, 1>(c, out, count, s);
     c.probe_close(probe, s);
 }
 template <class JobIn, class JobOut>
 void run_ntt_inv(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
     const long probe = c.probe_open(s, count);
-    launch_rows<false>(c, in, count, s);
-    launch_cols<false>(c, out, count, s);
+    if (c.ntt_lpb(count, false, true) == 2) launch_rows<false, 2>(c, in, count, s);
+    else launch_rows<false, 1>(c, in, count, s);
+    if (c.ntt_lpb(count, false, false) == 2) launch_cols<false, 2>(c, out, count, s);
+    else launch_cols<false, 1>(c, out, count, s);
     c.probe_close(probe, s);
 }
 

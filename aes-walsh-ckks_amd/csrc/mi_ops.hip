@@ -372,10 +372,8 @@ struct RescaleOutJob {
         const u64* half;
         u64 q, ip, ips;
         __device__ u64 load(int i) const { return src[i]; }
-        __device__ void store(int i, u64 v) const {
-            const u64 num = add_mod(x[i], half[i], q);
-            x[i] = shoup(sub_mod(num, v, q), ip, ips, q);
-        }
+        __device__ u64 pre(int i) const { return add_mod(x[i], half[i], q); }
+        __device__ void store_pre(int i, u64 v, u64 num) const { x[i] = shoup(sub_mod(num, v, q), ip, ips, q); }
     };
     __device__ View view(int k) const {
         const int i = per_limb.div(k), poly = k - i * npolys;  // modulus-major

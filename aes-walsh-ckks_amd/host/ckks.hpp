@@ -234,6 +234,28 @@ private:
     size_t words_ = 0;
 };
 
+// A switching key as held in HBM: [parts][2 (b, a)][nq+np][N], folded (the
+// form the context generates).  CKKSContext::getRotationKey, getRelinHybrid,
+// getConjugationKey and the dense/sparse switch-key getters return it where
+// the reference returns a span of per-digit key ciphertexts.
+struct CKKSSwitchingKey {
+    const uint64_t* device = nullptr;
+};
+
+// ckks_keyswitch.hpp:70-86: the hoisted QP digits of a ciphertext's a, kept
+// in HBM ([parts][L+K][N], evaluation domain).  The workspace is the
+// context's own per-stream scratch, so CKKSHoistWorkspace carries nothing.
+struct CKKSEvalHoistedCiphertextQP {
+    size_t N = 0;
+    size_t level = 0;
+    size_t q_limb_count = 0;
+    size_t p_limb_count = 0;
+    long double scale = 0.0L;
+    MyVector<uint64_t> moduli;
+    DeviceBuffer digits;
+};
+struct CKKSHoistWorkspace {};
+
 // ---------------------------------------------------------- ciphertext types
 // CKKSCiphertext (ckks_ciphertext.hpp:18-80): polys [b, a, (c2)], each
 // (level+1)*N words in the evaluation domain.  Move-only; clone() copies.
@@ -459,6 +481,16 @@ public:
     size_t getRotationKeyCount() const { return K().rotation_keys.size(); }
     bool hasRotationKey(int64_t rotation) const;
     const uint64_t* getRotationKeyDevice(int64_t rotation) const;
+    // the reference's key getters (ckks_context.hpp:78-110) over the device keys
+    CKKSSwitchingKey getRotationKey(int64_t rotation) const { return {getRotationKeyDevice(rotation)}; }
+    CKKSSwitchingKey getRelinHybrid(size_t /*level*/) const { return {getRelinKeyDevice()}; }
+    CKKSSwitchingKey getConjugationKey() const { return {getConjugationKeyDevice()}; }
+    CKKSSwitchingKey getDenseToSparseSwitchKey() const { return {getDenseToSparseSwitchKeyDevice()}; }
+    CKKSSwitchingKey getSparseToDenseSwitchKey() const { return {getSparseToDenseSwitchKeyDevice()}; }
+    // build_ntt_automorphism_map (rotation_key_generation.cpp:80-90) of a
+    // rotation's and of the conjugation's Galois element
+    MyVector<size_t> getRotationNttMap(int64_t rotation) const;
+    MyVector<size_t> getConjugationNttMap() const;
     std::vector<int64_t> getRotationKeyValues() const;
     // Generates the key for one more rotation (normalised mod N/2).
     void addRotationKey(int64_t rotation);
@@ -594,6 +626,9 @@ MyVector<Complex> ckks_decrypt(const CKKSContext& context, const CKKSCiphertext&
 std::vector<MyVector<Complex>> ckks_decrypt_batch(const CKKSContext& context,
                                                   const std::vector<const CKKSCiphertext*>& cts,
                                                   CKKSDecryptDomain domain = CKKSDecryptDomain::Slots);
+// The decrypted plaintext's coefficients (b + a s, inverse NTT), L x N
+// residues, downloaded: the input of ckks_decode_plaintext.
+MyVector<uint64_t> ckks_decrypt_coefficients(const CKKSContext& context, const CKKSCiphertext& ct);
 // Decode of a coefficient-domain integer plaintext (first limbs whose product
 // fits 128 bits, centred) at `scale` (ckks_encoding.cpp:75-170, 240-329).
 MyVector<Complex> ckks_decode_plaintext(const CKKSContext& context, const MyVector<uint64_t>& coeffs,
@@ -722,6 +757,14 @@ void zero_polys(const CKKSContext& c, CKKSCiphertext& ct, size_t from, const cha
 // scale raised to `target` by an integer factor (an all-zero ciphertext only
 // takes the new scale).
 void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double target);
+// ckks_mul_inplace(ct1, ct2) with ct2 read at ct1's level (ct2 at that level
+// or above): the result equals the product with a copy of ct2 dropped to
+// ct1's level, without the copy.
+void mul_at_level(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+// ckks_batch_bootstrap_evalmod_from_complex_packed with its two output lanes
+// stacked into one ciphertext of twice the members (out0's members, then
+// out1's), the EvalMods of both lanes run as that one batch.
+CKKSCiphertext bootstrap_lanes_stacked(const CKKSContext& c, CKKSCiphertext packed);
 }  // namespace ckks_detail
 
 // ckks_aes_ctr_walsh_sbox.hpp:16-21 / .cpp:979-1035: the 1408 expanded
@@ -863,6 +906,35 @@ CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& context, CKKS
 // ckks_bootstrap_evalmod_inplace.cpp:156-434.
 void ckks_key_switch_dense_to_sparse_inplace(const CKKSContext& context, CKKSCiphertext& ct);
 void ckks_key_switch_sparse_to_dense_inplace(const CKKSContext& context, CKKSCiphertext& ct);
+
+// ---- the keyswitch-level API (ckks_keyswitch.hpp:59-149) ----
+// Polynomials are host vectors as in the reference (uploaded, switched on the
+// device, downloaded); switching keys are the device-resident CKKSSwitchingKey.
+void ckks_apply_ntt_automorphism_to_flat_rns_poly_inplace(MyVector<uint64_t>& poly, size_t N, size_t limb_count,
+                                                          std::span<const size_t> dst_to_src_map);
+void ckks_hoist_ciphertext_a_qp_eval(const CKKSContext& context, const CKKSCiphertext& ct,
+                                     CKKSHoistWorkspace& workspace, CKKSEvalHoistedCiphertextQP& out);
+void ckks_hybrid_key_switch_inplace(const CKKSContext& context, size_t level, MyVector<uint64_t>& key_term,
+                                    CKKSSwitchingKey switching_key, MyVector<uint64_t>& b_out,
+                                    MyVector<uint64_t>& a_out, CKKSHybridKeySwitchCombine b_mode,
+                                    CKKSHybridKeySwitchCombine a_mode, bool switching_key_montgomery = false);
+void ckks_hybrid_key_switch_hoisted_inplace(const CKKSContext& context,
+                                            const CKKSEvalHoistedCiphertextQP& hoisted_key_term,
+                                            CKKSSwitchingKey switching_key, MyVector<uint64_t>& b_out,
+                                            MyVector<uint64_t>& a_out, CKKSHybridKeySwitchCombine b_mode,
+                                            CKKSHybridKeySwitchCombine a_mode, bool switching_key_montgomery,
+                                            std::span<const size_t> ntt_map = {});
+void ckks_hybrid_key_switch_hoisted_qp_rotated(const CKKSContext& context,
+                                               const CKKSEvalHoistedCiphertextQP& hoisted_key_term,
+                                               CKKSSwitchingKey switching_key, const MyVector<uint64_t>& b_q_eval,
+                                               MyVector<uint64_t>& b_qp_out, MyVector<uint64_t>& a_qp_out,
+                                               bool switching_key_montgomery, std::span<const size_t> ntt_map);
+void ckks_hybrid_key_switch_qp_from_key_term(const CKKSContext& context, size_t level,
+                                             const MyVector<uint64_t>& key_term_q_eval,
+                                             CKKSSwitchingKey switching_key, MyVector<uint64_t>& b_qp_out,
+                                             MyVector<uint64_t>& a_qp_out, bool switching_key_montgomery);
+void ckks_key_switch_inplace(const CKKSContext& context, CKKSCiphertext& ct, CKKSSwitchingKey switching_key,
+                             bool switching_key_montgomery = false);
 void ckks_modraise_inplace(const CKKSContext& context, CKKSCiphertext& ct, size_t target_level);
 CKKSCiphertext ckks_s2c_generalized(const CKKSContext& context, const CKKSCiphertext& ct, size_t stage_count);
 CKKSCiphertext ckks_c2s_generalized(const CKKSContext& context, const CKKSCiphertext& ct, double transform_scale);

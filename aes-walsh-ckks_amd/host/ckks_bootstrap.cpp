@@ -224,28 +224,46 @@ std::pair<CKKSCiphertext, CKKSCiphertext> ckks_bootstrap_pre_evalmod(const CKKSC
     sparse_trace(c, packed);
     packed = ckks_c2s_generalized(c, packed, c.getEvalModScaledC2STransformScale());
     CKKSCiphertext conj = ckks_conjugate(c, packed);
+    const long double scale = packed.getScale();
     CKKSCiphertext out0 = packed.clone();
     ckks_add_inplace(c, out0, conj);
-    out0.setScale(packed.getScale());
-    CKKSCiphertext out1 = packed.clone();
+    out0.setScale(scale);
+    // the reference's second copy of packed is its last use: packed itself
+    CKKSCiphertext out1 = std::move(packed);
     ckks_sub_inplace(c, out1, conj);
-    out1 = ckks_mul(c, out1, Complex(0.0, -1.0), 1.0);
-    out1.setScale(packed.getScale());
+    ckks_mul_inplace(c, out1, Complex(0.0, -1.0), 1.0);
+    out1.setScale(scale);
     return {std::move(out0), std::move(out1)};
 }
 
 std::pair<CKKSCiphertext, CKKSCiphertext> ckks_batch_bootstrap_evalmod_from_complex_packed(const CKKSContext& c,
                                                                                            CKKSCiphertext packed) {
+    const size_t B = packed.getBatchSize();
+    auto lanes = ckks_detail::bootstrap_lanes_stacked(c, std::move(packed)).unstack();
+    MyVector<CKKSCiphertext> m0, m1;
+    for (size_t m = 0; m < B; ++m) {
+        m0.push_back(std::move(lanes[m]));
+        m1.push_back(std::move(lanes[B + m]));
+    }
+    return {CKKSCiphertext::stack(std::move(m0)), CKKSCiphertext::stack(std::move(m1))};
+}
+
+CKKSCiphertext ckks_detail::bootstrap_lanes_stacked(const CKKSContext& c, CKKSCiphertext packed) {
     const char* label = "ckks_batch_bootstrap_evalmod_from_complex_packed";
     validate_packed_input(c, packed, label);
     const uint64_t ratio = input_scale_ratio(c, packed, label);
-    packed = ckks_apply_linear_transform_plan(c, packed.clone(), c.getScaledS2CPlan(ratio));
+    packed = ckks_apply_linear_transform_plan(c, std::move(packed), c.getScaledS2CPlan(ratio));
     // the S2C output's residues are normalised to q0; its scale restarts at the default
     packed.setScale(c.getParams().getScale());
     auto [out0, out1] = ckks_bootstrap_pre_evalmod(c, std::move(packed));
-    ckks_evalmod_binary_inplace(c, out0);
-    ckks_evalmod_binary_inplace(c, out1);
-    return {std::move(out0), std::move(out1)};
+    // the two lanes have one level and scale: their EvalMods (evalmod_binary_inplace
+    // of out0, then of out1) run as one batch of twice the members
+    MyVector<CKKSCiphertext> both;
+    both.push_back(std::move(out0));
+    both.push_back(std::move(out1));
+    CKKSCiphertext lanes = CKKSCiphertext::stack(std::move(both));
+    ckks_evalmod_binary_inplace(c, lanes);
+    return lanes;
 }
 
 void ckks_batch_bootstrap_evalmod_inplace(const CKKSContext& c, CKKSCiphertext& ct0, CKKSCiphertext& ct1) {

@@ -570,6 +570,8 @@ CKKSContext::CKKSContext(CKKSParams p) : params(std::move(p)) {
             std::vector<uint64_t> perm(perm_table.begin(), perm_table.end());
             mi_check(mi_set_encoding_tables(mi, reinterpret_cast<const double*>(twiddle_fft.data()), perm.data()),
                      "ckks_context");
+            mi_check(mi_set_decoding_tables(mi, reinterpret_cast<const double*>(twiddle_ifft.data()), perm.data()),
+                     "ckks_context");
         }
         Rng rng(params.getSeed());
         K().secret_coeffs = ckks_host::hamming_weight_ternary(N, params.getSecretKeyHammingWeight(), rng);
@@ -636,6 +638,8 @@ CKKSContext::CKKSContext(const CKKSContext& owner, int)
         mi_check(mi_stream_create(mi, &stream_), "ckks_context");
         std::vector<uint64_t> perm(perm_table.begin(), perm_table.end());
         mi_check(mi_set_encoding_tables(mi, reinterpret_cast<const double*>(twiddle_fft.data()), perm.data()),
+                 "ckks_context");
+        mi_check(mi_set_decoding_tables(mi, reinterpret_cast<const double*>(twiddle_ifft.data()), perm.data()),
                  "ckks_context");
     } catch (...) {
         if (stream_) mi_stream_destroy(mi, stream_);

@@ -14,6 +14,8 @@
 #include <bit>
 #include <cmath>
 
+#include <map>
+
 #include "ckks.hpp"
 #include "ckks_rng.hpp"
 
@@ -258,47 +260,84 @@ MyVector<Complex> decode_plaintext_custom_basis(size_t N, const MyVector<uint64_
 }
 
 // ckks_decrypt_with_secret_eval (ckks_encoding.cpp:96-170): m = b + a*s (+ c2*s^2)
-// on the device, INTT, then the host decode.
+// and the inverse NTT on the device, then the decode (CRT, scale division,
+// inverse special FFT) on the device too (mi_decode_batch, equal to the host
+// decode above bit for bit); only the slots come back to the host.
 namespace {
-// The device part of ckks_decrypt: b + a s (+ c2 s^2), inverse NTT, copied to the host.
-MyVector<uint64_t> decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret);
+// The device part of ckks_decrypt: b + a s (+ c2 s^2) and the inverse NTT.
+DeviceBuffer decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret);
+
+// scale = mant 2^exp with the top bit of mant set (the x87 extended value)
+void split_scale(long double scale, uint64_t& mant, int32_t& exp) {
+    if (!(scale > 0.0L) || !std::isfinite(scale)) throw std::invalid_argument("ckks_decrypt: invalid ciphertext scale");
+    int e = 0;
+    const long double f = std::frexp(scale, &e);
+    mant = (uint64_t)std::ldexp(f, 64);
+    exp = e - 64;
+}
+
+// Decodes plaintexts of one limb count already on the device.
+std::vector<MyVector<Complex>> decode_on_device(const CKKSContext& c, const std::vector<const uint64_t*>& coeffs,
+                                               size_t limbs, const std::vector<long double>& scales,
+                                               CKKSDecryptDomain domain) {
+    const size_t n = coeffs.size(), N = c.getParams().getN(), half = N / 2;
+    std::vector<uint64_t> mant(n);
+    std::vector<int32_t> ex(n);
+    for (size_t i = 0; i < n; ++i) split_scale(scales[i], mant[i], ex[i]);
+    DeviceBuffer out(c.handle(), n * N);  // n x N/2 complex
+    mi_check(mi_decode_batch(c.handle(), (int)n, coeffs.data(), (int)limbs, mant.data(), ex.data(),
+                             domain == CKKSDecryptDomain::Coefficients ? 1 : 0, reinterpret_cast<double*>(out.data()),
+                             c.stream()),
+             "ckks_decrypt");
+    std::vector<double> h(n * N);
+    mi_check(mi_memcpy(c.handle(), h.data(), out.data(), h.size() * 8, MI_COPY_D2H, c.stream()), "ckks_decrypt");
+    std::vector<MyVector<Complex>> res(n, MyVector<Complex>(half));
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < half; ++j) res[i][j] = Complex(h[i * N + 2 * j], h[i * N + 2 * j + 1]);
+    return res;
+}
 }  // namespace
 
 MyVector<Complex> ckks_decrypt(const CKKSContext& c, const CKKSCiphertext& ct, CKKSDecryptDomain domain,
                                bool use_low_secret) {
-    auto h = decrypt_coefficients(c, ct, use_low_secret);
-    return ckks_decode_plaintext(c, h, ct.getLevel() + 1, ct.getScale(), domain);
+    DeviceBuffer t = decrypt_coefficients(c, ct, use_low_secret);
+    return std::move(decode_on_device(c, {t.data()}, ct.getLevel() + 1, {ct.getScale()}, domain)[0]);
 }
 
 std::vector<MyVector<Complex>> ckks_decrypt_batch(const CKKSContext& c, const std::vector<const CKKSCiphertext*>& cts,
                                                   CKKSDecryptDomain domain) {
-    std::vector<MyVector<uint64_t>> coeffs;
+    std::vector<DeviceBuffer> coeffs;
     coeffs.reserve(cts.size());
     for (const CKKSCiphertext* ct : cts) {
         if (!ct) throw std::invalid_argument("ckks_decrypt: null ciphertext");
         coeffs.push_back(decrypt_coefficients(c, *ct, false));
     }
+    // one decode launch sequence per limb count
     std::vector<MyVector<Complex>> out(cts.size());
-    const size_t workers = std::max<size_t>(1, std::min<size_t>(cts.size(), std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
-    std::vector<std::exception_ptr> errors(workers);
-    std::vector<std::thread> threads;
-    for (size_t w = 0; w < workers; ++w)
-        threads.emplace_back([&, w] {
-            try {
-                for (size_t i = w; i < cts.size(); i += workers)
-                    out[i] = ckks_decode_plaintext(c, coeffs[i], cts[i]->getLevel() + 1, cts[i]->getScale(), domain);
-            } catch (...) {
-                errors[w] = std::current_exception();
-            }
-        });
-    for (auto& t : threads) t.join();
-    for (auto& e : errors)
-        if (e) std::rethrow_exception(e);
+    std::map<size_t, std::vector<size_t>> by_limbs;
+    for (size_t i = 0; i < cts.size(); ++i) by_limbs[cts[i]->getLevel() + 1].push_back(i);
+    for (const auto& [limbs, idx] : by_limbs) {
+        std::vector<const uint64_t*> ptrs;
+        std::vector<long double> scales;
+        for (size_t i : idx) {
+            ptrs.push_back(coeffs[i].data());
+            scales.push_back(cts[i]->getScale());
+        }
+        auto dec = decode_on_device(c, ptrs, limbs, scales, domain);
+        for (size_t k = 0; k < idx.size(); ++k) out[idx[k]] = std::move(dec[k]);
+    }
     return out;
 }
 
+MyVector<uint64_t> ckks_decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct) {
+    DeviceBuffer t = decrypt_coefficients(c, ct, false);
+    MyVector<uint64_t> h((ct.getLevel() + 1) * c.getParams().getN());
+    mi_check(mi_memcpy(c.handle(), h.data(), t.data(), h.size() * 8, MI_COPY_D2H, c.stream()), "ckks_decrypt");
+    return h;
+}
+
 namespace {
-MyVector<uint64_t> decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret) {
+DeviceBuffer decrypt_coefficients(const CKKSContext& c, const CKKSCiphertext& ct, bool use_low_secret) {
     if (use_low_secret || ct.getSecretOwner() == CKKSSecretOwner::BootstrapSparse)
         throw std::runtime_error("ckks_context: sparse bootstrap secret key is unavailable");
     const size_t polys = ct.getNumPolys();
@@ -320,8 +359,6 @@ MyVector<uint64_t> decrypt_coefficients(const CKKSContext& c, const CKKSCipherte
         mi_check(mi_stream_synchronize(mi, c.stream()), "ckks_decrypt");
     }
     mi_check(mi_ntt_inverse(mi, t.data(), 1, (int)L, W, 0, c.stream()), "ckks_decrypt");
-    MyVector<uint64_t> h(L * N);
-    mi_check(mi_memcpy(mi, h.data(), t.data(), h.size() * 8, MI_COPY_D2H, c.stream()), "ckks_decrypt");
-    return h;
+    return t;
 }
 }  // namespace

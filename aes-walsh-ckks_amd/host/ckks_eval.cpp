@@ -51,8 +51,14 @@ void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, const CKKSEncodi
 }
 
 void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {
+    if (ct1.getN() != ct2.getN() || ct1.getLevel() != ct2.getLevel())
+        throw std::invalid_argument("ckks_mul_inplace(ct,ct): ciphertext size mismatch");
+    ckks_detail::mul_at_level(c, ct1, ct2);
+}
+
+void ckks_detail::mul_at_level(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2) {
     const size_t lv = ct1.getLevel();
-    if (ct1.getN() != ct2.getN() || lv != ct2.getLevel())
+    if (ct1.getN() != ct2.getN() || lv > ct2.getLevel())
         throw std::invalid_argument("ckks_mul_inplace(ct,ct): ciphertext size mismatch");
     check_ct(c, ct1, "ckks_mul_inplace(ct,ct)");
     if (ct1.getNumPolys() != 2 || ct2.getNumPolys() != 2)

@@ -176,18 +176,56 @@ void mul_integer(const CKKSContext& c, CKKSCiphertext& ct, ld scalar, ld out_sca
     ct.setScale((double)out_scale);
 }
 
-void mul_real_then_add(const CKKSContext& c, const CKKSCiphertext& src, double scalar, ld scalar_scale,
-                       CKKSCiphertext& dst) {
-    if (src.getN() != dst.getN()) throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add size mismatch");
-    if (src.getLevel() != dst.getLevel())
-        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add level mismatch");
-    if (src.getNumPolys() != dst.getNumPolys())
-        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add degree mismatch");
+// out -= s * src for the integer s = round(scalar) (mul_integer's checks),
+// src read at out's level over its polys
+void sub_integer_multiple(const CKKSContext& c, CKKSCiphertext& out, const CKKSCiphertext& src, ld scalar) {
+    if (!std::isfinite(scalar) || scalar < 0.0L) throw std::invalid_argument("ckks_evalchebyshev: invalid scalar scale-up");
+    const ld r = std::round(scalar);
+    if (r < 1.0L || std::abs(scalar - r) > std::max<ld>(1.0L, std::abs(scalar)) * 0x1p-20L)
+        throw std::invalid_argument("ckks_evalchebyshev: non-integral scalar scale-up");
+    if (r > (ld)std::numeric_limits<uint64_t>::max())
+        throw std::invalid_argument("ckks_evalchebyshev: scalar scale-up is too large");
+    const size_t level = out.getLevel();
+    if (src.getLevel() < level) throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
+    if (src.getNumPolys() > out.getNumPolys() || src.getBatchSize() != out.getBatchSize())
+        throw std::invalid_argument("ckks_evalchebyshev: cannot shrink ciphertext degree by padding");
+    const uint64_t s = (uint64_t)r;
     const auto& q = c.getParams().getModuli();
-    const size_t lv = src.getLevel();
-    std::vector<uint64_t> r(lv + 1);
-    for (size_t l = 0; l <= lv; ++l) r[l] = scaled_constant_mod_q(scalar, scalar_scale, q[l]);
-    ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR_ADD, dst, &src, r.data(), "ckks_evalpoly");
+    std::vector<uint64_t> neg(level + 1);
+    for (size_t l = 0; l <= level; ++l) neg[l] = (q[l] - s % q[l]) % q[l];
+    std::vector<uint64_t*> x;
+    std::vector<const uint64_t*> y;
+    for (size_t k = 0; k < src.getNumPolys(); ++k)
+        for (size_t m = 0; m < src.getBatchSize(); ++m) {
+            x.push_back(out.polyData(k, m));
+            y.push_back(src.polyData(k, m));
+        }
+    mi_check(mi_poly_list_op(c.handle(), MI_LIST_MUL_SCALAR_ADD, (int)level, (int)x.size(), x.data(), y.data(),
+                             neg.data(), c.stream()),
+             "ckks_evalpoly");
+}
+
+// out += round(scalar * scalar_scale) * src over src's polys, src read at
+// out's level (src at that level or above, no more polys than out: out's
+// extra polys take + 0)
+void mul_real_then_add_at(const CKKSContext& c, const CKKSCiphertext& src, size_t level, double scalar,
+                          ld scalar_scale, CKKSCiphertext& dst) {
+    if (src.getN() != dst.getN() || src.getLevel() < level || dst.getLevel() != level ||
+        src.getNumPolys() > dst.getNumPolys() || src.getBatchSize() != dst.getBatchSize())
+        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add shape mismatch");
+    const auto& q = c.getParams().getModuli();
+    std::vector<uint64_t> r(level + 1);
+    for (size_t l = 0; l <= level; ++l) r[l] = scaled_constant_mod_q(scalar, scalar_scale, q[l]);
+    std::vector<uint64_t*> x;
+    std::vector<const uint64_t*> y;
+    for (size_t k = 0; k < src.getNumPolys(); ++k)
+        for (size_t m = 0; m < src.getBatchSize(); ++m) {
+            x.push_back(dst.polyData(k, m));
+            y.push_back(src.polyData(k, m));
+        }
+    mi_check(mi_poly_list_op(c.handle(), MI_LIST_MUL_SCALAR_ADD, (int)level, (int)x.size(), x.data(), y.data(),
+                             r.data(), c.stream()),
+             "ckks_evalpoly");
 }
 
 void raise_scale_to(const CKKSContext& c, CKKSCiphertext& ct, ld target) {
@@ -231,26 +269,28 @@ void mul_then_add(const CKKSContext& c, const CKKSCiphertext& op0, double scalar
     const double out_scale = out.getScale();
     const size_t polys = std::max(op0.getNumPolys(), out.getNumPolys());
     out = aligned(c, std::move(out), level, out_scale, polys);
-    CKKSCiphertext a = aligned(c, op0.clone(), level, op0.getScale(), polys);
+    // op0 is read in place at `level` (its first level+1 limbs); the
+    // reference's aligned copy only pads it with zero polys, which add nothing
+    const double a_scale = op0.getScale();
     ld s_rlwe = 0.0L;
-    if (scales_match(a.getScale(), out.getScale())) {
+    if (scales_match(a_scale, out.getScale())) {
         if (near_integer(scalar)) {
             s_rlwe = 1.0L;
         } else {
             s_rlwe = rescale_divisor(c, level);
             raise_scale_to(c, out, out.getScale() * s_rlwe);
         }
-    } else if (a.getScale() < out.getScale()) {
-        s_rlwe = out.getScale() / a.getScale();
+    } else if (a_scale < out.getScale()) {
+        s_rlwe = out.getScale() / a_scale;
     } else {
         std::ostringstream os;
-        os << "ckks_evalchebyshev: MulThenAdd requires op0.Scale <= opOut.Scale (op0=" << (double)a.getScale()
+        os << "ckks_evalchebyshev: MulThenAdd requires op0.Scale <= opOut.Scale (op0=" << a_scale
            << ", opOut=" << (double)out.getScale() << ", level=" << level << ")";
         throw std::invalid_argument(os.str());
     }
-    const ld term = a.getScale() * s_rlwe;
+    const ld term = a_scale * s_rlwe;
     if (!scales_match((double)term, out.getScale())) throw std::runtime_error("ckks_evalchebyshev: MulThenAdd scale mismatch");
-    mul_real_then_add(c, a, scalar, s_rlwe, out);
+    mul_real_then_add_at(c, op0, level, scalar, s_rlwe, out);
 }
 
 // ---- simulated (level, scale) planning
@@ -388,8 +428,7 @@ private:
         } else if (ct_.at(b).getLevel() == level) {
             ckks_mul_inplace(c_, out, ct_.at(b));   // the reference's clone at the same level changes nothing
         } else {
-            CKKSCiphertext rhs = ct_.at(b).cloneAtLevel(level);
-            ckks_mul_inplace(c_, out, rhs);
+            ckks_detail::mul_at_level(c_, out, ct_.at(b));  // read at `level`, as the reference's copy
         }
         ckks_relin_hybrid_inplace(c_, out);
         out.setScale(prod.value_d(q_));
@@ -399,13 +438,10 @@ private:
             add_scaled_constant(c_, out, -1.0, prod.value(q_));
         } else {
             gen(d);
-            CKKSCiphertext corr = ct_.at(d).clone();
+            // out -= ratio * T_d (T_d read at out's level): one fused
+            // multiply-add by -ratio mod q instead of a scaled copy and a subtraction
             const uint64_t ratio = scale_ratio_floor(prod, sc_.at(d), q_);
-            corr.setScale(sc_.at(d).value_d(q_));
-            const double cs = corr.getScale();
-            corr = aligned(c_, std::move(corr), out.getLevel(), cs, out.getNumPolys());
-            mul_integer(c_, corr, (ld)ratio, prod.value(q_));
-            ckks_sub_inplace(c_, out, corr);
+            sub_integer_multiple(c_, out, ct_.at(d), (ld)ratio);
         }
         ct_.insert_or_assign(n, std::move(out));
         sc_.insert_or_assign(n, prod);
@@ -443,9 +479,8 @@ CKKSCiphertext eval_leaf(const CKKSContext& c, const Poly& p, const Powers& pw) 
 void combine_giant(const CKKSContext& c, CKKSCiphertext& even, CKKSCiphertext& odd, const CKKSCiphertext& xpow) {
     if (odd.getNumPolys() == 3) ckks_relin_hybrid_inplace(c, odd);
     ckks_rescale(c, odd);
-    CKKSCiphertext xp = xpow.clone();
-    ckks_drop_to_level_inplace(xp, odd.getLevel());
-    ckks_mul_inplace(c, odd, xp);
+    if (xpow.getLevel() < odd.getLevel()) throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
+    ckks_detail::mul_at_level(c, odd, xpow);  // x^(2^k) read at odd's level, as the reference's dropped copy
     const double es = even.getScale();
     CKKSCiphertext ea = aligned(c, std::move(even), odd.getLevel(), es, odd.getNumPolys());
     if (!scales_match(ea.getScale(), odd.getScale())) {

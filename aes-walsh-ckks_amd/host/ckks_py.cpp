@@ -194,6 +194,13 @@ PYBIND11_MODULE(ckks_py, m) {
         .def("getEvalModS2CPlan", &CKKSContext::getEvalModS2CPlan, py::return_value_policy::reference_internal)
         .def("getEvalModScaledC2SPlan", &CKKSContext::getEvalModScaledC2SPlan, py::return_value_policy::reference_internal)
         .def("getEvalModScaledC2STransformScale", &CKKSContext::getEvalModScaledC2STransformScale)
+        .def("tuneReport", [](const CKKSContext& c) {
+            std::vector<char> buf(4096);
+            mi_check(mi_tune_report(c.handle(), buf.data(), buf.size()), "tuneReport");
+            return std::string(buf.data());
+        })
+        .def("getRotationNttMap", &CKKSContext::getRotationNttMap)
+        .def("getConjugationNttMap", &CKKSContext::getConjugationNttMap)
         .def("memoryStats", [](const CKKSContext& c) {
             int64_t st[4];
             mi_check(mi_memory_stats(c.handle(), st), "memoryStats");
@@ -290,6 +297,14 @@ PYBIND11_MODULE(ckks_py, m) {
               return res;
           },
           py::arg("context"), py::arg("cts"), py::arg("domain") = CKKSDecryptDomain::Slots);
+    // the decrypted coefficients decoded on the host (the reference's
+    // arithmetic, the ciphertext's exact scale): the check of the device decode
+    m.def("ckks_decrypt_host_decode",
+          [](const CKKSContext& c, const CKKSCiphertext& ct, CKKSDecryptDomain d) {
+              return complex_np(ckks_decode_plaintext(c, ckks_decrypt_coefficients(c, ct), ct.getLevel() + 1,
+                                                      ct.getScale(), d));
+          },
+          py::arg("context"), py::arg("ct"), py::arg("domain") = CKKSDecryptDomain::Slots);
     m.def("ckks_decode_plaintext",
           [](const CKKSContext& c, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> coeffs, size_t limbs,
              double scale, CKKSDecryptDomain d) {
@@ -438,6 +453,66 @@ PYBIND11_MODULE(ckks_py, m) {
           py::arg("diagonal_indices"), py::arg("baby_step_count"), py::arg("slot_count") = 0,
           py::arg("rescale_after") = true, py::keep_alive<0, 1>());
     m.def("ckks_key_switch_dense_to_sparse_inplace", &ckks_key_switch_dense_to_sparse_inplace);
+    // the keyswitch-level API: keys by name (rotation r, "relin", "conj"),
+    // polynomials as numpy arrays (copied in, results returned)
+    auto key_of = [](const CKKSContext& c, py::object which) -> CKKSSwitchingKey {
+        if (py::isinstance<py::str>(which)) {
+            const std::string w = which.cast<std::string>();
+            if (w == "relin") return c.getRelinHybrid(0);
+            if (w == "conj") return c.getConjugationKey();
+            if (w == "d2s") return c.getDenseToSparseSwitchKey();
+            if (w == "s2d") return c.getSparseToDenseSwitchKey();
+            throw std::invalid_argument("unknown switching key " + w);
+        }
+        return c.getRotationKey(which.cast<int64_t>());
+    };
+    auto combine = [](int m) { return m ? CKKSHybridKeySwitchCombine::Assign : CKKSHybridKeySwitchCombine::Add; };
+    m.def("ckks_hybrid_key_switch_inplace",
+          [key_of, combine](const CKKSContext& c, size_t level, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> kt,
+                            py::object key, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> b,
+                            py::array_t<uint64_t, py::array::c_style | py::array::forcecast> a, int bm, int am, bool mont) {
+              MyVector<uint64_t> k = from_np(kt), bo = from_np(b), ao = from_np(a);
+              ckks_hybrid_key_switch_inplace(c, level, k, key_of(c, key), bo, ao, combine(bm), combine(am), mont);
+              return py::make_tuple(to_np(bo), to_np(ao));
+          });
+    m.def("ckks_key_switch_inplace", [key_of](const CKKSContext& c, CKKSCiphertext& ct, py::object key, bool mont) {
+        ckks_key_switch_inplace(c, ct, key_of(c, key), mont);
+    });
+    py::class_<CKKSEvalHoistedCiphertextQP>(m, "CKKSEvalHoistedCiphertextQP")
+        .def(py::init<>())
+        .def_readonly("level", &CKKSEvalHoistedCiphertextQP::level)
+        .def_readonly("q_limb_count", &CKKSEvalHoistedCiphertextQP::q_limb_count)
+        .def_readonly("p_limb_count", &CKKSEvalHoistedCiphertextQP::p_limb_count);
+    m.def("ckks_hoist_ciphertext_a_qp_eval", [](const CKKSContext& c, const CKKSCiphertext& ct) {
+        CKKSHoistWorkspace ws;
+        CKKSEvalHoistedCiphertextQP out;
+        ckks_hoist_ciphertext_a_qp_eval(c, ct, ws, out);
+        return out;
+    }, py::keep_alive<0, 1>());
+    m.def("ckks_hybrid_key_switch_hoisted_inplace",
+          [key_of, combine](const CKKSContext& c, const CKKSEvalHoistedCiphertextQP& h, py::object key,
+                            py::array_t<uint64_t, py::array::c_style | py::array::forcecast> b,
+                            py::array_t<uint64_t, py::array::c_style | py::array::forcecast> a, int bm, int am, bool mont,
+                            std::vector<size_t> ntt_map) {
+              MyVector<uint64_t> bo = from_np(b), ao = from_np(a);
+              ckks_hybrid_key_switch_hoisted_inplace(c, h, key_of(c, key), bo, ao, combine(bm), combine(am), mont,
+                                                     std::span<const size_t>(ntt_map));
+              return py::make_tuple(to_np(bo), to_np(ao));
+          });
+    m.def("ckks_hybrid_key_switch_qp_from_key_term",
+          [key_of](const CKKSContext& c, size_t level, py::array_t<uint64_t, py::array::c_style | py::array::forcecast> kt,
+                   py::object key, bool mont) {
+              MyVector<uint64_t> b, a;
+              ckks_hybrid_key_switch_qp_from_key_term(c, level, from_np(kt), key_of(c, key), b, a, mont);
+              return py::make_tuple(to_np(b), to_np(a));
+          });
+    m.def("ckks_apply_ntt_automorphism_to_flat_rns_poly_inplace",
+          [](py::array_t<uint64_t, py::array::c_style | py::array::forcecast> poly, size_t N, size_t limbs,
+             std::vector<size_t> map) {
+              MyVector<uint64_t> p = from_np(poly);
+              ckks_apply_ntt_automorphism_to_flat_rns_poly_inplace(p, N, limbs, std::span<const size_t>(map));
+              return to_np(p);
+          });
     m.def("ckks_key_switch_sparse_to_dense_inplace", &ckks_key_switch_sparse_to_dense_inplace);
     m.def("ckks_modraise_inplace", &ckks_modraise_inplace);
     m.def("ckks_s2c_generalized", &ckks_s2c_generalized, py::keep_alive<0, 1>());

@@ -584,14 +584,26 @@ MyVector<CKKSCiphertext> ckks_walsh_sbox_complex_blocks_mask_packed(CKKSContext&
     if (input.front().getLevel() != context.getParams().getBootstrapS2CDepth())
         throw std::invalid_argument("ckks_walsh_sbox: input level must equal bootstrap S2C entry depth");
     auto packed = ckks_walsh_build_mask_packed_parity_input(context, input, block_count);
-    // bootstrap_complex_packed_parity_lanes: the parity sums refresh to parity bits
-    auto lanes = ckks_batch_bootstrap_evalmod_from_complex_packed(context, std::move(packed));
-    bit_to_sign(context, lanes.first);
-    bit_to_sign(context, lanes.second);
-    auto unpacked0 = ckks_walsh_unpack_mask_packed_signs(context, lanes.first, block_count);
-    auto unpacked1 = ckks_walsh_unpack_mask_packed_signs(context, lanes.second, block_count);
-    auto paired0 = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked0.first, unpacked0.second);
-    auto paired1 = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked1.first, unpacked1.second);
+    const size_t B = packed.getBatchSize();
+    // bootstrap_complex_packed_parity_lanes: the parity sums refresh to parity
+    // bits.  The two lanes go on as one batch (lane 0's members, then lane
+    // 1's): bit_to_sign, unpacking and recombination of lane 0 and of lane 1
+    // as the same steps over twice the members.
+    CKKSCiphertext lanes = ckks_detail::bootstrap_lanes_stacked(context, std::move(packed));
+    bit_to_sign(context, lanes);
+    auto unpacked = ckks_walsh_unpack_mask_packed_signs(context, lanes, block_count);
+    auto paired = ckks_walsh_recombine_signs_to_paired_outputs(context, unpacked.first, unpacked.second);
+    MyVector<CKKSCiphertext> paired0, paired1;
+    for (auto& p : paired) {
+        auto members = p.unstack();
+        MyVector<CKKSCiphertext> m0, m1;
+        for (size_t m = 0; m < B; ++m) {
+            m0.push_back(std::move(members[m]));
+            m1.push_back(std::move(members[B + m]));
+        }
+        paired0.push_back(CKKSCiphertext::stack(std::move(m0)));
+        paired1.push_back(CKKSCiphertext::stack(std::move(m1)));
+    }
     // combine_paired_lane_outputs_as_complex_bits (:1154-1193)
     std::array<std::optional<CKKSCiphertext>, 8> bits;
     // the eight conjugations of split_real_imag_pair as one key-switch batch

@@ -30,6 +30,10 @@ _SIGS = {
     "mi_memory_stats": (cint, [vp, vp]),
     "mi_set_memory_limit": (cint, [vp, i64]),
     "mi_device_memory_info": (cint, [vp, vp, vp]),
+    "mi_tune_report": (cint, [vp, vp, ctypes.c_size_t]),
+    "mi_set_decoding_tables": (cint, [vp, vp, vp]),
+    "mi_decode_batch": (cint, [vp, cint, vp, cint, vp, vp, cint, vp, vp]),
+    "mi_tune_set": (cint, [vp, vp, cint]),
     "mi_probe_begin": (cint, [vp]),
     "mi_probe_end": (cint, [vp, vp]),
     "mi_memcpy": (cint, [vp, vp, vp, ctypes.c_size_t, cint, vp]),
@@ -250,6 +254,16 @@ class Context:
         check(lib().mi_rescale(self.h, level, ptr(polys), npolys, stride, _stream(stream)))
 
     # ---- key switching
+    def tune_report(self):
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().mi_tune_report(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def tune_set(self, ntt_lpb, keyswitch_group):
+        arr = (ctypes.c_int * 6)(*ntt_lpb)
+        check(lib().mi_tune_set(self.h, arr, int(keyswitch_group)))
+
     def keyswitch(self, level, key_term, key, b_out, a_out, b_mode, a_mode, montgomery=True, stream=None):
         check(lib().mi_keyswitch(self.h, level, ptr(key_term), ptr(key), ptr(b_out), ptr(a_out), b_mode,
                                  a_mode, int(montgomery), _stream(stream)))

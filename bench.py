@@ -137,7 +137,8 @@ def bench_ntt(args, M, torch):
                                "forward + inverse, one launch pair per direction",
                    "limbs": limbs, "round_trip_exact": ok,
                    "plain_one_limb_per_launch_limb_ntt_per_s": plain_rate,
-                   "batched_over_plain": value / plain_rate if plain_rate else None},
+                   "batched_over_plain": value / plain_rate if plain_rate else None,
+                   "tuning": ctx.tune_report()},
         "roofline": roofline, "cpu_baseline": cpu,
     }
 
@@ -797,7 +798,8 @@ def bench_aes(args, M, torch):
                    else f"{len(ctxs)} S-box contexts on host threads",
                    "plain_blocks_per_s": None if plain_s is None else 2 * cb / plain_s,
                    "plain_bit_exact": None if plain_s is None else True,
-                   "context_and_key_setup_s": setup_s},
+                   "context_and_key_setup_s": setup_s,
+                   "tuning": json.loads(ctx.tuneReport())},
         "roofline": roofline,
         "cpu_baseline": cpu_baseline_aes(H, ctx.getParams(), 2 * cb) if args.cpu_baseline else None,
     }

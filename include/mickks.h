@@ -69,6 +69,18 @@ int mi_free(mi_context* ctx, void* ptr);
 int mi_memory_trim(mi_context* ctx);
 int mi_memory_stats(mi_context* ctx, int64_t* stats);
 int mi_set_memory_limit(mi_context* ctx, int64_t bytes);
+/* Autotuned launch plan (no counterpart in the reference): at context
+ * creation the library times the candidate configurations of its tunable
+ * launches on the device (NTT limbs per workgroup per launch-size bucket of
+ * 0-3, 4-15, 16-63, 64-255, 256-1023 and 1024+ limbs; key switches per launch
+ * group, 8/16/32) and keeps the fastest, cached per device and ring dimension
+ * in memory and in $MICKKS_TUNE_CACHE (else $HOME/.cache/mickks/tune.txt);
+ * MICKKS_TUNE=0 keeps the defaults.  Every plan computes the same residues.
+ * mi_tune_report writes the context's plan as a JSON object into buf;
+ * mi_tune_set replaces it (six limbs-per-workgroup values, 1 or 2, and a
+ * group size). */
+int mi_tune_report(mi_context* ctx, char* buf, size_t len);
+int mi_tune_set(mi_context* ctx, const int* ntt_lpb, int keyswitch_group);
 /* NTT probe (measurement only; no counterpart in the reference): between
  * mi_probe_begin and mi_probe_end every forward or inverse NTT launch pair
  * of the context (standalone, or inside key switches, rescales, ModRaise and
@@ -282,6 +294,21 @@ int mi_moddown_qp(mi_context* ctx, int level, uint64_t* qp, void* stream);
 int mi_set_encoding_tables(mi_context* ctx, const double* twiddles, const uint64_t* perm);
 int mi_encode_slots(mi_context* ctx, const double* values, int count, double scale, int limbs, int eval,
                     uint64_t* out, int64_t poly_stride, void* stream);
+/* ---- decoding (ckks_encoding.cpp:254-329, fft.cpp:37-117) ----------------
+ * mi_set_decoding_tables: the context's inverse special FFT twiddles (N
+ * complex as 2N doubles) and slot permutation (N/2 entries below N).
+ * mi_decode_batch: `count` coefficient-domain plaintexts (canonical residues,
+ * `limbs` x N words each, device) -> count x N/2 complex slots (device, 2
+ * doubles each; coeff_domain = 1: the coefficient-domain read-out (m[brev i],
+ * m[N/2 + brev i]) instead of the FFT).  The scale of each is given exactly
+ * as an x87 extended value: significand scale_mant (top bit set) and
+ * exponent scale_exp (scale = scale_mant 2^scale_exp).  The CRT uses the
+ * longest prefix of the chain whose product fits in 128 bits; the results
+ * equal the host decode bit for bit. */
+int mi_set_decoding_tables(mi_context* ctx, const double* twiddles, const uint64_t* perm);
+int mi_decode_batch(mi_context* ctx, int count, const uint64_t* const* coeffs, int limbs,
+                    const uint64_t* scale_mant, const int32_t* scale_exp, int coeff_domain, double* out,
+                    void* stream);
 
 /* ---- sampling (ckks_uniform_mask_polys / ckks_gaussian_noise_polys,
  * ckks_encoding.hpp:11-15; the reference draws from std::random_device-seeded

@@ -1,0 +1,69 @@
+"""The autotuned launch plan: every configuration the tuner can choose (NTT
+limbs per workgroup 1 or 2 in every launch-size bucket, key-switch groups of
+8, 16 and 32) gives the same residues as the oracle, and the context reports
+the plan it runs with."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import product, rand_residues
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tuned_plan_is_reported():
+    M = product()
+    p, q = O.find_primes([60] * 2, [50] * 6, 1 << 13)
+    ctx = M.Context(1 << 13, q, p, 3)
+    r = ctx.tune_report()
+    assert len(r["ntt_limbs_per_workgroup"]) == 6 and set(r["ntt_limbs_per_workgroup"]) <= {1, 2}
+    assert r["keyswitch_group"] in (8, 16, 32)
+    assert r["device"].split()[-1] == "13"
+
+
+@pytest.mark.parametrize("lpb", [1, 2])
+def test_every_ntt_plan_matches_the_oracle(lpb):
+    import torch
+    M = product()
+    N = 1 << 15
+    p, q = O.find_primes([61] * 4, [36, 61, 42, 42] + [38] * 7 + [50, 50], N)
+    ctx = M.Context(N, q, p, 3)
+    ctx.tune_set([lpb] * 6, 32)
+    rng = np.random.default_rng(90 + lpb)
+    basis = list(q) + list(p)
+    # 3 polys x 17 limbs (odd count: the last workgroup of a 2-limb plan has one limb)
+    host = np.concatenate([rand_residues(rng, basis, N) for _ in range(3)])
+    d = M.to_dev(host)
+    ctx.ntt_forward(d, 3, 17)
+    fwd = M.from_dev(d)
+    octx = O.Ctx(N, q, p, 3)
+    for k in (0, 16, 33, 50):  # spot limbs against the oracle (basis index = limb)
+        poly, limb = divmod(k, 17)
+        want = octx.ntt_fwd_basis(host[k * N:(k + 1) * N], [limb])
+        assert np.array_equal(fwd[k * N:(k + 1) * N], want), k
+    ctx.ntt_inverse(d, 3, 17)
+    assert np.array_equal(M.from_dev(d), host)
+
+
+@pytest.mark.parametrize("group", [8, 16, 32])
+def test_every_keyswitch_group_matches_the_oracle(group):
+    M = product()
+    N = 1 << 12
+    p, q = O.find_primes([55] * 3, [45] * 9, N)
+    ctx = M.Context(N, q, p, 3)
+    octx = O.Ctx(N, q, p, 3)
+    ctx.tune_set([1] * 6, group)
+    rng = np.random.default_rng(group)
+    basis = list(q) + list(p)
+    key = np.concatenate([rand_residues(rng, basis, N) for _ in range(2 * octx.parts(len(q) - 1))])
+    dkey = M.to_dev(key)
+    level = 6
+    items, want = [], []
+    for i in range(20):  # 20 switches: groups of 8 leave a partial group
+        kt, b, a = (rand_residues(rng, q[:level + 1], N) for _ in range(3))
+        ob, oa = octx.keyswitch(level, kt, key, b, a, 0, 1)
+        items.append((level, M.to_dev(kt), dkey, M.to_dev(b), M.to_dev(a), 0, 1))
+        want.append((ob, oa))
+    ctx.keyswitch_batch(items)
+    for (_, _, _, db, da, _, _), (ob, oa) in zip(items, want):
+        assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa)
