@@ -24,6 +24,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <array>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -742,6 +743,42 @@ std::vector<double> default_evalmod_polynomial(const CKKSParams& params);
 // Whether default_evalmod_polynomial returns the reference's fit.
 bool evalmod_polynomial_is_reference_fit(const CKKSParams& params);
 const std::array<uint8_t, 256>& aes_sbox_table();
+
+// A HIP graph of device work recorded from a context's stream: launch()
+// replays the same launches on the same buffers (no host work, no
+// allocation).  ckks_capture_graph records everything `body` issues on the
+// context's stream (and the side streams it forks); nothing executes while
+// recording.  Run the same body eagerly once first, so that workspaces,
+// pool blocks and cached encodings exist; buffers the graph reads or writes
+// must stay alive while it does (ciphertexts created inside `body` are
+// kept by whoever holds them), and pool blocks released during the capture
+// stay reserved while any graph of the context lives.
+class CKKSGraph {
+public:
+    CKKSGraph() = default;
+    CKKSGraph(const CKKSGraph&) = delete;
+    CKKSGraph& operator=(const CKKSGraph&) = delete;
+    CKKSGraph(CKKSGraph&& o) noexcept { std::swap(ctx_, o.ctx_), std::swap(g_, o.g_); }
+    CKKSGraph& operator=(CKKSGraph&& o) noexcept {
+        if (this != &o) {
+            reset();
+            std::swap(ctx_, o.ctx_);
+            std::swap(g_, o.g_);
+        }
+        return *this;
+    }
+    ~CKKSGraph() { reset(); }
+    void launch() const;  // on the context's stream
+    void reset();
+
+private:
+    friend CKKSGraph ckks_capture_graph(const CKKSContext& context, const std::function<void()>& body);
+    const CKKSContext* ctx_ = nullptr;
+    mi_graph* g_ = nullptr;
+};
+CKKSGraph ckks_capture_graph(const CKKSContext& context, const std::function<void()>& body);
+// dst's polys (all members) <- src's, same shape (a copy kernel; capturable)
+void ckks_copy_into(const CKKSContext& context, CKKSCiphertext& dst, const CKKSCiphertext& src);
 
 namespace ckks_detail {
 // Every poly of `ct` through mi_poly_list_op `op` (y = the same polys of

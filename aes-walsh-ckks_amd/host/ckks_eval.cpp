@@ -363,3 +363,38 @@ CKKSCiphertext ckks_conjugate(const CKKSContext& c, const CKKSCiphertext& ct) {
     ckks_conjugate_inplace(c, out);
     return out;
 }
+
+void CKKSGraph::launch() const {
+    if (!g_ || !ctx_) throw std::invalid_argument("ckks_graph: empty graph");
+    mi_check(mi_graph_launch(ctx_->handle(), g_, ctx_->stream()), "ckks_graph");
+}
+
+void CKKSGraph::reset() {
+    if (g_ && ctx_) (void)mi_graph_destroy(ctx_->handle(), g_);
+    g_ = nullptr;
+    ctx_ = nullptr;
+}
+
+CKKSGraph ckks_capture_graph(const CKKSContext& c, const std::function<void()>& body) {
+    mi_check(mi_graph_begin(c.handle(), c.stream()), "ckks_graph");
+    try {
+        body();
+    } catch (...) {
+        mi_graph* g = nullptr;
+        (void)mi_graph_end(c.handle(), c.stream(), &g);  // ends the capture; the partial graph is dropped
+        if (g) (void)mi_graph_destroy(c.handle(), g);
+        throw;
+    }
+    CKKSGraph out;
+    mi_check(mi_graph_end(c.handle(), c.stream(), &out.g_), "ckks_graph");
+    out.ctx_ = &c;
+    return out;
+}
+
+void ckks_copy_into(const CKKSContext& c, CKKSCiphertext& dst, const CKKSCiphertext& src) {
+    if (dst.getN() != src.getN() || dst.getLevel() != src.getLevel() || dst.getNumPolys() != src.getNumPolys() ||
+        dst.getBatchSize() != src.getBatchSize())
+        throw std::invalid_argument("ckks_copy_into: ciphertext shape mismatch");
+    ckks_detail::poly_list_op(c, MI_LIST_COPY, dst, &src, nullptr, "ckks_copy_into");
+    dst.setScale(src.getScale());
+}

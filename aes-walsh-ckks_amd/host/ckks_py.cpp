@@ -453,6 +453,13 @@ PYBIND11_MODULE(ckks_py, m) {
           py::arg("diagonal_indices"), py::arg("baby_step_count"), py::arg("slot_count") = 0,
           py::arg("rescale_after") = true, py::keep_alive<0, 1>());
     m.def("ckks_key_switch_dense_to_sparse_inplace", &ckks_key_switch_dense_to_sparse_inplace);
+    py::class_<CKKSGraph>(m, "CKKSGraph")
+        .def("launch", &CKKSGraph::launch)
+        .def("reset", &CKKSGraph::reset);
+    m.def("ckks_capture_graph",
+          [](const CKKSContext& c, py::function body) { return ckks_capture_graph(c, [&] { body(); }); },
+          py::keep_alive<0, 1>());
+    m.def("ckks_copy_into", &ckks_copy_into);
     // the keyswitch-level API: keys by name (rotation r, "relin", "conj"),
     // polynomials as numpy arrays (copied in, results returned)
     auto key_of = [](const CKKSContext& c, py::object which) -> CKKSSwitchingKey {

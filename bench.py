@@ -647,54 +647,80 @@ def walsh15_context(H, seed=15):
 
 
 def bench_sbox(args, M, torch):
-    """One AES S-box of the mask-packed Walsh circuit at the walsh15
-    parameters: 8 bit ciphertexts at the S2C entry level, 512 blocks per
-    lane, two lanes (real and imaginary slots); parity packing, the
-    complex-packed bootstrap, signs, unpacking, the Walsh output pairs of both
-    lanes and the split into 8 output bits, through the C++ host API.  The
-    outputs are decrypted and checked against the AES S-box."""
+    """One round of AES S-boxes of the mask-packed Walsh circuit at the
+    walsh15 parameters: 16 S-boxes as one ciphertext batch (bit k of 16
+    bytes stacked), 8 bit ciphertexts at the S2C entry level, 512 blocks per
+    lane, two lanes; parity packing, the complex-packed bootstrap, signs,
+    unpacking, the Walsh output pairs and the split into 8 output bits,
+    through the C++ host API.  Timed eagerly and as a HIP graph recorded
+    once and replayed (ckks_capture_graph); the replayed outputs are
+    compared limb for limb with the eager ones and decrypted against the
+    AES S-box."""
     H = _host()
     t0 = time.perf_counter()
     ctx = walsh15_context(H)
     setup_s = time.perf_counter() - t0
     P = ctx.getParams()
-    slots, bc = P.getSlots(), 512
+    slots, bc, B = P.getSlots(), 512, 16
     rng = np.random.default_rng(29)
-    x0, x1 = rng.integers(0, 256, bc), rng.integers(0, 256, bc)
     level = P.getBootstrapS2CDepth()
-    bits = []
-    for k in range(8):
-        v = np.zeros(slots, np.complex128)
-        v[:bc] = ((x0 >> k) & 1) + 1j * ((x1 >> k) & 1)
-        bits.append(H.ckks_encrypt(ctx, v, level))
-    for _ in range(args.warmup):
-        out = H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, bits, bc)
+    xs = [(rng.integers(0, 256, bc), rng.integers(0, 256, bc)) for _ in range(B)]
+    members = [[] for _ in range(8)]
+    for x0, x1 in xs:
+        for k in range(8):
+            v = np.zeros(slots, np.complex128)
+            v[:bc] = ((x0 >> k) & 1) + 1j * ((x1 >> k) & 1)
+            members[k].append(H.ckks_encrypt(ctx, v, level))
+    bits = [H.ckks_stack(ctx, m) for m in members]
+
+    def sbox():
+        return H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, [c.clone() for c in bits], bc)
+
+    for _ in range(max(1, args.warmup)):
+        out = sbox()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        out = H.ckks_walsh_sbox_complex_blocks_mask_packed(ctx, bits, bc)
+        out = sbox()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    held = []
+    graph = H.ckks_capture_graph(ctx, lambda: held.extend(sbox()))
+    graph.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same = all(np.array_equal(held[k].getMemberPoly(j, m), out[k].getMemberPoly(j, m))
+               for k in range(8) for m in range(B) for j in range(2))
     # the AES S-box from the Walsh table (its inverse transform)
     table = np.array(H.ckks_aes_walsh_table(), dtype=np.int64)
     par = np.array([[bin(m & int(v)).count("1") & 1 for v in range(256)] for m in range(256)])
     masks = np.arange(256)
     f = np.stack([4 * table[b][masks & 15, masks >> 4] for b in range(8)]) @ (1 - 2 * par)
-    sbox = ((f < 0).astype(np.int64) * (1 << np.arange(8))[:, None]).sum(axis=0)
+    sbox_tab = ((f < 0).astype(np.int64) * (1 << np.arange(8))[:, None]).sum(axis=0)
     worst, wrong = 0.0, 0
-    for k, ct in enumerate(out):
-        dec = H.ckks_decrypt(ctx, ct)[:bc]
-        want = ((sbox[x0] >> k) & 1) + 1j * ((sbox[x1] >> k) & 1)
-        worst = max(worst, float(np.max(np.abs(dec - want))))
-        wrong += int(np.sum(np.round(dec.real) != want.real) + np.sum(np.round(dec.imag) != want.imag))
+    members_out = [H.ckks_unstack(ctx, ct) for ct in held]
+    for k in range(8):
+        dec = H.ckks_decrypt_batch(ctx, members_out[k])
+        for m, (x0, x1) in enumerate(xs):
+            d = np.asarray(dec[m])[:bc]
+            want = ((sbox_tab[x0] >> k) & 1) + 1j * ((sbox_tab[x1] >> k) & 1)
+            worst = max(worst, float(np.max(np.abs(d - want))))
+            wrong += int(np.sum(np.round(d.real) != want.real) + np.sum(np.round(d.imag) != want.imag))
+    graph.reset()
     return {
-        "metric": "walsh15 AES S-box evaluations/s (mask-packed Walsh circuit, 512 blocks x 2 lanes)",
-        "value": 1e3 / ms, "unit": "sbox/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": "walsh15 AES S-box rounds/s (16 mask-packed Walsh S-boxes as one batch, 512 blocks x 2 lanes)",
+        "value": 1e3 / graph_ms, "unit": "rounds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64 (exact mod-q integer)", "data": "synthetic (random bytes, seeded keys)",
-        "config": {"workload": "one S-box: parity packing, bootstrap, unpacking, Walsh pairs, bit split",
-                   "byte_substitutions_per_s": 2 * bc * 1e3 / ms, "output_level": out[0].getLevel(),
-                   "max_abs_error": worst, "wrong_bits": wrong, "context_setup_s": setup_s},
+        "config": {"workload": "one AES round of S-boxes: 16 S-boxes as one batch, replayed as a HIP graph",
+                   "eager_ms": ms, "graph_ms": graph_ms, "graph_equals_eager": same,
+                   "byte_substitutions_per_s": B * 2 * bc * 1e3 / graph_ms, "output_level": out[0].getLevel(),
+                   "max_abs_error": worst, "wrong_bits": wrong, "context_setup_s": setup_s,
+                   "tuning": json.loads(ctx.tuneReport())},
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -897,6 +923,7 @@ def bench_ntt_qp(args, M, torch):
     us = r["isolated_qp_pair_us"]
     N = 1 << 15
     r.update({"achieved": 16 * N * 34 / (us / 1e6) / 1e9, "frac": r["isolated_qp_pair_frac"],
+              "traffic": _profile_traffic("ntt15_qp_forward"), "traffic_over_algorithmic": None, "valu_busy": None,
               "kernel": "forward NTT (column + row pass), N=2^15, 2 polys x 17 limbs (QP ciphertext)"})
     for k in ("launch_pairs", "limbs", "avg_pair_us", "avg_limbs_per_pair", "ntt_event_ms", "evaluation_ms"):
         r.pop(k)
@@ -937,8 +964,13 @@ def aes_ntt_roofline(M, torch, P, probe):
     torch.cuda.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
     achieved = ntt_bytes / (ntt_ms / 1e3)
+    # counters of the same launches (profiles/aes_walsh15_counters.md): HBM
+    # bytes per pair as a ratio to the algorithmic bytes, VALU busy fraction
+    ratio = _profile_traffic("ntt15_aes_inrun_ratio")
+    avg_bytes = ntt_bytes / max(1.0, pairs)
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": _profile_traffic("ntt15_qp_forward"),
+            "frac": achieved / HBM_PEAK, "traffic": None if ratio is None else ratio * avg_bytes,
+            "traffic_over_algorithmic": ratio, "valu_busy": _profile_traffic("ntt15_aes_inrun_valu_busy"),
             "kernel": "NTT launch pairs (column + row pass) inside one walsh15 AES-CTR evaluation, N=2^15",
             "launch_pairs": int(pairs), "limbs": int(ntt_limbs),
             "avg_pair_us": 1e3 * ntt_ms / max(1.0, pairs), "avg_limbs_per_pair": ntt_limbs / max(1.0, pairs),
