@@ -150,23 +150,24 @@ int mi_tune_report(mi_context* ctx, char* buf, size_t len) {
         check_ptr(buf);
         const mi::TunePlan& t = ctx->c->tune;
         std::string r = "{\"device\": \"" + ctx->c->tune_key + "\", \"tuned\": " + (t.tuned ? "true" : "false") +
-                        ", \"from_cache\": " + (t.from_cache ? "true" : "false") + ", \"ntt_limbs_per_workgroup\": [";
-        for (int b = 0; b < mi::TUNE_BUCKETS; ++b) r += (b ? ", " : "") + std::to_string(t.ntt_lpb[b]);
+                        ", \"from_cache\": " + (t.from_cache ? "true" : "false") + ", \"ntt_variant\": [";
+        for (int b = 0; b < mi::TUNE_BUCKETS; ++b) r += (b ? ", " : "") + std::to_string(t.ntt_variant[b]);
         r += "], \"ntt_bucket_limbs\": [0, 4, 16, 64, 256, 1024], \"keyswitch_group\": " +
-             std::to_string(t.ks_group) + ", \"ntt_forced\": " + std::to_string(ctx->c->ntt_lpb_forced) + "}";
+             std::to_string(t.ks_group) + ", \"ntt_forced\": " + std::to_string(ctx->c->ntt_forced) + "}";
         if (r.size() + 1 > len) throw mi::Error(MI_EINVAL, "buffer too small");
         std::memcpy(buf, r.c_str(), r.size() + 1);
     });
 }
 
-int mi_tune_set(mi_context* ctx, const int* ntt_lpb, int keyswitch_group) {
+int mi_tune_set(mi_context* ctx, const int* ntt_variant, int keyswitch_group) {
     return guard("mi_tune_set", [&] {
         check_ctx(ctx);
-        check_ptr(ntt_lpb);
+        check_ptr(ntt_variant);
         mi::TunePlan t = ctx->c->tune;
         for (int b = 0; b < mi::TUNE_BUCKETS; ++b) {
-            if (ntt_lpb[b] != 1 && ntt_lpb[b] != 2) throw mi::Error(MI_EINVAL, "limbs per workgroup must be 1 or 2");
-            t.ntt_lpb[b] = ntt_lpb[b];
+            if (ntt_variant[b] < 1 || ntt_variant[b] > mi::NTT_VARIANTS)
+                throw mi::Error(MI_EINVAL, "NTT variant must be 1, 2 or 3");
+            t.ntt_variant[b] = ntt_variant[b];
         }
         if (keyswitch_group != 8 && keyswitch_group != 16 && keyswitch_group != 32)
             throw mi::Error(MI_EINVAL, "key-switch group must be 8, 16 or 32");

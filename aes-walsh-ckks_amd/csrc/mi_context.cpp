@@ -129,9 +129,9 @@ void* Context::workspace(hipStream_t s, size_t bytes) {
     return w.ptr;
 }
 
-int Context::ntt_lpb(int count, bool, bool) const {
-    if (ntt_lpb_forced) return ntt_lpb_forced;
-    return tune.ntt_lpb[tune_bucket(count)];
+int Context::ntt_variant(int count) const {
+    if (ntt_forced) return ntt_forced;
+    return tune.ntt_variant[tune_bucket(count)];
 }
 
 long Context::probe_open(hipStream_t s, long long limbs) const {
@@ -205,9 +205,9 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     c->p.assign(p, p + np);
     MI_HIP(hipGetDevice(&c->device));
     noalloc_setup(c->device);
-    if (const char* e = std::getenv("MICKKS_NTT_LPB")) {
+    if (const char* e = std::getenv("MICKKS_NTT_VARIANT")) {
         const int v = std::atoi(e);
-        if (v == 1 || v == 2) c->ntt_lpb_forced = v;
+        if (v >= 1 && v <= NTT_VARIANTS) c->ntt_forced = v;
     }
     const int nb = nq + np;
     std::vector<u64> fwd(nb * N), fws(nb * N), inv(nb * N), ivs(nb * N);

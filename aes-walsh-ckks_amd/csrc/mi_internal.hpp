@@ -57,11 +57,14 @@ struct Workspace {
     size_t bytes = 0;
 };
 
-// Tuned launch configuration of a context (mi_tune.hip): NTT limbs per
-// workgroup per launch-size bucket, and key switches per launch group.
+// Tuned launch configuration of a context (mi_tune.hip): the NTT kernel
+// variant per launch-size bucket (1 = one limb per workgroup, 2 = two limbs
+// per workgroup, 3 = one limb with 32-bit LDS exchanges), and key switches
+// per launch group.
 constexpr int TUNE_BUCKETS = 6;
+constexpr int NTT_VARIANTS = 3;
 struct TunePlan {
-    int ntt_lpb[TUNE_BUCKETS] = {1, 1, 1, 1, 1, 1};
+    int ntt_variant[TUNE_BUCKETS] = {1, 1, 1, 1, 1, 1};
     int ks_group = 32;
     bool tuned = false;       // measured (now or in an earlier process)
     bool from_cache = false;  // read from the tuning cache file
@@ -137,10 +140,10 @@ struct Context {
     long probe_open(hipStream_t s, long long limbs) const;
     void probe_close(long slot, hipStream_t s) const;
 
-    // NTT plan: limbs per workgroup (1 or 2) of a pass over `count` limbs.
-    // ntt_lpb_forced (environment MICKKS_NTT_LPB) overrides the plan.
-    int ntt_lpb_forced = 0;
-    int ntt_lpb(int count, bool fwd, bool rows) const;
+    // NTT plan: the kernel variant (1..NTT_VARIANTS) of a pass over `count`
+    // limbs.  ntt_forced (environment MICKKS_NTT_VARIANT) overrides the plan.
+    int ntt_forced = 0;
+    int ntt_variant(int count) const;
     TunePlan tune;
     std::string tune_key;  // "<gfx arch> <compute units> <log2 N>"
 

@@ -148,31 +148,69 @@ __device__ __forceinline__ void group_inv(u64 (&x)[E], int t, u64* lds, const Bf
     }
 }
 
+// The same exchanges through 32-bit LDS words, the low halves and then the
+// high halves: half the LDS per workgroup (more workgroups per CU) for two
+// more barriers per exchange.
+template <int LOGR>
+__device__ __forceinline__ void to_contiguous_h(u64 (&x)[E], int t, u32* lds) {
+    using G = Grp<LOGR>;
+    u32 lo[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t + G::TG * j)] = (u32)x[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) lo[j] = lds[lpad(t * E + j)];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t + G::TG * j)] = (u32)(x[j] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) x[j] = ((u64)lds[lpad(t * E + j)] << 32) | lo[j];
+}
+template <int LOGR>
+__device__ __forceinline__ void to_strided_h(u64 (&x)[E], int t, u32* lds) {
+    using G = Grp<LOGR>;
+    u32 lo[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t * E + j)] = (u32)x[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) lo[j] = lds[lpad(t + G::TG * j)];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) lds[lpad(t * E + j)] = (u32)(x[j] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < E; ++j) x[j] = ((u64)lds[lpad(t + G::TG * j)] << 32) | lo[j];
+}
+
 // ---- several limbs per workgroup ----
 // LPB limbs per workgroup: every thread holds E words of each limb at the same
 // positions, the limbs' butterflies are interleaved (independent dependency
 // chains for the scheduler) and their LDS exchanges run one after another
 // through the one LDS region, so a workgroup keeps LPB limbs in flight for the
 // LDS (and occupancy) of one.
-template <int LOGR, int LPB>
-__device__ __forceinline__ void to_contiguous_n(u64 (&x)[LPB][E], int t, u64* lds) {
+template <int LOGR, int LPB, class LW>
+__device__ __forceinline__ void to_contiguous_n(u64 (&x)[LPB][E], int t, LW* lds) {
 #pragma unroll
     for (int i = 0; i < LPB; ++i) {
         if (i) __syncthreads();  // the previous limb's reads are done
-        to_contiguous<LOGR>(x[i], t, lds);
+        if constexpr (sizeof(LW) == 4) to_contiguous_h<LOGR>(x[i], t, lds);
+        else to_contiguous<LOGR>(x[i], t, lds);
     }
 }
-template <int LOGR, int LPB>
-__device__ __forceinline__ void to_strided_n(u64 (&x)[LPB][E], int t, u64* lds) {
+template <int LOGR, int LPB, class LW>
+__device__ __forceinline__ void to_strided_n(u64 (&x)[LPB][E], int t, LW* lds) {
 #pragma unroll
     for (int i = 0; i < LPB; ++i) {
         if (i) __syncthreads();
-        to_strided<LOGR>(x[i], t, lds);
+        if constexpr (sizeof(LW) == 4) to_strided_h<LOGR>(x[i], t, lds);
+        else to_strided<LOGR>(x[i], t, lds);
     }
 }
 
-template <int LOGR, int LPB, class IDX>
-__device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, u64* lds, const BfConst (&kq)[LPB],
+template <int LOGR, int LPB, class IDX, class LW>
+__device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, LW* lds, const BfConst (&kq)[LPB],
                                             const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
     using G = Grp<LOGR>;
 #pragma unroll
@@ -203,8 +241,8 @@ __device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, u64* lds, c
     to_strided_n<LOGR, LPB>(x, t, lds);
 }
 
-template <int LOGR, bool LAST, int LPB, class IDX>
-__device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, u64* lds, const BfConst (&kq)[LPB],
+template <int LOGR, bool LAST, int LPB, class IDX, class LW>
+__device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, LW* lds, const BfConst (&kq)[LPB],
                                             const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
     using G = Grp<LOGR>;
     __syncthreads();

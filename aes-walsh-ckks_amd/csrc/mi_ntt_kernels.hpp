@@ -95,12 +95,13 @@ __device__ __forceinline__ bool limb_views(const Job& job, int first, int count,
     return true;
 }
 
-template <int LOGN, bool FWD, int LPB, class Job>
+template <int LOGN, bool FWD, int LPB, bool HALF, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
 k_ntt_cols(NttTabs tb, Job job, int count) {
     constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n1;
     constexpr int G = pass_groups(n1, n2), wpl = pass_wpl(n1, n2);
-    __shared__ u64 lds[NTT_LDS_WORDS];
+    using LW = std::conditional_t<HALF, u32, u64>;  // LDS exchange word
+    __shared__ LW lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
     using V = decltype(job.view(0));
     const int blk = blockIdx.x / wpl, w = blockIdx.x - blk * wpl;
@@ -130,7 +131,7 @@ k_ntt_cols(NttTabs tb, Job job, int count) {
         W[i] = (FWD ? tb.fwd : tb.inv) + (long long)v[i].basis * tb.N;
         WS[i] = (FWD ? tb.fwd_shoup : tb.inv_shoup) + (long long)v[i].basis * tb.N;
     }
-    u64* my = lds + g * Gr::LDS;
+    LW* my = lds + g * Gr::LDS;
     if constexpr (FWD) {
         group_fwd_n<LOGR, LPB>(x, t, my, kq, W, WS, [](int s, int i) { return (1 << s) + i; });
     } else {
@@ -159,12 +160,13 @@ k_ntt_cols(NttTabs tb, Job job, int count) {
     }
 }
 
-template <int LOGN, bool FWD, int LPB, class Job>
+template <int LOGN, bool FWD, int LPB, bool HALF, class Job>
 __global__ void __launch_bounds__(NTT_THREADS)
 k_ntt_rows(NttTabs tb, Job job, int count) {
     constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n2;
     constexpr int G = pass_groups(n2, n1), wpl = pass_wpl(n2, n1);
-    __shared__ u64 lds[NTT_LDS_WORDS];
+    using LW = std::conditional_t<HALF, u32, u64>;  // LDS exchange word
+    __shared__ LW lds[NTT_LDS_WORDS];
     using Gr = Grp<LOGR>;
     using V = decltype(job.view(0));
     const int blk = blockIdx.x / wpl, w = blockIdx.x - blk * wpl;
@@ -195,7 +197,7 @@ k_ntt_rows(NttTabs tb, Job job, int count) {
         W[i] = (FWD ? tb.fwd : tb.inv) + (long long)v[i].basis * tb.N;
         WS[i] = (FWD ? tb.fwd_shoup : tb.inv_shoup) + (long long)v[i].basis * tb.N;
     }
-    u64* my = lds + g * Gr::LDS;
+    LW* my = lds + g * Gr::LDS;
     if constexpr (FWD) {
         group_fwd_n<LOGR, LPB>(x, t, my, kq, W, WS,
                                [row](int s, int i) { return (1 << (n1 + s)) + (row << s) + i; });
@@ -231,52 +233,67 @@ k_ntt_rows(NttTabs tb, Job job, int count) {
         default: throw Error(-1, "ntt: unsupported ring dimension");                                  \
     }
 
-template <bool FWD, int LPB, class Job>
+template <bool FWD, int LPB, bool HALF, class Job>
 void launch_cols(const Context& c, const Job& job, int count, hipStream_t s) {
     if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
     const unsigned blocks = (unsigned)((count + LPB - 1) / LPB);
 #define MI_COLS(LN)                                                                                   \
-    hipLaunchKernelGGL((k_ntt_cols<LN, FWD, LPB, Job>), dim3(blocks * pass_wpl(LN / 2, LN - LN / 2)),  \
+    hipLaunchKernelGGL((k_ntt_cols<LN, FWD, LPB, HALF, Job>), dim3(blocks * pass_wpl(LN / 2, LN - LN / 2)), \
                        dim3(pass_threads(LN / 2, LN - LN / 2)), 0, s, tb, job, count)
     MI_NTT_LOGN_CASES(MI_COLS)
 #undef MI_COLS
     MI_HIP(hipGetLastError());
 }
 
-template <bool FWD, int LPB, class Job>
+template <bool FWD, int LPB, bool HALF, class Job>
 void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
     if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
     const NttTabs tb = tabs_of(c);
     const unsigned blocks = (unsigned)((count + LPB - 1) / LPB);
 #define MI_ROWS(LN)                                                                                   \
-    hipLaunchKernelGGL((k_ntt_rows<LN, FWD, LPB, Job>), dim3(blocks * pass_wpl(LN - LN / 2, LN / 2)),  \
+    hipLaunchKernelGGL((k_ntt_rows<LN, FWD, LPB, HALF, Job>), dim3(blocks * pass_wpl(LN - LN / 2, LN / 2)), \
                        dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job, count)
     MI_NTT_LOGN_CASES(MI_ROWS)
 #undef MI_ROWS
     MI_HIP(hipGetLastError());
 }
 
+// One pass in the configuration `variant` of the context's NTT plan:
+// 1 = one limb per workgroup, 2 = two limbs per workgroup, 3 = one limb
+// with the LDS exchanges in 32-bit halves.
+template <bool FWD, bool ROWS, class Job>
+void launch_pass(const Context& c, const Job& job, int count, hipStream_t s) {
+    switch (c.ntt_variant(count)) {
+        case 2:
+            if constexpr (ROWS) launch_rows<FWD, 2, false>(c, job, count, s);
+            else launch_cols<FWD, 2, false>(c, job, count, s);
+            break;
+        case 3:
+            if constexpr (ROWS) launch_rows<FWD, 1, true>(c, job, count, s);
+            else launch_cols<FWD, 1, true>(c, job, count, s);
+            break;
+        default:
+            if constexpr (ROWS) launch_rows<FWD, 1, false>(c, job, count, s);
+            else launch_cols<FWD, 1, false>(c, job, count, s);
+    }
+}
+
 // Forward NTT: first pass runs job `in` (load input, store intermediate),
-// second pass runs job `out` (load intermediate, store output).  The limbs
-// per workgroup of each pass come from the context's NTT plan for this
-// launch (ntt_lpb: the autotuned choice for the ring, limb count and pass).
+// second pass runs job `out` (load intermediate, store output), each in the
+// configuration the context's tuned plan gives for the launch size.
 template <class JobIn, class JobOut>
 void run_ntt_fwd(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
     const long probe = c.probe_open(s, count);
-    if (c.ntt_lpb(count, true, false) == 2) launch_cols<true, 2>(c, in, count, s);
-    else launch_cols<true, 1>(c, in, count, s);
-    if (c.ntt_lpb(count, true, true) == 2) launch_rows<true, 2>(c, out, count, s);
-    else launch_rows<true, 1>(c, out, count, s);
+    launch_pass<true, false>(c, in, count, s);
+    launch_pass<true, true>(c, out, count, s);
     c.probe_close(probe, s);
 }
 template <class JobIn, class JobOut>
 void run_ntt_inv(const Context& c, const JobIn& in, const JobOut& out, int count, hipStream_t s) {
     const long probe = c.probe_open(s, count);
-    if (c.ntt_lpb(count, false, true) == 2) launch_rows<false, 2>(c, in, count, s);
-    else launch_rows<false, 1>(c, in, count, s);
-    if (c.ntt_lpb(count, false, false) == 2) launch_cols<false, 2>(c, out, count, s);
-    else launch_cols<false, 1>(c, out, count, s);
+    launch_pass<false, true>(c, in, count, s);
+    launch_pass<false, false>(c, out, count, s);
     c.probe_close(probe, s);
 }
 

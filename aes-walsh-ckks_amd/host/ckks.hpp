@@ -781,6 +781,12 @@ CKKSGraph ckks_capture_graph(const CKKSContext& context, const std::function<voi
 void ckks_copy_into(const CKKSContext& context, CKKSCiphertext& dst, const CKKSCiphertext& src);
 
 namespace ckks_detail {
+// The integer factor by which the AES circuit's add_term scales a ciphertext
+// of scale `from` up to `to` (ckks_aes_ctr_walsh_sbox.cpp:80-124): 1 when the
+// scales agree to 2^-45, else the rounded ratio when it is within the
+// reference's 1e-3 integrality tolerance; returns the reference's error text
+// where it would throw, nullptr otherwise.
+const char* aes_scale_up_factor(long double from, long double to, int64_t* factor);
 // Every poly of `ct` through mi_poly_list_op `op` (y = the same polys of
 // `other` where the op reads one, residues per limb where it scales).
 void poly_list_op(const CKKSContext& c, int op, CKKSCiphertext& ct, const CKKSCiphertext* other,

@@ -90,21 +90,10 @@ std::array<uint8_t, 16> aes128_ctr_crypt_block(const std::array<uint8_t, 16>& ke
     return out;
 }
 
-namespace {
-
-// pad_ciphertext_with_zero_polys (ckks_aes_ctr_walsh_sbox.cpp:58-66)
-void pad_with_zero_polys(const CKKSContext& context, CKKSCiphertext& ct, size_t target) {
-    if (target < ct.getNumPolys()) throw std::invalid_argument("ckks_aes_walsh: cannot shrink ciphertext degree");
-    while (ct.getNumPolys() < target) {
-        ct.addPoly();
-        mi_check(mi_memset_zero(context.handle(), ct.polyData(ct.getNumPolys() - 1),
-                                (ct.getLevel() + 1) * ct.getN() * sizeof(uint64_t), context.stream()),
-                 "ckks_aes_walsh");
-    }
-}
+namespace ckks_detail {
 
 // aes_scales_close (:42-49)
-bool aes_scales_close(long double lhs, long double rhs) {
+static bool aes_scales_close(long double lhs, long double rhs) {
     if (!std::isfinite(lhs) || !std::isfinite(rhs)) return false;
     const long double m = std::max<long double>({1.0L, std::abs(lhs), std::abs(rhs)});
     return std::abs(lhs - rhs) <= 0x1p-45L * m;
@@ -129,6 +118,23 @@ const char* aes_scale_up_factor(long double from, long double to, int64_t* facto
         return "ckks_aes_walsh: scalar scale-up is too large";
     *factor = (int64_t)rounded;
     return nullptr;
+}
+
+}  // namespace ckks_detail
+
+using ckks_detail::aes_scale_up_factor;
+
+namespace {
+
+// pad_ciphertext_with_zero_polys (ckks_aes_ctr_walsh_sbox.cpp:58-66)
+void pad_with_zero_polys(const CKKSContext& context, CKKSCiphertext& ct, size_t target) {
+    if (target < ct.getNumPolys()) throw std::invalid_argument("ckks_aes_walsh: cannot shrink ciphertext degree");
+    while (ct.getNumPolys() < target) {
+        ct.addPoly();
+        mi_check(mi_memset_zero(context.handle(), ct.polyData(ct.getNumPolys() - 1),
+                                (ct.getLevel() + 1) * ct.getN() * sizeof(uint64_t), context.stream()),
+                 "ckks_aes_walsh");
+    }
 }
 
 bool is_zero_ciphertext(const CKKSCiphertext& ct) {

@@ -551,6 +551,11 @@ PYBIND11_MODULE(ckks_py, m) {
         return v;
     };
     m.def("ckks_aes_walsh_table", []() { return ckks_aes_walsh_table(); });
+    m.def("aes_scale_up_factor", [](double from, double to) {
+        int64_t factor = 0;
+        const char* err = ckks_detail::aes_scale_up_factor(from, to, &factor);
+        return py::make_tuple(factor, err ? py::cast(std::string(err)) : py::none());
+    });
     m.def("ckks_walsh_sum_for_output_pair",
           [ct_list](const CKKSContext& c, const py::list& low, const py::list& high, size_t real_bit, size_t imag_bit,
                     const CKKSEncoding& encoded_i) {

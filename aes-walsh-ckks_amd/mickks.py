@@ -260,8 +260,8 @@ class Context:
         check(lib().mi_tune_report(self.h, buf, len(buf)))
         return json.loads(buf.value.decode())
 
-    def tune_set(self, ntt_lpb, keyswitch_group):
-        arr = (ctypes.c_int * 6)(*ntt_lpb)
+    def tune_set(self, ntt_variant, keyswitch_group):
+        arr = (ctypes.c_int * 6)(*ntt_variant)
         check(lib().mi_tune_set(self.h, arr, int(keyswitch_group)))
 
     def keyswitch(self, level, key_term, key, b_out, a_out, b_mode, a_mode, montgomery=True, stream=None):

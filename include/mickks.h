@@ -71,16 +71,18 @@ int mi_memory_stats(mi_context* ctx, int64_t* stats);
 int mi_set_memory_limit(mi_context* ctx, int64_t bytes);
 /* Autotuned launch plan (no counterpart in the reference): at context
  * creation the library times the candidate configurations of its tunable
- * launches on the device (NTT limbs per workgroup per launch-size bucket of
- * 0-3, 4-15, 16-63, 64-255, 256-1023 and 1024+ limbs; key switches per launch
- * group, 8/16/32) and keeps the fastest, cached per device and ring dimension
- * in memory and in $MICKKS_TUNE_CACHE (else $HOME/.cache/mickks/tune.txt);
- * MICKKS_TUNE=0 keeps the defaults.  Every plan computes the same residues.
- * mi_tune_report writes the context's plan as a JSON object into buf;
- * mi_tune_set replaces it (six limbs-per-workgroup values, 1 or 2, and a
- * group size). */
+ * launches on the device (the NTT kernel variant per launch-size bucket of
+ * 0-3, 4-15, 16-63, 64-255, 256-1023 and 1024+ limbs: 1 = one limb per
+ * workgroup, 2 = two limbs per workgroup, 3 = one limb with 32-bit LDS
+ * exchanges; key switches per launch group, 8/16/32) and keeps the fastest,
+ * cached per device and ring dimension in memory and in $MICKKS_TUNE_CACHE
+ * (else $HOME/.cache/mickks/tune.txt); MICKKS_TUNE=0 keeps the defaults and
+ * MICKKS_NTT_VARIANT=v forces one NTT variant.  Every plan computes the same
+ * residues.  mi_tune_report writes the context's plan as a JSON object into
+ * buf; mi_tune_set replaces it (six variants, each 1, 2 or 3, and a group
+ * size). */
 int mi_tune_report(mi_context* ctx, char* buf, size_t len);
-int mi_tune_set(mi_context* ctx, const int* ntt_lpb, int keyswitch_group);
+int mi_tune_set(mi_context* ctx, const int* ntt_variant, int keyswitch_group);
 /* NTT probe (measurement only; no counterpart in the reference): between
  * mi_probe_begin and mi_probe_end every forward or inverse NTT launch pair
  * of the context (standalone, or inside key switches, rescales, ModRaise and

@@ -100,3 +100,19 @@ def test_xboot_presets_reproduce_reference_chains():
     assert H.CKKSParams.AESXBoot14SecureSparse1024().getBootstrapActiveSlotCount() == 1024
     assert H.CKKSParams.AESXBoot15Sparse1024().getBootstrapActiveSlotCount() == 1024
     assert H.CKKSParams.AESXBoot15().getBootstrapActiveSlotCount() == 1 << 14
+
+
+def test_aes_scale_up_factor_uses_the_reference_tolerance():
+    """add_term's integer scale-up (ckks_aes_ctr_walsh_sbox.cpp:80-124): a
+    ratio within 1e-3 of an integer rounds to it (4 (1 + 1e-5) -> 4), one
+    further out is refused as non-integral, scales equal to 2^-45 need no
+    factor, and a scale-down is refused."""
+    H = host()
+    s = 2.0 ** 40
+    assert H.aes_scale_up_factor(s, 4 * s * (1 + 1e-5)) == (4, None)
+    assert H.aes_scale_up_factor(s, 3 * s) == (3, None)
+    assert H.aes_scale_up_factor(s, s) == (1, None)
+    f, err = H.aes_scale_up_factor(s, 4 * s * (1 + 2e-3))
+    assert err == "ckks_aes_walsh: non-integral scalar scale-up"
+    f, err = H.aes_scale_up_factor(4 * s, s)
+    assert err == "ckks_aes_walsh: cannot scale down ciphertext for addition"

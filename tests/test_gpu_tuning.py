@@ -1,7 +1,8 @@
 """The autotuned launch plan: every configuration the tuner can choose (NTT
-limbs per workgroup 1 or 2 in every launch-size bucket, key-switch groups of
-8, 16 and 32) gives the same residues as the oracle, and the context reports
-the plan it runs with."""
+variants 1 = one limb per workgroup, 2 = two limbs per workgroup, 3 = one
+limb with 32-bit LDS exchanges, in every launch-size bucket; key-switch
+groups of 8, 16 and 32) gives the same residues as the oracle, and the
+context reports the plan it runs with."""
 import numpy as np
 import pytest
 
@@ -16,20 +17,19 @@ def test_tuned_plan_is_reported():
     p, q = O.find_primes([60] * 2, [50] * 6, 1 << 13)
     ctx = M.Context(1 << 13, q, p, 3)
     r = ctx.tune_report()
-    assert len(r["ntt_limbs_per_workgroup"]) == 6 and set(r["ntt_limbs_per_workgroup"]) <= {1, 2}
+    assert len(r["ntt_variant"]) == 6 and set(r["ntt_variant"]) <= {1, 2, 3}
     assert r["keyswitch_group"] in (8, 16, 32)
     assert r["device"].split()[-1] == "13"
 
 
-@pytest.mark.parametrize("lpb", [1, 2])
-def test_every_ntt_plan_matches_the_oracle(lpb):
-    import torch
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_every_ntt_plan_matches_the_oracle(variant):
     M = product()
     N = 1 << 15
     p, q = O.find_primes([61] * 4, [36, 61, 42, 42] + [38] * 7 + [50, 50], N)
     ctx = M.Context(N, q, p, 3)
-    ctx.tune_set([lpb] * 6, 32)
-    rng = np.random.default_rng(90 + lpb)
+    ctx.tune_set([variant] * 6, 32)
+    rng = np.random.default_rng(90 + variant)
     basis = list(q) + list(p)
     # 3 polys x 17 limbs (odd count: the last workgroup of a 2-limb plan has one limb)
     host = np.concatenate([rand_residues(rng, basis, N) for _ in range(3)])
@@ -43,6 +43,29 @@ def test_every_ntt_plan_matches_the_oracle(lpb):
         assert np.array_equal(fwd[k * N:(k + 1) * N], want), k
     ctx.ntt_inverse(d, 3, 17)
     assert np.array_equal(M.from_dev(d), host)
+
+
+@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("logn", [10, 11, 12, 13, 14, 16])
+def test_ntt_variants_every_ring_dimension(variant, logn):
+    """Each ring dimension has its own column and row group sizes: variants
+    2 and 3 against variant 1 (itself pinned by the NTT parity tests)."""
+    M = product()
+    N = 1 << logn
+    p, q = O.find_primes([60] * 2, [50] * 5, N)
+    ctx = M.Context(N, q, p, 2)
+    rng = np.random.default_rng(logn * 10 + variant)
+    basis = list(q) + list(p)
+    host = np.concatenate([rand_residues(rng, basis, N) for _ in range(3)])
+    outs = []
+    for v in (1, variant):
+        ctx.tune_set([v] * 6, 32)
+        d = M.to_dev(host)
+        ctx.ntt_forward(d, 3, 7)
+        outs.append(M.from_dev(d))
+        ctx.ntt_inverse(d, 3, 7)
+        assert np.array_equal(M.from_dev(d), host), v
+    assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("group", [8, 16, 32])
