@@ -906,6 +906,31 @@ int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out
     });
 }
 
+int mi_scaled_sum(mi_context* ctx, int level, int npolys, uint64_t* const* out, int nterms,
+                  const uint64_t* const* in, const uint64_t* residues, int nconst, const uint64_t* constant,
+                  void* stream) {
+    return guard("ckks_evalchebyshev", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (npolys < 1 || !out || !in || !residues) throw mi::Error(MI_EINVAL, "invalid poly list");
+        if (nterms < 1 || nterms > 16) throw mi::Error(MI_EINVAL, "1..16 terms per scaled sum");
+        if (nconst < 0 || nconst > npolys || (nconst && !constant)) throw mi::Error(MI_EINVAL, "invalid constant");
+        const size_t W = (size_t)(level + 1) * ctx->c->N;
+        auto overlap = [W](const uint64_t* x, const uint64_t* y) { return x < y + W && y < x + W; };
+        for (int k = 0; k < npolys; ++k) {
+            check_ptr(out[k]);
+            for (int j = 0; j < k; ++j)
+                if (overlap(out[k], out[j])) throw mi::Error(MI_EINVAL, "output polys overlap");
+        }
+        for (size_t i = 0; i < (size_t)nterms * npolys; ++i) {
+            if (!in[i]) continue;
+            for (int j = 0; j < npolys; ++j)
+                if (overlap(in[i], out[j])) throw mi::Error(MI_EINVAL, "an output overlaps an input");
+        }
+        mi::scaled_sum(*ctx->c, level, npolys, out, nterms, in, residues, nconst, constant, S(stream));
+    });
+}
+
 int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_b, const uint64_t* const* low_a,
                          const uint64_t* const* high_b, const uint64_t* const* high_a, const int16_t* cr,
                          const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,

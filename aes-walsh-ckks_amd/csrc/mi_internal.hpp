@@ -253,6 +253,8 @@ void rescale_list(Context& c, int level, u64* const* polys, int npolys, hipStrea
 // polys[k] *= pt over limbs 0..level (pt Montgomery form if mont)
 void linear_combine(const Context& c, int level, int nout, u64* const* out, int nterms, const u64* const* in,
                     const unsigned* coef, hipStream_t s);
+void scaled_sum(const Context& c, int level, int npolys, u64* const* out, int nterms, const u64* const* in,
+                const u64* residues, int nconst, const u64* constant, hipStream_t s);
 void pair_list_op(const Context& c, int op, int level, u64* const* x, const u64* const* y, int npolys,
                   const u64* residues, hipStream_t s);
 void mul_plain_list(const Context& c, int level, u64* const* polys, int npolys, const u64* pt, bool mont,

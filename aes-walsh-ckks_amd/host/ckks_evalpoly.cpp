@@ -205,42 +205,6 @@ void sub_integer_multiple(const CKKSContext& c, CKKSCiphertext& out, const CKKSC
              "ckks_evalpoly");
 }
 
-// out += round(scalar * scalar_scale) * src over src's polys, src read at
-// out's level (src at that level or above, no more polys than out: out's
-// extra polys take + 0)
-void mul_real_then_add_at(const CKKSContext& c, const CKKSCiphertext& src, size_t level, double scalar,
-                          ld scalar_scale, CKKSCiphertext& dst) {
-    if (src.getN() != dst.getN() || src.getLevel() < level || dst.getLevel() != level ||
-        src.getNumPolys() > dst.getNumPolys() || src.getBatchSize() != dst.getBatchSize())
-        throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add shape mismatch");
-    const auto& q = c.getParams().getModuli();
-    std::vector<uint64_t> r(level + 1);
-    for (size_t l = 0; l <= level; ++l) r[l] = scaled_constant_mod_q(scalar, scalar_scale, q[l]);
-    std::vector<uint64_t*> x;
-    std::vector<const uint64_t*> y;
-    for (size_t k = 0; k < src.getNumPolys(); ++k)
-        for (size_t m = 0; m < src.getBatchSize(); ++m) {
-            x.push_back(dst.polyData(k, m));
-            y.push_back(src.polyData(k, m));
-        }
-    mi_check(mi_poly_list_op(c.handle(), MI_LIST_MUL_SCALAR_ADD, (int)level, (int)x.size(), x.data(), y.data(),
-                             r.data(), c.stream()),
-             "ckks_evalpoly");
-}
-
-void raise_scale_to(const CKKSContext& c, CKKSCiphertext& ct, ld target) {
-    const double cur = ct.getScale();
-    const double tgt = (double)target;
-    const double m = std::max({1.0, std::abs(cur), std::abs(tgt)});
-    if (std::isfinite(cur) && std::isfinite(tgt) && std::abs(cur - tgt) <= 0x1p-45 * m) {
-        ct.setScale((double)target);
-        return;
-    }
-    if (ct.getScale() > target)
-        throw std::invalid_argument("ckks_evalchebyshev: cannot scale down ciphertext during scale alignment");
-    mul_integer(c, ct, target / ct.getScale(), target);
-}
-
 size_t levels_per_rescale(const CKKSContext& c) { return std::log2(c.getParams().getScale()) <= 64.0 ? 1 : 2; }
 
 ld rescale_divisor(const CKKSContext& c, size_t level) {
@@ -262,36 +226,99 @@ CKKSCiphertext aligned(const CKKSContext& c, CKKSCiphertext ct, size_t level, do
     return ct;
 }
 
-// out += scalar * op0 with the scales reconciled (MulThenAdd)
-void mul_then_add(const CKKSContext& c, const CKKSCiphertext& op0, double scalar, CKKSCiphertext& out) {
-    if (scalar == 0.0) return;
-    const size_t level = std::min(op0.getLevel(), out.getLevel());
-    const double out_scale = out.getScale();
-    const size_t polys = std::max(op0.getNumPolys(), out.getNumPolys());
-    out = aligned(c, std::move(out), level, out_scale, polys);
-    // op0 is read in place at `level` (its first level+1 limbs); the
-    // reference's aligned copy only pads it with zero polys, which add nothing
-    const double a_scale = op0.getScale();
-    ld s_rlwe = 0.0L;
-    if (scales_match(a_scale, out.getScale())) {
-        if (near_integer(scalar)) {
-            s_rlwe = 1.0L;
-        } else {
-            s_rlwe = rescale_divisor(c, level);
-            raise_scale_to(c, out, out.getScale() * s_rlwe);
-        }
-    } else if (a_scale < out.getScale()) {
-        s_rlwe = out.getScale() / a_scale;
-    } else {
-        std::ostringstream os;
-        os << "ckks_evalchebyshev: MulThenAdd requires op0.Scale <= opOut.Scale (op0=" << a_scale
-           << ", opOut=" << (double)out.getScale() << ", level=" << level << ")";
-        throw std::invalid_argument(os.str());
+// A Chebyshev leaf as a linear form over the power basis, built on the host
+// by the reference's operation sequence (evaluate_polynomial_from_power_basis,
+// ckks_evalpoly.cpp:998-1049): its ciphertext metadata (level, scale, poly
+// count) follows every step exactly, its residues per limb follow the data —
+// the constant, each MulThenAdd term's rounded scalar, every integer
+// scale-up of the accumulator applied to what it holds so far — and one
+// mi_scaled_sum then writes the leaf.  All steps are exact mod q, so the
+// leaf's residues equal the step-by-step evaluation's.
+struct LeafForm {
+    const CKKSContext& c;
+    const MyVector<uint64_t>& q;
+    size_t level = 0;
+    ld scale = 0.0L;
+    size_t polys = 2;
+    std::vector<uint64_t> cst;                 // per limb
+    bool has_cst = false;
+    std::vector<const CKKSCiphertext*> terms;
+    std::vector<std::vector<uint64_t>> r;      // per term, per limb
+
+    static uint64_t mulmod(uint64_t a, uint64_t b, uint64_t m) {
+        return (uint64_t)((unsigned __int128)a * b % m);
     }
-    const ld term = a_scale * s_rlwe;
-    if (!scales_match((double)term, out.getScale())) throw std::runtime_error("ckks_evalchebyshev: MulThenAdd scale mismatch");
-    mul_real_then_add_at(c, op0, level, scalar, s_rlwe, out);
-}
+    // ckks_drop_to_level_inplace + setScale + pad_ciphertext_with_zero_polys
+    void align(size_t lv, double sc, size_t target_polys) {
+        if (lv > level) throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
+        level = lv;
+        scale = sc;
+        if (target_polys < polys) throw std::invalid_argument("ckks_evalchebyshev: cannot shrink ciphertext degree by padding");
+        polys = target_polys;
+    }
+    // mul_integer: the accumulator times round(scalar), checked as there
+    void mul_integer(ld scalar, ld out_scale) {
+        if (!std::isfinite(scalar) || scalar < 0.0L) throw std::invalid_argument("ckks_evalchebyshev: invalid scalar scale-up");
+        const ld rr = std::round(scalar);
+        if (rr < 1.0L || std::abs(scalar - rr) > std::max<ld>(1.0L, std::abs(scalar)) * 0x1p-20L)
+            throw std::invalid_argument("ckks_evalchebyshev: non-integral scalar scale-up");
+        if (rr > (ld)std::numeric_limits<uint64_t>::max())
+            throw std::invalid_argument("ckks_evalchebyshev: scalar scale-up is too large");
+        const uint64_t sv = (uint64_t)rr;
+        if (sv != 1)
+            for (size_t l = 0; l <= level; ++l) {
+                const uint64_t m = sv % q[l];
+                if (has_cst) cst[l] = mulmod(cst[l], m, q[l]);
+                for (auto& t : r) t[l] = mulmod(t[l], m, q[l]);
+            }
+        scale = (double)out_scale;
+    }
+    // raise_scale_to
+    void raise_scale_to(ld target) {
+        const double cur = (double)scale;
+        const double tgt = (double)target;
+        const double m = std::max({1.0, std::abs(cur), std::abs(tgt)});
+        if (std::isfinite(cur) && std::isfinite(tgt) && std::abs(cur - tgt) <= 0x1p-45 * m) {
+            scale = (double)target;
+            return;
+        }
+        if (scale > target)
+            throw std::invalid_argument("ckks_evalchebyshev: cannot scale down ciphertext during scale alignment");
+        mul_integer(target / scale, target);
+    }
+    // accumulator += scalar * op0 with the scales reconciled
+    // (multhenadd_real_scalar_scale_aware_inplace, ckks_evalpoly.cpp:461-524)
+    void mul_then_add(const CKKSCiphertext& op0, double scalar) {
+        if (scalar == 0.0) return;
+        const size_t lv = std::min(op0.getLevel(), level);
+        const double out_scale = (double)scale;
+        align(lv, out_scale, std::max(op0.getNumPolys(), polys));
+        const double a_scale = (double)op0.getScale();
+        ld s_rlwe = 0.0L;
+        if (scales_match(a_scale, (double)scale)) {
+            if (near_integer(scalar)) {
+                s_rlwe = 1.0L;
+            } else {
+                s_rlwe = rescale_divisor(c, lv);
+                raise_scale_to(scale * s_rlwe);
+            }
+        } else if (a_scale < scale) {
+            s_rlwe = scale / a_scale;
+        } else {
+            std::ostringstream os;
+            os << "ckks_evalchebyshev: MulThenAdd requires op0.Scale <= opOut.Scale (op0=" << a_scale
+               << ", opOut=" << (double)scale << ", level=" << lv << ")";
+            throw std::invalid_argument(os.str());
+        }
+        const ld term = a_scale * s_rlwe;
+        if (!scales_match((double)term, (double)scale)) throw std::runtime_error("ckks_evalchebyshev: MulThenAdd scale mismatch");
+        // fused_multiply_by_real_scalar_then_add_inplace (:352-398): + round(scalar s_rlwe) op0
+        std::vector<uint64_t> rt(level + 1);
+        for (size_t l = 0; l <= level; ++l) rt[l] = scaled_constant_mod_q(scalar, s_rlwe, q[l]);
+        terms.push_back(&op0);
+        r.push_back(std::move(rt));
+    }
+};
 
 // ---- simulated (level, scale) planning
 struct SimOp {
@@ -461,17 +488,46 @@ bool coeff_used(const Poly& p, size_t e) {
 CKKSCiphertext eval_leaf(const CKKSContext& c, const Poly& p, const Powers& pw) {
     const size_t deg = p.degree();
     const CKKSCiphertext& x1 = pw.at(1);
-    CKKSCiphertext res = zero_ct(c, p.level, (double)p.scale, x1.getSecretOwner(), x1.getMessageEncodingState(),
-                                 x1.getBatchSize());
-    if (p.even && p.c[0] != 0.0) add_scaled_constant(c, res, p.c[0], p.scale);
-    if (deg == 0) return res;
-    for (size_t e = deg; e > 0; --e) {
-        const double v = p.c[e];
-        if (v == 0.0 || !coeff_used(p, e)) continue;
-        const size_t polys = res.getNumPolys();
-        res = aligned(c, std::move(res), p.level, (double)p.scale, polys);
-        mul_then_add(c, pw.at(e), v, res);
+    const size_t B = x1.getBatchSize();
+    const auto& q = c.getParams().getModuli();
+    LeafForm f{c, q};
+    f.level = p.level;
+    f.scale = (double)p.scale;
+    f.cst.assign(p.level + 1, 0);
+    if (p.even && p.c[0] != 0.0) {
+        f.has_cst = true;
+        for (size_t l = 0; l <= p.level; ++l) f.cst[l] = scaled_constant_mod_q(p.c[0], p.scale, q[l]);
     }
+    if (deg > 0)
+        for (size_t e = deg; e > 0; --e) {
+            const double v = p.c[e];
+            if (v == 0.0 || !coeff_used(p, e)) continue;
+            f.align(p.level, (double)p.scale, f.polys);
+            const CKKSCiphertext& op0 = pw.at(e);
+            if (op0.getBatchSize() != B) throw std::invalid_argument("ckks_evalchebyshev: fused scalar multiply-add shape mismatch");
+            f.mul_then_add(op0, v);
+        }
+    if (f.terms.empty()) {
+        CKKSCiphertext res = zero_ct(c, p.level, (double)p.scale, x1.getSecretOwner(), x1.getMessageEncodingState(), B);
+        if (f.has_cst) add_scaled_constant(c, res, p.c[0], p.scale);
+        return res;
+    }
+    const size_t L = f.level, P = f.polys, T = f.terms.size();
+    CKKSCiphertext res(c, P, f.scale, L, x1.getSecretOwner(), x1.getMessageEncodingState(), B);
+    std::vector<uint64_t*> out;
+    for (size_t k = 0; k < P; ++k)
+        for (size_t m = 0; m < B; ++m) out.push_back(res.polyData(k, m));
+    std::vector<const uint64_t*> in(T * P * B, nullptr);
+    std::vector<uint64_t> resid(T * (L + 1));
+    for (size_t t = 0; t < T; ++t) {
+        const CKKSCiphertext& op0 = *f.terms[t];
+        for (size_t k = 0; k < op0.getNumPolys(); ++k)
+            for (size_t m = 0; m < B; ++m) in[t * P * B + k * B + m] = op0.polyData(k, m);
+        for (size_t l = 0; l <= L; ++l) resid[t * (L + 1) + l] = f.r[t][l];
+    }
+    mi_check(mi_scaled_sum(c.handle(), (int)L, (int)(P * B), out.data(), (int)T, in.data(), resid.data(),
+                           f.has_cst ? (int)B : 0, f.cst.data(), c.stream()),
+             "ckks_evalpoly");
     return res;
 }
 

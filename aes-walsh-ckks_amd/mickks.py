@@ -41,6 +41,7 @@ _SIGS = {
     "mi_stream_create": (cint, [vp, ctypes.POINTER(vp)]),
     "mi_poly_list_op": (cint, [vp, cint, cint, cint, vp, vp, vp, vp]),
     "mi_linear_combine": (cint, [vp, cint, cint, vp, cint, vp, vp, vp]),
+    "mi_scaled_sum": (cint, [vp, cint, cint, vp, cint, vp, vp, cint, vp, vp]),
     "mi_conjugate_batch": (cint, [vp, cint, cint, vp, vp, vp, cint, vp]),
     "mi_stream_destroy": (cint, [vp, vp]),
     "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),

@@ -418,6 +418,21 @@ int mi_mul_plain_rescale_list(mi_context* ctx, int level, int npolys, uint64_t* 
  * reference's sequence of additions. */
 int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out, int nterms,
                       const uint64_t* const* in, const uint32_t* coef, void* stream);
+/* out[k] = constant[limb] (k < nconst) + sum_t residues[t * (level+1) + limb]
+ * * in[t * npolys + k] mod q over limbs 0..level, for the npolys listed
+ * output polys (in[t * npolys + k] null: term t has no poly k; 1 <= nterms
+ * <= 16; residues and constant below their moduli; outputs overlap neither
+ * each other nor any input).  One Chebyshev leaf of the Paterson-Stockmeyer
+ * evaluation — the reference's zero ciphertext, its constant and its loop of
+ * MulThenAdd over the baby-step powers (evaluate_leaf / MulThenAdd,
+ * ckks_evalpoly.cpp:352-398 and the leaf loop that calls it), with every
+ * integer scale-up folded into the residues on the host — in one pass per
+ * 16 polys instead of a zero fill, a constant add and a read-modify-write
+ * per term.  The sum is exact mod q, so the result equals the reference's
+ * sequence of operations. */
+int mi_scaled_sum(mi_context* ctx, int level, int npolys, uint64_t* const* out, int nterms,
+                  const uint64_t* const* in, const uint64_t* residues, int nconst, const uint64_t* constant,
+                  void* stream);
 /* Also MI_LIST_ADD_SCALAR: x[k] + r (ckks_add_integer_constant_inplace,
  * ckks_addition.cpp:89-112, on the b polys of a batch; y unused),
  * MI_LIST_ZERO: x[k] = 0 (y and residues unused), MI_LIST_ADD_PLAIN_MONT:

@@ -114,6 +114,54 @@ def test_linear_combine_matches_numpy_and_checks_arguments():
                                      c17.ctypes.data_as(ctypes.c_void_p), s) < 0                                # > 16 terms
 
 
+@pytest.mark.parametrize("nlimbs,npolys,nterms", [(4, 40, 11), (40, 18, 16)])
+def test_scaled_sum_matches_python_ints(nlimbs, npolys, nterms):
+    """mi_scaled_sum (one Chebyshev leaf: constant + sum of per-limb scaled
+    terms) against Python integers: outputs in chunks of 16 polys, terms in
+    chunks (8 at 4 limbs, 5 at 40 limbs) that accumulate, absent terms,
+    a constant on the first polys, 61-bit moduli (128-bit sums)."""
+    M = product()
+    import torch
+    N = 1 << 10
+    level = nlimbs - 1
+    p, q = O.find_primes([61], [61] * nlimbs, N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(nlimbs * 100 + nterms)
+    L = level + 1
+    ins = [[rand_residues(rng, q[:L], N) if rng.random() > 0.15 else None for _ in range(npolys)]
+           for _ in range(nterms)]
+    din = [[M.to_dev(x) if x is not None else None for x in row] for row in ins]
+    res = np.array([[int(rng.integers(0, int(q[l]))) for l in range(L)] for _ in range(nterms)], dtype=np.uint64)
+    cst = np.array([int(rng.integers(0, int(q[l]))) for l in range(L)], dtype=np.uint64)
+    nconst = npolys // 3
+    dout = [M.to_dev(np.full(L * N, 7, np.uint64)) for _ in range(npolys)]
+    src = _arr([M.ptr(d) if d is not None else None for row in din for d in row])
+    s = M._stream(torch.cuda.current_stream())
+    print(f"scaled sum: {npolys} polys x {nterms} terms x {L} limbs", flush=True)
+    M.check(M.lib().mi_scaled_sum(ctx.h, level, npolys, _arr([M.ptr(d) for d in dout]), nterms, src,
+                                  res.ctypes.data_as(ctypes.c_void_p), nconst, cst.ctypes.data_as(ctypes.c_void_p), s))
+    torch.cuda.synchronize()
+    qv = np.repeat(np.array(q[:L], dtype=object), N)
+    for k in range(npolys):
+        want = np.repeat(cst.astype(object), N) if k < nconst else np.zeros(L * N, dtype=object)
+        for t in range(nterms):
+            if ins[t][k] is not None:
+                want = want + ins[t][k].astype(object) * np.repeat(res[t].astype(object), N)
+        assert np.array_equal(M.from_dev(dout[k]), np.array(want % qv, dtype=np.uint64)), k
+    one = _arr([M.ptr(din[0][0] if din[0][0] is not None else dout[0])])
+    r1 = res[:1].copy()
+    assert M.lib().mi_scaled_sum(ctx.h, level, 1, one, 1, one, r1.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0
+    assert "overlaps" in M.lib().mi_last_error().decode()
+    src17 = _arr([M.ptr(dout[1])] * 17)
+    r17 = np.zeros((17, L), np.uint64)
+    assert M.lib().mi_scaled_sum(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 17, src17,
+                                 r17.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0                # > 16 terms
+    bad = res[:1].copy()
+    bad[0, 0] = q[0]
+    assert M.lib().mi_scaled_sum(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 1, _arr([M.ptr(dout[1])]),
+                                 bad.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0                # residue >= q
+
+
 def test_conjugate_batch_equals_one_at_a_time():
     """ckks_conjugate_batch (mi_conjugate_batch) against ckks_conjugate of each
     ciphertext, limb for limb, on 5 ciphertexts."""
