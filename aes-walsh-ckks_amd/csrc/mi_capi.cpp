@@ -618,6 +618,26 @@ int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, vo
     });
 }
 
+int mi_hoist_batch(mi_context* ctx, int level, int count, const uint64_t* const* a, uint64_t* digits,
+                   int64_t digits_stride, void* stream) {
+    return guard("ckks_hoist_ciphertext_a_qp_eval", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(digits);
+        if (count < 1 || !a) throw mi::Error(MI_EINVAL, "invalid batch");
+        const mi::Context& c = *ctx->c;
+        const long long dw = (long long)c.parts(level) * (level + 1 + c.np) * (long long)c.N;
+        if (digits_stride < dw) throw mi::Error(MI_EINVAL, "digit stride smaller than one ciphertext's digits");
+        const long long W = (long long)(level + 1) * (long long)c.N;
+        const uint64_t* d_end = digits + (count - 1) * digits_stride + dw;
+        for (int i = 0; i < count; ++i) {
+            check_ptr(a[i]);
+            if (a[i] < d_end && digits < a[i] + W) throw mi::Error(MI_EINVAL, "digits overlap an input");
+        }
+        mi::hoist_batch(*ctx->c, level, count, a, digits, digits_stride, S(stream));
+    });
+}
+
 int mi_rotate_batch(mi_context* ctx, int level, int count, uint64_t* const* b, uint64_t* const* a,
                     const uint64_t* galois, const uint64_t* const* keys, int montgomery, void* stream) {
     return guard("ckks_rotate", [&] {
@@ -826,6 +846,41 @@ int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint
         check_ctx(ctx);
         check_ptr(poly_q0); check_ptr(out);
         mi::modraise(*ctx->c, target_level, poly_q0, out, S(stream));
+    });
+}
+
+int mi_modraise_batch(mi_context* ctx, int target_level, int count, const uint64_t* const* poly_q0,
+                      uint64_t* const* out, void* stream) {
+    return guard("modraise_eval_poly_from_q0", [&] {
+        check_ctx(ctx);
+        if (count < 1 || !poly_q0 || !out) throw mi::Error(MI_EINVAL, "invalid batch");
+        if (target_level < 1 || target_level >= ctx->c->nq) throw mi::Error(MI_EINVAL, "invalid target level");
+        const long long W = (long long)(target_level + 1) * (long long)ctx->c->N;
+        for (int i = 0; i < count; ++i) {
+            check_ptr(poly_q0[i]); check_ptr(out[i]);
+            for (int j = 0; j < i; ++j)
+                if (out[i] < out[j] + W && out[j] < out[i] + W) throw mi::Error(MI_EINVAL, "outputs overlap");
+        }
+        mi::modraise_batch(*ctx->c, target_level, count, poly_q0, out, S(stream));
+    });
+}
+
+int mi_q0p0_keyswitch_batch(mi_context* ctx, int count, uint64_t* const* b, uint64_t* const* a,
+                            const uint64_t* key, int montgomery, void* stream) {
+    return guard("ckks_q0p0_key_switch_level0_inplace", [&] {
+        check_ctx(ctx);
+        check_ptr(key);
+        if (count < 1 || !b || !a) throw mi::Error(MI_EINVAL, "invalid batch");
+        const long long W = (long long)ctx->c->N;
+        for (int i = 0; i < count; ++i) {
+            check_ptr(b[i]); check_ptr(a[i]);
+            for (int j = 0; j < i; ++j)  // switched in place: the ciphertexts must be distinct
+                for (const uint64_t* x : {b[i], a[i]})
+                    for (const uint64_t* y : {b[j], a[j]})
+                        if (x < y + W && y < x + W) throw mi::Error(MI_EINVAL, "ciphertexts of a batch overlap");
+            if (b[i] < a[i] + W && a[i] < b[i] + W) throw mi::Error(MI_EINVAL, "b and a overlap");
+        }
+        mi::q0p0_keyswitch_batch(*ctx->c, count, b, a, key, montgomery != 0, S(stream));
     });
 }
 

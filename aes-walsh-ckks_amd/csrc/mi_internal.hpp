@@ -342,6 +342,8 @@ void keyswitch_noalloc(Context& c, const KsItem& it, bool montgomery, hipStream_
 
 // Hoisting and QP-basis switches (ckks_keyswitch.cpp:344-373, 675-1328).
 void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s);
+void hoist_batch(Context& c, int level, int n, const u64* const* a_eval, u64* digits, long long dstride,
+                 hipStream_t s);
 void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key, u64* b_out,
                        u64* a_out, int b_mode, int a_mode, u64 galois, bool montgomery, hipStream_t s);
 // Rotations i < n of one hoisted ciphertext (digits of a, b): (b_out[i],
@@ -398,6 +400,9 @@ void sample_gaussian(Context& c, u64* out, int limbs, double sigma, u64 seed, u6
 
 // Bootstrapping pieces (mi_boot.hip).
 void modraise(Context& c, int target_level, const u64* poly_q0, u64* out, hipStream_t s);
+void modraise_batch(Context& c, int target_level, int n, const u64* const* poly_q0, u64* const* out, hipStream_t s);
+void q0p0_keyswitch_batch(Context& c, int n, u64* const* b, u64* const* a, const u64* key, bool montgomery,
+                          hipStream_t s);
 void q0p0_keyswitch(Context& c, u64* b, u64* a, const u64* key, bool montgomery, hipStream_t s);
 
 }  // namespace mi

@@ -668,6 +668,17 @@ void hoist(Context& c, int level, const u64* a_eval, u64* digits, hipStream_t s)
     ks_modup(c, level, a_eval, digits, C, s);
 }
 
+void hoist_batch(Context& c, int level, int n, const u64* const* a_eval, u64* digits, long long dstride,
+                 hipStream_t s) {
+    if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
+    if (n < 1) throw Error(-1, "need at least one ciphertext");
+    for (int i0 = 0; i0 < n; i0 += KS_MAX_ITEMS) {
+        const int m = std::min(KS_MAX_ITEMS, n - i0);
+        u64* C = (u64*)c.workspace(s, sizeof(u64) * (size_t)m * (level + 1) * c.N);
+        ks_modup_n(c, level, m, a_eval + i0, digits + (long long)i0 * dstride, dstride, C, s);
+    }
+}
+
 void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key, u64* b_out,
                        u64* a_out, int b_mode, int a_mode, u64 galois, bool montgomery, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");

@@ -105,10 +105,14 @@ void ckks_key_switch_dense_to_sparse_inplace(const CKKSContext& c, CKKSCiphertex
         throw std::invalid_argument("ckks_key_switch: dense-to-sparse expects a dense-secret ciphertext");
     if (ct.getNumPolys() != 2 || ct.getLevel() != 0)
         throw std::invalid_argument("ckks_key_switch: dense-to-sparse expects a level-0 2-polynomial ciphertext");
-    for (size_t m = 0; m < ct.getBatchSize(); ++m)
-        mi_check(mi_q0p0_keyswitch(c.handle(), ct.polyData(0, m), ct.polyData(1, m),
-                                   c.getDenseToSparseSwitchKeyDevice(), c.getParams().montgomery ? 1 : 0, c.stream()),
-                 "ckks_key_switch");
+    std::vector<uint64_t*> b, a;
+    for (size_t m = 0; m < ct.getBatchSize(); ++m) {
+        b.push_back(ct.polyData(0, m));
+        a.push_back(ct.polyData(1, m));
+    }
+    mi_check(mi_q0p0_keyswitch_batch(c.handle(), (int)b.size(), b.data(), a.data(), c.getDenseToSparseSwitchKeyDevice(),
+                                     c.getParams().montgomery ? 1 : 0, c.stream()),
+             "ckks_key_switch");
     ct.setSecretOwner(CKKSSecretOwner::BootstrapSparse);
 }
 
@@ -144,10 +148,16 @@ void ckks_modraise_inplace(const CKKSContext& c, CKKSCiphertext& ct, size_t targ
         throw std::invalid_argument("ckks_modraise: target level must be in [1, max_level]");
     CKKSCiphertext out(c, 2, ct.getScale(), target_level, ct.getSecretOwner(), ct.getMessageEncodingState(),
                        ct.getBatchSize());
+    // every poly of every member as one batch
+    std::vector<const uint64_t*> src;
+    std::vector<uint64_t*> dst;
     for (size_t m = 0; m < ct.getBatchSize(); ++m)
-        for (size_t i = 0; i < 2; ++i)
-            mi_check(mi_modraise(c.handle(), (int)target_level, ct.polyData(i, m), out.polyData(i, m), c.stream()),
-                     "ckks_modraise");
+        for (size_t i = 0; i < 2; ++i) {
+            src.push_back(ct.polyData(i, m));
+            dst.push_back(out.polyData(i, m));
+        }
+    mi_check(mi_modraise_batch(c.handle(), (int)target_level, (int)src.size(), src.data(), dst.data(), c.stream()),
+             "ckks_modraise");
     ct = std::move(out);
 }
 

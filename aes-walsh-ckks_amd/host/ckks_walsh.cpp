@@ -436,18 +436,22 @@ std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_
             galois.push_back(rotation_automorphism_index(N, r));
             keys.push_back(context.getRotationKeyDevice(r));
         }
-    // one hoist per member (the buffer is reused in stream order)
-    DeviceBuffer digits(context.handle(), parts * (L + K) * N);
+    // the hoists of every member as one batch, then each member's rotations
+    const size_t dw = parts * (L + K) * N;
+    DeviceBuffer digits(context.handle(), B * dw);
+    std::vector<const uint64_t*> a_src;
+    for (size_t m = 0; m < B; ++m) a_src.push_back(packed.polyData(1, m));
+    mi_check(mi_hoist_batch(context.handle(), (int)level, (int)B, a_src.data(), digits.data(), (int64_t)dw,
+                            context.stream()),
+             "ckks_walsh_sbox");
     for (size_t m = 0; m < B; ++m) {
-        mi_check(mi_hoist(context.handle(), (int)level, packed.polyData(1, m), digits.data(), context.stream()),
-                 "ckks_walsh_sbox");
         std::vector<uint64_t*> b_out, a_out;
         for (size_t idx : rotated) {
             CKKSCiphertext& dst = idx >= kMasks ? high[idx - kMasks] : low[idx];
             b_out.push_back(dst.polyData(0, m));
             a_out.push_back(dst.polyData(1, m));
         }
-        mi_check(mi_rotate_hoisted_batch(context.handle(), (int)level, digits.data(), packed.polyData(0, m),
+        mi_check(mi_rotate_hoisted_batch(context.handle(), (int)level, digits.data() + m * dw, packed.polyData(0, m),
                                          (int)galois.size(), galois.data(), keys.data(), b_out.data(), a_out.data(),
                                          P.montgomery ? 1 : 0, context.stream()),
                  "ckks_walsh_sbox");

@@ -42,6 +42,9 @@ _SIGS = {
     "mi_poly_list_op": (cint, [vp, cint, cint, cint, vp, vp, vp, vp]),
     "mi_linear_combine": (cint, [vp, cint, cint, vp, cint, vp, vp, vp]),
     "mi_scaled_sum": (cint, [vp, cint, cint, vp, cint, vp, vp, cint, vp, vp]),
+    "mi_hoist_batch": (cint, [vp, cint, cint, vp, vp, ctypes.c_int64, vp]),
+    "mi_modraise_batch": (cint, [vp, cint, cint, vp, vp, vp]),
+    "mi_q0p0_keyswitch_batch": (cint, [vp, cint, vp, vp, vp, cint, vp]),
     "mi_conjugate_batch": (cint, [vp, cint, cint, vp, vp, vp, cint, vp]),
     "mi_stream_destroy": (cint, [vp, vp]),
     "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),

@@ -238,6 +238,14 @@ int mi_conjugate(mi_context* ctx, int level, uint64_t* b, uint64_t* a, const uin
  * ckks_hoist_ciphertext_a_qp_eval (ckks_keyswitch.cpp:344-373): ModUp of every
  * digit of a (level+1 eval limbs) to the other Q limbs and P. */
 int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, void* stream);
+/* mi_hoist of `count` ciphertexts at one level (the per-member hoists of a
+ * batch, e.g. the mask-packed unpack's unpack_segment_from_hoist,
+ * ckks_walsh_sbox.cpp:498-509, over every member): the digits of a[i] at
+ * digits + i * digits_stride words (stride at least parts*(level+1+np)*N;
+ * the digit region must not overlap any input); the ModUps of up to 32
+ * ciphertexts run as one launch sequence. */
+int mi_hoist_batch(mi_context* ctx, int level, int count, const uint64_t* const* a, uint64_t* digits,
+                   int64_t digits_stride, void* stream);
 /* ckks_conjugate_inplace (ckks_conjugate.cpp:16-61) of `count` ciphertexts at
  * one level: each automorphism as in mi_conjugate, then the key switches of
  * all of them as one batch (up to 32 per launch sequence); each result
@@ -380,11 +388,23 @@ int mi_linear_transform_q_batch(mi_context* ctx, int level, int count, const uin
  * domain (target_level+1 limbs; out may alias poly_q0). */
 int mi_modraise(mi_context* ctx, int target_level, const uint64_t* poly_q0, uint64_t* out,
                 void* stream);
+/* mi_modraise of `count` polys (ckks_modraise_inplace over the polys and
+ * members of a ciphertext batch, ckks_modraise.cpp:76-110): out[i] from
+ * poly_q0[i]; up to 32 polys per launch sequence; out[i] may alias
+ * poly_q0[i], outputs must not overlap each other. */
+int mi_modraise_batch(mi_context* ctx, int target_level, int count, const uint64_t* const* poly_q0,
+                      uint64_t* const* out, void* stream);
 /* ckks_q0p0_key_switch_level0_inplace (ckks_keyswitch.cpp:1661-1816): the
  * level-0 dense-to-sparse switch over {q_0, p_0}; key = [b(q0), b(p0),
  * a(q0), a(p0)], N words each.  (b, a) <- (b + KS_b(a), KS_a(a)). */
 int mi_q0p0_keyswitch(mi_context* ctx, uint64_t* b, uint64_t* a, const uint64_t* key,
                       int montgomery, void* stream);
+/* mi_q0p0_keyswitch of `count` level-0 ciphertexts with one key (the
+ * members of a batch, ckks_key_switch_dense_to_sparse_inplace over each):
+ * every phase of up to 32 switches as one launch; ciphertexts must not
+ * overlap. */
+int mi_q0p0_keyswitch_batch(mi_context* ctx, int count, uint64_t* const* b, uint64_t* const* a,
+                            const uint64_t* key, int montgomery, void* stream);
 
 /* ckks_rescale (ckks_rescale.cpp:68-148) of `npolys` listed polys at one
  * level (the polys of one or more ciphertexts), in place; the top limb of

@@ -349,3 +349,56 @@ def test_qp_entry_points_reject_bad_arguments():
         ctx.modraise(0, a, qa)
     with pytest.raises(M.MiError, match="combine mode"):
         ctx.keyswitch_hoisted(3, dig, key, qb, qa, 2, 1)
+
+
+def _parr(ts):
+    import ctypes
+    return (ctypes.c_void_p * len(ts))(*[ptr for ptr in ts])
+
+
+def test_batched_bootstrap_pieces_match_oracle():
+    """The batched forms the bootstrap and the mask-packed unpack use, each
+    against the oracle's single-ciphertext form: mi_modraise_batch over 34
+    polys (two launch sequences; one output aliases its input),
+    mi_q0p0_keyswitch_batch over 35 level-0 ciphertexts, and mi_hoist_batch
+    over 33 ciphertexts with a padded digit stride."""
+    import torch
+    N, nq = 1 << 12, 6
+    M, ctx, octx, q, p, rng = _setup(N, nq, 2, 3, seed=91, qbits=50, pbits=58)
+    lib = M.lib()
+    s = M._stream(torch.cuda.current_stream())
+    target = nq - 1
+    xs = [rng.integers(0, int(q[0]), N, dtype=np.uint64) for _ in range(34)]
+    ins = [M.to_dev(np.concatenate([x, np.zeros(target * N, np.uint64)])) for x in xs]
+    outs = [ins[i] if i == 5 else _zeros(M, (target + 1) * N) for i in range(34)]
+    print("modraise batch of 34", flush=True)
+    M.check(lib.mi_modraise_batch(ctx.h, target, 34, _parr([M.ptr(t) for t in ins]),
+                                  _parr([M.ptr(t) for t in outs]), s))
+    for i, x in enumerate(xs):
+        assert np.array_equal(M.from_dev(outs[i]), octx.modraise(target, x)), i
+    for mont in (True, False):
+        key = np.concatenate([rng.integers(0, int(m), N, dtype=np.uint64) for m in (q[0], p[0], q[0], p[0])])
+        bs = [rng.integers(0, int(q[0]), N, dtype=np.uint64) for _ in range(35)]
+        as_ = [rng.integers(0, int(q[0]), N, dtype=np.uint64) for _ in range(35)]
+        db, da = [M.to_dev(b) for b in bs], [M.to_dev(a) for a in as_]
+        print("q0p0 batch of 35, montgomery", mont, flush=True)
+        M.check(lib.mi_q0p0_keyswitch_batch(ctx.h, 35, _parr([M.ptr(t) for t in db]), _parr([M.ptr(t) for t in da]),
+                                            M.ptr(M.to_dev(key)), int(mont), s))
+        for i in range(35):
+            ob, oa = octx.q0p0_keyswitch(bs[i], as_[i], key, montgomery=mont)
+            assert np.array_equal(M.from_dev(db[i]), ob) and np.array_equal(M.from_dev(da[i]), oa), (mont, i)
+    level = 4
+    L = level + 1
+    parts = octx.parts(level)
+    dw = parts * (L + len(p)) * N
+    stride = dw + 3 * N
+    a_in = [rand_residues(rng, q[:L], N) for _ in range(33)]
+    da = [M.to_dev(a) for a in a_in]
+    dig = _zeros(M, 33 * stride)
+    print("hoist batch of 33", flush=True)
+    M.check(lib.mi_hoist_batch(ctx.h, level, 33, _parr([M.ptr(t) for t in da]), M.ptr(dig), stride, s))
+    got = M.from_dev(dig)
+    for i, a in enumerate(a_in):
+        assert np.array_equal(got[i * stride:i * stride + dw], octx.hoist(level, a)), i
+    # a digit stride below one ciphertext's digits is refused
+    assert lib.mi_hoist_batch(ctx.h, level, 2, _parr([M.ptr(t) for t in da[:2]]), M.ptr(dig), dw - 1, s) < 0
