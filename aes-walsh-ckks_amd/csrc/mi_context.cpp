@@ -205,6 +205,7 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     c->p.assign(p, p + np);
     MI_HIP(hipGetDevice(&c->device));
     noalloc_setup(c->device);
+    if (const char* e = std::getenv("MICKKS_FUSE_MAC")) c->fuse_mac = std::string(e) != "0";
     if (const char* e = std::getenv("MICKKS_NTT_VARIANT")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= NTT_VARIANTS) c->ntt_forced = v;
@@ -236,6 +237,8 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     MI_HIP(hipMemcpy(c->d_inv, inv.data(), tb, hipMemcpyHostToDevice));
     MI_HIP(hipMemcpy(c->d_inv_shoup, ivs.data(), tb, hipMemcpyHostToDevice));
     MI_HIP(hipMemcpy(c->d_bc, c->h_bc.data(), sizeof(BasisConst) * nb, hipMemcpyHostToDevice));
+    MI_HIP(hipMalloc(&c->d_zero, sizeof(u64) * N));
+    MI_HIP(hipMemset(c->d_zero, 0, sizeof(u64) * N));
     c->table_bytes = 4 * tb + sizeof(BasisConst) * nb;
 
     // ---- ModUp / ModDown / rescale tables packed in one blob ----
@@ -476,6 +479,7 @@ void context_destroy(Context* c) {
     (void)hipFree(c->d_inv);
     (void)hipFree(c->d_inv_shoup);
     (void)hipFree(c->d_bc);
+    (void)hipFree(c->d_zero);
     (void)hipFree(c->d_blob);
     (void)hipFree(c->d_half);
     (void)hipFree(c->d_enc_tw);

@@ -143,6 +143,10 @@ struct Context {
     // NTT plan: the kernel variant (1..NTT_VARIANTS) of a pass over `count`
     // limbs.  ntt_forced (environment MICKKS_NTT_VARIANT) overrides the plan.
     int ntt_forced = 0;
+    // Key switches fold the key inner product into ModDown's NTT passes
+    // (MICKKS_FUSE_MAC=0: the separate k_mac pass, for comparison).
+    bool fuse_mac = true;
+    u64* d_zero = nullptr;  // N zero words (a zero row for branch-free jobs)
     int ntt_variant(int count) const;
     TunePlan tune;
     std::string tune_key;  // "<gfx arch> <compute units> <log2 N>"
@@ -310,7 +314,11 @@ MacArgs mac_args(const Context& c, int level, const u64* digits, const u64* key,
 //     (K*N words, destroyed); out[k] = base[k] - NTT(contribution_k), base
 //     may alias out; T = npoly*L*N words.
 void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* D, long long dstride,
-                u64* C, hipStream_t s);
+                u64* C, hipStream_t s, bool cols_only = false);
+// ModDown of n key switches (one level and modes) with the key inner
+// product of their digits folded in (split form); at most 3 digits.
+void ks_moddown_mac(Context& c, int level, int n, const KsItem* items, const u64* D, long long dstride, u64* AP,
+                    u64* T, bool montgomery, hipStream_t s);
 void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s);
 void ks_mac(const Context& c, const MacArgs& a, int level, hipStream_t s);
 void ks_moddown_n(Context& c, int level, int npoly, u64* const* p, const u64* const* base, u64* const* out,

@@ -191,7 +191,7 @@ struct DigitJob {
 };
 
 void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* D, long long dstride,
-                u64* C, hipStream_t s) {
+                u64* C, hipStream_t s, bool cols_only) {
     const int L = level + 1, K = c.np, parts = c.parts(level);
     const u64 N = c.N;
     if (n < 1 || n > KS_MAX_ITEMS) throw Error(-1, "ModUp batch size out of range");
@@ -249,7 +249,8 @@ void ks_modup_n(Context& c, int level, int n, const u64* const* key_terms, u64* 
         }
     }
     DigitJob j{D, dstride, L, K, c.nq, c.alpha, parts, n, N, fast_div(parts * n), fast_div(parts)};
-    run_ntt_fwd(c, j, j, parts * (L + K) * n, s);
+    if (cols_only) launch_pass<true, false>(c, j, parts * (L + K) * n, s);
+    else run_ntt_fwd(c, j, j, parts * (L + K) * n, s);
 }
 
 void ks_modup(Context& c, int level, const u64* key_term, u64* D, u64* C, hipStream_t s) {
@@ -514,6 +515,218 @@ void ks_moddown(Context& c, int level, u64* p0, u64* p1, const u64* base0, const
     ks_moddown_n(c, level, p1 ? 2 : 1, p, b, o, T, s);
 }
 
+// ---- ModDown with the key inner product folded in ----
+// The split form of k_mac (Q slots combined into the outputs, P slots kept
+// for ModDown) computed where ModDown reads it: the P slots' products in the
+// load of the inverse NTT's first pass, the Q slots' in the store of the
+// forward NTT's last pass (with the combine mode, the b term and the
+// subtraction of the NTT'd contribution).  The products are neither written
+// nor read back.  Each sum holds PARTS <= 3 canonical products (below q
+// 2^64: REDC's precondition), reduced once.
+template <bool MONT, int PARTS>
+__device__ __forceinline__ u64 key_product(const u64* dig, long long dstep, const u64* key, long long kstep, int i,
+                                           const BasisConst& b) {
+    U128 s{0, 0};
+#pragma unroll
+    for (int d = 0; d < PARTS; ++d) mac128(s, reduce_4q(dig[d * dstep + i], b.q), reduce_4q(key[d * kstep + i], b.q));
+    return MONT ? redc128(s.lo, s.hi, b.q, b.qninv) : reduce128(s.lo, s.hi, b.q, b.mu_lo, b.mu_hi);
+}
+
+struct KeyProductLists {
+    const u64* digits;  // item i's digits at digits + i * dstride: [parts][L+K][N]
+    long long dstride;
+    const u64* key[KS_MAX_ITEMS];
+    int L, K, nq;
+    u64 N;
+    const BasisConst* bc;
+    // slot `slot` of poly k (even: a, odd: b) of item k / 2
+    __device__ void at(int k, int slot, int basis, const u64*& dig, const u64*& key_p) const {
+        const int item = k >> 1;
+        dig = digits + item * dstride + (long long)slot * N;
+        key_p = key[item] + ((long long)((k & 1) ? 0 : nq + K) + basis) * N;
+    }
+};
+
+// inverse NTT first (row) pass over the P slots: load = key product
+template <bool MONT, int PARTS>
+struct MacPJob {
+    KeyProductLists kp;
+    u64* p[2 * KS_MAX_ITEMS];  // poly k's P accumulator (K limbs)
+    int npoly;
+    FastDiv per_limb;  // npoly
+    struct View {
+        int basis;
+        u64* p;
+        const u64* dig;
+        const u64* key;
+        long long dstep, kstep;
+        BasisConst b;
+        __device__ u64 load(int i) const { return key_product<MONT, PARTS>(dig, dstep, key, kstep, i, b); }
+        __device__ void store(int i, u64 v) const { p[i] = v; }
+    };
+    __device__ View view(int k) const {
+        const int pi = per_limb.div(k), poly = k - pi * npoly;
+        const int basis = kp.nq + pi;
+        View v;
+        v.basis = basis;
+        v.p = p[poly] + (long long)pi * kp.N;
+        kp.at(poly, kp.L + pi, basis, v.dig, v.key);
+        v.dstep = (long long)(kp.L + kp.K) * kp.N;
+        v.kstep = 2LL * (kp.nq + kp.K) * kp.N;
+        v.b = kp.bc[basis];
+        return v;
+    }
+    __device__ bool skip(int) const { return false; }
+};
+
+// forward NTT last (row) pass over the Q slots: out = combine(old, key
+// product + b term) - NTT(contribution); old and the b term read from a zero
+// row where the mode assigns / there is no b term
+template <bool MONT, int PARTS>
+struct MdMacOutJob {
+    KeyProductLists kp;
+    const u64* tmp;                        // [npoly][L][N] NTT'd contributions
+    u64* out[2 * KS_MAX_ITEMS];
+    const u64* old[2 * KS_MAX_ITEMS];      // out (add mode) or the zero row
+    const u64* add[2 * KS_MAX_ITEMS];      // the b term or the zero row
+    long long old_step[2 * KS_MAX_ITEMS];  // limb step of old / add: N, or 0 for the zero row
+    long long add_step[2 * KS_MAX_ITEMS];
+    int npoly;
+    FastDiv per_limb;  // npoly
+    struct View {
+        int basis;
+        const u64* src;
+        u64* out;
+        const u64* old;
+        const u64* add;
+        const u64* dig;
+        const u64* key;
+        long long dstep, kstep;
+        BasisConst b;
+        __device__ u64 load(int i) const { return src[i]; }
+        __device__ u64 pre(int i) const { return add_mod(old[i], add[i], b.q); }
+        __device__ void store_pre(int i, u64 v, u64 base) const {
+            const u64 acc = add_mod(key_product<MONT, PARTS>(dig, dstep, key, kstep, i, b), base, b.q);
+            out[i] = sub_mod(acc, v, b.q);
+        }
+    };
+    __device__ View view(int k) const {
+        const int t = per_limb.div(k), poly = k - t * npoly;
+        View v;
+        v.basis = t;
+        v.src = tmp + ((long long)poly * kp.L + t) * kp.N;
+        v.out = out[poly] + (long long)t * kp.N;
+        v.old = old[poly] + (long long)t * old_step[poly];
+        v.add = add[poly] + (long long)t * add_step[poly];
+        kp.at(poly, t, t, v.dig, v.key);
+        v.dstep = (long long)(kp.L + kp.K) * kp.N;
+        v.kstep = 2LL * (kp.nq + kp.K) * kp.N;
+        v.b = kp.bc[t];
+        return v;
+    }
+    __device__ bool skip(int) const { return false; }
+};
+
+template <bool MONT, int PARTS>
+void moddown_mac_t(Context& c, int level, int n, const KsItem* items, const u64* D, long long dstride, u64* AP,
+                   u64* T, hipStream_t s) {
+    const int L = level + 1, K = c.np, npoly = 2 * n;
+    const u64 N = c.N;
+    KeyProductLists kp{};
+    kp.digits = D;
+    kp.dstride = dstride;
+    for (int i = 0; i < n; ++i) kp.key[i] = items[i].key;
+    kp.L = L;
+    kp.K = K;
+    kp.nq = c.nq;
+    kp.N = N;
+    kp.bc = c.d_bc;
+    u64* p[2 * KS_MAX_ITEMS];
+    for (int i = 0; i < n; ++i) {
+        p[2 * i] = AP + (size_t)i * 2 * K * N;
+        p[2 * i + 1] = p[2 * i] + (size_t)K * N;
+    }
+    {
+        MacPJob<MONT, PARTS> in{};
+        in.kp = kp;
+        for (int k = 0; k < npoly; ++k) in.p[k] = p[k];
+        in.npoly = npoly;
+        in.per_limb = fast_div(npoly);
+        ListJob o{};
+        for (int k = 0; k < npoly; ++k) o.p[k] = p[k];
+        o.npoly = npoly;
+        o.per_limb = fast_div(npoly);
+        o.basis0 = c.nq;
+        o.N = N;
+        const long long probe = c.probe_open(s, K * npoly);
+        launch_rows_w4<false>(c, in, K * npoly, s);
+        launch_pass<false, false>(c, o, K * npoly, s);
+        c.probe_close(probe, s);
+    }
+    MdContribArgs a{};
+    for (int k = 0; k < npoly; ++k) a.src[k] = p[k];
+    a.tmp = T;
+    a.pinv = c.moddown[level].pinv;
+    a.L = L;
+    a.K = K;
+    a.nq = c.nq;
+    a.logN = c.logN;
+    a.bc = c.d_bc;
+    dim3 grid((unsigned)(N / KS_THREADS), npoly, (L + MODDOWN_TCH - 1) / MODDOWN_TCH);
+    switch (K) {
+#define MI_MODDOWN_CASE(KK) \
+        case KK: hipLaunchKernelGGL(k_moddown_contrib<KK>, grid, dim3(KS_THREADS), 0, s, a); break;
+        MI_NS_CASES(MI_MODDOWN_CASE)
+#undef MI_MODDOWN_CASE
+        default: throw Error(-1, "more than 16 special moduli");
+    }
+    MI_HIP(hipGetLastError());
+    MdInJob in{T, L, npoly, N, fast_div(npoly)};
+    MdMacOutJob<MONT, PARTS> o{};
+    o.kp = kp;
+    o.tmp = T;
+    for (int i = 0; i < n; ++i) {
+        const KsItem& it = items[i];
+        // poly 2i: a, poly 2i+1: b
+        o.out[2 * i] = it.a_out;
+        o.out[2 * i + 1] = it.b_out;
+        o.old[2 * i] = it.a_mode == 0 ? it.a_out : c.d_zero;
+        o.old_step[2 * i] = it.a_mode == 0 ? (long long)N : 0;
+        o.old[2 * i + 1] = it.b_mode == 0 ? it.b_out : c.d_zero;
+        o.old_step[2 * i + 1] = it.b_mode == 0 ? (long long)N : 0;
+        o.add[2 * i] = c.d_zero;
+        o.add_step[2 * i] = 0;
+        o.add[2 * i + 1] = it.init_b ? it.init_b : c.d_zero;
+        o.add_step[2 * i + 1] = it.init_b ? (long long)N : 0;
+    }
+    o.npoly = npoly;
+    o.per_limb = fast_div(npoly);
+    const long long probe = c.probe_open(s, npoly * L);
+    launch_pass<true, false>(c, in, npoly * L, s);
+    launch_rows_w4<true>(c, o, npoly * L, s);
+    c.probe_close(probe, s);
+}
+
+void ks_moddown_mac(Context& c, int level, int n, const KsItem* items, const u64* D, long long dstride, u64* AP,
+                    u64* T, bool montgomery, hipStream_t s) {
+    if (n < 1 || n > KS_MAX_ITEMS) throw Error(-1, "key switch batch size out of range");
+    if (c.np > MAX_ALPHA) throw Error(-1, "more than 16 special moduli");
+    const int parts = c.parts(level);
+#define MI_MDMAC(M, PP) moddown_mac_t<M, PP>(c, level, n, items, D, dstride, AP, T, s)
+    if (montgomery) {
+        if (parts == 1) MI_MDMAC(true, 1);
+        else if (parts == 2) MI_MDMAC(true, 2);
+        else if (parts == 3) MI_MDMAC(true, 3);
+        else throw Error(-1, "folded key product takes at most 3 digits");
+    } else {
+        if (parts == 1) MI_MDMAC(false, 1);
+        else if (parts == 2) MI_MDMAC(false, 2);
+        else if (parts == 3) MI_MDMAC(false, 3);
+        else throw Error(-1, "folded key product takes at most 3 digits");
+    }
+#undef MI_MDMAC
+}
+
 // ---------------------------------------------------------------- entry points
 size_t ks_item_words(const Context& c, int level) {
     const size_t L = level + 1, K = c.np, parts = c.parts(level), N = c.N;
@@ -542,6 +755,10 @@ void keyswitch_items_with(Context& c, const KsItem* items, int n, bool montgomer
     const u64* kt[KS_MAX_ITEMS];
     for (int i = 0; i < n; ++i) kt[i] = items[i].key_term;
     ks_modup_n(c, level, n, kt, D, dw, C, s);
+    if (parts <= 3 && c.fuse_mac) {
+        ks_moddown_mac(c, level, n, items, D, dw, AP, T, montgomery, s);
+        return;
+    }
     MacArgs a = mac_args(c, level, D, items[0].key, montgomery);
     a.items = n;
     a.digits_stride = dw;

@@ -161,8 +161,7 @@ k_ntt_cols(NttTabs tb, Job job, int count) {
 }
 
 template <int LOGN, bool FWD, int LPB, bool HALF, class Job>
-__global__ void __launch_bounds__(NTT_THREADS)
-k_ntt_rows(NttTabs tb, Job job, int count) {
+__device__ __forceinline__ void ntt_rows_body(const NttTabs& tb, const Job& job, int count) {
     constexpr int n1 = LOGN / 2, n2 = LOGN - n1, LOGR = n2;
     constexpr int G = pass_groups(n2, n1), wpl = pass_wpl(n2, n1);
     using LW = std::conditional_t<HALF, u32, u64>;  // LDS exchange word
@@ -221,6 +220,19 @@ k_ntt_rows(NttTabs tb, Job job, int count) {
     }
 }
 
+template <int LOGN, bool FWD, int LPB, bool HALF, class Job>
+__global__ void __launch_bounds__(NTT_THREADS) k_ntt_rows(NttTabs tb, Job job, int count) {
+    ntt_rows_body<LOGN, FWD, LPB, HALF>(tb, job, count);
+}
+// The same pass held to 128 VGPRs (four waves per SIMD, the LDS limit), for
+// jobs whose load or store does enough work of its own that the scheduler
+// would otherwise trade occupancy for loads hoisted across the whole pass.
+template <int LOGN, bool FWD, class Job>
+__global__ void __launch_bounds__(NTT_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+k_ntt_rows_w4(NttTabs tb, Job job, int count) {
+    ntt_rows_body<LOGN, FWD, 1, false>(tb, job, count);
+}
+
 #define MI_NTT_LOGN_CASES(K)                                                                          \
     switch (c.logN) {                                                                                 \
         case 10: K(10); break;                                                                        \
@@ -253,6 +265,18 @@ void launch_rows(const Context& c, const Job& job, int count, hipStream_t s) {
     const unsigned blocks = (unsigned)((count + LPB - 1) / LPB);
 #define MI_ROWS(LN)                                                                                   \
     hipLaunchKernelGGL((k_ntt_rows<LN, FWD, LPB, HALF, Job>), dim3(blocks * pass_wpl(LN - LN / 2, LN / 2)), \
+                       dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job, count)
+    MI_NTT_LOGN_CASES(MI_ROWS)
+#undef MI_ROWS
+    MI_HIP(hipGetLastError());
+}
+
+template <bool FWD, class Job>
+void launch_rows_w4(const Context& c, const Job& job, int count, hipStream_t s) {
+    if (count >= (1 << 20)) throw Error(-1, "ntt: more than 2^20 limbs in one launch");
+    const NttTabs tb = tabs_of(c);
+#define MI_ROWS(LN)                                                                                   \
+    hipLaunchKernelGGL((k_ntt_rows_w4<LN, FWD, Job>), dim3((unsigned)count * pass_wpl(LN - LN / 2, LN / 2)), \
                        dim3(pass_threads(LN - LN / 2, LN / 2)), 0, s, tb, job, count)
     MI_NTT_LOGN_CASES(MI_ROWS)
 #undef MI_ROWS

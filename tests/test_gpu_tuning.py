@@ -90,3 +90,29 @@ def test_every_keyswitch_group_matches_the_oracle(group):
     ctx.keyswitch_batch(items)
     for (_, _, _, db, da, _, _), (ob, oa) in zip(items, want):
         assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa)
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+@pytest.mark.parametrize("mont", [True, False])
+def test_folded_key_product_matches_the_oracle(fuse, mont, monkeypatch):
+    """Key switches with the key inner product folded into ModDown's NTT
+    passes (the default) and as the separate k_mac pass (MICKKS_FUSE_MAC=0),
+    at 1, 2 and 3 digits, with add and assign modes, against the oracle
+    (the b term the rotations add is covered by the batched rotation
+    tests)."""
+    monkeypatch.setenv("MICKKS_FUSE_MAC", fuse)
+    M = product()
+    N = 1 << 13
+    p, q = O.find_primes([58] * 3, [45] * 9, N)
+    ctx = M.Context(N, q, p, 3)
+    octx = O.Ctx(N, q, p, 3)
+    rng = np.random.default_rng(7 + int(fuse) + 2 * mont)
+    basis = list(q) + list(p)
+    key = np.concatenate([rand_residues(rng, basis, N) for _ in range(2 * octx.parts(len(q) - 1))])
+    dkey = M.to_dev(key)
+    for level, bm, am in ((8, 0, 1), (5, 1, 1), (2, 0, 0)):   # 3, 2 and 1 digits
+        kt, b, a = (rand_residues(rng, q[:level + 1], N) for _ in range(3))
+        ob, oa = octx.keyswitch(level, kt, key, b, a, bm, am, montgomery=mont)
+        db, da = M.to_dev(b), M.to_dev(a)
+        ctx.keyswitch(level, M.to_dev(kt), dkey, db, da, bm, am, montgomery=mont)
+        assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa), (level, bm, am)
