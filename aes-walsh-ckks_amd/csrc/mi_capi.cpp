@@ -675,6 +675,31 @@ int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, 
     });
 }
 
+int mi_rotate_hoisted_items(mi_context* ctx, int level, int count, const uint64_t* digits, int64_t digits_stride,
+                            const uint64_t* const* b, const uint64_t* galois, const uint64_t* const* keys,
+                            uint64_t* const* b_out, uint64_t* const* a_out, int montgomery, void* stream) {
+    return guard("ckks_hybrid_key_switch_hoisted", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        check_ptr(digits);
+        if (count < 1 || !b || !galois || !keys || !b_out || !a_out) throw mi::Error(MI_EINVAL, "invalid batch");
+        const mi::Context& c = *ctx->c;
+        const long long dw = (long long)c.parts(level) * (level + 1 + c.np) * (long long)c.N;
+        if (digits_stride != 0 && digits_stride < dw)
+            throw mi::Error(MI_EINVAL, "digit stride smaller than one ciphertext's digits");
+        const long long W = (long long)(level + 1) * (long long)c.N;
+        const uint64_t* d_end = digits + (digits_stride ? (count - 1) * digits_stride : 0) + dw;
+        for (int i = 0; i < count; ++i) {
+            check_galois(galois[i], false);
+            check_ptr(b[i]); check_ptr(keys[i]); check_ptr(b_out[i]); check_ptr(a_out[i]);
+            for (const uint64_t* o : {b_out[i], a_out[i]})
+                if (o < d_end && digits < o + W) throw mi::Error(MI_EINVAL, "an output overlaps the digits");
+        }
+        mi::rotate_hoisted_items(*ctx->c, level, digits, digits_stride, b, count, galois, keys, b_out, a_out,
+                                 montgomery != 0, S(stream));
+    });
+}
+
 int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* key,
                          uint64_t* b_out, uint64_t* a_out, int b_mode, int a_mode, uint64_t galois,
                          int montgomery, void* stream) {

@@ -359,6 +359,11 @@ void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key,
 // overlap digits or b.
 void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* galois,
                   const u64* const* keys, bool montgomery, hipStream_t s);
+// Hoisted rotations, item i from the digits at digits + i * dstride (0: one
+// shared hoist) with b[i] added to its b.
+void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, const u64* const* b, int n,
+                          const u64* galois, const u64* const* keys, u64* const* b_out, u64* const* a_out,
+                          bool montgomery, hipStream_t s);
 void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
                           const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
                           hipStream_t s);

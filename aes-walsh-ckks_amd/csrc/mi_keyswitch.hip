@@ -943,15 +943,16 @@ __global__ void __launch_bounds__(KS_THREADS) k_automorphism_items(AutoItemsArgs
     dst[i] = src[sidx];
 }
 
-void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
-                          const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
-                          hipStream_t s) {
+void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, const u64* const* b, int n,
+                          const u64* galois, const u64* const* keys, u64* const* b_out, u64* const* a_out,
+                          bool montgomery, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
     if (n < 1) throw Error(-1, "need at least one rotation");
     const int L = level + 1, K = c.np;
     const u64 N = c.N;
     const long long LN = (long long)L * (long long)N, KN = (long long)K * (long long)N;
     const int chunk = std::min(n, c.tune.ks_group);
+    const bool fold = c.parts(level) <= 3 && c.fuse_mac;
     // per rotation: P accumulators (a, b) | switched pair (a, b) | ModDown temp
     u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (2 * KN + 4 * LN));
     u64* AP = ws;
@@ -959,32 +960,42 @@ void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b
     u64* Tm = SW + chunk * 2 * LN;
     for (int i0 = 0; i0 < n; i0 += chunk) {
         const int m = std::min(chunk, n - i0);
-        // key products of the shared hoisted digits, b added in the Q limbs
-        // (ckks_hybrid_key_switch_hoisted_inplace with b Add, a Assign)
-        MacArgs ma = mac_args(c, level, digits, keys[i0], montgomery);
-        ma.items = m;
-        ma.digits_stride = 0;
-        ma.accp = AP;
-        ma.accp_stride = 2 * KN;
-        ma.b_mode = 1;
-        ma.a_mode = 1;
-        for (int it = 0; it < m; ++it) {
-            ma.key[it] = keys[i0 + it];
-            ma.a_out[it] = SW + it * 2 * LN;
-            ma.b_out[it] = SW + it * 2 * LN + LN;
-            ma.init_b[it] = b;
+        const u64* dig = digits + (long long)i0 * dstride;
+        // key products of the hoisted digits, b added in the Q limbs
+        // (ckks_hybrid_key_switch_hoisted_inplace with b Add, a Assign),
+        // into the staging pairs
+        if (fold) {
+            KsItem items[KS_MAX_ITEMS];
+            for (int it = 0; it < m; ++it)
+                items[it] = KsItem{nullptr, keys[i0 + it], SW + it * 2 * LN + LN, SW + it * 2 * LN, level, 1, 1,
+                                   b[i0 + it]};
+            ks_moddown_mac(c, level, m, items, dig, dstride, AP, Tm, montgomery, s);
+        } else {
+            MacArgs ma = mac_args(c, level, dig, keys[i0], montgomery);
+            ma.items = m;
+            ma.digits_stride = dstride;
+            ma.accp = AP;
+            ma.accp_stride = 2 * KN;
+            ma.b_mode = 1;
+            ma.a_mode = 1;
+            for (int it = 0; it < m; ++it) {
+                ma.key[it] = keys[i0 + it];
+                ma.a_out[it] = SW + it * 2 * LN;
+                ma.b_out[it] = SW + it * 2 * LN + LN;
+                ma.init_b[it] = b[i0 + it];
+            }
+            ks_mac(c, ma, level, s);
+            u64* p[2 * KS_MAX_ITEMS];
+            const u64* base[2 * KS_MAX_ITEMS];
+            u64* out[2 * KS_MAX_ITEMS];
+            for (int it = 0; it < m; ++it) {
+                p[2 * it] = AP + it * 2 * KN;
+                p[2 * it + 1] = AP + it * 2 * KN + KN;
+                base[2 * it] = out[2 * it] = SW + it * 2 * LN;
+                base[2 * it + 1] = out[2 * it + 1] = SW + it * 2 * LN + LN;
+            }
+            ks_moddown_n(c, level, 2 * m, p, base, out, Tm, s);
         }
-        ks_mac(c, ma, level, s);
-        u64* p[2 * KS_MAX_ITEMS];
-        const u64* base[2 * KS_MAX_ITEMS];
-        u64* out[2 * KS_MAX_ITEMS];
-        for (int it = 0; it < m; ++it) {
-            p[2 * it] = AP + it * 2 * KN;
-            p[2 * it + 1] = AP + it * 2 * KN + KN;
-            base[2 * it] = out[2 * it] = SW + it * 2 * LN;
-            base[2 * it + 1] = out[2 * it + 1] = SW + it * 2 * LN + LN;
-        }
-        ks_moddown_n(c, level, 2 * m, p, base, out, Tm, s);
         // the automorphisms (the rotation's NTT-domain map)
         AutoItemsArgs aa{};
         aa.src = SW;
@@ -999,6 +1010,13 @@ void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b
                            0, s, aa);
         MI_HIP(hipGetLastError());
     }
+}
+
+void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
+                          const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
+                          hipStream_t s) {
+    std::vector<const u64*> bs((size_t)std::max(n, 0), b);
+    rotate_hoisted_items(c, level, digits, 0, bs.data(), n, galois, keys, b_out, a_out, montgomery, s);
 }
 
 // In-place rotations of independent ciphertexts at one level

@@ -268,6 +268,17 @@ int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, con
 int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
                             const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
                             uint64_t* const* a_out, int montgomery, void* stream);
+/* mi_rotate_hoisted_batch with its own hoist per rotation: rotation i from
+ * the digits at digits + i * digits_stride words (0: one shared hoist; else
+ * at least parts*(level+1+np)*N, e.g. mi_hoist_batch's output) with b[i]
+ * added, (b_out[i], a_out[i]) = g_i(b[i] + KS_i(a_i)).  Ordered rotation-
+ * major over the members of a batch (the mask-packed unpack's
+ * unpack_segment_from_hoist, ckks_walsh_sbox.cpp:498-509, for every member
+ * at one rotation), a launch's rotations share one key.  Outputs must not
+ * overlap the digits. */
+int mi_rotate_hoisted_items(mi_context* ctx, int level, int count, const uint64_t* digits, int64_t digits_stride,
+                            const uint64_t* const* b, const uint64_t* galois, const uint64_t* const* keys,
+                            uint64_t* const* b_out, uint64_t* const* a_out, int montgomery, void* stream);
 /* ckks_rotate_inplace (ckks_rotation.cpp:41-96) for `count` independent
  * ciphertexts at one level, as the mask-packed parity input rotates its 29
  * parity sums (add_rotated_segment_term_inplace, ckks_walsh_sbox.cpp:

@@ -402,3 +402,45 @@ def test_batched_bootstrap_pieces_match_oracle():
         assert np.array_equal(got[i * stride:i * stride + dw], octx.hoist(level, a)), i
     # a digit stride below one ciphertext's digits is refused
     assert lib.mi_hoist_batch(ctx.h, level, 2, _parr([M.ptr(t) for t in da[:2]]), M.ptr(dig), dw - 1, s) < 0
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_rotate_hoisted_items_per_member_digits(fuse, monkeypatch):
+    """mi_rotate_hoisted_items over 35 members' own hoists (mi_hoist_batch,
+    one digit stride apart) at one rotation, and over one shared hoist
+    (stride 0) at several rotations, each equal to mi_keyswitch_hoisted of
+    that member (b Add, a Assign, galois), with the key product folded into
+    ModDown and as its own pass."""
+    import torch
+    monkeypatch.setenv("MICKKS_FUSE_MAC", fuse)
+    M, ctx, octx, q, p, rng = _setup(1 << 12, 9, 3, 3, seed=61)
+    lib = M.lib()
+    s = M._stream(torch.cuda.current_stream())
+    N, lv = 1 << 12, 7
+    L, T = lv + 1, lv + 1 + len(p)
+    dw = ctx.parts(lv) * T * N
+    B = 35
+    a_in = [M.to_dev(rand_residues(rng, q[:L], N)) for _ in range(B)]
+    b_in = [M.to_dev(rand_residues(rng, q[:L], N)) for _ in range(B)]
+    dig = _zeros(M, B * dw)
+    M.check(lib.mi_hoist_batch(ctx.h, lv, B, _parr([M.ptr(t) for t in a_in]), M.ptr(dig), dw, s))
+    key = M.to_dev(_key(rng, q, p, N, octx.parts(len(q) - 1)))
+    g = O.galois_rotation(N, 5)
+    outs = [(_zeros(M, L * N), _zeros(M, L * N)) for _ in range(B)]
+    gal = (ctypes_u64(B))(*([g] * B))
+    print("rotate_hoisted_items: 35 members, fuse", fuse, flush=True)
+    M.check(lib.mi_rotate_hoisted_items(ctx.h, lv, B, M.ptr(dig), dw, _parr([M.ptr(t) for t in b_in]), gal,
+                                        _parr([M.ptr(key)] * B), _parr([M.ptr(o[0]) for o in outs]),
+                                        _parr([M.ptr(o[1]) for o in outs]), 1, s))
+    for m in range(B):
+        one = _zeros(M, dw)
+        ctx.hoist(lv, a_in[m], one)
+        hb, ha = M.to_dev(M.from_dev(b_in[m])), _zeros(M, L * N)
+        ctx.keyswitch_hoisted(lv, one, key, hb, ha, 0, 1, galois=g)
+        assert np.array_equal(M.from_dev(outs[m][0]), M.from_dev(hb)), m
+        assert np.array_equal(M.from_dev(outs[m][1]), M.from_dev(ha)), m
+
+
+def ctypes_u64(n):
+    import ctypes
+    return ctypes.c_uint64 * n
