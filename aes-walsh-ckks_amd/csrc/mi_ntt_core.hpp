@@ -209,6 +209,15 @@ __device__ __forceinline__ void to_strided_n(u64 (&x)[LPB][E], int t, LW* lds) {
     }
 }
 
+// Twiddles of the first sub-stage after an LDS exchange, loaded before the
+// exchange's barrier (loads cannot move across it), so their latency
+// overlaps the exchange instead of stalling the first butterflies; one limb
+// per workgroup only (the registers of two limbs' sets would cost occupancy).
+template <int NT>
+struct TwPre {
+    u64 w[NT], ws[NT];
+};
+
 template <int LOGR, int LPB, class IDX, class LW>
 __device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, LW* lds, const BfConst (&kq)[LPB],
                                             const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
@@ -224,6 +233,17 @@ __device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, LW* lds, co
             for (int i = 0; i < LPB; ++i) bf_fwd(x[i][j0], x[i][j0 + dj], W[i][k], WS[i][k], kq[i]);
         }
     }
+    // first contiguous sub-stage (s = LOGE): twiddle (t*E + j0) >> (LOGR - LOGE)
+    constexpr int SH = LOGR - LOGE, NF = E >> SH;
+    TwPre<NF> pf;
+    if constexpr (LPB == 1) {
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            const int k = idx(LOGE, ((t * E) >> SH) + v);
+            pf.w[v] = W[0][k];
+            pf.ws[v] = WS[0][k];
+        }
+    }
     __syncthreads();
     to_contiguous_n<LOGR, LPB>(x, t, lds);
 #pragma unroll
@@ -232,6 +252,10 @@ __device__ __forceinline__ void group_fwd_n(u64 (&x)[LPB][E], int t, LW* lds, co
 #pragma unroll
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & d) continue;
+            if (LPB == 1 && s == LOGE) {
+                bf_fwd(x[0][j0], x[0][j0 + d], pf.w[j0 >> SH], pf.ws[j0 >> SH], kq[0]);
+                continue;
+            }
             const int k = idx(s, (t * E + j0) >> (LOGR - s));
 #pragma unroll
             for (int i = 0; i < LPB; ++i) bf_fwd(x[i][j0], x[i][j0 + d], W[i][k], WS[i][k], kq[i]);
@@ -245,6 +269,17 @@ template <int LOGR, bool LAST, int LPB, class IDX, class LW>
 __device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, LW* lds, const BfConst (&kq)[LPB],
                                             const u64* const (&W)[LPB], const u64* const (&WS)[LPB], IDX idx) {
     using G = Grp<LOGR>;
+    // first contiguous sub-stage (u = 0): twiddle (t*E + j0) >> 1
+    constexpr int NC = E / 2;
+    TwPre<NC> pc;
+    if constexpr (LPB == 1) {
+#pragma unroll
+        for (int v = 0; v < NC; ++v) {
+            const int k = idx(0, t * NC + v);
+            pc.w[v] = W[0][k];
+            pc.ws[v] = WS[0][k];
+        }
+    }
     __syncthreads();
     to_contiguous_n<LOGR, LPB>(x, t, lds);
 #pragma unroll
@@ -253,9 +288,25 @@ __device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, LW* lds, co
 #pragma unroll
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & d) continue;
+            if (LPB == 1 && u == 0) {
+                bf_inv(x[0][j0], x[0][j0 + d], pc.w[j0 >> 1], pc.ws[j0 >> 1], kq[0]);
+                continue;
+            }
             const int k = idx(u, (t * E + j0) >> (u + 1));
 #pragma unroll
             for (int i = 0; i < LPB; ++i) bf_inv(x[i][j0], x[i][j0 + d], W[i][k], WS[i][k], kq[i]);
+        }
+    }
+    // first strided sub-stage (u = LOGE): twiddle j0 >> (LOGE + 1 - LOGTG)
+    constexpr bool HAS_S = (LAST ? LOGR : LOGR - 1) > LOGE;
+    constexpr int SS = LOGE + 1 - G::LOGTG, NS = E >> SS;
+    TwPre<NS> ps;
+    if constexpr (LPB == 1 && HAS_S) {
+#pragma unroll
+        for (int v = 0; v < NS; ++v) {
+            const int k = idx(LOGE, v);
+            ps.w[v] = W[0][k];
+            ps.ws[v] = WS[0][k];
         }
     }
     __syncthreads();
@@ -266,6 +317,10 @@ __device__ __forceinline__ void group_inv_n(u64 (&x)[LPB][E], int t, LW* lds, co
 #pragma unroll
         for (int j0 = 0; j0 < E; ++j0) {
             if (j0 & dj) continue;
+            if (LPB == 1 && u == LOGE) {
+                bf_inv(x[0][j0], x[0][j0 + dj], ps.w[j0 >> SS], ps.ws[j0 >> SS], kq[0]);
+                continue;
+            }
             const int k = idx(u, j0 >> (u + 1 - G::LOGTG));
 #pragma unroll
             for (int i = 0; i < LPB; ++i) bf_inv(x[i][j0], x[i][j0 + dj], W[i][k], WS[i][k], kq[i]);
