@@ -993,7 +993,7 @@ int mi_scaled_sum(mi_context* ctx, int level, int npolys, uint64_t* const* out, 
         check_ctx(ctx);
         check_level(*ctx->c, level);
         if (npolys < 1 || !out || !in || !residues) throw mi::Error(MI_EINVAL, "invalid poly list");
-        if (nterms < 1 || nterms > 16) throw mi::Error(MI_EINVAL, "1..16 terms per scaled sum");
+        if (nterms < 1 || nterms > 64) throw mi::Error(MI_EINVAL, "1..64 terms per scaled sum");
         if (nconst < 0 || nconst > npolys || (nconst && !constant)) throw mi::Error(MI_EINVAL, "invalid constant");
         const size_t W = (size_t)(level + 1) * ctx->c->N;
         auto overlap = [W](const uint64_t* x, const uint64_t* y) { return x < y + W && y < x + W; };

@@ -563,20 +563,22 @@ void linear_combine(const Context& c, int level, int nout, u64* const* out, int 
 }
 
 // out[k] = cst[limb] (k < nconst) + sum_t r[t][limb] * in[t][k] mod q: a
-// whole Chebyshev leaf (its constant and its scaled baby-step powers) in one
-// pass per chunk of polys.  in[t][k] null: term t has no poly k.  The sum of
-// up to SS_TERMS canonical products is exact in 128 bits (8 q^2 < 2^128 for
-// q < 2^62) and reduced once.  Kernel arguments stay below 4 KB: a launch
-// takes SS_POLYS polys and as many terms as SS_RES residue words hold;
-// later term chunks accumulate into the output.
-constexpr int SS_POLYS = 16;
+// whole Chebyshev leaf (its constant and its scaled baby-step powers), a
+// parity sum or a packing sum in one pass per chunk of polys.  in[t][k]
+// null: term t has no poly k.  The sum of up to SS_TERMS canonical products
+// is exact in 128 bits (8 q^2 < 2^128 for q < 2^62) and reduced once.
+// Kernel arguments stay below 4 KB: a launch takes as many terms as SS_RES
+// residue words hold (at most SS_TERMS) and as many polys as SS_PTRS input
+// pointers allow for them; later term chunks accumulate into the output.
+constexpr int SS_MAXP = 64;
+constexpr int SS_PTRS = 128;
 constexpr int SS_TERMS = 8;
 constexpr int SS_RES = 256;
 struct ScaledSumArgs {
-    u64* out[SS_POLYS];
-    const u64* in[SS_TERMS][SS_POLYS];
-    u64 r[SS_RES];  // [term][limb], then the constant's residues at [nterms][limb]
-    int nterms, limbs, nconst, accumulate;
+    u64* out[SS_MAXP];
+    const u64* in[SS_PTRS];  // [term][poly of the launch]
+    u64 r[SS_RES];           // [term][limb], then the constant's residues at [nterms][limb]
+    int nterms, limbs, nconst, accumulate, np;
 };
 __global__ void __launch_bounds__(EW_THREADS)
 k_scaled_sum(const BasisConst* __restrict__ bc, ScaledSumArgs a, int logN) {
@@ -589,7 +591,7 @@ k_scaled_sum(const BasisConst* __restrict__ bc, ScaledSumArgs a, int logN) {
     if (a.accumulate) acc.lo = *po;
     else if (k < a.nconst) acc.lo = a.r[a.nterms * a.limbs + limb];
     for (int t = 0; t < a.nterms; ++t) {
-        const u64* pi = a.in[t][k];
+        const u64* pi = a.in[t * a.np + k];
         if (pi) mac128(acc, pi[r], a.r[t * a.limbs + limb]);
     }
     *po = reduce128(acc.lo, acc.hi, b.q, b.mu_lo, b.mu_hi);
@@ -598,7 +600,7 @@ k_scaled_sum(const BasisConst* __restrict__ bc, ScaledSumArgs a, int logN) {
 void scaled_sum(const Context& c, int level, int npolys, u64* const* out, int nterms, const u64* const* in,
                 const u64* residues, int nconst, const u64* constant, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
-    if (npolys < 1 || nterms < 1 || nterms > 16) throw Error(-1, "1..16 terms over at least one poly");
+    if (npolys < 1 || nterms < 1 || nterms > 64) throw Error(-1, "1..64 terms over at least one poly");
     if (nconst < 0 || nconst > npolys) throw Error(-1, "constant polys out of range");
     const int limbs = level + 1;
     for (int l = 0; l < limbs; ++l) {
@@ -609,18 +611,20 @@ void scaled_sum(const Context& c, int level, int npolys, u64* const* out, int nt
     // terms per launch: the residues of the chunk and the constant fit SS_RES words
     const int per = std::min(SS_TERMS, SS_RES / limbs - 1);
     if (per < 1) throw Error(-1, "too many limbs for a scaled sum");
-    for (int k0 = 0; k0 < npolys; k0 += SS_POLYS) {
-        const int n = std::min(SS_POLYS, npolys - k0);
-        for (int t0 = 0; t0 < nterms; t0 += per) {
-            const int m = std::min(per, nterms - t0);
+    for (int t0 = 0; t0 < nterms; t0 += per) {
+        const int m = std::min(per, nterms - t0);
+        const int np = std::min(SS_MAXP, SS_PTRS / m);
+        for (int k0 = 0; k0 < npolys; k0 += np) {
+            const int n = std::min(np, npolys - k0);
             ScaledSumArgs a{};
             a.nterms = m;
             a.limbs = limbs;
+            a.np = n;
             a.accumulate = t0 > 0;
             a.nconst = std::max(0, std::min(n, nconst - k0));
             for (int k = 0; k < n; ++k) {
                 a.out[k] = out[k0 + k];
-                for (int t = 0; t < m; ++t) a.in[t][k] = in[(size_t)(t0 + t) * npolys + k0 + k];
+                for (int t = 0; t < m; ++t) a.in[t * n + k] = in[(size_t)(t0 + t) * npolys + k0 + k];
             }
             for (int t = 0; t < m; ++t)
                 for (int l = 0; l < limbs; ++l) a.r[t * limbs + l] = residues[(size_t)(t0 + t) * limbs + l];

@@ -313,20 +313,6 @@ constexpr size_t kSegments = 30;  // kPackedWalshSignSegments: 15 low + 15 high 
 // packed_walsh_segment (ckks_walsh_sbox.cpp:476-482)
 size_t packed_segment(bool high_nibble, size_t mask) { return (high_nibble ? kMasks - 1 : 0) + (mask - 1); }
 
-// parity_sum (ckks_walsh_sbox.cpp:363-383): the sum of the masked input bits
-CKKSCiphertext parity_sum(const CKKSContext& context, const MyVector<CKKSCiphertext>& input, size_t bit_offset,
-                          size_t mask) {
-    std::optional<CKKSCiphertext> out;
-    for (size_t bit = 0; bit < 4; ++bit) {
-        if (((mask >> bit) & 1) == 0) continue;
-        const CKKSCiphertext& term = input[bit_offset + bit];
-        if (!out) out.emplace(term.clone());
-        else ckks_add_inplace(context, *out, term);
-    }
-    if (!out) throw std::invalid_argument("ckks_walsh_sbox: parity mask must be nonzero");
-    return std::move(*out);
-}
-
 CKKSCiphertext constant_one_like(const CKKSContext& context, const CKKSCiphertext& ref) {
     auto out = ref.clone();  // zero_like (:216-220), then the constant 1
     ckks_mul_inplace(context, out, int64_t{0});
@@ -365,15 +351,39 @@ CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& cont
             throw std::invalid_argument("ckks_walsh_sbox: inconsistent input ciphertexts");
     check_segment_keys(context, block_count);
     const size_t N = context.getParams().getN();
-    // the 30 parity sums in the reference's order (low mask m, then high mask m)
+    // the 30 parity sums in the reference's order (low mask m, then high mask
+    // m), as one scaled sum: term t's poly k is the sum of the masked input
+    // bits' poly k (parity_sum: the first set bit's copy plus the others)
+    const size_t B = ref.getBatchSize();
     MyVector<CKKSCiphertext> terms;
     std::vector<size_t> segments;
     terms.reserve(kSegments);
+    const size_t np = kSegments * 2 * B;
+    std::vector<uint64_t*> out;
+    std::vector<const uint64_t*> in(4 * np, nullptr);
+    out.reserve(np);
     for (size_t mask = 1; mask < kMasks; ++mask)
         for (int high = 0; high < 2; ++high) {
-            terms.push_back(parity_sum(context, input, high ? 4 : 0, mask));
+            const size_t off = high ? 4 : 0;
+            size_t first = 0;
+            while (((mask >> first) & 1) == 0) ++first;
+            const CKKSCiphertext& lead = input[off + first];
+            terms.emplace_back(context, 2, lead.getScale(), lead.getLevel(), lead.getSecretOwner(),
+                               lead.getMessageEncodingState(), B);
             segments.push_back(packed_segment(high != 0, mask));
+            for (size_t k = 0; k < 2; ++k)
+                for (size_t m = 0; m < B; ++m) {
+                    const size_t row = out.size();
+                    out.push_back(terms.back().polyData(k, m));
+                    for (size_t bit = 0; bit < 4; ++bit)
+                        if ((mask >> bit) & 1) in[bit * np + row] = input[off + bit].polyData(k, m);
+                }
         }
+    const size_t L = ref.getLevel() + 1;
+    const std::vector<uint64_t> ones(4 * L, 1);
+    mi_check(mi_scaled_sum(context.handle(), (int)ref.getLevel(), (int)np, out.data(), 4, in.data(), ones.data(), 0,
+                           nullptr, context.stream()),
+             "ckks_walsh_sbox");
     // rotate_segment_inplace (:484-495) of every term but segment 0, as one batch
     std::vector<uint64_t*> b, a;
     std::vector<uint64_t> galois;
@@ -391,9 +401,21 @@ CKKSCiphertext ckks_walsh_build_mask_packed_parity_input(const CKKSContext& cont
     mi_check(mi_rotate_batch(context.handle(), (int)ref.getLevel(), (int)b.size(), b.data(), a.data(), galois.data(),
                              keys.data(), context.getParams().montgomery ? 1 : 0, context.stream()),
              "ckks_rotate");
-    // add_rotated_segment_term_inplace's accumulation (:497-519)
-    CKKSCiphertext packed = std::move(terms[0]);
-    for (size_t t = 1; t < terms.size(); ++t) ckks_add_inplace(context, packed, terms[t]);
+    // add_rotated_segment_term_inplace's accumulation (:497-519) as one sum of
+    // the 30 terms (term 0 unrotated) into a new ciphertext with term 0's metadata
+    CKKSCiphertext packed(context, 2, terms[0].getScale(), terms[0].getLevel(), terms[0].getSecretOwner(),
+                          terms[0].getMessageEncodingState(), B);
+    std::vector<uint64_t*> pout;
+    std::vector<const uint64_t*> pin(terms.size() * 2 * B);
+    for (size_t k = 0; k < 2; ++k)
+        for (size_t m = 0; m < B; ++m) pout.push_back(packed.polyData(k, m));
+    for (size_t t = 0; t < terms.size(); ++t)
+        for (size_t k = 0; k < 2; ++k)
+            for (size_t m = 0; m < B; ++m) pin[t * 2 * B + k * B + m] = terms[t].polyData(k, m);
+    const std::vector<uint64_t> pones(terms.size() * L, 1);
+    mi_check(mi_scaled_sum(context.handle(), (int)ref.getLevel(), (int)(2 * B), pout.data(), (int)terms.size(),
+                           pin.data(), pones.data(), 0, nullptr, context.stream()),
+             "ckks_walsh_sbox");
     return packed;
 }
 

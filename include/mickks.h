@@ -452,15 +452,17 @@ int mi_linear_combine(mi_context* ctx, int level, int nout, uint64_t* const* out
 /* out[k] = constant[limb] (k < nconst) + sum_t residues[t * (level+1) + limb]
  * * in[t * npolys + k] mod q over limbs 0..level, for the npolys listed
  * output polys (in[t * npolys + k] null: term t has no poly k; 1 <= nterms
- * <= 16; residues and constant below their moduli; outputs overlap neither
+ * <= 64; residues and constant below their moduli; outputs overlap neither
  * each other nor any input).  One Chebyshev leaf of the Paterson-Stockmeyer
  * evaluation — the reference's zero ciphertext, its constant and its loop of
  * MulThenAdd over the baby-step powers (evaluate_leaf / MulThenAdd,
  * ckks_evalpoly.cpp:352-398 and the leaf loop that calls it), with every
  * integer scale-up folded into the residues on the host — in one pass per
  * 16 polys instead of a zero fill, a constant add and a read-modify-write
- * per term.  The sum is exact mod q, so the result equals the reference's
- * sequence of operations. */
+ * per term.  Also the mask-packed parity input's 30 parity sums (masked
+ * input bits, coefficients 1) and its packing sum of the 30 rotated terms
+ * (ckks_walsh_sbox.cpp:363-383, 497-519).  The sum is exact mod q, so the
+ * result equals the reference's sequence of operations. */
 int mi_scaled_sum(mi_context* ctx, int level, int npolys, uint64_t* const* out, int nterms,
                   const uint64_t* const* in, const uint64_t* residues, int nconst, const uint64_t* constant,
                   void* stream);

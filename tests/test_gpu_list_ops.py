@@ -114,12 +114,13 @@ def test_linear_combine_matches_numpy_and_checks_arguments():
                                      c17.ctypes.data_as(ctypes.c_void_p), s) < 0                                # > 16 terms
 
 
-@pytest.mark.parametrize("nlimbs,npolys,nterms", [(4, 40, 11), (40, 18, 16)])
+@pytest.mark.parametrize("nlimbs,npolys,nterms", [(4, 40, 11), (40, 18, 16), (3, 70, 2), (5, 33, 30)])
 def test_scaled_sum_matches_python_ints(nlimbs, npolys, nterms):
     """mi_scaled_sum (one Chebyshev leaf: constant + sum of per-limb scaled
-    terms) against Python integers: outputs in chunks of 16 polys, terms in
-    chunks (8 at 4 limbs, 5 at 40 limbs) that accumulate, absent terms,
-    a constant on the first polys, 61-bit moduli (128-bit sums)."""
+    terms) against Python integers: terms in chunks (8 at 4 limbs, 5 at 40
+    limbs) that accumulate, outputs in chunks of 16 to 64 polys by the
+    chunk's term count, up to 30 terms, absent terms, a constant on the
+    first polys, 61-bit moduli (128-bit sums)."""
     M = product()
     import torch
     N = 1 << 10
@@ -152,10 +153,10 @@ def test_scaled_sum_matches_python_ints(nlimbs, npolys, nterms):
     r1 = res[:1].copy()
     assert M.lib().mi_scaled_sum(ctx.h, level, 1, one, 1, one, r1.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0
     assert "overlaps" in M.lib().mi_last_error().decode()
-    src17 = _arr([M.ptr(dout[1])] * 17)
-    r17 = np.zeros((17, L), np.uint64)
-    assert M.lib().mi_scaled_sum(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 17, src17,
-                                 r17.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0                # > 16 terms
+    src65 = _arr([M.ptr(dout[1])] * 65)
+    r65 = np.zeros((65, L), np.uint64)
+    assert M.lib().mi_scaled_sum(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 65, src65,
+                                 r65.ctypes.data_as(ctypes.c_void_p), 0, None, s) < 0                # > 64 terms
     bad = res[:1].copy()
     bad[0, 0] = q[0]
     assert M.lib().mi_scaled_sum(ctx.h, level, 1, _arr([M.ptr(dout[0])]), 1, _arr([M.ptr(dout[1])]),
