@@ -34,9 +34,13 @@ struct WalshArgs {
 };
 
 // sum_m c[m] x_m mod q for |c| < 2^15, x_m = x1 2^31 + x0 given as 31-bit
-// halves: two signed 32x32+64 multiply-adds per term, |partial sums| < 2^50,
-// shifted non-negative by off (a multiple of q above 2^50), then one 128-bit
-// reduction of (A1 + off) 2^31 + (A0 + off).
+// halves: two signed 32x32+64 multiply-adds per term, |partial sums| < 2^50.
+// SMALL (q < 2^43, so x1 < 2^12 and |A1 2^31 + A0| < 2^62.1): the signed
+// 64-bit value A1 2^31 + A0 shifted non-negative by off (a multiple of q
+// near 2^63) and one 64-bit reduction.  Otherwise both halves shifted by off
+// (a multiple of q above 2^50) and one 128-bit reduction of
+// (A1 + off) 2^31 + (A0 + off).
+template <bool SMALL>
 __device__ __forceinline__ u64 walsh_lc(const int* c, const int (&x0)[WALSH_MASKS - 1],
                                         const int (&x1)[WALSH_MASKS - 1], u64 off, const BasisConst& b) {
     long long a0 = 0, a1 = 0;
@@ -46,10 +50,15 @@ __device__ __forceinline__ u64 walsh_lc(const int* c, const int (&x0)[WALSH_MASK
         a0 += (long long)cm * x0[m - 1];
         a1 += (long long)cm * x1[m - 1];
     }
-    const u64 u = (u64)(a1 + (long long)off), w = (u64)(a0 + (long long)off);
-    const u64 lo = (u << 31) + w;
-    const u64 hi = (u >> 33) + (lo < w ? 1 : 0);
-    return reduce128(lo, hi, b.q, b.mu_lo, b.mu_hi);
+    if constexpr (SMALL) {
+        const u64 v = (u64)(a1 * (1LL << 31) + a0) + off;
+        return reduce64(v, b.q, b.mu_hi);
+    } else {
+        const u64 u = (u64)(a1 + (long long)off), w = (u64)(a0 + (long long)off);
+        const u64 lo = (u << 31) + w;
+        const u64 hi = (u >> 33) + (lo < w ? 1 : 0);
+        return reduce128(lo, hi, b.q, b.mu_lo, b.mu_hi);
+    }
 }
 
 // A block holds 64 coefficients in 8 waves: side (b or a) x quarter of the
@@ -75,8 +84,10 @@ __global__ void __launch_bounds__(WALSH_BLOCK) k_walsh_product_sum(WalshArgs a) 
         const u64 r = reduce128(a.v_lo[t], a.v_hi[t], b.q, b.mu_lo, b.mu_hi);
         s_const[t] = ((a.v_neg >> t) & 1) && r ? b.q - r : r;
     }
-    // q * ceil(2^50 / q): a multiple of q above every partial sum's magnitude
-    const u64 off = b.q * (((1ull << 50) + b.q - 1) / b.q);
+    // a multiple of q above every partial sum's magnitude: q ceil(2^50 / q),
+    // or, for the one-word sum, q ceil(2^63 / q) (<= 2^63 + q < 2^64 - 2^62.1)
+    const bool small = b.q < (1ull << 43);  // uniform: one limb per block
+    const u64 off = small ? b.q * (((1ull << 63) + b.q - 1) / b.q) : b.q * (((1ull << 50) + b.q - 1) / b.q);
     int x0[WALSH_MASKS - 1], x1[WALSH_MASKS - 1];
 #pragma unroll
     for (int m = 0; m < WALSH_MASKS - 1; ++m) {
@@ -91,8 +102,8 @@ __global__ void __launch_bounds__(WALSH_BLOCK) k_walsh_product_sum(WalshArgs a) 
 #pragma unroll 1
     for (int k = 0; k < 4; ++k) {
         const int h = 4 * quarter + k;
-        u64 r = walsh_lc(a.cr[h], x0, x1, off, b);
-        u64 im = walsh_lc(a.ci[h], x0, x1, off, b);
+        u64 r = small ? walsh_lc<true>(a.cr[h], x0, x1, off, b) : walsh_lc<false>(a.cr[h], x0, x1, off, b);
+        u64 im = small ? walsh_lc<true>(a.ci[h], x0, x1, off, b) : walsh_lc<false>(a.ci[h], x0, x1, off, b);
         if (side == 0) {
             r = add_mod(r, s_const[2 * h], b.q);
             im = add_mod(im, s_const[2 * h + 1], b.q);

@@ -49,6 +49,8 @@ def _case(N, qbits, nq, level, seed, coeff_max, pt_mont, big_consts):
     (1 << 12, 50, 4, 3, 64, True, False),      # Walsh-table sized coefficients
     (1 << 11, 61, 3, 2, 32767, False, True),   # extreme coefficients, 61-bit moduli, 128-bit constants
     (1 << 13, 40, 6, 5, 8, True, True),
+    (1 << 11, 43, 3, 2, 32767, True, True),    # one-word sums at their bound: q < 2^43, extreme coefficients
+    (1 << 12, 36, 4, 3, 32767, False, False),
 ])
 def test_walsh_product_sum_matches_oracle(N, qbits, nq, level, coeff_max, pt_mont, big):
     print(f"walsh N={N} qbits={qbits} level={level} coeff_max={coeff_max} pt_mont={pt_mont}", flush=True)
