@@ -618,6 +618,14 @@ int mi_hoist(mi_context* ctx, int level, const uint64_t* a, uint64_t* digits, vo
     });
 }
 
+// Spans [lo, lo + W) of a batch's outputs must be pairwise disjoint (chunks
+// of a batch may run concurrently); sorted, so O(n log n) for large batches.
+static void check_disjoint(std::vector<const uint64_t*> lo, long long W, const char* what) {
+    std::sort(lo.begin(), lo.end());
+    for (size_t i = 1; i < lo.size(); ++i)
+        if (lo[i] < lo[i - 1] + W) throw mi::Error(MI_EINVAL, what);
+}
+
 int mi_hoist_batch(mi_context* ctx, int level, int count, const uint64_t* const* a, uint64_t* digits,
                    int64_t digits_stride, void* stream) {
     return guard("ckks_hoist_ciphertext_a_qp_eval", [&] {
@@ -666,18 +674,26 @@ int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, 
         check_level(*ctx->c, level);
         check_ptr(digits); check_ptr(b);
         if (count < 1 || !galois || !keys || !b_out || !a_out) throw mi::Error(MI_EINVAL, "invalid batch");
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        std::vector<const uint64_t*> outs;
         for (int i = 0; i < count; ++i) {
             check_galois(galois[i], false);
             check_ptr(keys[i]); check_ptr(b_out[i]); check_ptr(a_out[i]);
+            for (const uint64_t* o : {b_out[i], a_out[i]}) {
+                if (o < b + W && b < o + W) throw mi::Error(MI_EINVAL, "an output overlaps b");
+                outs.push_back(o);
+            }
         }
+        check_disjoint(outs, W, "outputs overlap");
         mi::rotate_hoisted_batch(*ctx->c, level, digits, b, count, galois, keys, b_out, a_out, montgomery != 0,
                                  S(stream));
     });
 }
 
 int mi_rotate_hoisted_items(mi_context* ctx, int level, int count, const uint64_t* digits, int64_t digits_stride,
-                            const uint64_t* const* b, const uint64_t* galois, const uint64_t* const* keys,
-                            uint64_t* const* b_out, uint64_t* const* a_out, int montgomery, void* stream) {
+                            int digits_period, const uint64_t* const* b, const uint64_t* galois,
+                            const uint64_t* const* keys, uint64_t* const* b_out, uint64_t* const* a_out,
+                            int montgomery, void* stream) {
     return guard("ckks_hybrid_key_switch_hoisted", [&] {
         check_ctx(ctx);
         check_level(*ctx->c, level);
@@ -688,15 +704,30 @@ int mi_rotate_hoisted_items(mi_context* ctx, int level, int count, const uint64_
         if (digits_stride != 0 && digits_stride < dw)
             throw mi::Error(MI_EINVAL, "digit stride smaller than one ciphertext's digits");
         const long long W = (long long)(level + 1) * (long long)c.N;
-        const uint64_t* d_end = digits + (digits_stride ? (count - 1) * digits_stride : 0) + dw;
+        if (digits_period < 0) throw mi::Error(MI_EINVAL, "negative digit period");
+        const int period = digits_period == 0 || digits_period > count ? count : digits_period;
+        const uint64_t* d_end = digits + (digits_stride ? (long long)(period - 1) * digits_stride : 0) + dw;
+        std::vector<const uint64_t*> outs, ins;
         for (int i = 0; i < count; ++i) {
             check_galois(galois[i], false);
             check_ptr(b[i]); check_ptr(keys[i]); check_ptr(b_out[i]); check_ptr(a_out[i]);
-            for (const uint64_t* o : {b_out[i], a_out[i]})
+            for (const uint64_t* o : {b_out[i], a_out[i]}) {
                 if (o < d_end && digits < o + W) throw mi::Error(MI_EINVAL, "an output overlaps the digits");
+                outs.push_back(o);
+            }
+            ins.push_back(b[i]);
         }
-        mi::rotate_hoisted_items(*ctx->c, level, digits, digits_stride, b, count, galois, keys, b_out, a_out,
-                                 montgomery != 0, S(stream));
+        check_disjoint(outs, W, "outputs overlap");
+        // no output may overlap an input b (chunks run concurrently)
+        std::sort(ins.begin(), ins.end());
+        ins.erase(std::unique(ins.begin(), ins.end()), ins.end());
+        for (const uint64_t* o : outs) {
+            auto it = std::upper_bound(ins.begin(), ins.end(), o);
+            if (it != ins.end() && *it < o + W) throw mi::Error(MI_EINVAL, "an output overlaps an input");
+            if (it != ins.begin() && o < *(it - 1) + W) throw mi::Error(MI_EINVAL, "an output overlaps an input");
+        }
+        mi::rotate_hoisted_items(*ctx->c, level, digits, digits_stride, period, b, count, galois, keys, b_out,
+                                 a_out, montgomery != 0, S(stream));
     });
 }
 

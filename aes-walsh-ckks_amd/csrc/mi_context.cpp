@@ -206,6 +206,7 @@ Context* context_create(u64 N, const u64* q, int nq, const u64* p, int np, int d
     MI_HIP(hipGetDevice(&c->device));
     noalloc_setup(c->device);
     if (const char* e = std::getenv("MICKKS_FUSE_MAC")) c->fuse_mac = std::string(e) != "0";
+    if (const char* e = std::getenv("MICKKS_MULTI_STREAM")) c->multi_stream = std::string(e) != "0";
     if (const char* e = std::getenv("MICKKS_NTT_VARIANT")) {
         const int v = std::atoi(e);
         if (v >= 1 && v <= NTT_VARIANTS) c->ntt_forced = v;

@@ -146,6 +146,9 @@ struct Context {
     // Key switches fold the key inner product into ModDown's NTT passes
     // (MICKKS_FUSE_MAC=0: the separate k_mac pass, for comparison).
     bool fuse_mac = true;
+    // Independent chunks of a batch run on the side streams too
+    // (MICKKS_MULTI_STREAM=0: all on the caller's stream).
+    bool multi_stream = true;
     u64* d_zero = nullptr;  // N zero words (a zero row for branch-free jobs)
     int ntt_variant(int count) const;
     TunePlan tune;
@@ -359,11 +362,11 @@ void keyswitch_hoisted(Context& c, int level, const u64* digits, const u64* key,
 // overlap digits or b.
 void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, const u64* galois,
                   const u64* const* keys, bool montgomery, hipStream_t s);
-// Hoisted rotations, item i from the digits at digits + i * dstride (0: one
-// shared hoist) with b[i] added to its b.
-void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, const u64* const* b, int n,
-                          const u64* galois, const u64* const* keys, u64* const* b_out, u64* const* a_out,
-                          bool montgomery, hipStream_t s);
+// Hoisted rotations, item i from the digits at digits + (i mod period) *
+// dstride (dstride 0: one shared hoist) with b[i] added to its b.
+void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, int period,
+                          const u64* const* b, int n, const u64* galois, const u64* const* keys, u64* const* b_out,
+                          u64* const* a_out, bool montgomery, hipStream_t s);
 void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
                           const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
                           hipStream_t s);

@@ -943,24 +943,66 @@ __global__ void __launch_bounds__(KS_THREADS) k_automorphism_items(AutoItemsArgs
     dst[i] = src[sidx];
 }
 
-void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, const u64* const* b, int n,
-                          const u64* galois, const u64* const* keys, u64* const* b_out, u64* const* a_out,
-                          bool montgomery, hipStream_t s) {
+// Independent chunks of a batch spread over the caller's stream and the
+// context's side streams (each with its own workspace of `ws_bytes`): the
+// side streams start after the work already on s, and s continues after all
+// of them.  A chunk's launches overlap the fill and drain of another's.
+// fn(stream, workspace, chunk index).
+template <class F>
+void over_streams(Context& c, hipStream_t s, int nchunks, size_t ws_bytes, F&& fn) {
+    if (nchunks <= 1 || !c.multi_stream) {
+        u64* ws = (u64*)c.workspace(s, ws_bytes);
+        for (int i = 0; i < nchunks; ++i) fn(s, ws, i);
+        return;
+    }
+    const auto& side = c.side_streams();
+    const int ns = std::min(nchunks, 1 + (int)side.size());
+    std::vector<hipStream_t> st(ns);
+    std::vector<u64*> ws(ns);
+    st[0] = s;
+    for (int k = 1; k < ns; ++k) st[k] = side[k - 1];
+    for (int k = 0; k < ns; ++k) ws[k] = (u64*)c.workspace(st[k], ws_bytes);
+    hipEvent_t fork;
+    MI_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    MI_HIP(hipEventRecord(fork, s));
+    for (int k = 1; k < ns; ++k) MI_HIP(hipStreamWaitEvent(st[k], fork, 0));
+    for (int i = 0; i < nchunks; ++i) fn(st[i % ns], ws[i % ns], i);
+    for (int k = 1; k < ns; ++k) {
+        hipEvent_t j;
+        MI_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        MI_HIP(hipEventRecord(j, st[k]));
+        MI_HIP(hipStreamWaitEvent(s, j, 0));
+        MI_HIP(hipEventDestroy(j));
+    }
+    MI_HIP(hipEventDestroy(fork));
+}
+
+void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, int period,
+                          const u64* const* b, int n, const u64* galois, const u64* const* keys, u64* const* b_out,
+                          u64* const* a_out, bool montgomery, hipStream_t s) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid ciphertext level");
     if (n < 1) throw Error(-1, "need at least one rotation");
+    if (period < 1 || period > n) period = n;
     const int L = level + 1, K = c.np;
     const u64 N = c.N;
     const long long LN = (long long)L * (long long)N, KN = (long long)K * (long long)N;
-    const int chunk = std::min(n, c.tune.ks_group);
+    // item i reads the hoist at digits + (i mod period) * dstride: chunks of
+    // up to the group size inside each run of `period` items
+    const int chunk = std::min(period, c.tune.ks_group);
+    const int per_block = (period + chunk - 1) / chunk;
+    const int nblocks = (n + period - 1) / period;
     const bool fold = c.parts(level) <= 3 && c.fuse_mac;
     // per rotation: P accumulators (a, b) | switched pair (a, b) | ModDown temp
-    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (2 * KN + 4 * LN));
-    u64* AP = ws;
-    u64* SW = AP + chunk * 2 * KN;
-    u64* Tm = SW + chunk * 2 * LN;
-    for (int i0 = 0; i0 < n; i0 += chunk) {
-        const int m = std::min(chunk, n - i0);
-        const u64* dig = digits + (long long)i0 * dstride;
+    const size_t ws_bytes = sizeof(u64) * (size_t)chunk * (2 * KN + 4 * LN);
+    over_streams(c, s, nblocks * per_block, ws_bytes, [&](hipStream_t st, u64* ws, int ci) {
+        const int blk = ci / per_block, cb = ci - blk * per_block;
+        const int i0 = blk * period + cb * chunk;
+        if (i0 >= n) return;
+        const int m = std::min(std::min(chunk, period - cb * chunk), n - i0);
+        u64* AP = ws;
+        u64* SW = AP + chunk * 2 * KN;
+        u64* Tm = SW + chunk * 2 * LN;
+        const u64* dig = digits + (long long)(cb * chunk) * dstride;
         // key products of the hoisted digits, b added in the Q limbs
         // (ckks_hybrid_key_switch_hoisted_inplace with b Add, a Assign),
         // into the staging pairs
@@ -969,7 +1011,7 @@ void rotate_hoisted_items(Context& c, int level, const u64* digits, long long ds
             for (int it = 0; it < m; ++it)
                 items[it] = KsItem{nullptr, keys[i0 + it], SW + it * 2 * LN + LN, SW + it * 2 * LN, level, 1, 1,
                                    b[i0 + it]};
-            ks_moddown_mac(c, level, m, items, dig, dstride, AP, Tm, montgomery, s);
+            ks_moddown_mac(c, level, m, items, dig, dstride, AP, Tm, montgomery, st);
         } else {
             MacArgs ma = mac_args(c, level, dig, keys[i0], montgomery);
             ma.items = m;
@@ -984,7 +1026,7 @@ void rotate_hoisted_items(Context& c, int level, const u64* digits, long long ds
                 ma.b_out[it] = SW + it * 2 * LN + LN;
                 ma.init_b[it] = b[i0 + it];
             }
-            ks_mac(c, ma, level, s);
+            ks_mac(c, ma, level, st);
             u64* p[2 * KS_MAX_ITEMS];
             const u64* base[2 * KS_MAX_ITEMS];
             u64* out[2 * KS_MAX_ITEMS];
@@ -994,7 +1036,7 @@ void rotate_hoisted_items(Context& c, int level, const u64* digits, long long ds
                 base[2 * it] = out[2 * it] = SW + it * 2 * LN;
                 base[2 * it + 1] = out[2 * it + 1] = SW + it * 2 * LN + LN;
             }
-            ks_moddown_n(c, level, 2 * m, p, base, out, Tm, s);
+            ks_moddown_n(c, level, 2 * m, p, base, out, Tm, st);
         }
         // the automorphisms (the rotation's NTT-domain map)
         AutoItemsArgs aa{};
@@ -1007,16 +1049,16 @@ void rotate_hoisted_items(Context& c, int level, const u64* digits, long long ds
         aa.L = L;
         aa.logN = c.logN;
         hipLaunchKernelGGL(k_automorphism_items, dim3((unsigned)(m * 2 * L * (N / KS_THREADS))), dim3(KS_THREADS),
-                           0, s, aa);
+                           0, st, aa);
         MI_HIP(hipGetLastError());
-    }
+    });
 }
 
 void rotate_hoisted_batch(Context& c, int level, const u64* digits, const u64* b, int n, const u64* galois,
                           const u64* const* keys, u64* const* b_out, u64* const* a_out, bool montgomery,
                           hipStream_t s) {
     std::vector<const u64*> bs((size_t)std::max(n, 0), b);
-    rotate_hoisted_items(c, level, digits, 0, bs.data(), n, galois, keys, b_out, a_out, montgomery, s);
+    rotate_hoisted_items(c, level, digits, 0, n, bs.data(), n, galois, keys, b_out, a_out, montgomery, s);
 }
 
 // In-place rotations of independent ciphertexts at one level
@@ -1034,17 +1076,18 @@ void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, co
     const long long LN = (long long)L * (long long)N;
     const int chunk = std::min(n, c.tune.ks_group);
     const size_t kw = ks_item_words(c, level);
-    u64* ws = (u64*)c.workspace(s, sizeof(u64) * (size_t)chunk * (kw + 2 * LN));
-    u64* SW = ws + (size_t)chunk * kw;  // per item: a' then b'
-    for (int i0 = 0; i0 < n; i0 += chunk) {
+    over_streams(c, s, (n + chunk - 1) / chunk, sizeof(u64) * (size_t)chunk * (kw + 2 * LN),
+                 [&](hipStream_t st, u64* ws, int ci) {
+        const int i0 = ci * chunk;
         const int m = std::min(chunk, n - i0);
+        u64* SW = ws + (size_t)chunk * kw;  // per item: a' then b'
         KsItem items[KS_MAX_ITEMS];
         for (int it = 0; it < m; ++it) {
             u64* a_sw = SW + it * 2 * LN;
             u64* b_sw = a_sw + LN;
             items[it] = KsItem{a[i0 + it], keys[i0 + it], b_sw, a_sw, level, 1, 1, b[i0 + it]};
         }
-        keyswitch_items_with(c, items, m, montgomery, ws, s);
+        keyswitch_items_with(c, items, m, montgomery, ws, st);
         AutoItemsArgs aa{};
         aa.src = SW;
         for (int it = 0; it < m; ++it) {
@@ -1055,9 +1098,9 @@ void rotate_batch(Context& c, int level, int n, u64* const* b, u64* const* a, co
         aa.L = L;
         aa.logN = c.logN;
         hipLaunchKernelGGL(k_automorphism_items, dim3((unsigned)(m * 2 * L * (N / KS_THREADS))), dim3(KS_THREADS),
-                           0, s, aa);
+                           0, st, aa);
         MI_HIP(hipGetLastError());
-    }
+    });
 }
 
 // dst[k] = automorphism g of src[k] (L limbs each), for up to 2 KS_MAX_ITEMS polys

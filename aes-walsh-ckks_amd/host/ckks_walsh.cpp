@@ -466,25 +466,28 @@ std::pair<MyVector<CKKSCiphertext>, MyVector<CKKSCiphertext>> ckks_walsh_unpack_
     mi_check(mi_hoist_batch(context.handle(), (int)level, (int)B, a_src.data(), digits.data(), (int64_t)dw,
                             context.stream()),
              "ckks_walsh_sbox");
-    // every member's rotation by one segment offset as one launch sequence
-    // (rotation-major: the members of a launch share the rotation key)
-    std::vector<const uint64_t*> b_in(B);
-    for (size_t m = 0; m < B; ++m) b_in[m] = packed.polyData(0, m);
+    // every member's rotation by every segment offset in one call,
+    // rotation-major (the members of a launch share the rotation key; the
+    // library spreads the launches over its streams)
+    std::vector<const uint64_t*> b_in, key;
+    std::vector<uint64_t*> b_out, a_out;
+    std::vector<uint64_t> gal;
     for (size_t r = 0; r < rotated.size(); ++r) {
         const size_t idx = rotated[r];
         CKKSCiphertext& dst = idx >= kMasks ? high[idx - kMasks] : low[idx];
-        std::vector<uint64_t*> b_out(B), a_out(B);
-        std::vector<uint64_t> gal(B, galois[r]);
-        std::vector<const uint64_t*> key(B, keys[r]);
         for (size_t m = 0; m < B; ++m) {
-            b_out[m] = dst.polyData(0, m);
-            a_out[m] = dst.polyData(1, m);
+            b_in.push_back(packed.polyData(0, m));
+            b_out.push_back(dst.polyData(0, m));
+            a_out.push_back(dst.polyData(1, m));
+            gal.push_back(galois[r]);
+            key.push_back(keys[r]);
         }
-        mi_check(mi_rotate_hoisted_items(context.handle(), (int)level, (int)B, digits.data(), (int64_t)dw, b_in.data(),
-                                         gal.data(), key.data(), b_out.data(), a_out.data(), P.montgomery ? 1 : 0,
-                                         context.stream()),
-                 "ckks_walsh_sbox");
     }
+    // item r * B + m reads member m's digits (period B, stride dw)
+    mi_check(mi_rotate_hoisted_items(context.handle(), (int)level, (int)b_in.size(), digits.data(), (int64_t)dw,
+                                     (int)B, b_in.data(), gal.data(), key.data(), b_out.data(), a_out.data(),
+                                     P.montgomery ? 1 : 0, context.stream()),
+             "ckks_walsh_sbox");
     return {std::move(low), std::move(high)};
 }
 

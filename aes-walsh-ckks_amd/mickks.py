@@ -45,7 +45,7 @@ _SIGS = {
     "mi_hoist_batch": (cint, [vp, cint, cint, vp, vp, ctypes.c_int64, vp]),
     "mi_modraise_batch": (cint, [vp, cint, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch_batch": (cint, [vp, cint, vp, vp, vp, cint, vp]),
-    "mi_rotate_hoisted_items": (cint, [vp, cint, cint, vp, ctypes.c_int64, vp, vp, vp, vp, vp, cint, vp]),
+    "mi_rotate_hoisted_items": (cint, [vp, cint, cint, vp, ctypes.c_int64, cint, vp, vp, vp, vp, vp, cint, vp]),
     "mi_conjugate_batch": (cint, [vp, cint, cint, vp, vp, vp, cint, vp]),
     "mi_stream_destroy": (cint, [vp, vp]),
     "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),

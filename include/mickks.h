@@ -264,21 +264,24 @@ int mi_keyswitch_hoisted(mi_context* ctx, int level, const uint64_t* digits, con
  * with b Add and a Assign and the map of galois[i], i.e.
  * (b_out[i], a_out[i]) = g_i(b + KS_i(a)) with digits = mi_hoist of a.
  * keys[i], b_out[i], a_out[i]: device pointers (host arrays); outputs must
- * not overlap digits or b. */
+ * be pairwise disjoint and overlap neither digits nor b. */
 int mi_rotate_hoisted_batch(mi_context* ctx, int level, const uint64_t* digits, const uint64_t* b, int count,
                             const uint64_t* galois, const uint64_t* const* keys, uint64_t* const* b_out,
                             uint64_t* const* a_out, int montgomery, void* stream);
 /* mi_rotate_hoisted_batch with its own hoist per rotation: rotation i from
- * the digits at digits + i * digits_stride words (0: one shared hoist; else
- * at least parts*(level+1+np)*N, e.g. mi_hoist_batch's output) with b[i]
- * added, (b_out[i], a_out[i]) = g_i(b[i] + KS_i(a_i)).  Ordered rotation-
- * major over the members of a batch (the mask-packed unpack's
- * unpack_segment_from_hoist, ckks_walsh_sbox.cpp:498-509, for every member
- * at one rotation), a launch's rotations share one key.  Outputs must not
- * overlap the digits. */
+ * the digits at digits + (i mod digits_period) * digits_stride words
+ * (digits_period 0: the count; stride 0: one shared hoist; else at least
+ * parts*(level+1+np)*N, e.g. mi_hoist_batch's output) with b[i] added,
+ * (b_out[i], a_out[i]) = g_i(b[i] + KS_i(a_i)).  Ordered rotation-major over
+ * the members of a batch (the mask-packed unpack's unpack_segment_from_hoist,
+ * ckks_walsh_sbox.cpp:498-509, for every member and segment: period = the
+ * members), a launch's rotations share one key; launches of independent
+ * chunks run on the library's side streams too.  Outputs must be pairwise
+ * disjoint and overlap neither the digits nor any b. */
 int mi_rotate_hoisted_items(mi_context* ctx, int level, int count, const uint64_t* digits, int64_t digits_stride,
-                            const uint64_t* const* b, const uint64_t* galois, const uint64_t* const* keys,
-                            uint64_t* const* b_out, uint64_t* const* a_out, int montgomery, void* stream);
+                            int digits_period, const uint64_t* const* b, const uint64_t* galois,
+                            const uint64_t* const* keys, uint64_t* const* b_out, uint64_t* const* a_out,
+                            int montgomery, void* stream);
 /* ckks_rotate_inplace (ckks_rotation.cpp:41-96) for `count` independent
  * ciphertexts at one level, as the mask-packed parity input rotates its 29
  * parity sums (add_rotated_segment_term_inplace, ckks_walsh_sbox.cpp:

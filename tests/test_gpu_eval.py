@@ -296,10 +296,13 @@ def test_device_samplers_match_philox_and_their_distributions():
         assert abs(m - 0.5) < 0.01
 
 
-def test_rotate_batch_matches_oracle():
+@pytest.mark.parametrize("streams", ["1", "0"])
+def test_rotate_batch_matches_oracle(streams, monkeypatch):
     """mi_rotate_batch: 35 independent ciphertexts (two chunks of key
-    switches) rotated in place, each equal to the oracle's rotation; batches
-    whose ciphertexts overlap are rejected."""
+    switches, on the library's side streams and on one stream) rotated in
+    place, each equal to the oracle's rotation; batches whose ciphertexts
+    overlap are rejected."""
+    monkeypatch.setenv("MICKKS_MULTI_STREAM", streams)
     N = 1 << 12
     M, ctx, octx, q, p, rng = _random_setup(N, 6, 2, 3, seed=21)
     level = 4

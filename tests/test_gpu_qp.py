@@ -429,7 +429,7 @@ def test_rotate_hoisted_items_per_member_digits(fuse, monkeypatch):
     outs = [(_zeros(M, L * N), _zeros(M, L * N)) for _ in range(B)]
     gal = (ctypes_u64(B))(*([g] * B))
     print("rotate_hoisted_items: 35 members, fuse", fuse, flush=True)
-    M.check(lib.mi_rotate_hoisted_items(ctx.h, lv, B, M.ptr(dig), dw, _parr([M.ptr(t) for t in b_in]), gal,
+    M.check(lib.mi_rotate_hoisted_items(ctx.h, lv, B, M.ptr(dig), dw, 0, _parr([M.ptr(t) for t in b_in]), gal,
                                         _parr([M.ptr(key)] * B), _parr([M.ptr(o[0]) for o in outs]),
                                         _parr([M.ptr(o[1]) for o in outs]), 1, s))
     for m in range(B):
@@ -444,3 +444,45 @@ def test_rotate_hoisted_items_per_member_digits(fuse, monkeypatch):
 def ctypes_u64(n):
     import ctypes
     return ctypes.c_uint64 * n
+
+
+@pytest.mark.parametrize("streams", ["1", "0"])
+def test_rotate_hoisted_items_rotation_major_over_members(streams, monkeypatch):
+    """mi_rotate_hoisted_items over 3 rotations x 35 members (105 items,
+    digit period 35: chunks of 32 and 3 inside each rotation's run of
+    members), the chunks on the library's side streams and on one stream;
+    every output equal to mi_keyswitch_hoisted of its member and rotation."""
+    import torch
+    monkeypatch.setenv("MICKKS_MULTI_STREAM", streams)
+    M, ctx, octx, q, p, rng = _setup(1 << 12, 9, 3, 3, seed=71)
+    lib = M.lib()
+    s = M._stream(torch.cuda.current_stream())
+    N, lv = 1 << 12, 6
+    L, T = lv + 1, lv + 1 + len(p)
+    dw = ctx.parts(lv) * T * N
+    B, R = 35, 3
+    a_in = [M.to_dev(rand_residues(rng, q[:L], N)) for _ in range(B)]
+    b_in = [M.to_dev(rand_residues(rng, q[:L], N)) for _ in range(B)]
+    dig = _zeros(M, B * dw)
+    M.check(lib.mi_hoist_batch(ctx.h, lv, B, _parr([M.ptr(t) for t in a_in]), M.ptr(dig), dw, s))
+    keys = [M.to_dev(_key(rng, q, p, N, octx.parts(len(q) - 1))) for _ in range(R)]
+    gals = [O.galois_rotation(N, r) for r in (3, 17, 40)]
+    outs = [(_zeros(M, L * N), _zeros(M, L * N)) for _ in range(R * B)]
+    gal = (ctypes_u64(R * B))(*[gals[i // B] for i in range(R * B)])
+    print("rotate_hoisted_items: 3 rotations x 35 members, streams", streams, flush=True)
+    M.check(lib.mi_rotate_hoisted_items(ctx.h, lv, R * B, M.ptr(dig), dw, B,
+                                        _parr([M.ptr(b_in[i % B]) for i in range(R * B)]), gal,
+                                        _parr([M.ptr(keys[i // B]) for i in range(R * B)]),
+                                        _parr([M.ptr(o[0]) for o in outs]), _parr([M.ptr(o[1]) for o in outs]), 1, s))
+    for i in range(R * B):
+        r, m = divmod(i, B)
+        one = _zeros(M, dw)
+        ctx.hoist(lv, a_in[m], one)
+        hb, ha = M.to_dev(M.from_dev(b_in[m])), _zeros(M, L * N)
+        ctx.keyswitch_hoisted(lv, one, keys[r], hb, ha, 0, 1, galois=gals[r])
+        assert np.array_equal(M.from_dev(outs[i][0]), M.from_dev(hb)), i
+        assert np.array_equal(M.from_dev(outs[i][1]), M.from_dev(ha)), i
+    # overlapping outputs are refused
+    assert lib.mi_rotate_hoisted_items(ctx.h, lv, 2, M.ptr(dig), dw, B, _parr([M.ptr(b_in[0])] * 2), gal,
+                                       _parr([M.ptr(keys[0])] * 2), _parr([M.ptr(outs[0][0])] * 2),
+                                       _parr([M.ptr(outs[0][1]), M.ptr(outs[1][1])]), 1, s) < 0
