@@ -807,35 +807,33 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         int level, b_mode, a_mode;
         std::vector<KsItem> v;
         std::vector<Span> writes, reads;
+        bool full = false;
     };
     std::vector<Group> open;
-    // Open groups are independent of each other: one group runs on the
-    // caller's stream, the others on side streams forked from it and joined
-    // back, so small levels share the GPU instead of queueing.
+    // Open groups (full ones included) are independent of each other: they
+    // run spread over the caller's stream and the side streams, forked from
+    // it and joined back, so groups overlap each other's fill and drain.
     auto flush_all = [&]() {
-        // a group that was just dispatched whole (KS_MAX_ITEMS items) is left empty
-        open.erase(std::remove_if(open.begin(), open.end(), [](const Group& g) { return g.v.empty(); }), open.end());
-        if (open.size() == 1) {
-            keyswitch_items(c, open[0].v.data(), (int)open[0].v.size(), montgomery, s);
-        } else if (open.size() > 1) {
+        std::vector<const Group*> gs;
+        for (const auto& g : open)
+            if (!g.v.empty()) gs.push_back(&g);
+        if (gs.size() == 1 || (!gs.empty() && !c.multi_stream)) {
+            for (const Group* g : gs) keyswitch_items(c, g->v.data(), (int)g->v.size(), montgomery, s);
+        } else if (gs.size() > 1) {
             const auto& side = c.side_streams();
+            const size_t ns = std::min(gs.size(), side.size() + 1);
             hipEvent_t fork;
             MI_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
             MI_HIP(hipEventRecord(fork, s));
-            std::vector<hipEvent_t> joins;
-            for (size_t g = 0; g < open.size(); ++g) {
-                hipStream_t st = g == 0 ? s : side[(g - 1) % side.size()];
-                if (st != s) MI_HIP(hipStreamWaitEvent(st, fork, 0));
-                keyswitch_items(c, open[g].v.data(), (int)open[g].v.size(), montgomery, st);
-                if (st != s && (g + side.size() >= open.size())) {
-                    // last group on this side stream: join it back
-                    hipEvent_t j;
-                    MI_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-                    MI_HIP(hipEventRecord(j, st));
-                    joins.push_back(j);
-                }
+            for (size_t k = 1; k < ns; ++k) MI_HIP(hipStreamWaitEvent(side[k - 1], fork, 0));
+            for (size_t g = 0; g < gs.size(); ++g) {
+                hipStream_t st = g % ns == 0 ? s : side[g % ns - 1];
+                keyswitch_items(c, gs[g]->v.data(), (int)gs[g]->v.size(), montgomery, st);
             }
-            for (hipEvent_t j : joins) {
+            for (size_t k = 1; k < ns; ++k) {
+                hipEvent_t j;
+                MI_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+                MI_HIP(hipEventRecord(j, side[k - 1]));
                 MI_HIP(hipStreamWaitEvent(s, j, 0));
                 MI_HIP(hipEventDestroy(j));
             }
@@ -859,9 +857,9 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         if (conflict) flush_all();
         Group* g = nullptr;
         for (auto& x : open)
-            if (x.level == it.level && x.b_mode == it.b_mode && x.a_mode == it.a_mode) g = &x;
+            if (!x.full && x.level == it.level && x.b_mode == it.b_mode && x.a_mode == it.a_mode) g = &x;
         if (!g) {
-            open.push_back(Group{it.level, it.b_mode, it.a_mode, {}, {}, {}});
+            open.push_back(Group{it.level, it.b_mode, it.a_mode, {}, {}, {}, false});
             g = &open.back();
         }
         g->v.push_back(it);
@@ -869,12 +867,7 @@ void keyswitch_batch(Context& c, const KsItem* items, int n, bool montgomery, hi
         g->writes.push_back(wa);
         g->reads.push_back(rd);
         if (it.init_b) g->reads.push_back(Span{it.init_b, it.init_b + W});
-        if ((int)g->v.size() == c.tune.ks_group) {
-            keyswitch_items(c, g->v.data(), (int)g->v.size(), montgomery, s);
-            g->v.clear();
-            g->writes.clear();
-            g->reads.clear();
-        }
+        if ((int)g->v.size() == c.tune.ks_group) g->full = true;  // runs at the next flush, with the others
     }
     flush_all();
 }
