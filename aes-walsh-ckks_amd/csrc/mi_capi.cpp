@@ -1064,6 +1064,53 @@ int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_
     });
 }
 
+int mi_walsh_product_sum_batch(mi_context* ctx, int level, int count, const uint64_t* const* low_b,
+                               const uint64_t* const* low_a, const uint64_t* const* high_b,
+                               const uint64_t* const* high_a, const int16_t* const* cr, const int16_t* const* ci,
+                               const uint64_t* const* const_abs, const int32_t* const* const_neg, const uint64_t* pt,
+                               int pt_montgomery, uint64_t* const* acc_b, uint64_t* const* acc_a,
+                               uint64_t* const* raw_b, uint64_t* const* raw_a, uint64_t* const* raw_c2, void* stream) {
+    return guard("walsh_sum_for_output_pair", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1 || !low_b || !low_a || !high_b || !high_a || !cr || !ci || !const_abs || !const_neg || !acc_b ||
+            !acc_a || !raw_b || !raw_a || !raw_c2)
+            throw mi::Error(MI_EINVAL, "null table");
+        check_ptr(pt);
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        std::vector<const uint64_t*> outs, ins;
+        for (int i = 0; i < count; ++i) {
+            if (!cr[i] || !ci[i] || !const_abs[i] || !const_neg[i]) throw mi::Error(MI_EINVAL, "null table");
+            for (const uint64_t* o : {acc_b[i], acc_a[i], raw_b[i], raw_a[i], raw_c2[i]}) {
+                check_ptr(o);
+                outs.push_back(o);
+            }
+            for (int m = 0; m < 15; ++m)
+                for (const uint64_t* x : {low_b[15 * i + m], low_a[15 * i + m], high_b[15 * i + m],
+                                          high_a[15 * i + m]}) {
+                    check_ptr(x);
+                    ins.push_back(x);
+                }
+        }
+        ins.push_back(pt);
+        // members run concurrently: outputs pairwise disjoint, none over an input
+        check_disjoint(outs, W, "outputs overlap");
+        std::sort(ins.begin(), ins.end());
+        ins.erase(std::unique(ins.begin(), ins.end()), ins.end());
+        for (const uint64_t* o : outs) {
+            auto it = std::upper_bound(ins.begin(), ins.end(), o);
+            if (it != ins.end() && *it < o + W) throw mi::Error(MI_EINVAL, "outputs must not alias the inputs");
+            if (it != ins.begin() && o < *(it - 1) + W) throw mi::Error(MI_EINVAL, "outputs must not alias the inputs");
+        }
+        mi::Context& c = *ctx->c;
+        mi::over_streams(c, S(stream), count, 0, [&](hipStream_t st, mi::u64*, int i) {
+            mi::walsh_product_sum(c, level, low_b + 15 * i, low_a + 15 * i, high_b + 15 * i, high_a + 15 * i,
+                                  (const short*)cr[i], (const short*)ci[i], const_abs[i], (const int*)const_neg[i],
+                                  pt, pt_montgomery != 0, acc_b[i], acc_a[i], raw_b[i], raw_a[i], raw_c2[i], st);
+        });
+    });
+}
+
 int mi_graph_begin(mi_context* ctx, void* stream) {
     return guard("mi_graph_begin", [&] {
         check_ctx(ctx);

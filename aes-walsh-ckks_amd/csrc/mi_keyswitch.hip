@@ -936,40 +936,6 @@ __global__ void __launch_bounds__(KS_THREADS) k_automorphism_items(AutoItemsArgs
     dst[i] = src[sidx];
 }
 
-// Independent chunks of a batch spread over the caller's stream and the
-// context's side streams (each with its own workspace of `ws_bytes`): the
-// side streams start after the work already on s, and s continues after all
-// of them.  A chunk's launches overlap the fill and drain of another's.
-// fn(stream, workspace, chunk index).
-template <class F>
-void over_streams(Context& c, hipStream_t s, int nchunks, size_t ws_bytes, F&& fn) {
-    if (nchunks <= 1 || !c.multi_stream) {
-        u64* ws = (u64*)c.workspace(s, ws_bytes);
-        for (int i = 0; i < nchunks; ++i) fn(s, ws, i);
-        return;
-    }
-    const auto& side = c.side_streams();
-    const int ns = std::min(nchunks, 1 + (int)side.size());
-    std::vector<hipStream_t> st(ns);
-    std::vector<u64*> ws(ns);
-    st[0] = s;
-    for (int k = 1; k < ns; ++k) st[k] = side[k - 1];
-    for (int k = 0; k < ns; ++k) ws[k] = (u64*)c.workspace(st[k], ws_bytes);
-    hipEvent_t fork;
-    MI_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    MI_HIP(hipEventRecord(fork, s));
-    for (int k = 1; k < ns; ++k) MI_HIP(hipStreamWaitEvent(st[k], fork, 0));
-    for (int i = 0; i < nchunks; ++i) fn(st[i % ns], ws[i % ns], i);
-    for (int k = 1; k < ns; ++k) {
-        hipEvent_t j;
-        MI_HIP(hipEventCreateWithFlags(&j, hipEventDisableTiming));
-        MI_HIP(hipEventRecord(j, st[k]));
-        MI_HIP(hipStreamWaitEvent(s, j, 0));
-        MI_HIP(hipEventDestroy(j));
-    }
-    MI_HIP(hipEventDestroy(fork));
-}
-
 void rotate_hoisted_items(Context& c, int level, const u64* digits, long long dstride, int period,
                           const u64* const* b, int n, const u64* galois, const u64* const* keys, u64* const* b_out,
                           u64* const* a_out, bool montgomery, hipStream_t s) {

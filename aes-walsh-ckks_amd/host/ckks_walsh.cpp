@@ -165,11 +165,19 @@ namespace {
 
 // The device part of walsh_sum_for_output_pair: acc (2 polys, the h = 0 term)
 // and raw (3 polys, the raw products' sum) of one output pair.
-std::pair<CKKSCiphertext, CKKSCiphertext> walsh_product_sum(const CKKSContext& context,
-                                                            const MyVector<CKKSCiphertext>& low_signs,
-                                                            const MyVector<CKKSCiphertext>& high_signs,
-                                                            size_t real_out_bit, size_t imag_out_bit,
-                                                            const CKKSEncoding& encoded_i) {
+// One output pair's product sum, prepared: its tables and its (uninitialised)
+// acc and raw ciphertexts; walsh_product_sum_launch runs it (one pair, or the
+// four pairs of an S-box as one batch).
+struct WalshPairPlan {
+    int16_t cr[kMasks][kMasks], ci[kMasks][kMasks];
+    uint64_t vabs[4 * kMasks];
+    int32_t vneg[2 * kMasks];
+    std::optional<CKKSCiphertext> acc, raw;
+};
+
+WalshPairPlan walsh_product_plan(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
+                                 const MyVector<CKKSCiphertext>& high_signs, size_t real_out_bit, size_t imag_out_bit,
+                                 const CKKSEncoding& encoded_i) {
     if (low_signs.size() != kMasks || high_signs.size() != kMasks)
         throw std::invalid_argument("ckks_walsh_sbox: expected 16 low and 16 high sign ciphertexts");
     if (real_out_bit >= 8 || imag_out_bit >= 8) throw std::invalid_argument("ckks_walsh_sbox: invalid output bit");
@@ -202,38 +210,69 @@ std::pair<CKKSCiphertext, CKKSCiphertext> walsh_product_sum(const CKKSContext& c
         throw std::invalid_argument("ckks_walsh_sbox: encoded i level mismatch");
 
     const auto& table = ckks_aes_walsh_table();
-    int16_t cr[kMasks][kMasks], ci[kMasks][kMasks];
-    uint64_t vabs[4 * kMasks];
-    int32_t vneg[2 * kMasks];
+    WalshPairPlan plan;
     for (size_t h = 0; h < kMasks; ++h) {
         for (size_t m = 0; m < kMasks; ++m) {
-            cr[h][m] = table[real_out_bit][m][h];
-            ci[h][m] = table[imag_out_bit][m][h];
+            plan.cr[h][m] = table[real_out_bit][m][h];
+            plan.ci[h][m] = table[imag_out_bit][m][h];
         }
-        rounded_constant(cr[h][0], S, &vabs[4 * h], &vneg[2 * h]);
-        rounded_constant(ci[h][0], S, &vabs[4 * h + 2], &vneg[2 * h + 1]);
+        rounded_constant(plan.cr[h][0], S, &plan.vabs[4 * h], &plan.vneg[2 * h]);
+        rounded_constant(plan.ci[h][0], S, &plan.vabs[4 * h + 2], &plan.vneg[2 * h + 1]);
     }
     const size_t B = ref.getBatchSize();
     for (size_t m = 1; m < kMasks; ++m)
         if (low_signs[m].getBatchSize() != B || high_signs[m].getBatchSize() != B)
             throw std::invalid_argument("ckks_walsh_sbox: sign ciphertexts differ in batch size");
-    CKKSCiphertext acc(context, 2, S, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
-    CKKSCiphertext raw(context, 3, P, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
-    for (size_t j = 0; j < B; ++j) {
-        const uint64_t *lb[15], *la[15], *hb[15], *ha[15];
-        for (size_t m = 1; m < kMasks; ++m) {
-            lb[m - 1] = low_signs[m].polyData(0, j);
-            la[m - 1] = low_signs[m].polyData(1, j);
-            hb[m - 1] = high_signs[m].polyData(0, j);
-            ha[m - 1] = high_signs[m].polyData(1, j);
+    plan.acc.emplace(context, 2, S, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
+    plan.raw.emplace(context, 3, P, level, ref.getSecretOwner(), ref.getMessageEncodingState(), B);
+    return plan;
+}
+
+// The product sums of the planned pairs, every member of every pair as one
+// mi_walsh_product_sum_batch call.
+void walsh_product_launch(const CKKSContext& context, const MyVector<CKKSCiphertext>& low_signs,
+                          const MyVector<CKKSCiphertext>& high_signs, const CKKSEncoding& encoded_i,
+                          std::vector<WalshPairPlan>& plans) {
+    const CKKSCiphertext& ref = low_signs.front();
+    const size_t B = ref.getBatchSize();
+    std::vector<const uint64_t*> lb, la, hb, ha, vabs;
+    std::vector<const int16_t*> cr, ci;
+    std::vector<const int32_t*> vneg;
+    std::vector<uint64_t*> acc_b, acc_a, raw_b, raw_a, raw_c2;
+    for (auto& plan : plans)
+        for (size_t j = 0; j < B; ++j) {
+            for (size_t m = 1; m < kMasks; ++m) {
+                lb.push_back(low_signs[m].polyData(0, j));
+                la.push_back(low_signs[m].polyData(1, j));
+                hb.push_back(high_signs[m].polyData(0, j));
+                ha.push_back(high_signs[m].polyData(1, j));
+            }
+            cr.push_back(&plan.cr[0][0]);
+            ci.push_back(&plan.ci[0][0]);
+            vabs.push_back(plan.vabs);
+            vneg.push_back(plan.vneg);
+            acc_b.push_back(plan.acc->polyData(0, j));
+            acc_a.push_back(plan.acc->polyData(1, j));
+            raw_b.push_back(plan.raw->polyData(0, j));
+            raw_a.push_back(plan.raw->polyData(1, j));
+            raw_c2.push_back(plan.raw->polyData(2, j));
         }
-        mi_check(mi_walsh_product_sum(context.handle(), (int)level, lb, la, hb, ha, &cr[0][0], &ci[0][0], vabs, vneg,
-                                      encoded_i.data(), encoded_i.isMontgomeryForm() ? 1 : 0, acc.polyData(0, j),
-                                      acc.polyData(1, j), raw.polyData(0, j), raw.polyData(1, j), raw.polyData(2, j),
-                                      context.stream()),
-                 "ckks_walsh_sbox");
-    }
-    return {std::move(acc), std::move(raw)};
+    mi_check(mi_walsh_product_sum_batch(context.handle(), (int)ref.getLevel(), (int)acc_b.size(), lb.data(), la.data(),
+                                        hb.data(), ha.data(), cr.data(), ci.data(), vabs.data(), vneg.data(),
+                                        encoded_i.data(), encoded_i.isMontgomeryForm() ? 1 : 0, acc_b.data(),
+                                        acc_a.data(), raw_b.data(), raw_a.data(), raw_c2.data(), context.stream()),
+             "ckks_walsh_sbox");
+}
+
+std::pair<CKKSCiphertext, CKKSCiphertext> walsh_product_sum(const CKKSContext& context,
+                                                            const MyVector<CKKSCiphertext>& low_signs,
+                                                            const MyVector<CKKSCiphertext>& high_signs,
+                                                            size_t real_out_bit, size_t imag_out_bit,
+                                                            const CKKSEncoding& encoded_i) {
+    std::vector<WalshPairPlan> plans;
+    plans.push_back(walsh_product_plan(context, low_signs, high_signs, real_out_bit, imag_out_bit, encoded_i));
+    walsh_product_launch(context, low_signs, high_signs, encoded_i, plans);
+    return {std::move(*plans[0].acc), std::move(*plans[0].raw)};
 }
 
 // add_term_inplace(acc, relinearised raw) (:288-295)
@@ -281,10 +320,14 @@ MyVector<CKKSCiphertext> ckks_walsh_recombine_signs_to_paired_outputs(const CKKS
     const CKKSEncoding& encoded_i = constant_encoding(context, Complex(0.0, 1.0), 1.0L, low_signs[1].getLevel(), own);
     // the four pairs are independent: their product sums first, then their
     // relinearisations as one key-switch batch (one launch per phase)
+    std::vector<WalshPairPlan> plans;
+    plans.reserve(4);
+    for (size_t pair = 0; pair < 4; ++pair)
+        plans.push_back(walsh_product_plan(context, low_signs, high_signs, pair, pair + 4, encoded_i));
+    walsh_product_launch(context, low_signs, high_signs, encoded_i, plans);
     std::vector<std::pair<CKKSCiphertext, CKKSCiphertext>> sums;
     sums.reserve(4);
-    for (size_t pair = 0; pair < 4; ++pair)
-        sums.push_back(walsh_product_sum(context, low_signs, high_signs, pair, pair + 4, encoded_i));
+    for (auto& plan : plans) sums.emplace_back(std::move(*plan.acc), std::move(*plan.raw));
     if (low_signs.front().getSecretOwner() != CKKSSecretOwner::Dense)
         throw std::invalid_argument("ckks_relin: sparse-secret relinearization keys are not available");
     std::vector<mi_keyswitch_item> items;

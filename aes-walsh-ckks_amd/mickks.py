@@ -46,6 +46,7 @@ _SIGS = {
     "mi_modraise_batch": (cint, [vp, cint, cint, vp, vp, vp]),
     "mi_q0p0_keyswitch_batch": (cint, [vp, cint, vp, vp, vp, cint, vp]),
     "mi_rotate_hoisted_items": (cint, [vp, cint, cint, vp, ctypes.c_int64, cint, vp, vp, vp, vp, vp, cint, vp]),
+    "mi_walsh_product_sum_batch": (cint, [vp, cint, cint] + [vp] * 9 + [cint] + [vp] * 5 + [vp]),
     "mi_conjugate_batch": (cint, [vp, cint, cint, vp, vp, vp, cint, vp]),
     "mi_stream_destroy": (cint, [vp, vp]),
     "mi_poly_op": (cint, [vp, cint, vp, vp, cint, cint, cint, i64, vp]),

@@ -508,6 +508,19 @@ int mi_walsh_product_sum(mi_context* ctx, int level, const uint64_t* const* low_
                          const int16_t* ci, const uint64_t* const_abs, const int32_t* const_neg, const uint64_t* pt,
                          int pt_montgomery, uint64_t* acc_b, uint64_t* acc_a, uint64_t* raw_b, uint64_t* raw_a,
                          uint64_t* raw_c2, void* stream);
+/* mi_walsh_product_sum for `count` (member, output pair) items of a batch:
+ * item i's 15 + 15 sign pointers at low_b[15 i ...] etc., its tables cr[i],
+ * ci[i], const_abs[i], const_neg[i], its outputs acc_b[i] ...; one encoded
+ * i plaintext for all.  The four output pairs of every member of an S-box
+ * batch (the loop over output pairs, ckks_walsh_sbox.cpp:1137-1152)
+ * in one call, the launches spread over the library's streams.  Outputs
+ * pairwise disjoint and overlapping no input. */
+int mi_walsh_product_sum_batch(mi_context* ctx, int level, int count, const uint64_t* const* low_b,
+                               const uint64_t* const* low_a, const uint64_t* const* high_b,
+                               const uint64_t* const* high_a, const int16_t* const* cr, const int16_t* const* ci,
+                               const uint64_t* const* const_abs, const int32_t* const* const_neg, const uint64_t* pt,
+                               int pt_montgomery, uint64_t* const* acc_b, uint64_t* const* acc_a,
+                               uint64_t* const* raw_b, uint64_t* const* raw_a, uint64_t* const* raw_c2, void* stream);
 
 /* ---- HIP graph capture ----
  * No counterpart in the reference, whose evaluator runs eagerly on host
