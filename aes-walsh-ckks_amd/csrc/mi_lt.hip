@@ -287,7 +287,10 @@ void linear_transform_qp_batch(Context& c, const LtStage& st, int n, const u64* 
         // (hoisted QP rotations, :675-852), b added in the Q limbs
         if (nbaby > 0) {
             ks_modup_n(c, level, m, a + i0, D, dw, C, s);
-            for (int sl = 1; sl < k.nslots; ++sl) {
+            // the baby steps' key products are independent (each writes its
+            // own baby slots from the shared digits): spread over the streams
+            over_streams(c, s, k.nslots - 1, 0, [&](hipStream_t bst, u64*, int i) {
+                const int sl = i + 1;
                 MacArgs ma = mac_args(c, level, D, k.baby_key[sl], st.montgomery != 0);
                 ma.items = m;
                 ma.digits_stride = dw;
@@ -299,8 +302,8 @@ void linear_transform_qp_batch(Context& c, const LtStage& st, int n, const u64* 
                     ma.b_out[it] = baby(it, sl) + W;
                     ma.init_b[it] = b[i0 + it];
                 }
-                ks_mac(c, ma, level, s);
-            }
+                ks_mac(c, ma, level, bst);
+            });
         }
 
         launch_inner(c, k, st, m, K, a + i0, b + i0, BAB, 2LL * nbaby * W, 2 * W, OAB, 2 * W, IAB, 2 * W,
@@ -389,6 +392,14 @@ void linear_transform_q_batch(Context& c, const LtStage& st, int n, const u64* c
         // starting from b_init), modes Assign: the hoisted or full switch
         // of ckks_hybrid_key_switch_(hoisted_)inplace
         auto mac_moddown = [&](const u64* key, const u64* const* b_init) {
+            if (parts <= 3 && c.fuse_mac) {
+                // the key product folded into ModDown's NTT passes
+                KsItem items[KS_MAX_ITEMS];
+                for (int it = 0; it < m; ++it)
+                    items[it] = KsItem{nullptr, key, SW + it * 2 * LN + LN, SW + it * 2 * LN, level, 1, 1, b_init[it]};
+                ks_moddown_mac(c, level, m, items, D, dw, AP, Tm, st.montgomery != 0, s);
+                return;
+            }
             MacArgs ma = mac_args(c, level, D, key, st.montgomery != 0);
             ma.items = m;
             ma.digits_stride = dw;
