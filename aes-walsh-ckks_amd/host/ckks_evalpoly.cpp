@@ -158,53 +158,6 @@ void add_scaled_constant(const CKKSContext& c, CKKSCiphertext& ct, double consta
     ckks_detail::b_list_op(c, MI_LIST_ADD_SCALAR, ct, r.data(), "ckks_evalpoly");
 }
 
-void mul_integer(const CKKSContext& c, CKKSCiphertext& ct, ld scalar, ld out_scale) {
-    if (!std::isfinite(scalar) || scalar < 0.0L) throw std::invalid_argument("ckks_evalchebyshev: invalid scalar scale-up");
-    const ld r = std::round(scalar);
-    if (r < 1.0L || std::abs(scalar - r) > std::max<ld>(1.0L, std::abs(scalar)) * 0x1p-20L)
-        throw std::invalid_argument("ckks_evalchebyshev: non-integral scalar scale-up");
-    if (r > (ld)std::numeric_limits<uint64_t>::max())
-        throw std::invalid_argument("ckks_evalchebyshev: scalar scale-up is too large");
-    const uint64_t s = (uint64_t)r;
-    if (s != 1) {
-        const auto& q = c.getParams().getModuli();
-        const size_t lv = ct.getLevel();
-        std::vector<uint64_t> res(lv + 1);
-        for (size_t l = 0; l <= lv; ++l) res[l] = s % q[l];
-        ckks_detail::poly_list_op(c, MI_LIST_MUL_SCALAR, ct, nullptr, res.data(), "ckks_evalpoly");
-    }
-    ct.setScale((double)out_scale);
-}
-
-// out -= s * src for the integer s = round(scalar) (mul_integer's checks),
-// src read at out's level over its polys
-void sub_integer_multiple(const CKKSContext& c, CKKSCiphertext& out, const CKKSCiphertext& src, ld scalar) {
-    if (!std::isfinite(scalar) || scalar < 0.0L) throw std::invalid_argument("ckks_evalchebyshev: invalid scalar scale-up");
-    const ld r = std::round(scalar);
-    if (r < 1.0L || std::abs(scalar - r) > std::max<ld>(1.0L, std::abs(scalar)) * 0x1p-20L)
-        throw std::invalid_argument("ckks_evalchebyshev: non-integral scalar scale-up");
-    if (r > (ld)std::numeric_limits<uint64_t>::max())
-        throw std::invalid_argument("ckks_evalchebyshev: scalar scale-up is too large");
-    const size_t level = out.getLevel();
-    if (src.getLevel() < level) throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
-    if (src.getNumPolys() > out.getNumPolys() || src.getBatchSize() != out.getBatchSize())
-        throw std::invalid_argument("ckks_evalchebyshev: cannot shrink ciphertext degree by padding");
-    const uint64_t s = (uint64_t)r;
-    const auto& q = c.getParams().getModuli();
-    std::vector<uint64_t> neg(level + 1);
-    for (size_t l = 0; l <= level; ++l) neg[l] = (q[l] - s % q[l]) % q[l];
-    std::vector<uint64_t*> x;
-    std::vector<const uint64_t*> y;
-    for (size_t k = 0; k < src.getNumPolys(); ++k)
-        for (size_t m = 0; m < src.getBatchSize(); ++m) {
-            x.push_back(out.polyData(k, m));
-            y.push_back(src.polyData(k, m));
-        }
-    mi_check(mi_poly_list_op(c.handle(), MI_LIST_MUL_SCALAR_ADD, (int)level, (int)x.size(), x.data(), y.data(),
-                             neg.data(), c.stream()),
-             "ckks_evalpoly");
-}
-
 size_t levels_per_rescale(const CKKSContext& c) { return std::log2(c.getParams().getScale()) <= 64.0 ? 1 : 2; }
 
 ld rescale_divisor(const CKKSContext& c, size_t level) {
@@ -459,20 +412,47 @@ private:
         }
         ckks_relin_hybrid_inplace(c_, out);
         out.setScale(prod.value_d(q_));
-        mul_integer(c_, out, 2.0L, prod.value(q_));
+        // T_n = 2 (T_a T_b) - T_d (d = |a - b| > 0) or 2 T_a^2 - 1 (d = 0): the
+        // doubling (mul_integer by 2) and the subtraction of ratio * T_d (T_d
+        // read at out's level) or the constant -1 at the product's scale, as
+        // one scaled sum into a new ciphertext
         const size_t d = a > b ? a - b : b - a;
-        if (d == 0) {
-            add_scaled_constant(c_, out, -1.0, prod.value(q_));
-        } else {
-            gen(d);
-            // out -= ratio * T_d (T_d read at out's level): one fused
-            // multiply-add by -ratio mod q instead of a scaled copy and a subtraction
-            const uint64_t ratio = scale_ratio_floor(prod, sc_.at(d), q_);
-            sub_integer_multiple(c_, out, ct_.at(d), (ld)ratio);
-        }
-        ct_.insert_or_assign(n, std::move(out));
+        if (d > 0) gen(d);
+        ct_.insert_or_assign(n, doubled_minus(out, d, prod));
         sc_.insert_or_assign(n, prod);
         return true;
+    }
+
+    CKKSCiphertext doubled_minus(const CKKSCiphertext& out, size_t d, const ScaleTerm& prod) {
+        const size_t L = out.getLevel() + 1, P = out.getNumPolys(), B = out.getBatchSize();
+        CKKSCiphertext res(c_, P, prod.value_d(q_), out.getLevel(), out.getSecretOwner(), out.getMessageEncodingState(),
+                           B);
+        std::vector<uint64_t*> dst;
+        for (size_t k = 0; k < P; ++k)
+            for (size_t m = 0; m < B; ++m) dst.push_back(res.polyData(k, m));
+        const size_t terms = d > 0 ? 2 : 1;
+        std::vector<const uint64_t*> in(terms * P * B, nullptr);
+        std::vector<uint64_t> r(terms * L), cst(L, 0);
+        for (size_t k = 0; k < P; ++k)
+            for (size_t m = 0; m < B; ++m) in[k * B + m] = out.polyData(k, m);
+        for (size_t l = 0; l < L; ++l) r[l] = 2 % q_[l];
+        if (d > 0) {
+            const CKKSCiphertext& td = ct_.at(d);
+            if (td.getLevel() < out.getLevel())
+                throw std::invalid_argument("ckks_drop_to_level_inplace: cannot raise ciphertext level");
+            if (td.getNumPolys() > P || td.getBatchSize() != B)
+                throw std::invalid_argument("ckks_evalchebyshev: cannot shrink ciphertext degree by padding");
+            const uint64_t ratio = scale_ratio_floor(prod, sc_.at(d), q_);
+            for (size_t k = 0; k < td.getNumPolys(); ++k)
+                for (size_t m = 0; m < B; ++m) in[P * B + k * B + m] = td.polyData(k, m);
+            for (size_t l = 0; l < L; ++l) r[L + l] = (q_[l] - ratio % q_[l]) % q_[l];
+        } else {
+            for (size_t l = 0; l < L; ++l) cst[l] = scaled_constant_mod_q(-1.0, prod.value(q_), q_[l]);
+        }
+        mi_check(mi_scaled_sum(c_.handle(), (int)out.getLevel(), (int)(P * B), dst.data(), (int)terms, in.data(),
+                               r.data(), d > 0 ? 0 : (int)B, cst.data(), c_.stream()),
+                 "ckks_evalpoly");
+        return res;
     }
 
     const CKKSContext& c_;
