@@ -116,6 +116,8 @@ int mi_probe_begin(mi_context* ctx) {
         check_ctx(ctx);
         mi::Context& c = *ctx->c;
         std::lock_guard<std::mutex> g(c.probe_mu);
+        if (!c.probe.on) c.probe.saved_multi_stream = c.multi_stream;
+        c.multi_stream = false;
         c.probe.on = true;
         c.probe.used = 0;
         c.probe.bytes = 0.0;
@@ -129,6 +131,7 @@ int mi_probe_end(mi_context* ctx, double* stats) {
         check_ptr(stats);
         mi::Context& c = *ctx->c;
         std::lock_guard<std::mutex> g(c.probe_mu);
+        if (c.probe.on) c.multi_stream = c.probe.saved_multi_stream;
         c.probe.on = false;
         double ms = 0.0;
         for (size_t i = 0; i + 1 < c.probe.used; i += 2) {

@@ -125,9 +125,12 @@ struct Context {
 
     // NTT probe (mi_probe_begin / mi_probe_end): while on, every forward or
     // inverse NTT launch pair is bracketed by events on its stream and its
-    // algorithmic bytes (16 N per limb) are counted.
+    // algorithmic bytes (16 N per limb) are counted.  Work that would spread
+    // over the side streams runs on the caller's stream meanwhile, so each
+    // timed pair has the device to itself; mi_probe_end restores the setting.
     struct NttProbe {
         bool on = false;
+        bool saved_multi_stream = true;
         std::vector<hipEvent_t> events;  // start/stop pairs, reused across probes
         size_t used = 0;
         double bytes = 0.0;

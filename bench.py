@@ -792,7 +792,9 @@ def bench_aes(args, M, torch):
     ms = 1e3 * sum(lat) / len(lat)
     mism, max_err, avg_err = aes_validate(H, ctx, out, plains, cb)
     # the roofline kernel inside the evaluation itself: one more evaluation
-    # with HIP events around every NTT launch pair on its stream
+    # with HIP events around every NTT launch pair on its stream (the probe
+    # keeps the side-stream work on the caller's stream meanwhile, so each
+    # pair's events time that pair alone, not pairs overlapping other work)
     ctx.nttProbeBegin()
     t1 = time.perf_counter()
     evaluate()
@@ -936,9 +938,10 @@ def bench_ntt_qp(args, M, torch):
 
 def aes_ntt_roofline(M, torch, P, probe):
     """The NTT passes are the largest kernel family of the AES evaluation
-    (about 40 % of its kernel time; the rest is spread over key-switch inner
-    products, element-wise ops, encoding FFT stages and copies).  `probe` is
-    what the context's NTT probe measured over one whole evaluation: HIP
+    (about half of its kernel time; the rest is spread over the Walsh product
+    sums, key-switch inner products, element-wise ops and copies).  `probe`
+    is what the context's NTT probe measured over one whole evaluation, run
+    with the side streams off so that no pair overlaps other work: HIP
     events on each launch pair's own stream around every forward and inverse
     NTT (column + row pass; standalone, inside key switches, rescales,
     ModRaise and the linear transforms), 16 N bytes per limb algorithmically

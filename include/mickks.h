@@ -86,7 +86,10 @@ int mi_tune_set(mi_context* ctx, const int* ntt_variant, int keyswitch_group);
 /* NTT probe (measurement only; no counterpart in the reference): between
  * mi_probe_begin and mi_probe_end every forward or inverse NTT launch pair
  * of the context (standalone, or inside key switches, rescales, ModRaise and
- * linear transforms) is bracketed by HIP events on its stream.  mi_probe_end
+ * linear transforms) is bracketed by HIP events on its stream, and work that
+ * would spread over the side streams stays on the caller's stream, so each
+ * timed pair has the device to itself (mi_probe_end restores the spreading;
+ * outputs are the same either way).  mi_probe_end
  * waits for them and returns stats[0] = summed event time of the pairs (ms),
  * stats[1] = launch pairs, stats[2] = algorithmic bytes (16 N per limb: each
  * limb read and written once), stats[3] = limbs transformed. */
