@@ -804,6 +804,10 @@ void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double tar
 // or above): the result equals the product with a copy of ct2 dropped to
 // ct1's level, without the copy.
 void mul_at_level(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+// ckks_apply_linear_transform_plan reading `in` in place (its stages never
+// write their input, so callers that keep `in` need no copy of it)
+CKKSCiphertext apply_linear_transform_plan_from(const CKKSContext& c, const CKKSCiphertext& in,
+                                                const CKKSLinearTransformPlan& plan);
 // ckks_batch_bootstrap_evalmod_from_complex_packed with its two output lanes
 // stacked into one ciphertext of twice the members (out0's members, then
 // out1's), the EvalMods of both lanes run as that one batch.

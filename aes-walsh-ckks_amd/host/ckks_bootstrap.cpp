@@ -178,7 +178,7 @@ CKKSCiphertext ckks_s2c_generalized(const CKKSContext& c, const CKKSCiphertext& 
     if (lv != depth)
         throw std::runtime_error("ckks_s2c_generalized: cached EvalMod S2C DFT plan requires input level " +
                                  std::to_string(depth) + ", got level " + std::to_string(lv));
-    return ckks_apply_linear_transform_plan(c, ct.clone(), c.getEvalModS2CPlan());
+    return ckks_detail::apply_linear_transform_plan_from(c, ct, c.getEvalModS2CPlan());
 }
 
 CKKSCiphertext ckks_c2s_generalized(const CKKSContext& c, const CKKSCiphertext& ct, double transform_scale) {
@@ -203,7 +203,7 @@ CKKSCiphertext ckks_c2s_generalized(const CKKSContext& c, const CKKSCiphertext& 
     if (std::abs(transform_scale - cached) > 1e-12 * std::max({1.0, std::abs(transform_scale), std::abs(cached)}))
         throw std::runtime_error("ckks_c2s_generalized: cached EvalMod C2S DFT plan requires transform scale " +
                                  std::to_string(cached) + ", got " + std::to_string(transform_scale));
-    return ckks_apply_linear_transform_plan(c, ct.clone(), c.getEvalModScaledC2SPlan());
+    return ckks_detail::apply_linear_transform_plan_from(c, ct, c.getEvalModScaledC2SPlan());
 }
 
 void ckks_evalmod_binary_inplace(const CKKSContext& c, CKKSCiphertext& ct) {

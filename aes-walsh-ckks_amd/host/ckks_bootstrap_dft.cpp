@@ -649,15 +649,23 @@ CKKSCiphertext ckks_apply_linear_transform_owned_stage_q(const CKKSContext& c, c
                          st.baby_step_count, st.rescale_after, st.output_encoding_state);
 }
 
-CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& c, CKKSCiphertext ct,
-                                                const CKKSLinearTransformPlan& plan) {
+CKKSCiphertext ckks_detail::apply_linear_transform_plan_from(const CKKSContext& c, const CKKSCiphertext& in,
+                                                            const CKKSLinearTransformPlan& plan) {
     if (plan.stages.empty()) throw std::invalid_argument("ckks_linear_transform: empty transform plan");
-    for (const auto& st : plan.stages) {
+    // every stage reads its input and writes a new ciphertext, so the first
+    // one reads the caller's ciphertext in place (no copy of it)
+    const auto stage = [&](const CKKSCiphertext& x, const CKKSLinearTransformOwnedStage& st) {
         // should_use_qp_linear_transform (ckks_linear_transform.cpp:67-74)
         if (!st.diagonal_qp_plaintexts.empty() && plan.use_qp_evaluator)
-            ct = ckks_apply_linear_transform_owned_stage_qp(c, ct, st);
-        else
-            ct = ckks_apply_linear_transform_owned_stage_q(c, ct, st);
-    }
+            return ckks_apply_linear_transform_owned_stage_qp(c, x, st);
+        return ckks_apply_linear_transform_owned_stage_q(c, x, st);
+    };
+    CKKSCiphertext ct = stage(in, plan.stages.front());
+    for (size_t i = 1; i < plan.stages.size(); ++i) ct = stage(ct, plan.stages[i]);
     return ct;
+}
+
+CKKSCiphertext ckks_apply_linear_transform_plan(const CKKSContext& c, CKKSCiphertext ct,
+                                                const CKKSLinearTransformPlan& plan) {
+    return ckks_detail::apply_linear_transform_plan_from(c, ct, plan);
 }

@@ -451,7 +451,8 @@ def test_rotate_hoisted_items_rotation_major_over_members(streams, monkeypatch):
     """mi_rotate_hoisted_items over 3 rotations x 35 members (105 items,
     digit period 35: chunks of 32 and 3 inside each rotation's run of
     members), the chunks on the library's side streams and on one stream;
-    every output equal to mi_keyswitch_hoisted of its member and rotation."""
+    every output equal to mi_keyswitch_hoisted of its member and rotation,
+    and again under the NTT probe (which keeps every chunk on one stream)."""
     import torch
     monkeypatch.setenv("MICKKS_MULTI_STREAM", streams)
     M, ctx, octx, q, p, rng = _setup(1 << 12, 9, 3, 3, seed=71)
@@ -482,6 +483,22 @@ def test_rotate_hoisted_items_rotation_major_over_members(streams, monkeypatch):
         ctx.keyswitch_hoisted(lv, one, keys[r], hb, ha, 0, 1, galois=gals[r])
         assert np.array_equal(M.from_dev(outs[i][0]), M.from_dev(hb)), i
         assert np.array_equal(M.from_dev(outs[i][1]), M.from_dev(ha)), i
+    # the NTT probe keeps the side-stream chunks on the caller's stream while
+    # it is on: the same outputs, and every pair counted at 16 N bytes per limb
+    import ctypes
+    M.check(lib.mi_probe_begin(ctx.h))
+    again = [(_zeros(M, L * N), _zeros(M, L * N)) for _ in range(R * B)]
+    M.check(lib.mi_rotate_hoisted_items(ctx.h, lv, R * B, M.ptr(dig), dw, B,
+                                        _parr([M.ptr(b_in[i % B]) for i in range(R * B)]), gal,
+                                        _parr([M.ptr(keys[i // B]) for i in range(R * B)]),
+                                        _parr([M.ptr(o[0]) for o in again]), _parr([M.ptr(o[1]) for o in again]), 1, s))
+    stats = (ctypes.c_double * 4)()
+    M.check(lib.mi_probe_end(ctx.h, stats))
+    assert stats[0] > 0 and stats[1] >= 1 and stats[3] >= stats[1]
+    assert stats[2] == 16.0 * N * stats[3]
+    for i in range(R * B):
+        assert np.array_equal(M.from_dev(again[i][0]), M.from_dev(outs[i][0])), i
+        assert np.array_equal(M.from_dev(again[i][1]), M.from_dev(outs[i][1])), i
     # overlapping outputs are refused
     assert lib.mi_rotate_hoisted_items(ctx.h, lv, 2, M.ptr(dig), dw, B, _parr([M.ptr(b_in[0])] * 2), gal,
                                        _parr([M.ptr(keys[0])] * 2), _parr([M.ptr(outs[0][0])] * 2),
