@@ -411,3 +411,66 @@ def test_walsh_sbox_end_to_end_on_a_bootstrapping_chain():
         worst = max(worst, err)
         print(f"bit {k}: level {ct.getLevel()} max error {err:.3e}", flush=True)
     assert worst < 1e-4, worst
+
+
+@pytest.mark.parametrize("streams", ["1", "0"])
+def test_walsh_product_sum_batch_matches_per_item_calls(streams, monkeypatch):
+    """mi_walsh_product_sum_batch over 3 members x 3 output pairs (9 items,
+    each with its own coefficient tables and constants; the items of a member
+    share its sign ciphertexts), on the library's side streams and on one
+    stream: every output equal, bit for bit, to mi_walsh_product_sum of its
+    item alone; overlapping outputs are refused."""
+    import ctypes
+    import torch
+    monkeypatch.setenv("MICKKS_MULTI_STREAM", streams)
+    M = product()
+    N, level = 1 << 11, 2
+    p, q = O.find_primes([51], [50] * 3, N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(211)
+    qs = q[: level + 1]
+    W = (level + 1) * N
+    rnd = lambda: [M.to_dev(rand_residues(rng, qs, N)), M.to_dev(rand_residues(rng, qs, N))]
+    members, pairs = 3, 3
+    signs = [([rnd() for _ in range(15)], [rnd() for _ in range(15)]) for _ in range(members)]
+    pt = M.to_dev(rand_residues(rng, qs, N))
+    items = []
+    for i in range(members * pairs):
+        cr = np.ascontiguousarray(rng.integers(-64, 65, (16, 16)).astype(np.int16))
+        ci = np.ascontiguousarray(rng.integers(-64, 65, (16, 16)).astype(np.int16))
+        consts = [(int(rng.integers(-2**62, 2**62)) * (2**20), -int(rng.integers(0, 2**50))) for _ in range(16)]
+        absv = np.zeros(64, dtype=np.uint64)
+        neg = np.zeros(32, dtype=np.int32)
+        for h in range(16):
+            for r in range(2):
+                v, t = consts[h][r], 2 * h + r
+                absv[2 * t], absv[2 * t + 1], neg[t] = abs(v) & (2**64 - 1), abs(v) >> 64, int(v < 0)
+        items.append((i // pairs, cr, ci, consts, absv, neg))
+    want = []
+    for m, cr, ci, consts, _, _ in items:
+        acc = [M.to_dev(np.zeros(W, np.uint64)) for _ in range(2)]
+        raw = [M.to_dev(np.zeros(W, np.uint64)) for _ in range(3)]
+        ctx.walsh_product_sum(level, signs[m][0], signs[m][1], cr, ci, consts, pt, acc, raw)
+        want.append([M.from_dev(t) for t in acc + raw])
+    K = len(items)
+    outs = [[M.to_dev(np.zeros(W, np.uint64)) for _ in range(5)] for _ in range(K)]
+    P = lambda xs: (ctypes.c_void_p * len(xs))(*xs)
+    sig = lambda part, comp: P([signs[it[0]][part][h][comp].data_ptr() for it in items for h in range(15)])
+    lib = M.lib()
+    s = M._stream(torch.cuda.current_stream())
+
+    def call(o):
+        return lib.mi_walsh_product_sum_batch(
+            ctx.h, level, K, sig(0, 0), sig(0, 1), sig(1, 0), sig(1, 1),
+            P([it[1].ctypes.data for it in items]), P([it[2].ctypes.data for it in items]),
+            P([it[4].ctypes.data for it in items]), P([it[5].ctypes.data for it in items]), pt.data_ptr(), 1,
+            *[P([o[k][j].data_ptr() for k in range(K)]) for j in range(5)], s)
+
+    print("walsh batch: 9 items, streams", streams, flush=True)
+    M.check(call(outs))
+    for k in range(K):
+        for j in range(5):
+            assert np.array_equal(M.from_dev(outs[k][j]), want[k][j]), (k, j)
+    clash = [list(o) for o in outs]
+    clash[1][0] = clash[0][0]
+    assert call(clash) < 0
