@@ -162,16 +162,21 @@ def test_keyswitch_batch_mixed_levels():
         assert np.array_equal(M.from_dev(db), ob) and np.array_equal(M.from_dev(da), oa)
 
 
-def test_keyswitch_batch_groups_and_dependencies():
-    """Items at one level are switched together (up to 32 per launch); an
-    item that reads another's output, or adds into the same buffers, still
-    sees the result of switching the items one by one in order."""
+@pytest.mark.parametrize("count,streams", [(37, "1"), (80, "1"), (80, "0")])
+def test_keyswitch_batch_groups_and_dependencies(count, streams, monkeypatch):
+    """Items at one level are switched together (up to 32 per launch; with 80
+    items, two full groups at level 4 that wait for the flush and run on the
+    side streams, or on one stream); an item that reads another's output, or
+    adds into the same buffers, still sees the result of switching the items
+    one by one in order."""
+    monkeypatch.setenv("MICKKS_MULTI_STREAM", streams)
     N = 1 << 12
     M, ctx, octx, q, p, rng = _random_setup(N, 6, 2, 2, seed=13)
     keys = [_rand_key(rng, q, p, N, octx.parts(5)) for _ in range(2)]
     dkeys = [M.to_dev(k) for k in keys]
     host, specs = {}, []
-    for i in range(37):
+    print(f"keyswitch batch: {count} items, streams {streams}", flush=True)
+    for i in range(count):
         level = 4 if i % 5 else 2
         for name in ("kt", "b", "a"):
             host[(name, i)] = rand_residues(rng, q[: level + 1], N)
