@@ -332,6 +332,8 @@ int mi_tensor(mi_context* ctx, int level, uint64_t* b1, uint64_t* a1, const uint
     });
 }
 
+static void check_disjoint(std::vector<const uint64_t*> lo, long long W, const char* what);
+
 int mi_tensor_list(mi_context* ctx, int level, int count, uint64_t* const* b1, uint64_t* const* a1,
                    const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* c2, void* stream) {
     return guard(b2 ? "ckks_mul_inplace(ct,ct)" : "ckks_square_inplace", [&] {
@@ -358,6 +360,35 @@ int mi_tensor_list(mi_context* ctx, int level, int count, uint64_t* const* b1, u
             }
         }
         mi::tensor_list(*ctx->c, level, count, b1, a1, b2, a2, c2, S(stream));
+    });
+}
+
+int mi_tensor_list_into(mi_context* ctx, int level, int count, const uint64_t* const* b1, const uint64_t* const* a1,
+                        const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* b_out,
+                        uint64_t* const* a_out, uint64_t* const* c2, void* stream) {
+    return guard(b2 ? "ckks_mul(ct,ct)" : "ckks_square", [&] {
+        check_ctx(ctx);
+        check_level(*ctx->c, level);
+        if (count < 1 || !b1 || !a1 || !b_out || !a_out || !c2 || (!b2 != !a2))
+            throw mi::Error(MI_EINVAL, "invalid ciphertext list");
+        const long long W = (long long)(level + 1) * (long long)ctx->c->N;
+        std::vector<const uint64_t*> outs, ins;
+        for (int k = 0; k < count; ++k) {
+            check_ptr(b1[k]); check_ptr(a1[k]);
+            check_ptr(b_out[k]); check_ptr(a_out[k]); check_ptr(c2[k]);
+            outs.insert(outs.end(), {b_out[k], a_out[k], c2[k]});
+            ins.insert(ins.end(), {b1[k], a1[k]});
+            if (b2) {
+                check_ptr(b2[k]);
+                check_ptr(a2[k]);
+                ins.insert(ins.end(), {b2[k], a2[k]});
+            }
+        }
+        check_disjoint(outs, W, "output polys overlap");
+        for (const uint64_t* o : outs)
+            for (const uint64_t* x : ins)
+                if (o < x + W && x < o + W) throw mi::Error(MI_EINVAL, "an output overlaps an input");
+        mi::tensor_list(*ctx->c, level, count, b_out, a_out, b2, a2, c2, S(stream), b1, a1);
     });
 }
 

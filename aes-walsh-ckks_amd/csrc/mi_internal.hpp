@@ -250,9 +250,11 @@ void ew_scalar_mac(const Context& c, u64* dst, const u64* src, const u64* scalar
 void tensor(const Context& c, int limbs, u64* b1, u64* a1, const u64* b2, const u64* a2, u64* c2,
             hipStream_t s);
 void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s);
-// listed tensor products (b2 == nullptr: squares), level+1 limbs each
+// listed tensor products (b2 == nullptr: squares), level+1 limbs each; the
+// first operand read from (s1b, s1a) when given, else from (b1, a1)
 void tensor_list(const Context& c, int level, int count, u64* const* b1, u64* const* a1, const u64* const* b2,
-                 const u64* const* a2, u64* const* c2, hipStream_t s);
+                 const u64* const* a2, u64* const* c2, hipStream_t s, const u64* const* s1b = nullptr,
+                 const u64* const* s1a = nullptr);
 void automorphism(const Context& c, int limbs, u64 galois, const u64* src, u64* dst, int npolys,
                   long long sstride, long long dstride, hipStream_t s);
 // dst (+)= automorphism(src); src and dst must not overlap

@@ -182,9 +182,13 @@ void square(const Context& c, int limbs, u64* b, u64* a, u64* c2, hipStream_t s)
 
 // The tensor product (or square, when b2 is null) of listed ciphertext
 // pairs: blockIdx.y is the pair, so a batch of ciphertexts at one level is
-// one launch per TL_MAX pairs.
+// one launch per TL_MAX pairs.  The first operand is read from (s1b, s1a)
+// and the product written to (b1, a1, c2): the same polys in place, or new
+// ones.
 constexpr int TL_MAX = 16;
 struct TensorList {
+    const u64* s1b[TL_MAX];
+    const u64* s1a[TL_MAX];
     u64* b1[TL_MAX];
     u64* a1[TL_MAX];
     const u64* b2[TL_MAX];
@@ -196,7 +200,7 @@ __global__ void __launch_bounds__(EW_THREADS) k_tensor_list(const BasisConst* __
     const long long i = (long long)blockIdx.x * EW_THREADS + threadIdx.x;
     const int k = blockIdx.y;
     const BasisConst b = bc[ew_limb(logN)];
-    const u64 vb1 = l.b1[k][i], va1 = l.a1[k][i];
+    const u64 vb1 = l.s1b[k][i], va1 = l.s1a[k][i];
     if constexpr (SQUARE) {
         l.c2[k][i] = mul_mod(va1, va1, b.q, b.mu_lo, b.mu_hi);
         l.b1[k][i] = mul_mod(vb1, vb1, b.q, b.mu_lo, b.mu_hi);
@@ -211,13 +215,16 @@ __global__ void __launch_bounds__(EW_THREADS) k_tensor_list(const BasisConst* __
 }
 
 void tensor_list(const Context& c, int level, int count, u64* const* b1, u64* const* a1, const u64* const* b2,
-                 const u64* const* a2, u64* const* c2, hipStream_t s) {
+                 const u64* const* a2, u64* const* c2, hipStream_t s, const u64* const* s1b,
+                 const u64* const* s1a) {
     if (level < 0 || level >= c.nq) throw Error(-1, "invalid level");
     const bool sq = b2 == nullptr;
     for (int k0 = 0; k0 < count; k0 += TL_MAX) {
         const int n = std::min(TL_MAX, count - k0);
         TensorList l{};
         for (int k = 0; k < n; ++k) {
+            l.s1b[k] = s1b ? s1b[k0 + k] : b1[k0 + k];
+            l.s1a[k] = s1a ? s1a[k0 + k] : a1[k0 + k];
             l.b1[k] = b1[k0 + k];
             l.a1[k] = a1[k0 + k];
             l.b2[k] = sq ? nullptr : b2[k0 + k];

@@ -804,6 +804,10 @@ void scale_up_to(const CKKSContext& context, CKKSCiphertext& ct, long double tar
 // or above): the result equals the product with a copy of ct2 dropped to
 // ct1's level, without the copy.
 void mul_at_level(const CKKSContext& c, CKKSCiphertext& ct1, const CKKSCiphertext& ct2);
+// the tensor product of lhs and rhs (rhs null: the square of lhs), both read
+// at `level`, into a new 3-poly ciphertext at `level`; the inputs unchanged
+CKKSCiphertext mul_into_at_level(const CKKSContext& c, const CKKSCiphertext& lhs, const CKKSCiphertext* rhs,
+                                 size_t level);
 // ckks_apply_linear_transform_plan reading `in` in place (its stages never
 // write their input, so callers that keep `in` need no copy of it)
 CKKSCiphertext apply_linear_transform_plan_from(const CKKSContext& c, const CKKSCiphertext& in,

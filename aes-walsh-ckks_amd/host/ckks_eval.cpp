@@ -114,6 +114,38 @@ void ckks_square_inplace(const CKKSContext& c, CKKSCiphertext& ct) {
     ct.setScale(ct.getScale() * ct.getScale());
 }
 
+CKKSCiphertext ckks_detail::mul_into_at_level(const CKKSContext& c, const CKKSCiphertext& lhs,
+                                              const CKKSCiphertext* rhs, size_t level) {
+    const char* fn = rhs ? "ckks_mul_inplace(ct,ct)" : "ckks_square_inplace";
+    check_ct(c, lhs, fn);
+    const CKKSCiphertext& r = rhs ? *rhs : lhs;
+    if (r.getN() != lhs.getN() || level > lhs.getLevel() || level > r.getLevel())
+        throw std::invalid_argument(std::string(fn) + ": ciphertext size mismatch");
+    if (lhs.getNumPolys() != 2 || r.getNumPolys() != 2)
+        throw std::invalid_argument(std::string(fn) + ": expected 2-component inputs");
+    if (lhs.getSecretOwner() != r.getSecretOwner() || lhs.getMessageEncodingState() != r.getMessageEncodingState())
+        throw std::invalid_argument(std::string(fn) + ": ciphertext metadata mismatch");
+    if (lhs.getBatchSize() != r.getBatchSize()) throw std::invalid_argument(std::string(fn) + ": batch size mismatch");
+    const size_t B = lhs.getBatchSize();
+    CKKSCiphertext out(c, 3, lhs.getScale() * r.getScale(), level, lhs.getSecretOwner(),
+                       lhs.getMessageEncodingState(), B);
+    std::vector<const uint64_t*> b1(B), a1(B), b2(B), a2(B);
+    std::vector<uint64_t*> bo(B), ao(B), c2(B);
+    for (size_t m = 0; m < B; ++m) {
+        b1[m] = lhs.polyData(0, m);
+        a1[m] = lhs.polyData(1, m);
+        b2[m] = r.polyData(0, m);
+        a2[m] = r.polyData(1, m);
+        bo[m] = out.polyData(0, m);
+        ao[m] = out.polyData(1, m);
+        c2[m] = out.polyData(2, m);
+    }
+    mi_check(mi_tensor_list_into(c.handle(), (int)level, (int)B, b1.data(), a1.data(), rhs ? b2.data() : nullptr,
+                                 rhs ? a2.data() : nullptr, bo.data(), ao.data(), c2.data(), c.stream()),
+             fn);
+    return out;
+}
+
 void ckks_mul_inplace(const CKKSContext& c, CKKSCiphertext& ct, int64_t scalar) {
     check_ct(c, ct, "ckks_mul_inplace(ct,int)");
     if (ct.getNumPolys() == 0) throw std::invalid_argument("ckks_mul_inplace(ct,int): empty ciphertext");

@@ -401,15 +401,11 @@ private:
         if (rb) rescale_entry(b);
         const CKKSCiphertext& lhs_src = ct_.at(a);
         const size_t level = a == b ? lhs_src.getLevel() : std::min(lhs_src.getLevel(), ct_.at(b).getLevel());
-        CKKSCiphertext out = lhs_src.cloneAtLevel(level);
+        // the reference's clone of T_a (and T_b) at `level` followed by the
+        // in-place product (ckks_evalpoly.cpp:908-921): here the product of
+        // both operands read at `level` is written into a new ciphertext
+        CKKSCiphertext out = ckks_detail::mul_into_at_level(c_, lhs_src, a == b ? nullptr : &ct_.at(b), level);
         const ScaleTerm prod = scale_product(sc_.at(a), sc_.at(b));
-        if (a == b) {
-            ckks_square_inplace(c_, out);
-        } else if (ct_.at(b).getLevel() == level) {
-            ckks_mul_inplace(c_, out, ct_.at(b));   // the reference's clone at the same level changes nothing
-        } else {
-            ckks_detail::mul_at_level(c_, out, ct_.at(b));  // read at `level`, as the reference's copy
-        }
         ckks_relin_hybrid_inplace(c_, out);
         out.setScale(prod.value_d(q_));
         // T_n = 2 (T_a T_b) - T_d (d = |a - b| > 0) or 2 T_a^2 - 1 (d = 0): the

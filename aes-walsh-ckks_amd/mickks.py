@@ -58,6 +58,7 @@ _SIGS = {
     "mi_tensor": (cint, [vp, cint, vp, vp, vp, vp, vp, vp]),
     "mi_square": (cint, [vp, cint, vp, vp, vp, vp]),
     "mi_tensor_list": (cint, [vp, cint, cint, vp, vp, vp, vp, vp, vp]),
+    "mi_tensor_list_into": (cint, [vp, cint, cint, vp, vp, vp, vp, vp, vp, vp, vp]),
     "mi_mul_plain": (cint, [vp, cint, vp, cint, i64, vp, cint, vp]),
     "mi_mul_scalar": (cint, [vp, cint, vp, cint, i64, vp, vp]),
     "mi_add": (cint, [vp, cint, vp, vp, cint, i64, vp]),

@@ -148,6 +148,16 @@ int mi_square(mi_context* ctx, int level, uint64_t* b, uint64_t* a, uint64_t* c2
  * output polys may overlap. */
 int mi_tensor_list(mi_context* ctx, int level, int count, uint64_t* const* b1, uint64_t* const* a1,
                    const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* c2, void* stream);
+/* mi_tensor_list into new polys: (b_out, a_out, c2)[k] = the tensor of
+ * (b1, a1)[k] with (b2, a2)[k], or its square (b2 and a2 null), the inputs
+ * read at `level` (their first level+1 limbs) and left unchanged.  It
+ * replaces the reference's copy-then-multiply of the power basis
+ * (ckks_evalpoly.cpp:908-921: T_a cloned, dropped to the level, then
+ * ckks_square_inplace / ckks_mul_inplace), whose product does not depend on
+ * the copy.  Outputs pairwise disjoint and overlapping no input. */
+int mi_tensor_list_into(mi_context* ctx, int level, int count, const uint64_t* const* b1, const uint64_t* const* a1,
+                        const uint64_t* const* b2, const uint64_t* const* a2, uint64_t* const* b_out,
+                        uint64_t* const* a_out, uint64_t* const* c2, void* stream);
 /* ckks_mul_inplace(ct, pt), ckks_mul.cpp:160-212 (pt_montgomery: plaintext stored x 2^64) */
 int mi_mul_plain(mi_context* ctx, int level, uint64_t* polys, int npolys, int64_t poly_stride,
                  const uint64_t* pt, int pt_montgomery, void* stream);

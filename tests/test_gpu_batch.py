@@ -94,6 +94,47 @@ def test_tensor_list_matches_numpy(square):
                                   M._stream(torch.cuda.current_stream())) < 0
 
 
+@pytest.mark.parametrize("square", [False, True])
+def test_tensor_list_into_matches_numpy(square):
+    """mi_tensor_list_into: the inputs one level above the product's level
+    (read at that level, their first limbs), the products in new polys, the
+    inputs unchanged; an output on an input or on another output is refused."""
+    M = product()
+    import torch
+    N, level = 1 << 10, 2
+    p, q = O.find_primes([60], [60, 50, 45, 40], N)
+    ctx = M.Context(N, q, p, 1)
+    rng = np.random.default_rng(95 + square)
+    L, Lin = level + 1, level + 2
+    count = 19  # more than one launch of 16
+    polys = [[rand_residues(rng, q[:Lin], N) for _ in range(4)] for _ in range(count)]
+    dev = [[M.to_dev(x) for x in item] for item in polys]
+    outs = [[M.to_dev(np.zeros(L * N, np.uint64)) for _ in range(3)] for _ in range(count)]
+    b2 = None if square else _arr([M.ptr(d[2]) for d in dev])
+    a2 = None if square else _arr([M.ptr(d[3]) for d in dev])
+    s = M._stream(torch.cuda.current_stream())
+    print("tensor_list_into: square", square, flush=True)
+    M.check(M.lib().mi_tensor_list_into(ctx.h, level, count, _arr([M.ptr(d[0]) for d in dev]),
+                                        _arr([M.ptr(d[1]) for d in dev]), b2, a2,
+                                        *[_arr([M.ptr(o[j]) for o in outs]) for j in range(3)], s))
+    torch.cuda.synchronize()
+    qv = np.repeat(np.array(q[:L], dtype=object), N)
+    for item, d, o in zip(polys, dev, outs):
+        b1, a1, b2v, a2v = (x[: L * N].astype(object) for x in item)
+        if square:
+            b2v, a2v = b1, a1
+        want = [(b1 * b2v) % qv, (a1 * b2v + b1 * a2v) % qv, (a1 * a2v) % qv]
+        for w, g in zip(want, o):
+            assert np.array_equal(M.from_dev(g), np.array(w, dtype=np.uint64))
+        for x, dx in zip(item, d):
+            assert np.array_equal(M.from_dev(dx), x)
+    one = lambda t: _arr([M.ptr(t)])
+    assert M.lib().mi_tensor_list_into(ctx.h, level, 1, one(dev[0][0]), one(dev[0][1]), None, None,
+                                       one(dev[0][0]), one(outs[0][1]), one(outs[0][2]), s) < 0
+    assert M.lib().mi_tensor_list_into(ctx.h, level, 1, one(dev[0][0]), one(dev[0][1]), None, None,
+                                       one(outs[0][0]), one(outs[0][0]), one(outs[0][2]), s) < 0
+
+
 def _chain_context(H, seed=21):
     c = H.CKKSParamsConfig()
     c.N = 1 << 12
