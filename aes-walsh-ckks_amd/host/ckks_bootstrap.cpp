@@ -48,8 +48,9 @@ void double_angle(const CKKSContext& c, CKKSCiphertext& ct, size_t steps) {
     for (size_t i = 0; i < steps; ++i) {
         if (ct.getLevel() == 0)
             throw std::invalid_argument("evalmod_binary_inplace: insufficient level for BinBoot double-angle");
-        CKKSCiphertext sq = ct.clone();
-        ckks_square_inplace(c, sq);
+        // the square of ct written into a new ciphertext (the same product as
+        // squaring a copy of ct in place)
+        CKKSCiphertext sq = ckks_detail::mul_into_at_level(c, ct, nullptr, ct.getLevel());
         ckks_relin_hybrid_inplace(c, sq);
         ckks_rescale(c, sq);
         mul_real_to_scale(c, sq, -4.0, target);
